@@ -1,0 +1,282 @@
+"""axia_gpu — host side of the MI355X linear-static hot path of axia-fem.
+
+Thin Python mirror of the reference's solve boundary over the C ABI
+(include/axia_gpu.h); all arithmetic runs in libaxia_gpu.so (HIP, gfx950).
+
+  solve_kff(n, (ti, tj, tv), rhs)   linalg::solve_kff      (linalg.rs:311-404)
+  SparseResult                       linalg::SparseResult   (linalg.rs:261-266)
+  csr_from_triplets(...)             linalg::csr_from_triplets (linalg.rs:10-37)
+  Mesh(...).assemble()/.solve()      element_ke + triplet loop + DofMap::reduce
+                                     + solve_kff of solve_linear
+                                     (analysis.rs:574-668, 1018-1024)
+  Mesh.nodal_stress(u)               stress recovery (analysis.rs:1061-1162)
+  Mesh.compliance_adjoint(u, f)      dc/dE per element (new path)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (AxgError, PREC_FP64, PREC_MIXED, PRECOND_BLOCK3, PRECOND_JACOBI, check, dptr,
+                   f64, i32, i64, iptr, lptr)
+
+__all__ = [
+    "AxgError", "Context", "Mesh", "SparseResult", "solve_kff", "csr_from_triplets",
+    "pattern_host", "pressure_loads", "default_context", "PREC_FP64", "PREC_MIXED",
+    "PRECOND_JACOBI", "PRECOND_BLOCK3",
+]
+
+
+@dataclass
+class SparseResult:
+    """linalg.rs:261-266: x, solver name, iterations, residual ||K x - b||."""
+    x: np.ndarray
+    name: str
+    iters: int
+    residual: float
+
+
+def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inner_tol=1e-4,
+          max_outer=30, timing=False):
+    o = _lib.SolveOpts()
+    _lib.lib().axg_default_opts(C.byref(o))
+    o.precision = {"fp64": PREC_FP64, "mixed": PREC_MIXED}.get(precision, precision)
+    o.precond = {"jacobi": PRECOND_JACOBI, "block3": PRECOND_BLOCK3}.get(precond, precond)
+    o.tol = tol
+    o.max_iter = max_iter
+    o.inner_tol = inner_tol
+    o.max_outer = max_outer
+    o.timing = 1 if timing else 0
+    return o
+
+
+def _info_dict(info):
+    return {
+        "iters": int(info.iters), "outer_iters": int(info.outer_iters), "nrhs": int(info.nrhs),
+        "residual": float(info.residual), "rel_residual": float(info.rel_residual),
+        "solve_ms": float(info.solve_ms), "spmv_ms": float(info.spmv_ms),
+        "spmv_launches": int(info.spmv_launches), "name": info.name.decode(),
+    }
+
+
+class Context:
+    """One device, one HIP stream (axg_create)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(_lib.lib().axg_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return _lib.lib().axg_stream(self._h)
+
+    def close(self):
+        if self._h:
+            _lib.lib().axg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Mesh:
+    """All-C3D8 or all-C3D20 isotropic mesh on the device (axg_mesh_create).
+
+    xyz (nnode, 3); conn (nelem, nn) 0-based node indices; e, nu per element;
+    ind_of (3 nnode) free index per DOF or -1 (DofMap::build numbering).
+    """
+
+    def __init__(self, nn, xyz, conn, e, nu, ind_of, ctx=None):
+        self.ctx = ctx or default_context()
+        self.nn = int(nn)
+        self.xyz = f64(xyz).reshape(-1, 3)
+        self.conn = i32(conn).reshape(-1, self.nn)
+        self.nnode = self.xyz.shape[0]
+        self.nelem = self.conn.shape[0]
+        self.e = f64(np.broadcast_to(e, (self.nelem,)))
+        self.nu = f64(np.broadcast_to(nu, (self.nelem,)))
+        self.ind_of = i64(ind_of)
+        h = C.c_void_p()
+        check(_lib.lib().axg_mesh_create(self.ctx.handle, self.nn, self.nnode, dptr(self.xyz),
+                                         self.nelem, iptr(self.conn), dptr(self.e), dptr(self.nu),
+                                         lptr(self.ind_of), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().axg_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        nf, nnz, nc = C.c_int64(), C.c_int64(), C.c_int64()
+        sym = C.c_double()
+        check(_lib.lib().axg_mesh_info(self._h, C.byref(nf), C.byref(nnz), C.byref(nc),
+                                       C.byref(sym)))
+        return {"n_free": nf.value, "nnz": nnz.value, "ncontrib": nc.value,
+                "symbolic_s": sym.value}
+
+    @property
+    def n_free(self):
+        return self.info()["n_free"]
+
+    def set_modulus(self, e):
+        self.e = f64(np.broadcast_to(e, (self.nelem,)))
+        check(_lib.lib().axg_mesh_set_modulus(self._h, dptr(self.e)))
+
+    def assemble(self, precision="fp64"):
+        """Numeric assembly; returns device milliseconds."""
+        ms = C.c_double()
+        prec = {"fp64": PREC_FP64, "mixed": PREC_MIXED}.get(precision, precision)
+        check(_lib.lib().axg_assemble(self._h, prec, C.byref(ms)))
+        return ms.value
+
+    def csr(self):
+        inf = self.info()
+        n, nnz = inf["n_free"], inf["nnz"]
+        indptr = np.zeros(n + 1, np.int64)
+        indices = np.zeros(max(nnz, 1), np.int64)
+        data = np.zeros(max(nnz, 1), np.float64)
+        check(_lib.lib().axg_csr_export(self._h, lptr(indptr), lptr(indices), dptr(data)))
+        return indptr, indices[:nnz], data[:nnz]
+
+    def scatter_map(self):
+        inf = self.info()
+        sp = np.zeros(inf["nnz"] + 1, np.int64)
+        cb = np.zeros(max(inf["ncontrib"], 1), np.int64)
+        check(_lib.lib().axg_scatter_map_export(self._h, lptr(sp), lptr(cb)))
+        return sp, cb[:inf["ncontrib"]]
+
+    def ke(self, first=0, count=None):
+        count = self.nelem - first if count is None else count
+        m = 3 * self.nn
+        out = np.zeros(count * m * m)
+        check(_lib.lib().axg_ke_export(self._h, first, count, dptr(out)))
+        return out.reshape(count, m, m)
+
+    def solve(self, rhs, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
+              inner_tol=1e-4, max_outer=30, timing=False):
+        """Solve K_ff x = rhs; rhs (n_free,) or (nrhs, n_free). Returns (x, info)."""
+        b = f64(rhs)
+        single = b.ndim == 1
+        b2 = b.reshape(1, -1) if single else b
+        nrhs = b2.shape[0]
+        x = np.zeros_like(b2)
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing)
+        check(_lib.lib().axg_mesh_solve(self._h, dptr(b2), nrhs, C.byref(o), dptr(x),
+                                        C.byref(info)))
+        return (x[0] if single else x), _info_dict(info)
+
+    def solve_device(self, rhs_ptr, nrhs, x_ptr, precision="fp64", precond="jacobi", tol=1e-8,
+                     max_iter=0, inner_tol=1e-4, max_outer=30, timing=False):
+        """Device-pointer solve (inputs already in HBM); returns info."""
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing)
+        check(_lib.lib().axg_mesh_solve_device(self._h, C.c_void_p(rhs_ptr), nrhs, C.byref(o),
+                                               C.c_void_p(x_ptr), C.byref(info)))
+        return _info_dict(info)
+
+    def nodal_stress(self, u_full):
+        u = f64(u_full).ravel()
+        stress = np.zeros((self.nnode, 6))
+        vm = np.zeros(self.nnode)
+        en = np.zeros((self.nelem, self.nn, 6))
+        check(_lib.lib().axg_nodal_stress(self._h, dptr(u), dptr(stress), dptr(vm), dptr(en)))
+        return stress, vm, en
+
+    def compliance_adjoint(self, u_full, f_full):
+        """c = f^T u and dc/dE_e = -u_e^T K1_e u_e (K1: Ke at unit modulus)."""
+        u = f64(u_full).ravel()
+        f = f64(f_full).ravel()
+        c = C.c_double()
+        dc = np.zeros(self.nelem)
+        check(_lib.lib().axg_compliance_adjoint(self._h, dptr(u), dptr(f), C.byref(c), dptr(dc)))
+        return c.value, dc
+
+
+def csr_from_triplets(n, ti, tj, tv, ctx=None):
+    """linalg::csr_from_triplets on the device: (indptr, indices, data)."""
+    ctx = ctx or default_context()
+    ti, tj, tv = i64(ti), i64(tj), f64(tv)
+    indptr = np.zeros(n + 1, np.int64)
+    L = _lib.lib()
+    check(L.axg_csr_from_triplets(ctx.handle, n, len(ti), lptr(ti), lptr(tj), dptr(tv),
+                                  lptr(indptr), None, None))
+    nnz = int(indptr[n])
+    indices = np.zeros(max(nnz, 1), np.int64)
+    data = np.zeros(max(nnz, 1), np.float64)
+    check(L.axg_csr_from_triplets(ctx.handle, n, len(ti), lptr(ti), lptr(tj), dptr(tv),
+                                  lptr(indptr), lptr(indices), dptr(data)))
+    return indptr, indices[:nnz], data[:nnz]
+
+
+def solve_kff(n, trips, rhs, ctx=None, tol=1e-8, max_iter=0, timing=False):
+    """linalg::solve_kff drop-in: trips = (ti, tj, tv) arrays; returns SparseResult."""
+    ctx = ctx or default_context()
+    ti, tj, tv = (i64(trips[0]), i64(trips[1]), f64(trips[2]))
+    b = f64(rhs)
+    x = np.zeros(n)
+    info = _lib.SolveInfo()
+    o = _opts(PREC_FP64, PRECOND_JACOBI, tol, max_iter, timing=timing)
+    check(_lib.lib().axg_solve_kff(ctx.handle, n, len(ti), lptr(ti), lptr(tj), dptr(tv), dptr(b),
+                                   C.byref(o), dptr(x), C.byref(info)))
+    d = _info_dict(info)
+    return SparseResult(x, d["name"], d["iters"], d["residual"])
+
+
+def pattern_host(nn, nnode, conn, ind_of):
+    """Symbolic phase on the host: (rowptr, colind, slot_ptr, contrib)."""
+    conn = i32(conn)
+    ind_of = i64(ind_of)
+    nelem = conn.size // nn
+    nf, nnz, nc = C.c_int64(), C.c_int64(), C.c_int64()
+    L = _lib.lib()
+    check(L.axg_pattern_host(nn, nnode, nelem, iptr(conn), lptr(ind_of), C.byref(nf), C.byref(nnz),
+                             C.byref(nc), None, None, None, None))
+    rp = np.zeros(nf.value + 1, np.int64)
+    ci = np.zeros(max(nnz.value, 1), np.int64)
+    sp = np.zeros(nnz.value + 1, np.int64)
+    cb = np.zeros(max(nc.value, 1), np.int64)
+    check(L.axg_pattern_host(nn, nnode, nelem, iptr(conn), lptr(ind_of), C.byref(nf), C.byref(nnz),
+                             C.byref(nc), lptr(rp), lptr(ci), lptr(sp), lptr(cb)))
+    return rp, ci[:nnz.value], sp, cb[:nc.value]
+
+
+def pressure_loads(nn, xyz, elem_conn, faces, mags, f_full):
+    """Accumulate face-pressure loads into f_full (in place) in the given order."""
+    xyz = f64(xyz)
+    ec = i32(elem_conn)
+    fa = i32(faces)
+    mg = f64(mags)
+    assert f_full.dtype == np.float64 and f_full.flags.c_contiguous
+    check(_lib.lib().axg_pressure_loads(nn, xyz.size // 3, dptr(xyz), len(fa), iptr(ec), iptr(fa),
+                                        dptr(mg), dptr(f_full)))
+    return f_full

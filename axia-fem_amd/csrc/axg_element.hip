@@ -1,0 +1,531 @@
+// Element kernels for gfx950: C3D8 / C3D20 stiffness integration, the
+// deterministic gather into CSR slots, stress recovery and the element
+// strain energies used by the compliance adjoint.
+//
+// FP64 kernels reproduce the reference's operation order (elem.rs:10-219,
+// 723-779; quadratic.rs:36-244) and this file is compiled with
+// -ffp-contract=off, so Ke entries, K_ff values and nodal stresses carry the
+// same FP64 bits as the CPU restatement. The dense B^T D B of gemm_bt_d_b
+// (elem.rs:90-112) multiplies many exact zeros; a product with an exact zero
+// added to a running sum that started at +0.0 never changes the sum, so the
+// kernels evaluate only the nonzero products, in the reference's k order.
+#include <cmath>
+
+#include "axg_internal.h"
+
+namespace axg {
+
+// ------------------------------------------------------------ shape tables
+// dN/dxi at the integration points, built on the host with the reference
+// formulas (the values depend only on the rule, not on the element).
+struct ShapeTables {
+    double h8_dn[8][8][3];     // stiffness rule, elem.rs:204-207 loop order
+    double h8s_dn[8][8][3];    // stress points HEX_XI[i] * G2, elem.rs:751-756
+    double h8_ext[8][8];       // N_g(HEX_XI[a] * sqrt 3), elem.rs:766-777
+    double h20_dn[27][20][3];  // 27-point rule, quadratic.rs:100-120
+    double h20_w[27];          // W_i W_j W_k
+    double h20_ext[20][27];    // lx[i] ly[j] lz[k], quadratic.rs:208-230
+};
+__constant__ ShapeTables c_tab;
+
+static const double G2 = 0.5773502691896257;
+static const double G3[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
+static const double W3[3] = {0.5555555555555556, 0.8888888888888888, 0.5555555555555556};
+static const double HEX_XI[8][3] = {{-1.0, -1.0, -1.0}, {1.0, -1.0, -1.0}, {1.0, 1.0, -1.0},
+                                    {-1.0, 1.0, -1.0},  {-1.0, -1.0, 1.0}, {1.0, -1.0, 1.0},
+                                    {1.0, 1.0, 1.0},    {-1.0, 1.0, 1.0}};
+static const double HEX20_XI[20][3] = {
+    {-1.0, -1.0, -1.0}, {1.0, -1.0, -1.0}, {1.0, 1.0, -1.0}, {-1.0, 1.0, -1.0},
+    {-1.0, -1.0, 1.0},  {1.0, -1.0, 1.0},  {1.0, 1.0, 1.0},  {-1.0, 1.0, 1.0},
+    {0.0, -1.0, -1.0},  {1.0, 0.0, -1.0},  {0.0, 1.0, -1.0}, {-1.0, 0.0, -1.0},
+    {0.0, -1.0, 1.0},   {1.0, 0.0, 1.0},   {0.0, 1.0, 1.0},  {-1.0, 0.0, 1.0},
+    {-1.0, -1.0, 0.0},  {1.0, -1.0, 0.0},  {1.0, 1.0, 0.0},  {-1.0, 1.0, 0.0}};
+
+static void hex8_shape_h(double xi, double eta, double zeta, double n[8], double dn[8][3]) {
+    for (int i = 0; i < 8; i++) {
+        double x = HEX_XI[i][0], e = HEX_XI[i][1], z = HEX_XI[i][2];
+        n[i] = 0.125 * (1.0 + x * xi) * (1.0 + e * eta) * (1.0 + z * zeta);
+        dn[i][0] = 0.125 * x * (1.0 + e * eta) * (1.0 + z * zeta);
+        dn[i][1] = 0.125 * e * (1.0 + x * xi) * (1.0 + z * zeta);
+        dn[i][2] = 0.125 * z * (1.0 + x * xi) * (1.0 + e * eta);
+    }
+}
+
+static void hex20_shape_h(double xi, double eta, double zeta, double dn[20][3]) {
+    for (int i = 0; i < 8; i++) {
+        double x = HEX20_XI[i][0], e = HEX20_XI[i][1], z = HEX20_XI[i][2];
+        double a = 1.0 + x * xi, b = 1.0 + e * eta, c = 1.0 + z * zeta;
+        dn[i][0] = 0.125 * x * b * c * (2.0 * x * xi + e * eta + z * zeta - 1.0);
+        dn[i][1] = 0.125 * e * a * c * (x * xi + 2.0 * e * eta + z * zeta - 1.0);
+        dn[i][2] = 0.125 * z * a * b * (x * xi + e * eta + 2.0 * z * zeta - 1.0);
+    }
+    for (int i = 8; i < 20; i++) {
+        double x = HEX20_XI[i][0], e = HEX20_XI[i][1], z = HEX20_XI[i][2];
+        if (std::fabs(x) < 0.5) {
+            dn[i][0] = 0.25 * (-2.0 * xi) * (1.0 + e * eta) * (1.0 + z * zeta);
+            dn[i][1] = 0.25 * (1.0 - xi * xi) * e * (1.0 + z * zeta);
+            dn[i][2] = 0.25 * (1.0 - xi * xi) * (1.0 + e * eta) * z;
+        } else if (std::fabs(e) < 0.5) {
+            dn[i][0] = 0.25 * (1.0 - eta * eta) * x * (1.0 + z * zeta);
+            dn[i][1] = 0.25 * (-2.0 * eta) * (1.0 + x * xi) * (1.0 + z * zeta);
+            dn[i][2] = 0.25 * (1.0 - eta * eta) * (1.0 + x * xi) * z;
+        } else {
+            dn[i][0] = 0.25 * (1.0 - zeta * zeta) * x * (1.0 + e * eta);
+            dn[i][1] = 0.25 * (1.0 - zeta * zeta) * (1.0 + x * xi) * e;
+            dn[i][2] = 0.25 * (-2.0 * zeta) * (1.0 + x * xi) * (1.0 + e * eta);
+        }
+    }
+}
+
+static void lagrange3(double x, double w[3]) {
+    for (int i = 0; i < 3; i++) {
+        w[i] = 1.0;
+        for (int j = 0; j < 3; j++) {
+            if (i == j) continue;
+            w[i] *= (x - G3[j]) / (G3[i] - G3[j]);
+        }
+    }
+}
+
+void upload_shape_tables() {
+    static ShapeTables t;  // host copy
+    double n8[8];
+    const double pts[2] = {-G2, G2};
+    int g = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) hex8_shape_h(pts[a], pts[b], pts[c], n8, t.h8_dn[g++]);
+    for (int i = 0; i < 8; i++)
+        hex8_shape_h(HEX_XI[i][0] * G2, HEX_XI[i][1] * G2, HEX_XI[i][2] * G2, n8, t.h8s_dn[i]);
+    double s3 = std::sqrt(3.0);
+    for (int a = 0; a < 8; a++) {
+        double dn[8][3];
+        hex8_shape_h(HEX_XI[a][0] * s3, HEX_XI[a][1] * s3, HEX_XI[a][2] * s3, t.h8_ext[a], dn);
+    }
+    g = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                hex20_shape_h(G3[i], G3[j], G3[k], t.h20_dn[g]);
+                t.h20_w[g] = W3[i] * W3[j] * W3[k];
+                g++;
+            }
+    for (int a = 0; a < 20; a++) {
+        double lx[3], ly[3], lz[3];
+        lagrange3(HEX20_XI[a][0], lx);
+        lagrange3(HEX20_XI[a][1], ly);
+        lagrange3(HEX20_XI[a][2], lz);
+        int idx = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) t.h20_ext[a][idx++] = lx[i] * ly[j] * lz[k];
+    }
+    AXG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(t)));
+}
+
+// ------------------------------------------------------------- geometry
+// Error record: code in err[0] (first error wins), element index in err[1].
+__device__ inline void report(int32_t *err, int32_t code, int64_t el) {
+    if (atomicCAS(&err[0], 0, code) == 0) err[1] = (int32_t)el;
+}
+
+// J, its inverse and dN/dx at one point: elem.rs:10-29, 180-192.
+template <int NN>
+__device__ inline int geometry(const double (*dn)[3], const double *x, double *dndx,
+                               double &det) {
+    double j00 = 0.0, j01 = 0.0, j02 = 0.0, j10 = 0.0, j11 = 0.0, j12 = 0.0, j20 = 0.0,
+           j21 = 0.0, j22 = 0.0;
+    // J[q][p] += dN_a/dxi_p * x_{a,q}, summed over a in order
+#pragma unroll 4
+    for (int a = 0; a < NN; a++) {
+        double d0 = dn[a][0], d1 = dn[a][1], d2 = dn[a][2];
+        double x0 = x[3 * a], x1 = x[3 * a + 1], x2 = x[3 * a + 2];
+        j00 += d0 * x0;
+        j10 += d0 * x1;
+        j20 += d0 * x2;
+        j01 += d1 * x0;
+        j11 += d1 * x1;
+        j21 += d1 * x2;
+        j02 += d2 * x0;
+        j12 += d2 * x1;
+        j22 += d2 * x2;
+    }
+    det = j00 * (j11 * j22 - j12 * j21) - j01 * (j10 * j22 - j12 * j20) +
+          j02 * (j10 * j21 - j11 * j20);
+    if (fabs(det) < 1e-18) return AXG_E_JAC_SINGULAR;
+    double id = 1.0 / det;
+    double b00 = (j11 * j22 - j12 * j21) * id;
+    double b01 = (j02 * j21 - j01 * j22) * id;
+    double b02 = (j01 * j12 - j02 * j11) * id;
+    double b10 = (j12 * j20 - j10 * j22) * id;
+    double b11 = (j00 * j22 - j02 * j20) * id;
+    double b12 = (j02 * j10 - j00 * j12) * id;
+    double b20 = (j10 * j21 - j11 * j20) * id;
+    double b21 = (j01 * j20 - j00 * j21) * id;
+    double b22 = (j00 * j11 - j01 * j10) * id;
+    // dN_a/dx_i = inv[0][i] dN/dxi_0 + inv[1][i] dN/dxi_1 + inv[2][i] dN/dxi_2
+#pragma unroll 4
+    for (int a = 0; a < NN; a++) {
+        double d0 = dn[a][0], d1 = dn[a][1], d2 = dn[a][2];
+        dndx[3 * a + 0] = b00 * d0 + b10 * d1 + b20 * d2;
+        dndx[3 * a + 1] = b01 * d0 + b11 * d1 + b21 * d2;
+        dndx[3 * a + 2] = b02 * d0 + b12 * d1 + b22 * d2;
+    }
+    return AXG_OK;
+}
+
+// d_iso_3d, elem.rs:41-59
+__device__ inline bool lame(double e, double nu, double &lam, double &mu, double &diag) {
+    if (nu >= 0.5 || nu <= -1.0 || e <= 0.0) return false;
+    lam = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    mu = e / (2.0 * (1.0 + nu));
+    diag = lam + 2.0 * mu;
+    return true;
+}
+
+// The 3x3 block of w * B_a^T D B_b for one point, added to k[9] (row-major
+// p, q). Nonzero products of gemm_bt_d_b in ascending k order.
+__device__ inline void block_btdb(double ax, double ay, double az, double bx, double by, double bz,
+                                  double lam, double mu, double diag, double w, double *k) {
+    // tmp[k][3b+q] = D[k][q] * dN_b,q (k < 3) or mu * B[k][3b+q] (k >= 3)
+    double t0x = diag * bx, t0y = lam * by, t0z = lam * bz;
+    double t1x = lam * bx, t1y = diag * by, t1z = lam * bz;
+    double t2x = lam * bx, t2y = lam * by, t2z = diag * bz;
+    double t3x = mu * by, t3y = mu * bx;  // row xy
+    double t4y = mu * bz, t4z = mu * by;  // row yz
+    double t5x = mu * bz, t5z = mu * bx;  // row zx
+    double s;
+    // p = 0: k in {0, 3, 5} with B = (ax, ay, az)
+    s = ax * t0x + ay * t3x + az * t5x;
+    k[0] += w * s;
+    s = ax * t0y + ay * t3y;
+    k[1] += w * s;
+    s = ax * t0z + az * t5z;
+    k[2] += w * s;
+    // p = 1: k in {1, 3, 4} with B = (ay, ax, az)
+    s = ay * t1x + ax * t3x;
+    k[3] += w * s;
+    s = ay * t1y + ax * t3y + az * t4y;
+    k[4] += w * s;
+    s = ay * t1z + az * t4z;
+    k[5] += w * s;
+    // p = 2: k in {2, 4, 5} with B = (az, ay, ax)
+    s = az * t2x + ax * t5x;
+    k[6] += w * s;
+    s = az * t2y + ay * t4y;
+    k[7] += w * s;
+    s = az * t2z + ay * t4z + ax * t5z;
+    k[8] += w * s;
+}
+
+// ------------------------------------------------------- FP64 Ke kernels
+// One workgroup integrates EB elements: phase 1 stages dN/dx and the weight of
+// every (element, point) in LDS, phase 2 gives each thread (element, a, b)
+// node pairs and accumulates the 3x3 block over the points in registers.
+template <int NN, int NG, int EB>
+__global__ void __launch_bounds__(256) ke_fp64_kernel(const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ conn,
+                                                      const double *__restrict__ E,
+                                                      const double *__restrict__ NU, int64_t nelem,
+                                                      double *__restrict__ ke,
+                                                      int32_t *__restrict__ err) {
+    constexpr int M = 3 * NN;
+    __shared__ double s_dndx[EB][NG][3 * NN];
+    __shared__ double s_w[EB][NG];
+    __shared__ double s_x[EB][3 * NN];
+    __shared__ double s_mat[EB][3];
+    const int64_t e0 = (int64_t)blockIdx.x * EB;
+    const int tid = threadIdx.x;
+    // stage coordinates
+    for (int t = tid; t < EB * NN; t += blockDim.x) {
+        int el = t / NN, a = t % NN;
+        int64_t e = e0 + el;
+        if (e < nelem) {
+            int64_t node = conn[e * NN + a];
+            s_x[el][3 * a + 0] = xyz[3 * node + 0];
+            s_x[el][3 * a + 1] = xyz[3 * node + 1];
+            s_x[el][3 * a + 2] = xyz[3 * node + 2];
+        }
+    }
+    if (tid < EB) {
+        int64_t e = e0 + tid;
+        if (e < nelem) {
+            double lam, mu, diag;
+            if (!lame(E[e], NU[e], lam, mu, diag)) {
+                report(err, AXG_E_MATERIAL, e);
+                lam = mu = diag = 0.0;
+            }
+            s_mat[tid][0] = lam;
+            s_mat[tid][1] = mu;
+            s_mat[tid][2] = diag;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < EB * NG; t += blockDim.x) {
+        int el = t / NG, g = t % NG;
+        int64_t e = e0 + el;
+        if (e >= nelem) continue;
+        double det;
+        const double(*dn)[3] = (NN == 8) ? c_tab.h8_dn[g] : c_tab.h20_dn[g];
+        int st = geometry<NN>(dn, s_x[el], s_dndx[el][g], det);
+        if (st != AXG_OK) {
+            report(err, st, e);
+            det = 0.0;
+        } else if (det <= 0.0) {
+            report(err, AXG_E_NEG_JACOBIAN, e);
+        }
+        s_w[el][g] = (NN == 8) ? det : c_tab.h20_w[g] * det;
+    }
+    __syncthreads();
+    for (int t = tid; t < EB * NN * NN; t += blockDim.x) {
+        int el = t / (NN * NN), ab = t % (NN * NN);
+        int a = ab / NN, b = ab % NN;
+        int64_t e = e0 + el;
+        if (e >= nelem) continue;
+        double lam = s_mat[el][0], mu = s_mat[el][1], diag = s_mat[el][2];
+        double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int g = 0; g < NG; g++) {
+            const double *da = &s_dndx[el][g][3 * a];
+            const double *db = &s_dndx[el][g][3 * b];
+            block_btdb(da[0], da[1], da[2], db[0], db[1], db[2], lam, mu, diag, s_w[el][g], k);
+        }
+        double *out = ke + e * (int64_t)(M * M);
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) out[(3 * a + p) * M + 3 * b + q] = k[3 * p + q];
+    }
+}
+
+void launch_ke_fp64(axg_mesh *m, hipStream_t s, int32_t *d_err) {
+    if (m->nelem == 0) return;
+    if (m->nn == 8) {
+        constexpr int EB = 16;
+        dim3 grid((unsigned)((m->nelem + EB - 1) / EB));
+        hipLaunchKernelGGL((ke_fp64_kernel<8, 8, EB>), grid, dim3(256), 0, s, m->xyz.p, m->conn.p,
+                           m->e.p, m->nu.p, m->nelem, m->ke.p, d_err);
+    } else {
+        constexpr int EB = 2;
+        dim3 grid((unsigned)((m->nelem + EB - 1) / EB));
+        hipLaunchKernelGGL((ke_fp64_kernel<20, 27, EB>), grid, dim3(256), 0, s, m->xyz.p,
+                           m->conn.p, m->e.p, m->nu.p, m->nelem, m->ke.p, d_err);
+    }
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- gather
+// Slot s of K_ff = sum of its contributions in ascending element order
+// (csr_from_triplets, linalg.rs:18-24). A slot whose contributions are all
+// exact zeros would be absent from the reference (|v| > 0 filter,
+// analysis.rs:664 and linalg.rs:11); it is flagged for the host.
+__global__ void gather_fp64_kernel(const uint32_t *__restrict__ slotptr,
+                                   const uint32_t *__restrict__ contrib,
+                                   const double *__restrict__ ke, int64_t nslot,
+                                   double *__restrict__ val, uint8_t *__restrict__ slot_nz,
+                                   uint32_t *__restrict__ zero_count) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslot;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t c0 = slotptr[s], c1 = slotptr[s + 1];
+        double v = 0.0;
+        bool nz = false;
+        for (uint32_t c = c0; c < c1; c++) {
+            double kv = ke[contrib[c]];
+            v += kv;
+            nz |= (kv != 0.0);
+        }
+        val[s] = v;
+        if (slot_nz) slot_nz[s] = nz ? 1 : 0;
+        if (zero_count && !nz) atomicAdd(zero_count, 1u);
+    }
+}
+
+void launch_gather_fp64(axg_mesh *m, hipStream_t s, bool topo, uint32_t *d_zero_count) {
+    int64_t nslot = topo ? m->nnz_topo : m->nnz;
+    if (nslot == 0) return;
+    const uint32_t *sp = topo ? m->slotptr_t.p : m->slotptr_c.p;
+    const uint32_t *cp = topo ? m->contrib_t.p : m->contrib_c.p;
+    int64_t blocks = std::min<int64_t>((nslot + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(gather_fp64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, sp, cp,
+                       m->ke.p, nslot, m->val.p, topo ? m->slot_nz.p : nullptr, d_zero_count);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------- stress recovery
+// elem.rs:723-779 (C3D8) and quadratic.rs:148-244 (C3D20), one thread per
+// element; nonzero products of sigma_from_b in ascending j order.
+template <int NN, int NG>
+__global__ void nodal_stress_kernel(const double *__restrict__ xyz,
+                                    const int32_t *__restrict__ conn, const double *__restrict__ E,
+                                    const double *__restrict__ NU, int64_t nelem,
+                                    const double *__restrict__ u, double *__restrict__ out,
+                                    int32_t *__restrict__ err) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    double x[3 * NN], ue[3 * NN];
+    for (int a = 0; a < NN; a++) {
+        int64_t node = conn[e * NN + a];
+        for (int q = 0; q < 3; q++) {
+            x[3 * a + q] = xyz[3 * node + q];
+            ue[3 * a + q] = u[3 * node + q];
+        }
+    }
+    double lam, mu, diag;
+    if (!lame(E[e], NU[e], lam, mu, diag)) {
+        report(err, AXG_E_MATERIAL, e);
+        return;
+    }
+    double gsig[NG][6];
+    double dndx[3 * NN];
+    for (int g = 0; g < NG; g++) {
+        double det;
+        const double(*dn)[3] = (NN == 8) ? c_tab.h8s_dn[g] : c_tab.h20_dn[g];
+        int st = geometry<NN>(dn, x, dndx, det);
+        if (st != AXG_OK) {
+            report(err, st, e);
+            return;
+        }
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+        for (int a = 0; a < NN; a++) {
+            double dx = dndx[3 * a], dy = dndx[3 * a + 1], dz = dndx[3 * a + 2];
+            double u0 = ue[3 * a], u1 = ue[3 * a + 1], u2 = ue[3 * a + 2];
+            e0 += dx * u0;
+            e1 += dy * u1;
+            e2 += dz * u2;
+            e3 += dy * u0;
+            e3 += dx * u1;
+            e4 += dz * u1;
+            e4 += dy * u2;
+            e5 += dz * u0;
+            e5 += dx * u2;
+        }
+        gsig[g][0] = diag * e0 + lam * e1 + lam * e2;
+        gsig[g][1] = lam * e0 + diag * e1 + lam * e2;
+        gsig[g][2] = lam * e0 + lam * e1 + diag * e2;
+        gsig[g][3] = mu * e3;
+        gsig[g][4] = mu * e4;
+        gsig[g][5] = mu * e5;
+    }
+    double *o = out + e * NN * 6;
+    for (int a = 0; a < NN; a++)
+        for (int c = 0; c < 6; c++) {
+            double v = 0.0;
+            for (int g = 0; g < NG; g++)
+                v += ((NN == 8) ? c_tab.h8_ext[a][g] : c_tab.h20_ext[a][g]) * gsig[g][c];
+            o[a * 6 + c] = v;
+        }
+}
+
+// Plain mean over incident elements in ascending element order
+// (analysis.rs:1136-1153), then von Mises (elem.rs:879-889).
+__global__ void nodal_average_kernel(const int64_t *__restrict__ inc_ptr,
+                                     const int64_t *__restrict__ inc_slot, int64_t nnode, int nn,
+                                     const double *__restrict__ en, double *__restrict__ stress,
+                                     double *__restrict__ vm) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnode) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0}, cnt = 0.0;
+    for (int64_t k = inc_ptr[i]; k < inc_ptr[i + 1]; k++) {
+        const double *s = en + inc_slot[k] * 6;  // (element * nn + local) * 6
+        for (int c = 0; c < 6; c++) acc[c] += s[c];
+        cnt += 1.0;
+    }
+    double st[6];
+    for (int c = 0; c < 6; c++) st[c] = cnt > 0.0 ? acc[c] / cnt : 0.0;
+    for (int c = 0; c < 6; c++) stress[i * 6 + c] = st[c];
+    if (vm) {
+        double a = st[0] - st[1], b = st[1] - st[2], c = st[2] - st[0];
+        vm[i] = sqrt(0.5 * (a * a + b * b + c * c) +
+                     3.0 * (st[3] * st[3] + st[4] * st[4] + st[5] * st[5]));
+    }
+}
+
+// --------------------------------------------------------- strain energy
+// u_e^T K1_e u_e with K1 the stiffness at unit modulus (same nu):
+// sum over points of w * eps^T D1 eps.
+template <int NN, int NG>
+__global__ void strain_energy_kernel(const double *__restrict__ xyz,
+                                     const int32_t *__restrict__ conn,
+                                     const double *__restrict__ NU, int64_t nelem,
+                                     const double *__restrict__ u, double *__restrict__ energy,
+                                     int32_t *__restrict__ err) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    double x[3 * NN], ue[3 * NN];
+    for (int a = 0; a < NN; a++) {
+        int64_t node = conn[e * NN + a];
+        for (int q = 0; q < 3; q++) {
+            x[3 * a + q] = xyz[3 * node + q];
+            ue[3 * a + q] = u[3 * node + q];
+        }
+    }
+    double lam, mu, diag;
+    if (!lame(1.0, NU[e], lam, mu, diag)) {
+        report(err, AXG_E_MATERIAL, e);
+        return;
+    }
+    double dndx[3 * NN];
+    double en = 0.0;
+    for (int g = 0; g < NG; g++) {
+        double det;
+        const double(*dn)[3] = (NN == 8) ? c_tab.h8_dn[g] : c_tab.h20_dn[g];
+        int st = geometry<NN>(dn, x, dndx, det);
+        if (st != AXG_OK) {
+            report(err, st, e);
+            return;
+        }
+        double w = (NN == 8) ? det : c_tab.h20_w[g] * det;
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+        for (int a = 0; a < NN; a++) {
+            double dx = dndx[3 * a], dy = dndx[3 * a + 1], dz = dndx[3 * a + 2];
+            double u0 = ue[3 * a], u1 = ue[3 * a + 1], u2 = ue[3 * a + 2];
+            e0 += dx * u0;
+            e1 += dy * u1;
+            e2 += dz * u2;
+            e3 += dy * u0 + dx * u1;
+            e4 += dz * u1 + dy * u2;
+            e5 += dz * u0 + dx * u2;
+        }
+        double s0 = diag * e0 + lam * e1 + lam * e2;
+        double s1 = lam * e0 + diag * e1 + lam * e2;
+        double s2 = lam * e0 + lam * e1 + diag * e2;
+        en += w * (e0 * s0 + e1 * s1 + e2 * s2 + mu * (e3 * e3 + e4 * e4 + e5 * e5));
+    }
+    energy[e] = en;
+}
+
+void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double *d_energy,
+                          int32_t *d_err) {
+    if (m->nelem == 0) return;
+    dim3 grid((unsigned)((m->nelem + 127) / 128));
+    if (m->nn == 8)
+        hipLaunchKernelGGL((strain_energy_kernel<8, 8>), grid, dim3(128), 0, s, m->xyz.p,
+                           m->conn.p, m->nu.p, m->nelem, d_u, d_energy, d_err);
+    else
+        hipLaunchKernelGGL((strain_energy_kernel<20, 27>), grid, dim3(128), 0, s, m->xyz.p,
+                           m->conn.p, m->nu.p, m->nelem, d_u, d_energy, d_err);
+    AXG_HIP(hipGetLastError());
+}
+
+}  // namespace axg
+
+namespace axg {
+void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
+                         double *d_stress, double *d_vm, int32_t *d_err) {
+    if (m->nelem > 0) {
+        dim3 grid((unsigned)((m->nelem + 63) / 64));
+        if (m->nn == 8)
+            hipLaunchKernelGGL((nodal_stress_kernel<8, 8>), grid, dim3(64), 0, s, m->xyz.p,
+                               m->conn.p, m->e.p, m->nu.p, m->nelem, d_u, d_elem_nodal, d_err);
+        else
+            hipLaunchKernelGGL((nodal_stress_kernel<20, 27>), grid, dim3(64), 0, s, m->xyz.p,
+                               m->conn.p, m->e.p, m->nu.p, m->nelem, d_u, d_elem_nodal, d_err);
+        AXG_HIP(hipGetLastError());
+    }
+    if (m->nnode > 0) {
+        dim3 grid((unsigned)((m->nnode + 255) / 256));
+        hipLaunchKernelGGL(nodal_average_kernel, grid, dim3(256), 0, s, m->inc_ptr.p,
+                           m->inc_slot.p, m->nnode, m->nn, d_elem_nodal, d_stress, d_vm);
+        AXG_HIP(hipGetLastError());
+    }
+}
+}  // namespace axg

@@ -1,0 +1,199 @@
+// Internal declarations shared by the host and device translation units of
+// libaxia_gpu. Nothing here crosses the C ABI (include/axia_gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/axia_gpu.h"
+
+namespace axg {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+void clear_error();
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define AXG_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (call);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            throw ::axg::Error{_e == hipErrorOutOfMemory ? AXG_E_OOM : AXG_E_DEVICE,     \
+                               std::string("HIP: ") + hipGetErrorString(_e) + " at " +   \
+                                   __FILE__ + ":" + std::to_string(__LINE__)};           \
+        }                                                                                \
+    } while (0)
+
+#define AXG_REQUIRE(cond, code, msg)                \
+    do {                                            \
+        if (!(cond)) throw ::axg::Error{(code), (msg)}; \
+    } while (0)
+
+// ----------------------------------------------------------- device buffer
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p, n = o.n;
+            o.p = nullptr, o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        AXG_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n || p == nullptr) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr, n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Pinned host staging for small scalar read-backs.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b) {
+        if (b <= bytes && p) return;
+        if (p) (void)hipHostFree(p);
+        AXG_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+    }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+// ----------------------------------------------------------- PCG scalars
+// One record per right-hand side, resident in device memory. Kernels read
+// alpha/beta/done from here so that iterations need no host round trip.
+struct PcgScalars {
+    double bnorm;     // ||b||_2
+    double tol_abs;   // tol * ||b||_2
+    double rz;        // r . z
+    double alpha, beta;
+    double residual;  // ||r||_2 (recursive)
+    double pap;
+    double t_done_ms; // set by the host for batched solves
+    int64_t iters;
+    int64_t max_iter;
+    int32_t done;     // 1 once converged / failed / max_iter reached
+    int32_t status;   // AXG_OK, AXG_E_NOT_SPD, AXG_E_NOT_CONVERGED
+    uint32_t ticket[4];
+    double pad[1];
+};
+static_assert(sizeof(PcgScalars) % 16 == 0, "PcgScalars alignment");
+
+// Reusable device workspace of a context (grown on demand).
+struct Workspace {
+    DevBuf<double> r, p, ap, z, part, t0, t1;
+    DevBuf<PcgScalars> sc;
+    DevBuf<float> f0, f1, f2, f3, f4;
+    DevBuf<int32_t> err;
+    DevBuf<uint32_t> cnt;
+};
+}  // namespace axg
+
+struct axg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> timing_events;
+    axg::PinnedBuf pinned;
+    axg::Workspace ws;
+    int cu_count = 256;
+    std::string arch;
+};
+
+struct axg_mesh {
+    axg_ctx *ctx = nullptr;
+    int nn = 8, m = 24;
+    int64_t nnode = 0, nelem = 0, ndof = 0, n_free = 0;
+    int64_t nnz = 0, ncontrib = 0;          // of the current (possibly compacted) pattern
+    int64_t nnz_topo = 0, ncontrib_topo = 0; // of the topological pattern
+    double symbolic_s = 0.0;
+    // mesh inputs (device)
+    axg::DevBuf<double> xyz, e, nu;
+    axg::DevBuf<int32_t> conn, ind_of;
+    // topological pattern and scatter map (device)
+    axg::DevBuf<int32_t> rowptr_t, colind_t;
+    axg::DevBuf<uint32_t> slotptr_t, contrib_t;
+    // active pattern (== topological unless exact-zero slots were dropped)
+    bool compacted = false;
+    axg::DevBuf<int32_t> rowptr_c, colind_c;
+    axg::DevBuf<uint32_t> slotptr_c, contrib_c;
+    // values
+    axg::DevBuf<double> val;       // FP64 K_ff values (nnz)
+    axg::DevBuf<double> ke;        // FP64 element matrices (nelem * m * m)
+    axg::DevBuf<uint8_t> slot_nz;  // per topological slot: any nonzero contribution
+    axg::DevBuf<double> dsel;      // Jacobi divisor (diag or 1.0)
+    axg::DevBuf<int64_t> inc_ptr, inc_slot;  // node -> (element * nn + local), ascending element
+    bool assembled64 = false;
+    // mixed precision operator (BSR-3, FP32)
+    bool assembled32 = false;
+    int64_t nb = 0;                 // block rows (nodes with a free DOF)
+    int64_t nnzb = 0;               // 3x3 blocks
+    axg::DevBuf<int32_t> b_rowptr, b_colind;  // block pattern
+    axg::DevBuf<int32_t> b_node;              // node of each block row
+    axg::DevBuf<int32_t> free_of_blk;         // 3*nb: free index or -1
+    axg::DevBuf<int32_t> blk_of_free;         // n_free: 3*blockrow + d
+    axg::DevBuf<uint32_t> b_slotptr;          // per block entry (9*nnzb + 1)
+    axg::DevBuf<uint32_t> b_contrib;          // FP32 Ke offsets
+    axg::DevBuf<float> ke32;                  // FP32 element matrices
+    axg::DevBuf<float> bval;                  // 9*nnzb block values (row-major 3x3)
+    axg::DevBuf<float> binv32;                // 9*nb block-Jacobi inverse (FP32 copy)
+    axg::DevBuf<float> dinv32;                // 3*nb scalar Jacobi inverse
+    axg::DevBuf<double> binv64;               // 9*nb FP64 block inverse over free DOFs
+    axg::DevBuf<int32_t> diag_slot;           // n_free: CSR slot of (i,i) or -1
+    // host mirror of the active pattern (kept for export)
+    std::vector<int32_t> h_rowptr, h_colind;
+};
+
+namespace axg {
+// symbolic phase (host)
+struct Pattern {
+    std::vector<int32_t> rowptr, colind;
+    std::vector<uint32_t> slotptr, contrib;
+};
+void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
+                   const int64_t *ind_of, int64_t n_free, Pattern &out);
+
+// kernels (launch wrappers), axg_element.hip
+void launch_ke_fp64(axg_mesh *m, hipStream_t s, int32_t *d_err);
+void launch_gather_fp64(axg_mesh *m, hipStream_t s, bool topo, uint32_t *d_zero_count);
+void launch_ke_fp32(axg_mesh *m, hipStream_t s, int32_t *d_err);
+void launch_gather_bsr32(axg_mesh *m, hipStream_t s);
+void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
+                         double *d_stress, double *d_vm, int32_t *d_err);
+void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double *d_energy,
+                          int32_t *d_err);
+void upload_shape_tables();
+}  // namespace axg
+
+namespace axg {
+// face pressure of one element, host (axg_loads.cpp)
+int face_pressure_host(int nn, const double *xyz, int face, double p, double *fe);
+}  // namespace axg

@@ -1,0 +1,191 @@
+/*
+ * axia_gpu — C ABI of the MI355X (gfx950) linear-static hot path of axia-fem.
+ *
+ * Plain C types only: pointers, sizes, integer status codes. Every call is
+ * synchronous with respect to the host buffers it receives. A context owns one
+ * HIP stream and the device memory of the meshes created on it; a context is
+ * not shared across threads without external locking, separate contexts may
+ * run concurrently.
+ *
+ * The reference boundary these entry points replace (solver/src/...):
+ *   linalg::solve_kff            linalg.rs:305-404   -> axg_solve_kff
+ *   linalg::csr_from_triplets    linalg.rs:10-37     -> axg_csr_from_triplets
+ *   the per-element element_ke + triplet loop + DofMap::reduce +
+ *   csr_from_triplets of solve_linear (analysis.rs:574-668, 1018-1020;
+ *   elem.rs:199-219; quadratic.rs:122-146; constraint.rs:138-173)
+ *                                                    -> axg_mesh_create + axg_assemble
+ *   element_nodal_stress + nodal averaging (analysis.rs:1061-1162,
+ *   elem.rs:743-779, quadratic.rs:148-244)           -> axg_nodal_stress
+ * Error strings mirror the reference's FemError texts (error.rs:3-18).
+ */
+#ifndef AXIA_GPU_H
+#define AXIA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum axg_status {
+    AXG_OK = 0,
+    AXG_E_INVALID = 1,        /* bad argument, unsupported mesh, size limit     */
+    AXG_E_JAC_SINGULAR = 2,   /* "Jakobimatrix ist singulär (entartetes Element)." elem.rs:14 */
+    AXG_E_NEG_JACOBIAN = 3,   /* "C3D8: negative Jakobideterminante." elem.rs:210 */
+    AXG_E_MATERIAL = 4,       /* "Ungültige Querkontraktion" / "Ungültiger E-Modul" elem.rs:43-48 */
+    AXG_E_NOT_SPD = 5,        /* "CG: unerwartetes Null-Pivot ..." linalg.rs:231-233 */
+    AXG_E_NOT_CONVERGED = 6,  /* "CG konvergierte nicht (...)" linalg.rs:253-257 */
+    AXG_E_SINGULAR = 7,       /* "Steifigkeitsmatrix ist singulär ..." linalg.rs:89-92 */
+    AXG_E_DEVICE = 8,         /* HIP runtime error                              */
+    AXG_E_OOM = 9             /* device allocation failed                       */
+};
+
+enum axg_precision {
+    AXG_PREC_FP64 = 0,   /* FP64 integration, FP64 CSR, FP64 PCG (linalg::pcg)      */
+    AXG_PREC_MIXED = 1   /* FP32 integration + FP32 BSR-3 PCG inside FP64 refinement */
+};
+
+enum axg_precond {
+    AXG_PRECOND_JACOBI = 0,  /* z = r / diag (linalg.rs:213-216)                   */
+    AXG_PRECOND_BLOCK3 = 1   /* 3x3 node-block Jacobi (inverse formed in FP64)      */
+};
+
+typedef struct axg_solve_opts {
+    int32_t precision;   /* enum axg_precision                                       */
+    int32_t precond;     /* enum axg_precond                                         */
+    double tol;          /* stop when ||f - K u||_2 <= tol * ||f||_2 (default 1e-8)  */
+    int64_t max_iter;    /* total PCG iterations; <= 0 means max(4n, 200)            */
+    double inner_tol;    /* mixed: relative residual drop per refinement (1e-4)      */
+    int32_t max_outer;   /* mixed: refinement sweeps (default 30)                    */
+    int32_t timing;      /* 1: record per-kernel HIP events (SpMV durations in info) */
+} axg_solve_opts;
+
+typedef struct axg_solve_info {
+    int64_t iters;        /* PCG iterations (FP64) or total inner iterations (mixed) */
+    int32_t outer_iters;  /* refinement sweeps (mixed), 1 for FP64                  */
+    int32_t nrhs;
+    double residual;      /* final FP64 ||f - K u||_2 (max over right-hand sides)   */
+    double rel_residual;  /* residual / ||f||_2                                     */
+    double solve_ms;      /* device time of the solve (HIP events)                  */
+    double spmv_ms;       /* summed SpMV kernel time when opts.timing = 1           */
+    int64_t spmv_launches;
+    char name[64];        /* e.g. "GPU PCG (FP64, Jacobi)" — SparseResult.name      */
+} axg_solve_info;
+
+typedef struct axg_ctx axg_ctx;
+typedef struct axg_mesh axg_mesh;
+
+/* ------------------------------------------------------------- context */
+/* Thread-local text of the last error (FemError message), "" if none. */
+const char *axg_last_error(void);
+const char *axg_version(void);
+void axg_default_opts(axg_solve_opts *opts);
+int axg_create(int device, axg_ctx **out);
+int axg_destroy(axg_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that enqueue around it. */
+void *axg_stream(axg_ctx *ctx);
+
+/* --------------------------------------------------- solve_kff drop-in */
+/* linalg::solve_kff (linalg.rs:311): triplets (i, j, v) in free-DOF space, any
+ * order, duplicates summed in the given order, exact zeros dropped
+ * (csr_from_triplets rules), then GPU PCG. x_out has n entries. */
+int axg_solve_kff(axg_ctx *ctx, int64_t n, int64_t ntrip, const int64_t *ti, const int64_t *tj,
+                  const double *tv, const double *rhs, const axg_solve_opts *opts, double *x_out,
+                  axg_solve_info *info);
+
+/* linalg::csr_from_triplets (linalg.rs:10-37) on the device. Two-call
+ * protocol: with indices == NULL only indptr (n+1) is written and
+ * indptr[n] = nnz; then call again with indices/data of nnz entries. */
+int axg_csr_from_triplets(axg_ctx *ctx, int64_t n, int64_t ntrip, const int64_t *ti,
+                          const int64_t *tj, const double *tv, int64_t *indptr, int64_t *indices,
+                          double *data);
+
+/* Solve K x = rhs for a host CSR matrix (int64 indptr, int64 indices). */
+int axg_solve_csr(axg_ctx *ctx, int64_t n, const int64_t *indptr, const int64_t *indices,
+                  const double *data, const double *rhs, const axg_solve_opts *opts,
+                  double *x_out, axg_solve_info *info);
+
+/* ------------------------------------------------------------ assembly */
+/* An all-C3D8 (nn = 8) or all-C3D20 (nn = 20) isotropic mesh.
+ * xyz: nnode x 3; conn: nelem x nn, 0-based node indices in the reference's
+ * node order (model.rs:1234-1280 sorts by id); e, nu: per element;
+ * ind_of: 3*nnode entries, free index of DOF 3*node+d or -1 if prescribed
+ * (DofMap::build, constraint.rs:99-108). Builds the K_ff CSR pattern and the
+ * deterministic scatter map on the host (one-time, symbolic). */
+int axg_mesh_create(axg_ctx *ctx, int32_t nn, int64_t nnode, const double *xyz, int64_t nelem,
+                    const int32_t *conn, const double *e, const double *nu, const int64_t *ind_of,
+                    axg_mesh **out);
+int axg_mesh_destroy(axg_mesh *mesh);
+/* n_free, nnz of K_ff, number of scatter contributions, seconds of the symbolic phase. */
+int axg_mesh_info(axg_mesh *mesh, int64_t *n_free, int64_t *nnz, int64_t *ncontrib,
+                  double *symbolic_s);
+/* Replace per-element moduli (e.g. SIMP E = Emin + rho^3 E0); call axg_assemble again. */
+int axg_mesh_set_modulus(axg_mesh *mesh, const double *e);
+/* Numeric assembly on the device: Ke of every element and the gather into the
+ * CSR slots, summed in ascending element order (csr_from_triplets order).
+ * precision AXG_PREC_FP64 builds the FP64 CSR; AXG_PREC_MIXED also builds the
+ * FP32 BSR-3 inner operator from FP32 (MFMA) integration. elapsed_ms (may be
+ * NULL) receives the device time of the numeric assembly. */
+int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms);
+/* Copy the FP64 K_ff CSR to the host (indptr n+1, indices nnz, data nnz). */
+int axg_csr_export(axg_mesh *mesh, int64_t *indptr, int64_t *indices, double *data);
+/* Copy the scatter map: slot_ptr (nnz+1), contrib (ncontrib) where each
+ * contribution is el * (3nn)^2 + i * 3nn + j (element, local row, local col). */
+int axg_scatter_map_export(axg_mesh *mesh, int64_t *slot_ptr, int64_t *contrib);
+/* Copy the FP64 element matrices of elements [first, first+count) (row-major). */
+int axg_ke_export(axg_mesh *mesh, int64_t first, int64_t count, double *ke);
+
+/* Solve K_ff x = rhs for nrhs right-hand sides (column k at rhs + k*n_free). */
+int axg_mesh_solve(axg_mesh *mesh, const double *rhs, int32_t nrhs, const axg_solve_opts *opts,
+                   double *x_out, axg_solve_info *info);
+/* Same with device pointers (HBM-resident inputs and outputs). */
+int axg_mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
+                          const axg_solve_opts *opts, double *d_x, axg_solve_info *info);
+
+/* Stress recovery (analysis.rs:1061-1162): per-element nodal stresses
+ * (elem_nodal: nelem x nn x 6, may be NULL), their plain mean per node
+ * (stress: nnode x 6) and von Mises per node (vm: nnode, may be NULL).
+ * u_full: 3*nnode displacements in the full DOF space. */
+int axg_nodal_stress(axg_mesh *mesh, const double *u_full, double *stress, double *vm,
+                     double *elem_nodal);
+
+/* Compliance adjoint (no counterpart in the reference; new path). With
+ * K = sum_e E_e K1_e and c = f^T u, the system is self-adjoint, so
+ * dc/dE_e = -u_e^T K1_e u_e. Computes c from f_full and u_full and writes
+ * dc_de (nelem). u_full, f_full: 3*nnode entries. */
+int axg_compliance_adjoint(axg_mesh *mesh, const double *u_full, const double *f_full, double *c,
+                           double *dc_de);
+
+/* Batched solves: nmesh meshes (assembled), each with nrhs[m] right-hand
+ * sides rhs[m] (n_free[m] * nrhs[m], column-major by right-hand side),
+ * solved in one graph-captured launch sequence with per-(mesh, rhs)
+ * convergence. x[m] receives the solutions, iters (sum nrhs entries) the
+ * iteration count of each solve, solve_ms[] (sum nrhs entries, may be NULL)
+ * the device time at which each solve converged. */
+int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *meshes,
+                    const double *const *rhs, const int32_t *nrhs, const axg_solve_opts *opts,
+                    double *const *x, int64_t *iters, double *solve_ms, axg_solve_info *info);
+
+/* ------------------------------------------------- host-only helpers */
+/* The symbolic phase alone, on the host (no device): K_ff pattern and scatter
+ * map as built by axg_mesh_create, before any exact-zero slot is dropped.
+ * Two-call protocol: pass NULL arrays to get n_free, nnz, ncontrib; then
+ * rowptr (n_free+1), colind (nnz), slot_ptr (nnz+1), contrib (ncontrib). */
+int axg_pattern_host(int32_t nn, int64_t nnode, int64_t nelem, const int32_t *conn,
+                     const int64_t *ind_of, int64_t *n_free, int64_t *nnz, int64_t *ncontrib,
+                     int64_t *rowptr, int64_t *colind, int64_t *slot_ptr, int64_t *contrib);
+
+/* Face-pressure loads (elem.rs:892-950, quadratic.rs:266-315; applied by
+ * apply_pressure/scatter_fe, analysis.rs:723-760, 1365-1450, 1546-1553):
+ * for each loaded face k, elem_conn[k*nn..] are the element's node indices,
+ * face[k] in 1..6 (CalculiX Pn), mag[k] the pressure (positive into the
+ * element). Accumulated into f_full (3*nnode) in the given order. */
+int axg_pressure_loads(int32_t nn, int64_t nnode, const double *xyz, int64_t nface,
+                       const int32_t *elem_conn, const int32_t *face, const double *mag,
+                       double *f_full);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AXIA_GPU_H */

@@ -1,0 +1,870 @@
+"""CPU FP64 oracle for the axia linear-static hot path — TEST INFRASTRUCTURE.
+
+Python side of the oracle: a ctypes binding of ``liboracle`` (the C
+restatement of the reference's element, CSR and PCG routines, built from
+``oracle/axia_oracle.c``), a restatement of the reference's CalculiX INP
+subset reader, and of the driver ``solve_linear`` for all-Hex8 / all-Hex20
+isotropic decks.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module; it is the checker, never the product.
+
+Reference files followed (solver/src/...):
+  inp.rs:11-309, 383-1061, 2256-2330   deck subset, deferred sets
+  model.rs:1234-1309, 1391-1407        compact (sorting), elset rebinding,
+                                       material lookup
+  analysis.rs:548-668, 723-760, 926-1035, 1061-1162, 2164-2355, 2709-2764
+  constraint.rs:17-218                  DofMap (prescribed DOFs only)
+  linalg.rs:10-37, 194-259, 305-404     CSR, PCG, solve_kff dispatch
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libaxia_oracle.so")
+_lib = None
+
+ERRORS = {
+    1: "invalid input",
+    2: "Jakobimatrix ist singulär (entartetes Element).",
+    3: "negative Jakobideterminante.",
+    4: "Ungültige Materialkonstanten",
+    5: "CG: unerwartetes Null-Pivot — Matrix nicht positiv definit.",
+    6: "CG konvergierte nicht",
+    7: "Steifigkeitsmatrix ist indefinit.",
+    8: "Steifigkeitsmatrix ist singulär — Randbedingungen unzureichend "
+       "(Starrkörperbewegung oder entartete Elemente).",
+    9: "out of memory",
+}
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg=None):
+        self.code = code
+        super().__init__(msg or ERRORS.get(code, f"oracle error {code}"))
+
+
+def build():
+    """Compile liboracle with the committed Makefile (gcc, no GPU needed)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        lp = C.POINTER(C.c_int64)
+        ip = C.POINTER(C.c_int32)
+        i64 = C.c_int64
+        L.or_d_iso_3d.argtypes = [C.c_double, C.c_double, dp]
+        L.or_hex8_shape.argtypes = [C.c_double] * 3 + [dp, dp]
+        L.or_hex20_shape.argtypes = [C.c_double] * 3 + [dp, dp]
+        L.or_hex_dndx.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+        L.or_hex_stiffness.argtypes = [C.c_int, dp, dp, dp, dp]
+        L.or_hex_face_pressure.argtypes = [C.c_int, dp, C.c_int, C.c_double, dp]
+        L.or_hex_nodal_stress.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp]
+        L.or_von_mises.argtypes = [dp]
+        L.or_von_mises.restype = C.c_double
+        L.or_assemble_triplets.argtypes = [C.c_int, i64, dp, i64, ip, dp, dp, lp, lp, dp, lp, lp]
+        L.or_reduce.argtypes = [i64, lp, dp, i64, i64, lp, lp, dp, dp, lp, lp, dp, lp, dp]
+        L.or_csr_from_triplets.argtypes = [i64, i64, lp, lp, dp, lp, lp, dp]
+        L.or_csr_matvec.argtypes = [i64, lp, lp, dp, dp, dp]
+        L.or_pcg.argtypes = [i64, lp, lp, dp, dp, C.c_double, i64, dp, lp, dp]
+        L.or_chol_solve.argtypes = [i64, dp, dp, dp]
+        L.or_element_ke_batch.argtypes = [C.c_int, i64, dp, i64, ip, dp, dp, dp, C.c_int]
+        L.or_pcg_omp.argtypes = [i64, lp, lp, dp, dp, C.c_double, i64, i64, dp, lp, dp]
+        L.or_omp_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _l(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _check(st):
+    if st != 0:
+        raise OracleError(st)
+
+
+# ----------------------------------------------------------------- elements
+def d_iso_3d(e, nu):
+    d = np.zeros(36)
+    _check(lib().or_d_iso_3d(e, nu, _d(d)))
+    return d.reshape(6, 6)
+
+
+def hex_shape(nn, xi, eta, zeta):
+    n = np.zeros(nn)
+    dn = np.zeros(3 * nn)
+    f = lib().or_hex8_shape if nn == 8 else lib().or_hex20_shape
+    f(xi, eta, zeta, _d(n), _d(dn))
+    return n, dn.reshape(nn, 3)
+
+
+def hex_dndx(nn, xyz, xi, eta, zeta):
+    xyz = _f64(xyz)
+    dndx = np.zeros(3 * nn)
+    det = np.zeros(1)
+    n = np.zeros(nn)
+    _check(lib().or_hex_dndx(nn, _d(xyz), xi, eta, zeta, _d(dndx), _d(det), _d(n)))
+    return dndx.reshape(nn, 3), float(det[0]), n
+
+
+def hex_stiffness(xyz, e, nu, d=None):
+    xyz = _f64(xyz)
+    nn = xyz.shape[0]
+    if d is None:
+        d = d_iso_3d(e, nu)
+    d = _f64(d)
+    ke = np.zeros((3 * nn) ** 2)
+    vol = np.zeros(1)
+    _check(lib().or_hex_stiffness(nn, _d(xyz), _d(d), _d(ke), _d(vol)))
+    return ke.reshape(3 * nn, 3 * nn), float(vol[0])
+
+
+def hex_face_pressure(xyz, face, p):
+    xyz = _f64(xyz)
+    nn = xyz.shape[0]
+    fe = np.zeros(3 * nn)
+    _check(lib().or_hex_face_pressure(nn, _d(xyz), int(face), float(p), _d(fe)))
+    return fe
+
+
+def hex_nodal_stress(xyz, ue, e, nu):
+    xyz = _f64(xyz)
+    ue = _f64(ue)
+    nn = xyz.shape[0]
+    out = np.zeros(nn * 6)
+    _check(lib().or_hex_nodal_stress(nn, _d(xyz), _d(ue), float(e), float(nu), _d(out)))
+    return out.reshape(nn, 6)
+
+
+def von_mises(s):
+    s = _f64(s)
+    return lib().or_von_mises(_d(s))
+
+
+# ----------------------------------------------------------------- assembly
+def assemble_triplets(nn, xyz, conn, e, nu):
+    """Full-space triplets of every element, analysis.rs:574-668."""
+    xyz = _f64(xyz).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, nn)
+    nelem = conn.shape[0]
+    e = _f64(np.broadcast_to(e, (nelem,)))
+    nu = _f64(np.broadcast_to(nu, (nelem,)))
+    cap = nelem * (3 * nn) ** 2
+    ti = np.empty(cap, np.int64)
+    tj = np.empty(cap, np.int64)
+    tv = np.empty(cap, np.float64)
+    nt = np.zeros(1, np.int64)
+    bad = np.full(1, -1, np.int64)
+    st = lib().or_assemble_triplets(nn, xyz.shape[0], _d(xyz), nelem, _i(conn), _d(e), _d(nu),
+                                    _l(ti), _l(tj), _d(tv), _l(nt), _l(bad))
+    if st:
+        raise OracleError(st, f"{ERRORS.get(st)} (element index {int(bad[0])})")
+    k = int(nt[0])
+    return ti[:k], tj[:k], tv[:k]
+
+
+def reduce(ind_of, u0, n_free, ti, tj, tv, f_full):
+    ind_of = np.ascontiguousarray(ind_of, np.int64)
+    u0 = _f64(u0)
+    f_full = _f64(f_full)
+    ti = np.ascontiguousarray(ti, np.int64)
+    tj = np.ascontiguousarray(tj, np.int64)
+    tv = _f64(tv)
+    nt = len(ti)
+    oi = np.empty(nt, np.int64)
+    oj = np.empty(nt, np.int64)
+    ov = np.empty(nt, np.float64)
+    nout = np.zeros(1, np.int64)
+    rhs = np.zeros(n_free)
+    _check(lib().or_reduce(len(ind_of), _l(ind_of), _d(u0), n_free, nt, _l(ti), _l(tj), _d(tv),
+                           _d(f_full), _l(oi), _l(oj), _d(ov), _l(nout), _d(rhs)))
+    k = int(nout[0])
+    return oi[:k], oj[:k], ov[:k], rhs
+
+
+@dataclass
+class Csr:
+    n: int
+    indptr: np.ndarray
+    indices: np.ndarray
+    data: np.ndarray
+
+    @property
+    def nnz(self):
+        return int(self.indptr[-1])
+
+    def matvec(self, x):
+        y = np.zeros(self.n)
+        x = _f64(x)
+        lib().or_csr_matvec(self.n, _l(self.indptr), _l(self.indices), _d(self.data), _d(x), _d(y))
+        return y
+
+    def to_dense(self):
+        a = np.zeros((self.n, self.n))
+        for i in range(self.n):
+            s, t = self.indptr[i], self.indptr[i + 1]
+            a[i, self.indices[s:t]] = self.data[s:t]
+        return a
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.data, self.indices, self.indptr), shape=(self.n, self.n))
+
+
+def csr_from_triplets(n, ti, tj, tv):
+    """linalg.rs:10-37 (exact-zero filter, stable (i, j) sort, left-to-right sums)."""
+    ti = np.ascontiguousarray(ti, np.int64)
+    tj = np.ascontiguousarray(tj, np.int64)
+    tv = _f64(tv)
+    indptr = np.zeros(n + 1, np.int64)
+    L = lib()
+    _check(L.or_csr_from_triplets(n, len(ti), _l(ti), _l(tj), _d(tv), _l(indptr), None, None))
+    nnz = int(indptr[n])
+    indices = np.empty(max(nnz, 1), np.int64)
+    data = np.empty(max(nnz, 1), np.float64)
+    _check(L.or_csr_from_triplets(n, len(ti), _l(ti), _l(tj), _d(tv), _l(indptr), _l(indices),
+                                  _d(data)))
+    return Csr(n, indptr, indices[:nnz], data[:nnz])
+
+
+def pcg(csr, b, tol=1e-8, max_iter=None):
+    """linalg.rs:194-259. Returns (x, iters, residual)."""
+    n = csr.n
+    if max_iter is None:
+        max_iter = max(4 * n, 200)
+    b = _f64(b)
+    x = np.zeros(n)
+    it = np.zeros(1, np.int64)
+    res = np.zeros(1)
+    st = lib().or_pcg(n, _l(csr.indptr), _l(csr.indices), _d(csr.data), _d(b), tol, max_iter,
+                      _d(x), _l(it), _d(res))
+    if st == 6:
+        raise OracleError(st, f"CG konvergierte nicht (Residuum {res[0]:.3e} nach {it[0]} "
+                              "Iterationen).")
+    _check(st)
+    return x, int(it[0]), float(res[0])
+
+
+def chol_solve(a, b):
+    """linalg.rs:78-119 (dense lower Cholesky)."""
+    a = np.array(a, dtype=np.float64, order="C", copy=True)
+    n = a.shape[0]
+    b = _f64(b)
+    x = np.zeros(n)
+    _check(lib().or_chol_solve(n, _d(a), _d(b), _d(x)))
+    return x
+
+
+def residual_of(csr, x, b):
+    """linalg.rs:268-278."""
+    d = csr.matvec(x) - b
+    return float(math.sqrt(float(np.sum(d * d))))
+
+
+# ---------------------------------------------------------------- INP reader
+def _parse_f64(s):
+    """inp.rs:100-141: Fortran floats (1.d0, 1.0D-3, 1., .5, 1.0+3)."""
+    raw = s.strip()
+    if not raw:
+        return None
+    for cand in (raw.rstrip("."), raw):
+        try:
+            return float(cand)
+        except ValueError:
+            pass
+    b = raw.replace("d", "e").replace("D", "e").rstrip(".")
+    try:
+        return float(b)
+    except ValueError:
+        pass
+    for i in range(1, len(b)):
+        if b[i] in "+-" and (b[i - 1].isdigit() or b[i - 1] == "."):
+            try:
+                return float(b[:i] + "e" + b[i:])
+            except ValueError:
+                return None
+    return None
+
+
+def _parse_i32(s):
+    t = s.strip()
+    if re.fullmatch(r"[+-]?\d+", t):
+        return int(t)
+    f = _parse_f64(t)
+    if f is None:
+        raise ValueError(f"Keine ganze Zahl: {s}")
+    return int(round(f))
+
+
+def _is_int(s):
+    return re.fullmatch(r"[+-]?\d+", s.strip()) is not None
+
+
+def _tokens(line):
+    return [t for t in re.split(r"[,\s]+", line) if t]
+
+
+def _is_kw(line):
+    t = line.strip()
+    return t.startswith("*") and not t.startswith("**")
+
+
+def _keyword(line):
+    t = line.strip().lstrip("﻿").upper()
+    parts = t.split(",")
+    kw = parts[0].strip()
+    params = {}
+    for p in parts[1:]:
+        p = p.strip()
+        if not p:
+            continue
+        if "=" in p:
+            k, v = p.split("=", 1)
+            params[k.strip()] = v.strip()
+        else:
+            params[p] = ""
+    kw = {"*SOLIDSECTION": "*SOLID SECTION", "*ENDSTEP": "*END STEP"}.get(kw, kw)
+    return kw, params
+
+
+@dataclass
+class Model:
+    node_ids: list = field(default_factory=list)
+    coords: list = field(default_factory=list)
+    elements: list = field(default_factory=list)  # (id, kind_nn, nodes(list of ids), elset)
+    nsets: dict = field(default_factory=dict)
+    elsets: dict = field(default_factory=dict)
+    materials: dict = field(default_factory=dict)  # name -> (E, nu)
+    elset_material: dict = field(default_factory=dict)
+    bcs: list = field(default_factory=list)  # (node id, dof0, value)
+    cloads: list = field(default_factory=list)  # (node id, dof0, mag)
+    dloads: list = field(default_factory=list)  # (elem id, face, mag)
+    id_to_index: dict = field(default_factory=dict)
+    deferred: list = field(default_factory=list)
+
+    def node_index(self, nid):
+        return self.id_to_index[nid]
+
+    def expand_nset(self, name):
+        if _is_int(name):
+            return [int(name)]
+        up = name.upper()
+        if up in self.nsets:
+            return list(self.nsets[up])
+        if up in ("NALL", "ALL"):
+            return list(self.node_ids)
+        raise ValueError(f"Unbekanntes Knotenset {name}")
+
+    def material_for(self, elset):
+        """model.rs:1391-1407."""
+        m = self.elset_material.get(elset)
+        if m is not None and m in self.materials:
+            return self.materials[m]
+        if len(self.materials) == 1:
+            return next(iter(self.materials.values()))
+        if "STEEL" in self.materials:
+            return self.materials["STEEL"]
+        raise ValueError(f"Kein Material für ELSET={elset}")
+
+    # -- arrays for the hot path
+    def mesh_arrays(self):
+        """(nn, xyz[nnode,3], conn[nelem,nn] 0-based, E[nelem], nu[nelem])."""
+        kinds = {el[1] for el in self.elements}
+        if len(kinds) != 1 or next(iter(kinds)) not in (8, 20):
+            raise ValueError("only all-C3D8 or all-C3D20 decks are on this path")
+        nn = next(iter(kinds))
+        xyz = np.asarray(self.coords, dtype=np.float64).reshape(-1, 3)
+        conn = np.array([[self.id_to_index[i] for i in el[2]] for el in self.elements],
+                        dtype=np.int32).reshape(-1, nn)
+        mats = [self.material_for(el[3]) for el in self.elements]
+        e = np.array([m[0] for m in mats], dtype=np.float64)
+        nu = np.array([m[1] for m in mats], dtype=np.float64)
+        return nn, xyz, conn, e, nu
+
+
+_ELEM_NN = {"C3D8": 8, "C3D20": 20}
+
+
+def parse_inp(text):
+    """Restates inp.rs:383-1061 for the keyword subset on this path."""
+    m = Model()
+    lines = text.splitlines()
+    n = len(lines)
+    i = 0
+    cur_mat = None
+
+    def collect(j):
+        toks = []
+        while j < n:
+            raw = lines[j]
+            if raw.strip().startswith("**") or not raw.strip():
+                j += 1
+                continue
+            if _is_kw(raw):
+                break
+            toks.extend(_tokens(raw))
+            j += 1
+        return toks, j
+
+    def gen_ids(toks):
+        ids = []
+        k = 0
+        while k + 1 < len(toks):
+            a, b = _parse_i32(toks[k]), _parse_i32(toks[k + 1])
+            if k + 2 < len(toks):
+                s = _parse_i32(toks[k + 2]) or 1
+                k += 3
+            else:
+                s = 1
+                k += 2
+            v = a
+            if s > 0:
+                while v <= b:
+                    ids.append(v)
+                    v += s
+            else:
+                while v >= b:
+                    ids.append(v)
+                    v += s
+        return ids
+
+    while i < n:
+        raw = lines[i]
+        if raw.strip().startswith("**") or not raw.strip() or not _is_kw(raw):
+            i += 1
+            continue
+        kw, params = _keyword(raw)
+        if kw == "*NODE":
+            nset = params.get("NSET")
+            i += 1
+            added = []
+            while i < n:
+                raw = lines[i]
+                if raw.strip().startswith("**") or not raw.strip():
+                    i += 1
+                    continue
+                if _is_kw(raw):
+                    break
+                t = _tokens(raw)
+                i += 1
+                if not t:
+                    continue
+                nid = _parse_i32(t[0])
+                xyz = [(_parse_f64(t[k]) if len(t) > k else 0.0) or 0.0 for k in (1, 2, 3)]
+                m.node_ids.append(nid)
+                m.coords.append(xyz)
+                added.append(nid)
+            if nset:
+                m.nsets.setdefault(nset, []).extend(added)
+        elif kw == "*ELEMENT":
+            typ = params.get("TYPE")
+            if typ not in _ELEM_NN:
+                raise ValueError(f"Nicht unterstützter Elementtyp {typ} (this path: C3D8, C3D20).")
+            nn = _ELEM_NN[typ]
+            elset = params.get("ELSET", "EALL")
+            toks, i = collect(i + 1)
+            k = 0
+            added = []
+            while k + nn < len(toks):
+                eid = _parse_i32(toks[k])
+                nodes = [_parse_i32(t) for t in toks[k + 1:k + 1 + nn]]
+                k += 1 + nn
+                m.elements.append((eid, nn, nodes, elset))
+                added.append(eid)
+            m.elsets.setdefault(elset, []).extend(added)
+        elif kw in ("*NSET", "*ELSET"):
+            key = kw[1:]
+            name = params.get(key)
+            if name is None:
+                raise ValueError(f"{kw} ohne {key}=")
+            toks, i = collect(i + 1)
+            if "GENERATE" in params:
+                ids = gen_ids(toks)
+            else:
+                ids = []
+                for t in toks:
+                    try:
+                        ids.append(_parse_i32(t))
+                    except ValueError:
+                        m.deferred.append(("NESTED_" + key, name, t.upper()))
+            (m.nsets if key == "NSET" else m.elsets).setdefault(name, []).extend(ids)
+        elif kw == "*MATERIAL":
+            cur_mat = params.get("NAME", "MATERIAL-1")
+            m.materials.setdefault(cur_mat, (0.0, 0.0))
+            i += 1
+        elif kw == "*ELASTIC":
+            toks, i = collect(i + 1)
+            if len(toks) < 2:
+                raise ValueError("*ELASTIC benötigt E und nu")
+            e = _parse_f64(toks[0])
+            nu = _parse_f64(toks[1])
+            if abs(e) <= 1e-18:
+                cands = [v for v in (_parse_f64(t) for t in toks) if v is not None and abs(v) > 1e-18]
+                e = cands[0] if cands else 1.0
+            nu = min(max(nu, -0.49), 0.49)
+            m.materials[cur_mat or "MATERIAL-1"] = (e, nu)
+        elif kw == "*SOLID SECTION":
+            elset = params.get("ELSET", "EALL")
+            if "MATERIAL" in params:
+                m.elset_material[elset] = params["MATERIAL"]
+            _, i = collect(i + 1)
+        elif kw == "*BOUNDARY":
+            i += 1
+            while i < n:
+                raw = lines[i]
+                if raw.strip().startswith("**") or not raw.strip():
+                    i += 1
+                    continue
+                if _is_kw(raw):
+                    break
+                card = _boundary_card(_tokens(raw))
+                if card:
+                    _push_boundary(m, *card)
+                i += 1
+        elif kw == "*CLOAD":
+            toks, i = collect(i + 1)
+            k = 0
+            while k + 2 < len(toks):
+                name = toks[k].upper()
+                dof = _parse_i32(toks[k + 1])
+                mag = _parse_f64(toks[k + 2])
+                k += 3
+                if dof < 1 or dof > 6:
+                    continue
+                if _is_int(name):
+                    m.cloads.append((int(name), dof - 1, mag))
+                elif m.nsets.get(name):
+                    for nid in m.nsets[name]:
+                        m.cloads.append((nid, dof - 1, mag))
+                else:
+                    m.deferred.append(("CL", name, dof - 1, mag))
+        elif kw == "*DLOAD":
+            toks, i = collect(i + 1)
+            k = 0
+            while k + 1 < len(toks):
+                name = toks[k].upper()
+                typ = toks[k + 1].upper()
+                k += 2
+                mt = re.fullmatch(r"P(\d+)", typ)
+                if not mt:
+                    raise ValueError(f"DLOAD-Typ {typ} is not on this path")
+                face = int(mt.group(1))
+                mag = _parse_f64(toks[k])
+                k += 1
+                if _is_int(name):
+                    m.dloads.append((int(name), face, mag))
+                elif m.elsets.get(name):
+                    for eid in m.elsets[name]:
+                        m.dloads.append((eid, face, mag))
+                else:
+                    m.deferred.append(("DL", name, face, mag))
+        else:
+            i += 1
+    _compact(m)
+    _expand_deferred(m)
+    return m
+
+
+def _boundary_card(toks):
+    """inp.rs:204-230."""
+    if len(toks) < 2:
+        return None
+    name = toks[0].upper()
+    d1 = _parse_i32(toks[1])
+
+    def dof_id(s):
+        if not _is_int(s):
+            return None
+        v = int(s)
+        return v if (1 <= v <= 6 or v == 11) else None
+
+    if len(toks) == 2:
+        return name, d1, d1, 0.0
+    if len(toks) == 3:
+        d2 = dof_id(toks[2])
+        if d2 is not None:
+            return name, d1, d2, 0.0
+        return name, d1, d1, _parse_f64(toks[2]) or 0.0
+    d2 = dof_id(toks[2])
+    return name, d1, d1 if d2 is None else d2, _parse_f64(toks[3]) or 0.0
+
+
+def _push_boundary(m, name, first, last, val):
+    """inp.rs:232-268."""
+    first = max(first, 1)
+    last = max(last, first)
+    if _is_int(name):
+        ids = [int(name)]
+    elif m.nsets.get(name):
+        ids = list(m.nsets[name])
+    else:
+        for d in range(first, last + 1):
+            if 1 <= d <= 6:
+                m.deferred.append(("BC", name, d - 1, val))
+        return
+    for nid in ids:
+        for d in range(first, last + 1):
+            if 1 <= d <= 6:
+                m.bcs.append((nid, d - 1, val))
+
+
+def _compact(m):
+    """model.rs:1234-1309: sort nodes by id (first wins), elements by id, rebind elsets."""
+    pairs = sorted(zip(m.node_ids, m.coords), key=lambda p: p[0])
+    ids, coords, seen = [], [], set()
+    for nid, c in pairs:
+        if nid in seen:
+            continue
+        seen.add(nid)
+        ids.append(nid)
+        coords.append(c)
+    m.node_ids, m.coords = ids, coords
+    m.id_to_index = {nid: k for k, nid in enumerate(ids)}
+    m.elements.sort(key=lambda e: e[0])
+    m.nsets["NALL"] = list(ids)
+    m.elsets["EALL"] = [e[0] for e in m.elements]
+    sectioned = set(m.elset_material)
+    if sectioned:
+        names = sorted(sectioned, key=lambda a: (0 if a in m.elset_material else 1, a))
+        id_to_set = {}
+        for name in names:
+            for eid in m.elsets.get(name, []):
+                id_to_set.setdefault(eid, name)
+        m.elements = [(e[0], e[1], e[2], e[3] if e[3] in sectioned else id_to_set.get(e[0], e[3]))
+                      for e in m.elements]
+
+
+def _expand_deferred(m):
+    """inp.rs:2256-2330 (the subset: nested sets, BC, CLOAD, DLOAD)."""
+    for item in m.deferred:
+        if item[0] == "NESTED_NSET":
+            m.nsets.setdefault(item[1], []).extend(m.expand_nset(item[2]))
+        elif item[0] == "NESTED_ELSET":
+            m.elsets.setdefault(item[1], []).extend(m.elsets.get(item[2], []))
+    for item in m.deferred:
+        if item[0] == "BC":
+            for nid in m.expand_nset(item[1]):
+                m.bcs.append((nid, item[2], item[3]))
+        elif item[0] == "CL":
+            for nid in m.expand_nset(item[1]):
+                m.cloads.append((nid, item[2], item[3]))
+        elif item[0] == "DL":
+            for eid in m.elsets.get(item[1], []):
+                m.dloads.append((eid, item[2], item[3]))
+    m.deferred = []
+
+
+# ------------------------------------------------------ linear static driver
+@dataclass
+class LinearSetup:
+    """Everything solve_linear builds before the solve (full DOF space)."""
+    nn: int
+    xyz: np.ndarray
+    conn: np.ndarray
+    e: np.ndarray
+    nu: np.ndarray
+    ndof: int
+    f_full: np.ndarray
+    prescribed: dict
+    ind_of: np.ndarray
+    u0: np.ndarray
+    n_free: int
+    trips: tuple  # full-space triplets (ti, tj, tv)
+
+
+def linear_setup(nn, xyz, conn, e, nu, bcs, cloads, dloads, autospc=True):
+    """analysis.rs:548-946 for an all-hex isotropic model.
+
+    bcs: iterable of (node index, dof0, value) in deck order (last wins)
+    cloads: iterable of (node index, dof0, mag)
+    dloads: iterable of (element index, face, mag) in deck order
+    """
+    xyz = _f64(xyz).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, np.int32).reshape(-1, nn)
+    nnode = xyz.shape[0]
+    nelem = conn.shape[0]
+    ndof = 3 * nnode
+    e = _f64(np.broadcast_to(e, (nelem,)))
+    nu = _f64(np.broadcast_to(nu, (nelem,)))
+    prescribed = {}
+    for ni, d, v in bcs:  # analysis.rs:555-563, last BC wins
+        if d < 3:
+            prescribed[3 * ni + d] = float(v)
+    # pin_unused_dofs (analysis.rs:2230-2355): isolated nodes
+    attached = np.zeros(nnode, bool)
+    attached[conn.ravel()] = True
+    for ni in np.nonzero(~attached)[0]:
+        for d in range(3):
+            prescribed.setdefault(3 * int(ni) + d, 0.0)
+    # element loop: pressure loads in element order, then dload order
+    f_full = np.zeros(ndof)
+    by_elem = {}
+    for k, (el, face, mag) in enumerate(dloads):
+        by_elem.setdefault(int(el), []).append((face, mag))
+    for el in sorted(by_elem):
+        nodes = conn[el]
+        for face, mag in by_elem[el]:
+            fe = hex_face_pressure(xyz[nodes], face, mag)
+            for a in range(nn):  # scatter_fe, analysis.rs:1546-1553
+                for d in range(3):
+                    f_full[3 * nodes[a] + d] += fe[3 * a + d]
+    ti, tj, tv = assemble_triplets(nn, xyz, conn, e, nu)
+    # add_cloads (analysis.rs:2709-2764): per (node, dof) sums, then added once
+    acc = {}
+    for ni, d, mag in cloads:
+        if d < 3:
+            acc[(ni, d)] = acc.get((ni, d), 0.0) + mag * 1.0
+    for (ni, d), mag in acc.items():
+        f_full[3 * ni + d] += mag
+    # autospc_null_diag (analysis.rs:2164-2228)
+    if autospc:
+        diag = np.zeros(ndof)
+        sel = ti == tj
+        for i, v in zip(ti[sel], tv[sel]):
+            diag[i] += v
+        max_d = float(np.max(np.abs(diag))) if ndof else 0.0
+        if max_d >= 1e-30:
+            tol = max_d * 1e-14
+            for i in np.nonzero(np.abs(diag) <= tol)[0]:
+                prescribed.setdefault(int(i), 0.0)
+    # DofMap::build (constraint.rs:26-136) without MPCs
+    u0 = np.zeros(ndof)
+    is_ind = np.ones(ndof, bool)
+    for d, v in prescribed.items():
+        if d < ndof:
+            u0[d] = v
+            is_ind[d] = False
+    ind_of = np.full(ndof, -1, np.int64)
+    n_free = int(is_ind.sum())
+    ind_of[is_ind] = np.arange(n_free)
+    if n_free == 0 and not prescribed:
+        raise ValueError("Modell ohne freie Freiheitsgrade.")
+    return LinearSetup(nn, xyz, conn, e, nu, ndof, f_full, prescribed, ind_of, u0, n_free,
+                       (ti, tj, tv))
+
+
+@dataclass
+class LinearResult:
+    setup: LinearSetup
+    csr: Csr
+    rhs: np.ndarray
+    x: np.ndarray
+    u_full: np.ndarray
+    rf_full: np.ndarray
+    stress: np.ndarray  # nodal averaged, nnode x 6
+    elem_nodal_stress: np.ndarray  # nelem x nn x 6
+    von_mises: np.ndarray
+    solver: str
+    iters: int
+    residual: float
+
+
+def solve_kff(n, csr, rhs, solver="auto", tol=1e-8):
+    """linalg.rs:311-404 with faer replaced by the dense / LAPACK stand-ins.
+
+    solver: "cholesky" (reference chol_solve, n <= 900 as in the reference),
+    "lapack" (numpy Cholesky, faer stand-in for n up to ~2e4), "pcg"
+    (reference linalg::pcg), "auto" (cholesky if n <= 900 else lapack).
+    """
+    if solver == "auto":
+        solver = "cholesky" if n <= 900 else "lapack"
+    if solver == "cholesky":
+        x = chol_solve(csr.to_dense(), rhs)
+        return x, "Cholesky", 1, residual_of(csr, x, rhs)
+    if solver == "lapack":
+        a = csr.to_dense()
+        lchol = np.linalg.cholesky(a)
+        y = np.linalg.solve(lchol, rhs)
+        x = np.linalg.solve(lchol.T, y)
+        return x, "dense LAPACK Cholesky (faer stand-in)", 1, residual_of(csr, x, rhs)
+    if solver == "superlu":
+        import scipy.sparse.linalg as sla
+        x = sla.spsolve(csr.to_scipy().tocsc(), rhs)
+        return x, "scipy SuperLU (faer stand-in)", 1, residual_of(csr, x, rhs)
+    if solver == "pcg":
+        x, it, res = pcg(csr, rhs, tol, max(4 * n, 200))
+        return x, "PCG", it, res
+    raise ValueError(solver)
+
+
+def linear_solve(setup: LinearSetup, solver="auto", tol=1e-8):
+    """analysis.rs:1018-1162: reduce, solve_kff, reconstruct, reactions, stresses."""
+    s = setup
+    ti, tj, tv = s.trips
+    oi, oj, ov, rhs = reduce(s.ind_of, s.u0, s.n_free, ti, tj, tv, s.f_full)
+    csr = csr_from_triplets(s.n_free, oi, oj, ov)
+    x, name, iters, res = solve_kff(s.n_free, csr, rhs, solver, tol)
+    u_full = s.u0.copy()
+    free = s.ind_of >= 0
+    u_full[free] = u_full[free] + 1.0 * x[s.ind_of[free]]
+    ku = np.zeros(s.ndof)
+    np.add.at(ku, ti, tv * u_full[tj])  # same order as the sequential loop per index
+    rf = ku - s.f_full
+    stress, sn_all, vm = nodal_stress(s.nn, s.xyz, s.conn, s.e, s.nu, u_full)
+    return LinearResult(s, csr, rhs, x, u_full, rf, stress, sn_all, vm, name, iters, res)
+
+
+def nodal_stress(nn, xyz, conn, e, nu, u_full):
+    """analysis.rs:1061-1162: per-element nodal stress, plain mean per node."""
+    nnode = xyz.shape[0]
+    acc = np.zeros((nnode, 6))
+    cnt = np.zeros(nnode)
+    nelem = conn.shape[0]
+    sn_all = np.zeros((nelem, nn, 6))
+    u3 = u_full.reshape(-1, 3)
+    for el in range(nelem):
+        nodes = conn[el]
+        ue = u3[nodes].ravel()
+        sn = hex_nodal_stress(xyz[nodes], ue, e[el], nu[el])
+        sn_all[el] = sn
+        for a in range(nn):
+            acc[nodes[a]] += sn[a]
+            cnt[nodes[a]] += 1.0
+    stress = np.zeros((nnode, 6))
+    has = cnt > 0
+    stress[has] = acc[has] / cnt[has, None]
+    vm = np.array([von_mises(s) for s in stress])
+    return stress, sn_all, vm
+
+
+def deck_setup(model: Model):
+    """Model (from parse_inp) -> LinearSetup, mapping ids to indices."""
+    nn, xyz, conn, e, nu = model.mesh_arrays()
+    bcs = [(model.node_index(nid), d, v) for nid, d, v in model.bcs]
+    cl = [(model.node_index(nid), d, mag) for nid, d, mag in model.cloads]
+    eidx = {el[0]: k for k, el in enumerate(model.elements)}
+    dl = [(eidx[eid], face, mag) for eid, face, mag in model.dloads]
+    return linear_setup(nn, xyz, conn, e, nu, bcs, cl, dl)
+
+
+def solve_deck(text, solver="auto", tol=1e-8):
+    """solve_native for the decks on this path: parse -> setup -> solve."""
+    m = parse_inp(text)
+    return m, linear_solve(deck_setup(m), solver, tol)

@@ -1,0 +1,93 @@
+"""Write the committed golden fixtures for the hot-path parity tests.
+
+Inputs are the reference's own decks on this path, copied verbatim as data:
+  cube_tension.inp   C3D8 patch test deck, solver/src/lib.rs:235-274
+  c3d20_patch.inp    C3D20 patch test deck, solver/src/lib.rs:700-749
+  le10_c3d20.inp, le10_c3d20_fine.inp   examples/nafems/ (NAFEMS LE10)
+  patch_c3d8.inp     examples/patch_c3d8.inp
+plus the synthetic cantilever of the small benchmark config (20x4x4 and
+20x9x9 C3D8). Expected outputs come from the CPU oracle (oracle/oracle.py),
+whose pinning is documented there; the reference's own known answers for
+these decks (patch u_x = +-0.01, sigma_xx = +-210, LE10 sigma_yy(D) =
+-5.51 / -5.50) are asserted in tests/test_oracle_decks.py.
+
+Run from the repo root: python tests/golden/make_golden.py
+(reads /root/reference; the outputs are committed, the GPU box never reads
+the reference).
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "axia-fem_amd"))
+
+from oracle import oracle as O  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def lib_deck(fn):
+    src = open(os.path.join(REF, "solver/src/lib.rs")).read()
+    m = re.search(r"fn " + fn + r"\(\) -> String \{\s*r#\"(.*?)\"#", src, re.S)
+    return m.group(1)
+
+
+def save_deck(name, text):
+    with open(os.path.join(HERE, name), "w") as f:
+        f.write(text)
+
+
+def golden_for_setup(tag, setup, solver):
+    res = O.linear_solve(setup, solver=solver)
+    csr = res.csr
+    out = {
+        "n_free": setup.n_free, "nnz": csr.nnz,
+        "sha_indptr": sha(csr.indptr.astype(np.int64)),
+        "sha_indices": sha(csr.indices.astype(np.int64)),
+        "sha_data": sha(csr.data.astype(np.float64)),
+        "rhs": res.rhs, "x": res.x, "u_full": res.u_full, "rf_full": res.rf_full,
+        "stress": res.stress, "von_mises": res.von_mises, "elem_nodal": res.elem_nodal_stress,
+        "ind_of": setup.ind_of, "f_full": setup.f_full,
+    }
+    if csr.nnz <= 100000:  # small matrices are stored whole, larger ones as hashes
+        out.update(indptr=csr.indptr, indices=csr.indices, data=csr.data)
+    if setup.xyz.shape[0] > 2000:
+        out.pop("elem_nodal")
+    np.savez_compressed(os.path.join(HERE, f"{tag}.npz"), **out)
+    print(f"{tag}: n={setup.n_free} nnz={csr.nnz} solver={res.solver} "
+          f"pattern={sha(csr.indices)[:12]} data={sha(csr.data)[:12]}")
+
+
+def main():
+    save_deck("cube_tension.inp", lib_deck("cube_tension"))
+    save_deck("c3d20_patch.inp", lib_deck("c3d20_patch"))
+    for f in ("le10_c3d20.inp", "le10_c3d20_fine.inp"):
+        save_deck(f, open(os.path.join(REF, "examples/nafems", f)).read())
+    save_deck("patch_c3d8.inp", open(os.path.join(REF, "examples/patch_c3d8.inp")).read())
+    for deck in ("cube_tension", "c3d20_patch", "patch_c3d8", "le10_c3d20", "le10_c3d20_fine"):
+        m = O.parse_inp(open(os.path.join(HERE, deck + ".inp")).read())
+        golden_for_setup(deck, O.deck_setup(m), "auto")
+    from axia_gpu import meshgen as MG
+    for shape in ((20, 4, 4), (20, 9, 9)):
+        p = MG.cantilever(*shape)
+        bcs = [(int(d // 3), int(d % 3), 0.0) for d in np.nonzero(p.ind_of < 0)[0]]
+        tip = np.nonzero(p.f_full[0])[0]
+        cl = [(int(d // 3), int(d % 3), float(p.f_full[0][d])) for d in tip]
+        s = O.linear_setup(8, p.xyz, p.conn, p.e, p.nu, bcs, cl, [])
+        assert np.array_equal(s.ind_of, p.ind_of)
+        golden_for_setup("cantilever_%dx%dx%d" % shape, s, "lapack")
+
+
+if __name__ == "__main__":
+    main()

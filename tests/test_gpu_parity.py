@@ -1,0 +1,246 @@
+"""GPU parity of the hot path against the oracle and the committed goldens.
+
+Bars: CSR pattern, scatter map and FP64 values bit-exact; Ke bit-exact;
+displacements within 1e-8 relative of the direct FP64 solve; stresses from the
+same displacements bit-exact (von Mises within 1e-14), and within 1e-8 from
+the GPU displacements. Every test calls through the C ABI (axia_gpu).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import axia_gpu as G
+from axia_gpu import meshgen as MG
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+DECKS = ["cube_tension", "c3d20_patch", "patch_c3d8", "le10_c3d20", "le10_c3d20_fine"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def deck(name):
+    m = O.parse_inp(open(os.path.join(GOLD, name + ".inp")).read())
+    return m, O.deck_setup(m)
+
+
+def mesh_of(ctx, s):
+    return G.Mesh(s.nn, s.xyz, s.conn, s.e, s.nu, s.ind_of, ctx=ctx)
+
+
+def sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_assembly_bitexact_decks(ctx, name):
+    say("case: assembly", name)
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    m, s = deck(name)
+    mesh = mesh_of(ctx, s)
+    mesh.assemble("fp64")
+    indptr, indices, data = mesh.csr()
+    assert int(g["n_free"]) == len(indptr) - 1
+    assert sha(indptr.astype(np.int64)) == str(g["sha_indptr"])
+    assert sha(indices.astype(np.int64)) == str(g["sha_indices"])
+    assert sha(data.astype(np.float64)) == str(g["sha_data"]), "FP64 K_ff values not bit-exact"
+    # scatter map: oracle Ke summed through the exported map gives the same bits
+    sp, cb = mesh.scatter_map()
+    kes = np.stack([O.hex_stiffness(s.xyz[s.conn[e]], s.e[e], s.nu[e])[0]
+                    for e in range(len(s.conn))]).ravel()
+    vals = np.array([sum(kes[cb[sp[k]:sp[k + 1]]].tolist(), 0.0) for k in range(len(indices))])
+    assert np.array_equal(vals, data)
+
+
+@pytest.mark.parametrize("nn,seed", [(8, 1), (20, 2)])
+def test_ke_bitexact_jittered(ctx, nn, seed):
+    say("case: Ke", nn, seed)
+    if nn == 8:
+        xyz, conn = MG.brick_hex8(3, 2, 2)
+    else:
+        xyz, conn = MG.brick_hex20(2, 2, 1)
+    rng = np.random.default_rng(seed)
+    xyz = xyz + rng.uniform(-0.1, 0.1, xyz.shape)
+    ne = len(conn)
+    e = rng.uniform(1e3, 3e5, ne)
+    nu = rng.uniform(0.0, 0.45, ne)
+    ind = MG.free_map(len(xyz), [0, 1, 2])
+    mesh = G.Mesh(nn, xyz, conn, e, nu, ind, ctx=ctx)
+    mesh.assemble("fp64")
+    ke = mesh.ke()
+    for el in range(ne):
+        ref, _ = O.hex_stiffness(xyz[conn[el]], e[el], nu[el])
+        assert np.array_equal(ke[el], ref), f"element {el}"
+
+
+@pytest.mark.parametrize("name", DECKS + ["cantilever_20x4x4", "cantilever_20x9x9"])
+def test_solve_and_stress_vs_oracle(ctx, name):
+    say("case: solve", name)
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    if name.startswith("cantilever"):
+        shape = tuple(int(v) for v in name.split("_")[1].split("x"))
+        p = MG.cantilever(*shape)
+        mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+        nn, nnode = 8, p.nnode
+    else:
+        m, s = deck(name)
+        mesh = mesh_of(ctx, s)
+        nn, nnode = s.nn, len(s.xyz)
+    mesh.assemble("fp64")
+    rhs = g["rhs"]
+    x, info = mesh.solve(rhs, tol=1e-12)
+    err = rel(x, g["x"])
+    say(f"  n={len(rhs)} iters={info['iters']} rel.err={err:.2e} {info['name']}")
+    assert err < 1e-8
+    assert info["rel_residual"] < 1e-11
+    # stresses from the oracle's displacements: same FP64 bits
+    st, vm, en = mesh.nodal_stress(g["u_full"])
+    assert np.array_equal(st, g["stress"]), f"max diff {np.abs(st - g['stress']).max()}"
+    assert np.allclose(vm, g["von_mises"], rtol=1e-14, atol=0)
+    if "elem_nodal" in g:
+        assert np.array_equal(en, g["elem_nodal"])
+    # stresses from the GPU displacements
+    u = np.zeros(3 * nnode)
+    free = g["ind_of"] >= 0
+    u[free] = x[g["ind_of"][free]]
+    st2, _, _ = mesh.nodal_stress(u)
+    assert rel(st2, g["stress"]) < 1e-8
+
+
+def test_patch_tests_through_gpu(ctx):
+    """lib.rs:429-462 and 751-774: the reference's acceptance values."""
+    for name, ux_ref, sxx_ref, tol_u, tol_s in (("cube_tension", 0.01, 210.0, 1e-6, 1e-3),
+                                                ("c3d20_patch", -0.01, -210.0, 1e-5, 1.0)):
+        say("case: patch", name)
+        m, s = deck(name)
+        mesh = mesh_of(ctx, s)
+        mesh.assemble("fp64")
+        oi, oj, ov, rhs = O.reduce(s.ind_of, s.u0, s.n_free, *s.trips, s.f_full)
+        x, _ = mesh.solve(rhs, tol=1e-12)
+        u = np.zeros(s.ndof)
+        u[s.ind_of >= 0] = x[s.ind_of[s.ind_of >= 0]]
+        ids = [m.node_index(i) for i in (2, 3, 6, 7)]
+        ux = np.mean([u[3 * i] for i in ids])
+        assert abs(ux - ux_ref) < tol_u
+        st, _, _ = mesh.nodal_stress(u)
+        assert abs(st[:, 0].mean() - sxx_ref) < tol_s
+
+
+@pytest.mark.parametrize("name,target", [("le10_c3d20", -5.51), ("le10_c3d20_fine", -5.50)])
+def test_nafems_le10_through_gpu(ctx, name, target):
+    """nafems.rs:50-90 and examples/nafems/README.md:12-13."""
+    say("case: LE10", name)
+    m, s = deck(name)
+    mesh = mesh_of(ctx, s)
+    mesh.assemble("fp64")
+    oi, oj, ov, rhs = O.reduce(s.ind_of, s.u0, s.n_free, *s.trips, s.f_full)
+    x, _ = mesh.solve(rhs, tol=1e-12)
+    u = np.zeros(s.ndof)
+    u[s.ind_of >= 0] = x[s.ind_of[s.ind_of >= 0]]
+    st, _, _ = mesh.nodal_stress(u)
+    syy = st[m.node_index(m.nsets["D"][0]), 1]
+    say(f"  sigma_yy(D) = {syy:.4f}")
+    assert round(syy, 2) == target
+    assert abs((syy + 5.38) / 5.38) < (0.05 if name == "le10_c3d20" else 0.03)
+
+
+def test_solve_kff_dropin(ctx):
+    """linalg::solve_kff on the reduced triplets of a deck (analysis.rs:1019-1020)."""
+    say("case: solve_kff")
+    m, s = deck("le10_c3d20")
+    oi, oj, ov, rhs = O.reduce(s.ind_of, s.u0, s.n_free, *s.trips, s.f_full)
+    g = np.load(os.path.join(GOLD, "le10_c3d20.npz"))
+    r = G.solve_kff(s.n_free, (oi, oj, ov), rhs, ctx=ctx, tol=1e-12)
+    assert isinstance(r, G.SparseResult)
+    assert r.name.startswith("GPU PCG")
+    assert rel(r.x, g["x"]) < 1e-8
+    assert r.iters > 0 and r.residual < 1e-10 * np.linalg.norm(rhs)
+
+
+def test_csr_from_triplets_device_bitexact(ctx):
+    say("case: csr_from_triplets")
+    rng = np.random.default_rng(3)
+    n, t = 300, 20000
+    ti = rng.integers(-2, n + 2, t)
+    tj = rng.integers(-2, n + 2, t)
+    tv = rng.standard_normal(t)
+    tv[rng.random(t) < 0.1] = 0.0
+    tv[rng.random(t) < 0.02] = -0.0
+    ref = O.csr_from_triplets(n, ti, tj, tv)
+    indptr, indices, data = G.csr_from_triplets(n, ti, tj, tv, ctx=ctx)
+    assert np.array_equal(indptr, ref.indptr)
+    assert np.array_equal(indices, ref.indices)
+    assert np.array_equal(data, ref.data)
+
+
+def test_block_jacobi_fp64(ctx):
+    say("case: block-Jacobi")
+    g = np.load(os.path.join(GOLD, "le10_c3d20.npz"))
+    m, s = deck("le10_c3d20")
+    mesh = mesh_of(ctx, s)
+    mesh.assemble("fp64")
+    x, info = mesh.solve(g["rhs"], precond="block3", tol=1e-12)
+    assert rel(x, g["x"]) < 1e-8
+    assert "block" in info["name"]
+
+
+def test_error_paths(ctx):
+    say("case: errors")
+    xyz, conn = MG.brick_hex8(2, 1, 1)
+    ind = MG.free_map(len(xyz), [0, 3])
+    bad = conn.copy()
+    bad[1] = bad[1][[4, 5, 6, 7, 0, 1, 2, 3]]  # mirrored element: det < 0
+    mesh = G.Mesh(8, xyz, bad, 2e5, 0.3, ind, ctx=ctx)
+    with pytest.raises(G.AxgError) as ei:
+        mesh.assemble()
+    assert ei.value.code == 3 and "negative Jakobideterminante" in str(ei.value)
+    mesh = G.Mesh(8, xyz, conn, 2e5, 0.5, ind, ctx=ctx)
+    with pytest.raises(G.AxgError) as ei:
+        mesh.assemble()
+    assert ei.value.code == 4 and "Querkontraktion" in str(ei.value)
+    # not converged: too few iterations (linalg.rs:253-257)
+    g = np.load(os.path.join(GOLD, "cantilever_20x4x4.npz"))
+    p = MG.cantilever(20, 4, 4)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble()
+    with pytest.raises(G.AxgError) as ei:
+        mesh.solve(g["rhs"], tol=1e-10, max_iter=5)
+    assert ei.value.code == 6 and "CG konvergierte nicht" in str(ei.value)
+    # zero pivot (linalg.rs:231-233): p.Ap = 0 on [[0,1],[1,0]]
+    with pytest.raises(G.AxgError) as ei:
+        G.solve_kff(2, ([0, 1], [1, 0], [1.0, 1.0]), np.array([1.0, 0.0]), ctx=ctx)
+    assert ei.value.code == 5 and "Null-Pivot" in str(ei.value)
+    # zero right-hand side: x = 0, zero iterations (linalg.rs:206-214)
+    r = G.solve_kff(2, ([0, 1], [0, 1], [2.0, 3.0]), np.zeros(2), ctx=ctx)
+    assert r.iters == 0 and np.all(r.x == 0.0)
+
+
+def test_brick_properties_40(ctx):
+    """Size-independent checks on a 40^3 lognormal brick: FP64 residual on the host,
+    energy identity u^T K u = f^T u, symmetric pattern."""
+    say("case: brick 40^3")
+    import scipy.sparse as sp
+    p = MG.brick_lognormal(40, seed=1)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    ms = mesh.assemble("fp64")
+    indptr, indices, data = mesh.csr()
+    n = len(indptr) - 1
+    K = sp.csr_matrix((data, indices, indptr), shape=(n, n))
+    assert (abs(K - K.T) > 1e-9 * abs(K).max()).nnz == 0
+    b = p.rhs()
+    x, info = mesh.solve(b, tol=1e-10)
+    r = b - K @ x
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 2e-10
+    assert abs(x @ (K @ x) - b @ x) / abs(b @ x) < 1e-9
+    say(f"  assembly {ms:.2f} ms, iters {info['iters']}, solve {info['solve_ms']:.1f} ms")
