@@ -181,6 +181,14 @@ class Mesh:
         check(_lib.lib().axg_ke_export(self._h, first, count, dptr(out)))
         return out.reshape(count, m, m)
 
+    def ke32(self, first=0, count=None):
+        count = self.nelem - first if count is None else count
+        m = 3 * self.nn
+        out = np.zeros(count * m * m, np.float32)
+        check(_lib.lib().axg_ke32_export(self._h, first, count,
+                                         out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out.reshape(count, m, m)
+
     def solve(self, rhs, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
               inner_tol=1e-4, max_outer=30, timing=False):
         """Solve K_ff x = rhs; rhs (n_free,) or (nrhs, n_free). Returns (x, info)."""

@@ -509,6 +509,8 @@ int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
     diag_fp64(ctx, A, mesh->dsel.p, nullptr);
     AXG_HIP(hipStreamSynchronize(s));
     mesh->assembled64 = true;
+    mesh->binv_valid = false;
+    mesh->assembled32 = false;
     if (precision == AXG_PREC_MIXED) {
         double ms32 = 0.0;
         assemble_mixed(mesh, &ms32);
@@ -560,6 +562,16 @@ int axg_ke_export(axg_mesh *mesh, int64_t first, int64_t count, double *ke) {
     AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "axg_ke_export: mesh not assembled");
     size_t mm = (size_t)mesh->m * mesh->m;
     AXG_HIP(hipMemcpy(ke, mesh->ke.p + first * mm, 8 * mm * count, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int axg_ke32_export(axg_mesh *mesh, int64_t first, int64_t count, float *ke) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && ke && first >= 0 && count >= 0 && first + count <= mesh->nelem,
+                AXG_E_INVALID, "axg_ke32_export: invalid range");
+    AXG_REQUIRE(mesh->assembled32, AXG_E_INVALID, "axg_ke32_export: mixed operator not assembled");
+    size_t mm = (size_t)mesh->m * mesh->m;
+    AXG_HIP(hipMemcpy(ke, mesh->ke32.p + first * mm, 4 * mm * count, hipMemcpyDeviceToHost));
     API_END
 }
 

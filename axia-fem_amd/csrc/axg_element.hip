@@ -121,6 +121,7 @@ void upload_shape_tables() {
                 for (int k = 0; k < 3; k++) t.h20_ext[a][idx++] = lx[i] * ly[j] * lz[k];
     }
     AXG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(t)));
+    upload_shape_tables32(&t.h8_dn[0][0][0], &t.h20_dn[0][0][0], t.h20_w);
 }
 
 // ------------------------------------------------------------- geometry

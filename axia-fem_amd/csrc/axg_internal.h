@@ -167,7 +167,8 @@ struct axg_mesh {
     axg::DevBuf<float> binv32;                // 9*nb block-Jacobi inverse (FP32 copy)
     axg::DevBuf<float> dinv32;                // 3*nb scalar Jacobi inverse
     axg::DevBuf<double> binv64;               // 9*nb FP64 block inverse over free DOFs
-    axg::DevBuf<int32_t> diag_slot;           // n_free: CSR slot of (i,i) or -1
+    axg::DevBuf<int32_t> entry_of_slot;       // topological slot -> BSR plane position
+    bool binv_valid = false;                  // block inverses match the current values
     // host mirror of the active pattern (kept for export)
     std::vector<int32_t> h_rowptr, h_colind;
 };
@@ -185,7 +186,8 @@ void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
 void launch_ke_fp64(axg_mesh *m, hipStream_t s, int32_t *d_err);
 void launch_gather_fp64(axg_mesh *m, hipStream_t s, bool topo, uint32_t *d_zero_count);
 void launch_ke_fp32(axg_mesh *m, hipStream_t s, int32_t *d_err);
-void launch_gather_bsr32(axg_mesh *m, hipStream_t s);
+void launch_gather_bsr32(axg_mesh *m, hipStream_t s, const int32_t *entry_of_slot);
+void upload_shape_tables32(const double *h8_dn, const double *h20_dn, const double *h20_w);
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
                          double *d_stress, double *d_vm, int32_t *d_err);
 void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double *d_energy,

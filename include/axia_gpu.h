@@ -136,6 +136,10 @@ int axg_scatter_map_export(axg_mesh *mesh, int64_t *slot_ptr, int64_t *contrib);
 /* Copy the FP64 element matrices of elements [first, first+count) (row-major). */
 int axg_ke_export(axg_mesh *mesh, int64_t first, int64_t count, double *ke);
 
+/* Copy the FP32 element matrices of the mixed operator (after
+ * axg_assemble(..., AXG_PREC_MIXED)) of elements [first, first+count). */
+int axg_ke32_export(axg_mesh *mesh, int64_t first, int64_t count, float *ke);
+
 /* Solve K_ff x = rhs for nrhs right-hand sides (column k at rhs + k*n_free). */
 int axg_mesh_solve(axg_mesh *mesh, const double *rhs, int32_t nrhs, const axg_solve_opts *opts,
                    double *x_out, axg_solve_info *info);

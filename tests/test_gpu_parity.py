@@ -244,3 +244,57 @@ def test_brick_properties_40(ctx):
     assert np.linalg.norm(r) / np.linalg.norm(b) < 2e-10
     assert abs(x @ (K @ x) - b @ x) / abs(b @ x) < 1e-9
     say(f"  assembly {ms:.2f} ms, iters {info['iters']}, solve {info['solve_ms']:.1f} ms")
+
+
+# ------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("nn,seed", [(8, 4), (20, 5)])
+def test_mixed_ke32_mfma_vs_fp64(ctx, nn, seed):
+    """FP32 MFMA integration agrees with the FP64 element matrices to FP32 accuracy."""
+    say("case: Ke32", nn, seed)
+    if nn == 8:
+        xyz, conn = MG.brick_hex8(3, 3, 2)
+    else:
+        xyz, conn = MG.brick_hex20(2, 2, 2)
+    rng = np.random.default_rng(seed)
+    xyz = xyz + rng.uniform(-0.1, 0.1, xyz.shape)
+    ne = len(conn)
+    e = rng.uniform(1e3, 3e5, ne)
+    mesh = G.Mesh(nn, xyz, conn, e, 0.3, MG.free_map(len(xyz), [0, 1]), ctx=ctx)
+    mesh.assemble("mixed")
+    k64 = mesh.ke()
+    k32 = mesh.ke32().astype(np.float64)
+    for el in range(ne):
+        scale = np.abs(k64[el]).max()
+        assert np.abs(k32[el] - k64[el]).max() < 2e-6 * scale, f"element {el}"
+
+
+@pytest.mark.parametrize("name,precond", [("le10_c3d20", "block3"), ("le10_c3d20_fine", "jacobi"),
+                                          ("cantilever_20x9x9", "block3")])
+def test_mixed_solve_vs_oracle(ctx, name, precond):
+    say("case: mixed", name, precond)
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    if name.startswith("cantilever"):
+        p = MG.cantilever(20, 9, 9)
+        mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    else:
+        m, s = deck(name)
+        mesh = mesh_of(ctx, s)
+    mesh.assemble("mixed")
+    x, info = mesh.solve(g["rhs"], precision="mixed", precond=precond, tol=1e-12)
+    err = rel(x, g["x"])
+    say(f"  inner={info['iters']} sweeps={info['outer_iters']} rel.err={err:.2e} {info['name']}")
+    assert err < 1e-8
+    assert info["rel_residual"] <= 1e-12
+
+
+def test_mixed_multi_rhs_matches_fp64(ctx):
+    say("case: mixed multi-rhs")
+    p = MG.brick_lognormal(12, seed=3, nloads=4)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble("mixed")
+    B = p.rhs_all()
+    X64, _ = mesh.solve(B, precision="fp64", tol=1e-13)
+    X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-12)
+    assert info["nrhs"] == 4
+    for k in range(4):
+        assert rel(X[k], X64[k]) < 1e-9
