@@ -158,6 +158,12 @@ class Mesh:
         check(_lib.lib().axg_assemble(self._h, prec, C.byref(ms)))
         return ms.value
 
+    def assembly_times(self):
+        """Device ms of the last assembly: ke64, gather64, ke32, gather32."""
+        v = [C.c_double() for _ in range(4)]
+        check(_lib.lib().axg_assembly_times(self._h, *[C.byref(x) for x in v]))
+        return {k: x.value for k, x in zip(("ke64", "gather64", "ke32", "gather32"), v)}
+
     def csr(self):
         inf = self.info()
         n, nnz = inf["n_free"], inf["nnz"]

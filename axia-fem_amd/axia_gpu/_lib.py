@@ -91,6 +91,7 @@ SIGNATURES = {
     "axg_mesh_info": (C.c_int, [_vp, _lp, _lp, _lp, _dp]),
     "axg_mesh_set_modulus": (C.c_int, [_vp, _dp]),
     "axg_assemble": (C.c_int, [_vp, _i32, _dp]),
+    "axg_assembly_times": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "axg_csr_export": (C.c_int, [_vp, _lp, _lp, _dp]),
     "axg_scatter_map_export": (C.c_int, [_vp, _lp, _lp]),
     "axg_ke_export": (C.c_int, [_vp, _i64, _i64, _dp]),

@@ -155,6 +155,7 @@ int axg_create(int device, axg_ctx **out) {
         AXG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         AXG_HIP(hipEventCreate(&c->ev0));
         AXG_HIP(hipEventCreate(&c->ev1));
+        AXG_HIP(hipEventCreate(&c->evA));
         hipDeviceProp_t prop;
         AXG_HIP(hipGetDeviceProperties(&prop, device));
         c->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -177,6 +178,7 @@ int axg_destroy(axg_ctx *ctx) {
     for (auto e : ctx->timing_events) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->evA) (void)hipEventDestroy(ctx->evA);
     ctx->ws = Workspace();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -412,6 +414,17 @@ int axg_mesh_info(axg_mesh *mesh, int64_t *n_free, int64_t *nnz, int64_t *ncontr
     API_END
 }
 
+int axg_assembly_times(axg_mesh *mesh, double *ke64_ms, double *gather64_ms, double *ke32_ms,
+                       double *gather32_ms) {
+    API_BEGIN
+    AXG_REQUIRE(mesh, AXG_E_INVALID, "axg_assembly_times: NULL mesh");
+    if (ke64_ms) *ke64_ms = mesh->t_ke64;
+    if (gather64_ms) *gather64_ms = mesh->t_gather64;
+    if (ke32_ms) *ke32_ms = mesh->t_ke32;
+    if (gather32_ms) *gather32_ms = mesh->t_gather32;
+    API_END
+}
+
 int axg_mesh_set_modulus(axg_mesh *mesh, const double *e) {
     API_BEGIN
     AXG_REQUIRE(mesh && e, AXG_E_INVALID, "axg_mesh_set_modulus: NULL argument");
@@ -482,6 +495,7 @@ int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
     AXG_HIP(hipMemsetAsync(w.cnt.p, 0, 4 * sizeof(uint32_t), s));
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     launch_ke_fp64(mesh, s, w.err.p);
+    AXG_HIP(hipEventRecord(ctx->evA, s));
     launch_gather_fp64(mesh, s, true, w.cnt.p);
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     int32_t err[2];
@@ -489,8 +503,11 @@ int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
     AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipMemcpyAsync(&zero_slots, w.cnt.p, 4, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
-    float ms = 0.f;
+    float ms = 0.f, ms_a = 0.f;
     AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    AXG_HIP(hipEventElapsedTime(&ms_a, ctx->ev0, ctx->evA));
+    mesh->t_ke64 = ms_a;
+    mesh->t_gather64 = ms - ms_a;
     if (err[0] != 0) {
         std::vector<double> e(mesh->nelem), nu(mesh->nelem);
         AXG_HIP(hipMemcpy(e.data(), mesh->e.p, 8 * mesh->nelem, hipMemcpyDeviceToHost));

@@ -120,7 +120,7 @@ struct Workspace {
 struct axg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr;
     std::vector<hipEvent_t> timing_events;
     axg::PinnedBuf pinned;
     axg::Workspace ws;
@@ -135,6 +135,7 @@ struct axg_mesh {
     int64_t nnz = 0, ncontrib = 0;          // of the current (possibly compacted) pattern
     int64_t nnz_topo = 0, ncontrib_topo = 0; // of the topological pattern
     double symbolic_s = 0.0;
+    double t_ke64 = 0, t_gather64 = 0, t_ke32 = 0, t_gather32 = 0;  // ms, last assembly
     // mesh inputs (device)
     axg::DevBuf<double> xyz, e, nu;
     axg::DevBuf<int32_t> conn, ind_of;

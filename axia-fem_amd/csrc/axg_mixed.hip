@@ -191,6 +191,7 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 9 * m->nnzb, s));
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     launch_ke_fp32(m, s, w.err.p);
+    AXG_HIP(hipEventRecord(ctx->evA, s));
     launch_gather_bsr32(m, s, m->entry_of_slot.p);
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     int32_t err[2];
@@ -198,8 +199,11 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     AXG_HIP(hipStreamSynchronize(s));
     if (err[0]) throw Error{err[0], "FP32 integration: negative Jakobideterminante (element index " +
                                         std::to_string(err[1]) + ")"};
-    float ms = 0.f;
+    float ms = 0.f, ms_a = 0.f;
     AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    AXG_HIP(hipEventElapsedTime(&ms_a, ctx->ev0, ctx->evA));
+    m->t_ke32 = ms_a;
+    m->t_gather32 = ms - ms_a;
     if (elapsed_ms) *elapsed_ms = ms;
     m->assembled32 = true;
 }

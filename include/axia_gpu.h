@@ -128,6 +128,10 @@ int axg_mesh_set_modulus(axg_mesh *mesh, const double *e);
  * FP32 BSR-3 inner operator from FP32 (MFMA) integration. elapsed_ms (may be
  * NULL) receives the device time of the numeric assembly. */
 int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms);
+/* Device times (ms) of the last assembly: FP64 integration, FP64 gather,
+ * FP32 (MFMA) integration, FP32 BSR-3 gather. */
+int axg_assembly_times(axg_mesh *mesh, double *ke64_ms, double *gather64_ms, double *ke32_ms,
+                       double *gather32_ms);
 /* Copy the FP64 K_ff CSR to the host (indptr n+1, indices nnz, data nnz). */
 int axg_csr_export(axg_mesh *mesh, int64_t *indptr, int64_t *indices, double *data);
 /* Copy the scatter map: slot_ptr (nnz+1), contrib (ncontrib) where each

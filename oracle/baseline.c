@@ -167,3 +167,14 @@ int or_pcg_omp(int64_t n, const int64_t *indptr, const int64_t *indices, const d
     if (max_iter_cap <= 0 && residual > tol * bnorm * 10.0) return OR_E_NOT_CONVERGED;
     return OR_OK;
 }
+
+/* Thread count for the OpenMP variants (<= 0 restores the default). */
+void or_set_threads(int n) {
+#ifdef _OPENMP
+    static int dflt = 0;
+    if (!dflt) dflt = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : dflt);
+#else
+    (void)n;
+#endif
+}
