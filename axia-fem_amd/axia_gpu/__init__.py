@@ -218,6 +218,12 @@ class Mesh:
                                                C.c_void_p(x_ptr), C.byref(info)))
         return _info_dict(info)
 
+    def bench_operator(self, which, nrhs=1, reps=20):
+        """Average ms of one operator launch: 0 FP64 solver SpMV, 1 FP64 CSR, 2 FP32 SpMM."""
+        ms = C.c_double()
+        check(_lib.lib().axg_bench_operator(self._h, which, nrhs, reps, C.byref(ms)))
+        return ms.value
+
     def nodal_stress(self, u_full):
         u = f64(u_full).ravel()
         stress = np.zeros((self.nnode, 6))
@@ -234,6 +240,14 @@ class Mesh:
         dc = np.zeros(self.nelem)
         check(_lib.lib().axg_compliance_adjoint(self._h, dptr(u), dptr(f), C.byref(c), dptr(dc)))
         return c.value, dc
+
+
+def bench_stream(n=1 << 27, reps=10, ctx=None):
+    """STREAM-style copy bandwidth (GB/s, read + write) of n doubles."""
+    ctx = ctx or default_context()
+    gbs = C.c_double()
+    check(_lib.lib().axg_bench_stream(ctx.handle, n, reps, C.byref(gbs)))
+    return gbs.value
 
 
 def csr_from_triplets(n, ti, tj, tv, ctx=None):

@@ -73,6 +73,7 @@ CsrView active_csr(axg_mesh *m) {
     A.rowptr = m->compacted ? m->rowptr_c.p : m->rowptr_t.p;
     A.colind = m->compacted ? m->colind_c.p : m->colind_t.p;
     A.val = m->val.p;
+    A.shape = &m->spmv_shape;
     return A;
 }
 
@@ -517,11 +518,14 @@ int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
     if (zero_slots > 0) {
         compact_pattern(mesh);
         launch_gather_fp64(mesh, s, false, nullptr);
+        mesh->sell_valid = false;
     } else if (mesh->compacted) {
         mesh->compacted = false;
         mesh->nnz = mesh->nnz_topo;
         mesh->ncontrib = mesh->ncontrib_topo;
+        mesh->sell_valid = false;
     }
+    mesh->sell_fresh = false;
     CsrView A = active_csr(mesh);
     diag_fp64(ctx, A, mesh->dsel.p, nullptr);
     AXG_HIP(hipStreamSynchronize(s));
@@ -597,7 +601,7 @@ static void mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
                               const axg_solve_opts &o, double *d_x, axg_solve_info *info) {
     AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
     axg_ctx *ctx = mesh->ctx;
-    CsrView A = active_csr(mesh);
+    CsrView A = solver_csr(mesh);
     const int64_t n = mesh->n_free;
     const int64_t max_iter = default_max_iter(o, n);
     if (o.precision == AXG_PREC_MIXED) {

@@ -170,6 +170,19 @@ struct axg_mesh {
     axg::DevBuf<double> binv64;               // 9*nb FP64 block inverse over free DOFs
     axg::DevBuf<int32_t> entry_of_slot;       // topological slot -> BSR plane position
     bool binv_valid = false;                  // block inverses match the current values
+    int spmv_shape = -1;                      // tuned FP64 SpMV shape
+    // sliced-ELL copy of the active FP64 K_ff (solver operator)
+    bool sell_valid = false;  // layout matches the active pattern
+    bool sell_fresh = false;  // values match the last assembly
+    int64_t sell_nslots = 0;
+    axg::DevBuf<int64_t> sell_sptr;
+    axg::DevBuf<int32_t> sell_width, sell_col, sell_src;
+    axg::DevBuf<double> sell_val;
+    // block-sliced-ELL layout of the FP32 operator (64 block rows per slice)
+    int64_t bs_nslots = 0;
+    axg::DevBuf<int64_t> bs_sptr;
+    axg::DevBuf<int32_t> bs_width, bs_col;
+    int spmm_shape[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};  // tuned BSR SpMM shape per k
     // host mirror of the active pattern (kept for export)
     std::vector<int32_t> h_rowptr, h_colind;
 };

@@ -6,9 +6,10 @@
 //
 // Vectors of the inner solve live in block space: 3 entries per node that
 // keeps a free DOF (constrained entries stay zero), k right-hand sides
-// interleaved (entry (3 b + d) * K + j). BSR values are stored as nine planes
-// (plane 3 d + e holds entry (d, e) of every block) so that consecutive lanes
-// read consecutive floats.
+// interleaved (entry (3 b + d) * K + j). BSR values are stored as three
+// planes: a float4 plane with entries (0,0) (0,1) (0,2) (1,0) of every block,
+// a float4 plane with (1,1) (1,2) (2,0) (2,1), and a float plane with (2,2),
+// so consecutive lanes read consecutive 16-byte words.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -16,6 +17,7 @@
 
 #include "axg_internal.h"
 #include "axg_mixed.h"
+#include "axg_reduce.h"
 
 namespace axg {
 
@@ -155,25 +157,44 @@ static void build_bsr(axg_mesh *m) {
         brp[I + 1] = (int32_t)bci.size();
     }
     const int64_t nnzb = (int64_t)bci.size();
+    // block-sliced-ELL layout: 64 block rows per slice, block k of row I at
+    // slot sptr[I / 64] + 64 (k - brp[I]) + I % 64
+    std::vector<int64_t> blen(nb);
+    for (int64_t I = 0; I < nb; I++) blen[I] = brp[I + 1] - brp[I];
+    SellHost L;
+    sell_layout(blen, L);
+    const int64_t nsl = L.nslots;
+    AXG_REQUIRE(9 * nsl < INT32_MAX, AXG_E_INVALID, "FP32 operator exceeds 2^31 values");
+    std::vector<int32_t> bcol(nsl, 0);
+    for (int64_t I = 0; I < nb; I++) {
+        int64_t base = L.sptr[I >> 6] + (I & 63);
+        for (int32_t k = brp[I]; k < brp[I + 1]; k++) bcol[base + 64 * (int64_t)(k - brp[I])] = bci[k];
+        for (int64_t k = blen[I]; k < L.width[I >> 6]; k++) bcol[base + 64 * k] = (int32_t)I;
+    }
     std::vector<int32_t> eos(nnz);
     for (int64_t i = 0; i < n; i++) {
         int32_t I = bof[i] / 3, d = bof[i] % 3;
         int32_t k0 = brp[I], k1 = brp[I + 1];
+        int64_t base = L.sptr[I >> 6] + (I & 63);
         for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
             int32_t J = bof[ci[k]] / 3, e = bof[ci[k]] % 3;
-            int32_t pos = (int32_t)(std::lower_bound(bci.begin() + k0, bci.begin() + k1, J) -
-                                    bci.begin());
-            eos[k] = (int32_t)((3 * d + e) * nnzb + pos);
+            int32_t kb = (int32_t)(std::lower_bound(bci.begin() + k0, bci.begin() + k1, J) -
+                                   bci.begin());
+            int64_t pos = base + 64 * (int64_t)(kb - k0);
+            int q = 3 * d + e;  // planes: entries 0-3 and 4-7 as float4, entry 8 alone
+            eos[k] = (int32_t)(q < 4 ? 4 * pos + q : q < 8 ? 4 * nsl + 4 * pos + (q - 4) : 8 * nsl + pos);
         }
     }
-    AXG_REQUIRE(9 * nnzb < INT32_MAX, AXG_E_INVALID, "BSR-3 operator exceeds 2^31 entries");
     m->nnzb = nnzb;
-    m->b_rowptr.alloc(nb + 1);
-    m->b_colind.alloc(std::max<int64_t>(nnzb, 1));
+    m->bs_nslots = nsl;
+    m->bs_sptr.alloc(L.sptr.size());
+    m->bs_width.alloc(L.width.size());
+    m->bs_col.alloc(std::max<int64_t>(nsl, 1));
     m->entry_of_slot.alloc(std::max<int64_t>(nnz, 1));
-    AXG_HIP(hipMemcpyAsync(m->b_rowptr.p, brp.data(), 4 * (nb + 1), hipMemcpyHostToDevice, s));
-    if (nnzb)
-        AXG_HIP(hipMemcpyAsync(m->b_colind.p, bci.data(), 4 * nnzb, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(m->bs_sptr.p, L.sptr.data(), 8 * L.sptr.size(), hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(m->bs_width.p, L.width.data(), 4 * L.width.size(), hipMemcpyHostToDevice,
+                           s));
+    if (nsl) AXG_HIP(hipMemcpyAsync(m->bs_col.p, bcol.data(), 4 * nsl, hipMemcpyHostToDevice, s));
     if (nnz)
         AXG_HIP(hipMemcpyAsync(m->entry_of_slot.p, eos.data(), 4 * nnz, hipMemcpyHostToDevice, s));
     AXG_HIP(hipStreamSynchronize(s));
@@ -185,10 +206,10 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     if (m->nnzb == 0) build_bsr(m);
     ensure_block_inverse64(m);
     m->ke32.ensure((size_t)m->nelem * m->m * m->m);
-    m->bval.ensure(std::max<int64_t>(9 * m->nnzb, 1));
+    m->bval.ensure(std::max<int64_t>(9 * m->bs_nslots, 1));
     Workspace &w = ctx->ws;
     AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
-    AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 9 * m->nnzb, s));
+    AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 9 * m->bs_nslots, s));
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     launch_ke_fp32(m, s, w.err.p);
     AXG_HIP(hipEventRecord(ctx->evA, s));
@@ -208,57 +229,33 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     m->assembled32 = true;
 }
 
-// ---------------------------------------------------------- reductions
-__device__ inline double wsum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
+// ------------------------------------------------------- small helpers
+typedef float f4v __attribute__((ext_vector_type(4)));
 
-// K workgroup sums of v[j]; results valid in thread 0.
 template <int K>
-__device__ inline void block_sums(double (&v)[K], double *sh /* K * MBS/64 */) {
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+__device__ inline void load_k(const float *__restrict__ p, float (&v)[K]) {
+    if constexpr (K % 4 == 0) {
 #pragma unroll
-    for (int j = 0; j < K; j++) v[j] = wsum(v[j]);
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) sh[j * (MBS / 64) + wid] = v[j];
-    __syncthreads();
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            double t = 0.0;
-            for (int w = 0; w < MBS / 64; w++) t += sh[j * (MBS / 64) + w];
-            v[j] = t;
+        for (int j = 0; j < K; j += 4) {
+            float4 t = *reinterpret_cast<const float4 *>(p + j);
+            v[j] = t.x, v[j + 1] = t.y, v[j + 2] = t.z, v[j + 3] = t.w;
         }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) v[j] = p[j];
+    }
 }
 
-__device__ inline bool last_block_m(uint32_t *ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
-
-// Sum of part[b * stride + comp] over b < nblk in index order (whole group).
-__device__ inline double ordered_sum(const double *part, int nblk, int stride, int comp) {
-    __shared__ double t[MBS];
-    int per = (nblk + MBS - 1) / MBS;
-    int a = threadIdx.x * per, b = min(nblk, a + per);
-    double s = 0.0;
-    for (int i = a; i < b; i++) s += part[(size_t)i * stride + comp];
-    t[threadIdx.x] = s;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < MBS; i++) r += t[i];
-    __syncthreads();
-    return r;
+template <int K>
+__device__ inline void store_k(float *__restrict__ p, const float (&v)[K]) {
+    if constexpr (K % 4 == 0) {
+#pragma unroll
+        for (int j = 0; j < K; j += 4)
+            *reinterpret_cast<float4 *>(p + j) = make_float4(v[j], v[j + 1], v[j + 2], v[j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) p[j] = v[j];
+    }
 }
 
 // Per right-hand-side scalars of the inner solve (device).
@@ -271,81 +268,162 @@ struct InnerScalars {
     uint32_t ticket[4];
 };
 
-// ------------------------------------------------------ BSR-3 SpMM (FP32)
-// G lanes per block row; lane l takes blocks k0 + l, k0 + l + G, ... and
-// accumulates 3 x K partial outputs, reduced over the G lanes.
-template <int K, int G>
-__global__ void __launch_bounds__(MBS) bsr_spmm_dot_kernel(
-    int64_t nb, const int32_t *__restrict__ brp, const int32_t *__restrict__ bci,
-    const float *__restrict__ bval, int64_t nnzb, const float *__restrict__ P,
-    float *__restrict__ AP, double *__restrict__ part, InnerScalars *sc) {
-    if (sc->all_done) return;
-    __shared__ double sh[K * MBS / 64];
-    const int lane = threadIdx.x % G;
-    const int64_t group = ((int64_t)blockIdx.x * MBS + threadIdx.x) / G;
-    const int64_t ngroups = (int64_t)gridDim.x * MBS / G;
+// ---------------------------------------- block-sliced-ELL SpMM (FP32)
+// One thread per block row (the 64 block rows of a slice are the lanes of a
+// wavefront): block j of every row is one coalesced load of two float4 planes
+// and one float plane; x of the column block (3 K floats) is gathered through
+// the caches. U blocks per lane are loaded before their x gathers.
+template <int K, int U, bool DOT>
+__global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
+    int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+    const int32_t *__restrict__ col, const float *__restrict__ bval, int64_t nsl,
+    const float *__restrict__ P, float *__restrict__ AP, double *__restrict__ part,
+    InnerScalars *sc) {
+    if (DOT && sc->all_done) return;
+    const f4v *__restrict__ pa = reinterpret_cast<const f4v *>(bval);
+    const f4v *__restrict__ pb = reinterpret_cast<const f4v *>(bval + 4 * nsl);
+    const float *__restrict__ pc = bval + 8 * nsl;
     double dot[K];
 #pragma unroll
     for (int j = 0; j < K; j++) dot[j] = 0.0;
-    for (int64_t I = group; I < nb; I += ngroups) {
+    for (int64_t I = (int64_t)blockIdx.x * MBS + threadIdx.x; I < nb;
+         I += (int64_t)gridDim.x * MBS) {
+        const int64_t sl = I >> 6, base = sptr[sl] + (I & 63);
+        const int w = width[sl];
         float acc[3][K];
 #pragma unroll
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int j = 0; j < K; j++) acc[d][j] = 0.f;
-        int32_t k1 = brp[I + 1];
-        for (int32_t k = brp[I] + lane; k < k1; k += G) {
-            int32_t J = bci[k];
-            float v[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) v[q] = bval[(int64_t)q * nnzb + k];
+        auto fma_block = [&](const f4v &va, const f4v &vb, float vc, int32_t J) {
+            float x0[K], x1[K], x2[K];
             const float *x = P + (int64_t)3 * J * K;
+            load_k<K>(x, x0);
+            load_k<K>(x + K, x1);
+            load_k<K>(x + 2 * K, x2);
 #pragma unroll
-            for (int e = 0; e < 3; e++)
+            for (int j = 0; j < K; j++) {
+                acc[0][j] += va.x * x0[j] + va.y * x1[j] + va.z * x2[j];
+                acc[1][j] += va.w * x0[j] + vb.x * x1[j] + vb.y * x2[j];
+                acc[2][j] += vb.z * x0[j] + vb.w * x1[j] + vc * x2[j];
+            }
+        };
+        int jb = 0;
+        for (; jb + U <= w; jb += U) {
+            f4v va[U], vb[U];
+            float vc[U];
+            int32_t J[U];
 #pragma unroll
-                for (int j = 0; j < K; j++) {
-                    float xe = x[e * K + j];
-                    acc[0][j] += v[e] * xe;
-                    acc[1][j] += v[3 + e] * xe;
-                    acc[2][j] += v[6 + e] * xe;
-                }
+            for (int u = 0; u < U; u++) {
+                const int64_t p = base + 64 * (int64_t)(jb + u);
+                J[u] = __builtin_nontemporal_load(&col[p]);
+                va[u] = __builtin_nontemporal_load(&pa[p]);
+                vb[u] = __builtin_nontemporal_load(&pb[p]);
+                vc[u] = __builtin_nontemporal_load(&pc[p]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) fma_block(va[u], vb[u], vc[u], J[u]);
+        }
+        for (; jb < w; jb++) {
+            const int64_t p = base + 64 * (int64_t)jb;
+            fma_block(pa[p], pb[p], pc[p], col[p]);
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++)
+        for (int d = 0; d < 3; d++) {
+            store_k<K>(AP + (3 * I + d) * K, acc[d]);
+            if (DOT) {
+                float pv[K];
+                load_k<K>(P + (3 * I + d) * K, pv);
 #pragma unroll
-            for (int j = 0; j < K; j++)
-#pragma unroll
-                for (int off = G / 2; off > 0; off >>= 1)
-                    acc[d][j] += __shfl_down(acc[d][j], off, G);
-        if (lane == 0) {
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-#pragma unroll
-                for (int j = 0; j < K; j++) {
-                    int64_t idx = (3 * I + d) * K + j;
-                    AP[idx] = acc[d][j];
-                    dot[j] += (double)P[idx] * (double)acc[d][j];
-                }
+                for (int j = 0; j < K; j++) dot[j] += (double)pv[j] * (double)acc[d][j];
+            }
         }
     }
-    block_sums<K>(dot, sh);
+    if (!DOT) return;
+    group_sums<K, MBS>(dot);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * K + j] = dot[j];
-    if (last_block_m(&sc->ticket[0])) {
+}
+
+// ------------------------------------------------------ finalize kernels
+// One workgroup each: sum the partials of the kernel launched just before
+// (fixed order) and update the per-right-hand-side scalars.
+template <int K>
+__global__ void __launch_bounds__(MBS) fin_spmm_kernel(const double *__restrict__ part, int nblk,
+                                                      InnerScalars *sc) {
+    if (sc->all_done) return;
+    double pap[K];
+    sum_partials<K, MBS>(part, nblk, pap);
+    if (threadIdx.x == 0) {
         for (int j = 0; j < K; j++) {
-            double pap = ordered_sum(part, gridDim.x, K, j);
-            if (threadIdx.x == 0 && !sc->done[j]) {
-                if (fabs(pap) < 1e-300 || !(pap > 0.0)) {
-                    sc->done[j] = 1;  // breakdown: the sweep ends here, refinement continues
-                    sc->alpha[j] = 0.0;
-                } else {
-                    sc->alpha[j] = sc->rz[j] / pap;
-                }
+            if (sc->done[j]) continue;
+            if (!(pap[j] > 1e-300)) {
+                sc->done[j] = 1;  // breakdown: this sweep ends, refinement goes on
+                sc->alpha[j] = 0.0;
+            } else {
+                sc->alpha[j] = sc->rz[j] / pap[j];
             }
         }
-        if (threadIdx.x == 0) sc->ticket[0] = 0;
     }
+}
+
+template <int K>
+__global__ void __launch_bounds__(MBS) fin_inner_update_kernel(const double *__restrict__ part,
+                                                              int nblk, InnerScalars *sc) {
+    if (sc->all_done) return;
+    double tot[2 * K];
+    sum_partials<2 * K, MBS>(part, nblk, tot);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int j = 0; j < K; j++) {
+            if (!sc->done[j]) {
+                double res = sqrt(tot[j]);
+                sc->res[j] = res;
+                sc->iters[j] += 1;
+                if (res <= sc->tol_abs[j] || sc->iters[j] >= sc->max_iter) {
+                    sc->done[j] = 1;
+                } else {
+                    sc->beta[j] = tot[K + j] / sc->rz[j];
+                    sc->rz[j] = tot[K + j];
+                }
+            }
+            all &= sc->done[j];
+        }
+        sc->all_done = all;
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(MBS) fin_inner_init_kernel(const double *__restrict__ part,
+                                                            int nblk, InnerScalars *sc,
+                                                            double inner_tol) {
+    double tot[2 * K];
+    sum_partials<2 * K, MBS>(part, nblk, tot);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int j = 0; j < K; j++) {
+            double bn = sqrt(tot[j]);
+            sc->bnorm[j] = bn;
+            sc->res[j] = bn;
+            sc->tol_abs[j] = inner_tol * bn;
+            sc->rz[j] = tot[K + j];
+            sc->iters[j] = 0;
+            sc->done[j] = (bn < 1e-30) ? 1 : 0;
+            all &= sc->done[j];
+        }
+        sc->all_done = all;
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(MBS) fin_norms_kernel(const double *__restrict__ part, int nblk,
+                                                       double *__restrict__ out) {
+    double t[K];
+    sum_partials<K, MBS>(part, nblk, t);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < K; j++) out[j] = sqrt(t[j]);
 }
 
 // x += alpha p, r -= alpha Ap, z = M r (3x3 block or diagonal), ||r||^2, r.z;
@@ -356,33 +434,31 @@ __global__ void __launch_bounds__(MBS) inner_update_kernel(
     const float *__restrict__ AP, float *__restrict__ Z, const float *__restrict__ binv,
     const float *__restrict__ dinv, double *__restrict__ part, InnerScalars *sc) {
     if (sc->all_done) return;
-    __shared__ double sh[K * MBS / 64];
     float alpha[K];
-    bool live[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        live[j] = !sc->done[j];
-        alpha[j] = live[j] ? (float)sc->alpha[j] : 0.f;
-    }
-    double rr[K], rz[K];
+    for (int j = 0; j < K; j++) alpha[j] = sc->done[j] ? 0.f : (float)sc->alpha[j];
+    double acc[2 * K];  // ||r||^2 then r.z
 #pragma unroll
-    for (int j = 0; j < K; j++) rr[j] = rz[j] = 0.0;
+    for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
     for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
          b += (int64_t)gridDim.x * MBS) {
         float r[3][K];
 #pragma unroll
-        for (int d = 0; d < 3; d++)
+        for (int d = 0; d < 3; d++) {
+            const int64_t o = (3 * b + d) * K;
+            float x[K], p[K], ap[K];
+            load_k<K>(R + o, r[d]);
+            load_k<K>(X + o, x);
+            load_k<K>(P + o, p);
+            load_k<K>(AP + o, ap);
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                int64_t idx = (3 * b + d) * K + j;
-                float rv = R[idx];
-                if (live[j]) {
-                    X[idx] += alpha[j] * P[idx];
-                    rv -= alpha[j] * AP[idx];
-                    R[idx] = rv;
-                }
-                r[d][j] = rv;
+                x[j] += alpha[j] * p[j];
+                r[d][j] -= alpha[j] * ap[j];
             }
+            store_k<K>(X + o, x);
+            store_k<K>(R + o, r[d]);
+        }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             float m0 = 0.f, m1 = 0.f, m2 = 0.f, dd = 0.f;
@@ -391,46 +467,20 @@ __global__ void __launch_bounds__(MBS) inner_update_kernel(
             } else {
                 dd = dinv[3 * b + d];
             }
+            float z[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                float z = BLOCK ? (m0 * r[0][j] + m1 * r[1][j] + m2 * r[2][j]) : dd * r[d][j];
-                Z[(3 * b + d) * K + j] = z;
-                rr[j] += (double)r[d][j] * r[d][j];
-                rz[j] += (double)r[d][j] * z;
+                z[j] = BLOCK ? (m0 * r[0][j] + m1 * r[1][j] + m2 * r[2][j]) : dd * r[d][j];
+                acc[j] += (double)r[d][j] * r[d][j];
+                acc[K + j] += (double)r[d][j] * z[j];
             }
+            store_k<K>(Z + (3 * b + d) * K, z);
         }
     }
-    block_sums<K>(rr, sh);
+    group_sums<2 * K, MBS>(acc);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * 2 * K + j] = rr[j];
-    block_sums<K>(rz, sh);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * 2 * K + K + j] = rz[j];
-    if (last_block_m(&sc->ticket[1])) {
-        for (int j = 0; j < K; j++) {
-            double rrs = ordered_sum(part, gridDim.x, 2 * K, j);
-            double rzs = ordered_sum(part, gridDim.x, 2 * K, K + j);
-            if (threadIdx.x == 0 && !sc->done[j]) {
-                double res = sqrt(rrs);
-                sc->res[j] = res;
-                sc->iters[j] += 1;
-                if (res <= sc->tol_abs[j] || sc->iters[j] >= sc->max_iter) {
-                    sc->done[j] = 1;
-                } else {
-                    sc->beta[j] = rzs / sc->rz[j];
-                    sc->rz[j] = rzs;
-                }
-            }
-        }
-        if (threadIdx.x == 0) {
-            int all = 1;
-            for (int j = 0; j < K; j++) all &= sc->done[j];
-            sc->all_done = all;
-            sc->ticket[1] = 0;
-        }
-    }
+        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
 }
 
 template <int K>
@@ -439,17 +489,19 @@ __global__ void __launch_bounds__(MBS) inner_update_p_kernel(int64_t nv, float *
                                                             InnerScalars *sc, int init) {
     if (sc->all_done) return;
     float beta[K];
-    bool live[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        live[j] = !sc->done[j];
-        beta[j] = init ? 0.f : (float)sc->beta[j];
-    }
+    for (int j = 0; j < K; j++) beta[j] = (init || sc->done[j]) ? 0.f : (float)sc->beta[j];
+    bool keep[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) keep[j] = init || !sc->done[j];
     for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * MBS) {
+        float p[K], z[K];
+        load_k<K>(P + i * K, p);
+        load_k<K>(Z + i * K, z);
 #pragma unroll
-        for (int j = 0; j < K; j++)
-            if (live[j]) P[i * K + j] = init ? Z[i * K + j] : Z[i * K + j] + beta[j] * P[i * K + j];
+        for (int j = 0; j < K; j++) p[j] = keep[j] ? z[j] + beta[j] * p[j] : p[j];
+        store_k<K>(P + i * K, p);
     }
 }
 
@@ -460,63 +512,39 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
     int64_t nb, const float *__restrict__ B, float *__restrict__ X, float *__restrict__ R,
     float *__restrict__ Z, const float *__restrict__ binv, const float *__restrict__ dinv,
     double *__restrict__ part, InnerScalars *sc, double inner_tol) {
-    __shared__ double sh[K * MBS / 64];
-    double bb[K], rz[K];
+    double acc[2 * K];
 #pragma unroll
-    for (int j = 0; j < K; j++) bb[j] = rz[j] = 0.0;
+    for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
     for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
          b += (int64_t)gridDim.x * MBS) {
-        float r[3][K];
+        float r[3][K], zero[K];
 #pragma unroll
-        for (int d = 0; d < 3; d++)
+        for (int j = 0; j < K; j++) zero[j] = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            load_k<K>(B + (3 * b + d) * K, r[d]);
+            store_k<K>(R + (3 * b + d) * K, r[d]);
+            store_k<K>(X + (3 * b + d) * K, zero);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            float z[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                int64_t idx = (3 * b + d) * K + j;
-                r[d][j] = B[idx];
-                R[idx] = r[d][j];
-                X[idx] = 0.f;
+                z[j] = BLOCK ? (binv[9 * b + 3 * d] * r[0][j] + binv[9 * b + 3 * d + 1] * r[1][j] +
+                                binv[9 * b + 3 * d + 2] * r[2][j])
+                             : dinv[3 * b + d] * r[d][j];
+                acc[j] += (double)r[d][j] * r[d][j];
+                acc[K + j] += (double)r[d][j] * z[j];
             }
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                float z = BLOCK ? (binv[9 * b + 3 * d] * r[0][j] + binv[9 * b + 3 * d + 1] * r[1][j] +
-                                   binv[9 * b + 3 * d + 2] * r[2][j])
-                                : dinv[3 * b + d] * r[d][j];
-                Z[(3 * b + d) * K + j] = z;
-                bb[j] += (double)r[d][j] * r[d][j];
-                rz[j] += (double)r[d][j] * z;
-            }
-    }
-    block_sums<K>(bb, sh);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * 2 * K + j] = bb[j];
-    block_sums<K>(rz, sh);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * 2 * K + K + j] = rz[j];
-    if (last_block_m(&sc->ticket[1])) {
-        for (int j = 0; j < K; j++) {
-            double bbs = ordered_sum(part, gridDim.x, 2 * K, j);
-            double rzs = ordered_sum(part, gridDim.x, 2 * K, K + j);
-            if (threadIdx.x == 0) {
-                double bn = sqrt(bbs);
-                sc->bnorm[j] = bn;
-                sc->res[j] = bn;
-                sc->tol_abs[j] = inner_tol * bn;
-                sc->rz[j] = rzs;
-                sc->iters[j] = 0;
-                sc->done[j] = (bn < 1e-30) ? 1 : 0;
-            }
-        }
-        if (threadIdx.x == 0) {
-            int all = 1;
-            for (int j = 0; j < K; j++) all &= sc->done[j];
-            sc->all_done = all;
-            sc->ticket[1] = 0;
+            store_k<K>(Z + (3 * b + d) * K, z);
         }
     }
+    group_sums<2 * K, MBS>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
+    (void)sc, (void)inner_tol;
 }
 
 // ------------------------------------------------- FP64 residual pieces
@@ -535,8 +563,8 @@ __global__ void __launch_bounds__(MBS) csr_residual_kernel(
 #pragma unroll
         for (int j = 0; j < K; j++) acc[j] = 0.0;
         for (int32_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += G) {
-            double v = val[k];
-            int64_t c = colind[k];
+            double v = __builtin_nontemporal_load(&val[k]);
+            int64_t c = __builtin_nontemporal_load(&colind[k]);
 #pragma unroll
             for (int j = 0; j < K; j++) acc[j] += v * X[c + j * n];
         }
@@ -556,24 +584,17 @@ __global__ void __launch_bounds__(MBS) col_norms_kernel(int64_t n, const double 
                                                        double *__restrict__ part,
                                                        double *__restrict__ out,
                                                        uint32_t *ticket) {
-    __shared__ double sh[K * MBS / 64];
     double s[K];
 #pragma unroll
     for (int j = 0; j < K; j++) s[j] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * MBS)
 #pragma unroll
         for (int j = 0; j < K; j++) s[j] += Y[i + j * n] * Y[i + j * n];
-    block_sums<K>(s, sh);
+    group_sums<K, MBS>(s);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * K + j] = s[j];
-    if (last_block_m(ticket)) {
-        for (int j = 0; j < K; j++) {
-            double t = ordered_sum(part, gridDim.x, K, j);
-            if (threadIdx.x == 0) out[j] = sqrt(t);
-        }
-        if (threadIdx.x == 0) *ticket = 0;
-    }
+    (void)out, (void)ticket;
 }
 
 // FP64 free-space residual -> FP32 block space, scaled by 1/|r_j|.
@@ -603,14 +624,122 @@ __global__ void unpack_add_kernel(int64_t n, const float *__restrict__ D,
     for (int j = 0; j < K; j++) U[i + j * n] += scale[j] * (double)D[bi * K + j];
 }
 
-// ------------------------------------------------------------ driver
-static int lanes_for(double avg) {
-    if (avg <= 6) return 4;
-    if (avg <= 12) return 8;
-    if (avg <= 24) return 16;
-    if (avg <= 48) return 32;
-    return 64;
+// ----------------------------------------------------- SpMM shapes
+// U (blocks loaded per lane before their gathers) is picked per operator and
+// right-hand-side count by timing the candidates.
+static const int kSpmmU[] = {1, 2, 3, 4};
+static const int kNumSpmm = 4;
+#define AXG_SPMM_SWITCH(IDX, MACRO) \
+    switch (IDX) {                  \
+        case 0: MACRO(1); break;    \
+        case 1: MACRO(2); break;    \
+        case 2: MACRO(3); break;    \
+        default: MACRO(4); break;   \
+    }
+
+static int spmm_grid(axg_ctx *ctx, int64_t nb) {
+    return std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS, (int64_t)ctx->cu_count * 8));
 }
+
+template <int K>
+static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
+    if (m->spmm_shape[K] >= 0) return m->spmm_shape[K];
+    axg_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    int best = 1;
+    const int grid = spmm_grid(ctx, m->nb);
+    if (m->nnzb > 200000) {
+        float best_ms = 1e30f;
+        for (int c = 0; c < kNumSpmm; c++) {
+#define AXG_T(UU)                                                                                \
+    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
+                       m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, P, AP,  \
+                       nullptr, nullptr)
+            AXG_SPMM_SWITCH(c, AXG_T)
+            AXG_HIP(hipEventRecord(ctx->ev0, s));
+            for (int r = 0; r < 5; r++) {
+                AXG_SPMM_SWITCH(c, AXG_T)
+            }
+#undef AXG_T
+            AXG_HIP(hipEventRecord(ctx->ev1, s));
+            AXG_HIP(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best_ms) best_ms = ms, best = c;
+        }
+        AXG_HIP(hipGetLastError());
+    }
+    m->spmm_shape[K] = best;
+    return best;
+}
+
+// Y[:, j] = F[:, j] - A X[:, j] on the sliced-ELL FP64 operator (one thread
+// per row, K right-hand sides, column-major n x K).
+template <int K>
+__global__ void __launch_bounds__(MBS) sell_residual_kernel(
+    int64_t n, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+    const int32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ X,
+    const double *__restrict__ F, double *__restrict__ Y) {
+    for (int64_t row = (int64_t)blockIdx.x * MBS + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * MBS) {
+        const int64_t sl = row >> 6, base = sptr[sl] + (row & 63);
+        const int w = width[sl];
+        double acc[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) acc[j] = 0.0;
+        for (int jj = 0; jj < w; jj++) {
+            const int64_t p = base + 64 * (int64_t)jj;
+            double v = __builtin_nontemporal_load(&val[p]);
+            int64_t c = __builtin_nontemporal_load(&col[p]);
+#pragma unroll
+            for (int j = 0; j < K; j++) acc[j] += v * X[c + j * n];
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) Y[row + j * n] = F[row + j * n] - acc[j];
+    }
+}
+
+// Repeated launches of the FP32 operator (microbenchmark / profiling).
+template <int K>
+static void bench_spmm_k(axg_mesh *m, int reps, double *avg_ms) {
+    axg_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t nv = 3 * m->nb;
+    DevBuf<float> P(nv * K), AP(nv * K);
+    AXG_HIP(hipMemsetAsync(P.p, 0, 4 * nv * K, s));
+    const int shp = tune_spmm<K>(m, P.p, AP.p);
+    const int grid = spmm_grid(ctx, m->nb);
+#define AXG_T(UU)                                                                                \
+    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
+                       m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, P.p,    \
+                       AP.p, nullptr, nullptr)
+    AXG_SPMM_SWITCH(shp, AXG_T)
+    AXG_HIP(hipEventRecord(ctx->ev0, s));
+    for (int r = 0; r < reps; r++) {
+        AXG_SPMM_SWITCH(shp, AXG_T)
+    }
+#undef AXG_T
+    AXG_HIP(hipEventRecord(ctx->ev1, s));
+    AXG_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *avg_ms = ms / reps;
+}
+
+void bench_operator_mixed(axg_mesh *m, int nrhs, int reps, double *avg_ms) {
+    if (!m->assembled32) {
+        double ms;
+        assemble_mixed(m, &ms);
+    }
+    switch (nrhs) {
+        case 8: bench_spmm_k<8>(m, reps, avg_ms); break;
+        case 4: bench_spmm_k<4>(m, reps, avg_ms); break;
+        case 2: bench_spmm_k<2>(m, reps, avg_ms); break;
+        default: bench_spmm_k<1>(m, reps, avg_ms); break;
+    }
+}
+
+// ------------------------------------------------------------ driver
 
 template <int K>
 struct MixedSolver {
@@ -620,7 +749,7 @@ struct MixedSolver {
         hipStream_t s = ctx->stream;
         const int64_t n = m->n_free, nb = m->nb, nv = 3 * nb;
         const bool block = o.precond == AXG_PRECOND_BLOCK3;
-        CsrView A = active_csr(m);
+        CsrView A = solver_csr(m);
         // workspace
         Workspace &w = ctx->ws;
         w.f0.ensure(nv * K);  // B (inner rhs)
@@ -631,13 +760,11 @@ struct MixedSolver {
         DevBuf<float> Z(nv * K);
         w.t0.ensure(n * K);   // r64
         w.t1.ensure(16);      // norms / scales
-        const int gb = lanes_for(m->nb ? (double)m->nnzb / (double)m->nb : 1.0);
-        const int grid_mv = std::max(1, (int)std::min<int64_t>((nb * gb + MBS - 1) / MBS,
-                                                               (int64_t)ctx->cu_count * 8));
+        const int shp = tune_spmm<K>(m, w.f3.p, w.f4.p);
+        const int grid_mv = spmm_grid(ctx, nb);
         const int grid_v = std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
                                                               (int64_t)ctx->cu_count * 4));
-        const int gr = lanes_for(n ? (double)A.nnz / (double)n : 1.0);
-        const int grid_r = std::max(1, (int)std::min<int64_t>((n * gr + MBS - 1) / MBS,
+        const int grid_r = std::max(1, (int)std::min<int64_t>((n + MBS - 1) / MBS,
                                                               (int64_t)ctx->cu_count * 8));
         w.part.ensure((size_t)std::max({grid_mv, grid_v, grid_r}) * 2 * K + 8);
         DevBuf<InnerScalars> dsc(1);
@@ -653,19 +780,15 @@ struct MixedSolver {
 
         auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1) {
             if (e0) AXG_HIP(hipEventRecord(e0, s));
-#define AXG_SPMM(GG)                                                                          \
-    hipLaunchKernelGGL((bsr_spmm_dot_kernel<K, GG>), dim3(grid_mv), dim3(MBS), 0, s, nb,        \
-                       m->b_rowptr.p, m->b_colind.p, m->bval.p, m->nnzb, w.f3.p, w.f4.p, w.part.p, \
-                       dsc.p)
-            switch (gb) {
-                case 4: AXG_SPMM(4); break;
-                case 8: AXG_SPMM(8); break;
-                case 16: AXG_SPMM(16); break;
-                case 32: AXG_SPMM(32); break;
-                default: AXG_SPMM(64); break;
-            }
+#define AXG_SPMM(UU)                                                                            \
+    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s, nb,      \
+                       m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, w.f3.p,  \
+                       w.f4.p, w.part.p, dsc.p)
+            AXG_SPMM_SWITCH(shp, AXG_SPMM)
 #undef AXG_SPMM
             if (e1) AXG_HIP(hipEventRecord(e1, s));
+            hipLaunchKernelGGL((fin_spmm_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p, grid_mv,
+                               dsc.p);
         };
         auto launch_update = [&]() {
             if (block)
@@ -676,23 +799,19 @@ struct MixedSolver {
                 hipLaunchKernelGGL((inner_update_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
                                    nb, w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p,
                                    m->dinv32.p, w.part.p, dsc.p);
+            hipLaunchKernelGGL((fin_inner_update_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p,
+                               grid_v, dsc.p);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
                                w.f3.p, Z.p, dsc.p, 0);
         };
         auto residual = [&]() {
-#define AXG_RES(GG)                                                                              \
-    hipLaunchKernelGGL((csr_residual_kernel<K, GG>), dim3(grid_r), dim3(MBS), 0, s, n, A.rowptr,   \
-                       A.colind, A.val, d_u, d_f, w.t0.p)
-            switch (gr) {
-                case 4: AXG_RES(4); break;
-                case 8: AXG_RES(8); break;
-                case 16: AXG_RES(16); break;
-                case 32: AXG_RES(32); break;
-                default: AXG_RES(64); break;
-            }
-#undef AXG_RES
+            hipLaunchKernelGGL((sell_residual_kernel<K>), dim3(grid_r), dim3(MBS), 0, s, n,
+                               A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, d_u, d_f, w.t0.p);
+
             hipLaunchKernelGGL((col_norms_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, n, w.t0.p,
                                w.part.p, w.t1.p, ticket.p);
+            hipLaunchKernelGGL((fin_norms_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p, grid_v,
+                               w.t1.p);
             AXG_HIP(hipMemcpyAsync(hn, w.t1.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
             AXG_HIP(hipStreamSynchronize(s));
         };
@@ -756,6 +875,8 @@ struct MixedSolver {
                 hipLaunchKernelGGL((inner_init_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
                                    nb, w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p,
                                    w.part.p, dsc.p, o.inner_tol);
+            hipLaunchKernelGGL((fin_inner_init_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p,
+                               grid_v, dsc.p, o.inner_tol);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
                                w.f3.p, Z.p, dsc.p, 1);
             AXG_HIP(hipGetLastError());

@@ -9,7 +9,26 @@ struct CsrView {
     const int32_t *rowptr = nullptr;
     const int32_t *colind = nullptr;
     const double *val = nullptr;
+    int *shape = nullptr;  // tuned SpMV shape cache (-1: not tuned yet), owned by the caller
+    // optional sliced-ELL copy (64 rows per slice); used by the solver when set
+    const int64_t *sell_sptr = nullptr;
+    const int32_t *sell_width = nullptr;
+    const int32_t *sell_col = nullptr;
+    const double *sell_val = nullptr;
 };
+
+// Sliced ELL: slice s holds rows 64 s .. 64 s + 63; entry j of row r sits at
+// sptr[s] + 64 j + (r % 64); rows are padded to the slice's longest row with
+// column = the row itself and value 0.
+struct SellHost {
+    int64_t nslots = 0;
+    std::vector<int64_t> sptr;
+    std::vector<int32_t> width;
+};
+void sell_layout(const std::vector<int64_t> &row_len, SellHost &out);
+void ensure_sell(axg_mesh *m);
+CsrView solver_csr(axg_mesh *m);
+CsrView active_csr(axg_mesh *m);
 
 struct Precond64Args {
     const double *dsel = nullptr;         // Jacobi divisor
@@ -24,6 +43,7 @@ struct PcgResult {
     double residual = 0.0, bnorm = 0.0;
     double solve_ms = 0.0, spmv_ms = 0.0;
     int64_t spmv_launches = 0;
+    int spmv_shape_L = 0, spmv_shape_U = 0;
 };
 
 void spmv_fp64(axg_ctx *ctx, const CsrView &A, const double *x, double *y);

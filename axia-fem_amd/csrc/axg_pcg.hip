@@ -1,90 +1,58 @@
-// Preconditioned conjugate gradients on gfx950.
+// FP64 preconditioned conjugate gradients on gfx950.
 //
-// FP64 path: the reference's Jacobi PCG (linalg.rs:194-259) — x0 = 0,
-// recursive residual, stop when ||r|| <= tol ||b||, "not SPD" when
-// |p.Ap| < 1e-30, "not converged" when the final ||r|| > 10 tol ||b|| — on a
-// CSR matrix. Each iteration is three kernels:
+// The reference's Jacobi PCG (linalg.rs:194-259): x0 = 0, recursive residual,
+// stop when ||r|| <= tol ||b||, "not SPD" when |p.Ap| < 1e-30, "not
+// converged" when the final ||r|| > 10 tol ||b||, z = r / diag. Each
+// iteration is three kernels:
 //   spmv_dot   Ap = A p and p.Ap (last workgroup forms alpha)
-//   update_xr  x += alpha p, r -= alpha Ap, ||r||^2, r.z (last workgroup forms
-//              beta and the stop decision)
+//   update_xr  x += alpha p, r -= alpha Ap, ||r||^2 and, with Jacobi, z and
+//              r.z (last workgroup: stop decision and beta)
 //   update_p   p = z + beta p
-// Scalars live in device memory (PcgScalars), so a batch of iterations runs
-// without host round trips; it is captured once into a hipGraph and replayed
-// until the device-side "done" flag is set. Reductions are deterministic:
-// fixed grid, per-workgroup partials, summed in index order by the last
-// workgroup to finish.
+// (block-Jacobi adds a precond_dot kernel: its z needs the other two DOFs of
+// the node). Scalars live in device memory, so batches of iterations run
+// without host round trips; a batch is captured once into a hipGraph and
+// replayed until the device-side "done" flag is set.
 //
-// Mixed path: the same iteration on an FP32 BSR-3 operator with k
-// right-hand sides at once (SpMM reuses each block for all k), 3x3
-// block-Jacobi or Jacobi preconditioner, inside an FP64 iterative refinement
-// whose true residual f - K u uses the FP64 CSR.
+// SpMV: L lanes per CSR row, each lane keeping U loads in flight per pass
+// (val and column streamed with non-temporal loads, x gathered through the
+// caches). (L, U) is picked per matrix by timing the candidates.
 #include <cmath>
+#include <cstring>
 
 #include "axg_internal.h"
 #include "axg_pcg.h"
+#include "axg_reduce.h"
+#include "axg_mixed.h"
 
 namespace axg {
 
-constexpr int BS = 256;  // threads per workgroup for vector kernels
+constexpr int BS = 256;  // threads per workgroup
 
-// ------------------------------------------------------------ reductions
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Sum over the workgroup; result valid in thread 0. sh: BS/64 doubles.
-__device__ inline double block_sum(double v, double *sh) {
-    v = wave_sum(v);
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
-    return t;
-}
-
-// Called by every thread after thread 0 stored this workgroup's partial(s).
-// Returns true in the workgroup that finished last; it may then read all
-// partials. The ticket is reset for the next launch.
-__device__ inline bool last_block(uint32_t *ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = atomicAdd(ticket, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
-
-// Thread 0 of the last workgroup sums nblk partials (stride `stride`,
-// component `comp`) in index order, in chunks handled by the whole group.
-__device__ inline double reduce_partials(const double *part, int nblk, int stride, int comp,
-                                         double *sh) {
-    // each thread sums a contiguous range; then thread 0 sums thread sums in order
-    int per = (nblk + blockDim.x - 1) / blockDim.x;
-    int a = threadIdx.x * per, b = min(nblk, a + per);
+// ----------------------------------------------------------------- SpMV
+// Row sum of one CSR row by the L lanes of a group; valid in lane 0.
+template <int L, int U>
+__device__ inline double row_dot(const int32_t *__restrict__ colind, const double *__restrict__ val,
+                                 const double *__restrict__ x, int32_t k0, int32_t k1, int lane) {
     double s = 0.0;
-    for (int i = a; i < b; i++) s += part[(size_t)i * stride + comp];
-    __shared__ double tsum[BS];
-    tsum[threadIdx.x] = s;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)blockDim.x; i++) t += tsum[i];
-    __syncthreads();
-    (void)sh;
-    return t;
+    for (int32_t base = k0 + lane; base < k1; base += L * U) {
+        int32_t c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int32_t k = base + u * L;
+            bool ok = k < k1;
+            c[u] = ok ? __builtin_nontemporal_load(&colind[k]) : 0;
+            v[u] = ok ? __builtin_nontemporal_load(&val[k]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) s += v[u] * x[c[u]];
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
+    return s;
 }
 
-// ------------------------------------------------------------- FP64 CSR
-template <int L>
+template <int L, int U>
 __global__ void __launch_bounds__(BS) spmv_dot_kernel(int64_t n, const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colind,
                                                      const double *__restrict__ val,
@@ -92,41 +60,22 @@ __global__ void __launch_bounds__(BS) spmv_dot_kernel(int64_t n, const int32_t *
                                                      double *__restrict__ ap,
                                                      double *__restrict__ part, PcgScalars *sc) {
     if (sc->done) return;
-    __shared__ double sh[BS / 64];
     const int lane = threadIdx.x % L;
     const int64_t group = ((int64_t)blockIdx.x * BS + threadIdx.x) / L;
     const int64_t ngroups = (int64_t)gridDim.x * BS / L;
-    double dot = 0.0;
+    double dot[1] = {0.0};
     for (int64_t row = group; row < n; row += ngroups) {
-        int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-        double s = 0.0;
-        for (int32_t k = k0 + lane; k < k1; k += L) s += val[k] * p[colind[k]];
-#pragma unroll
-        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
+        double s = row_dot<L, U>(colind, val, p, rowptr[row], rowptr[row + 1], lane);
         if (lane == 0) {
             ap[row] = s;
-            dot += p[row] * s;
+            dot[0] += p[row] * s;
         }
     }
-    double t = block_sum(dot, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-    if (last_block(&sc->ticket[0])) {
-        double pap = reduce_partials(part, gridDim.x, 1, 0, sh);
-        if (threadIdx.x == 0) {
-            sc->pap = pap;
-            if (fabs(pap) < 1e-30) {
-                sc->status = AXG_E_NOT_SPD;
-                sc->done = 1;
-            } else {
-                sc->alpha = sc->rz / pap;
-            }
-            sc->ticket[0] = 0;
-        }
-    }
+    group_sums<1, BS>(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = dot[0];
 }
 
-// Plain y = A x (no reduction), used for true residuals and reactions.
-template <int L>
+template <int L, int U>
 __global__ void __launch_bounds__(BS) spmv_kernel(int64_t n, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ colind,
                                                  const double *__restrict__ val,
@@ -136,22 +85,278 @@ __global__ void __launch_bounds__(BS) spmv_kernel(int64_t n, const int32_t *__re
     const int64_t group = ((int64_t)blockIdx.x * BS + threadIdx.x) / L;
     const int64_t ngroups = (int64_t)gridDim.x * BS / L;
     for (int64_t row = group; row < n; row += ngroups) {
-        int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-        double s = 0.0;
-        for (int32_t k = k0 + lane; k < k1; k += L) s += val[k] * x[colind[k]];
-#pragma unroll
-        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
+        double s = row_dot<L, U>(colind, val, x, rowptr[row], rowptr[row + 1], lane);
         if (lane == 0) y[row] = s;
     }
 }
 
+// ------------------------------------------------------ sliced ELL
+// One thread per row; the 64 rows of a slice are the 64 lanes of a
+// wavefront, so entry j of all rows is one coalesced 64-wide load, and the
+// row sum needs no cross-lane reduction. U entries per lane are loaded before
+// the x gathers are issued.
+template <int U>
+__device__ inline double sell_row(const int32_t *__restrict__ col, const double *__restrict__ val,
+                                  const double *__restrict__ x, int64_t base, int w) {
+    double acc = 0.0;
+    int j = 0;
+    for (; j + U <= w; j += U) {
+        int32_t c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            c[u] = __builtin_nontemporal_load(&col[base + 64 * (int64_t)(j + u)]);
+            v[u] = __builtin_nontemporal_load(&val[base + 64 * (int64_t)(j + u)]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += v[u] * x[c[u]];
+    }
+    for (; j < w; j++) acc += val[base + 64 * (int64_t)j] * x[col[base + 64 * (int64_t)j]];
+    return acc;
+}
+
+template <int U>
+__global__ void __launch_bounds__(BS) sell_spmv_dot_kernel(int64_t n, const int64_t *__restrict__ sptr,
+                                                          const int32_t *__restrict__ width,
+                                                          const int32_t *__restrict__ col,
+                                                          const double *__restrict__ val,
+                                                          const double *__restrict__ p,
+                                                          double *__restrict__ ap,
+                                                          double *__restrict__ part,
+                                                          PcgScalars *sc) {
+    if (sc->done) return;
+    double dot[1] = {0.0};
+    for (int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * BS) {
+        const int64_t sl = row >> 6;
+        double s = sell_row<U>(col, val, p, sptr[sl] + (row & 63), width[sl]);
+        ap[row] = s;
+        dot[0] += p[row] * s;
+    }
+    group_sums<1, BS>(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = dot[0];
+}
+
+template <int U>
+__global__ void __launch_bounds__(BS) sell_spmv_kernel(int64_t n, const int64_t *__restrict__ sptr,
+                                                      const int32_t *__restrict__ width,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ y) {
+    for (int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * BS) {
+        const int64_t sl = row >> 6;
+        y[row] = sell_row<U>(col, val, x, sptr[sl] + (row & 63), width[sl]);
+    }
+}
+
+// Build the sliced-ELL columns and the source slot of every entry from the
+// CSR pattern (one thread per row).
+__global__ void sell_build_kernel(int64_t n, const int32_t *__restrict__ rowptr,
+                                  const int32_t *__restrict__ colind,
+                                  const int64_t *__restrict__ sptr,
+                                  const int32_t *__restrict__ width, int32_t *__restrict__ scol,
+                                  int32_t *__restrict__ ssrc) {
+    int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int64_t sl = row >> 6, base = sptr[sl] + (row & 63);
+    const int32_t k0 = rowptr[row], len = rowptr[row + 1] - k0, w = width[sl];
+    for (int j = 0; j < w; j++) {
+        int64_t p = base + 64 * (int64_t)j;
+        scol[p] = j < len ? colind[k0 + j] : (int32_t)row;
+        ssrc[p] = j < len ? k0 + j : -1;
+    }
+}
+
+// Rows of the last slice past n: columns pointing at row 0, value 0.
+__global__ void sell_tail_kernel(int64_t n, const int64_t *__restrict__ sptr,
+                                 const int32_t *__restrict__ width, int32_t *__restrict__ scol,
+                                 int32_t *__restrict__ ssrc) {
+    const int64_t sl = (n - 1) >> 6;
+    const int lane = threadIdx.x;
+    if (lane >= 64 || (sl * 64 + lane) < n) return;
+    for (int j = 0; j < width[sl]; j++) {
+        int64_t p = sptr[sl] + lane + 64 * (int64_t)j;
+        scol[p] = 0;
+        ssrc[p] = -1;
+    }
+}
+
+__global__ void sell_fill_kernel(int64_t nslots, const int32_t *__restrict__ ssrc,
+                                 const double *__restrict__ val, double *__restrict__ sval) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nslots;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t k = ssrc[p];
+        sval[p] = k >= 0 ? val[k] : 0.0;
+    }
+}
+
+// Candidate (L, U) shapes; the index is what the autotuner stores.
+struct SpmvShape {
+    int L, U;
+};
+static const SpmvShape kShapes[] = {{8, 4}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {32, 3},
+                                    {64, 1}, {64, 2}, {64, 4}};
+static const int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+#define AXG_SHAPE_SWITCH(IDX, MACRO) \
+    switch (IDX) {                   \
+        case 0: MACRO(8, 4); break;  \
+        case 1: MACRO(16, 2); break; \
+        case 2: MACRO(16, 4); break; \
+        case 3: MACRO(32, 1); break; \
+        case 4: MACRO(32, 2); break; \
+        case 5: MACRO(32, 3); break; \
+        case 6: MACRO(64, 1); break; \
+        case 7: MACRO(64, 2); break; \
+        default: MACRO(64, 4); break; \
+    }
+
+static int grid_for(axg_ctx *ctx, int64_t work_threads, int per_cu) {
+    int64_t g = (work_threads + BS - 1) / BS;
+    int64_t cap = (int64_t)ctx->cu_count * per_cu;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+static int default_shape(const CsrView &A) {
+    double avg = A.n > 0 ? (double)A.nnz / (double)A.n : 1.0;
+    if (avg <= 24) return 0;
+    if (avg <= 64) return 2;
+    if (avg <= 100) return 5;
+    return 8;
+}
+
+static void launch_spmv_shape(axg_ctx *ctx, int shape, const CsrView &A, const double *x,
+                              double *y) {
+    const int L = kShapes[shape].L;
+    const int grid = grid_for(ctx, A.n * L, 8);
+    hipStream_t s = ctx->stream;
+#define AXG_L(LL, UU)                                                                            \
+    hipLaunchKernelGGL((spmv_kernel<LL, UU>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr,         \
+                       A.colind, A.val, x, y)
+    AXG_SHAPE_SWITCH(shape, AXG_L)
+#undef AXG_L
+}
+
+// Autotuner: time every candidate on this matrix (a few launches each) and
+// keep the fastest. Called once per matrix; small matrices keep the default.
+int tune_spmv(axg_ctx *ctx, const CsrView &A) {
+    if (A.nnz < 2000000) return default_shape(A);
+    hipStream_t s = ctx->stream;
+    DevBuf<double> x(A.n), y(A.n);
+    AXG_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * A.n, s));
+    int best = default_shape(A);
+    float best_ms = 1e30f;
+    for (int c = 0; c < kNumShapes; c++) {
+        launch_spmv_shape(ctx, c, A, x.p, y.p);  // warm
+        AXG_HIP(hipEventRecord(ctx->ev0, s));
+        for (int r = 0; r < 5; r++) launch_spmv_shape(ctx, c, A, x.p, y.p);
+        AXG_HIP(hipEventRecord(ctx->ev1, s));
+        AXG_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ms < best_ms) best_ms = ms, best = c;
+    }
+    AXG_HIP(hipGetLastError());
+    return best;
+}
+
+static int shape_of(axg_ctx *ctx, const CsrView &A) {
+    if (A.shape && *A.shape >= 0) return *A.shape;
+    int sh = tune_spmv(ctx, A);
+    if (A.shape) *A.shape = sh;
+    return sh;
+}
+
+void spmv_fp64(axg_ctx *ctx, const CsrView &A, const double *x, double *y) {
+    if (A.sell_val) {
+        const int grid = grid_for(ctx, A.n, 8);
+        hipLaunchKernelGGL((sell_spmv_kernel<4>), dim3(grid), dim3(BS), 0, ctx->stream, A.n,
+                           A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, x, y);
+    } else {
+        launch_spmv_shape(ctx, shape_of(ctx, A), A, x, y);
+    }
+    AXG_HIP(hipGetLastError());
+}
+
+void sell_layout(const std::vector<int64_t> &row_len, SellHost &out) {
+    const int64_t n = (int64_t)row_len.size(), ns = (n + 63) / 64;
+    out.sptr.assign(ns + 1, 0);
+    out.width.assign(ns, 0);
+    for (int64_t s = 0; s < ns; s++) {
+        int64_t w = 0;
+        for (int64_t r = 64 * s; r < std::min(n, 64 * s + 64); r++) w = std::max(w, row_len[r]);
+        out.width[s] = (int32_t)w;
+        out.sptr[s + 1] = out.sptr[s] + 64 * w;
+    }
+    out.nslots = out.sptr[ns];
+}
+
+// Sliced-ELL copy of the mesh's active K_ff: layout once per pattern, values
+// refreshed after every assembly.
+void ensure_sell(axg_mesh *m) {
+    hipStream_t s = m->ctx->stream;
+    CsrView A = active_csr(m);
+    const int64_t n = A.n;
+    if (n == 0) return;
+    if (!m->sell_valid) {
+        std::vector<int32_t> rp(n + 1);
+        AXG_HIP(hipMemcpyAsync(rp.data(), A.rowptr, 4 * (n + 1), hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        std::vector<int64_t> len(n);
+        for (int64_t i = 0; i < n; i++) len[i] = rp[i + 1] - rp[i];
+        SellHost L;
+        sell_layout(len, L);
+        AXG_REQUIRE(L.nslots < INT32_MAX, AXG_E_INVALID, "sliced-ELL operator exceeds 2^31 slots");
+        m->sell_nslots = L.nslots;
+        m->sell_sptr.alloc(L.sptr.size());
+        m->sell_width.alloc(L.width.size());
+        m->sell_col.alloc(L.nslots);
+        m->sell_src.alloc(L.nslots);
+        m->sell_val.alloc(L.nslots);
+        AXG_HIP(hipMemcpyAsync(m->sell_sptr.p, L.sptr.data(), 8 * L.sptr.size(),
+                               hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(m->sell_width.p, L.width.data(), 4 * L.width.size(),
+                               hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(sell_build_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           n, A.rowptr, A.colind, m->sell_sptr.p, m->sell_width.p, m->sell_col.p,
+                           m->sell_src.p);
+        hipLaunchKernelGGL(sell_tail_kernel, dim3(1), dim3(64), 0, s, n, m->sell_sptr.p,
+                           m->sell_width.p, m->sell_col.p, m->sell_src.p);
+        AXG_HIP(hipGetLastError());
+        m->sell_valid = true;
+    }
+    if (!m->sell_fresh) {
+        int64_t blocks = std::min<int64_t>((m->sell_nslots + 255) / 256, 1 << 20);
+        hipLaunchKernelGGL(sell_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                           m->sell_nslots, m->sell_src.p, A.val, m->sell_val.p);
+        AXG_HIP(hipGetLastError());
+        m->sell_fresh = true;
+    }
+}
+
+// The solver's view of the mesh operator: active CSR plus its sliced-ELL copy.
+CsrView solver_csr(axg_mesh *m) {
+    ensure_sell(m);
+    CsrView A = active_csr(m);
+    A.sell_sptr = m->sell_sptr.p;
+    A.sell_width = m->sell_width.p;
+    A.sell_col = m->sell_col.p;
+    A.sell_val = m->sell_val.p;
+    return A;
+}
+
+// ------------------------------------------------------- vector kernels
 // z = M^-1 r for one DOF: Jacobi (r / d, linalg.rs:213-216) or the 3x3
 // node-block inverse (block of DOF i, row d).
 struct Precond64 {
-    const double *dsel;       // Jacobi divisor
-    const double *binv;       // 9 per block, or null
-    const int32_t *blk_of_free;  // 3*blockrow + d
-    const int32_t *free_of_blk;  // free index or -1
+    const double *dsel;
+    const double *binv;
+    const int32_t *blk_of_free;
+    const int32_t *free_of_blk;
     __device__ inline double apply(const double *r, int64_t i) const {
         if (!binv) return r[i] / dsel[i];
         int32_t bd = blk_of_free[i];
@@ -167,9 +372,6 @@ struct Precond64 {
     }
 };
 
-// x += alpha p, r -= alpha Ap, ||r||^2; with Jacobi also z = r / d and r.z
-// (FUSED). The last workgroup forms ||r||, counts the iteration, decides
-// convergence (linalg.rs:241-245) and, with FUSED, beta.
 template <bool FUSED>
 __global__ void __launch_bounds__(BS) update_xr_kernel(int64_t n, double *__restrict__ x,
                                                       double *__restrict__ r,
@@ -179,70 +381,108 @@ __global__ void __launch_bounds__(BS) update_xr_kernel(int64_t n, double *__rest
                                                       const double *__restrict__ dsel,
                                                       double *__restrict__ part, PcgScalars *sc) {
     if (sc->done) return;
-    __shared__ double sh[BS / 64];
     const double alpha = sc->alpha;
-    double rr = 0.0, rz = 0.0;
+    double acc[2] = {0.0, 0.0};  // ||r||^2, r.z
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
         x[i] += alpha * p[i];
         double ri = r[i] - alpha * ap[i];
         r[i] = ri;
-        rr += ri * ri;
+        acc[0] += ri * ri;
         if (FUSED) {
             double zi = ri / dsel[i];
             z[i] = zi;
-            rz += ri * zi;
+            acc[1] += ri * zi;
         }
     }
-    double t = block_sum(rr, sh);
-    if (threadIdx.x == 0) part[2 * blockIdx.x] = t;
-    if (FUSED) {
-        t = block_sum(rz, sh);
-        if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = t;
+    group_sums<2, BS>(acc);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = acc[0];
+        part[2 * blockIdx.x + 1] = acc[1];
     }
-    if (last_block(&sc->ticket[1])) {
-        double rrs = reduce_partials(part, gridDim.x, 2, 0, sh);
-        double rzs = FUSED ? reduce_partials(part, gridDim.x, 2, 1, sh) : 0.0;
-        if (threadIdx.x == 0) {
-            double res = sqrt(rrs);
-            sc->residual = res;
-            sc->iters += 1;
-            if (res <= sc->tol_abs) {
-                sc->done = 1;
-            } else if (sc->iters >= sc->max_iter) {
-                sc->done = 1;
-                sc->status = AXG_E_NOT_CONVERGED;
-            } else if (FUSED) {
-                sc->beta = rzs / sc->rz;
-                sc->rz = rzs;
-            }
-            sc->ticket[1] = 0;
+}
+
+// ----------------------------------------------------- finalize kernels
+// One workgroup: sum the partials of the kernel just before (fixed order)
+// and update the scalars.
+__global__ void __launch_bounds__(BS) fin_pap_kernel(const double *__restrict__ part, int nblk,
+                                                    PcgScalars *sc) {
+    if (sc->done) return;
+    double pap[1];
+    sum_partials<1, BS>(part, nblk, pap);
+    if (threadIdx.x == 0) {
+        sc->pap = pap[0];
+        if (fabs(pap[0]) < 1e-30) {  // linalg.rs:231-233
+            sc->status = AXG_E_NOT_SPD;
+            sc->done = 1;
+        } else {
+            sc->alpha = sc->rz / pap[0];
         }
     }
 }
 
-// z = M^-1 r, r.z, then (last workgroup) beta.
+// ||r||, iteration count, stop decision (linalg.rs:241-245); with FUSED also
+// beta = r.z_new / r.z.
+template <bool FUSED>
+__global__ void __launch_bounds__(BS) fin_update_kernel(const double *__restrict__ part, int nblk,
+                                                       PcgScalars *sc) {
+    if (sc->done) return;
+    double tot[2];
+    sum_partials<2, BS>(part, nblk, tot);
+    if (threadIdx.x == 0) {
+        double res = sqrt(tot[0]);
+        sc->residual = res;
+        sc->iters += 1;
+        if (res <= sc->tol_abs) {
+            sc->done = 1;
+        } else if (sc->iters >= sc->max_iter) {
+            sc->done = 1;
+            sc->status = AXG_E_NOT_CONVERGED;
+        } else if (FUSED) {
+            sc->beta = tot[1] / sc->rz;
+            sc->rz = tot[1];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BS) fin_rz_kernel(const double *__restrict__ part, int nblk,
+                                                   PcgScalars *sc, int init) {
+    if (sc->done) return;
+    double tot[1];
+    sum_partials<1, BS>(part, nblk, tot);
+    if (threadIdx.x == 0) {
+        if (!init) sc->beta = tot[0] / sc->rz;
+        sc->rz = tot[0];
+    }
+}
+
+__global__ void __launch_bounds__(BS) fin_init_kernel(const double *__restrict__ part, int nblk,
+                                                     PcgScalars *sc, double tol) {
+    double tot[1];
+    sum_partials<1, BS>(part, nblk, tot);
+    if (threadIdx.x == 0) {
+        double bn = sqrt(tot[0]);
+        sc->bnorm = bn;
+        sc->tol_abs = tol * bn;
+        sc->residual = bn;
+        if (bn < 1e-30) sc->done = 1;  // linalg.rs:206-214
+    }
+}
+
+// z = M^-1 r and r.z; the last workgroup forms beta (or, at start, rz).
 __global__ void __launch_bounds__(BS) precond_dot_kernel(int64_t n, const double *__restrict__ r,
                                                         double *__restrict__ z, Precond64 M,
                                                         double *__restrict__ part,
                                                         PcgScalars *sc, int init) {
     if (sc->done) return;
-    __shared__ double sh[BS / 64];
-    double rz = 0.0;
+    double acc[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
         double zi = M.apply(r, i);
         z[i] = zi;
-        rz += r[i] * zi;
+        acc[0] += r[i] * zi;
     }
-    double t = block_sum(rz, sh);
-    if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = t;
-    if (last_block(&sc->ticket[2])) {
-        double rzs = reduce_partials(part, gridDim.x, 2, 1, sh);
-        if (threadIdx.x == 0) {
-            if (!init) sc->beta = rzs / sc->rz;
-            sc->rz = rzs;
-            sc->ticket[2] = 0;
-        }
-    }
+    group_sums<1, BS>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+    (void)init;
 }
 
 __global__ void __launch_bounds__(BS) update_p_kernel(int64_t n, double *__restrict__ p,
@@ -254,32 +494,21 @@ __global__ void __launch_bounds__(BS) update_p_kernel(int64_t n, double *__restr
         p[i] = init ? z[i] : z[i] + beta * p[i];
 }
 
-// x = 0, r = b, ||b|| (last workgroup sets tol_abs and the zero-rhs exit).
+// x = 0, r = b, ||b|| (last workgroup: tol_abs and the zero right-hand side).
 __global__ void __launch_bounds__(BS) init_kernel(int64_t n, const double *__restrict__ b,
                                                  double *__restrict__ x, double *__restrict__ r,
                                                  double *__restrict__ part, PcgScalars *sc,
                                                  double tol) {
-    __shared__ double sh[BS / 64];
-    double bb = 0.0;
+    double acc[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
         double bi = b[i];
         x[i] = 0.0;
         r[i] = bi;
-        bb += bi * bi;
+        acc[0] += bi * bi;
     }
-    double t = block_sum(bb, sh);
-    if (threadIdx.x == 0) part[2 * blockIdx.x] = t;
-    if (last_block(&sc->ticket[1])) {
-        double s = reduce_partials(part, gridDim.x, 2, 0, sh);
-        if (threadIdx.x == 0) {
-            double bn = sqrt(s);
-            sc->bnorm = bn;
-            sc->tol_abs = tol * bn;
-            sc->residual = bn;
-            if (bn < 1e-30) sc->done = 1;
-            sc->ticket[1] = 0;
-        }
-    }
+    group_sums<1, BS>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+    (void)sc, (void)tol;
 }
 
 // dsel[i] = first (i, i) entry of row i if |d| > 1e-30, else 1.0 (Csr::diag,
@@ -299,56 +528,6 @@ __global__ void diag_kernel(int64_t n, const int32_t *__restrict__ rowptr,
     dsel[i] = fabs(d) > 1e-30 ? d : 1.0;
 }
 
-// ------------------------------------------------------------ host driver
-static int pick_lanes(int64_t n, int64_t nnz) {
-    double avg = n > 0 ? (double)nnz / (double)n : 1.0;
-    if (avg <= 12) return 4;
-    if (avg <= 24) return 8;
-    if (avg <= 48) return 16;
-    if (avg <= 160) return 32;
-    return 64;
-}
-
-static int grid_for(axg_ctx *ctx, int64_t work_threads, int per_cu) {
-    int64_t g = (work_threads + BS - 1) / BS;
-    int64_t cap = (int64_t)ctx->cu_count * per_cu;
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    return (int)g;
-}
-
-template <int L>
-static void launch_spmv_dot_l(hipStream_t s, int grid, const CsrView &A, const double *p,
-                              double *ap, double *part, PcgScalars *sc) {
-    hipLaunchKernelGGL((spmv_dot_kernel<L>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr, A.colind,
-                       A.val, p, ap, part, sc);
-}
-
-static void launch_spmv_dot(int L, hipStream_t s, int grid, const CsrView &A, const double *p,
-                            double *ap, double *part, PcgScalars *sc) {
-    switch (L) {
-        case 4: launch_spmv_dot_l<4>(s, grid, A, p, ap, part, sc); break;
-        case 8: launch_spmv_dot_l<8>(s, grid, A, p, ap, part, sc); break;
-        case 16: launch_spmv_dot_l<16>(s, grid, A, p, ap, part, sc); break;
-        case 32: launch_spmv_dot_l<32>(s, grid, A, p, ap, part, sc); break;
-        default: launch_spmv_dot_l<64>(s, grid, A, p, ap, part, sc); break;
-    }
-}
-
-void spmv_fp64(axg_ctx *ctx, const CsrView &A, const double *x, double *y) {
-    int L = pick_lanes(A.n, A.nnz);
-    int grid = grid_for(ctx, A.n * L, 16);
-    hipStream_t s = ctx->stream;
-    switch (L) {
-        case 4: hipLaunchKernelGGL((spmv_kernel<4>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr, A.colind, A.val, x, y); break;
-        case 8: hipLaunchKernelGGL((spmv_kernel<8>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr, A.colind, A.val, x, y); break;
-        case 16: hipLaunchKernelGGL((spmv_kernel<16>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr, A.colind, A.val, x, y); break;
-        case 32: hipLaunchKernelGGL((spmv_kernel<32>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr, A.colind, A.val, x, y); break;
-        default: hipLaunchKernelGGL((spmv_kernel<64>), dim3(grid), dim3(BS), 0, s, A.n, A.rowptr, A.colind, A.val, x, y); break;
-    }
-    AXG_HIP(hipGetLastError());
-}
-
 void diag_fp64(axg_ctx *ctx, const CsrView &A, double *dsel, double *diag_raw) {
     if (A.n == 0) return;
     hipLaunchKernelGGL(diag_kernel, dim3((unsigned)((A.n + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -356,7 +535,7 @@ void diag_fp64(axg_ctx *ctx, const CsrView &A, double *dsel, double *diag_raw) {
     AXG_HIP(hipGetLastError());
 }
 
-// One FP64 PCG solve. b, x: device vectors of length A.n.
+// ------------------------------------------------------------ host driver
 void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b,
               double *x, double tol, int64_t max_iter, bool timing, PcgResult &res) {
     hipStream_t s = ctx->stream;
@@ -366,39 +545,59 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
     w.p.ensure(n);
     w.ap.ensure(n);
     w.z.ensure(n);
-    const int L = pick_lanes(n, A.nnz);
-    const int grid_mv = grid_for(ctx, n * L, 8);
+    const bool sell = A.sell_val != nullptr;
+    const int shape = sell ? 0 : shape_of(ctx, A);
+    const int grid_mv = sell ? grid_for(ctx, n, 8) : grid_for(ctx, n * kShapes[shape].L, 8);
     const int grid_v = grid_for(ctx, n, 4);
     w.part.ensure(2 * (size_t)std::max(grid_mv, grid_v) + 2);
     w.sc.ensure(1);
     Precond64 M{pa.dsel, pa.binv, pa.blk_of_free, pa.free_of_blk};
+    const bool fused = pa.binv == nullptr;
 
-    PcgScalars init{};
-    init.max_iter = max_iter;
     ctx->pinned.ensure(sizeof(PcgScalars) * 2);
     PcgScalars *h = (PcgScalars *)ctx->pinned.p;
-    *h = init;
+    std::memset((void *)h, 0, sizeof(PcgScalars));
+    h->max_iter = max_iter;
     AXG_HIP(hipMemcpyAsync(w.sc.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     hipLaunchKernelGGL(init_kernel, dim3(grid_v), dim3(BS), 0, s, n, b, x, w.r.p, w.part.p, w.sc.p,
                        tol);
+    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_v, w.sc.p, tol);
     hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.r.p, w.z.p, M,
                        w.part.p, w.sc.p, 1);
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_v, w.sc.p, 1);
     hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p, 1);
     AXG_HIP(hipGetLastError());
 
     auto one_iter = [&](hipEvent_t e0, hipEvent_t e1) {
         if (e0) AXG_HIP(hipEventRecord(e0, s));
-        launch_spmv_dot(L, s, grid_mv, A, w.p.p, w.ap.p, w.part.p, w.sc.p);
+        if (sell) {
+            hipLaunchKernelGGL((sell_spmv_dot_kernel<4>), dim3(grid_mv), dim3(BS), 0, s, n,
+                               A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, w.p.p, w.ap.p,
+                               w.part.p, w.sc.p);
+        } else {
+#define AXG_L(LL, UU)                                                                            \
+    hipLaunchKernelGGL((spmv_dot_kernel<LL, UU>), dim3(grid_mv), dim3(BS), 0, s, n, A.rowptr,     \
+                       A.colind, A.val, w.p.p, w.ap.p, w.part.p, w.sc.p)
+            AXG_SHAPE_SWITCH(shape, AXG_L)
+#undef AXG_L
+        }
         if (e1) AXG_HIP(hipEventRecord(e1, s));
-        if (M.binv == nullptr) {
+        hipLaunchKernelGGL(fin_pap_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_mv, w.sc.p);
+        if (fused) {
             hipLaunchKernelGGL((update_xr_kernel<true>), dim3(grid_v), dim3(BS), 0, s, n, x, w.r.p,
                                w.p.p, w.ap.p, w.z.p, M.dsel, w.part.p, w.sc.p);
+            hipLaunchKernelGGL((fin_update_kernel<true>), dim3(1), dim3(BS), 0, s, w.part.p,
+                               grid_v, w.sc.p);
         } else {
             hipLaunchKernelGGL((update_xr_kernel<false>), dim3(grid_v), dim3(BS), 0, s, n, x,
                                w.r.p, w.p.p, w.ap.p, w.z.p, M.dsel, w.part.p, w.sc.p);
+            hipLaunchKernelGGL((fin_update_kernel<false>), dim3(1), dim3(BS), 0, s, w.part.p,
+                               grid_v, w.sc.p);
             hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.r.p, w.z.p,
                                M, w.part.p, w.sc.p, 0);
+            hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_v, w.sc.p,
+                               0);
         }
         hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p,
                            0);
@@ -416,8 +615,7 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
         AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         AXG_HIP(hipGraphDestroy(graph));
     } else {
-        size_t need = 2 * (size_t)batch;
-        while (ctx->timing_events.size() < need) {
+        while (ctx->timing_events.size() < 2 * (size_t)batch) {
             hipEvent_t e;
             AXG_HIP(hipEventCreate(&e));
             ctx->timing_events.push_back(e);
@@ -429,19 +627,20 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
         AXG_HIP(hipStreamSynchronize(s));
         if (h->done || launched >= max_iter) break;
         if (timing) {
+            int64_t before = h->iters;
             for (int i = 0; i < batch; i++)
                 one_iter(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1]);
             AXG_HIP(hipMemcpyAsync(h, w.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
             AXG_HIP(hipStreamSynchronize(s));
-            // SpMV launches that did work: the iterations counted in this batch
-            int64_t did = h->iters - res.spmv_launches;
+            // SpMV launches that did work: the iterations this batch counted
+            int64_t did = h->iters - before;
             for (int64_t i = 0; i < did && i < batch; i++) {
                 float ms = 0.f;
                 AXG_HIP(hipEventElapsedTime(&ms, ctx->timing_events[2 * i],
                                             ctx->timing_events[2 * i + 1]));
                 res.spmv_ms += ms;
+                res.spmv_launches++;
             }
-            res.spmv_launches = h->iters;
         } else {
             AXG_HIP(hipGraphLaunch(gexec, s));
         }
@@ -457,6 +656,8 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
     res.residual = h->residual;
     res.bnorm = h->bnorm;
     res.status = h->status;
+    res.spmv_shape_L = sell ? 1 : kShapes[shape].L;
+    res.spmv_shape_U = sell ? 4 : kShapes[shape].U;
     if (res.status == AXG_E_NOT_CONVERGED && h->residual <= 10.0 * tol * h->bnorm)
         res.status = AXG_OK;  // linalg.rs:253: only > 10 tol ||b|| is an error
     if (res.status == AXG_OK && !h->done && h->bnorm >= 1e-30) res.status = AXG_E_NOT_CONVERGED;

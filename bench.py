@@ -81,6 +81,29 @@ def cpu_baseline(n_sample=40, pcg_iters=60, asm_n=28):
     }
 
 
+def kernels_only(a, G, mesh, prob, n, nnz):
+    """Repeated launches of the dominant kernels (input for rocprofv3 runs)."""
+    out = {"workload": prob.name, "n_free": n, "nnz": nnz, "reps": a.reps}
+    nstream = 1 << 27  # 1 GiB per array
+    out["stream_copy_gbs"] = G.bench_stream(nstream, a.reps, ctx=mesh.ctx)
+    log(f"stream copy {out['stream_copy_gbs']:.0f} GB/s")
+    mesh.assemble("mixed")
+    t = mesh.assembly_times()
+    out["assembly_ms"] = t
+    nb, nnzb = n // 3, nnz // 9
+    for which, name, nbytes in ((0, "fp64_sell_spmv", 12 * nnz + 16 * n),
+                                (1, "fp64_csr_spmv", 12 * nnz + 4 * (n + 1) + 16 * n)):
+        ms = mesh.bench_operator(which, 1, a.reps)
+        out[name] = {"ms": ms, "gbs": nbytes / (ms / 1e3) / 1e9}
+        log(f"{name}: {ms:.4f} ms {out[name]['gbs']:.0f} GB/s")
+    for k in (1, 4, 8):
+        ms = mesh.bench_operator(2, k, a.reps)
+        nbytes = 40 * nnzb + 2 * 4 * 3 * nb * k
+        out[f"fp32_bsell_spmm_k{k}"] = {"ms": ms, "gbs": nbytes / (ms / 1e3) / 1e9}
+        log(f"fp32 spmm k={k}: {ms:.4f} ms {out[f'fp32_bsell_spmm_k{k}']['gbs']:.0f} GB/s")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +116,9 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kernels", action="store_true",
+                    help="only the operator kernels (repeated launches), for profiling")
+    ap.add_argument("--reps", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -115,6 +141,9 @@ def main():
     n, nnz = info["n_free"], info["nnz"]
     log(f"mesh {prob.name}: n={n} nnz={nnz} contributions={info['ncontrib']} "
         f"symbolic {info['symbolic_s']:.2f}s setup {time.perf_counter() - t0:.1f}s")
+    if a.kernels:
+        kernels_only(a, G, mesh, prob, n, nnz)
+        return
     B = torch.tensor(prob.rhs_all(), dtype=torch.float64, device=dev).contiguous()
     X = torch.zeros_like(B)
     Xref = torch.zeros_like(B)

@@ -175,6 +175,14 @@ int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *meshes,
                     const double *const *rhs, const int32_t *nrhs, const axg_solve_opts *opts,
                     double *const *x, int64_t *iters, double *solve_ms, axg_solve_info *info);
 
+/* ------------------------------------------------------ measurement */
+/* STREAM-style copy of n doubles, reps launches: achieved GB/s (read+write). */
+int axg_bench_stream(axg_ctx *ctx, int64_t n, int32_t reps, double *copy_gbs);
+/* Average device time of one launch of a solver operator kernel on an
+ * assembled mesh: which = 0 FP64 SpMV (solver layout), 1 FP64 CSR SpMV,
+ * 2 FP32 block SpMM with nrhs right-hand sides (1, 2, 4 or 8). */
+int axg_bench_operator(axg_mesh *mesh, int32_t which, int32_t nrhs, int32_t reps, double *avg_ms);
+
 /* ------------------------------------------------- host-only helpers */
 /* The symbolic phase alone, on the host (no device): K_ff pattern and scatter
  * map as built by axg_mesh_create, before any exact-zero slot is dropped.
