@@ -218,6 +218,15 @@ def main():
     nnzb = nnz // 9  # every free node keeps all 3 DOFs in this workload
     spmm_bytes = 36 * nnzb + 4 * nnzb + 4 * (nb + 1) + 2 * 4 * 3 * nb * k
     achieved = spmm_bytes / (spmv_ms / 1e3) / 1e9
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary of this workload (profiles/hbm_traffic_brick80.json), when it applies
+    traffic = None
+    try:
+        prof = json.load(open(os.path.join(ROOT, "profiles", "hbm_traffic_brick80.json")))
+        if a.n == 80:
+            traffic = prof["bytes_per_launch"].get(f"axg::bsell_spmm_kernel<{k}, 1, false>")
+    except (OSError, ValueError, KeyError):
+        traffic = None
     flops_el = 8 * (72 * 24 + 14 * 24 * 24)
     ke32_tflops = flops_el * nelem / (asm["ke32"] / 1e3) / 1e12
     out = {
@@ -256,9 +265,9 @@ def main():
         },
         "speedup_mixed_vs_fp64_time_to_solution": (fp64["solve_ms"] / solve_ms) if fp64 else None,
         "roofline": {
-            "bound": "hbm", "kernel": f"bsr_spmm_dot_kernel<K={k}>",
+            "bound": "hbm", "kernel": f"bsell_spmm_kernel<K={k}> (FP32 block-sliced-ELL SpMM)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "bytes_per_launch": spmm_bytes, "avg_launch_ms": spmv_ms,
         },
         "reference_fp64_iters": ref["iters"],
