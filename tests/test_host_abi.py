@@ -1,0 +1,133 @@
+"""Host-side checks of the C ABI library (no GPU needed).
+
+The library must load and export every entry point include/axia_gpu.h
+declares; the host-only entry points (symbolic phase, face-pressure loads)
+are compared with the oracle here.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import axia_gpu as G
+from axia_gpu import _lib
+from axia_gpu import meshgen as MG
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "axia_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(axg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), f"{s} not exported"
+        assert s in _lib.SIGNATURES, f"{s} has no ctypes signature"
+    assert L.axg_version().decode().startswith("axia_gpu")
+
+
+def test_default_opts_match_reference_pcg():
+    import ctypes as C
+    o = _lib.SolveOpts()
+    _lib.lib().axg_default_opts(C.byref(o))
+    assert o.tol == 1e-8 and o.max_iter == 0 and o.precision == 0 and o.precond == 0
+
+
+def oracle_csr(p):
+    ti, tj, tv = O.assemble_triplets(p.nn, p.xyz, p.conn, p.e, p.nu)
+    sel = (p.ind_of[ti] >= 0) & (p.ind_of[tj] >= 0)
+    return O.csr_from_triplets(p.n_free, p.ind_of[ti[sel]], p.ind_of[tj[sel]], tv[sel])
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: MG.cantilever(5, 2, 2),
+    lambda: MG.brick_lognormal(3, seed=4),
+    lambda: MG.brick_c3d20_jitter(3, 2, 2, seed=1),
+])
+def test_symbolic_pattern_and_scatter_map_vs_oracle(maker):
+    """Pattern bit-exact, and oracle Ke summed through the map gives the oracle's K_ff bits."""
+    p = maker()
+    csr = oracle_csr(p)
+    rp, ci, sp, cb = G.pattern_host(p.nn, p.nnode, p.conn, p.ind_of)
+    assert np.array_equal(rp, csr.indptr)
+    assert np.array_equal(ci, csr.indices)
+    kes = np.stack([O.hex_stiffness(p.xyz[p.conn[e]], p.e[e], p.nu[e])[0]
+                    for e in range(p.nelem)]).ravel()
+    vals = np.array([sum(kes[cb[sp[k]:sp[k + 1]]].tolist(), 0.0) for k in range(len(ci))])
+    assert np.array_equal(vals, csr.data)
+    # contributions of one slot come in ascending element order
+    m2 = (3 * p.nn) ** 2
+    for k in range(0, len(ci), max(1, len(ci) // 200)):
+        els = cb[sp[k]:sp[k + 1]] // m2
+        assert np.all(np.diff(els) > 0)
+
+
+def test_symbolic_partial_constraints():
+    """Nodes with only some DOFs prescribed (LE10-style YS,2,2) keep their other rows."""
+    xyz, conn = MG.brick_hex8(3, 2, 2)
+    pres = np.zeros(3 * len(xyz), bool)
+    pres[3 * np.nonzero(xyz[:, 1] == 0)[0] + 1] = True
+    pres[3 * np.nonzero(xyz[:, 0] == 0)[0] + 0] = True
+    pres[[2, 5]] = True
+    ind = np.full(3 * len(xyz), -1, np.int64)
+    ind[~pres] = np.arange(int((~pres).sum()))
+    p = MG.Problem("partial", 8, xyz, conn, np.full(len(conn), 1e5), np.full(len(conn), 0.3), ind)
+    csr = oracle_csr(p)
+    rp, ci, sp, cb = G.pattern_host(8, len(xyz), conn, ind)
+    assert np.array_equal(rp, csr.indptr) and np.array_equal(ci, csr.indices)
+
+
+def test_symbolic_rejects_bad_meshes():
+    xyz, conn = MG.brick_hex8(1, 1, 1)
+    bad = conn.copy()
+    bad[0, 1] = bad[0, 0]
+    with pytest.raises(G.AxgError) as e:
+        G.pattern_host(8, len(xyz), bad, MG.free_map(len(xyz), [0]))
+    assert "repeats a node" in str(e.value)
+    bad = conn.copy()
+    bad[0, 3] = 99
+    with pytest.raises(G.AxgError):
+        G.pattern_host(8, len(xyz), bad, MG.free_map(len(xyz), [0]))
+
+
+@pytest.mark.parametrize("nn", [8, 20])
+def test_pressure_loads_bitexact_vs_oracle(nn):
+    rng = np.random.default_rng(nn)
+    xyz, conn = MG.brick_hex8(2, 2, 1) if nn == 8 else MG.brick_hex20(2, 1, 1)
+    xyz = xyz + rng.uniform(-0.1, 0.1, xyz.shape)
+    faces = rng.integers(1, 7, len(conn)).astype(np.int32)
+    mags = rng.uniform(-2, 2, len(conn))
+    f = np.zeros(3 * len(xyz))
+    G.pressure_loads(nn, xyz, conn, faces, mags, f)
+    ref = np.zeros(3 * len(xyz))
+    for e in range(len(conn)):
+        fe = O.hex_face_pressure(xyz[conn[e]], faces[e], mags[e])
+        for a in range(nn):
+            for d in range(3):
+                ref[3 * conn[e][a] + d] += fe[3 * a + d]
+    assert np.array_equal(f, ref)
+    with pytest.raises(G.AxgError):
+        G.pressure_loads(nn, xyz, conn[:1], np.array([7], np.int32), np.array([1.0]), f)
+
+
+def test_meshgen_counts_match_survey_configs():
+    """Exact node / element / DOF counts of the benchmark shapes."""
+    xyz, conn = MG.brick_hex20(96, 24, 24)
+    assert len(xyz) == 237025 and len(conn) == 55296
+    ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 0] == 0)[0])
+    assert int((ind >= 0).sum()) == 705600
+    xyz, conn = MG.brick_hex8(80, 80, 80)
+    assert len(xyz) == 531441
+    ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 2] == 0)[0])
+    assert int((ind >= 0).sum()) == 1574640
+    p = MG.cantilever(20, 9, 9)
+    assert p.nnode == 2100 and p.n_free == 6000
+    assert abs(p.f_full.__getitem__(0).sum() + 1000.0) < 1e-9
