@@ -242,6 +242,33 @@ class Mesh:
         return c.value, dc
 
 
+def solve_batch(meshes, rhs_list, tol=1e-8, max_iter=0, ctx=None):
+    """Solve all meshes' load cases in one batched launch sequence.
+
+    meshes: assembled Mesh objects on one context; rhs_list[m]: (k_m, n_free_m)
+    array (k_m <= 8). Returns (xs, iters, latency_ms, info): xs[m] has the
+    shape of rhs_list[m]; iters and latency_ms list every (mesh, load case)
+    solve in order, latency measured from the start of the batch to that
+    solve's convergence on the device clock.
+    """
+    ctx = ctx or meshes[0].ctx
+    nm = len(meshes)
+    bs = [f64(np.atleast_2d(b)) for b in rhs_list]
+    xs = [np.zeros_like(b) for b in bs]
+    nrhs = np.array([b.shape[0] for b in bs], np.int32)
+    H = (C.c_void_p * nm)(*[m._h for m in meshes])
+    R = (_lib._dp * nm)(*[dptr(b) for b in bs])
+    X = (_lib._dp * nm)(*[dptr(x) for x in xs])
+    total = int(nrhs.sum())
+    iters = np.zeros(total, np.int64)
+    lat = np.zeros(total)
+    info = _lib.SolveInfo()
+    o = _opts(PREC_FP64, PRECOND_JACOBI, tol, max_iter)
+    check(_lib.lib().axg_solve_batch(ctx.handle, nm, H, R, iptr(nrhs), C.byref(o), X, lptr(iters),
+                                     dptr(lat), C.byref(info)))
+    return xs, iters, lat, _info_dict(info)
+
+
 def bench_stream(n=1 << 27, reps=10, ctx=None):
     """STREAM-style copy bandwidth (GB/s, read + write) of n doubles."""
     ctx = ctx or default_context()

@@ -28,7 +28,7 @@ COMMON = [
 EXACT = {"axg_element.hip", "axg_triplets.hip", "axg_loads.cpp"}
 SOURCES = [
     "axg_api.cpp", "axg_symbolic.cpp", "axg_loads.cpp", "axg_element.hip", "axg_triplets.hip",
-    "axg_pcg.hip", "axg_mixed.hip", "axg_element32.hip", "axg_bench.hip",
+    "axg_pcg.hip", "axg_mixed.hip", "axg_element32.hip", "axg_bench.hip", "axg_batch.hip",
 ]
 
 

@@ -1,0 +1,537 @@
+// Batched solves: many meshes, several right-hand sides each, advanced
+// together by one sequence of launches per CG iteration (captured once into a
+// hipGraph and replayed).
+//
+// The assembled FP64 sliced-ELL operators of all meshes are concatenated into
+// one block-diagonal operator. Each mesh is a segment whose rows start on a
+// 256-row boundary, so every workgroup works on exactly one segment and its
+// partial sums belong to one mesh; the finalize kernels run one workgroup per
+// segment. Scalars (alpha, beta, r.z, ||r||, done) are kept per (segment,
+// right-hand side): every solve follows the reference's Jacobi PCG
+// (linalg.rs:194-259) on its own, converging at its own iteration, and its
+// convergence time is stamped from the device's constant-rate clock.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "axg_internal.h"
+#include "axg_mixed.h"
+#include "axg_pcg.h"
+#include "axg_reduce.h"
+
+namespace axg {
+
+constexpr int TB = 256;
+
+struct SegScalars {
+    double rz, alpha, beta, res, tol_abs, bnorm;
+    int64_t iters;
+    int64_t t_done;   // s_memrealtime stamp at convergence
+    int32_t done, status;
+};
+
+struct BatchDev {
+    const int64_t *sptr;    // per slice
+    const int32_t *width;   // per slice
+    const int32_t *col;
+    const double *val;
+    const double *dsel;     // Jacobi divisor per row (1 on padding rows)
+    const int32_t *wg_seg;  // segment of each workgroup
+    const int64_t *wg_row;  // first row of each workgroup
+    const int64_t *seg_row; // seg_row[m + 1]: end of the real rows of segment m
+    const int32_t *seg_wg;  // first workgroup per segment (nseg + 1)
+    int32_t *seg_live;      // right-hand sides still iterating per segment
+};
+
+__device__ inline int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
+
+__global__ void stamp_kernel(int64_t *t) { *t = now_ticks(); }
+
+template <int K, int U, bool DOT>
+__global__ void __launch_bounds__(TB) batch_spmv_kernel(BatchDev B, const double *__restrict__ P,
+                                                       double *__restrict__ AP,
+                                                       double *__restrict__ part) {
+    const int seg = B.wg_seg[blockIdx.x];
+    if (DOT && B.seg_live[seg] == 0) return;
+    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
+    double dot[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) dot[j] = 0.0;
+    if (row < B.seg_row[seg + 1]) {
+        const int64_t sl = row >> 6, base = B.sptr[sl] + (row & 63);
+        const int w = B.width[sl];
+        double acc[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) acc[j] = 0.0;
+        int jj = 0;
+        for (; jj + U <= w; jj += U) {
+            int32_t c[U];
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                c[u] = __builtin_nontemporal_load(&B.col[base + 64 * (int64_t)(jj + u)]);
+                v[u] = __builtin_nontemporal_load(&B.val[base + 64 * (int64_t)(jj + u)]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int j = 0; j < K; j++) acc[j] += v[u] * P[(int64_t)c[u] * K + j];
+        }
+        for (; jj < w; jj++) {
+            const int64_t p = base + 64 * (int64_t)jj;
+            const double v = B.val[p];
+            const int64_t c = B.col[p];
+#pragma unroll
+            for (int j = 0; j < K; j++) acc[j] += v * P[c * K + j];
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            AP[row * K + j] = acc[j];
+            if (DOT) dot[j] += P[row * K + j] * acc[j];
+        }
+    }
+    if (!DOT) return;
+    group_sums<K, TB>(dot);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * K + j] = dot[j];
+}
+
+// One workgroup per segment: sums of its workgroups' partials (fixed order).
+template <int NC>
+__device__ inline void seg_sums(const BatchDev &B, const double *part, int seg, double (&out)[NC]) {
+    const int w0 = B.seg_wg[seg], w1 = B.seg_wg[seg + 1];
+    sum_partials<NC, TB>(part + (size_t)w0 * NC, w1 - w0, out);
+}
+
+template <int K>
+__global__ void __launch_bounds__(TB) batch_fin_pap_kernel(BatchDev B, const double *part,
+                                                          SegScalars *sc, int32_t *live_total) {
+    const int seg = blockIdx.x;
+    if (B.seg_live[seg] == 0) return;
+    double pap[K];
+    seg_sums<K>(B, part, seg, pap);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < K; j++) {
+            SegScalars &s = sc[seg * K + j];
+            if (s.done) continue;
+            if (fabs(pap[j]) < 1e-30) {  // linalg.rs:231-233
+                s.status = AXG_E_NOT_SPD;
+                s.done = 1;
+                s.t_done = now_ticks();
+                B.seg_live[seg]--;
+                atomicSub(live_total, 1);
+            } else {
+                s.alpha = s.rz / pap[j];
+            }
+        }
+}
+
+// x += alpha p, r -= alpha Ap, z = r / d, ||r||^2, r.z (per right-hand side).
+template <int K>
+__global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__restrict__ X,
+                                                         double *__restrict__ R,
+                                                         const double *__restrict__ P,
+                                                         const double *__restrict__ AP,
+                                                         double *__restrict__ Z,
+                                                         double *__restrict__ part,
+                                                         const SegScalars *sc) {
+    const int seg = B.wg_seg[blockIdx.x];
+    if (B.seg_live[seg] == 0) return;
+    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
+    double acc[2 * K];
+#pragma unroll
+    for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
+    if (row < B.seg_row[seg + 1]) {
+        const double d = B.dsel[row];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const SegScalars &s = sc[seg * K + j];
+            const int64_t i = row * K + j;
+            double r = R[i];
+            if (!s.done) {
+                X[i] += s.alpha * P[i];
+                r -= s.alpha * AP[i];
+                R[i] = r;
+            }
+            const double z = r / d;
+            Z[i] = z;
+            acc[j] += r * r;
+            acc[K + j] += r * z;
+        }
+    }
+    group_sums<2 * K, TB>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
+}
+
+template <int K>
+__global__ void __launch_bounds__(TB) batch_fin_update_kernel(BatchDev B, const double *part,
+                                                             SegScalars *sc, int init,
+                                                             double tol, int64_t max_iter,
+                                                             int32_t *live_total) {
+    const int seg = blockIdx.x;
+    if (!init && B.seg_live[seg] == 0) return;
+    double tot[2 * K];
+    seg_sums<2 * K>(B, part, seg, tot);
+    if (threadIdx.x != 0) return;
+    int finished = 0;
+    for (int j = 0; j < K; j++) {
+        SegScalars &s = sc[seg * K + j];
+        if (init) {
+            const double bn = sqrt(tot[j]);
+            s.bnorm = bn;
+            s.res = bn;
+            s.tol_abs = tol * bn;
+            s.rz = tot[K + j];
+            if (bn < 1e-30) {  // zero right-hand side: x = 0 (linalg.rs:206-214)
+                s.done = 1;
+                s.t_done = now_ticks();
+                finished++;
+            }
+            continue;
+        }
+        if (s.done) continue;
+        const double res = sqrt(tot[j]);
+        s.res = res;
+        s.iters += 1;
+        if (res <= s.tol_abs || s.iters >= max_iter) {
+            s.done = 1;
+            if (res > s.tol_abs && res > 10.0 * s.tol_abs) s.status = AXG_E_NOT_CONVERGED;
+            s.t_done = now_ticks();
+            finished++;
+        } else {
+            s.beta = tot[K + j] / s.rz;
+            s.rz = tot[K + j];
+        }
+    }
+    if (finished) {
+        B.seg_live[seg] -= finished;
+        atomicSub(live_total, finished);
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(TB) batch_update_p_kernel(BatchDev B, double *__restrict__ P,
+                                                           const double *__restrict__ Z,
+                                                           const SegScalars *sc, int init) {
+    const int seg = B.wg_seg[blockIdx.x];
+    if (!init && B.seg_live[seg] == 0) return;
+    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
+    if (row >= B.seg_row[seg + 1]) return;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const SegScalars &s = sc[seg * K + j];
+        const int64_t i = row * K + j;
+        if (init)
+            P[i] = Z[i];
+        else if (!s.done)
+            P[i] = Z[i] + s.beta * P[i];
+    }
+}
+
+// x = 0, r = b, z = r / d and the sums for ||b|| and r.z.
+template <int K>
+__global__ void __launch_bounds__(TB) batch_init_kernel(BatchDev B, const double *__restrict__ Bv,
+                                                       double *__restrict__ X,
+                                                       double *__restrict__ R,
+                                                       double *__restrict__ Z,
+                                                       double *__restrict__ part) {
+    const int seg = B.wg_seg[blockIdx.x];
+    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
+    double acc[2 * K];
+#pragma unroll
+    for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
+    if (row < B.seg_row[seg + 1]) {
+        const double d = B.dsel[row];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int64_t i = row * K + j;
+            const double b = Bv[i];
+            X[i] = 0.0;
+            R[i] = b;
+            const double z = b / d;
+            Z[i] = z;
+            acc[j] += b * b;
+            acc[K + j] += b * z;
+        }
+    }
+    group_sums<2 * K, TB>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
+}
+
+// Rebase the column indices of one mesh's sliced-ELL copy into the batch.
+__global__ void rebase_cols_kernel(int64_t nslots, const int32_t *__restrict__ src,
+                                   int32_t *__restrict__ dst, int64_t row_off) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nslots;
+         p += (int64_t)gridDim.x * blockDim.x)
+        dst[p] = (int32_t)(src[p] + row_off);
+}
+
+__global__ void fill_kernel(int64_t n, double *__restrict__ a, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+template <int K>
+static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const double *const *rhs,
+                      const int32_t *nrhs, const axg_solve_opts &o, double *const *x,
+                      int64_t *iters_out, double *ms_out, axg_solve_info *info) {
+    hipStream_t s = ctx->stream;
+    // layout: segment m rows [row0, row0 + n_m), padded to 256 rows
+    std::vector<int64_t> row0(nmesh + 1, 0), slot0(nmesh + 1, 0), slice0(nmesh + 1, 0);
+    std::vector<int32_t> seg_wg(nmesh + 1, 0);
+    int64_t max_n = 0;
+    for (int m = 0; m < nmesh; m++) {
+        axg_mesh *me = meshes[m];
+        ensure_sell(me);
+        const int64_t n = me->n_free, npad = ((n + TB - 1) / TB) * TB;
+        max_n = std::max(max_n, n);
+        row0[m + 1] = row0[m] + npad;
+        // the mesh's own slices, then empty slices up to the padded row count
+        slice0[m + 1] = slice0[m] + npad / 64;
+        slot0[m + 1] = slot0[m] + me->sell_nslots;
+        seg_wg[m + 1] = seg_wg[m] + (int32_t)(npad / TB);
+    }
+    const int64_t nrow = row0[nmesh], nslice = slice0[nmesh], nslot = slot0[nmesh];
+    const int nwg = seg_wg[nmesh];
+    AXG_REQUIRE(nrow < INT32_MAX && nslot < INT32_MAX, AXG_E_INVALID,
+                "batch exceeds 2^31 rows or slots");
+    DevBuf<int64_t> sptr(nslice + 1), wg_row(nwg);
+    DevBuf<int32_t> width(nslice), col(std::max<int64_t>(nslot, 1)), wg_seg(nwg),
+        segwg(nmesh + 1), seg_live(nmesh);
+    DevBuf<double> val(std::max<int64_t>(nslot, 1)), dsel(nrow);
+    std::vector<int64_t> h_sptr(nslice + 1, 0), h_wg_row(nwg);
+    std::vector<int32_t> h_width(nslice, 0), h_wg_seg(nwg), h_live(nmesh, 0);
+    for (int m = 0; m < nmesh; m++) {
+        axg_mesh *me = meshes[m];
+        const int64_t ns = (me->n_free + 63) / 64;
+        std::vector<int64_t> sp(ns + 1);
+        std::vector<int32_t> wd(ns);
+        AXG_HIP(hipMemcpy(sp.data(), me->sell_sptr.p, 8 * (ns + 1), hipMemcpyDeviceToHost));
+        AXG_HIP(hipMemcpy(wd.data(), me->sell_width.p, 4 * ns, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < slice0[m + 1] - slice0[m]; k++) {
+            int64_t g = slice0[m] + k;
+            h_width[g] = k < ns ? wd[k] : 0;
+            h_sptr[g] = slot0[m] + (k < ns ? sp[k] : sp[ns]);
+        }
+        for (int32_t wgi = seg_wg[m]; wgi < seg_wg[m + 1]; wgi++) {
+            h_wg_seg[wgi] = m;
+            h_wg_row[wgi] = row0[m] + (int64_t)(wgi - seg_wg[m]) * TB;
+        }
+        h_live[m] = K;
+        // values, columns and the Jacobi divisor of this mesh
+        if (me->sell_nslots) {
+            AXG_HIP(hipMemcpyAsync(val.p + slot0[m], me->sell_val.p, 8 * me->sell_nslots,
+                                   hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(rebase_cols_kernel, dim3(1024), dim3(256), 0, s, me->sell_nslots,
+                               me->sell_col.p, col.p + slot0[m], row0[m]);
+        }
+        hipLaunchKernelGGL(fill_kernel, dim3(64), dim3(256), 0, s, row0[m + 1] - row0[m],
+                           dsel.p + row0[m], 1.0);
+        AXG_HIP(hipMemcpyAsync(dsel.p + row0[m], me->dsel.p, 8 * me->n_free,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    h_sptr[nslice] = nslot;
+    // seg_end[m + 1] = end of segment m's real rows (workgroups bound the start)
+    DevBuf<int64_t> seg_end(nmesh + 1);
+    std::vector<int64_t> h_end(nmesh + 1);
+    for (int m = 0; m < nmesh; m++) h_end[m + 1] = row0[m] + meshes[m]->n_free;
+    h_end[0] = 0;
+    AXG_HIP(hipMemcpyAsync(sptr.p, h_sptr.data(), 8 * (nslice + 1), hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(width.p, h_width.data(), 4 * nslice, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(wg_seg.p, h_wg_seg.data(), 4 * nwg, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(wg_row.p, h_wg_row.data(), 8 * nwg, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(seg_end.p, h_end.data(), 8 * (nmesh + 1), hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(segwg.p, seg_wg.data(), 4 * (nmesh + 1), hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(seg_live.p, h_live.data(), 4 * nmesh, hipMemcpyHostToDevice, s));
+    BatchDev B;
+    B.sptr = sptr.p;
+    B.width = width.p;
+    B.col = col.p;
+    B.val = val.p;
+    B.dsel = dsel.p;
+    B.wg_seg = wg_seg.p;
+    B.wg_row = wg_row.p;
+    B.seg_row = seg_end.p;
+    B.seg_wg = segwg.p;
+    B.seg_live = seg_live.p;
+    // right-hand sides: row-major [row][K], zero on padding rows and missing columns
+    std::vector<double> hb((size_t)nrow * K, 0.0);
+    for (int m = 0; m < nmesh; m++) {
+        const int64_t n = meshes[m]->n_free;
+        for (int j = 0; j < std::min(K, (int)nrhs[m]); j++)
+            for (int64_t i = 0; i < n; i++) hb[(row0[m] + i) * K + j] = rhs[m][j * n + i];
+    }
+    DevBuf<double> Bv(nrow * K), X(nrow * K), R(nrow * K), P(nrow * K), AP(nrow * K),
+        Z(nrow * K), part((size_t)nwg * 2 * K + 8);
+    DevBuf<SegScalars> sc(nmesh * K);
+    DevBuf<int32_t> live_total(1);
+    AXG_HIP(hipMemcpyAsync(Bv.p, hb.data(), 8 * hb.size(), hipMemcpyHostToDevice, s));
+    std::vector<SegScalars> hs(nmesh * K);
+    std::memset(hs.data(), 0, sizeof(SegScalars) * hs.size());
+    AXG_HIP(hipMemcpyAsync(sc.p, hs.data(), sizeof(SegScalars) * hs.size(), hipMemcpyHostToDevice,
+                           s));
+    int32_t live0 = nmesh * K;
+    AXG_HIP(hipMemcpyAsync(live_total.p, &live0, 4, hipMemcpyHostToDevice, s));
+    const int64_t max_iter = o.max_iter > 0 ? o.max_iter : std::max<int64_t>(4 * max_n, 200);
+    // autotune loads in flight (U) on this operator
+    int U = 2;
+    {
+        float best = 1e30f;
+        for (int cand : {1, 2, 4}) {
+            AXG_HIP(hipEventRecord(ctx->ev0, s));
+            for (int r = 0; r < 3; r++) {
+                if (cand == 1)
+                    hipLaunchKernelGGL((batch_spmv_kernel<K, 1, false>), dim3(nwg), dim3(TB), 0, s, B,
+                                       Bv.p, AP.p, part.p);
+                else if (cand == 2)
+                    hipLaunchKernelGGL((batch_spmv_kernel<K, 2, false>), dim3(nwg), dim3(TB), 0, s, B,
+                                       Bv.p, AP.p, part.p);
+                else
+                    hipLaunchKernelGGL((batch_spmv_kernel<K, 4, false>), dim3(nwg), dim3(TB), 0, s, B,
+                                       Bv.p, AP.p, part.p);
+            }
+            AXG_HIP(hipEventRecord(ctx->ev1, s));
+            AXG_HIP(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) best = ms, U = cand;
+        }
+    }
+    auto spmv_dot = [&]() {
+        if (U == 1)
+            hipLaunchKernelGGL((batch_spmv_kernel<K, 1, true>), dim3(nwg), dim3(TB), 0, s, B, P.p,
+                               AP.p, part.p);
+        else if (U == 2)
+            hipLaunchKernelGGL((batch_spmv_kernel<K, 2, true>), dim3(nwg), dim3(TB), 0, s, B, P.p,
+                               AP.p, part.p);
+        else
+            hipLaunchKernelGGL((batch_spmv_kernel<K, 4, true>), dim3(nwg), dim3(TB), 0, s, B, P.p,
+                               AP.p, part.p);
+    };
+    auto one_iter = [&]() {
+        spmv_dot();
+        hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p, sc.p,
+                           live_total.p);
+        hipLaunchKernelGGL((batch_update_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, X.p, R.p, P.p,
+                           AP.p, Z.p, part.p, sc.p);
+        hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p,
+                           sc.p, 0, o.tol, max_iter, live_total.p);
+        hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p,
+                           0);
+    };
+    const int batch = 32;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < batch; i++) one_iter();
+    AXG_HIP(hipStreamEndCapture(s, &graph));
+    AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    AXG_HIP(hipGraphDestroy(graph));
+
+    AXG_HIP(hipStreamSynchronize(s));
+    DevBuf<int64_t> t0(1);
+    AXG_HIP(hipEventRecord(ctx->ev0, s));
+    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, t0.p);  // start of the solves
+    hipLaunchKernelGGL((batch_init_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p, R.p, Z.p,
+                       part.p);
+    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p, sc.p,
+                       1, o.tol, max_iter, live_total.p);
+    hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p, 1);
+    AXG_HIP(hipGetLastError());
+    int32_t live = live0;
+    int64_t launched = 0;
+    for (;;) {
+        AXG_HIP(hipMemcpyAsync(&live, live_total.p, 4, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        if (live <= 0 || launched >= max_iter) break;
+        AXG_HIP(hipGraphLaunch(gexec, s));
+        launched += batch;
+    }
+    AXG_HIP(hipEventRecord(ctx->ev1, s));
+    AXG_HIP(hipEventSynchronize(ctx->ev1));
+    AXG_HIP(hipGraphExecDestroy(gexec));
+    float total_ms = 0.f;
+    AXG_HIP(hipEventElapsedTime(&total_ms, ctx->ev0, ctx->ev1));
+    int64_t h_t0 = 0;
+    AXG_HIP(hipMemcpy(&h_t0, t0.p, 8, hipMemcpyDeviceToHost));
+    AXG_HIP(hipMemcpy(hs.data(), sc.p, sizeof(SegScalars) * hs.size(), hipMemcpyDeviceToHost));
+    std::vector<double> hx((size_t)nrow * K);
+    AXG_HIP(hipMemcpy(hx.data(), X.p, 8 * hx.size(), hipMemcpyDeviceToHost));
+    int status = AXG_OK;
+    int64_t total_iters = 0, q = 0;
+    double worst = 0.0, worst_b = 1.0;
+    for (int m = 0; m < nmesh; m++) {
+        const int64_t n = meshes[m]->n_free;
+        for (int j = 0; j < nrhs[m]; j++, q++) {
+            if (j >= K) {  // more right-hand sides than this call handles
+                status = AXG_E_INVALID;
+                continue;
+            }
+            const SegScalars &r = hs[m * K + j];
+            for (int64_t i = 0; i < n; i++) x[m][j * n + i] = hx[(row0[m] + i) * K + j];
+            if (iters_out) iters_out[q] = r.iters;
+            // s_memrealtime runs at 100 MHz
+            if (ms_out) ms_out[q] = r.done ? (double)(r.t_done - h_t0) / 1e5 : (double)total_ms;
+            total_iters += r.iters;
+            if (!r.done) status = AXG_E_NOT_CONVERGED;
+            else if (r.status != AXG_OK && status == AXG_OK) status = r.status;
+            if (r.bnorm > 0 && r.res / r.bnorm > worst / worst_b) worst = r.res, worst_b = r.bnorm;
+        }
+    }
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iters = total_iters;
+        info->outer_iters = 1;
+        info->nrhs = K;
+        info->residual = worst;
+        info->rel_residual = worst / worst_b;
+        info->solve_ms = total_ms;
+        std::snprintf(info->name, sizeof(info->name), "GPU batched PCG (FP64, Jacobi, %d meshes)",
+                      nmesh);
+    }
+    if (status != AXG_OK)
+        throw Error{status, status == AXG_E_NOT_SPD
+                                ? "CG: unerwartetes Null-Pivot — Matrix nicht positiv definit."
+                                : "CG konvergierte nicht (batched solve)."};
+}
+
+}  // namespace axg
+
+using namespace axg;
+
+extern "C" int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *meshes,
+                               const double *const *rhs, const int32_t *nrhs,
+                               const axg_solve_opts *opts, double *const *x, int64_t *iters,
+                               double *solve_ms, axg_solve_info *info) {
+    try {
+        clear_error();
+        AXG_REQUIRE(ctx && nmesh > 0 && meshes && rhs && nrhs && x, AXG_E_INVALID,
+                    "axg_solve_batch: invalid argument");
+        axg_solve_opts o;
+        axg_default_opts(&o);
+        if (opts) o = *opts;
+        AXG_HIP(hipSetDevice(ctx->device));
+        int kmax = 0;
+        for (int m = 0; m < nmesh; m++) {
+            AXG_REQUIRE(meshes[m] && meshes[m]->ctx == ctx && meshes[m]->assembled64,
+                        AXG_E_INVALID, "axg_solve_batch: every mesh must be assembled on ctx");
+            AXG_REQUIRE(nrhs[m] >= 1 && nrhs[m] <= 8 && rhs[m] && x[m], AXG_E_INVALID,
+                        "axg_solve_batch: 1 to 8 right-hand sides per mesh");
+            kmax = std::max(kmax, (int)nrhs[m]);
+        }
+        if (kmax == 1) run_batch<1>(ctx, nmesh, meshes, rhs, nrhs, o, x, iters, solve_ms, info);
+        else if (kmax == 2) run_batch<2>(ctx, nmesh, meshes, rhs, nrhs, o, x, iters, solve_ms, info);
+        else if (kmax <= 4) run_batch<4>(ctx, nmesh, meshes, rhs, nrhs, o, x, iters, solve_ms, info);
+        else run_batch<8>(ctx, nmesh, meshes, rhs, nrhs, o, x, iters, solve_ms, info);
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    return AXG_OK;
+}

@@ -969,13 +969,3 @@ void solve_mixed(axg_mesh *m, const double *d_rhs, int32_t nrhs, const axg_solve
 }
 
 }  // namespace axg
-
-extern "C" int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *meshes,
-                               const double *const *rhs, const int32_t *nrhs,
-                               const axg_solve_opts *opts, double *const *x, int64_t *iters,
-                               double *solve_ms, axg_solve_info *info) {
-    (void)ctx, (void)nmesh, (void)meshes, (void)rhs, (void)nrhs, (void)opts, (void)x, (void)iters;
-    (void)solve_ms, (void)info;
-    axg::set_error("batched solve not built yet");
-    return AXG_E_INVALID;
-}

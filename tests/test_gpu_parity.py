@@ -298,3 +298,28 @@ def test_mixed_multi_rhs_matches_fp64(ctx):
     assert info["nrhs"] == 4
     for k in range(4):
         assert rel(X[k], X64[k]) < 1e-9
+
+
+# ------------------------------------------------------------------ batched
+def test_batched_multi_mesh_matches_single_solves(ctx):
+    """Uneven meshes, several load cases each, one batched launch sequence; every
+    solve matches the single-mesh FP64 PCG and the reference's stopping rule."""
+    say("case: batch")
+    probs = [MG.batch_meshes(count=1, lo=m, hi=m, nloads=k, seed=m)[0]
+             for m, k in ((4, 4), (7, 2), (5, 3), (9, 1))]
+    meshes = []
+    for p in probs:
+        me = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+        me.assemble("fp64")
+        meshes.append(me)
+    rhs = [p.rhs_all() for p in probs]
+    xs, iters, lat, info = G.solve_batch(meshes, rhs, tol=1e-12)
+    q = 0
+    for p, me, B, X in zip(probs, meshes, rhs, xs):
+        for k in range(B.shape[0]):
+            x1, inf1 = me.solve(B[k], tol=1e-12)
+            assert rel(X[k], x1) < 1e-9
+            assert abs(int(iters[q]) - inf1["iters"]) <= 2
+            assert 0.0 < lat[q] <= info["solve_ms"] * 1.05 + 1.0
+            q += 1
+    assert info["rel_residual"] <= 1e-12 * 10
