@@ -164,14 +164,20 @@ def brick_lognormal(n=80, seed=0, median=210000.0, sigma=0.5, nu=0.3, pressure=1
 
 
 def brick_c3d20_jitter(nx=96, ny=24, nz=24, seed=0, e=70000.0, nu=0.3, jitter=0.15):
-    """C3D20 brick with interior nodes jittered by U(-j, j) h; x = 0 clamped; tip shear 1 N/node in z."""
+    """C3D20 brick, interior corner nodes jittered by U(-j, j) h per coordinate and every
+    midside node at the midpoint of its (straight) edge, so boundary faces stay planar
+    and no element folds; x = 0 clamped; tip shear 1 N/node in z."""
     xyz, conn = brick_hex20(nx, ny, nz)
     rng = np.random.default_rng(seed)
-    bnd = ((xyz[:, 0] == 0) | (xyz[:, 0] == nx) | (xyz[:, 1] == 0) | (xyz[:, 1] == ny) |
-           (xyz[:, 2] == 0) | (xyz[:, 2] == nz))
-    inner = np.nonzero(~bnd)[0]
-    xyz = xyz.copy()
-    xyz[inner] += rng.uniform(-jitter, jitter, size=(len(inner), 3))
+    d = rng.uniform(-jitter, jitter, size=(nx + 1, ny + 1, nz + 1, 3))
+    d[0], d[nx] = 0.0, 0.0
+    d[:, 0], d[:, ny] = 0.0, 0.0
+    d[:, :, 0], d[:, :, nz] = 0.0, 0.0
+    lo = np.floor(xyz).astype(np.int64)
+    hi = np.ceil(xyz).astype(np.int64)
+    # a corner node has lo == hi; a midside node differs in exactly one axis
+    disp = 0.5 * (d[lo[:, 0], lo[:, 1], lo[:, 2]] + d[hi[:, 0], hi[:, 1], hi[:, 2]])
+    xyz = xyz + disp
     ind = free_map(len(xyz), np.nonzero(xyz[:, 0] == 0.0)[0])
     tip = np.nonzero(xyz[:, 0] == float(nx))[0]
     f = np.zeros(3 * len(xyz))

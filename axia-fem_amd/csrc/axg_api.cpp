@@ -141,6 +141,8 @@ void axg_default_opts(axg_solve_opts *o) {
     o->timing = 0;
 }
 
+static void ctx_free(axg_ctx *ctx);
+
 int axg_create(int device, axg_ctx **out) {
     API_BEGIN
     AXG_REQUIRE(out, AXG_E_INVALID, "axg_create: out is NULL");
@@ -165,15 +167,14 @@ int axg_create(int device, axg_ctx **out) {
         c->ws.err.ensure(4);
         c->ws.cnt.ensure(4);
     } catch (...) {
-        axg_destroy(c);
+        ctx_free(c);
         throw;
     }
     *out = c;
     API_END
 }
 
-int axg_destroy(axg_ctx *ctx) {
-    if (!ctx) return AXG_OK;
+static void ctx_free(axg_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->timing_events) (void)hipEventDestroy(e);
@@ -183,6 +184,15 @@ int axg_destroy(axg_ctx *ctx) {
     ctx->ws = Workspace();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+static void ctx_release(axg_ctx *ctx) {
+    if (ctx->refs.fetch_sub(1) == 1) ctx_free(ctx);
+}
+
+int axg_destroy(axg_ctx *ctx) {
+    if (!ctx) return AXG_OK;
+    ctx_release(ctx);
     return AXG_OK;
 }
 
@@ -392,15 +402,18 @@ int axg_mesh_create(axg_ctx *ctx, int32_t nn, int64_t nnode, const double *xyz, 
     AXG_HIP(hipStreamSynchronize(s));
     mesh->symbolic_s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ctx->refs.fetch_add(1);
     *out = mesh.release();
     API_END
 }
 
 int axg_mesh_destroy(axg_mesh *mesh) {
     if (!mesh) return AXG_OK;
-    (void)hipSetDevice(mesh->ctx->device);
-    (void)hipStreamSynchronize(mesh->ctx->stream);
+    axg_ctx *ctx = mesh->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     delete mesh;
+    ctx_release(ctx);
     return AXG_OK;
 }
 

@@ -326,13 +326,23 @@ __global__ void gather_fp64_kernel(const uint32_t *__restrict__ slotptr,
                                    uint32_t *__restrict__ zero_count) {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslot;
          s += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t c0 = slotptr[s], c1 = slotptr[s + 1];
+        const uint32_t c0 = slotptr[s], c1 = slotptr[s + 1];
         double v = 0.0;
         bool nz = false;
-        for (uint32_t c = c0; c < c1; c++) {
-            double kv = ke[contrib[c]];
-            v += kv;
-            nz |= (kv != 0.0);
+        // up to 8 contributions in flight, summed in their (ascending element) order
+        for (uint32_t c = c0; c < c1; c += 8) {
+            uint32_t idx[8];
+            double kv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) idx[u] = c + u < c1 ? contrib[c + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < 8; u++) kv[u] = c + u < c1 ? ke[idx[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (c + u < c1) {
+                    v += kv[u];
+                    nz |= (kv[u] != 0.0);
+                }
         }
         val[s] = v;
         if (slot_nz) slot_nz[s] = nz ? 1 : 0;

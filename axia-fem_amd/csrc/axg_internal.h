@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -118,6 +119,9 @@ struct Workspace {
 }  // namespace axg
 
 struct axg_ctx {
+    // one reference held by the creator (axg_destroy drops it) and one per live
+    // mesh: the context outlives every mesh made on it, whatever the call order
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr;

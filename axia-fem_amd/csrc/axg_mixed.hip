@@ -500,7 +500,8 @@ __global__ void __launch_bounds__(MBS) inner_update_p_kernel(int64_t nv, float *
         load_k<K>(P + i * K, p);
         load_k<K>(Z + i * K, z);
 #pragma unroll
-        for (int j = 0; j < K; j++) p[j] = keep[j] ? z[j] + beta[j] * p[j] : p[j];
+        // the first direction is z itself: P may hold stale bits (0 * NaN = NaN)
+        for (int j = 0; j < K; j++) p[j] = init ? z[j] : keep[j] ? z[j] + beta[j] * p[j] : p[j];
         store_k<K>(P + i * K, p);
     }
 }

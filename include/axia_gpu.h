@@ -82,6 +82,8 @@ const char *axg_last_error(void);
 const char *axg_version(void);
 void axg_default_opts(axg_solve_opts *opts);
 int axg_create(int device, axg_ctx **out);
+/* Drops the caller's reference; the context (stream, workspace) is freed once
+ * every mesh created on it has been destroyed as well, in either order. */
 int axg_destroy(axg_ctx *ctx);
 /* The context's HIP stream (hipStream_t), for callers that enqueue around it. */
 void *axg_stream(axg_ctx *ctx);

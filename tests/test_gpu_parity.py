@@ -247,16 +247,17 @@ def test_brick_properties_40(ctx):
 
 
 # ------------------------------------------------------------ mixed precision
-@pytest.mark.parametrize("nn,seed", [(8, 4), (20, 5)])
-def test_mixed_ke32_mfma_vs_fp64(ctx, nn, seed):
-    """FP32 MFMA integration agrees with the FP64 element matrices to FP32 accuracy."""
-    say("case: Ke32", nn, seed)
+@pytest.mark.parametrize("nn,seed,offset", [(8, 4, 0.0), (20, 5, 0.0), (8, 6, 1e5), (20, 7, 1e5)])
+def test_mixed_ke32_mfma_vs_fp64(ctx, nn, seed, offset):
+    """FP32 MFMA integration agrees with the FP64 element matrices to FP32 accuracy,
+    also far from the origin (coordinates are taken relative to each element's first node)."""
+    say("case: Ke32", nn, seed, offset)
     if nn == 8:
         xyz, conn = MG.brick_hex8(3, 3, 2)
     else:
         xyz, conn = MG.brick_hex20(2, 2, 2)
     rng = np.random.default_rng(seed)
-    xyz = xyz + rng.uniform(-0.1, 0.1, xyz.shape)
+    xyz = xyz + rng.uniform(-0.1, 0.1, xyz.shape) + offset
     ne = len(conn)
     e = rng.uniform(1e3, 3e5, ne)
     mesh = G.Mesh(nn, xyz, conn, e, 0.3, MG.free_map(len(xyz), [0, 1]), ctx=ctx)
@@ -298,6 +299,30 @@ def test_mixed_multi_rhs_matches_fp64(ctx):
     assert info["nrhs"] == 4
     for k in range(4):
         assert rel(X[k], X64[k]) < 1e-9
+
+
+def test_mixed_after_fp64_multi_rhs_on_fresh_context():
+    """A multi-RHS FP64 solve before the first mixed solve of a context leaves stale
+    FP64 bits where the FP32 search directions are allocated; the inner PCG must not
+    read them (its first direction is z itself)."""
+    say("case: mixed after fp64 multi-rhs, fresh context")
+    c = G.Context(0)
+    p = MG.brick_lognormal(12, seed=3, nloads=4)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=c)
+    mesh.assemble("mixed")
+    B = p.rhs_all()
+    X64, _ = mesh.solve(B, precision="fp64", tol=1e-13)
+    for k in range(4):
+        x, info = mesh.solve(B[k], precision="mixed", precond="block3", tol=1e-12)
+        assert rel(x, X64[k]) < 1e-9
+    X, _ = mesh.solve(B, precision="mixed", precond="jacobi", tol=1e-12)
+    for k in range(4):
+        assert rel(X[k], X64[k]) < 1e-9
+    # the context may be released before its meshes: the mesh keeps it alive
+    c.close()
+    x, _ = mesh.solve(B[0], precision="fp64", tol=1e-13)
+    assert rel(x, X64[0]) < 1e-12
+    mesh.close()
 
 
 # ------------------------------------------------------------------ batched
