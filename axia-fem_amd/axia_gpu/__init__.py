@@ -20,13 +20,13 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (AxgError, PREC_FP64, PREC_MIXED, PRECOND_BLOCK3, PRECOND_JACOBI, check, dptr,
-                   f64, i32, i64, iptr, lptr)
+from ._lib import (AxgError, PREC_FP64, PREC_MIXED, PRECOND_BLOCK3, PRECOND_JACOBI, REFINE_FCG,
+                   REFINE_PLAIN, check, dptr, f64, i32, i64, iptr, lptr)
 
 __all__ = [
     "AxgError", "Context", "Mesh", "SparseResult", "solve_kff", "csr_from_triplets",
     "pattern_host", "pressure_loads", "default_context", "PREC_FP64", "PREC_MIXED",
-    "PRECOND_JACOBI", "PRECOND_BLOCK3",
+    "PRECOND_JACOBI", "PRECOND_BLOCK3", "REFINE_FCG", "REFINE_PLAIN",
 ]
 
 
@@ -40,7 +40,7 @@ class SparseResult:
 
 
 def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inner_tol=1e-4,
-          max_outer=30, timing=False):
+          max_outer=30, timing=False, refinement="fcg"):
     o = _lib.SolveOpts()
     _lib.lib().axg_default_opts(C.byref(o))
     o.precision = {"fp64": PREC_FP64, "mixed": PREC_MIXED}.get(precision, precision)
@@ -50,6 +50,7 @@ def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inn
     o.inner_tol = inner_tol
     o.max_outer = max_outer
     o.timing = 1 if timing else 0
+    o.refinement = {"fcg": REFINE_FCG, "plain": REFINE_PLAIN}.get(refinement, refinement)
     return o
 
 
@@ -196,7 +197,7 @@ class Mesh:
         return out.reshape(count, m, m)
 
     def solve(self, rhs, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
-              inner_tol=1e-4, max_outer=30, timing=False):
+              inner_tol=1e-4, max_outer=30, timing=False, refinement="fcg"):
         """Solve K_ff x = rhs; rhs (n_free,) or (nrhs, n_free). Returns (x, info)."""
         b = f64(rhs)
         single = b.ndim == 1
@@ -204,16 +205,16 @@ class Mesh:
         nrhs = b2.shape[0]
         x = np.zeros_like(b2)
         info = _lib.SolveInfo()
-        o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing)
+        o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing, refinement)
         check(_lib.lib().axg_mesh_solve(self._h, dptr(b2), nrhs, C.byref(o), dptr(x),
                                         C.byref(info)))
         return (x[0] if single else x), _info_dict(info)
 
     def solve_device(self, rhs_ptr, nrhs, x_ptr, precision="fp64", precond="jacobi", tol=1e-8,
-                     max_iter=0, inner_tol=1e-4, max_outer=30, timing=False):
+                     max_iter=0, inner_tol=1e-4, max_outer=30, timing=False, refinement="fcg"):
         """Device-pointer solve (inputs already in HBM); returns info."""
         info = _lib.SolveInfo()
-        o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing)
+        o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing, refinement)
         check(_lib.lib().axg_mesh_solve_device(self._h, C.c_void_p(rhs_ptr), nrhs, C.byref(o),
                                                C.c_void_p(x_ptr), C.byref(info)))
         return _info_dict(info)

@@ -29,6 +29,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PRECOND_JACOBI = 0
 PRECOND_BLOCK3 = 1
+REFINE_FCG = 0
+REFINE_PLAIN = 1
 
 
 class SolveOpts(C.Structure):
@@ -40,6 +42,8 @@ class SolveOpts(C.Structure):
         ("inner_tol", C.c_double),
         ("max_outer", C.c_int32),
         ("timing", C.c_int32),
+        ("refinement", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -139,6 +139,8 @@ void axg_default_opts(axg_solve_opts *o) {
     o->inner_tol = 1e-4;
     o->max_outer = 30;
     o->timing = 0;
+    o->refinement = AXG_REFINE_FCG;
+    o->reserved = 0;
 }
 
 static void ctx_free(axg_ctx *ctx);
@@ -632,7 +634,8 @@ static void mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
             throw Error{mr.status, pcg_error(r)};
         }
         char name[64];
-        std::snprintf(name, sizeof name, "GPU PCG (FP32 BSR-3 + FP64 IR, %s)",
+        std::snprintf(name, sizeof name, "GPU PCG (FP32 BSR-3 + FP64 %s, %s)",
+                      o.refinement == AXG_REFINE_PLAIN ? "IR" : "FCG",
                       o.precond == AXG_PRECOND_BLOCK3 ? "block-Jacobi" : "Jacobi");
         fill_info(info, name, mr.inner_iters, mr.outer_iters, mr.residual, mr.bnorm, mr.solve_ms,
                   mr.spmv_ms, mr.spmv_launches, nrhs);

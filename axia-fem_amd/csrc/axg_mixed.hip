@@ -265,7 +265,6 @@ struct InnerScalars {
     int32_t iters[8];
     int32_t all_done;
     int32_t max_iter;
-    uint32_t ticket[4];
 };
 
 // ---------------------------------------- block-sliced-ELL SpMM (FP32)
@@ -286,8 +285,8 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
     double dot[K];
 #pragma unroll
     for (int j = 0; j < K; j++) dot[j] = 0.0;
-    for (int64_t I = (int64_t)blockIdx.x * MBS + threadIdx.x; I < nb;
-         I += (int64_t)gridDim.x * MBS) {
+    const RowBand rb = row_band(nb, MBS);
+    for (int64_t I = rb.first; I < rb.end; I += rb.step) {
         const int64_t sl = I >> 6, base = sptr[sl] + (I & 63);
         const int w = width[sl];
         float acc[3][K];
@@ -343,7 +342,7 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
     group_sums<K, MBS>(dot);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * K + j] = dot[j];
+        for (int j = 0; j < K; j++) part[(size_t)rb.id * K + j] = dot[j];
 }
 
 // ------------------------------------------------------ finalize kernels
@@ -414,16 +413,6 @@ __global__ void __launch_bounds__(MBS) fin_inner_init_kernel(const double *__res
         }
         sc->all_done = all;
     }
-}
-
-template <int K>
-__global__ void __launch_bounds__(MBS) fin_norms_kernel(const double *__restrict__ part, int nblk,
-                                                       double *__restrict__ out) {
-    double t[K];
-    sum_partials<K, MBS>(part, nblk, t);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) out[j] = sqrt(t[j]);
 }
 
 // x += alpha p, r -= alpha Ap, z = M r (3x3 block or diagonal), ||r||^2, r.z;
@@ -548,56 +537,6 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
     (void)sc, (void)inner_tol;
 }
 
-// ------------------------------------------------- FP64 residual pieces
-// Y[:, j] = F[:, j] - A X[:, j] for K right-hand sides (column-major n x K),
-// matrix streamed once. G lanes per row.
-template <int K, int G>
-__global__ void __launch_bounds__(MBS) csr_residual_kernel(
-    int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
-    const double *__restrict__ val, const double *__restrict__ X, const double *__restrict__ F,
-    double *__restrict__ Y) {
-    const int lane = threadIdx.x % G;
-    const int64_t group = ((int64_t)blockIdx.x * MBS + threadIdx.x) / G;
-    const int64_t ngroups = (int64_t)gridDim.x * MBS / G;
-    for (int64_t row = group; row < n; row += ngroups) {
-        double acc[K];
-#pragma unroll
-        for (int j = 0; j < K; j++) acc[j] = 0.0;
-        for (int32_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += G) {
-            double v = __builtin_nontemporal_load(&val[k]);
-            int64_t c = __builtin_nontemporal_load(&colind[k]);
-#pragma unroll
-            for (int j = 0; j < K; j++) acc[j] += v * X[c + j * n];
-        }
-#pragma unroll
-        for (int j = 0; j < K; j++)
-#pragma unroll
-            for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, G);
-        if (lane == 0)
-#pragma unroll
-            for (int j = 0; j < K; j++) Y[row + j * n] = F[row + j * n] - acc[j];
-    }
-}
-
-// Column norms of an n x K column-major FP64 array (deterministic).
-template <int K>
-__global__ void __launch_bounds__(MBS) col_norms_kernel(int64_t n, const double *__restrict__ Y,
-                                                       double *__restrict__ part,
-                                                       double *__restrict__ out,
-                                                       uint32_t *ticket) {
-    double s[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) s[j] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * MBS)
-#pragma unroll
-        for (int j = 0; j < K; j++) s[j] += Y[i + j * n] * Y[i + j * n];
-    group_sums<K, MBS>(s);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * K + j] = s[j];
-    (void)out, (void)ticket;
-}
-
 // FP64 free-space residual -> FP32 block space, scaled by 1/|r_j|.
 template <int K>
 __global__ void pack_kernel(int64_t n, const double *__restrict__ Y,
@@ -674,15 +613,15 @@ static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
     return best;
 }
 
-// Y[:, j] = F[:, j] - A X[:, j] on the sliced-ELL FP64 operator (one thread
-// per row, K right-hand sides, column-major n x K).
-template <int K>
-__global__ void __launch_bounds__(MBS) sell_residual_kernel(
+// Y[:, j] = F[:, j] - A X[:, j] (SUB) or A X[:, j] on the sliced-ELL FP64
+// operator (one thread per row, K right-hand sides, column-major n x K).
+template <int K, bool SUB>
+__global__ void __launch_bounds__(MBS) sell_apply_kernel(
     int64_t n, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
     const int32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ X,
     const double *__restrict__ F, double *__restrict__ Y) {
-    for (int64_t row = (int64_t)blockIdx.x * MBS + threadIdx.x; row < n;
-         row += (int64_t)gridDim.x * MBS) {
+    const RowBand rb = row_band(n, MBS);
+    for (int64_t row = rb.first; row < rb.end; row += rb.step) {
         const int64_t sl = row >> 6, base = sptr[sl] + (row & 63);
         const int w = width[sl];
         double acc[K];
@@ -696,7 +635,62 @@ __global__ void __launch_bounds__(MBS) sell_residual_kernel(
             for (int j = 0; j < K; j++) acc[j] += v * X[c + j * n];
         }
 #pragma unroll
-        for (int j = 0; j < K; j++) Y[row + j * n] = F[row + j * n] - acc[j];
+        for (int j = 0; j < K; j++) Y[row + j * n] = SUB ? F[row + j * n] - acc[j] : acc[j];
+    }
+}
+
+// Per-column a[:, j] . b[:, j], one partial per workgroup (summed on the host
+// in workgroup order).
+template <int K>
+__global__ void __launch_bounds__(MBS) col_dot_kernel(int64_t n, const double *__restrict__ a,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ part) {
+    double s[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) s[j] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * MBS)
+#pragma unroll
+        for (int j = 0; j < K; j++) s[j] += a[i + j * n] * b[i + j * n];
+    group_sums<K, MBS>(s);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < K; j++) part[(size_t)blockIdx.x * K + j] = s[j];
+}
+
+// Z[:, j] = scale_j D[:, j]: the FP32 correction back in FP64 free-DOF space.
+template <int K>
+__global__ void unpack_kernel(int64_t n, const float *__restrict__ D,
+                              const int32_t *__restrict__ blk_of_free,
+                              const double *__restrict__ scale, double *__restrict__ Z) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t bi = blk_of_free[i];
+#pragma unroll
+    for (int j = 0; j < K; j++) Z[i + j * n] = scale[j] * (double)D[bi * K + j];
+}
+
+// Outer direction: P = Z (first) or Z + beta_j P.
+template <int K>
+__global__ void fcg_dir_kernel(int64_t n, const double *__restrict__ Z, double *__restrict__ P,
+                               const double *__restrict__ beta, int first) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        P[i + j * n] = first ? Z[i + j * n] : Z[i + j * n] + beta[j] * P[i + j * n];
+}
+
+// Outer step: U += alpha_j P, R -= alpha_j Q.
+template <int K>
+__global__ void fcg_step_kernel(int64_t n, double *__restrict__ U, double *__restrict__ R,
+                                const double *__restrict__ P, const double *__restrict__ Q,
+                                const double *__restrict__ alpha) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        U[i + j * n] += alpha[j] * P[i + j * n];
+        R[i + j * n] -= alpha[j] * Q[i + j * n];
     }
 }
 
@@ -760,7 +754,7 @@ struct MixedSolver {
         w.f4.ensure(nv * K);  // AP
         DevBuf<float> Z(nv * K);
         w.t0.ensure(n * K);   // r64
-        w.t1.ensure(16);      // norms / scales
+        w.t1.ensure(32);      // scales, alpha, beta
         const int shp = tune_spmm<K>(m, w.f3.p, w.f4.p);
         const int grid_mv = spmm_grid(ctx, nb);
         const int grid_v = std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
@@ -769,12 +763,10 @@ struct MixedSolver {
                                                               (int64_t)ctx->cu_count * 8));
         w.part.ensure((size_t)std::max({grid_mv, grid_v, grid_r}) * 2 * K + 8);
         DevBuf<InnerScalars> dsc(1);
-        DevBuf<uint32_t> ticket(1);
         ctx->pinned.ensure(sizeof(InnerScalars) + 256);
         InnerScalars *h = (InnerScalars *)ctx->pinned.p;
         double *hn = (double *)((char *)ctx->pinned.p + sizeof(InnerScalars));
         std::memset(h, 0, sizeof(InnerScalars));
-        AXG_HIP(hipMemcpyAsync(ticket.p, h, 4, hipMemcpyHostToDevice, s));
         AXG_HIP(hipMemsetAsync(w.f0.p, 0, sizeof(float) * nv * K, s));
         AXG_HIP(hipMemsetAsync(d_u, 0, sizeof(double) * n * K, s));
         AXG_HIP(hipEventRecord(ctx->ev0, s));
@@ -805,22 +797,40 @@ struct MixedSolver {
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
                                w.f3.p, Z.p, dsc.p, 0);
         };
-        auto residual = [&]() {
-            hipLaunchKernelGGL((sell_residual_kernel<K>), dim3(grid_r), dim3(MBS), 0, s, n,
-                               A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, d_u, d_f, w.t0.p);
-
-            hipLaunchKernelGGL((col_norms_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, n, w.t0.p,
-                               w.part.p, w.t1.p, ticket.p);
-            hipLaunchKernelGGL((fin_norms_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p, grid_v,
-                               w.t1.p);
-            AXG_HIP(hipMemcpyAsync(hn, w.t1.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+        const int grid_n = (int)((n + 255) / 256);
+        std::vector<double> hpart((size_t)grid_v * K);
+        // per-column dot products a . b (fixed-order sums of the workgroup partials)
+        auto dots = [&](const double *da, const double *db, double (&out)[K]) {
+            hipLaunchKernelGGL((col_dot_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, n, da, db,
+                               w.part.p);
+            AXG_HIP(hipMemcpyAsync(hpart.data(), w.part.p, sizeof(double) * grid_v * K,
+                                   hipMemcpyDeviceToHost, s));
             AXG_HIP(hipStreamSynchronize(s));
+            for (int j = 0; j < K; j++) out[j] = 0.0;
+            for (int g = 0; g < grid_v; g++)
+                for (int j = 0; j < K; j++) out[j] += hpart[(size_t)g * K + j];
+        };
+        // device scalars: t1[8..] scales, t1[16..] alpha, t1[24..] beta (pinned staging in hn)
+        auto put = [&](int slot, const double (&v)[K]) {
+            for (int j = 0; j < K; j++) hn[slot + j] = v[j];
+            AXG_HIP(hipMemcpyAsync(w.t1.p + slot, hn + slot, sizeof(double) * K,
+                                   hipMemcpyHostToDevice, s));
+        };
+        // true residual R = F - A U and its column norms
+        double rnorm[K];
+        auto residual = [&]() {
+            hipLaunchKernelGGL((sell_apply_kernel<K, true>), dim3(grid_r), dim3(MBS), 0, s, n,
+                               A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, d_u, d_f,
+                               w.t0.p);
+            double rr[K];
+            dots(w.t0.p, w.t0.p, rr);
+            for (int j = 0; j < K; j++) rnorm[j] = std::sqrt(rr[j]);
         };
 
         // |f_j| from the first residual (u = 0)
         residual();
-        double fnorm[K], rnorm[K];
-        for (int j = 0; j < K; j++) fnorm[j] = rnorm[j] = hn[j];
+        double fnorm[K];
+        for (int j = 0; j < K; j++) fnorm[j] = rnorm[j];
         const int inner_batch = nb > 100000 ? 16 : 64;
         const int64_t inner_cap = o.max_iter > 0 ? o.max_iter : std::max<int64_t>(4 * n, 200);
         int64_t total_inner = 0;
@@ -845,26 +855,11 @@ struct MixedSolver {
                 ctx->timing_events.push_back(e);
             }
         }
-        int status = AXG_OK;
-        for (;;) {
-            bool conv = true;
-            double scale[K];
-            for (int j = 0; j < K; j++) {
-                bool cj = rnorm[j] <= o.tol * fnorm[j] || fnorm[j] < 1e-30;
-                conv &= cj;
-                scale[j] = cj ? 0.0 : rnorm[j];
-            }
-            if (conv) break;
-            if (sweeps >= o.max_outer || total_inner >= inner_cap) {
-                status = AXG_E_NOT_CONVERGED;
-                break;
-            }
-            sweeps++;
-            std::memcpy(hn, scale, sizeof(double) * K);
-            AXG_HIP(hipMemcpyAsync(w.t1.p + 8, hn, sizeof(double) * K, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((pack_kernel<K>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                               n, w.t0.p, m->blk_of_free.p, w.t1.p + 8, w.f0.p);
-            // inner scalars
+        // FP32 inner PCG on A32 X = R / |R| (columns with scale 0 stay X = 0)
+        auto inner_solve = [&](const double (&scale)[K]) {
+            put(8, scale);
+            hipLaunchKernelGGL((pack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.t0.p,
+                               m->blk_of_free.p, w.t1.p + 8, w.f0.p);
             std::memset(h, 0, sizeof(InnerScalars));
             h->max_iter = (int32_t)std::min<int64_t>(inner_cap - total_inner, INT32_MAX);
             AXG_HIP(hipMemcpyAsync(dsc.p, h, sizeof(InnerScalars), hipMemcpyHostToDevice, s));
@@ -912,10 +907,100 @@ struct MixedSolver {
             int32_t mx = 0;
             for (int j = 0; j < K; j++) mx = std::max(mx, h->iters[j]);
             total_inner += mx;
-            hipLaunchKernelGGL((unpack_add_kernel<K>), dim3((unsigned)((n + 255) / 256)), dim3(256),
-                               0, s, n, w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u);
+        };
+        // the FP64 residual of a stiff problem has a round-off floor: stop once
+        // two true residuals in a row improved no column by half
+        double best_true[K];
+        int stalls = 0;
+        auto true_residual_check = [&]() {
             residual();
-            for (int j = 0; j < K; j++) rnorm[j] = hn[j];
+            bool gained = false;
+            for (int j = 0; j < K; j++) {
+                const bool open = !(rnorm[j] <= o.tol * fnorm[j] || fnorm[j] < 1e-30);
+                if (open && rnorm[j] < 0.5 * best_true[j]) gained = true;
+                best_true[j] = std::min(best_true[j], rnorm[j]);
+            }
+            stalls = gained ? 0 : stalls + 1;
+        };
+        for (int j = 0; j < K; j++) best_true[j] = rnorm[j];
+        auto converged = [&](double (&scale)[K]) {
+            bool conv = true;
+            for (int j = 0; j < K; j++) {
+                bool cj = rnorm[j] <= o.tol * fnorm[j] || fnorm[j] < 1e-30;
+                conv &= cj;
+                scale[j] = cj ? 0.0 : rnorm[j];
+            }
+            return conv;
+        };
+        int status = AXG_OK;
+        if (o.refinement == AXG_REFINE_PLAIN) {
+            // classical refinement: u += M32(r), r = f - K u in FP64
+            for (;;) {
+                double scale[K];
+                if (converged(scale)) break;
+                if (sweeps >= o.max_outer || total_inner >= inner_cap || stalls >= 2) {
+                    status = AXG_E_NOT_CONVERGED;
+                    break;
+                }
+                sweeps++;
+                inner_solve(scale);
+                hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
+                                   w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u);
+                true_residual_check();
+            }
+        } else {
+            // Krylov-accelerated refinement: flexible CG in FP64 (one previous
+            // direction kept) whose preconditioner is the FP32 inner solve.
+            // The recursive residual decides when to stop; the true FP64
+            // residual confirms, and a miss restarts from it.
+            DevBuf<double> Zd(n * K), Pd(n * K), Qd(n * K);
+            double pq_prev[K];
+            bool first = true;
+            for (;;) {
+                double scale[K];
+                if (converged(scale)) {
+                    true_residual_check();
+                    if (converged(scale)) break;
+                    first = true;  // recursive residual drifted: restart from the true one
+                }
+                if (sweeps >= o.max_outer || total_inner >= inner_cap || stalls >= 2) {
+                    residual();
+                    status = converged(scale) ? AXG_OK : AXG_E_NOT_CONVERGED;
+                    break;
+                }
+                sweeps++;
+                inner_solve(scale);
+                hipLaunchKernelGGL((unpack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.f1.p,
+                                   m->blk_of_free.p, w.t1.p + 8, Zd.p);
+                double beta[K];
+                if (first) {
+                    for (int j = 0; j < K; j++) beta[j] = 0.0;
+                } else {
+                    double zq[K];
+                    dots(Zd.p, Qd.p, zq);
+                    for (int j = 0; j < K; j++) beta[j] = pq_prev[j] > 0.0 ? -zq[j] / pq_prev[j] : 0.0;
+                }
+                put(24, beta);
+                hipLaunchKernelGGL((fcg_dir_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, Zd.p,
+                                   Pd.p, w.t1.p + 24, first ? 1 : 0);
+                first = false;
+                hipLaunchKernelGGL((sell_apply_kernel<K, false>), dim3(grid_r), dim3(MBS), 0, s, n,
+                                   A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, Pd.p,
+                                   nullptr, Qd.p);
+                double pq[K], pr[K], alpha[K];
+                dots(Pd.p, Qd.p, pq);
+                dots(Pd.p, w.t0.p, pr);
+                for (int j = 0; j < K; j++) {
+                    alpha[j] = pq[j] > 0.0 ? pr[j] / pq[j] : 0.0;
+                    pq_prev[j] = pq[j];
+                }
+                put(16, alpha);
+                hipLaunchKernelGGL((fcg_step_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, d_u,
+                                   w.t0.p, Pd.p, Qd.p, w.t1.p + 16);
+                double rr[K];
+                dots(w.t0.p, w.t0.p, rr);
+                for (int j = 0; j < K; j++) rnorm[j] = std::sqrt(rr[j]);
+            }
         }
         AXG_HIP(hipEventRecord(ctx->ev1, s));
         AXG_HIP(hipEventSynchronize(ctx->ev1));
@@ -923,6 +1008,12 @@ struct MixedSolver {
         float ms = 0.f;
         AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         res.solve_ms = ms;
+        if (status == AXG_E_NOT_CONVERGED) {
+            // linalg.rs:253: a final residual within 10 tol ||f|| is not an error
+            bool near = true;
+            for (int j = 0; j < K; j++) near &= rnorm[j] <= 10.0 * o.tol * fnorm[j];
+            if (near) status = AXG_OK;
+        }
         res.status = status;
         res.inner_iters = total_inner;
         res.outer_iters = sweeps;

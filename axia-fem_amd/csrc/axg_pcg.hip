@@ -126,15 +126,15 @@ __global__ void __launch_bounds__(BS) sell_spmv_dot_kernel(int64_t n, const int6
                                                           PcgScalars *sc) {
     if (sc->done) return;
     double dot[1] = {0.0};
-    for (int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x; row < n;
-         row += (int64_t)gridDim.x * BS) {
+    const RowBand rb = row_band(n, BS);
+    for (int64_t row = rb.first; row < rb.end; row += rb.step) {
         const int64_t sl = row >> 6;
         double s = sell_row<U>(col, val, p, sptr[sl] + (row & 63), width[sl]);
         ap[row] = s;
         dot[0] += p[row] * s;
     }
     group_sums<1, BS>(dot);
-    if (threadIdx.x == 0) part[blockIdx.x] = dot[0];
+    if (threadIdx.x == 0) part[rb.id] = dot[0];
 }
 
 template <int U>
@@ -144,8 +144,8 @@ __global__ void __launch_bounds__(BS) sell_spmv_kernel(int64_t n, const int64_t 
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ x,
                                                       double *__restrict__ y) {
-    for (int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x; row < n;
-         row += (int64_t)gridDim.x * BS) {
+    const RowBand rb = row_band(n, BS);
+    for (int64_t row = rb.first; row < rb.end; row += rb.step) {
         const int64_t sl = row >> 6;
         y[row] = sell_row<U>(col, val, x, sptr[sl] + (row & 63), width[sl]);
     }

@@ -12,6 +12,29 @@
 
 namespace axg {
 
+// Rows of a streaming operator kernel, by workgroup. Workgroups are dealt to
+// the 8 XCDs round-robin (workgroup b starts on XCD b % 8), each XCD with its
+// own L2. Renumbering them so that one XCD's workgroups hold consecutive
+// logical ids, and giving each logical id one contiguous run of row blocks,
+// hands every XCD one band of rows: the x gathers of a banded matrix then hit
+// that XCD's L2 instead of being fetched once per XCD. The logical id also
+// indexes the workgroup's reduction partial, so sums keep a fixed order.
+struct RowBand {
+    uint32_t id;          // logical workgroup id
+    int64_t first, end;   // rows [first, end)
+    int64_t step;
+};
+
+__device__ inline RowBand row_band(int64_t nrows, int bsz) {
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    RowBand r;
+    r.id = (g % 8 == 0) ? (b % 8) * (g / 8) + b / 8 : b;
+    r.first = (int64_t)r.id * bsz + threadIdx.x;
+    r.end = nrows;
+    r.step = (int64_t)g * bsz;
+    return r;
+}
+
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -51,14 +51,22 @@ enum axg_precond {
     AXG_PRECOND_BLOCK3 = 1   /* 3x3 node-block Jacobi (inverse formed in FP64)      */
 };
 
+/* Mixed precision: how the FP64 outer loop uses the FP32 inner solves. */
+enum axg_refinement {
+    AXG_REFINE_FCG = 0,   /* flexible CG in FP64, the inner solve as preconditioner  */
+    AXG_REFINE_PLAIN = 1  /* classical refinement u += M32(f - K u)                  */
+};
+
 typedef struct axg_solve_opts {
     int32_t precision;   /* enum axg_precision                                       */
     int32_t precond;     /* enum axg_precond                                         */
     double tol;          /* stop when ||f - K u||_2 <= tol * ||f||_2 (default 1e-8)  */
     int64_t max_iter;    /* total PCG iterations; <= 0 means max(4n, 200)            */
     double inner_tol;    /* mixed: relative residual drop per refinement (1e-4)      */
-    int32_t max_outer;   /* mixed: refinement sweeps (default 30)                    */
+    int32_t max_outer;   /* mixed: refinement sweeps / inner solves (default 30)     */
     int32_t timing;      /* 1: record per-kernel HIP events (SpMV durations in info) */
+    int32_t refinement;  /* mixed: enum axg_refinement (default FCG)                 */
+    int32_t reserved;    /* 0                                                        */
 } axg_solve_opts;
 
 typedef struct axg_solve_info {
@@ -68,7 +76,9 @@ typedef struct axg_solve_info {
     double residual;      /* final FP64 ||f - K u||_2 (max over right-hand sides)   */
     double rel_residual;  /* residual / ||f||_2                                     */
     double solve_ms;      /* device time of the solve (HIP events)                  */
-    double spmv_ms;       /* summed SpMV kernel time when opts.timing = 1           */
+    double spmv_ms;       /* summed SpMV kernel time when opts.timing = 1; batched
+                             solves: launches that did work x one launch timed on
+                             the same operator after the solve                     */
     int64_t spmv_launches;
     char name[64];        /* e.g. "GPU PCG (FP64, Jacobi)" — SparseResult.name      */
 } axg_solve_info;

@@ -6,6 +6,7 @@ same displacements bit-exact (von Mises within 1e-14), and within 1e-8 from
 the GPU displacements. Every test calls through the C ABI (axia_gpu).
 """
 import os
+import re
 import sys
 
 import numpy as np
@@ -299,6 +300,45 @@ def test_mixed_multi_rhs_matches_fp64(ctx):
     assert info["nrhs"] == 4
     for k in range(4):
         assert rel(X[k], X64[k]) < 1e-9
+
+
+@pytest.mark.parametrize("refinement", ["fcg", "plain"])
+def test_mixed_refinement_modes_match_fp64(ctx, refinement):
+    """Flexible-CG and classical refinement both reach the FP64 solution."""
+    say("case: refinement", refinement)
+    p = MG.brick_lognormal(10, seed=8, nloads=2)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble("mixed")
+    B = p.rhs_all()
+    X64, _ = mesh.solve(B, precision="fp64", precond="block3", tol=1e-13)
+    X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-12, refinement=refinement)
+    assert ("FCG" if refinement == "fcg" else "IR") in info["name"]
+    for k in range(2):
+        assert rel(X[k], X64[k]) < 1e-9
+    assert info["rel_residual"] <= 1e-12
+
+
+def test_mixed_c3d20_stiff_beam_floor():
+    """A slender jittered C3D20 beam: its FP64 residual has a round-off floor near 1e-10
+    relative. The mixed solve reaches 1e-9 (displacements within 1e-8 of FP64), and an
+    unreachable 1e-14 stops on stagnation after a few sweeps instead of running to the
+    iteration cap."""
+    say("case: c3d20 beam floor")
+    c = G.Context(0)
+    p = MG.brick_c3d20_jitter(32, 6, 6, seed=3)
+    mesh = G.Mesh(20, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=c)
+    mesh.assemble("mixed")
+    b = p.rhs()
+    x64, _ = mesh.solve(b, precision="fp64", precond="block3", tol=1e-13)
+    x, info = mesh.solve(b, precision="mixed", precond="block3", tol=1e-9)
+    assert rel(x, x64) < 1e-8
+    with pytest.raises(G.AxgError) as e:
+        mesh.solve(b, precision="mixed", precond="block3", tol=1e-14)
+    m = re.search(r"nach (\d+) Iterationen", str(e.value))
+    say(f"  stopped after {m.group(1) if m else '?'} inner iterations (n = {p.n_free})")
+    assert m and int(m.group(1)) < 4 * p.n_free
+    mesh.close()
+    c.close()
 
 
 def test_mixed_after_fp64_multi_rhs_on_fresh_context():
