@@ -233,6 +233,26 @@ class Mesh:
         check(_lib.lib().axg_nodal_stress(self._h, dptr(u), dptr(stress), dptr(vm), dptr(en)))
         return stress, vm, en
 
+    def reactions(self, u_full, f_full):
+        """rf = K_full u_full - f_full over all DOFs (analysis.rs:1027-1035)."""
+        rf = np.zeros(3 * self.nnode)
+        check(_lib.lib().axg_reactions(self._h, dptr(f64(u_full).ravel()),
+                                       dptr(f64(f_full).ravel()), dptr(rf)))
+        return rf
+
+    def reduce_rhs(self, f_full, u0):
+        """DofMap::reduce of the load vector with prescribed values u0 (constraint.rs:138-173)."""
+        rhs = np.zeros(max(self.n_free, 1))
+        check(_lib.lib().axg_reduce_rhs(self._h, dptr(f64(f_full).ravel()), dptr(f64(u0).ravel()),
+                                        dptr(rhs)))
+        return rhs[:self.n_free]
+
+    def full_diagonal(self):
+        """Diagonal of the full stiffness over all DOFs, summed in element order."""
+        d = np.zeros(3 * self.nnode)
+        check(_lib.lib().axg_full_diagonal(self._h, dptr(d)))
+        return d
+
     def compliance_adjoint(self, u_full, f_full):
         """c = f^T u and dc/dE_e = -u_e^T K1_e u_e (K1: Ke at unit modulus)."""
         u = f64(u_full).ravel()

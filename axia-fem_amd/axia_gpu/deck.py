@@ -1,0 +1,102 @@
+"""INP decks through the native reader and the device linear-static driver.
+
+Deck(text) parses the keyword subset of the reference's reader (inp.rs) and
+builds solve_linear's setup (analysis.rs:548-946) in libaxia_gpu; Deck.solve
+runs the rest of solve_linear on the device (assembly, reduced rhs, PCG,
+reconstruction, reactions, nodal stresses) and returns a LinearResult that
+mirrors SolveOutput (analysis.rs:24-54).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr, lptr
+
+
+@dataclass
+class LinearResult:
+    node_ids: np.ndarray    # (nnode,) ascending node ids
+    u_full: np.ndarray      # (3 nnode,) displacements
+    rf: np.ndarray          # (3 nnode,) reactions K u - f
+    stress: np.ndarray      # (nnode, 6) nodal mean stress, Voigt xx yy zz xy yz zx
+    von_mises: np.ndarray   # (nnode,)
+    solver: str
+    iters: int
+    residual: float
+    nfree: int
+    ndof: int
+
+    def node_index(self, nid):
+        i = int(np.searchsorted(self.node_ids, nid))
+        if i >= len(self.node_ids) or self.node_ids[i] != nid:
+            raise KeyError(nid)
+        return i
+
+
+class Deck:
+    """A parsed deck and its linear-static setup (host side of libaxia_gpu)."""
+
+    def __init__(self, text):
+        raw = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+        h = C.c_void_p()
+        check(_lib.lib().axg_deck_parse(raw, len(raw), C.byref(h)))
+        self._h = h
+        nn, nnode, nelem, nfree = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(_lib.lib().axg_deck_info(self._h, C.byref(nn), C.byref(nnode), C.byref(nelem),
+                                       C.byref(nfree)))
+        self.nn, self.nnode, self.nelem, self.n_free = (nn.value, nnode.value, nelem.value,
+                                                        nfree.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().axg_deck_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arrays(self):
+        """dict of node_ids, elem_ids, xyz, conn, e, nu, ind_of, u0, f_full."""
+        ndof = 3 * self.nnode
+        out = {
+            "node_ids": np.zeros(self.nnode, np.int32), "elem_ids": np.zeros(self.nelem, np.int32),
+            "xyz": np.zeros((self.nnode, 3)), "conn": np.zeros((self.nelem, self.nn), np.int32),
+            "e": np.zeros(self.nelem), "nu": np.zeros(self.nelem),
+            "ind_of": np.zeros(ndof, np.int64), "u0": np.zeros(ndof), "f_full": np.zeros(ndof),
+        }
+        check(_lib.lib().axg_deck_arrays(
+            self._h, iptr(out["node_ids"]), iptr(out["elem_ids"]), dptr(out["xyz"]),
+            iptr(out["conn"]), dptr(out["e"]), dptr(out["nu"]), lptr(out["ind_of"]),
+            dptr(out["u0"]), dptr(out["f_full"])))
+        return out
+
+    def solve(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
+              inner_tol=1e-4):
+        from . import _info_dict, _opts, default_context
+        ctx = ctx or default_context()
+        ndof = 3 * self.nnode
+        u = np.zeros(ndof)
+        rf = np.zeros(ndof)
+        stress = np.zeros((self.nnode, 6))
+        vm = np.zeros(self.nnode)
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol)
+        check(_lib.lib().axg_deck_solve(ctx.handle, self._h, C.byref(o), dptr(u), dptr(rf),
+                                        dptr(stress), dptr(vm), C.byref(info)))
+        inf = _info_dict(info)
+        ids = self.arrays()["node_ids"]
+        return LinearResult(ids, u, rf, stress, vm, inf["name"], inf["iters"], inf["residual"],
+                            self.n_free, ndof)
+
+
+def solve_deck(text, ctx=None, **kw):
+    """Parse and solve a deck: (Deck, LinearResult)."""
+    d = Deck(text)
+    return d, d.solve(ctx=ctx, **kw)

@@ -106,25 +106,6 @@ double true_residual(axg_ctx *ctx, const CsrView &A, const double *d_x, const do
 
 using namespace axg;
 
-#define API_BEGIN \
-    try {         \
-        clear_error();
-#define API_END                                   \
-    }                                             \
-    catch (const Error &e) {                      \
-        set_error(e.msg);                         \
-        return e.code;                            \
-    }                                             \
-    catch (const std::bad_alloc &) {             \
-        set_error("host allocation failed");      \
-        return AXG_E_OOM;                         \
-    }                                             \
-    catch (const std::exception &e) {             \
-        set_error(e.what());                      \
-        return AXG_E_INVALID;                     \
-    }                                             \
-    return AXG_OK;
-
 extern "C" {
 
 const char *axg_last_error(void) { return g_err.c_str(); }
@@ -753,6 +734,43 @@ int axg_compliance_adjoint(axg_mesh *mesh, const double *u_full, const double *f
         for (int64_t d = 0; d < mesh->ndof; d++) acc += f_full[d] * u_full[d];
         *c = acc;
     }
+    API_END
+}
+
+// Full-space products with the stored FP64 element matrices (last assembly).
+// mode 0: reactions (ndof out), 1: reduced rhs (n_free out), 2: diagonal (ndof out)
+static void row_apply(axg_mesh *mesh, int mode, const double *w, const double *f, double *out) {
+    AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    axg_ctx *ctx = mesh->ctx;
+    AXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int64_t ndof = mesh->ndof, nout = mode == 1 ? mesh->n_free : ndof;
+    DevBuf<double> dw(ndof), df(ndof), dout(std::max<int64_t>(nout, 1));
+    if (w) AXG_HIP(hipMemcpyAsync(dw.p, w, 8 * ndof, hipMemcpyHostToDevice, s));
+    if (f) AXG_HIP(hipMemcpyAsync(df.p, f, 8 * ndof, hipMemcpyHostToDevice, s));
+    launch_row_apply(mesh, s, mode, dw.p, df.p, dout.p);
+    if (nout) AXG_HIP(hipMemcpyAsync(out, dout.p, 8 * nout, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+}
+
+int axg_reactions(axg_mesh *mesh, const double *u_full, const double *f_full, double *rf) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && u_full && f_full && rf, AXG_E_INVALID, "axg_reactions: NULL argument");
+    row_apply(mesh, 0, u_full, f_full, rf);
+    API_END
+}
+
+int axg_reduce_rhs(axg_mesh *mesh, const double *f_full, const double *u0, double *rhs) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && f_full && u0 && rhs, AXG_E_INVALID, "axg_reduce_rhs: NULL argument");
+    row_apply(mesh, 1, u0, f_full, rhs);
+    API_END
+}
+
+int axg_full_diagonal(axg_mesh *mesh, double *diag) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && diag, AXG_E_INVALID, "axg_full_diagonal: NULL argument");
+    row_apply(mesh, 2, nullptr, nullptr, diag);
     API_END
 }
 

@@ -1,0 +1,803 @@
+// INP deck ingestion and the linear-static driver for all-C3D8 / all-C3D20
+// isotropic decks: the keyword subset of inp.rs (parse_with_base, :302-1061),
+// the model compaction of model.rs (:1234-1309), and solve_linear's setup,
+// reduction, solve, reconstruction, reactions and stresses (analysis.rs:548-1162).
+// The setup runs on the host (it is cheap and sequential by definition); every
+// per-element or per-DOF loop of the solve runs on the device through the
+// C ABI of this library. Compiled with -ffp-contract=off: the load vector is
+// summed exactly as the reference sums it.
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "axg_internal.h"
+
+using namespace axg;
+
+namespace {
+
+struct DeckElem {
+    int32_t id;
+    int nn;
+    std::vector<int32_t> nodes;  // node ids
+    std::string elset;
+};
+
+struct Deferred {
+    enum Kind { NESTED_NSET, NESTED_ELSET, BC, CL, DL } kind;
+    std::string name, target;
+    int dof = 0;  // BC / CL: 0-based dof; DL: face
+    double val = 0.0;
+};
+
+struct NodeLoad {
+    int32_t id;  // node id (BC, CLOAD) or element id (DLOAD)
+    int k;       // 0-based dof, or the face for DLOAD
+    double v;
+};
+
+std::string trim(const std::string &s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) a++;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) b--;
+    return s.substr(a, b - a);
+}
+
+std::string upper(std::string s) {
+    for (char &c : s) c = (char)std::toupper((unsigned char)c);
+    return s;
+}
+
+std::vector<std::string> split_tokens(const std::string &line) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (char c : line) {
+        if (c == ',' || std::isspace((unsigned char)c)) {
+            if (!cur.empty()) out.push_back(cur), cur.clear();
+        } else {
+            cur.push_back(c);
+        }
+    }
+    if (!cur.empty()) out.push_back(cur);
+    return out;
+}
+
+bool is_int_token(const std::string &s) {
+    std::string t = trim(s);
+    size_t i = 0;
+    if (i < t.size() && (t[i] == '+' || t[i] == '-')) i++;
+    if (i == t.size()) return false;
+    for (; i < t.size(); i++)
+        if (!std::isdigit((unsigned char)t[i])) return false;
+    return true;
+}
+
+bool full_strtod(const std::string &s, double &out) {
+    if (s.empty()) return false;
+    const char *p = s.c_str();
+    char *end = nullptr;
+    double v = std::strtod(p, &end);
+    if (end != p + s.size()) return false;
+    // hexadecimal and "inf"/"nan" spellings are not numbers in a deck
+    for (char c : s)
+        if (c == 'x' || c == 'X' || c == 'n' || c == 'N' || c == 'i' || c == 'I') return false;
+    out = v;
+    return true;
+}
+
+// inp.rs:100-141: Fortran reals (1.d0, 1.0D-3, 1., .5, 1.0+3).
+bool parse_f64(const std::string &tok, double &out) {
+    std::string raw = trim(tok);
+    if (raw.empty()) return false;
+    std::string stripped = raw;
+    while (!stripped.empty() && stripped.back() == '.') stripped.pop_back();
+    if (full_strtod(stripped, out) || full_strtod(raw, out)) return true;
+    std::string b = raw;
+    for (char &c : b)
+        if (c == 'd' || c == 'D') c = 'e';
+    while (!b.empty() && b.back() == '.') b.pop_back();
+    if (full_strtod(b, out)) return true;
+    for (size_t i = 1; i < b.size(); i++) {
+        if ((b[i] == '+' || b[i] == '-') &&
+            (std::isdigit((unsigned char)b[i - 1]) || b[i - 1] == '.')) {
+            return full_strtod(b.substr(0, i) + "e" + b.substr(i), out);
+        }
+    }
+    return false;
+}
+
+double f64_or0(const std::string &tok) {
+    double v = 0.0;
+    if (!parse_f64(tok, v)) return 0.0;
+    return v == 0.0 ? 0.0 : v;
+}
+
+bool parse_i32(const std::string &tok, int32_t &out) {
+    std::string t = trim(tok);
+    if (is_int_token(t)) {
+        out = (int32_t)std::strtoll(t.c_str(), nullptr, 10);
+        return true;
+    }
+    double v;
+    if (!parse_f64(t, v) || !std::isfinite(v)) return false;
+    out = (int32_t)std::llround(v);
+    return true;
+}
+
+int32_t need_i32(const std::string &tok) {
+    int32_t v;
+    AXG_REQUIRE(parse_i32(tok, v), AXG_E_INVALID, "Keine ganze Zahl: " + tok);
+    return v;
+}
+
+bool is_comment_or_blank(const std::string &raw) {
+    std::string t = trim(raw);
+    return t.empty() || t.compare(0, 2, "**") == 0;
+}
+
+bool is_keyword(const std::string &raw) {
+    std::string t = trim(raw);
+    return !t.empty() && t[0] == '*' && t.compare(0, 2, "**") != 0;
+}
+
+void parse_keyword(const std::string &raw, std::string &kw, std::map<std::string, std::string> &params) {
+    std::string t = trim(raw);
+    if (t.compare(0, 3, "\xEF\xBB\xBF") == 0) t = t.substr(3);
+    t = upper(t);
+    params.clear();
+    size_t start = 0;
+    bool first = true;
+    while (start <= t.size()) {
+        size_t comma = t.find(',', start);
+        std::string part = trim(t.substr(start, comma == std::string::npos ? std::string::npos
+                                                                           : comma - start));
+        if (first) {
+            kw = part;
+            first = false;
+        } else if (!part.empty()) {
+            size_t eq = part.find('=');
+            if (eq == std::string::npos)
+                params[part] = "";
+            else
+                params[trim(part.substr(0, eq))] = trim(part.substr(eq + 1));
+        }
+        if (comma == std::string::npos) break;
+        start = comma + 1;
+    }
+    if (kw == "*SOLIDSECTION") kw = "*SOLID SECTION";
+    if (kw == "*ENDSTEP") kw = "*END STEP";
+}
+
+}  // namespace
+
+struct axg_deck {
+    std::vector<int32_t> node_ids;
+    std::vector<double> coords;  // 3 per node
+    std::vector<DeckElem> elements;
+    std::map<std::string, std::vector<int32_t>> nsets, elsets;
+    std::map<std::string, std::pair<double, double>> materials;
+    std::map<std::string, std::string> elset_material;
+    std::vector<NodeLoad> bcs, cloads, dloads;
+    std::vector<Deferred> deferred;
+    std::unordered_map<int32_t, int64_t> id_to_index;
+    // linear-static setup (full DOF space)
+    int nn = 0;
+    int64_t nnode = 0, nelem = 0, ndof = 0, n_free = 0;
+    std::vector<double> xyz, e, nu, u0, f_full;
+    std::vector<int32_t> conn;
+    std::vector<int64_t> ind_of;
+    std::vector<uint8_t> prescribed;
+
+    std::vector<int32_t> expand_nset(const std::string &name) const {
+        if (is_int_token(name)) return {(int32_t)std::strtoll(trim(name).c_str(), nullptr, 10)};
+        const std::string up = upper(name);
+        auto it = nsets.find(up);
+        if (it != nsets.end()) return it->second;
+        if (up == "NALL" || up == "ALL") return node_ids;
+        throw Error{AXG_E_INVALID, "Unbekanntes Knotenset " + name};
+    }
+
+    // model.rs:1391-1407
+    std::pair<double, double> material_for(const std::string &elset) const {
+        auto m = elset_material.find(elset);
+        if (m != elset_material.end()) {
+            auto it = materials.find(m->second);
+            if (it != materials.end()) return it->second;
+        }
+        if (materials.size() == 1) return materials.begin()->second;
+        auto st = materials.find("STEEL");
+        if (st != materials.end()) return st->second;
+        throw Error{AXG_E_INVALID, "Kein Material für ELSET=" + elset};
+    }
+};
+
+namespace {
+
+// inp.rs:204-230: "set, first[, last][, value]"
+bool boundary_card(const std::vector<std::string> &t, std::string &name, int &first, int &last,
+                   double &val) {
+    if (t.size() < 2) return false;
+    name = upper(t[0]);
+    first = need_i32(t[1]);
+    auto dof_id = [](const std::string &s, int &d) {
+        if (!is_int_token(s)) return false;
+        long v = std::strtol(trim(s).c_str(), nullptr, 10);
+        if ((v >= 1 && v <= 6) || v == 11) {
+            d = (int)v;
+            return true;
+        }
+        return false;
+    };
+    val = 0.0;
+    if (t.size() == 2) {
+        last = first;
+        return true;
+    }
+    int d2;
+    if (t.size() == 3) {
+        if (dof_id(t[2], d2)) {
+            last = d2;
+        } else {
+            last = first;
+            val = f64_or0(t[2]);
+        }
+        return true;
+    }
+    last = dof_id(t[2], d2) ? d2 : first;
+    val = f64_or0(t[3]);
+    return true;
+}
+
+// inp.rs:232-268
+void push_boundary(axg_deck &m, const std::string &name, int first, int last, double val) {
+    first = std::max(first, 1);
+    last = std::max(last, first);
+    std::vector<int32_t> ids;
+    if (is_int_token(name)) {
+        ids.push_back((int32_t)std::strtoll(trim(name).c_str(), nullptr, 10));
+    } else {
+        auto it = m.nsets.find(name);
+        if (it != m.nsets.end() && !it->second.empty()) {
+            ids = it->second;
+        } else {
+            for (int d = first; d <= last; d++)
+                if (d >= 1 && d <= 6) m.deferred.push_back({Deferred::BC, name, "", d - 1, val});
+            return;
+        }
+    }
+    for (int32_t nid : ids)
+        for (int d = first; d <= last; d++)
+            if (d >= 1 && d <= 6) m.bcs.push_back({nid, d - 1, val});
+}
+
+// model.rs:1234-1309: nodes sorted by id (first definition wins), elements by
+// id, NALL / EALL, elements outside any sectioned set rebound to the first
+// sectioned set (by name) that lists them.
+void compact(axg_deck &m) {
+    const size_t n = m.node_ids.size();
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return m.node_ids[a] < m.node_ids[b]; });
+    std::vector<int32_t> ids;
+    std::vector<double> coords;
+    ids.reserve(n);
+    coords.reserve(3 * n);
+    for (size_t k = 0; k < n; k++) {
+        size_t i = order[k];
+        if (!ids.empty() && ids.back() == m.node_ids[i]) continue;
+        ids.push_back(m.node_ids[i]);
+        for (int q = 0; q < 3; q++) coords.push_back(m.coords[3 * i + q]);
+    }
+    m.node_ids.swap(ids);
+    m.coords.swap(coords);
+    m.id_to_index.clear();
+    for (size_t k = 0; k < m.node_ids.size(); k++) m.id_to_index[m.node_ids[k]] = (int64_t)k;
+    std::stable_sort(m.elements.begin(), m.elements.end(),
+                     [](const DeckElem &a, const DeckElem &b) { return a.id < b.id; });
+    m.nsets["NALL"] = m.node_ids;
+    std::vector<int32_t> eall;
+    for (auto &el : m.elements) eall.push_back(el.id);
+    m.elsets["EALL"] = eall;
+    if (!m.elset_material.empty()) {
+        std::unordered_map<int32_t, std::string> id_to_set;
+        for (auto &kv : m.elset_material) {  // sectioned set names in sorted order
+            auto it = m.elsets.find(kv.first);
+            if (it == m.elsets.end()) continue;
+            for (int32_t eid : it->second) id_to_set.emplace(eid, kv.first);
+        }
+        for (auto &el : m.elements) {
+            if (m.elset_material.count(el.elset)) continue;
+            auto it = id_to_set.find(el.id);
+            if (it != id_to_set.end()) el.elset = it->second;
+        }
+    }
+}
+
+// inp.rs:2256-2330 (the subset: nested sets, then BC, CLOAD, DLOAD)
+void expand_deferred(axg_deck &m) {
+    for (auto &d : m.deferred) {
+        if (d.kind == Deferred::NESTED_NSET) {
+            auto add = m.expand_nset(d.target);
+            auto &dst = m.nsets[d.name];
+            dst.insert(dst.end(), add.begin(), add.end());
+        } else if (d.kind == Deferred::NESTED_ELSET) {
+            auto it = m.elsets.find(d.target);
+            std::vector<int32_t> add = it == m.elsets.end() ? std::vector<int32_t>() : it->second;
+            auto &dst = m.elsets[d.name];
+            dst.insert(dst.end(), add.begin(), add.end());
+        }
+    }
+    for (auto &d : m.deferred) {
+        if (d.kind == Deferred::BC) {
+            for (int32_t nid : m.expand_nset(d.name)) m.bcs.push_back({nid, d.dof, d.val});
+        } else if (d.kind == Deferred::CL) {
+            for (int32_t nid : m.expand_nset(d.name)) m.cloads.push_back({nid, d.dof, d.val});
+        } else if (d.kind == Deferred::DL) {
+            auto it = m.elsets.find(d.name);
+            if (it != m.elsets.end())
+                for (int32_t eid : it->second) m.dloads.push_back({eid, d.dof, d.val});
+        }
+    }
+    m.deferred.clear();
+}
+
+void parse_deck(axg_deck &m, const std::string &text) {
+    std::vector<std::string> lines;
+    {
+        size_t a = 0;
+        while (a <= text.size()) {
+            size_t b = text.find('\n', a);
+            std::string line = text.substr(a, b == std::string::npos ? std::string::npos : b - a);
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            lines.push_back(line);
+            if (b == std::string::npos) break;
+            a = b + 1;
+        }
+        if (!lines.empty() && lines.back().empty()) lines.pop_back();
+    }
+    const size_t n = lines.size();
+    size_t i = 0;
+    std::string cur_mat;
+    bool have_mat = false;
+    auto collect = [&](size_t j, std::vector<std::string> &toks) {
+        toks.clear();
+        while (j < n) {
+            const std::string &raw = lines[j];
+            if (is_comment_or_blank(raw)) {
+                j++;
+                continue;
+            }
+            if (is_keyword(raw)) break;
+            auto t = split_tokens(raw);
+            toks.insert(toks.end(), t.begin(), t.end());
+            j++;
+        }
+        return j;
+    };
+    std::vector<std::string> toks;
+    std::map<std::string, std::string> params;
+    std::string kw;
+    while (i < n) {
+        const std::string &raw = lines[i];
+        if (is_comment_or_blank(raw) || !is_keyword(raw)) {
+            i++;
+            continue;
+        }
+        parse_keyword(raw, kw, params);
+        auto param = [&](const char *key, const std::string &dflt) {
+            auto it = params.find(key);
+            return it == params.end() ? dflt : it->second;
+        };
+        if (kw == "*NODE") {
+            const std::string nset = param("NSET", "");
+            const bool has_set = !nset.empty();
+            i++;
+            std::vector<int32_t> added;
+            while (i < n) {
+                const std::string &l = lines[i];
+                if (is_comment_or_blank(l)) {
+                    i++;
+                    continue;
+                }
+                if (is_keyword(l)) break;
+                auto t = split_tokens(l);
+                i++;
+                if (t.empty()) continue;
+                const int32_t nid = need_i32(t[0]);
+                m.node_ids.push_back(nid);
+                for (int q = 1; q <= 3; q++)
+                    m.coords.push_back(t.size() > (size_t)q ? f64_or0(t[q]) : 0.0);
+                added.push_back(nid);
+            }
+            if (has_set) {
+                auto &dst = m.nsets[nset];
+                dst.insert(dst.end(), added.begin(), added.end());
+            }
+        } else if (kw == "*ELEMENT") {
+            const std::string typ = param("TYPE", "");
+            int nn = 0;
+            if (typ == "C3D8") nn = 8;
+            else if (typ == "C3D20") nn = 20;
+            AXG_REQUIRE(nn > 0, AXG_E_INVALID,
+                        "Nicht unterstützter Elementtyp " + typ + " (this path: C3D8, C3D20).");
+            const std::string elset = param("ELSET", "EALL");
+            i = collect(i + 1, toks);
+            std::vector<int32_t> added;
+            size_t k = 0;
+            while (k + nn < toks.size()) {
+                DeckElem el;
+                el.id = need_i32(toks[k]);
+                el.nn = nn;
+                for (int a = 0; a < nn; a++) el.nodes.push_back(need_i32(toks[k + 1 + a]));
+                el.elset = elset;
+                k += 1 + nn;
+                added.push_back(el.id);
+                m.elements.push_back(std::move(el));
+            }
+            auto &dst = m.elsets[elset];
+            dst.insert(dst.end(), added.begin(), added.end());
+        } else if (kw == "*NSET" || kw == "*ELSET") {
+            const bool nsetkw = kw == "*NSET";
+            const char *key = nsetkw ? "NSET" : "ELSET";
+            AXG_REQUIRE(params.count(key), AXG_E_INVALID, kw + " ohne " + key + "=");
+            const std::string name = params[key];
+            i = collect(i + 1, toks);
+            std::vector<int32_t> ids;
+            if (params.count("GENERATE")) {
+                size_t k = 0;
+                while (k + 1 < toks.size()) {
+                    int32_t a = need_i32(toks[k]), b = need_i32(toks[k + 1]), st = 1;
+                    if (k + 2 < toks.size()) {
+                        st = need_i32(toks[k + 2]);
+                        if (st == 0) st = 1;
+                        k += 3;
+                    } else {
+                        k += 2;
+                    }
+                    if (st > 0)
+                        for (int64_t v = a; v <= b; v += st) ids.push_back((int32_t)v);
+                    else
+                        for (int64_t v = a; v >= b; v += st) ids.push_back((int32_t)v);
+                }
+            } else {
+                for (auto &t : toks) {
+                    int32_t v;
+                    if (parse_i32(t, v))
+                        ids.push_back(v);
+                    else
+                        m.deferred.push_back({nsetkw ? Deferred::NESTED_NSET : Deferred::NESTED_ELSET,
+                                              name, upper(t), 0, 0.0});
+                }
+            }
+            auto &dst = nsetkw ? m.nsets[name] : m.elsets[name];
+            dst.insert(dst.end(), ids.begin(), ids.end());
+        } else if (kw == "*MATERIAL") {
+            cur_mat = param("NAME", "MATERIAL-1");
+            have_mat = true;
+            m.materials.emplace(cur_mat, std::make_pair(0.0, 0.0));
+            i++;
+        } else if (kw == "*ELASTIC") {
+            i = collect(i + 1, toks);
+            AXG_REQUIRE(toks.size() >= 2, AXG_E_INVALID, "*ELASTIC benötigt E und nu");
+            double e = 0.0, nu = 0.0;
+            parse_f64(toks[0], e);
+            parse_f64(toks[1], nu);
+            if (std::fabs(e) <= 1e-18) {
+                e = 1.0;
+                for (auto &t : toks) {
+                    double v;
+                    if (parse_f64(t, v) && std::fabs(v) > 1e-18) {
+                        e = v;
+                        break;
+                    }
+                }
+            }
+            nu = std::min(std::max(nu, -0.49), 0.49);
+            m.materials[have_mat && !cur_mat.empty() ? cur_mat : "MATERIAL-1"] = {e, nu};
+        } else if (kw == "*SOLID SECTION") {
+            const std::string elset = param("ELSET", "EALL");
+            if (params.count("MATERIAL")) m.elset_material[elset] = params["MATERIAL"];
+            i = collect(i + 1, toks);
+        } else if (kw == "*BOUNDARY") {
+            i++;
+            while (i < n) {
+                const std::string &l = lines[i];
+                if (is_comment_or_blank(l)) {
+                    i++;
+                    continue;
+                }
+                if (is_keyword(l)) break;
+                std::string name;
+                int first, last;
+                double val;
+                if (boundary_card(split_tokens(l), name, first, last, val))
+                    push_boundary(m, name, first, last, val);
+                i++;
+            }
+        } else if (kw == "*CLOAD") {
+            i = collect(i + 1, toks);
+            size_t k = 0;
+            while (k + 2 < toks.size()) {
+                const std::string name = upper(toks[k]);
+                const int32_t dof = need_i32(toks[k + 1]);
+                double mag = 0.0;
+                parse_f64(toks[k + 2], mag);
+                k += 3;
+                if (dof < 1 || dof > 6) continue;
+                auto it = m.nsets.find(name);
+                if (is_int_token(name)) {
+                    m.cloads.push_back({(int32_t)std::strtoll(name.c_str(), nullptr, 10), dof - 1, mag});
+                } else if (it != m.nsets.end() && !it->second.empty()) {
+                    for (int32_t nid : it->second) m.cloads.push_back({nid, dof - 1, mag});
+                } else {
+                    m.deferred.push_back({Deferred::CL, name, "", dof - 1, mag});
+                }
+            }
+        } else if (kw == "*DLOAD") {
+            i = collect(i + 1, toks);
+            size_t k = 0;
+            while (k + 1 < toks.size()) {
+                const std::string name = upper(toks[k]), typ = upper(toks[k + 1]);
+                k += 2;
+                AXG_REQUIRE(typ.size() >= 2 && typ[0] == 'P' &&
+                                std::all_of(typ.begin() + 1, typ.end(),
+                                            [](char c) { return std::isdigit((unsigned char)c); }),
+                            AXG_E_INVALID, "DLOAD-Typ " + typ + " is not on this path");
+                const int face = std::atoi(typ.c_str() + 1);
+                AXG_REQUIRE(k < toks.size(), AXG_E_INVALID, "*DLOAD ohne Betrag");
+                double mag = 0.0;
+                parse_f64(toks[k], mag);
+                k += 1;
+                auto it = m.elsets.find(name);
+                if (is_int_token(name)) {
+                    m.dloads.push_back({(int32_t)std::strtoll(name.c_str(), nullptr, 10), face, mag});
+                } else if (it != m.elsets.end() && !it->second.empty()) {
+                    for (int32_t eid : it->second) m.dloads.push_back({eid, face, mag});
+                } else {
+                    m.deferred.push_back({Deferred::DL, name, "", face, mag});
+                }
+            }
+        } else {
+            i++;
+        }
+    }
+    compact(m);
+    expand_deferred(m);
+}
+
+// solve_linear's setup (analysis.rs:548-946) for an all-hex isotropic model,
+// before assembly: arrays, prescribed values (last BC wins), isolated nodes
+// pinned, the load vector (pressure faces in element order, then nodal loads
+// summed per node and DOF).
+void linear_setup(axg_deck &m) {
+    AXG_REQUIRE(!m.elements.empty(), AXG_E_INVALID, "Modell ohne Elemente.");
+    const int nn = m.elements[0].nn;
+    for (auto &el : m.elements)
+        AXG_REQUIRE(el.nn == nn, AXG_E_INVALID,
+                    "only all-C3D8 or all-C3D20 decks are on this path");
+    m.nn = nn;
+    m.nnode = (int64_t)m.node_ids.size();
+    m.nelem = (int64_t)m.elements.size();
+    m.ndof = 3 * m.nnode;
+    m.xyz = m.coords;
+    m.conn.assign((size_t)m.nelem * nn, 0);
+    m.e.assign(m.nelem, 0.0);
+    m.nu.assign(m.nelem, 0.0);
+    std::unordered_map<int32_t, int64_t> eidx;
+    for (int64_t k = 0; k < m.nelem; k++) {
+        const DeckElem &el = m.elements[k];
+        eidx.emplace(el.id, k);
+        for (int a = 0; a < nn; a++) {
+            auto it = m.id_to_index.find(el.nodes[a]);
+            AXG_REQUIRE(it != m.id_to_index.end(), AXG_E_INVALID,
+                        "Element " + std::to_string(el.id) + ": unbekannter Knoten " +
+                            std::to_string(el.nodes[a]));
+            m.conn[k * nn + a] = (int32_t)it->second;
+        }
+        auto mat = m.material_for(el.elset);
+        m.e[k] = mat.first;
+        m.nu[k] = mat.second;
+    }
+    auto node_index = [&](int32_t nid) {
+        auto it = m.id_to_index.find(nid);
+        AXG_REQUIRE(it != m.id_to_index.end(), AXG_E_INVALID,
+                    "Unbekannter Knoten " + std::to_string(nid));
+        return it->second;
+    };
+    // prescribed values, last BC wins (analysis.rs:555-563)
+    m.prescribed.assign(m.ndof, 0);
+    m.u0.assign(m.ndof, 0.0);
+    for (auto &b : m.bcs) {
+        if (b.k >= 3) continue;
+        const int64_t d = 3 * node_index(b.id) + b.k;
+        m.prescribed[d] = 1;
+        m.u0[d] = b.v;
+    }
+    // pin_unused_dofs (analysis.rs:2230-2355): nodes no element uses
+    std::vector<uint8_t> attached(m.nnode, 0);
+    for (int32_t c : m.conn) attached[c] = 1;
+    for (int64_t i = 0; i < m.nnode; i++)
+        if (!attached[i])
+            for (int d = 0; d < 3; d++) m.prescribed[3 * i + d] = 1;  // u0 keeps its value (0)
+    // pressure faces: elements ascending, each element's faces in deck order
+    m.f_full.assign(m.ndof, 0.0);
+    std::map<int64_t, std::vector<std::pair<int, double>>> by_elem;
+    for (auto &dl : m.dloads) {
+        auto it = eidx.find(dl.id);
+        AXG_REQUIRE(it != eidx.end(), AXG_E_INVALID,
+                    "DLOAD auf unbekanntes Element " + std::to_string(dl.id));
+        by_elem[it->second].push_back({dl.k, dl.v});
+    }
+    double exyz[60], fe[60];
+    for (auto &kv : by_elem) {
+        const int32_t *en = &m.conn[kv.first * nn];
+        for (int a = 0; a < nn; a++)
+            for (int q = 0; q < 3; q++) exyz[3 * a + q] = m.xyz[3 * (int64_t)en[a] + q];
+        for (auto &fm : kv.second) {
+            int st = face_pressure_host(nn, exyz, fm.first, fm.second, fe);
+            AXG_REQUIRE(st == AXG_OK, AXG_E_INVALID,
+                        std::string("Ungültige ") + (nn == 8 ? "C3D8" : "C3D20") + "-Fläche P" +
+                            std::to_string(fm.first));
+            for (int a = 0; a < nn; a++)
+                for (int q = 0; q < 3; q++) m.f_full[3 * (int64_t)en[a] + q] += fe[3 * a + q];
+        }
+    }
+    // add_cloads (analysis.rs:2709-2764): summed per (node, dof), then added once
+    std::map<int64_t, double> acc;
+    for (auto &cl : m.cloads) {
+        if (cl.k >= 3) continue;
+        const int64_t d = 3 * node_index(cl.id) + cl.k;
+        auto it = acc.find(d);
+        if (it == acc.end())
+            acc.emplace(d, 0.0 + cl.v * 1.0);
+        else
+            it->second += cl.v * 1.0;
+    }
+    for (auto &kv : acc) m.f_full[kv.first] += kv.second;
+}
+
+// DofMap::build (constraint.rs:26-136) without MPCs
+void number_dofs(axg_deck &m) {
+    m.ind_of.assign(m.ndof, -1);
+    int64_t k = 0;
+    bool any = false;
+    for (int64_t d = 0; d < m.ndof; d++) {
+        if (m.prescribed[d]) {
+            any = true;
+            continue;
+        }
+        m.ind_of[d] = k++;
+    }
+    m.n_free = k;
+    AXG_REQUIRE(k > 0 || any, AXG_E_INVALID, "Modell ohne freie Freiheitsgrade.");
+}
+
+}  // namespace
+
+extern "C" {
+
+int axg_deck_parse(const char *text, int64_t len, axg_deck **out) {
+    API_BEGIN
+    AXG_REQUIRE(text && out && len >= 0, AXG_E_INVALID, "axg_deck_parse: invalid argument");
+    *out = nullptr;
+    auto d = std::make_unique<axg_deck>();
+    parse_deck(*d, std::string(text, (size_t)len));
+    linear_setup(*d);
+    number_dofs(*d);
+    *out = d.release();
+    API_END
+}
+
+int axg_deck_destroy(axg_deck *deck) {
+    delete deck;
+    return AXG_OK;
+}
+
+int axg_deck_info(axg_deck *deck, int32_t *nn, int64_t *nnode, int64_t *nelem, int64_t *n_free) {
+    API_BEGIN
+    AXG_REQUIRE(deck, AXG_E_INVALID, "axg_deck_info: NULL deck");
+    if (nn) *nn = deck->nn;
+    if (nnode) *nnode = deck->nnode;
+    if (nelem) *nelem = deck->nelem;
+    if (n_free) *n_free = deck->n_free;
+    API_END
+}
+
+int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double *xyz,
+                    int32_t *conn, double *e, double *nu, int64_t *ind_of, double *u0,
+                    double *f_full) {
+    API_BEGIN
+    AXG_REQUIRE(deck, AXG_E_INVALID, "axg_deck_arrays: NULL deck");
+    const axg_deck &d = *deck;
+    auto put = [](auto *dst, const auto &src) {
+        if (dst) std::copy(src.begin(), src.end(), dst);
+    };
+    put(node_ids, d.node_ids);
+    if (elem_ids)
+        for (int64_t k = 0; k < d.nelem; k++) elem_ids[k] = d.elements[k].id;
+    put(xyz, d.xyz);
+    put(conn, d.conn);
+    put(e, d.e);
+    put(nu, d.nu);
+    put(ind_of, d.ind_of);
+    put(u0, d.u0);
+    put(f_full, d.f_full);
+    API_END
+}
+
+// solve_linear for the deck (analysis.rs:548-1162): assembly on the device,
+// autospc_null_diag from the full diagonal (rebuilding the DOF map when it
+// pins more DOFs), reduced right-hand side, GPU PCG, reconstruction,
+// reactions and nodal stresses.
+int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
+                   double *rf, double *stress, double *vm, axg_solve_info *info) {
+    API_BEGIN
+    AXG_REQUIRE(ctx && deck && u_full, AXG_E_INVALID, "axg_deck_solve: invalid argument");
+    axg_deck &d = *deck;
+    const int precision = opts ? opts->precision : AXG_PREC_FP64;
+    auto check = [](int st) {
+        if (st != AXG_OK) throw Error{st, axg_last_error()};
+    };
+    struct MeshGuard {
+        axg_mesh *m = nullptr;
+        ~MeshGuard() {
+            if (m) axg_mesh_destroy(m);
+        }
+    } g;
+    auto build = [&]() {
+        if (g.m) axg_mesh_destroy(g.m), g.m = nullptr;
+        check(axg_mesh_create(ctx, d.nn, d.nnode, d.xyz.data(), d.nelem, d.conn.data(), d.e.data(),
+                              d.nu.data(), d.ind_of.data(), &g.m));
+        check(axg_assemble(g.m, AXG_PREC_FP64, nullptr));
+    };
+    build();
+    // autospc_null_diag (analysis.rs:2164-2228): DOFs whose diagonal is at most
+    // 1e-14 of the largest are pinned at zero
+    {
+        std::vector<double> diag(d.ndof);
+        check(axg_full_diagonal(g.m, diag.data()));
+        double max_d = 0.0;
+        for (double v : diag) max_d = std::max(max_d, std::fabs(v));
+        bool more = false;
+        if (max_d >= 1e-30) {
+            const double tol = max_d * 1e-14;
+            for (int64_t i = 0; i < d.ndof; i++)
+                if (std::fabs(diag[i]) <= tol && !d.prescribed[i]) {
+                    d.prescribed[i] = 1;
+                    d.u0[i] = 0.0;
+                    more = true;
+                }
+        }
+        if (more) {
+            number_dofs(d);
+            build();
+        }
+    }
+    if (precision == AXG_PREC_MIXED) check(axg_assemble(g.m, AXG_PREC_MIXED, nullptr));
+    std::vector<double> rhs(std::max<int64_t>(d.n_free, 1)), x(std::max<int64_t>(d.n_free, 1));
+    check(axg_reduce_rhs(g.m, d.f_full.data(), d.u0.data(), rhs.data()));
+    axg_solve_info si;
+    std::memset(&si, 0, sizeof si);
+    if (d.n_free > 0) check(axg_mesh_solve(g.m, rhs.data(), 1, opts, x.data(), &si));
+    // DofMap::reconstruct (constraint.rs:207-218)
+    for (int64_t i = 0; i < d.ndof; i++) {
+        u_full[i] = d.u0[i];
+        if (d.ind_of[i] >= 0) u_full[i] = u_full[i] + 1.0 * x[d.ind_of[i]];
+    }
+    if (rf) check(axg_reactions(g.m, u_full, d.f_full.data(), rf));
+    if (stress) check(axg_nodal_stress(g.m, u_full, stress, vm, nullptr));
+    if (info) *info = si;
+    API_END
+}
+
+}  // extern "C"

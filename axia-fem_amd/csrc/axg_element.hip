@@ -504,6 +504,79 @@ __global__ void strain_energy_kernel(const double *__restrict__ xyz,
     energy[e] = en;
 }
 
+// Rows of the full (unconstrained) stiffness, from the stored FP64 element
+// matrices, one thread per DOF r = 3 node + d. Terms come in the reference's
+// triplet order (elements ascending, then the element row's columns) and skip
+// exact zeros, as the triplet list does (analysis.rs:651-668):
+//   ROW_REACT: out[r] = (sum_t v_t w_j) - f[r]             (analysis.rs:1027-1035)
+//   ROW_RHS:   out[ind_of[r]] = f[r] - v_t w_j - ... over prescribed j with
+//              w_j != 0, free r only (DofMap::reduce, constraint.rs:138-173)
+//   ROW_DIAG:  out[r] = sum of the diagonal triplets of row r
+//              (autospc_null_diag, analysis.rs:2164-2228)
+enum { ROW_REACT = 0, ROW_RHS = 1, ROW_DIAG = 2 };
+template <int NN, int MODE>
+__global__ void __launch_bounds__(128) row_apply_kernel(
+    int64_t ndof, const int64_t *__restrict__ inc_ptr, const int64_t *__restrict__ inc_slot,
+    const int32_t *__restrict__ conn, const double *__restrict__ ke,
+    const int32_t *__restrict__ ind_of, const double *__restrict__ w,
+    const double *__restrict__ f, double *__restrict__ out) {
+    constexpr int M = 3 * NN;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ndof) return;
+    if (MODE == ROW_RHS && ind_of[r] < 0) return;
+    const int64_t node = r / 3;
+    const int d = (int)(r - 3 * node);
+    double acc = 0.0;
+    if (MODE == ROW_RHS && f[r] != 0.0) acc += f[r];
+    for (int64_t k = inc_ptr[node]; k < inc_ptr[node + 1]; k++) {
+        const int64_t slot = inc_slot[k];
+        const int64_t e = slot / NN;
+        const int a = (int)(slot - e * NN);
+        const double *kr = ke + e * (int64_t)(M * M) + (int64_t)(3 * a + d) * M;
+        if (MODE == ROW_DIAG) {
+            acc += kr[3 * a + d];
+            continue;
+        }
+        const int32_t *en = conn + e * NN;
+        for (int j = 0; j < M; j++) {
+            const double v = kr[j];
+            if (v == 0.0) continue;
+            const int64_t gj = 3 * (int64_t)en[j / 3] + (j % 3);
+            if (MODE == ROW_REACT) {
+                acc += v * w[gj];
+            } else if (ind_of[gj] < 0 && w[gj] != 0.0) {
+                acc -= v * w[gj];
+            }
+        }
+    }
+    if (MODE == ROW_REACT)
+        out[r] = acc - f[r];
+    else if (MODE == ROW_RHS)
+        out[ind_of[r]] = acc;
+    else
+        out[r] = acc;
+}
+
+void launch_row_apply(axg_mesh *m, hipStream_t s, int mode, const double *d_w,
+                      const double *d_f, double *d_out) {
+    if (m->ndof == 0) return;
+    dim3 grid((unsigned)((m->ndof + 127) / 128));
+#define AXG_RA(NNV, MV)                                                                         \
+    hipLaunchKernelGGL((row_apply_kernel<NNV, MV>), grid, dim3(128), 0, s, m->ndof,              \
+                       m->inc_ptr.p, m->inc_slot.p, m->conn.p, m->ke.p, m->ind_of.p, d_w, d_f, d_out)
+    if (m->nn == 8) {
+        if (mode == ROW_REACT) AXG_RA(8, ROW_REACT);
+        else if (mode == ROW_RHS) AXG_RA(8, ROW_RHS);
+        else AXG_RA(8, ROW_DIAG);
+    } else {
+        if (mode == ROW_REACT) AXG_RA(20, ROW_REACT);
+        else if (mode == ROW_RHS) AXG_RA(20, ROW_RHS);
+        else AXG_RA(20, ROW_DIAG);
+    }
+#undef AXG_RA
+    AXG_HIP(hipGetLastError());
+}
+
 void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double *d_energy,
                           int32_t *d_err) {
     if (m->nelem == 0) return;

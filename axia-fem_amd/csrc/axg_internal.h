@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <exception>
+#include <new>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -118,6 +120,32 @@ struct Workspace {
 };
 }  // namespace axg
 
+namespace axg {
+// thread-local text behind axg_last_error
+void set_error(const std::string &msg);
+void clear_error();
+}  // namespace axg
+
+// Body of every extern "C" entry point: errors become status codes + text.
+#define API_BEGIN      \
+    try {              \
+        axg::clear_error();
+#define API_END                                   \
+    }                                             \
+    catch (const axg::Error &e) {                 \
+        axg::set_error(e.msg);                    \
+        return e.code;                            \
+    }                                             \
+    catch (const std::bad_alloc &) {              \
+        axg::set_error("host allocation failed"); \
+        return AXG_E_OOM;                         \
+    }                                             \
+    catch (const std::exception &e) {             \
+        axg::set_error(e.what());                 \
+        return AXG_E_INVALID;                     \
+    }                                             \
+    return AXG_OK;
+
 struct axg_ctx {
     // one reference held by the creator (axg_destroy drops it) and one per live
     // mesh: the context outlives every mesh made on it, whatever the call order
@@ -208,6 +236,9 @@ void launch_gather_bsr32(axg_mesh *m, hipStream_t s, const int32_t *entry_of_slo
 void upload_shape_tables32(const double *h8_dn, const double *h20_dn, const double *h20_w);
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
                          double *d_stress, double *d_vm, int32_t *d_err);
+// mode: 0 reactions, 1 reduced right-hand side, 2 full diagonal
+void launch_row_apply(axg_mesh *m, hipStream_t s, int mode, const double *d_w,
+                      const double *d_f, double *d_out);
 void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double *d_energy,
                           int32_t *d_err);
 void upload_shape_tables();

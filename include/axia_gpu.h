@@ -85,6 +85,7 @@ typedef struct axg_solve_info {
 
 typedef struct axg_ctx axg_ctx;
 typedef struct axg_mesh axg_mesh;
+typedef struct axg_deck axg_deck;
 
 /* ------------------------------------------------------------- context */
 /* Thread-local text of the last error (FemError message), "" if none. */
@@ -170,12 +171,50 @@ int axg_mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
 int axg_nodal_stress(axg_mesh *mesh, const double *u_full, double *stress, double *vm,
                      double *elem_nodal);
 
+/* Reactions rf = K_full u_full - f_full over all 3*nnode DOFs, constrained
+ * ones included (analysis.rs:1027-1035), from the FP64 element matrices of
+ * the last assembly, summed in the reference's triplet order. */
+int axg_reactions(axg_mesh *mesh, const double *u_full, const double *f_full, double *rf);
+
+/* Reduced right-hand side (DofMap::reduce, constraint.rs:138-173):
+ * rhs[ind_of[i]] = f_full[i] - sum_j K_ij u0_j over prescribed j with
+ * u0_j != 0, for every free DOF i (n_free entries). */
+int axg_reduce_rhs(axg_mesh *mesh, const double *f_full, const double *u0, double *rhs);
+
+/* Diagonal of the full stiffness, all 3*nnode DOFs: the sum of the diagonal
+ * triplets per DOF in element order (autospc_null_diag, analysis.rs:2164-2228). */
+int axg_full_diagonal(axg_mesh *mesh, double *diag);
+
 /* Compliance adjoint (no counterpart in the reference; new path). With
  * K = sum_e E_e K1_e and c = f^T u, the system is self-adjoint, so
  * dc/dE_e = -u_e^T K1_e u_e. Computes c from f_full and u_full and writes
  * dc_de (nelem). u_full, f_full: 3*nnode entries. */
 int axg_compliance_adjoint(axg_mesh *mesh, const double *u_full, const double *f_full, double *c,
                            double *dc_de);
+
+/* ------------------------------------------------------- deck ingestion */
+/* Parses an INP deck (the keyword subset of inp.rs:383-1061: *NODE, *ELEMENT
+ * TYPE=C3D8|C3D20, *NSET/*ELSET (GENERATE, nested names), *MATERIAL, *ELASTIC,
+ * *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn) into the compacted model
+ * (model.rs:1234-1309) and solve_linear's setup on the host
+ * (analysis.rs:548-946): arrays, prescribed values (last BC wins), isolated
+ * nodes pinned, load vector, free-DOF numbering (DofMap::build). text need not
+ * be NUL-terminated. */
+int axg_deck_parse(const char *text, int64_t len, axg_deck **out);
+int axg_deck_destroy(axg_deck *deck);
+int axg_deck_info(axg_deck *deck, int32_t *nn, int64_t *nnode, int64_t *nelem, int64_t *n_free);
+/* Copies of the setup (any pointer may be NULL): node ids (nnode, ascending),
+ * element ids (nelem, ascending), xyz (nnode x 3), conn (nelem x nn, 0-based),
+ * e, nu (nelem), ind_of, u0, f_full (3*nnode). */
+int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double *xyz,
+                    int32_t *conn, double *e, double *nu, int64_t *ind_of, double *u0,
+                    double *f_full);
+/* solve_linear (analysis.rs:548-1162) on the device: assembly, autospc of
+ * null diagonals (analysis.rs:2164-2228), reduced rhs, PCG (opts), u_full
+ * (3*nnode), reactions rf (3*nnode, may be NULL), nodal stress (nnode x 6)
+ * and von Mises (nnode) (both may be NULL). */
+int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
+                   double *rf, double *stress, double *vm, axg_solve_info *info);
 
 /* Batched solves: nmesh meshes (assembled), each with nrhs[m] right-hand
  * sides rhs[m] (n_free[m] * nrhs[m], column-major by right-hand side),

@@ -1,0 +1,133 @@
+"""The native INP reader and linear-static setup (axg_deck_parse) against the
+oracle's restatement of inp.rs / model.rs / analysis.rs:548-946 (CPU only).
+
+Every array of the setup (node ids, coordinates, connectivity, moduli, DOF
+numbering, prescribed values, load vector) must match bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+from axia_gpu import AxgError
+from axia_gpu import deck as D
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+DECKS = ["cube_tension", "c3d20_patch", "patch_c3d8", "le10_c3d20", "le10_c3d20_fine"]
+
+
+def read(name):
+    return open(os.path.join(GOLD, name + ".inp")).read()
+
+
+def assert_same_setup(text):
+    d = D.Deck(text)
+    a = d.arrays()
+    m = O.parse_inp(text)
+    s = O.deck_setup(m)
+    assert d.nn == s.nn and d.nnode == len(m.node_ids) and d.n_free == s.n_free
+    assert np.array_equal(a["node_ids"], np.array(m.node_ids))
+    assert np.array_equal(a["elem_ids"], np.array([e[0] for e in m.elements]))
+    for key, ref in (("xyz", s.xyz), ("conn", s.conn), ("e", s.e), ("nu", s.nu),
+                     ("ind_of", s.ind_of), ("u0", s.u0), ("f_full", s.f_full)):
+        assert np.array_equal(a[key], ref), key
+    return d, a, s
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_reference_decks_setup_bitexact(name):
+    assert_same_setup(read(name))
+
+
+def test_deferred_elset_and_second_material():
+    """lib.rs:410-427: ELSET defined after the elements, a dummy material besides STEEL."""
+    text = read("cube_tension").replace("*ELEMENT, TYPE=C3D8, ELSET=SOLID", "*ELEMENT, TYPE=C3D8")
+    text = text.replace("*SOLID SECTION, ELSET=SOLID, MATERIAL=STEEL",
+                        "*ELSET, ELSET=SOLID\n1\n*MATERIAL, NAME=DUMMY\n*ELASTIC\n1.0, 0.0\n"
+                        "*SOLID SECTION, ELSET=SOLID, MATERIAL=STEEL")
+    _, a, _ = assert_same_setup(text)
+    assert a["e"][0] == 210000.0
+
+
+BRICK = """*HEADING
+two C3D8 bricks
+*NODE, NSET=ALLN
+1, 0., 0., 0.
+2, 1.d0, 0., 0.
+3, 1., 1., 0.
+4, 0., 1.0D0, 0.
+5, 0., 0., 1.
+6, 1., 0., 1.
+7, 1., 1., 1.
+8, 0., 1., 1.
+9, 2., 0., 0.
+10, 2., 1., 0.
+11, 2., 1., 1.
+12, 2., 0., 1.
+** a comment line
+*ELEMENT, TYPE=C3D8, ELSET=BLK
+1, 1, 2, 3, 4, 5, 6, 7, 8
+2, 2, 9, 10, 3, 6, 12, 11, 7
+*NSET, NSET=LEFT, GENERATE
+1, 8, 1
+*NSET, NSET=LEFTFACE
+1, 4, 5, 8
+*NSET, NSET=RIGHT
+9, 10, 11, 12
+*NSET, NSET=BOTH
+LEFTFACE, RIGHT
+*MATERIAL, NAME=ALU
+*ELASTIC
+7.0+4, 0.33
+*SOLID SECTION, ELSET=BLK, MATERIAL=ALU
+*STEP
+*STATIC
+*BOUNDARY
+LEFTFACE, 1, 3
+2, 2, 2, 0.0
+*CLOAD
+RIGHT, 3, -2.5
+LATE, 1, 1.0
+9, 3, -1.
+*DLOAD
+BLK, P2, 3.0
+2, P4, -1.5
+*NSET, NSET=LATE
+11
+*END STEP
+"""
+
+
+def test_synthetic_deck_keywords():
+    """Fortran reals (1.d0, 7.0+4), GENERATE, nested sets, BOUNDARY ranges, CLOAD on a set
+    defined later (deferred), DLOAD on a set and on an element id."""
+    d, a, s = assert_same_setup(BRICK)
+    assert d.nnode == 12 and d.nelem == 2
+    assert a["e"][0] == 70000.0 and a["nu"][0] == 0.33
+    assert a["xyz"][1, 0] == 1.0 and a["xyz"][3, 1] == 1.0
+    # node 11 gets the deferred CLOAD in x
+    assert a["f_full"][3 * 10 + 0] != 0.0
+
+
+def test_boundary_values_and_overrides():
+    """Prescribed values, last BC wins, node ids out of order and a repeated node id
+    (first definition wins)."""
+    text = BRICK.replace("2, 2, 2, 0.0", "2, 2, 2, 0.25\n2, 2, 2, -0.5\n12, 1, 3, 0.125")
+    text = text.replace("12, 2., 0., 1.\n", "12, 2., 0., 1.\n3, 9., 9., 9.\n")
+    _, a, s = assert_same_setup(text)
+    assert a["u0"][3 * 1 + 1] == -0.5
+    assert a["xyz"][2, 0] == 1.0
+
+
+def test_unsupported_and_malformed_decks():
+    with pytest.raises(AxgError) as e:
+        D.Deck(BRICK.replace("TYPE=C3D8", "TYPE=C3D10"))
+    assert "Nicht unterstützter Elementtyp" in str(e.value)
+    with pytest.raises(AxgError) as e:
+        D.Deck(BRICK.replace("LATE, 1, 1.0", "NOSUCH, 1, 1.0"))
+    assert "Unbekanntes Knotenset" in str(e.value)
+    with pytest.raises(AxgError):
+        D.Deck(BRICK.replace("2, 2, 9, 10, 3", "2, 2, 99, 10, 3"))
+    with pytest.raises(AxgError) as e:
+        D.Deck(BRICK.replace("BLK, P2, 3.0", "BLK, BX, 3.0"))
+    assert "DLOAD" in str(e.value)
