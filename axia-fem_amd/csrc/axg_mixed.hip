@@ -143,22 +143,29 @@ static void build_bsr(axg_mesh *m) {
     bci.reserve(nnz / 3 + 16);
     // block columns of block row I = block rows of the columns of I's first free row
     // (all free rows of a node share the same node neighbourhood)
+    std::vector<int32_t> row;
     for (int64_t I = 0; I < nb; I++) {
         int32_t r = -1;
         for (int d = 0; d < 3 && r < 0; d++) r = fob[3 * I + d];
-        int32_t last = -1;
-        for (int32_t k = rp[r]; k < rp[r + 1]; k++) {
-            int32_t J = bof[ci[k]] / 3;
-            if (J != last) {
-                bci.push_back(J);
-                last = J;
-            }
-        }
+        row.clear();
+        for (int32_t k = rp[r]; k < rp[r + 1]; k++) row.push_back(bof[ci[k]] / 3);
+        std::sort(row.begin(), row.end());  // any free-DOF numbering
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        AXG_REQUIRE(std::binary_search(row.begin(), row.end(), (int32_t)I), AXG_E_INVALID,
+                    "block row without its diagonal block");
+        bci.insert(bci.end(), row.begin(), row.end());
         brp[I + 1] = (int32_t)bci.size();
     }
     const int64_t nnzb = (int64_t)bci.size();
-    // block-sliced-ELL layout: 64 block rows per slice, block k of row I at
-    // slot sptr[I / 64] + 64 (k - brp[I]) + I % 64
+    // block-sliced-ELL layout: 64 block rows per slice; row I's diagonal block
+    // first, then its other blocks by ascending column, position k of row I at
+    // slot sptr[I / 64] + 64 k + I % 64
+    auto pos_in_row = [&](int64_t I, int32_t J) {
+        const auto b0 = bci.begin() + brp[I], b1 = bci.begin() + brp[I + 1];
+        const int64_t kb = std::lower_bound(b0, b1, J) - b0;
+        const int64_t kd = std::lower_bound(b0, b1, (int32_t)I) - b0;
+        return J == (int32_t)I ? (int64_t)0 : (kb < kd ? kb + 1 : kb);
+    };
     std::vector<int64_t> blen(nb);
     for (int64_t I = 0; I < nb; I++) blen[I] = brp[I + 1] - brp[I];
     SellHost L;
@@ -168,19 +175,16 @@ static void build_bsr(axg_mesh *m) {
     std::vector<int32_t> bcol(nsl, 0);
     for (int64_t I = 0; I < nb; I++) {
         int64_t base = L.sptr[I >> 6] + (I & 63);
-        for (int32_t k = brp[I]; k < brp[I + 1]; k++) bcol[base + 64 * (int64_t)(k - brp[I])] = bci[k];
+        for (int32_t k = brp[I]; k < brp[I + 1]; k++) bcol[base + 64 * pos_in_row(I, bci[k])] = bci[k];
         for (int64_t k = blen[I]; k < L.width[I >> 6]; k++) bcol[base + 64 * k] = (int32_t)I;
     }
     std::vector<int32_t> eos(nnz);
     for (int64_t i = 0; i < n; i++) {
         int32_t I = bof[i] / 3, d = bof[i] % 3;
-        int32_t k0 = brp[I], k1 = brp[I + 1];
         int64_t base = L.sptr[I >> 6] + (I & 63);
         for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
             int32_t J = bof[ci[k]] / 3, e = bof[ci[k]] % 3;
-            int32_t kb = (int32_t)(std::lower_bound(bci.begin() + k0, bci.begin() + k1, J) -
-                                   bci.begin());
-            int64_t pos = base + 64 * (int64_t)(kb - k0);
+            int64_t pos = base + 64 * pos_in_row(I, J);
             int q = 3 * d + e;  // planes: entries 0-3 and 4-7 as float4, entry 8 alone
             eos[k] = (int32_t)(q < 4 ? 4 * pos + q : q < 8 ? 4 * nsl + 4 * pos + (q - 4) : 8 * nsl + pos);
         }
@@ -307,7 +311,25 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
                 acc[2][j] += vb.z * x0[j] + vb.w * x1[j] + vc * x2[j];
             }
         };
-        int jb = 0;
+        // block 0 of every row is its diagonal block: x there is this row's p
+        float pv[3][K];
+        {
+            const int64_t p = base;
+            const float *x = P + (int64_t)3 * I * K;
+            load_k<K>(x, pv[0]);
+            load_k<K>(x + K, pv[1]);
+            load_k<K>(x + 2 * K, pv[2]);
+            const f4v va = __builtin_nontemporal_load(&pa[p]);
+            const f4v vb = __builtin_nontemporal_load(&pb[p]);
+            const float vc = __builtin_nontemporal_load(&pc[p]);
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                acc[0][j] += va.x * pv[0][j] + va.y * pv[1][j] + va.z * pv[2][j];
+                acc[1][j] += va.w * pv[0][j] + vb.x * pv[1][j] + vb.y * pv[2][j];
+                acc[2][j] += vb.z * pv[0][j] + vb.w * pv[1][j] + vc * pv[2][j];
+            }
+        }
+        int jb = 1;
         for (; jb + U <= w; jb += U) {
             f4v va[U], vb[U];
             float vc[U];
@@ -330,12 +352,9 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             store_k<K>(AP + (3 * I + d) * K, acc[d]);
-            if (DOT) {
-                float pv[K];
-                load_k<K>(P + (3 * I + d) * K, pv);
+            if (DOT)
 #pragma unroll
-                for (int j = 0; j < K; j++) dot[j] += (double)pv[j] * (double)acc[d][j];
-            }
+                for (int j = 0; j < K; j++) dot[j] += (double)pv[d][j] * (double)acc[d][j];
         }
     }
     if (!DOT) return;
@@ -577,8 +596,19 @@ static const int kNumSpmm = 4;
         default: MACRO(4); break;   \
     }
 
-static int spmm_grid(axg_ctx *ctx, int64_t nb) {
-    return std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS, (int64_t)ctx->cu_count * 8));
+// Workgroups of one SpMM launch: the same number on every CU (4, or fewer
+// when the variant's occupancy is lower). Measured on the 80^3 brick, an even
+// per-CU count matters more than an even per-workgroup row count: 1024
+// workgroups beat 1032 (a few CUs with one extra) by 25 %, 2048 by 7 %.
+template <int K, bool DOT>
+static int spmm_grid(axg_ctx *ctx, int64_t nb, int shape) {
+    int per_cu = 1;
+#define AXG_OCC(UU) \
+    per_cu = resident_per_cu(reinterpret_cast<const void *>(bsell_spmm_kernel<K, UU, DOT>), MBS)
+    AXG_SPMM_SWITCH(shape, AXG_OCC)
+#undef AXG_OCC
+    return std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
+                                              (int64_t)ctx->cu_count * std::min(per_cu, 4)));
 }
 
 template <int K>
@@ -587,10 +617,10 @@ static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     int best = 1;
-    const int grid = spmm_grid(ctx, m->nb);
     if (m->nnzb > 200000) {
         float best_ms = 1e30f;
         for (int c = 0; c < kNumSpmm; c++) {
+            const int grid = spmm_grid<K, false>(ctx, m->nb, c);
 #define AXG_T(UU)                                                                                \
     hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
                        m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, P, AP,  \
@@ -703,7 +733,7 @@ static void bench_spmm_k(axg_mesh *m, int reps, double *avg_ms) {
     DevBuf<float> P(nv * K), AP(nv * K);
     AXG_HIP(hipMemsetAsync(P.p, 0, 4 * nv * K, s));
     const int shp = tune_spmm<K>(m, P.p, AP.p);
-    const int grid = spmm_grid(ctx, m->nb);
+    const int grid = spmm_grid<K, false>(ctx, m->nb, shp);
 #define AXG_T(UU)                                                                                \
     hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
                        m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, P.p,    \
@@ -756,11 +786,11 @@ struct MixedSolver {
         w.t0.ensure(n * K);   // r64
         w.t1.ensure(32);      // scales, alpha, beta
         const int shp = tune_spmm<K>(m, w.f3.p, w.f4.p);
-        const int grid_mv = spmm_grid(ctx, nb);
+        const int grid_mv = spmm_grid<K, true>(ctx, nb, shp);
         const int grid_v = std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
-                                                              (int64_t)ctx->cu_count * 4));
+                                                              (int64_t)ctx->cu_count * 2));
         const int grid_r = std::max(1, (int)std::min<int64_t>((n + MBS - 1) / MBS,
-                                                              (int64_t)ctx->cu_count * 8));
+                                                              (int64_t)ctx->cu_count * 4));
         w.part.ensure((size_t)std::max({grid_mv, grid_v, grid_r}) * 2 * K + 8);
         DevBuf<InnerScalars> dsc(1);
         ctx->pinned.ensure(sizeof(InnerScalars) + 256);

@@ -12,6 +12,15 @@
 
 namespace axg {
 
+// Workgroups of one kernel resident per CU (its occupancy), at least 1.
+inline int resident_per_cu(const void *kernel, int bsz) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bsz, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu;
+}
+
 // Rows of a streaming operator kernel, by workgroup. Workgroups are dealt to
 // the 8 XCDs round-robin (workgroup b starts on XCD b % 8), each XCD with its
 // own L2. Renumbering them so that one XCD's workgroups hold consecutive
