@@ -329,22 +329,38 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
                 acc[2][j] += vb.z * pv[0][j] + vb.w * pv[1][j] + vc * pv[2][j];
             }
         }
-        int jb = 1;
-        for (; jb + U <= w; jb += U) {
-            f4v va[U], vb[U];
-            float vc[U];
-            int32_t J[U];
+        // batches of U blocks after the diagonal one; the next batch's values
+        // and columns are loaded before this batch's x gathers
+        const int nfull = (w - 1) / U;
+        f4v va[U], vb[U];
+        float vc[U];
+        int32_t J[U];
+        auto load_batch = [&](int b, f4v (&ra)[U], f4v (&rb)[U], float (&rc)[U],
+                              int32_t (&rj)[U]) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int64_t p = base + 64 * (int64_t)(jb + u);
-                J[u] = __builtin_nontemporal_load(&col[p]);
-                va[u] = __builtin_nontemporal_load(&pa[p]);
-                vb[u] = __builtin_nontemporal_load(&pb[p]);
-                vc[u] = __builtin_nontemporal_load(&pc[p]);
+                const int64_t p = base + 64 * (int64_t)(1 + b * U + u);
+                rj[u] = __builtin_nontemporal_load(&col[p]);
+                ra[u] = __builtin_nontemporal_load(&pa[p]);
+                rb[u] = __builtin_nontemporal_load(&pb[p]);
+                rc[u] = __builtin_nontemporal_load(&pc[p]);
             }
+        };
+        if (nfull > 0) load_batch(0, va, vb, vc, J);
+        for (int b = 0; b < nfull; b++) {
+            f4v na[U], nbv[U];
+            float nc[U];
+            int32_t nj[U];
+            const bool more = b + 1 < nfull;
+            if (more) load_batch(b + 1, na, nbv, nc, nj);
 #pragma unroll
             for (int u = 0; u < U; u++) fma_block(va[u], vb[u], vc[u], J[u]);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; u++) va[u] = na[u], vb[u] = nbv[u], vc[u] = nc[u], J[u] = nj[u];
+            }
         }
+        int jb = 1 + nfull * U;
         for (; jb < w; jb++) {
             const int64_t p = base + 64 * (int64_t)jb;
             fma_block(pa[p], pb[p], pc[p], col[p]);
