@@ -15,6 +15,8 @@
 #include <cstring>
 #include <memory>
 
+#include <hip/hip_ext.h>
+
 #include "axg_internal.h"
 #include "axg_mixed.h"
 #include "axg_reduce.h"
@@ -817,15 +819,22 @@ struct MixedSolver {
         AXG_HIP(hipMemsetAsync(d_u, 0, sizeof(double) * n * K, s));
         AXG_HIP(hipEventRecord(ctx->ev0, s));
 
+        // timing: the events are stamped by the dispatch itself (kernel start
+        // and end), not by separate stream markers
         auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1) {
-            if (e0) AXG_HIP(hipEventRecord(e0, s));
 #define AXG_SPMM(UU)                                                                            \
-    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s, nb,      \
-                       m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, w.f3.p,  \
-                       w.f4.p, w.part.p, dsc.p)
+    if (e0)                                                                                     \
+        hipExtLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s,  \
+                              e0, e1, 0, nb, (const int64_t *)m->bs_sptr.p,                     \
+                              (const int32_t *)m->bs_width.p, (const int32_t *)m->bs_col.p,     \
+                              (const float *)m->bval.p, m->bs_nslots, (const float *)w.f3.p,   \
+                              w.f4.p, w.part.p, dsc.p);                                         \
+    else                                                                                        \
+        hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s, nb,  \
+                           m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots,    \
+                           w.f3.p, w.f4.p, w.part.p, dsc.p)
             AXG_SPMM_SWITCH(shp, AXG_SPMM)
 #undef AXG_SPMM
-            if (e1) AXG_HIP(hipEventRecord(e1, s));
             hipLaunchKernelGGL((fin_spmm_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p, grid_mv,
                                dsc.p);
         };

@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "axg_internal.h"
 #include "axg_pcg.h"
 #include "axg_reduce.h"
@@ -573,20 +575,30 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
     hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p, 1);
     AXG_HIP(hipGetLastError());
 
+    // timing: the events are stamped by the SpMV dispatch itself
     auto one_iter = [&](hipEvent_t e0, hipEvent_t e1) {
-        if (e0) AXG_HIP(hipEventRecord(e0, s));
-        if (sell) {
+        if (sell && e0) {
+            hipExtLaunchKernelGGL((sell_spmv_dot_kernel<4>), dim3(grid_mv), dim3(BS), 0, s, e0, e1,
+                                  0, n, (const int64_t *)A.sell_sptr, (const int32_t *)A.sell_width,
+                                  (const int32_t *)A.sell_col, (const double *)A.sell_val,
+                                  (const double *)w.p.p, w.ap.p, w.part.p, w.sc.p);
+        } else if (sell) {
             hipLaunchKernelGGL((sell_spmv_dot_kernel<4>), dim3(grid_mv), dim3(BS), 0, s, n,
                                A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, w.p.p, w.ap.p,
                                w.part.p, w.sc.p);
         } else {
 #define AXG_L(LL, UU)                                                                            \
-    hipLaunchKernelGGL((spmv_dot_kernel<LL, UU>), dim3(grid_mv), dim3(BS), 0, s, n, A.rowptr,     \
-                       A.colind, A.val, w.p.p, w.ap.p, w.part.p, w.sc.p)
+    if (e0)                                                                                      \
+        hipExtLaunchKernelGGL((spmv_dot_kernel<LL, UU>), dim3(grid_mv), dim3(BS), 0, s, e0, e1, 0, \
+                              n, (const int32_t *)A.rowptr, (const int32_t *)A.colind,           \
+                              (const double *)A.val, (const double *)w.p.p, w.ap.p, w.part.p,   \
+                              w.sc.p);                                                           \
+    else                                                                                         \
+        hipLaunchKernelGGL((spmv_dot_kernel<LL, UU>), dim3(grid_mv), dim3(BS), 0, s, n, A.rowptr, \
+                           A.colind, A.val, w.p.p, w.ap.p, w.part.p, w.sc.p)
             AXG_SHAPE_SWITCH(shape, AXG_L)
 #undef AXG_L
         }
-        if (e1) AXG_HIP(hipEventRecord(e1, s));
         hipLaunchKernelGGL(fin_pap_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_mv, w.sc.p);
         if (fused) {
             hipLaunchKernelGGL((update_xr_kernel<true>), dim3(grid_v), dim3(BS), 0, s, n, x, w.r.p,

@@ -294,8 +294,7 @@ def run_brick(a, G, MG, torch, np):
         "speedup_mixed_vs_fp64_time_to_solution": (fp64["solve_ms"] / solve_ms) if fp64 else None,
         "roofline": roofline(f"bsell_spmm_kernel<K={k}> (FP32 block-sliced-ELL SpMM)",
                              spmm_bytes(nb, nnzb, k), spmv_ms,
-                             traffic_for(f"axg::bsell_spmm_kernel<{k}, 1, false>", f"brick{a.n}")
-                             if a.n == 80 else None),
+                             traffic_for(f"axg::bsell_spmm_kernel<{k}, U, true>", f"brick{a.n}")),
         "reference_fp64_iters": ref["iters"],
     })
     if not a.no_cpu:
