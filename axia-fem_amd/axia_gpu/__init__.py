@@ -39,7 +39,7 @@ class SparseResult:
     residual: float
 
 
-def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inner_tol=1e-4,
+def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inner_tol=1e-3,
           max_outer=30, timing=False, refinement="fcg"):
     o = _lib.SolveOpts()
     _lib.lib().axg_default_opts(C.byref(o))
@@ -197,7 +197,7 @@ class Mesh:
         return out.reshape(count, m, m)
 
     def solve(self, rhs, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
-              inner_tol=1e-4, max_outer=30, timing=False, refinement="fcg"):
+              inner_tol=1e-3, max_outer=30, timing=False, refinement="fcg"):
         """Solve K_ff x = rhs; rhs (n_free,) or (nrhs, n_free). Returns (x, info)."""
         b = f64(rhs)
         single = b.ndim == 1
@@ -211,7 +211,7 @@ class Mesh:
         return (x[0] if single else x), _info_dict(info)
 
     def solve_device(self, rhs_ptr, nrhs, x_ptr, precision="fp64", precond="jacobi", tol=1e-8,
-                     max_iter=0, inner_tol=1e-4, max_outer=30, timing=False, refinement="fcg"):
+                     max_iter=0, inner_tol=1e-3, max_outer=30, timing=False, refinement="fcg"):
         """Device-pointer solve (inputs already in HBM); returns info."""
         info = _lib.SolveInfo()
         o = _opts(precision, precond, tol, max_iter, inner_tol, max_outer, timing, refinement)

@@ -78,7 +78,7 @@ class Deck:
         return out
 
     def solve(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
-              inner_tol=1e-4):
+              inner_tol=1e-3):
         from . import _info_dict, _opts, default_context
         ctx = ctx or default_context()
         ndof = 3 * self.nnode

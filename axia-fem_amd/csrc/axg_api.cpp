@@ -117,7 +117,7 @@ void axg_default_opts(axg_solve_opts *o) {
     o->precond = AXG_PRECOND_JACOBI;
     o->tol = 1e-8;
     o->max_iter = 0;
-    o->inner_tol = 1e-4;
+    o->inner_tol = 1e-3;
     o->max_outer = 30;
     o->timing = 0;
     o->refinement = AXG_REFINE_FCG;

@@ -370,8 +370,12 @@ def run_c3d20(a, G, MG, torch, np):
     x = torch.zeros_like(b)
     xref = torch.zeros_like(b)
     # the FP64 residual of this slender beam bottoms out near 1e-10 relative (round-off
-    # of f - K u), so the solves stop at 1e-9; displacements are compared at 1e-8
+    # of f - K u), so the solves stop at 1e-9; displacements are compared at 1e-8.
+    # Its first inner solve gains only ~3x whatever the inner tolerance (the FP32
+    # operator's rounding, amplified by the beam's condition number, limits it), so
+    # each inner solve stops at 1e-2 (a sweep of 1e-1 ... 1e-4 measured this fastest)
     tol = max(a.tol, 1e-9)
+    inner_tol = 1e-2
     mesh.assemble("mixed")
     ref = mesh.solve_device(b.data_ptr(), 1, xref.data_ptr(), precision="fp64", precond="block3",
                             tol=1e-13)
@@ -383,7 +387,7 @@ def run_c3d20(a, G, MG, torch, np):
     def step(timing, precond="block3"):
         ms = mesh.assemble("mixed")
         inf = mesh.solve_device(b.data_ptr(), 1, x.data_ptr(), precision="mixed", precond=precond,
-                                tol=tol, inner_tol=a.inner_tol, timing=timing)
+                                tol=tol, inner_tol=inner_tol, timing=timing)
         return ms, mesh.assembly_times(), inf
 
     for _ in range(a.warmup):
@@ -419,7 +423,7 @@ def run_c3d20(a, G, MG, torch, np):
         "workload": "C3D20 brick 96x24x24, jittered interior corners +-0.15 h (seed 0), x=0 "
                     f"clamped, tip shear 1 N/node; mixed block-Jacobi PCG to {tol:g}",
         "elements": ne, "n_free": n, "nnz": nnz, "nrhs": 1, "precond": "block3",
-        "inner_tol": a.inner_tol})
+        "inner_tol": inner_tol})
     out.update({
         "variants": variants,
         "speedup_mixed_vs_fp64_block3": v64["solve_ms"] / solve_ms,
@@ -646,7 +650,7 @@ def main():
     ap.add_argument("--nrhs", type=int, default=4)
     ap.add_argument("--tol", type=float, default=1e-11)
     ap.add_argument("--precond", default="block3")
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--fd-elements", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

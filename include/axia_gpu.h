@@ -62,7 +62,7 @@ typedef struct axg_solve_opts {
     int32_t precond;     /* enum axg_precond                                         */
     double tol;          /* stop when ||f - K u||_2 <= tol * ||f||_2 (default 1e-8)  */
     int64_t max_iter;    /* total PCG iterations; <= 0 means max(4n, 200)            */
-    double inner_tol;    /* mixed: relative residual drop per refinement (1e-4)      */
+    double inner_tol;    /* mixed: relative residual drop per inner solve (1e-3)    */
     int32_t max_outer;   /* mixed: refinement sweeps / inner solves (default 30)     */
     int32_t timing;      /* 1: record per-kernel HIP events (SpMV durations in info) */
     int32_t refinement;  /* mixed: enum axg_refinement (default FCG)                 */
