@@ -173,22 +173,22 @@ static void build_bsr(axg_mesh *m) {
     SellHost L;
     sell_layout(blen, L);
     const int64_t nsl = L.nslots;
-    AXG_REQUIRE(9 * nsl < INT32_MAX, AXG_E_INVALID, "FP32 operator exceeds 2^31 values");
+    AXG_REQUIRE(10 * nsl < INT32_MAX, AXG_E_INVALID, "FP32 operator exceeds 2^31 values");
     std::vector<int32_t> bcol(nsl, 0);
     for (int64_t I = 0; I < nb; I++) {
-        int64_t base = L.sptr[I >> 6] + (I & 63);
+        const int64_t base = L.sptr[I >> 6] + (I & 63);
         for (int32_t k = brp[I]; k < brp[I + 1]; k++) bcol[base + 64 * pos_in_row(I, bci[k])] = bci[k];
         for (int64_t k = blen[I]; k < L.width[I >> 6]; k++) bcol[base + 64 * k] = (int32_t)I;
     }
     std::vector<int32_t> eos(nnz);
     for (int64_t i = 0; i < n; i++) {
         int32_t I = bof[i] / 3, d = bof[i] % 3;
-        int64_t base = L.sptr[I >> 6] + (I & 63);
+        const int64_t base = L.sptr[I >> 6] + (I & 63);
         for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
             int32_t J = bof[ci[k]] / 3, e = bof[ci[k]] % 3;
             int64_t pos = base + 64 * pos_in_row(I, J);
-            int q = 3 * d + e;  // planes: entries 0-3 and 4-7 as float4, entry 8 alone
-            eos[k] = (int32_t)(q < 4 ? 4 * pos + q : q < 8 ? 4 * nsl + 4 * pos + (q - 4) : 8 * nsl + pos);
+            int q = 3 * d + e;  // planes: entries 0-3, 4-7 as float4; entry 8 beside the column
+            eos[k] = (int32_t)(q < 4 ? 4 * pos + q : q < 8 ? 4 * nsl + 4 * pos + (q - 4) : 8 * nsl + 2 * pos);
         }
     }
     m->nnzb = nnzb;
@@ -206,16 +206,27 @@ static void build_bsr(axg_mesh *m) {
     AXG_HIP(hipStreamSynchronize(s));
 }
 
+// The third plane holds (entry 8, column) pairs: the column's bits go in
+// beside the value, so one 8-byte load fetches both.
+__global__ void col_plane_kernel(int64_t nsl, const int32_t *__restrict__ col,
+                                 float *__restrict__ c2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsl) c2[2 * i + 1] = __int_as_float(col[i]);
+}
+
 void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     if (m->nnzb == 0) build_bsr(m);
     ensure_block_inverse64(m);
     m->ke32.ensure((size_t)m->nelem * m->m * m->m);
-    m->bval.ensure(std::max<int64_t>(9 * m->bs_nslots, 1));
+    m->bval.ensure(std::max<int64_t>(10 * m->bs_nslots, 1));
     Workspace &w = ctx->ws;
     AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
-    AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 9 * m->bs_nslots, s));
+    AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 10 * m->bs_nslots, s));
+    if (m->bs_nslots)
+        hipLaunchKernelGGL(col_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)), dim3(256),
+                           0, s, m->bs_nslots, m->bs_col.p, m->bval.p + 8 * m->bs_nslots);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     launch_ke_fp32(m, s, w.err.p);
     AXG_HIP(hipEventRecord(ctx->evA, s));
@@ -281,13 +292,13 @@ struct InnerScalars {
 template <int K, int U, bool DOT>
 __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
     int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
-    const int32_t *__restrict__ col, const float *__restrict__ bval, int64_t nsl,
-    const float *__restrict__ P, float *__restrict__ AP, double *__restrict__ part,
-    InnerScalars *sc) {
+    const float *__restrict__ bval, int64_t nsl, const float *__restrict__ P,
+    float *__restrict__ AP, double *__restrict__ part, InnerScalars *sc) {
     if (DOT && sc->all_done) return;
     const f4v *__restrict__ pa = reinterpret_cast<const f4v *>(bval);
     const f4v *__restrict__ pb = reinterpret_cast<const f4v *>(bval + 4 * nsl);
-    const float *__restrict__ pc = bval + 8 * nsl;
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v *__restrict__ pc = reinterpret_cast<const f2v *>(bval + 8 * nsl);
     double dot[K];
 #pragma unroll
     for (int j = 0; j < K; j++) dot[j] = 0.0;
@@ -323,7 +334,7 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
             load_k<K>(x + 2 * K, pv[2]);
             const f4v va = __builtin_nontemporal_load(&pa[p]);
             const f4v vb = __builtin_nontemporal_load(&pb[p]);
-            const float vc = __builtin_nontemporal_load(&pc[p]);
+            const float vc = __builtin_nontemporal_load(&pc[p]).x;
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 acc[0][j] += va.x * pv[0][j] + va.y * pv[1][j] + va.z * pv[2][j];
@@ -342,10 +353,11 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t p = base + 64 * (int64_t)(1 + b * U + u);
-                rj[u] = __builtin_nontemporal_load(&col[p]);
+                const f2v cj = __builtin_nontemporal_load(&pc[p]);
                 ra[u] = __builtin_nontemporal_load(&pa[p]);
                 rb[u] = __builtin_nontemporal_load(&pb[p]);
-                rc[u] = __builtin_nontemporal_load(&pc[p]);
+                rc[u] = cj.x;
+                rj[u] = __float_as_int(cj.y);
             }
         };
         if (nfull > 0) load_batch(0, va, vb, vc, J);
@@ -365,7 +377,8 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
         int jb = 1 + nfull * U;
         for (; jb < w; jb++) {
             const int64_t p = base + 64 * (int64_t)jb;
-            fma_block(pa[p], pb[p], pc[p], col[p]);
+            const f2v cj = pc[p];
+            fma_block(pa[p], pb[p], cj.x, __float_as_int(cj.y));
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -641,7 +654,7 @@ static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
             const int grid = spmm_grid<K, false>(ctx, m->nb, c);
 #define AXG_T(UU)                                                                                \
     hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
-                       m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, P, AP,  \
+                       m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots, P, AP,  \
                        nullptr, nullptr)
             AXG_SPMM_SWITCH(c, AXG_T)
             AXG_HIP(hipEventRecord(ctx->ev0, s));
@@ -754,7 +767,7 @@ static void bench_spmm_k(axg_mesh *m, int reps, double *avg_ms) {
     const int grid = spmm_grid<K, false>(ctx, m->nb, shp);
 #define AXG_T(UU)                                                                                \
     hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
-                       m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots, P.p,    \
+                       m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots, P.p,    \
                        AP.p, nullptr, nullptr)
     AXG_SPMM_SWITCH(shp, AXG_T)
     AXG_HIP(hipEventRecord(ctx->ev0, s));
@@ -826,12 +839,11 @@ struct MixedSolver {
     if (e0)                                                                                     \
         hipExtLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s,  \
                               e0, e1, 0, nb, (const int64_t *)m->bs_sptr.p,                     \
-                              (const int32_t *)m->bs_width.p, (const int32_t *)m->bs_col.p,     \
-                              (const float *)m->bval.p, m->bs_nslots, (const float *)w.f3.p,   \
-                              w.f4.p, w.part.p, dsc.p);                                         \
+                              (const int32_t *)m->bs_width.p, (const float *)m->bval.p,          \
+                              m->bs_nslots, (const float *)w.f3.p, w.f4.p, w.part.p, dsc.p);    \
     else                                                                                        \
         hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s, nb,  \
-                           m->bs_sptr.p, m->bs_width.p, m->bs_col.p, m->bval.p, m->bs_nslots,    \
+                           m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots,    \
                            w.f3.p, w.f4.p, w.part.p, dsc.p)
             AXG_SPMM_SWITCH(shp, AXG_SPMM)
 #undef AXG_SPMM
