@@ -253,6 +253,37 @@ class Mesh:
         check(_lib.lib().axg_full_diagonal(self._h, dptr(d)))
         return d
 
+    def element_volumes(self):
+        """Element volumes of the last FP64 assembly (Gauss weight x det J, point order)."""
+        v = np.zeros(self.nelem)
+        check(_lib.lib().axg_element_volumes(self._h, dptr(v)))
+        return v
+
+    def lumped_mass(self, density):
+        """Lumped mass over all DOFs: density_e |vol_e| / nn per node DOF, element order."""
+        rho = f64(np.broadcast_to(np.asarray(density, np.float64), (self.nelem,))).copy()
+        m = np.zeros(3 * self.nnode)
+        check(_lib.lib().axg_lumped_mass(self._h, dptr(rho), dptr(m)))
+        return m
+
+    def eigen(self, m_diag, nmodes, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
+              inner_tol=1e-3, vectors=True):
+        """Lowest nmodes of K_ff phi = lambda diag(m_diag) phi by subspace iteration
+        (eigen.rs:12-111). Returns (values, vectors (k, n_free) or None, sweeps, info)."""
+        m = f64(m_diag).ravel()
+        if m.size != self.n_free:
+            raise ValueError("m_diag must have n_free entries")
+        vals = np.zeros(nmodes)
+        vecs = np.zeros((nmodes, self.n_free)) if vectors else None
+        nf, sw = C.c_int32(), C.c_int32()
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol)
+        check(_lib.lib().axg_eigen(self._h, dptr(m), nmodes, C.byref(o), dptr(vals),
+                                   dptr(vecs) if vectors else None, C.byref(nf), C.byref(sw),
+                                   C.byref(info)))
+        k = nf.value
+        return vals[:k], (vecs[:k] if vectors else None), sw.value, _info_dict(info)
+
     def compliance_adjoint(self, u_full, f_full):
         """c = f^T u and dc/dE_e = -u_e^T K1_e u_e (K1: Ke at unit modulus)."""
         u = f64(u_full).ravel()

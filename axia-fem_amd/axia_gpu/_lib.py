@@ -115,10 +115,17 @@ SIGNATURES = {
     "axg_reactions": (C.c_int, [_vp, _dp, _dp, _dp]),
     "axg_reduce_rhs": (C.c_int, [_vp, _dp, _dp, _dp]),
     "axg_full_diagonal": (C.c_int, [_vp, _dp]),
+    "axg_element_volumes": (C.c_int, [_vp, _dp]),
+    "axg_lumped_mass": (C.c_int, [_vp, _dp, _dp]),
+    "axg_eigen": (C.c_int, [_vp, _dp, _i32, C.POINTER(SolveOpts), _dp, _dp, _ip, _ip,
+                            C.POINTER(SolveInfo)]),
     "axg_deck_parse": (C.c_int, [C.c_char_p, _i64, C.POINTER(_vp)]),
     "axg_deck_destroy": (C.c_int, [_vp]),
     "axg_deck_info": (C.c_int, [_vp, C.POINTER(C.c_int32), _lp, _lp, _lp]),
     "axg_deck_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _ip, _dp, _dp, _lp, _dp, _dp]),
+    "axg_deck_nmodes": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "axg_deck_frequency": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _ip, _ip,
+                                     C.POINTER(SolveInfo)]),
     "axg_deck_solve": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _dp,
                                  C.POINTER(SolveInfo)]),
 }

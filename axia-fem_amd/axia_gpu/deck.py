@@ -4,7 +4,9 @@ Deck(text) parses the keyword subset of the reference's reader (inp.rs) and
 builds solve_linear's setup (analysis.rs:548-946) in libaxia_gpu; Deck.solve
 runs the rest of solve_linear on the device (assembly, reduced rhs, PCG,
 reconstruction, reactions, nodal stresses) and returns a LinearResult that
-mirrors SolveOutput (analysis.rs:24-54).
+mirrors SolveOutput (analysis.rs:24-54). Deck.frequency runs the *FREQUENCY
+branch (analysis.rs:955-1005): lumped mass, subspace iteration on the device,
+frequencies and mass-normalised mode shapes.
 """
 from __future__ import annotations
 
@@ -35,6 +37,19 @@ class LinearResult:
         if i >= len(self.node_ids) or self.node_ids[i] != nid:
             raise KeyError(nid)
         return i
+
+
+@dataclass
+class FrequencyResult:
+    node_ids: np.ndarray     # (nnode,) ascending node ids
+    frequencies: np.ndarray  # (k,) cycles per time, sqrt(lambda) / (2 pi)
+    eigenvalues: np.ndarray  # (k,) lambda = omega^2
+    modes: np.ndarray        # (k, 3 nnode) mass-normalised mode shapes
+    sweeps: int
+    solver: str
+    iters: int
+    nfree: int
+    ndof: int
 
 
 class Deck:
@@ -96,7 +111,50 @@ class Deck:
                             self.n_free, ndof)
 
 
+    @property
+    def nmodes(self):
+        """Modes the deck's *FREQUENCY step asks for (0 without one)."""
+        n = C.c_int32()
+        check(_lib.lib().axg_deck_nmodes(self._h, C.byref(n)))
+        return n.value
+
+    def frequency(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
+                  inner_tol=1e-3, modes=True):
+        from . import _info_dict, _opts, default_context
+        ctx = ctx or default_context()
+        k = self.nmodes
+        if k <= 0:
+            raise ValueError("the deck has no *FREQUENCY step")
+        ndof = 3 * self.nnode
+        freqs, vals = np.zeros(k), np.zeros(k)
+        shapes = np.zeros((k, ndof)) if modes else None
+        nf, sw = C.c_int32(), C.c_int32()
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol)
+        check(_lib.lib().axg_deck_frequency(ctx.handle, self._h, C.byref(o), dptr(freqs),
+                                            dptr(vals), dptr(shapes) if modes else None,
+                                            C.byref(nf), C.byref(sw), C.byref(info)))
+        inf = _info_dict(info)
+        self._refresh()
+        n = nf.value
+        return FrequencyResult(self.arrays()["node_ids"], freqs[:n], vals[:n],
+                               shapes[:n] if modes else None, sw.value, inf["name"], inf["iters"],
+                               self.n_free, ndof)
+
+    def _refresh(self):
+        nn, nnode, nelem, nfree = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(_lib.lib().axg_deck_info(self._h, C.byref(nn), C.byref(nnode), C.byref(nelem),
+                                       C.byref(nfree)))
+        self.n_free = nfree.value
+
+
 def solve_deck(text, ctx=None, **kw):
     """Parse and solve a deck: (Deck, LinearResult)."""
     d = Deck(text)
     return d, d.solve(ctx=ctx, **kw)
+
+
+def frequency_deck(text, ctx=None, **kw):
+    """Parse a *FREQUENCY deck and run it: (Deck, FrequencyResult)."""
+    d = Deck(text)
+    return d, d.frequency(ctx=ctx, **kw)

@@ -382,6 +382,7 @@ int axg_mesh_create(axg_ctx *ctx, int32_t nn, int64_t nnode, const double *xyz, 
     mesh->slot_nz.alloc(std::max<int64_t>(mesh->nnz_topo, 1));
     mesh->dsel.alloc(std::max<int64_t>(n_free, 1));
     mesh->ke.alloc((size_t)nelem * m * m);
+    mesh->vol.alloc(std::max<int64_t>(nelem, 1));
     AXG_HIP(hipStreamSynchronize(s));
     mesh->symbolic_s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -771,6 +772,34 @@ int axg_full_diagonal(axg_mesh *mesh, double *diag) {
     API_BEGIN
     AXG_REQUIRE(mesh && diag, AXG_E_INVALID, "axg_full_diagonal: NULL argument");
     row_apply(mesh, 2, nullptr, nullptr, diag);
+    API_END
+}
+
+int axg_element_volumes(axg_mesh *mesh, double *vol) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && vol, AXG_E_INVALID, "axg_element_volumes: NULL argument");
+    AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    AXG_HIP(hipSetDevice(mesh->ctx->device));
+    hipStream_t s = mesh->ctx->stream;
+    if (mesh->nelem > 0)
+        AXG_HIP(hipMemcpyAsync(vol, mesh->vol.p, 8 * mesh->nelem, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    API_END
+}
+
+int axg_lumped_mass(axg_mesh *mesh, const double *density, double *m_full) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && density && m_full, AXG_E_INVALID, "axg_lumped_mass: NULL argument");
+    AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    AXG_HIP(hipSetDevice(mesh->ctx->device));
+    hipStream_t s = mesh->ctx->stream;
+    DevBuf<double> rho(std::max<int64_t>(mesh->nelem, 1)), dm(std::max<int64_t>(mesh->ndof, 1));
+    if (mesh->nelem > 0)
+        AXG_HIP(hipMemcpyAsync(rho.p, density, 8 * mesh->nelem, hipMemcpyHostToDevice, s));
+    launch_lumped_mass(mesh, s, rho.p, dm.p);
+    if (mesh->ndof > 0)
+        AXG_HIP(hipMemcpyAsync(m_full, dm.p, 8 * mesh->ndof, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
     API_END
 }
 

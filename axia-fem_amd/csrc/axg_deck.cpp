@@ -185,6 +185,8 @@ struct axg_deck {
     std::vector<DeckElem> elements;
     std::map<std::string, std::vector<int32_t>> nsets, elsets;
     std::map<std::string, std::pair<double, double>> materials;
+    std::map<std::string, double> densities;  // *DENSITY per material
+    int32_t nmodes = 0;                       // *FREQUENCY n (0: static step)
     std::map<std::string, std::string> elset_material;
     std::vector<NodeLoad> bcs, cloads, dloads;
     std::vector<Deferred> deferred;
@@ -192,7 +194,7 @@ struct axg_deck {
     // linear-static setup (full DOF space)
     int nn = 0;
     int64_t nnode = 0, nelem = 0, ndof = 0, n_free = 0;
-    std::vector<double> xyz, e, nu, u0, f_full;
+    std::vector<double> xyz, e, nu, rho, u0, f_full;
     std::vector<int32_t> conn;
     std::vector<int64_t> ind_of;
     std::vector<uint8_t> prescribed;
@@ -206,17 +208,24 @@ struct axg_deck {
         throw Error{AXG_E_INVALID, "Unbekanntes Knotenset " + name};
     }
 
-    // model.rs:1391-1407
-    std::pair<double, double> material_for(const std::string &elset) const {
+    // model.rs:1391-1407: the section's material, else the only one, else STEEL
+    const std::string &material_name_for(const std::string &elset) const {
         auto m = elset_material.find(elset);
         if (m != elset_material.end()) {
             auto it = materials.find(m->second);
-            if (it != materials.end()) return it->second;
+            if (it != materials.end()) return it->first;
         }
-        if (materials.size() == 1) return materials.begin()->second;
+        if (materials.size() == 1) return materials.begin()->first;
         auto st = materials.find("STEEL");
-        if (st != materials.end()) return st->second;
+        if (st != materials.end()) return st->first;
         throw Error{AXG_E_INVALID, "Kein Material für ELSET=" + elset};
+    }
+    std::pair<double, double> material_for(const std::string &elset) const {
+        return materials.at(material_name_for(elset));
+    }
+    double density_for(const std::string &elset) const {
+        auto it = densities.find(material_name_for(elset));
+        return it == densities.end() ? 0.0 : it->second;
     }
 };
 
@@ -504,6 +513,21 @@ void parse_deck(axg_deck &m, const std::string &text) {
             }
             nu = std::min(std::max(nu, -0.49), 0.49);
             m.materials[have_mat && !cur_mat.empty() ? cur_mat : "MATERIAL-1"] = {e, nu};
+        } else if (kw == "*DENSITY") {
+            // inp.rs:675-686
+            i = collect(i + 1, toks);
+            AXG_REQUIRE(!toks.empty(), AXG_E_INVALID, "*DENSITY ohne Wert");
+            double rho = 0.0;
+            AXG_REQUIRE(parse_f64(toks[0], rho), AXG_E_INVALID, "Keine Zahl: " + toks[0]);
+            const std::string name = have_mat && !cur_mat.empty() ? cur_mat : "MATERIAL-1";
+            m.materials.emplace(name, std::make_pair(0.0, 0.0));
+            m.densities[name] = rho;
+        } else if (kw == "*FREQUENCY") {
+            // inp.rs:1062-1071: n modes, 10 when absent or unreadable, at least 1
+            i = collect(i + 1, toks);
+            int32_t nm = 10;
+            if (!toks.empty() && !parse_i32(toks[0], nm)) nm = 10;
+            m.nmodes = std::max(nm, 1);
         } else if (kw == "*SOLID SECTION") {
             const std::string elset = param("ELSET", "EALL");
             if (params.count("MATERIAL")) m.elset_material[elset] = params["MATERIAL"];
@@ -593,6 +617,7 @@ void linear_setup(axg_deck &m) {
     m.conn.assign((size_t)m.nelem * nn, 0);
     m.e.assign(m.nelem, 0.0);
     m.nu.assign(m.nelem, 0.0);
+    m.rho.assign(m.nelem, 0.0);
     std::unordered_map<int32_t, int64_t> eidx;
     for (int64_t k = 0; k < m.nelem; k++) {
         const DeckElem &el = m.elements[k];
@@ -607,6 +632,7 @@ void linear_setup(axg_deck &m) {
         auto mat = m.material_for(el.elset);
         m.e[k] = mat.first;
         m.nu[k] = mat.second;
+        m.rho[k] = m.density_for(el.elset);
     }
     auto node_index = [&](int32_t nid) {
         auto it = m.id_to_index.find(nid);
@@ -735,68 +761,121 @@ int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double
     API_END
 }
 
+// Assembly of the deck's K_ff on the device with autospc_null_diag
+// (analysis.rs:2164-2228): DOFs whose full diagonal is at most 1e-14 of the
+// largest are pinned at zero and the DOF map rebuilt before the final assembly.
+struct DeckMesh {
+    axg_mesh *m = nullptr;
+    ~DeckMesh() {
+        if (m) axg_mesh_destroy(m);
+    }
+};
+
+static void check_status(int st) {
+    if (st != AXG_OK) throw Error{st, axg_last_error()};
+}
+
+static void deck_mesh(axg_ctx *ctx, axg_deck &d, DeckMesh &g) {
+    auto build = [&]() {
+        if (g.m) axg_mesh_destroy(g.m), g.m = nullptr;
+        check_status(axg_mesh_create(ctx, d.nn, d.nnode, d.xyz.data(), d.nelem, d.conn.data(),
+                                     d.e.data(), d.nu.data(), d.ind_of.data(), &g.m));
+        check_status(axg_assemble(g.m, AXG_PREC_FP64, nullptr));
+    };
+    build();
+    std::vector<double> diag(d.ndof);
+    check_status(axg_full_diagonal(g.m, diag.data()));
+    double max_d = 0.0;
+    for (double v : diag) max_d = std::max(max_d, std::fabs(v));
+    bool more = false;
+    if (max_d >= 1e-30) {
+        const double tol = max_d * 1e-14;
+        for (int64_t i = 0; i < d.ndof; i++)
+            if (std::fabs(diag[i]) <= tol && !d.prescribed[i]) {
+                d.prescribed[i] = 1;
+                d.u0[i] = 0.0;
+                more = true;
+            }
+    }
+    if (more) {
+        number_dofs(d);
+        build();
+    }
+}
+
+// DofMap::reconstruct (constraint.rs:207-218)
+static void reconstruct(const axg_deck &d, const double *x, double *u_full) {
+    for (int64_t i = 0; i < d.ndof; i++) {
+        u_full[i] = d.u0[i];
+        if (d.ind_of[i] >= 0) u_full[i] = u_full[i] + 1.0 * x[d.ind_of[i]];
+    }
+}
+
+int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes) {
+    API_BEGIN
+    AXG_REQUIRE(deck && nmodes, AXG_E_INVALID, "axg_deck_nmodes: NULL argument");
+    *nmodes = deck->nmodes;
+    API_END
+}
+
 // solve_linear for the deck (analysis.rs:548-1162): assembly on the device,
-// autospc_null_diag from the full diagonal (rebuilding the DOF map when it
-// pins more DOFs), reduced right-hand side, GPU PCG, reconstruction,
-// reactions and nodal stresses.
+// autospc, reduced right-hand side, GPU PCG, reconstruction, reactions and
+// nodal stresses.
 int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
                    double *rf, double *stress, double *vm, axg_solve_info *info) {
     API_BEGIN
     AXG_REQUIRE(ctx && deck && u_full, AXG_E_INVALID, "axg_deck_solve: invalid argument");
     axg_deck &d = *deck;
     const int precision = opts ? opts->precision : AXG_PREC_FP64;
-    auto check = [](int st) {
-        if (st != AXG_OK) throw Error{st, axg_last_error()};
-    };
-    struct MeshGuard {
-        axg_mesh *m = nullptr;
-        ~MeshGuard() {
-            if (m) axg_mesh_destroy(m);
-        }
-    } g;
-    auto build = [&]() {
-        if (g.m) axg_mesh_destroy(g.m), g.m = nullptr;
-        check(axg_mesh_create(ctx, d.nn, d.nnode, d.xyz.data(), d.nelem, d.conn.data(), d.e.data(),
-                              d.nu.data(), d.ind_of.data(), &g.m));
-        check(axg_assemble(g.m, AXG_PREC_FP64, nullptr));
-    };
-    build();
-    // autospc_null_diag (analysis.rs:2164-2228): DOFs whose diagonal is at most
-    // 1e-14 of the largest are pinned at zero
-    {
-        std::vector<double> diag(d.ndof);
-        check(axg_full_diagonal(g.m, diag.data()));
-        double max_d = 0.0;
-        for (double v : diag) max_d = std::max(max_d, std::fabs(v));
-        bool more = false;
-        if (max_d >= 1e-30) {
-            const double tol = max_d * 1e-14;
-            for (int64_t i = 0; i < d.ndof; i++)
-                if (std::fabs(diag[i]) <= tol && !d.prescribed[i]) {
-                    d.prescribed[i] = 1;
-                    d.u0[i] = 0.0;
-                    more = true;
-                }
-        }
-        if (more) {
-            number_dofs(d);
-            build();
-        }
-    }
-    if (precision == AXG_PREC_MIXED) check(axg_assemble(g.m, AXG_PREC_MIXED, nullptr));
+    DeckMesh g;
+    deck_mesh(ctx, d, g);
+    if (precision == AXG_PREC_MIXED) check_status(axg_assemble(g.m, AXG_PREC_MIXED, nullptr));
     std::vector<double> rhs(std::max<int64_t>(d.n_free, 1)), x(std::max<int64_t>(d.n_free, 1));
-    check(axg_reduce_rhs(g.m, d.f_full.data(), d.u0.data(), rhs.data()));
+    check_status(axg_reduce_rhs(g.m, d.f_full.data(), d.u0.data(), rhs.data()));
     axg_solve_info si;
     std::memset(&si, 0, sizeof si);
-    if (d.n_free > 0) check(axg_mesh_solve(g.m, rhs.data(), 1, opts, x.data(), &si));
-    // DofMap::reconstruct (constraint.rs:207-218)
-    for (int64_t i = 0; i < d.ndof; i++) {
-        u_full[i] = d.u0[i];
-        if (d.ind_of[i] >= 0) u_full[i] = u_full[i] + 1.0 * x[d.ind_of[i]];
-    }
-    if (rf) check(axg_reactions(g.m, u_full, d.f_full.data(), rf));
-    if (stress) check(axg_nodal_stress(g.m, u_full, stress, vm, nullptr));
+    if (d.n_free > 0) check_status(axg_mesh_solve(g.m, rhs.data(), 1, opts, x.data(), &si));
+    reconstruct(d, x.data(), u_full);
+    if (rf) check_status(axg_reactions(g.m, u_full, d.f_full.data(), rf));
+    if (stress) check_status(axg_nodal_stress(g.m, u_full, stress, vm, nullptr));
     if (info) *info = si;
+    API_END
+}
+
+// The *FREQUENCY branch of solve_linear (analysis.rs:955-1005) for a deck
+// without consistent-mass terms: lumped mass (analysis.rs:669-717) restricted
+// to the free DOFs, subspace iteration on the device, f = sqrt(lambda) / 2 pi
+// (0 for lambda <= 0), mode shapes reconstructed over the full DOF space
+// (mass-normalised, the reference's default EigNorm::Mass).
+int axg_deck_frequency(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *freqs,
+                       double *values, double *modes_full, int32_t *nfound, int32_t *sweeps,
+                       axg_solve_info *info) {
+    API_BEGIN
+    AXG_REQUIRE(ctx && deck && freqs && nfound, AXG_E_INVALID,
+                "axg_deck_frequency: invalid argument");
+    axg_deck &d = *deck;
+    AXG_REQUIRE(d.nmodes > 0, AXG_E_INVALID, "axg_deck_frequency: the deck has no *FREQUENCY step");
+    DeckMesh g;
+    deck_mesh(ctx, d, g);
+    AXG_REQUIRE(d.n_free > 0, AXG_E_INVALID, "Eigenproblem ohne freie DOF / ohne Moden.");
+    std::vector<double> m_full(d.ndof), m_ind(d.n_free, 0.0);
+    check_status(axg_lumped_mass(g.m, d.rho.data(), m_full.data()));
+    for (int64_t i = 0; i < d.ndof; i++)
+        if (d.ind_of[i] >= 0) m_ind[d.ind_of[i]] += 1.0 * 1.0 * m_full[i];
+    bool any = false;
+    for (double v : m_ind) any = any || v > 0.0;
+    AXG_REQUIRE(any, AXG_E_INVALID, "*FREQUENCY: *DENSITY fehlt oder Masse ist null.");
+    const int32_t nm = d.nmodes;
+    std::vector<double> vals(nm), vecs(modes_full ? (size_t)nm * d.n_free : 0);
+    int32_t k = 0;
+    check_status(axg_eigen(g.m, m_ind.data(), nm, opts, vals.data(),
+                           modes_full ? vecs.data() : nullptr, &k, sweeps, info));
+    for (int32_t j = 0; j < k; j++) {
+        freqs[j] = vals[j] > 0.0 ? std::sqrt(vals[j]) / (2.0 * M_PI) : 0.0;
+        if (values) values[j] = vals[j];
+        if (modes_full) reconstruct(d, vecs.data() + (size_t)j * d.n_free, modes_full + (size_t)j * d.ndof);
+    }
+    *nfound = k;
     API_END
 }
 

@@ -1,0 +1,294 @@
+// Lowest modes of K phi = lambda M phi with a diagonal (lumped) M by inverse
+// subspace iteration, after eigen.rs:12-111 (subspace_gen): nvec = nmodes + 4
+// vectors from the same seeded generator, at most 40 sweeps of
+//   Y = K^-1 M X (one multi-right-hand-side PCG solve for all vectors),
+//   M-orthonormalise Y (modified Gram-Schmidt), Kr = Y^T K Y, Kr Q = Q diag(l)
+//   (cyclic Jacobi on the host, nvec x nvec), X = Y Q, M-orthonormalise X,
+// stopping once every Ritz value moved by at most 1e-8 (1 + |l|) after sweep 5.
+// The vectors live on the device (nvec contiguous vectors of n_free); the
+// direct solve of the reference becomes the GPU PCG chosen by the options.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "axg_internal.h"
+#include "axg_pcg.h"
+#include "axg_reduce.h"
+
+namespace axg {
+
+constexpr int EBS = 256;   // workgroup size of the vector kernels
+constexpr int NVMAX = 64;  // most vectors one multi-dot handles
+
+// Partials of sum_i a_i w_i b_i (w may be null: weight 1), one per workgroup.
+__global__ void __launch_bounds__(EBS) wdot_kernel(int64_t n, const double *__restrict__ a,
+                                                   const double *__restrict__ w,
+                                                   const double *__restrict__ b,
+                                                   double *__restrict__ part) {
+    double s[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * EBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * EBS)
+        s[0] += w ? a[i] * w[i] * b[i] : a[i] * b[i];
+    group_sums<1, EBS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Partials of y_k . v for every vector k < nvec (y: nvec vectors of n).
+template <int NV>
+__global__ void __launch_bounds__(EBS) multi_dot_kernel(int64_t n, int nvec,
+                                                        const double *__restrict__ y,
+                                                        const double *__restrict__ v,
+                                                        double *__restrict__ part) {
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * EBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * EBS) {
+        const double vi = v[i];
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if (k < nvec) s[k] += y[(int64_t)k * n + i] * vi;
+    }
+    group_sums<NV, EBS>(s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nvec; k++) part[(size_t)blockIdx.x * nvec + k] = s[k];
+}
+
+// a -= c b  (c from the host)
+__global__ void axpy_kernel(int64_t n, double c, const double *__restrict__ b,
+                            double *__restrict__ a) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] -= c * b[i];
+}
+
+__global__ void scale_kernel(int64_t n, double c, double *__restrict__ a) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] /= c;
+}
+
+// out_k = (m .* x_k) * scale for every vector (scale: a power of two)
+__global__ void mass_apply_kernel(int64_t n, int nvec, const double *__restrict__ m,
+                                  const double *__restrict__ x, double scale,
+                                  double *__restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nvec) return;
+    out[t] = (m[t % n] * x[t]) * scale;
+}
+
+// x_j = sum_k y_k q[k][j] over the nonzero q (eigen.rs:63-74)
+__global__ void combine_kernel(int64_t n, int nvec, const double *__restrict__ y,
+                               const double *__restrict__ q, double *__restrict__ x) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int j = 0; j < nvec; j++) {
+        double acc = 0.0;
+        for (int k = 0; k < nvec; k++) {
+            const double c = q[k * nvec + j];
+            if (c == 0.0) continue;
+            acc += y[(int64_t)k * n + i] * c;
+        }
+        x[(int64_t)j * n + i] = acc;
+    }
+}
+
+// Cyclic-by-largest Jacobi rotations of a dense symmetric matrix (row-major),
+// eigen.rs:208-266; q receives the eigenvectors as columns.
+static void jacobi_eigen(std::vector<double> &a, int n, std::vector<double> &vals,
+                         std::vector<double> &q) {
+    q.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) q[i * n + i] = 1.0;
+    for (int it = 0; it < 40 * std::max(n, 1); it++) {
+        int p = 0, r = 1;
+        double best = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++)
+                if (std::fabs(a[i * n + j]) > best) best = std::fabs(a[i * n + j]), p = i, r = j;
+        if (best < 1e-14) break;
+        const double app = a[p * n + p], aqq = a[r * n + r], apq = a[p * n + r];
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+        for (int i = 0; i < n; i++) {
+            if (i == p || i == r) continue;
+            const double aip = a[i * n + p], aiq = a[i * n + r];
+            a[i * n + p] = a[p * n + i] = c * aip - s * aiq;
+            a[i * n + r] = a[r * n + i] = s * aip + c * aiq;
+        }
+        a[p * n + p] = c * c * app - 2.0 * s * c * apq + s * s * aqq;
+        a[r * n + r] = s * s * app + 2.0 * s * c * apq + c * c * aqq;
+        a[p * n + r] = a[r * n + p] = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double vp = q[i * n + p], vq = q[i * n + r];
+            q[i * n + p] = c * vp - s * vq;
+            q[i * n + r] = s * vp + c * vq;
+        }
+    }
+    vals.resize(n);
+    for (int i = 0; i < n; i++) vals[i] = a[i * n + i];
+}
+
+}  // namespace axg
+
+using namespace axg;
+
+extern "C" int axg_eigen(axg_mesh *mesh, const double *m_diag, int32_t nmodes,
+                         const axg_solve_opts *opts, double *values, double *vectors,
+                         int32_t *nfound, int32_t *sweeps, axg_solve_info *info) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && m_diag && values && nfound, AXG_E_INVALID, "axg_eigen: NULL argument");
+    AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    const int64_t n = mesh->n_free;
+    AXG_REQUIRE(n > 0 && nmodes > 0, AXG_E_INVALID, "Eigenproblem ohne freie DOF / ohne Moden.");
+    const int64_t nm = std::min<int64_t>(nmodes, n);
+    const int nvec = (int)std::max(std::min<int64_t>(nm + 4, n), nm);
+    AXG_REQUIRE(nvec <= NVMAX, AXG_E_INVALID, "axg_eigen: at most 60 modes");
+    axg_ctx *ctx = mesh->ctx;
+    AXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    axg_solve_opts o;
+    axg_default_opts(&o);
+    if (opts) o = *opts;
+
+    const size_t nv = (size_t)n * nvec;
+    DevBuf<double> dm(n), X(nv), Y(nv), R(nv), KY(n), dq((size_t)nvec * nvec);
+    const int grid = (int)std::min<int64_t>((n + EBS - 1) / EBS, (int64_t)ctx->cu_count * 4);
+    DevBuf<double> part((size_t)grid * nvec);
+    std::vector<double> hpart((size_t)grid * nvec);
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    AXG_HIP(hipMemcpyAsync(dm.p, m_diag, 8 * n, hipMemcpyHostToDevice, s));
+
+    // start block (eigen.rs:24-34)
+    {
+        std::vector<double> hx(nv);
+        uint64_t rng = 1;
+        for (int v = 0; v < nvec; v++)
+            for (int64_t i = 0; i < n; i++) {
+                rng = rng * 6364136223846793005ull + 1ull;
+                double val = (double)(rng >> 33) / (double)(1ull << 31) - 1.0;
+                hx[(size_t)v * n + i] = m_diag[i] <= 0.0 ? 0.0 : val;
+            }
+        AXG_HIP(hipMemcpyAsync(X.p, hx.data(), 8 * nv, hipMemcpyHostToDevice, s));
+        AXG_HIP(hipStreamSynchronize(s));
+    }
+    auto wdot = [&](const double *a, const double *w, const double *b) {
+        hipLaunchKernelGGL(wdot_kernel, dim3(grid), dim3(EBS), 0, s, n, a, w, b, part.p);
+        AXG_HIP(hipMemcpyAsync(hpart.data(), part.p, 8 * grid, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (int g = 0; g < grid; g++) acc += hpart[g];
+        return acc;
+    };
+    // eigen.rs:181-206
+    auto m_orthonormalize = [&](double *x) {
+        for (int a = 0; a < nvec; a++) {
+            double *xa = x + (size_t)a * n;
+            for (int b = 0; b < a; b++) {
+                const double *xb = x + (size_t)b * n;
+                const double d = wdot(xa, dm.p, xb);
+                hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s, n, d, xb, xa);
+            }
+            const double nrm = std::sqrt(wdot(xa, dm.p, xa));
+            if (nrm > 1e-30) hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s, n, nrm, xa);
+        }
+        AXG_HIP(hipGetLastError());
+    };
+    m_orthonormalize(X.p);
+
+    CsrView A = solver_csr(mesh);
+    // Power-of-two scale of the right-hand sides M X. The PCG stops with "not
+    // positive definite" when |p^T K p| < 1e-30 (an absolute guard); with lumped
+    // masses of order 1e-10 the curvatures of a converging solve fall below it.
+    // Scaling to R^T D^-1 R ~ 1 (D: the Jacobi divisor; an M-orthonormal x gives
+    // R^T D^-1 R ~ mean(m / D)) keeps them far above it, and a power of two
+    // changes no rounding: the iterates are exactly 2^k times the unscaled ones
+    // and the M-orthonormalisation that follows removes the factor exactly.
+    double rhs_scale = 1.0;
+    {
+        std::vector<double> hd(n);
+        AXG_HIP(hipMemcpyAsync(hd.data(), mesh->dsel.p, 8 * n, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (int64_t i = 0; i < n; i++) acc += m_diag[i] / hd[i];
+        acc /= (double)n;
+        if (acc > 0.0 && std::isfinite(acc))
+            rhs_scale = std::ldexp(1.0, -(int)std::lround(0.5 * std::log2(acc)));
+    }
+    std::vector<double> vals(nvec, 0.0), ev, q, kr((size_t)nvec * nvec);
+    int it = 0;
+    int64_t solve_iters = 0;
+    double solve_ms = 0.0, worst_res = 0.0, worst_rel = 0.0;
+    for (; it < 40; it++) {
+        hipLaunchKernelGGL(mass_apply_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s,
+                           n, nvec, dm.p, X.p, rhs_scale, R.p);
+        axg_solve_info si;
+        const int st = axg_mesh_solve_device(mesh, R.p, nvec, &o, Y.p, &si);
+        if (st != AXG_OK) throw Error{st, axg_last_error()};
+        solve_iters += si.iters;
+        solve_ms += si.solve_ms;
+        if (si.rel_residual >= worst_rel) worst_rel = si.rel_residual, worst_res = si.residual;
+        m_orthonormalize(Y.p);
+        // Kr = Y^T K Y, column j from K y_j
+        for (int j = 0; j < nvec; j++) {
+            spmv_fp64(ctx, A, Y.p + (size_t)j * n, KY.p);
+            hipLaunchKernelGGL((multi_dot_kernel<NVMAX>), dim3(grid), dim3(EBS), 0, s, n, nvec, Y.p,
+                               KY.p, part.p);
+            AXG_HIP(hipMemcpyAsync(hpart.data(), part.p, 8 * (size_t)grid * nvec,
+                                   hipMemcpyDeviceToHost, s));
+            AXG_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < nvec; i++) {
+                double acc = 0.0;
+                for (int g = 0; g < grid; g++) acc += hpart[(size_t)g * nvec + i];
+                kr[(size_t)i * nvec + j] = acc;
+            }
+        }
+        jacobi_eigen(kr, nvec, ev, q);
+        AXG_HIP(hipMemcpyAsync(dq.p, q.data(), 8 * q.size(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(combine_kernel, dim3(gn), dim3(256), 0, s, n, nvec, Y.p, dq.p, X.p);
+        AXG_HIP(hipStreamSynchronize(s));
+        m_orthonormalize(X.p);
+        bool conv = true;
+        for (int i = 0; i < nvec; i++) {
+            if (std::fabs(ev[i] - vals[i]) > 1e-8 * (1.0 + std::fabs(ev[i]))) conv = false;
+            vals[i] = ev[i];
+        }
+        if (conv && it > 4) {
+            it++;
+            break;
+        }
+    }
+    if (it > 40) it = 40;
+    // ascending, the first nmodes (eigen.rs:88-110)
+    std::vector<int> order(nvec);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return vals[a] < vals[b]; });
+    std::vector<double> hx;
+    if (vectors) {
+        hx.resize(nv);
+        AXG_HIP(hipMemcpyAsync(hx.data(), X.p, 8 * nv, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+    }
+    int32_t k = 0;
+    for (int r = 0; r < (int)nm; r++) {
+        const int i = order[r];
+        if (!std::isfinite(vals[i])) continue;
+        values[k] = vals[i];
+        if (vectors) std::copy(hx.begin() + (size_t)i * n, hx.begin() + (size_t)(i + 1) * n,
+                               vectors + (size_t)k * n);
+        k++;
+    }
+    AXG_REQUIRE(k > 0, AXG_E_INVALID, "Keine Eigenwerte gefunden.");
+    *nfound = k;
+    if (sweeps) *sweeps = it;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iters = solve_iters;
+        info->outer_iters = it;
+        info->nrhs = nvec;
+        info->residual = worst_res;
+        info->rel_residual = worst_rel;
+        info->solve_ms = solve_ms;
+        std::snprintf(info->name, sizeof(info->name), "subspace (%d vectors, GPU PCG)", nvec);
+    }
+    API_END
+}

@@ -229,6 +229,7 @@ __global__ void __launch_bounds__(256) ke_fp64_kernel(const double *__restrict__
                                                       const double *__restrict__ E,
                                                       const double *__restrict__ NU, int64_t nelem,
                                                       double *__restrict__ ke,
+                                                      double *__restrict__ vol,
                                                       int32_t *__restrict__ err) {
     constexpr int M = 3 * NN;
     __shared__ double s_dndx[EB][NG][3 * NN];
@@ -278,6 +279,13 @@ __global__ void __launch_bounds__(256) ke_fp64_kernel(const double *__restrict__
         s_w[el][g] = (NN == 8) ? det : c_tab.h20_w[g] * det;
     }
     __syncthreads();
+    // element volume: the weights summed over the points in order (the
+    // reference's kef.volume, elem.rs:199-219, quadratic.rs:122-146)
+    if (tid < EB && e0 + tid < nelem) {
+        double v = 0.0;
+        for (int g = 0; g < NG; g++) v += s_w[tid][g];
+        vol[e0 + tid] = v;
+    }
     for (int t = tid; t < EB * NN * NN; t += blockDim.x) {
         int el = t / (NN * NN), ab = t % (NN * NN);
         int a = ab / NN, b = ab % NN;
@@ -304,12 +312,12 @@ void launch_ke_fp64(axg_mesh *m, hipStream_t s, int32_t *d_err) {
         constexpr int EB = 16;
         dim3 grid((unsigned)((m->nelem + EB - 1) / EB));
         hipLaunchKernelGGL((ke_fp64_kernel<8, 8, EB>), grid, dim3(256), 0, s, m->xyz.p, m->conn.p,
-                           m->e.p, m->nu.p, m->nelem, m->ke.p, d_err);
+                           m->e.p, m->nu.p, m->nelem, m->ke.p, m->vol.p, d_err);
     } else {
         constexpr int EB = 2;
         dim3 grid((unsigned)((m->nelem + EB - 1) / EB));
         hipLaunchKernelGGL((ke_fp64_kernel<20, 27, EB>), grid, dim3(256), 0, s, m->xyz.p,
-                           m->conn.p, m->e.p, m->nu.p, m->nelem, m->ke.p, d_err);
+                           m->conn.p, m->e.p, m->nu.p, m->nelem, m->ke.p, m->vol.p, d_err);
     }
     AXG_HIP(hipGetLastError());
 }
@@ -589,10 +597,37 @@ void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double 
                            m->conn.p, m->nu.p, m->nelem, d_u, d_energy, d_err);
     AXG_HIP(hipGetLastError());
 }
+// Lumped mass of solid elements (analysis.rs:669-717): density x |volume| of each
+// element, 1/nn of it on each translational DOF of each of its nodes, summed per
+// node over the incident elements in ascending element order.
+__global__ void lumped_mass_kernel(const int64_t *__restrict__ inc_ptr,
+                                   const int64_t *__restrict__ inc_slot, int64_t nnode, int nn,
+                                   const double *__restrict__ vol, const double *__restrict__ rho,
+                                   double *__restrict__ m_full) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnode) return;
+    const double w = 1.0 / nn;
+    double acc = 0.0;
+    for (int64_t k = inc_ptr[i]; k < inc_ptr[i + 1]; k++) {
+        const int64_t el = inc_slot[k] / nn;
+        const double r = rho[el], v = vol[el];
+        if (!(fabs(r) > 0.0) || !(fabs(v) > 0.0)) continue;
+        acc += r * fabs(v) * w;
+    }
+    for (int d = 0; d < 3; d++) m_full[i * 3 + d] = acc;
+}
 
 }  // namespace axg
 
 namespace axg {
+void launch_lumped_mass(axg_mesh *m, hipStream_t s, const double *d_rho, double *d_m_full) {
+    if (m->nnode == 0) return;
+    dim3 grid((unsigned)((m->nnode + 255) / 256));
+    hipLaunchKernelGGL(lumped_mass_kernel, grid, dim3(256), 0, s, m->inc_ptr.p, m->inc_slot.p,
+                       m->nnode, m->nn, m->vol.p, d_rho, d_m_full);
+    AXG_HIP(hipGetLastError());
+}
+
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
                          double *d_stress, double *d_vm, int32_t *d_err) {
     if (m->nelem > 0) {

@@ -181,6 +181,7 @@ struct axg_mesh {
     // values
     axg::DevBuf<double> val;       // FP64 K_ff values (nnz)
     axg::DevBuf<double> ke;        // FP64 element matrices (nelem * m * m)
+    axg::DevBuf<double> vol;       // element volumes (sum of Gauss weights x det J)
     axg::DevBuf<uint8_t> slot_nz;  // per topological slot: any nonzero contribution
     axg::DevBuf<double> dsel;      // Jacobi divisor (diag or 1.0)
     axg::DevBuf<int64_t> inc_ptr, inc_slot;  // node -> (element * nn + local), ascending element
@@ -237,6 +238,7 @@ void upload_shape_tables32(const double *h8_dn, const double *h20_dn, const doub
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
                          double *d_stress, double *d_vm, int32_t *d_err);
 // mode: 0 reactions, 1 reduced right-hand side, 2 full diagonal
+void launch_lumped_mass(axg_mesh *m, hipStream_t s, const double *d_rho, double *d_m_full);
 void launch_row_apply(axg_mesh *m, hipStream_t s, int mode, const double *d_w,
                       const double *d_f, double *d_out);
 void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double *d_energy,

@@ -192,10 +192,29 @@ int axg_full_diagonal(axg_mesh *mesh, double *diag);
 int axg_compliance_adjoint(axg_mesh *mesh, const double *u_full, const double *f_full, double *c,
                            double *dc_de);
 
+/* ------------------------------------------------- eigen (*FREQUENCY) */
+/* Element volumes (nelem) of the last FP64 assembly: the sum over the Gauss
+ * points of weight x det J, in point order, as the stiffness loop of
+ * element_ke accumulates it (elem.rs:199-219, quadratic.rs:122-146). */
+int axg_element_volumes(axg_mesh *mesh, double *vol);
+/* Lumped mass (analysis.rs:669-717) over the full DOF space (3*nnode):
+ * density[e] x |vol_e| / nn on each translational DOF of each node of e,
+ * summed per node in element order. density: nelem entries. */
+int axg_lumped_mass(axg_mesh *mesh, const double *density, double *m_full);
+/* eigen::subspace_gen (eigen.rs:12-111): the lowest nmodes of
+ * K_ff phi = lambda M phi with M = diag(m_diag) (n_free entries), by subspace
+ * iteration over nmodes + 4 vectors with the solve of opts (GPU PCG) in place
+ * of the direct factorisation. values: ascending (nmodes); vectors: nmodes x
+ * n_free, M-orthonormal (may be NULL); nfound: number written; sweeps: the
+ * subspace sweeps taken (may be NULL); info: summed solver work (may be NULL). */
+int axg_eigen(axg_mesh *mesh, const double *m_diag, int32_t nmodes, const axg_solve_opts *opts,
+              double *values, double *vectors, int32_t *nfound, int32_t *sweeps,
+              axg_solve_info *info);
+
 /* ------------------------------------------------------- deck ingestion */
 /* Parses an INP deck (the keyword subset of inp.rs:383-1061: *NODE, *ELEMENT
  * TYPE=C3D8|C3D20, *NSET/*ELSET (GENERATE, nested names), *MATERIAL, *ELASTIC,
- * *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn) into the compacted model
+ * *DENSITY, *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn, *FREQUENCY) into the compacted model
  * (model.rs:1234-1309) and solve_linear's setup on the host
  * (analysis.rs:548-946): arrays, prescribed values (last BC wins), isolated
  * nodes pinned, load vector, free-DOF numbering (DofMap::build). text need not
@@ -215,6 +234,19 @@ int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double
  * and von Mises (nnode) (both may be NULL). */
 int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
                    double *rf, double *stress, double *vm, axg_solve_info *info);
+
+/* Number of modes the deck's *FREQUENCY step asks for (inp.rs:1062-1071;
+ * 0 when the deck has no *FREQUENCY step). */
+int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes);
+/* The *FREQUENCY branch of solve_linear (analysis.rs:955-1005) without a
+ * preload: assembly and autospc as in axg_deck_solve, lumped mass from the
+ * materials' *DENSITY, axg_eigen with opts, freqs = sqrt(lambda) / (2 pi)
+ * (0 for lambda <= 0). freqs, values (may be NULL): nmodes entries;
+ * modes_full (may be NULL): nmodes x 3*nnode mass-normalised mode shapes;
+ * nfound: modes written; sweeps, info as for axg_eigen. */
+int axg_deck_frequency(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *freqs,
+                       double *values, double *modes_full, int32_t *nfound, int32_t *sweeps,
+                       axg_solve_info *info);
 
 /* Batched solves: nmesh meshes (assembled), each with nrhs[m] right-hand
  * sides rhs[m] (n_free[m] * nrhs[m], column-major by right-hand side),

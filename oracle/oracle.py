@@ -368,6 +368,8 @@ class Model:
     dloads: list = field(default_factory=list)  # (elem id, face, mag)
     id_to_index: dict = field(default_factory=dict)
     deferred: list = field(default_factory=list)
+    densities: dict = field(default_factory=dict)  # material name -> density (*DENSITY)
+    nmodes: int = 0  # *FREQUENCY n (0: static step)
 
     def node_index(self, nid):
         return self.id_to_index[nid]
@@ -382,16 +384,22 @@ class Model:
             return list(self.node_ids)
         raise ValueError(f"Unbekanntes Knotenset {name}")
 
-    def material_for(self, elset):
-        """model.rs:1391-1407."""
+    def material_name_for(self, elset):
+        """model.rs:1391-1407: the section's material, else the only one, else STEEL."""
         m = self.elset_material.get(elset)
         if m is not None and m in self.materials:
-            return self.materials[m]
+            return m
         if len(self.materials) == 1:
-            return next(iter(self.materials.values()))
+            return next(iter(self.materials))
         if "STEEL" in self.materials:
-            return self.materials["STEEL"]
+            return "STEEL"
         raise ValueError(f"Kein Material für ELSET={elset}")
+
+    def material_for(self, elset):
+        return self.materials[self.material_name_for(elset)]
+
+    def density_for(self, elset):
+        return self.densities.get(self.material_name_for(elset), 0.0)
 
     # -- arrays for the hot path
     def mesh_arrays(self):
@@ -530,6 +538,25 @@ def parse_inp(text):
                 e = cands[0] if cands else 1.0
             nu = min(max(nu, -0.49), 0.49)
             m.materials[cur_mat or "MATERIAL-1"] = (e, nu)
+        elif kw == "*DENSITY":
+            toks, i = collect(i + 1)
+            if not toks:
+                raise ValueError("*DENSITY ohne Wert")
+            rho = _parse_f64(toks[0])
+            if rho is None:
+                raise ValueError(f"Keine Zahl: {toks[0]}")
+            name = cur_mat or "MATERIAL-1"
+            m.materials.setdefault(name, (0.0, 0.0))
+            m.densities[name] = rho
+        elif kw == "*FREQUENCY":
+            toks, i = collect(i + 1)
+            nm = 10
+            if toks:
+                try:
+                    nm = _parse_i32(toks[0])
+                except ValueError:
+                    nm = 10
+            m.nmodes = max(nm, 1)
         elif kw == "*SOLID SECTION":
             elset = params.get("ELSET", "EALL")
             if "MATERIAL" in params:
@@ -868,3 +895,142 @@ def solve_deck(text, solver="auto", tol=1e-8):
     """solve_native for the decks on this path: parse -> setup -> solve."""
     m = parse_inp(text)
     return m, linear_solve(deck_setup(m), solver, tol)
+
+
+# ------------------------------------------------------------- eigenvalues
+def lumped_mass(nn, xyz, conn, density):
+    """analysis.rs:669-717 for solid elements: mass = density |volume| of each element (the
+    volume summed with the stiffness' Gauss loop), 1/nn of it on each translational DOF of each
+    node, elements in order."""
+    xyz = _f64(xyz).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, np.int32).reshape(-1, nn)
+    m_full = np.zeros(3 * xyz.shape[0])
+    w = 1.0 / nn
+    for el in range(conn.shape[0]):
+        rho = float(density[el])
+        if not abs(rho) > 0.0:
+            continue
+        _, vol = hex_stiffness(xyz[conn[el]], 1.0, 0.0)
+        if not abs(vol) > 0.0:
+            continue
+        mnode = rho * abs(vol) * w
+        for a in conn[el]:
+            for d in range(3):
+                m_full[3 * a + d] += mnode
+    return m_full
+
+
+def lcg_start_vectors(n, nvec, m_diag):
+    """eigen.rs:24-34: the seeded start block (LCG, zero where the mass is not positive)."""
+    x = np.zeros((nvec, n))
+    rng = 1
+    mask = (1 << 64) - 1
+    for v in range(nvec):
+        for i in range(n):
+            rng = (rng * 6364136223846793005 + 1) & mask
+            x[v, i] = ((rng >> 33) / float(1 << 31)) - 1.0
+            if m_diag[i] <= 0.0:
+                x[v, i] = 0.0
+    return x
+
+
+def m_orthonormalize(x, m):
+    """eigen.rs:181-206: modified Gram-Schmidt in the diagonal M inner product."""
+    for a in range(x.shape[0]):
+        for b in range(a):
+            dot = float(np.sum(x[a] * m * x[b]))
+            x[a] -= dot * x[b]
+        s = math.sqrt(float(np.sum(x[a] * m * x[a])))
+        if s > 1e-30:
+            x[a] /= s
+    return x
+
+
+def jacobi_eigen(a):
+    """eigen.rs:208-266: cyclic-by-largest Jacobi rotations of a dense symmetric matrix.
+    Returns (eigenvalues, eigenvectors as columns)."""
+    a = np.array(a, dtype=np.float64, copy=True)
+    n = a.shape[0]
+    v = np.eye(n)
+    for _ in range(40 * max(n, 1)):
+        best, p, q = 0.0, 0, 1
+        for i in range(n):
+            for j in range(i + 1, n):
+                if abs(a[i, j]) > best:
+                    best, p, q = abs(a[i, j]), i, j
+        if best < 1e-14:
+            break
+        app, aqq, apq = a[p, p], a[q, q], a[p, q]
+        tau = (aqq - app) / (2.0 * apq)
+        t = (1.0 if tau >= 0.0 else -1.0) / (abs(tau) + math.sqrt(1.0 + tau * tau))
+        c = 1.0 / math.sqrt(1.0 + t * t)
+        s_ = t * c
+        for i in range(n):
+            if i != p and i != q:
+                aip, aiq = a[i, p], a[i, q]
+                a[i, p] = a[p, i] = c * aip - s_ * aiq
+                a[i, q] = a[q, i] = s_ * aip + c * aiq
+        a[p, p] = c * c * app - 2.0 * s_ * c * apq + s_ * s_ * aqq
+        a[q, q] = s_ * s_ * app + 2.0 * s_ * c * apq + c * c * aqq
+        a[p, q] = a[q, p] = 0.0
+        vp, vq = v[:, p].copy(), v[:, q].copy()
+        v[:, p] = c * vp - s_ * vq
+        v[:, q] = s_ * vp + c * vq
+    return np.diag(a).copy(), v
+
+
+def subspace_gen(csr, m_diag, nmodes, solve=None):
+    """eigen.rs:12-111: lowest nmodes of K phi = lambda M phi (diagonal M) by inverse subspace
+    iteration: nvec = nmodes + 4 vectors, 40 sweeps at most, Rayleigh-Ritz with Jacobi,
+    stop when every Ritz value moved by at most 1e-8 (1 + |lambda|) after sweep 5.
+    solve(B) returns K^-1 B for a block B (rows = vectors); the reference calls solve_kff
+    (faer LLT) here, the default is a dense Cholesky (small n)."""
+    n = csr.n
+    if n == 0 or nmodes == 0:
+        raise ValueError("Eigenproblem ohne freie DOF / ohne Moden.")
+    nvec = min(max(min(nmodes, n) + 4, min(nmodes, n)), n)
+    m = _f64(m_diag)
+    if solve is None:
+        lchol = np.linalg.cholesky(csr.to_dense())
+
+        def solve(B):
+            y = np.linalg.solve(lchol, B.T)
+            return np.linalg.solve(lchol.T, y).T
+    x = m_orthonormalize(lcg_start_vectors(n, nvec, m), m)
+    values = np.zeros(nvec)
+    k = csr.to_scipy()
+    for it in range(40):
+        y = m_orthonormalize(solve(x * m[None, :]), m)
+        kr = y @ (k @ y.T)
+        evals, evecs = jacobi_eigen(kr)
+        nx = m_orthonormalize(evecs.T @ y, m)
+        conv = bool(np.all(np.abs(evals - values) <= 1e-8 * (1.0 + np.abs(evals))))
+        values = evals
+        x = nx
+        if conv and it > 4:
+            break
+    order = np.argsort(values, kind="stable")[:min(nmodes, nvec)]
+    keep = [i for i in order if np.isfinite(values[i])]
+    if not keep:
+        raise ValueError("Keine Eigenwerte gefunden.")
+    return values[keep], x[keep], it + 1
+
+
+def frequency_deck(text):
+    """*FREQUENCY for an all-hex deck (analysis.rs:955-1000): K_ff, lumped M restricted to
+    the free DOFs, subspace iteration, f = sqrt(lambda) / (2 pi)."""
+    model = parse_inp(text)
+    s = deck_setup(model)
+    dens = np.array([model.density_for(el[3]) for el in model.elements])
+    m_full = lumped_mass(s.nn, s.xyz, s.conn, dens)
+    m_ind = np.zeros(s.n_free)
+    free = s.ind_of >= 0
+    np.add.at(m_ind, s.ind_of[free], 1.0 * 1.0 * m_full[free])
+    if np.all(m_ind <= 0.0):
+        raise ValueError("*FREQUENCY: *DENSITY fehlt oder Masse ist null.")
+    ti, tj, tv = s.trips
+    oi, oj, ov, _ = reduce(s.ind_of, s.u0, s.n_free, ti, tj, tv, s.f_full)
+    csr = csr_from_triplets(s.n_free, oi, oj, ov)
+    vals, vecs, sweeps = subspace_gen(csr, m_ind, max(model.nmodes, 1))
+    freqs = np.array([math.sqrt(v) / (2.0 * math.pi) if v > 0.0 else 0.0 for v in vals])
+    return model, s, csr, m_ind, vals, vecs, freqs, sweeps

@@ -1,0 +1,144 @@
+"""*FREQUENCY on the device against the oracle's restatement of eigen.rs and the
+lumped mass of analysis.rs: element volumes and lumped mass bit for bit, the
+lowest eigenvalues of the subspace iteration against the oracle (direct solves)
+and against LAPACK's dense generalized eigensolver, mode shapes M-orthonormal
+and solving K phi = lambda M phi, and closed forms through the deck path
+(fixed-free bar, Euler-Bernoulli cantilever)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import axia_gpu as G
+from axia_gpu import deck as D
+from axia_gpu import meshgen as MG
+from eigen_decks import bar_deck, mesh_deck
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+RHO = 7.85e-9
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def jittered(nn, nx, ny, nz, seed, j=0.2):
+    """A brick with interior corner nodes moved (midside nodes at edge midpoints)."""
+    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-j, j, size=(nx + 1, ny + 1, nz + 1, 3))
+    d[0], d[nx], d[:, 0], d[:, ny], d[:, :, 0], d[:, :, nz] = 0, 0, 0, 0, 0, 0
+    lo, hi = np.floor(xyz).astype(int), np.ceil(xyz).astype(int)
+    xyz = xyz + 0.5 * (d[lo[:, 0], lo[:, 1], lo[:, 2]] + d[hi[:, 0], hi[:, 1], hi[:, 2]])
+    return xyz, conn
+
+
+def cantilever_mesh(ctx, nn, nx, ny, nz, dims=(10.0, 1.0, 0.5), e=210000.0, nu=0.3):
+    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(nx, ny, nz)
+    xyz = xyz * np.array([dims[0] / nx, dims[1] / ny, dims[2] / nz])
+    ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 0] == 0.0)[0])
+    m = G.Mesh(nn, xyz, conn, e, nu, ind, ctx=ctx)
+    m.assemble("fp64")
+    return m, xyz, conn, ind
+
+
+@pytest.mark.parametrize("nn,shape,seed", [(8, (5, 4, 3), 1), (20, (3, 2, 2), 2)])
+def test_volumes_and_lumped_mass_bitexact(ctx, nn, shape, seed):
+    say("case: volumes/mass", nn, shape)
+    xyz, conn = jittered(nn, *shape, seed)
+    ind = MG.free_map(len(xyz), [0])
+    m = G.Mesh(nn, xyz, conn, 1000.0, 0.25, ind, ctx=ctx)
+    m.assemble("fp64")
+    vol = m.element_volumes()
+    ref = np.array([O.hex_stiffness(xyz[c], 1.0, 0.0)[1] for c in conn])
+    assert np.array_equal(vol, ref)
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(1e-9, 8e-9, len(conn))
+    rho[::4] = 0.0
+    assert np.array_equal(m.lumped_mass(rho), O.lumped_mass(nn, xyz, conn, rho))
+    # total mass: 3 rho V summed over elements
+    assert math.isclose(m.lumped_mass(rho).sum(), 3.0 * float(rho @ np.abs(ref)), rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("nn,shape,nmodes", [(8, (8, 3, 2), 3), (20, (6, 2, 1), 4)])
+def test_eigen_vs_oracle_and_lapack(ctx, nn, shape, nmodes):
+    say("case: eigen", nn, shape, nmodes)
+    m, xyz, conn, ind = cantilever_mesh(ctx, nn, *shape)
+    mfull = m.lumped_mass(np.full(len(conn), RHO))
+    mdiag = np.zeros(m.n_free)
+    free = ind >= 0
+    mdiag[ind[free]] = mfull[free]
+    vals, vecs, sweeps, info = m.eigen(mdiag, nmodes, tol=1e-11)
+    assert len(vals) == nmodes and info["nrhs"] == nmodes + 4
+    csr = O.Csr(m.n_free, *m.csr())
+    ref_vals, ref_vecs, ref_sweeps = O.subspace_gen(csr, mdiag, nmodes)
+    say("  values", vals, "oracle", ref_vals, "sweeps", sweeps, ref_sweeps)
+    assert np.allclose(vals, ref_vals, rtol=1e-9, atol=0)
+    dense = sla.eigh(csr.to_dense(), np.diag(mdiag), eigvals_only=True)[:nmodes]
+    assert np.allclose(vals, dense, rtol=1e-8, atol=0)
+    # M-orthonormal shapes solving the eigen-equation; same shapes as the oracle's up to sign
+    k = csr.to_scipy()
+    g = (vecs * mdiag) @ vecs.T
+    assert np.allclose(g, np.eye(nmodes), atol=1e-9)
+    for lam, v, rv in zip(vals, vecs, ref_vecs):
+        r = k @ v - lam * mdiag * v
+        assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(k @ v)
+        assert abs(abs(float((v * mdiag) @ rv)) - 1.0) < 1e-7
+
+
+def test_eigen_mixed_solves(ctx):
+    """The FP32-operator solver inside the subspace iteration gives the same modes on a
+    block whose stiffness FP32 resolves (a stubby 6 x 6 x 6 clamped block; a slender
+    cantilever's condition number puts the FP32 inner solves at their rounding floor)."""
+    m, xyz, conn, ind = cantilever_mesh(ctx, 8, 6, 6, 6, dims=(6.0, 6.0, 6.0))
+    mfull = m.lumped_mass(RHO)
+    mdiag = mfull[ind >= 0]
+    v64, _, _, _ = m.eigen(mdiag, 3, tol=1e-11)
+    m.assemble("mixed")
+    vmx, _, _, info = m.eigen(mdiag, 3, precision="mixed", tol=1e-10, vectors=False)
+    say("  fp64", v64, "mixed", vmx, info["name"])
+    assert np.allclose(vmx, v64, rtol=1e-9, atol=0)
+
+
+def test_deck_axial_bar_closed_form(ctx):
+    """Fixed-free bar, axial modes: f_k = (2k - 1) c / (4 L), c = sqrt(E / rho)."""
+    text = bar_deck(40, nmodes=2)
+    dk, res = D.frequency_deck(text, ctx=ctx)
+    _, _, _, _, vals, _, freqs, _ = O.frequency_deck(text)
+    say("  freqs", res.frequencies, "oracle", freqs)
+    assert np.allclose(res.frequencies, freqs, rtol=1e-9, atol=0)
+    c = math.sqrt(210000.0 / RHO)
+    f1 = c / 400.0
+    assert abs(res.frequencies[0] - f1) / f1 < 1e-3
+    assert abs(res.frequencies[1] - 3 * f1) / (3 * f1) < 1e-2
+    assert res.modes.shape == (2, 3 * dk.nnode)
+    assert np.all(res.modes[:, :12] == 0.0)  # the clamped end
+
+
+def test_deck_cantilever_c3d20_euler_bernoulli(ctx):
+    """First bending frequency of a C3D20 cantilever (10 x 1 x 0.5, nu = 0) against
+    Euler-Bernoulli, 1.8751^2 / (2 pi) sqrt(E I / (rho A L^4)), and the oracle."""
+    nx, ny, nz = 20, 2, 2
+    xyz, conn = MG.brick_hex20(nx, ny, nz)
+    xyz = xyz * np.array([10.0 / nx, 1.0 / ny, 0.5 / nz])
+    text = mesh_deck(20, xyz, conn, np.nonzero(xyz[:, 0] == 0.0)[0], nu=0.0, nmodes=4)
+    dk, res = D.frequency_deck(text, ctx=ctx, tol=1e-11)
+    _, _, _, _, vals, _, freqs, _ = O.frequency_deck(text)
+    say("  freqs", res.frequencies, "oracle", freqs, "sweeps", res.sweeps)
+    assert np.allclose(res.frequencies, freqs, rtol=1e-9, atol=0)
+    f1 = 1.875104 ** 2 / (2 * math.pi) * math.sqrt(210000.0 * (0.5 ** 3 / 12) / (RHO * 0.5 * 1e4))
+    assert abs(res.frequencies[0] / f1 - 1.0) < 1e-2
+    assert np.allclose(res.eigenvalues, (2 * math.pi * res.frequencies) ** 2, rtol=1e-14)
+
+
+def test_deck_frequency_errors(ctx):
+    with pytest.raises(G.AxgError, match="DENSITY"):
+        D.frequency_deck(bar_deck(3).replace("*DENSITY\n7.85e-9\n", ""), ctx=ctx)
+    dk = D.Deck(bar_deck(3).replace("*FREQUENCY\n3\n", ""))
+    assert dk.nmodes == 0
+    with pytest.raises(ValueError):
+        dk.frequency(ctx=ctx)
