@@ -284,6 +284,41 @@ class Mesh:
         k = nf.value
         return vals[:k], (vecs[:k] if vectors else None), sw.value, _info_dict(info)
 
+    def geometric_stiffness(self, u_full):
+        """(element mean stresses (nelem, 6), Kg_ff values on the topological pattern)."""
+        u = f64(u_full).ravel()
+        mean = np.zeros((self.nelem, 6))
+        nnz_t = self._nnz_topo()
+        kg = np.zeros(max(nnz_t, 1))
+        check(_lib.lib().axg_geometric_stiffness(self._h, dptr(u), dptr(mean), dptr(kg)))
+        return mean, kg[:nnz_t]
+
+    def _nnz_topo(self):
+        if getattr(self, "_pattern", None) is None:
+            self._pattern = pattern_host(self.nn, self.nnode, self.conn, self.ind_of)
+        return len(self._pattern[1])
+
+    def topological_pattern(self):
+        """(rowptr, colind) of K_ff before exact-zero slots are dropped."""
+        self._nnz_topo()
+        return self._pattern[0], self._pattern[1]
+
+    def buckle(self, u_full, nmodes, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
+               inner_tol=1e-3, vectors=True):
+        """Buckling factors of K_ff phi = lambda (-Kg_ff) phi for the displacements u_full
+        (eigen.rs:113-179). Returns (values, vectors (k, n_free) or None, info)."""
+        u = f64(u_full).ravel()
+        vals = np.zeros(nmodes)
+        vecs = np.zeros((nmodes, self.n_free)) if vectors else None
+        nf = C.c_int32()
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol)
+        check(_lib.lib().axg_buckle(self._h, dptr(u), nmodes, C.byref(o), dptr(vals),
+                                    dptr(vecs) if vectors else None, C.byref(nf),
+                                    C.byref(info)))
+        k = nf.value
+        return vals[:k], (vecs[:k] if vectors else None), _info_dict(info)
+
     def compliance_adjoint(self, u_full, f_full):
         """c = f^T u and dc/dE_e = -u_e^T K1_e u_e (K1: Ke at unit modulus)."""
         u = f64(u_full).ravel()

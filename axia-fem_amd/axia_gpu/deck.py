@@ -6,7 +6,9 @@ runs the rest of solve_linear on the device (assembly, reduced rhs, PCG,
 reconstruction, reactions, nodal stresses) and returns a LinearResult that
 mirrors SolveOutput (analysis.rs:24-54). Deck.frequency runs the *FREQUENCY
 branch (analysis.rs:955-1005): lumped mass, subspace iteration on the device,
-frequencies and mass-normalised mode shapes.
+frequencies and mass-normalised mode shapes. Deck.buckle runs the *BUCKLE
+branch (analysis.rs:1176-1260): the static solution of the step's loads, the
+geometric stiffness of its element mean stresses and the buckling factors.
 """
 from __future__ import annotations
 
@@ -46,6 +48,18 @@ class FrequencyResult:
     eigenvalues: np.ndarray  # (k,) lambda = omega^2
     modes: np.ndarray        # (k, 3 nnode) mass-normalised mode shapes
     sweeps: int
+    solver: str
+    iters: int
+    nfree: int
+    ndof: int
+
+
+@dataclass
+class BuckleResult:
+    node_ids: np.ndarray     # (nnode,) ascending node ids
+    factors: np.ndarray      # (k,) buckling load factors, ascending |lambda|
+    modes: np.ndarray        # (k, 3 nnode) buckling shapes (unit length over the free DOFs)
+    u_static: np.ndarray     # (3 nnode,) the preload's displacements
     solver: str
     iters: int
     nfree: int
@@ -115,8 +129,38 @@ class Deck:
     def nmodes(self):
         """Modes the deck's *FREQUENCY step asks for (0 without one)."""
         n = C.c_int32()
-        check(_lib.lib().axg_deck_nmodes(self._h, C.byref(n)))
+        check(_lib.lib().axg_deck_nmodes(self._h, C.byref(n), None))
         return n.value
+
+    @property
+    def nbuckle(self):
+        """Buckling factors the deck's *BUCKLE step asks for (0 without one)."""
+        n = C.c_int32()
+        check(_lib.lib().axg_deck_nmodes(self._h, None, C.byref(n)))
+        return n.value
+
+    def buckle(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
+               inner_tol=1e-3, modes=True):
+        from . import _info_dict, _opts, default_context
+        ctx = ctx or default_context()
+        k = self.nbuckle
+        if k <= 0:
+            raise ValueError("the deck has no *BUCKLE step")
+        ndof = 3 * self.nnode
+        fac = np.zeros(k)
+        shapes = np.zeros((k, ndof)) if modes else None
+        u = np.zeros(ndof)
+        nf = C.c_int32()
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol)
+        check(_lib.lib().axg_deck_buckle(ctx.handle, self._h, C.byref(o), dptr(fac),
+                                         dptr(shapes) if modes else None, dptr(u), C.byref(nf),
+                                         C.byref(info)))
+        inf = _info_dict(info)
+        self._refresh()
+        n = nf.value
+        return BuckleResult(self.arrays()["node_ids"], fac[:n], shapes[:n] if modes else None, u,
+                            inf["name"], inf["iters"], self.n_free, ndof)
 
     def frequency(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
                   inner_tol=1e-3, modes=True):
@@ -158,3 +202,9 @@ def frequency_deck(text, ctx=None, **kw):
     """Parse a *FREQUENCY deck and run it: (Deck, FrequencyResult)."""
     d = Deck(text)
     return d, d.frequency(ctx=ctx, **kw)
+
+
+def buckle_deck(text, ctx=None, **kw):
+    """Parse a *BUCKLE deck and run it: (Deck, BuckleResult)."""
+    d = Deck(text)
+    return d, d.buckle(ctx=ctx, **kw)

@@ -106,6 +106,34 @@ double true_residual(axg_ctx *ctx, const CsrView &A, const double *d_x, const do
 
 using namespace axg;
 
+namespace axg {
+// Kg_ff on the topological pattern for displacements d_u (full DOF space, on
+// the device): extrapolated nodal stresses, the element mean stresses, the
+// element Kg at those stresses and the gather, times sign.
+void geometric_stiffness_device(axg_mesh *mesh, const double *d_u, double sign, double *d_val,
+                                double *d_mean) {
+    axg_ctx *ctx = mesh->ctx;
+    hipStream_t s = ctx->stream;
+    Workspace &w = ctx->ws;
+    DevBuf<double> den((size_t)std::max<int64_t>(mesh->nelem, 1) * mesh->nn * 6),
+        dst((size_t)std::max<int64_t>(mesh->nnode, 1) * 6), dvm(std::max<int64_t>(mesh->nnode, 1)),
+        mean((size_t)std::max<int64_t>(mesh->nelem, 1) * 6),
+        kg((size_t)std::max<int64_t>(mesh->nelem, 1) * mesh->nn * mesh->nn);
+    AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
+    launch_nodal_stress(mesh, s, d_u, den.p, dst.p, dvm.p, w.err.p);
+    launch_element_mean(mesh, s, den.p, mean.p);
+    launch_kg(mesh, s, mean.p, kg.p, w.err.p);
+    launch_gather_kg(mesh, s, kg.p, sign, d_val);
+    int32_t err[2];
+    AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
+    if (d_mean && mesh->nelem > 0)
+        AXG_HIP(hipMemcpyAsync(d_mean, mean.p, 8 * (size_t)mesh->nelem * 6,
+                               hipMemcpyDeviceToDevice, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    if (err[0]) throw Error{err[0], element_error(err[0], mesh->nn, err[1], nullptr, nullptr)};
+}
+}  // namespace axg
+
 extern "C" {
 
 const char *axg_last_error(void) { return g_err.c_str(); }
@@ -708,6 +736,27 @@ int axg_nodal_stress(axg_mesh *mesh, const double *u_full, double *stress, doubl
                                hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
     if (err[0]) throw Error{err[0], element_error(err[0], mesh->nn, err[1], nullptr, nullptr)};
+    API_END
+}
+
+
+int axg_geometric_stiffness(axg_mesh *mesh, const double *u_full, double *mean, double *kg_val) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && u_full, AXG_E_INVALID, "axg_geometric_stiffness: NULL argument");
+    AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    axg_ctx *ctx = mesh->ctx;
+    AXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    DevBuf<double> du(std::max<int64_t>(mesh->ndof, 1)),
+        dval(std::max<int64_t>(mesh->nnz_topo, 1)),
+        dmean((size_t)std::max<int64_t>(mesh->nelem, 1) * 6);
+    AXG_HIP(hipMemcpyAsync(du.p, u_full, 8 * mesh->ndof, hipMemcpyHostToDevice, s));
+    geometric_stiffness_device(mesh, du.p, 1.0, dval.p, dmean.p);
+    if (mean && mesh->nelem > 0)
+        AXG_HIP(hipMemcpyAsync(mean, dmean.p, 8 * (size_t)mesh->nelem * 6, hipMemcpyDeviceToHost, s));
+    if (kg_val && mesh->nnz_topo > 0)
+        AXG_HIP(hipMemcpyAsync(kg_val, dval.p, 8 * mesh->nnz_topo, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
     API_END
 }
 

@@ -187,6 +187,7 @@ struct axg_deck {
     std::map<std::string, std::pair<double, double>> materials;
     std::map<std::string, double> densities;  // *DENSITY per material
     int32_t nmodes = 0;                       // *FREQUENCY n (0: static step)
+    int32_t nbuckle = 0;                      // *BUCKLE n (0: no buckling step)
     std::map<std::string, std::string> elset_material;
     std::vector<NodeLoad> bcs, cloads, dloads;
     std::vector<Deferred> deferred;
@@ -528,6 +529,12 @@ void parse_deck(axg_deck &m, const std::string &text) {
             int32_t nm = 10;
             if (!toks.empty() && !parse_i32(toks[0], nm)) nm = 10;
             m.nmodes = std::max(nm, 1);
+        } else if (kw == "*BUCKLE") {
+            // inp.rs:1072-1081: n factors, 1 when absent or unreadable, at least 1
+            i = collect(i + 1, toks);
+            int32_t nb = 1;
+            if (!toks.empty() && !parse_i32(toks[0], nb)) nb = 1;
+            m.nbuckle = std::max(nb, 1);
         } else if (kw == "*SOLID SECTION") {
             const std::string elset = param("ELSET", "EALL");
             if (params.count("MATERIAL")) m.elset_material[elset] = params["MATERIAL"];
@@ -811,10 +818,21 @@ static void reconstruct(const axg_deck &d, const double *x, double *u_full) {
     }
 }
 
-int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes) {
+// Reduced right-hand side, PCG and DofMap::reconstruct of the step's loads.
+static void static_solve(axg_deck &d, DeckMesh &g, const axg_solve_opts *opts, double *u_full,
+                         axg_solve_info *si) {
+    std::vector<double> rhs(std::max<int64_t>(d.n_free, 1)), x(std::max<int64_t>(d.n_free, 1));
+    check_status(axg_reduce_rhs(g.m, d.f_full.data(), d.u0.data(), rhs.data()));
+    std::memset(si, 0, sizeof *si);
+    if (d.n_free > 0) check_status(axg_mesh_solve(g.m, rhs.data(), 1, opts, x.data(), si));
+    reconstruct(d, x.data(), u_full);
+}
+
+int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle) {
     API_BEGIN
-    AXG_REQUIRE(deck && nmodes, AXG_E_INVALID, "axg_deck_nmodes: NULL argument");
-    *nmodes = deck->nmodes;
+    AXG_REQUIRE(deck, AXG_E_INVALID, "axg_deck_nmodes: NULL deck");
+    if (nmodes) *nmodes = deck->nmodes;
+    if (nbuckle) *nbuckle = deck->nbuckle;
     API_END
 }
 
@@ -830,12 +848,8 @@ int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, dou
     DeckMesh g;
     deck_mesh(ctx, d, g);
     if (precision == AXG_PREC_MIXED) check_status(axg_assemble(g.m, AXG_PREC_MIXED, nullptr));
-    std::vector<double> rhs(std::max<int64_t>(d.n_free, 1)), x(std::max<int64_t>(d.n_free, 1));
-    check_status(axg_reduce_rhs(g.m, d.f_full.data(), d.u0.data(), rhs.data()));
     axg_solve_info si;
-    std::memset(&si, 0, sizeof si);
-    if (d.n_free > 0) check_status(axg_mesh_solve(g.m, rhs.data(), 1, opts, x.data(), &si));
-    reconstruct(d, x.data(), u_full);
+    static_solve(d, g, opts, u_full, &si);
     if (rf) check_status(axg_reactions(g.m, u_full, d.f_full.data(), rf));
     if (stress) check_status(axg_nodal_stress(g.m, u_full, stress, vm, nullptr));
     if (info) *info = si;
@@ -873,6 +887,38 @@ int axg_deck_frequency(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts,
     for (int32_t j = 0; j < k; j++) {
         freqs[j] = vals[j] > 0.0 ? std::sqrt(vals[j]) / (2.0 * M_PI) : 0.0;
         if (values) values[j] = vals[j];
+        if (modes_full) reconstruct(d, vecs.data() + (size_t)j * d.n_free, modes_full + (size_t)j * d.ndof);
+    }
+    *nfound = k;
+    API_END
+}
+
+// The *BUCKLE branch of solve_linear (analysis.rs:1176-1260) for a solid
+// deck: the static solution of the step's loads, Kg_ff from its element mean
+// stresses, subspace_ab on (K_ff, -Kg_ff), mode shapes over the full DOF space
+// (unit Euclidean length in the free DOFs, as subspace_ab leaves them).
+int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *factors,
+                    double *modes_full, double *u_static, int32_t *nfound, axg_solve_info *info) {
+    API_BEGIN
+    AXG_REQUIRE(ctx && deck && factors && nfound, AXG_E_INVALID,
+                "axg_deck_buckle: invalid argument");
+    axg_deck &d = *deck;
+    AXG_REQUIRE(d.nbuckle > 0, AXG_E_INVALID, "axg_deck_buckle: the deck has no *BUCKLE step");
+    DeckMesh g;
+    deck_mesh(ctx, d, g);
+    AXG_REQUIRE(d.n_free > 0, AXG_E_INVALID, "Eigenproblem ohne freie DOF.");
+    std::vector<double> u(d.ndof);
+    axg_solve_info si;
+    static_solve(d, g, opts, u.data(), &si);
+    if (u_static) std::copy(u.begin(), u.end(), u_static);
+    const int32_t nb = d.nbuckle;
+    std::vector<double> vals(nb), vecs(modes_full ? (size_t)nb * d.n_free : 0);
+    int32_t k = 0;
+    check_status(axg_buckle(g.m, u.data(), nb, opts, vals.data(),
+                            modes_full ? vecs.data() : nullptr, &k, info));
+    // DofMap::reconstruct adds u0 on the prescribed DOFs (zero for a pinned support)
+    for (int32_t j = 0; j < k; j++) {
+        factors[j] = vals[j];
         if (modes_full) reconstruct(d, vecs.data() + (size_t)j * d.n_free, modes_full + (size_t)j * d.ndof);
     }
     *nfound = k;

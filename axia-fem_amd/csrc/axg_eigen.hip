@@ -292,3 +292,149 @@ extern "C" int axg_eigen(axg_mesh *mesh, const double *m_diag, int32_t nmodes,
     }
     API_END
 }
+
+// eigen::subspace_ab (eigen.rs:113-179) for *BUCKLE: K phi = lambda A phi with
+// A = -Kg_ff from the displacements u_full (analysis.rs:1176-1247). nmodes + 4
+// vectors from the seed-7 generator, each scaled to unit length; 30 sweeps of
+// Y = K^-1 A X (one multi-right-hand-side solve) with Euclidean Gram-Schmidt;
+// then lambda = v.Kv / v.Av per vector (|v.Av| < 1e-30 dropped), sorted by
+// |lambda| (stable), the first nmodes returned.
+extern "C" int axg_buckle(axg_mesh *mesh, const double *u_full, int32_t nmodes,
+                          const axg_solve_opts *opts, double *values, double *vectors,
+                          int32_t *nfound, axg_solve_info *info) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && u_full && values && nfound, AXG_E_INVALID, "axg_buckle: NULL argument");
+    AXG_REQUIRE(mesh->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    const int64_t n = mesh->n_free;
+    AXG_REQUIRE(n > 0 && nmodes > 0, AXG_E_INVALID, "Eigenproblem ohne freie DOF.");
+    const int64_t nm = std::min<int64_t>(nmodes, n);
+    const int nvec = (int)std::max(std::min<int64_t>(nm + 4, n), nm);
+    AXG_REQUIRE(nvec <= NVMAX, AXG_E_INVALID, "axg_buckle: at most 60 modes");
+    axg_ctx *ctx = mesh->ctx;
+    AXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    axg_solve_opts o;
+    axg_default_opts(&o);
+    if (opts) o = *opts;
+
+    const size_t nv = (size_t)n * nvec;
+    DevBuf<double> du(std::max<int64_t>(mesh->ndof, 1)), aval(std::max<int64_t>(mesh->nnz_topo, 1)),
+        X(nv), Y(nv), R(nv), T(n), dinv(n);
+    AXG_HIP(hipMemcpyAsync(du.p, u_full, 8 * mesh->ndof, hipMemcpyHostToDevice, s));
+    geometric_stiffness_device(mesh, du.p, -1.0, aval.p, nullptr);
+    int ashape = -1;
+    CsrView Aop;
+    Aop.n = n;
+    Aop.nnz = mesh->nnz_topo;
+    Aop.rowptr = mesh->rowptr_t.p;
+    Aop.colind = mesh->colind_t.p;
+    Aop.val = aval.p;
+    Aop.shape = &ashape;
+    CsrView K = solver_csr(mesh);
+
+    const int grid = (int)std::min<int64_t>((n + EBS - 1) / EBS, (int64_t)ctx->cu_count * 4);
+    DevBuf<double> part((size_t)grid);
+    std::vector<double> hpart(grid);
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    auto wdot = [&](const double *a, const double *w, const double *b) {
+        hipLaunchKernelGGL(wdot_kernel, dim3(grid), dim3(EBS), 0, s, n, a, w, b, part.p);
+        AXG_HIP(hipMemcpyAsync(hpart.data(), part.p, 8 * grid, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (int g = 0; g < grid; g++) acc += hpart[g];
+        return acc;
+    };
+    {
+        std::vector<double> hd(n);
+        AXG_HIP(hipMemcpyAsync(hd.data(), mesh->dsel.p, 8 * n, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; i++) hd[i] = 1.0 / hd[i];
+        AXG_HIP(hipMemcpyAsync(dinv.p, hd.data(), 8 * n, hipMemcpyHostToDevice, s));
+        // start block (eigen.rs:124-135)
+        std::vector<double> hx(nv);
+        uint64_t rng = 7;
+        for (int v = 0; v < nvec; v++) {
+            double *xv = hx.data() + (size_t)v * n;
+            for (int64_t i = 0; i < n; i++) {
+                rng = rng * 6364136223846793005ull + 1ull;
+                xv[i] = (double)(rng >> 33) / (double)(1ull << 31) - 1.0;
+            }
+            double ss = 0.0;
+            for (int64_t i = 0; i < n; i++) ss += xv[i] * xv[i];
+            const double nrm = std::max(std::sqrt(ss), 1e-30);
+            for (int64_t i = 0; i < n; i++) xv[i] /= nrm;
+        }
+        AXG_HIP(hipMemcpyAsync(X.p, hx.data(), 8 * nv, hipMemcpyHostToDevice, s));
+        AXG_HIP(hipStreamSynchronize(s));
+    }
+    int64_t solve_iters = 0;
+    double solve_ms = 0.0, worst_res = 0.0, worst_rel = 0.0;
+    double *x = X.p, *y = Y.p;
+    for (int it = 0; it < 30; it++) {
+        for (int k = 0; k < nvec; k++) {
+            double *r = R.p + (size_t)k * n;
+            spmv_fp64(ctx, Aop, x + (size_t)k * n, r);
+            // power-of-two scale to r^T D^-1 r ~ 1 (the PCG's absolute breakdown
+            // guard); exact, and removed exactly by the normalisation below
+            const double beta = wdot(r, dinv.p, r);
+            if (beta > 0.0 && std::isfinite(beta)) {
+                const double sc = std::ldexp(1.0, (int)std::lround(0.5 * std::log2(beta)));
+                hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s, n, sc, r);
+            }
+        }
+        axg_solve_info si;
+        const int st = axg_mesh_solve_device(mesh, R.p, nvec, &o, y, &si);
+        if (st != AXG_OK) throw Error{st, axg_last_error()};
+        solve_iters += si.iters;
+        solve_ms += si.solve_ms;
+        if (si.rel_residual >= worst_rel) worst_rel = si.rel_residual, worst_res = si.residual;
+        // Euclidean Gram-Schmidt (eigen.rs:143-156)
+        for (int a = 0; a < nvec; a++) {
+            double *ya = y + (size_t)a * n;
+            for (int b = 0; b < a; b++) {
+                const double *yb = y + (size_t)b * n;
+                const double d = wdot(ya, nullptr, yb);
+                hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s, n, d, yb, ya);
+            }
+            const double nrm = std::max(std::sqrt(wdot(ya, nullptr, ya)), 1e-30);
+            hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s, n, nrm, ya);
+        }
+        AXG_HIP(hipGetLastError());
+        std::swap(x, y);
+    }
+    // Rayleigh quotients (eigen.rs:159-173)
+    std::vector<std::pair<double, int>> pairs;
+    for (int k = 0; k < nvec; k++) {
+        const double *v = x + (size_t)k * n;
+        spmv_fp64(ctx, K, v, T.p);
+        const double num = wdot(v, nullptr, T.p);
+        spmv_fp64(ctx, Aop, v, T.p);
+        const double den = wdot(v, nullptr, T.p);
+        if (std::fabs(den) < 1e-30) continue;
+        pairs.push_back({num / den, k});
+    }
+    std::stable_sort(pairs.begin(), pairs.end(), [](const std::pair<double, int> &a,
+                                                    const std::pair<double, int> &b) {
+        return std::fabs(a.first) < std::fabs(b.first);
+    });
+    const int32_t take = (int32_t)std::min<int64_t>(nm, (int64_t)pairs.size());
+    for (int32_t k = 0; k < take; k++) {
+        values[k] = pairs[k].first;
+        if (vectors)
+            AXG_HIP(hipMemcpyAsync(vectors + (size_t)k * n, x + (size_t)pairs[k].second * n,
+                                   8 * n, hipMemcpyDeviceToHost, s));
+    }
+    AXG_HIP(hipStreamSynchronize(s));
+    *nfound = take;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iters = solve_iters;
+        info->outer_iters = 30;
+        info->nrhs = nvec;
+        info->residual = worst_res;
+        info->rel_residual = worst_rel;
+        info->solve_ms = solve_ms;
+        std::snprintf(info->name, sizeof(info->name), "buckle (%d vectors, GPU PCG)", nvec);
+    }
+    API_END
+}

@@ -597,6 +597,113 @@ void launch_strain_energy(axg_mesh *m, hipStream_t s, const double *d_u, double 
                            m->conn.p, m->nu.p, m->nelem, d_u, d_energy, d_err);
     AXG_HIP(hipGetLastError());
 }
+// stress_gp of solid elements (analysis.rs:1145-1151): the extrapolated nodal
+// stresses of an element, each divided by nn, summed in node order.
+__global__ void element_mean_kernel(const double *__restrict__ en, int64_t nelem, int nn,
+                                    double *__restrict__ mean) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nelem * 6) return;
+    const int64_t e = t / 6;
+    const int c = (int)(t % 6);
+    const double div = (double)nn;
+    double acc = 0.0;
+    for (int a = 0; a < nn; a++) acc += en[(e * nn + a) * 6 + c] / div;
+    mean[t] = acc;
+}
+
+// Geometric stiffness of one element under its constant mean stress
+// (eigen.rs:299-360, hex8_kg / hex20_kg with add_continuum_kg). Only the
+// direction-diagonal entries Kg[3a+d][3b+d] are nonzero and they are equal for
+// d = 0, 1, 2, so the kernel stores the nn x nn scalars
+//   kg_ab = sum over Gauss points with det > 0 of (grad N_a . s . grad N_b) w,
+// summed in point order; w = det (2x2x2) or W_i W_j W_k det (3x3x3).
+template <int NN, int NG, int EB>
+__global__ void __launch_bounds__(256) kg_kernel(const double *__restrict__ xyz,
+                                                 const int32_t *__restrict__ conn,
+                                                 const double *__restrict__ mean, int64_t nelem,
+                                                 double *__restrict__ kg,
+                                                 int32_t *__restrict__ err) {
+    __shared__ double s_dndx[EB][NG][3 * NN];
+    __shared__ double s_w[EB][NG];
+    __shared__ double s_x[EB][3 * NN];
+    __shared__ double s_sig[EB][6];
+    const int64_t e0 = (int64_t)blockIdx.x * EB;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < EB * NN; t += blockDim.x) {
+        int el = t / NN, a = t % NN;
+        int64_t e = e0 + el;
+        if (e < nelem) {
+            int64_t node = conn[e * NN + a];
+            s_x[el][3 * a + 0] = xyz[3 * node + 0];
+            s_x[el][3 * a + 1] = xyz[3 * node + 1];
+            s_x[el][3 * a + 2] = xyz[3 * node + 2];
+        }
+    }
+    for (int t = tid; t < EB * 6; t += blockDim.x) {
+        int64_t e = e0 + t / 6;
+        if (e < nelem) s_sig[t / 6][t % 6] = mean[e * 6 + t % 6];
+    }
+    __syncthreads();
+    for (int t = tid; t < EB * NG; t += blockDim.x) {
+        int el = t / NG, g = t % NG;
+        int64_t e = e0 + el;
+        if (e >= nelem) continue;
+        double det;
+        const double(*dn)[3] = (NN == 8) ? c_tab.h8_dn[g] : c_tab.h20_dn[g];
+        int st = geometry<NN>(dn, s_x[el], s_dndx[el][g], det);
+        if (st != AXG_OK) {
+            report(err, st, e);
+            det = 0.0;
+        }
+        // a point with det <= 0 adds nothing (its weight is flagged as skipped)
+        s_w[el][g] = det <= 0.0 ? -1.0 : ((NN == 8) ? det : c_tab.h20_w[g] * det);
+    }
+    __syncthreads();
+    for (int t = tid; t < EB * NN * NN; t += blockDim.x) {
+        int el = t / (NN * NN), ab = t % (NN * NN);
+        int a = ab / NN, b = ab % NN;
+        int64_t e = e0 + el;
+        if (e >= nelem) continue;
+        const double sxx = s_sig[el][0], syy = s_sig[el][1], szz = s_sig[el][2];
+        const double sxy = s_sig[el][3], syz = s_sig[el][4], szx = s_sig[el][5];
+        double k = 0.0;
+        for (int g = 0; g < NG; g++) {
+            const double w = s_w[el][g];
+            if (w < 0.0) continue;
+            const double *ga = &s_dndx[el][g][3 * a];
+            const double *gb = &s_dndx[el][g][3 * b];
+            const double gtg = ga[0] * (sxx * gb[0] + sxy * gb[1] + szx * gb[2]) +
+                               ga[1] * (sxy * gb[0] + syy * gb[1] + syz * gb[2]) +
+                               ga[2] * (szx * gb[0] + syz * gb[1] + szz * gb[2]);
+            k += gtg * w;
+        }
+        kg[e * (int64_t)(NN * NN) + ab] = k;
+    }
+}
+
+// Slot s of Kg_ff on the topological pattern: the contributions of the K
+// scatter map whose local row and column directions agree, summed in ascending
+// element order (the others are exact zeros, which the reference's triplet
+// list omits and which change no sum). sign = -1 stores A = -Kg_ff.
+__global__ void gather_kg_kernel(const uint32_t *__restrict__ slotptr,
+                                 const uint32_t *__restrict__ contrib,
+                                 const double *__restrict__ kg, int nn, int64_t nslot,
+                                 double sign, double *__restrict__ val) {
+    const uint32_t m = 3u * (uint32_t)nn, mm = m * m;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslot;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (uint32_t c = slotptr[s]; c < slotptr[s + 1]; c++) {
+            const uint32_t idx = contrib[c];
+            const uint32_t el = idx / mm, ij = idx % mm;
+            const uint32_t i = ij / m, j = ij % m;
+            if (i % 3 != j % 3) continue;
+            v += kg[(size_t)el * nn * nn + (i / 3) * nn + j / 3];
+        }
+        val[s] = sign * v;
+    }
+}
+
 // Lumped mass of solid elements (analysis.rs:669-717): density x |volume| of each
 // element, 1/nn of it on each translational DOF of each of its nodes, summed per
 // node over the incident elements in ascending element order.
@@ -625,6 +732,41 @@ void launch_lumped_mass(axg_mesh *m, hipStream_t s, const double *d_rho, double 
     dim3 grid((unsigned)((m->nnode + 255) / 256));
     hipLaunchKernelGGL(lumped_mass_kernel, grid, dim3(256), 0, s, m->inc_ptr.p, m->inc_slot.p,
                        m->nnode, m->nn, m->vol.p, d_rho, d_m_full);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_element_mean(axg_mesh *m, hipStream_t s, const double *d_elem_nodal,
+                         double *d_mean) {
+    if (m->nelem == 0) return;
+    const int64_t t = m->nelem * 6;
+    hipLaunchKernelGGL(element_mean_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s,
+                       d_elem_nodal, m->nelem, m->nn, d_mean);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_kg(axg_mesh *m, hipStream_t s, const double *d_mean, double *d_kg, int32_t *d_err) {
+    if (m->nelem == 0) return;
+    if (m->nn == 8) {
+        constexpr int EB = 16;
+        dim3 grid((unsigned)((m->nelem + EB - 1) / EB));
+        hipLaunchKernelGGL((kg_kernel<8, 8, EB>), grid, dim3(256), 0, s, m->xyz.p, m->conn.p,
+                           d_mean, m->nelem, d_kg, d_err);
+    } else {
+        constexpr int EB = 2;
+        dim3 grid((unsigned)((m->nelem + EB - 1) / EB));
+        hipLaunchKernelGGL((kg_kernel<20, 27, EB>), grid, dim3(256), 0, s, m->xyz.p, m->conn.p,
+                           d_mean, m->nelem, d_kg, d_err);
+    }
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_gather_kg(axg_mesh *m, hipStream_t s, const double *d_kg, double sign,
+                      double *d_val) {
+    const int64_t nslot = m->nnz_topo;
+    if (nslot == 0) return;
+    int64_t blocks = std::min<int64_t>((nslot + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(gather_kg_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       m->slotptr_t.p, m->contrib_t.p, d_kg, m->nn, nslot, sign, d_val);
     AXG_HIP(hipGetLastError());
 }
 

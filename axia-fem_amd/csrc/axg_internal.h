@@ -238,6 +238,11 @@ void upload_shape_tables32(const double *h8_dn, const double *h20_dn, const doub
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
                          double *d_stress, double *d_vm, int32_t *d_err);
 // mode: 0 reactions, 1 reduced right-hand side, 2 full diagonal
+void launch_element_mean(axg_mesh *m, hipStream_t s, const double *d_elem_nodal, double *d_mean);
+void launch_kg(axg_mesh *m, hipStream_t s, const double *d_mean, double *d_kg, int32_t *d_err);
+void launch_gather_kg(axg_mesh *m, hipStream_t s, const double *d_kg, double sign, double *d_val);
+void geometric_stiffness_device(axg_mesh *mesh, const double *d_u, double sign, double *d_val,
+                                double *d_mean);
 void launch_lumped_mass(axg_mesh *m, hipStream_t s, const double *d_rho, double *d_m_full);
 void launch_row_apply(axg_mesh *m, hipStream_t s, int mode, const double *d_w,
                       const double *d_f, double *d_out);

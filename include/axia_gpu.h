@@ -211,10 +211,26 @@ int axg_eigen(axg_mesh *mesh, const double *m_diag, int32_t nmodes, const axg_so
               double *values, double *vectors, int32_t *nfound, int32_t *sweeps,
               axg_solve_info *info);
 
+/* --------------------------------------------------- buckling (*BUCKLE) */
+/* Geometric stiffness for the displacements u_full (3*nnode): element mean
+ * stresses stress_gp (analysis.rs:1061-1151; mean: nelem x 6, may be NULL)
+ * and Kg_ff assembled from hex8_kg / hex20_kg (eigen.rs:299-360) at those
+ * stresses, in element order (analysis.rs:1176-1247). kg_val (may be NULL)
+ * holds one value per slot of the topological K_ff pattern (the pattern of
+ * axg_pattern_host), zero where Kg has no term. */
+int axg_geometric_stiffness(axg_mesh *mesh, const double *u_full, double *mean, double *kg_val);
+/* eigen::subspace_ab (eigen.rs:113-179) on K_ff phi = lambda (-Kg_ff) phi
+ * with Kg_ff from u_full: nmodes + 4 vectors, 30 sweeps of multi-right-hand-
+ * side solves (opts) with Euclidean Gram-Schmidt, Rayleigh quotients sorted
+ * by |lambda|. values: nmodes; vectors (may be NULL): nmodes x n_free of unit
+ * length; nfound: number written. */
+int axg_buckle(axg_mesh *mesh, const double *u_full, int32_t nmodes, const axg_solve_opts *opts,
+               double *values, double *vectors, int32_t *nfound, axg_solve_info *info);
+
 /* ------------------------------------------------------- deck ingestion */
 /* Parses an INP deck (the keyword subset of inp.rs:383-1061: *NODE, *ELEMENT
- * TYPE=C3D8|C3D20, *NSET/*ELSET (GENERATE, nested names), *MATERIAL, *ELASTIC,
- * *DENSITY, *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn, *FREQUENCY) into the compacted model
+ * TYPE=C3D8|C3D20, *NSET and *ELSET (GENERATE, nested names), *MATERIAL, *ELASTIC,
+ * *DENSITY, *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn, *FREQUENCY, *BUCKLE) into the compacted model
  * (model.rs:1234-1309) and solve_linear's setup on the host
  * (analysis.rs:548-946): arrays, prescribed values (last BC wins), isolated
  * nodes pinned, load vector, free-DOF numbering (DofMap::build). text need not
@@ -235,9 +251,10 @@ int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double
 int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
                    double *rf, double *stress, double *vm, axg_solve_info *info);
 
-/* Number of modes the deck's *FREQUENCY step asks for (inp.rs:1062-1071;
- * 0 when the deck has no *FREQUENCY step). */
-int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes);
+/* Number of modes the deck's *FREQUENCY step asks for (inp.rs:1062-1071)
+ * and of buckling factors its *BUCKLE step asks for (inp.rs:1072-1081); 0
+ * for a step that is absent. Either pointer may be NULL. */
+int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle);
 /* The *FREQUENCY branch of solve_linear (analysis.rs:955-1005) without a
  * preload: assembly and autospc as in axg_deck_solve, lumped mass from the
  * materials' *DENSITY, axg_eigen with opts, freqs = sqrt(lambda) / (2 pi)
@@ -247,6 +264,13 @@ int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes);
 int axg_deck_frequency(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *freqs,
                        double *values, double *modes_full, int32_t *nfound, int32_t *sweeps,
                        axg_solve_info *info);
+
+/* The *BUCKLE branch of solve_linear (analysis.rs:1176-1260): assembly and
+ * autospc as in axg_deck_solve, the static solution of the step's loads
+ * (u_static, 3*nnode, may be NULL), then axg_buckle. factors: nbuckle
+ * entries; modes_full (may be NULL): nbuckle x 3*nnode. */
+int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *factors,
+                    double *modes_full, double *u_static, int32_t *nfound, axg_solve_info *info);
 
 /* Batched solves: nmesh meshes (assembled), each with nrhs[m] right-hand
  * sides rhs[m] (n_free[m] * nrhs[m], column-major by right-hand side),

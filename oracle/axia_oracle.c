@@ -28,6 +28,9 @@
  *                          (same stable order), closed-form structured nnz.
  *   or_pcg / or_chol_solve numpy.linalg.solve on random SPD systems; error
  *                          paths (indefinite, singular, not converged).
+ *   or_hex_kg              symmetry, rigid translations in the null space,
+ *                          closed form for a uniaxial stress on a box
+ *                          (sigma A / L string stiffness), Euler buckling.
  */
 #include "axia_oracle.h"
 
@@ -242,6 +245,56 @@ int or_hex_stiffness(int nn, const double *xyz, const double d[36], double *ke, 
         return OR_E_INVALID;
     }
     if (vol_out) *vol_out = vol;
+    return OR_OK;
+}
+
+/* eigen.rs:299-360: continuum geometric stiffness from one constant stress
+ * (Voigt xx yy zz xy yz zx). At each Gauss point with det > 0,
+ * Kg[3a+d][3b+d] += (grad N_a . sigma . grad N_b) w for d = 0, 1, 2
+ * (add_continuum_kg); w = det (hex8, 2x2x2) or W_i W_j W_k det (hex20, 3x3x3).
+ * Points with det <= 0 are skipped. */
+static void add_continuum_kg(double *kg, int nnode, const double *dndx, const double s[6],
+                             double w) {
+    int n = 3 * nnode;
+    double sxx = s[0], syy = s[1], szz = s[2], sxy = s[3], syz = s[4], szx = s[5];
+    for (int a = 0; a < nnode; a++)
+        for (int b = 0; b < nnode; b++) {
+            const double *ga = dndx + 3 * a, *gb = dndx + 3 * b;
+            double gtg = ga[0] * (sxx * gb[0] + sxy * gb[1] + szx * gb[2]) +
+                         ga[1] * (sxy * gb[0] + syy * gb[1] + syz * gb[2]) +
+                         ga[2] * (szx * gb[0] + syz * gb[1] + szz * gb[2]);
+            double v = gtg * w;
+            for (int d = 0; d < 3; d++) kg[(3 * a + d) * n + (3 * b + d)] += v;
+        }
+}
+
+int or_hex_kg(int nn, const double *xyz, const double stress[6], double *kg) {
+    int n = 3 * nn;
+    double dndx[60], det;
+    for (int i = 0; i < n * n; i++) kg[i] = 0.0;
+    if (nn == 8) {
+        const double pts[2] = {-G2, G2};
+        for (int a = 0; a < 2; a++)
+            for (int c = 0; c < 2; c++)
+                for (int e = 0; e < 2; e++) {
+                    int st = or_hex_dndx(8, xyz, pts[a], pts[c], pts[e], dndx, &det, NULL);
+                    if (st) return st;
+                    if (det <= 0.0) continue;
+                    add_continuum_kg(kg, 8, dndx, stress, det);
+                }
+    } else if (nn == 20) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) {
+                    double w = W3[i] * W3[j] * W3[k];
+                    int st = or_hex_dndx(20, xyz, G3[i], G3[j], G3[k], dndx, &det, NULL);
+                    if (st) return st;
+                    if (det <= 0.0) continue;
+                    add_continuum_kg(kg, 20, dndx, stress, w * det);
+                }
+    } else {
+        return OR_E_INVALID;
+    }
     return OR_OK;
 }
 

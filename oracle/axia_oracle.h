@@ -48,6 +48,7 @@ int or_hex_stiffness(int nn, const double *xyz, const double d[36], double *ke, 
 /* elem.rs:892-950, quadratic.rs:266-315 ; face 1..6, fe has 3*nn entries */
 int or_hex_face_pressure(int nn, const double *xyz, int face, double p, double *fe);
 /* elem.rs:743-779, quadratic.rs:148-172,208-244 ; out nn x 6 */
+int or_hex_kg(int nn, const double *xyz, const double stress[6], double *kg);
 int or_hex_nodal_stress(int nn, const double *xyz, const double *ue, double e, double nu,
                         double *out);
 /* elem.rs:879-889 */

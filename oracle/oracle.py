@@ -74,6 +74,7 @@ def lib():
         L.or_hex_stiffness.argtypes = [C.c_int, dp, dp, dp, dp]
         L.or_hex_face_pressure.argtypes = [C.c_int, dp, C.c_int, C.c_double, dp]
         L.or_hex_nodal_stress.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp]
+        L.or_hex_kg.argtypes = [C.c_int, dp, dp, dp]
         L.or_von_mises.argtypes = [dp]
         L.or_von_mises.restype = C.c_double
         L.or_assemble_triplets.argtypes = [C.c_int, i64, dp, i64, ip, dp, dp, lp, lp, dp, lp, lp]
@@ -161,6 +162,16 @@ def hex_nodal_stress(xyz, ue, e, nu):
     out = np.zeros(nn * 6)
     _check(lib().or_hex_nodal_stress(nn, _d(xyz), _d(ue), float(e), float(nu), _d(out)))
     return out.reshape(nn, 6)
+
+
+def hex_kg(xyz, stress):
+    """eigen.rs:299-360 (hex8_kg / hex20_kg, add_continuum_kg): geometric stiffness of one
+    element under a constant stress (Voigt xx yy zz xy yz zx)."""
+    xyz = _f64(xyz)
+    nn = xyz.shape[0]
+    kg = np.zeros((3 * nn) ** 2)
+    _check(lib().or_hex_kg(nn, _d(xyz), _d(_f64(stress)), _d(kg)))
+    return kg.reshape(3 * nn, 3 * nn)
 
 
 def von_mises(s):
@@ -370,6 +381,7 @@ class Model:
     deferred: list = field(default_factory=list)
     densities: dict = field(default_factory=dict)  # material name -> density (*DENSITY)
     nmodes: int = 0  # *FREQUENCY n (0: static step)
+    nbuckle: int = 0  # *BUCKLE n (0: no buckling step)
 
     def node_index(self, nid):
         return self.id_to_index[nid]
@@ -557,6 +569,16 @@ def parse_inp(text):
                 except ValueError:
                     nm = 10
             m.nmodes = max(nm, 1)
+        elif kw == "*BUCKLE":
+            # inp.rs:1072-1081: n factors, 1 when absent or unreadable
+            toks, i = collect(i + 1)
+            nb = 1
+            if toks:
+                try:
+                    nb = _parse_i32(toks[0])
+                except ValueError:
+                    nb = 1
+            m.nbuckle = max(nb, 1)
         elif kw == "*SOLID SECTION":
             elset = params.get("ELSET", "EALL")
             if "MATERIAL" in params:
@@ -1034,3 +1056,93 @@ def frequency_deck(text):
     vals, vecs, sweeps = subspace_gen(csr, m_ind, max(model.nmodes, 1))
     freqs = np.array([math.sqrt(v) / (2.0 * math.pi) if v > 0.0 else 0.0 for v in vals])
     return model, s, csr, m_ind, vals, vecs, freqs, sweeps
+
+
+# ------------------------------------------------------------------ buckling
+def element_mean_stress(sn_all):
+    """analysis.rs:1145-1151: stress_gp of an element = sum over its nodes of the
+    extrapolated nodal stress / nn, in node order."""
+    nelem, nn, _ = sn_all.shape
+    mean = np.zeros((nelem, 6))
+    for a in range(nn):
+        mean += sn_all[:, a, :] / float(nn)
+    return mean
+
+
+def kg_triplets(nn, xyz, conn, mean):
+    """analysis.rs:1176-1247: per-element Kg (hex8_kg / hex20_kg at the element's mean
+    stress), nonzero entries in element order then row-major, full DOF space."""
+    m = 3 * nn
+    ti, tj, tv = [], [], []
+    for el in range(conn.shape[0]):
+        kg = hex_kg(xyz[conn[el]], mean[el])
+        g = (3 * conn[el][:, None] + np.arange(3)[None, :]).ravel()
+        ii, jj = np.nonzero(np.abs(kg) > 0.0)
+        ti.append(g[ii])
+        tj.append(g[jj])
+        tv.append(kg[ii, jj])
+    if not ti:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+    return (np.concatenate(ti).astype(np.int64), np.concatenate(tj).astype(np.int64),
+            np.concatenate(tv))
+
+
+def subspace_ab(k_csr, a_csr, nmodes, solve=None):
+    """eigen.rs:113-179: K phi = lambda A phi by 30 sweeps of inverse subspace iteration
+    with Euclidean Gram-Schmidt (start block: the LCG from seed 7, each vector scaled to
+    unit length), then Rayleigh quotients v.Kv / v.Av (|v.Av| < 1e-30 dropped), sorted
+    by |lambda| (stable). solve(B) returns K^-1 B (rows = vectors); default: dense
+    Cholesky (small n)."""
+    n = k_csr.n
+    if n == 0 or nmodes == 0:
+        raise ValueError("Eigenproblem ohne freie DOF.")
+    nvec = min(max(min(nmodes, n) + 4, min(nmodes, n)), n)
+    if solve is None:
+        lchol = np.linalg.cholesky(k_csr.to_dense())
+
+        def solve(B):
+            y = np.linalg.solve(lchol, B.T)
+            return np.linalg.solve(lchol.T, y).T
+    x = np.zeros((nvec, n))
+    rng = 7
+    mask = (1 << 64) - 1
+    for v in range(nvec):
+        for i in range(n):
+            rng = (rng * 6364136223846793005 + 1) & mask
+            x[v, i] = ((rng >> 33) / float(1 << 31)) - 1.0
+        x[v] /= max(math.sqrt(float(np.sum(x[v] * x[v]))), 1e-30)
+    a = a_csr.to_scipy()
+    k = k_csr.to_scipy()
+    for _ in range(30):
+        y = solve(np.asarray((a @ x.T).T))
+        for p in range(nvec):
+            for q in range(p):
+                y[p] -= float(y[p] @ y[q]) * y[q]
+            y[p] /= max(math.sqrt(float(y[p] @ y[p])), 1e-30)
+        x = y
+    pairs = []
+    for v in x:
+        num = float(v @ (k @ v))
+        den = float(v @ (a @ v))
+        if abs(den) < 1e-30:
+            continue
+        pairs.append((num / den, v))
+    pairs.sort(key=lambda t: abs(t[0]))
+    take = min(nmodes, len(pairs))
+    return np.array([p[0] for p in pairs[:take]]), np.array([p[1] for p in pairs[:take]])
+
+
+def buckle_deck(text):
+    """*BUCKLE for an all-hex deck (analysis.rs:1176-1260): the static solution of the
+    step's loads, element mean stresses, Kg_ff with A = -Kg_ff, subspace_ab on (K_ff, A).
+    Returns (model, linear result, kg csr (unnegated), factors, vectors)."""
+    model = parse_inp(text)
+    s = deck_setup(model)
+    res = linear_solve(s)
+    mean = element_mean_stress(res.elem_nodal_stress)
+    ti, tj, tv = kg_triplets(s.nn, s.xyz, s.conn, mean)
+    oi, oj, ov, _ = reduce(s.ind_of, s.u0, s.n_free, ti, tj, tv, s.f_full)
+    kg = csr_from_triplets(s.n_free, oi, oj, ov)
+    a = Csr(kg.n, kg.indptr, kg.indices, -kg.data)
+    vals, vecs = subspace_ab(res.csr, a, max(model.nbuckle, 1))
+    return model, res, kg, vals, vecs

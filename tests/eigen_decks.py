@@ -37,8 +37,7 @@ def bar_deck(nel, length=100.0, nmodes=3):
     return BAR.format(nodes="".join(nodes), elems="".join(elems), alln=ids + "\n", nmodes=nmodes)
 
 
-def mesh_deck(nn, xyz, conn, fixed_nodes, e=210000.0, nu=0.3, rho=7.85e-9, nmodes=4):
-    """A solid deck from arrays: one material with density, fixed_nodes clamped, *FREQUENCY."""
+def _solid_lines(nn, xyz, conn, e, nu, rho):
     typ = "C3D8" if nn == 8 else "C3D20"
     out = ["*NODE\n"]
     out += [f"{i + 1}, {float(x)!r}, {float(y)!r}, {float(z)!r}\n" for i, (x, y, z) in enumerate(xyz)]
@@ -46,16 +45,39 @@ def mesh_deck(nn, xyz, conn, fixed_nodes, e=210000.0, nu=0.3, rho=7.85e-9, nmode
     for k, row in enumerate(conn):
         ids = [str(int(a) + 1) for a in row]
         # 16 values per data line at most, continuation lines after a trailing comma
-        lines, cur = [], [str(k + 1)] + ids
+        cur = [str(k + 1)] + ids
         while len(cur) > 16:
-            lines.append(", ".join(cur[:16]) + ",\n")
+            out.append(", ".join(cur[:16]) + ",\n")
             cur = cur[16:]
-        lines.append(", ".join(cur) + "\n")
-        out += lines
-    out.append(f"*MATERIAL, NAME=MAT\n*ELASTIC\n{float(e)!r}, {float(nu)!r}\n*DENSITY\n{float(rho)!r}\n")
-    out.append("*SOLID SECTION, ELSET=SOLID, MATERIAL=MAT\n*NSET, NSET=FIX\n")
-    fx = [str(int(a) + 1) for a in fixed_nodes]
-    for i in range(0, len(fx), 16):
-        out.append(", ".join(fx[i:i + 16]) + "\n")
+        out.append(", ".join(cur) + "\n")
+    out.append(f"*MATERIAL, NAME=MAT\n*ELASTIC\n{float(e)!r}, {float(nu)!r}\n")
+    if rho is not None:
+        out.append(f"*DENSITY\n{float(rho)!r}\n")
+    out.append("*SOLID SECTION, ELSET=SOLID, MATERIAL=MAT\n")
+    return out
+
+
+def _nset(name, nodes):
+    out = [f"*NSET, NSET={name}\n"]
+    ids = [str(int(a) + 1) for a in nodes]
+    for i in range(0, len(ids), 16):
+        out.append(", ".join(ids[i:i + 16]) + "\n")
+    return out
+
+
+def mesh_deck(nn, xyz, conn, fixed_nodes, e=210000.0, nu=0.3, rho=7.85e-9, nmodes=4):
+    """A solid deck from arrays: one material with density, fixed_nodes clamped, *FREQUENCY."""
+    out = _solid_lines(nn, xyz, conn, e, nu, rho) + _nset("FIX", fixed_nodes)
     out.append(f"*BOUNDARY\nFIX, 1, 3\n*STEP\n*FREQUENCY\n{nmodes}\n*END STEP\n")
+    return "".join(out)
+
+
+def column_deck(nn, xyz, conn, fixed_nodes, tip_nodes, p_total=-1.0, e=210000.0, nu=0.3,
+                nbuckle=2):
+    """A solid column: fixed_nodes clamped, p_total spread equally over tip_nodes in x (the
+    *CLOAD magnitude applies to each node of the set), *BUCKLE."""
+    out = _solid_lines(nn, xyz, conn, e, nu, None) + _nset("FIX", fixed_nodes)
+    out += _nset("TIP", tip_nodes)
+    out.append(f"*BOUNDARY\nFIX, 1, 3\n*STEP\n*BUCKLE\n{nbuckle}\n"
+               f"*CLOAD\nTIP, 1, {p_total / len(tip_nodes)!r}\n*END STEP\n")
     return "".join(out)

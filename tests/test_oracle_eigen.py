@@ -63,3 +63,79 @@ def test_axial_bar_closed_form():
 def test_frequency_deck_errors():
     with pytest.raises(ValueError):
         O.frequency_deck(bar_deck(3).replace("*DENSITY\n7.85e-9\n", ""))
+
+
+# ------------------------------------------------------------------ buckling
+def _box(nn, lx, ly, lz):
+    from axia_gpu import meshgen as MG
+    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(1, 1, 1)
+    return xyz * np.array([lx, ly, lz]), conn[0]
+
+
+@pytest.mark.parametrize("nn", [8, 20])
+def test_hex_kg_symmetry_and_translations(nn):
+    rng = np.random.default_rng(nn)
+    xyz, conn = _box(nn, 2.0, 1.0, 0.5)
+    xyz = xyz + rng.uniform(-0.05, 0.05, xyz.shape)
+    s = rng.standard_normal(6)
+    kg = O.hex_kg(xyz[conn], s)
+    # symmetric to rounding (gtg(a, b) and gtg(b, a) sum in different orders)
+    assert np.allclose(kg, kg.T, rtol=0, atol=1e-14 * np.abs(kg).max())
+    for d in range(3):
+        t = np.zeros(3 * nn)
+        t[d::3] = 1.0
+        assert np.abs(kg @ t).max() <= 1e-12 * np.abs(kg).max()
+    # entries couple only equal directions
+    m = 3 * nn
+    i, j = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    assert np.all(kg[(i % 3) != (j % 3)] == 0.0)
+
+
+@pytest.mark.parametrize("nn", [8, 20])
+def test_hex_kg_string_closed_form(nn):
+    """Uniaxial sigma_xx on an L x A box: a transverse field u_y = x / L stores
+    sigma * A / L in v^T Kg v (the string's geometric stiffness)."""
+    L, b, h, sig = 3.0, 0.7, 0.4, 2.5
+    xyz, conn = _box(nn, L, b, h)
+    kg = O.hex_kg(xyz[conn], [sig, 0, 0, 0, 0, 0])
+    v = np.zeros(3 * nn)
+    v[1::3] = xyz[conn][:, 0] / L
+    assert math.isclose(float(v @ kg @ v), sig * b * h / L, rel_tol=1e-13)
+
+
+def _column(nn, nx, ny, nz, L=20.0, b=1.0):
+    from axia_gpu import meshgen as MG
+    from eigen_decks import column_deck
+    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(nx, ny, nz)
+    xyz = xyz * np.array([L / nx, b / ny, b / nz])
+    fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
+    tip = np.nonzero(xyz[:, 0] == L)[0]
+    return column_deck(nn, xyz, conn, fixed, tip, p_total=-1.0, nbuckle=2)
+
+
+def test_subspace_ab_vs_lapack_and_euler():
+    """Cantilever column under tip compression: the lowest buckling factors against
+    scipy's dense generalized eigensolver on (A, K), and Euler's pi^2 E I / (4 L^2 P)."""
+    text = _column(20, 10, 1, 1)
+    model, res, kg, vals, vecs = O.buckle_deck(text)
+    a = -kg.to_dense()
+    mu = sla.eigh(a, res.csr.to_dense(), eigvals_only=True)
+    ref = np.sort(1.0 / mu[np.abs(mu) > 0])
+    ref = ref[np.argsort(np.abs(ref), kind="stable")][:2]
+    assert np.allclose(vals, ref, rtol=1e-9)
+    euler = math.pi ** 2 * 210000.0 * (1.0 / 12.0) / (4 * 20.0 ** 2 * 1.0)
+    assert abs(vals[0] / euler - 1.0) < 2e-2
+    assert abs(vals[1] / vals[0] - 1.0) < 1e-6  # square section: a degenerate pair
+    # Kg_ff: symmetric, zero-free
+    assert np.all(kg.data != 0.0)
+    d = kg.to_dense()
+    assert np.allclose(d, d.T, rtol=0, atol=1e-14 * np.abs(d).max())
+
+
+def test_buckle_parse_defaults():
+    text = _column(8, 4, 1, 1)
+    assert O.parse_inp(text).nbuckle == 2
+    assert O.parse_inp(text.replace("*BUCKLE\n2\n", "*BUCKLE\n")).nbuckle == 1
+    with pytest.raises(ValueError):
+        O.subspace_ab(O.Csr(0, np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0)),
+                      None, 1)
