@@ -54,7 +54,7 @@ static std::string element_error(int code, int nn, int32_t el, const double *e, 
     }
 }
 
-static std::string pcg_error(const PcgResult &r) {
+std::string pcg_error(const PcgResult &r) {
     if (r.status == AXG_E_NOT_SPD)  // linalg.rs:231-233
         return "CG: unerwartetes Null-Pivot — Matrix nicht positiv definit.";
     if (r.status == AXG_E_NOT_CONVERGED) {  // linalg.rs:253-257
@@ -551,6 +551,7 @@ int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
         mesh->sell_valid = false;
     }
     mesh->sell_fresh = false;
+    mesh->bsell64_fresh = false;
     CsrView A = active_csr(mesh);
     diag_fp64(ctx, A, mesh->dsel.p, nullptr);
     AXG_HIP(hipStreamSynchronize(s));

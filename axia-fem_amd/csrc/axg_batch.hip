@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <vector>
 
 #include "axg_internal.h"
 #include "axg_mixed.h"
@@ -169,7 +171,8 @@ __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__
 template <int K>
 __global__ void __launch_bounds__(TB) batch_fin_update_kernel(BatchDev B, const double *part,
                                                              SegScalars *sc, int init,
-                                                             double tol, int64_t max_iter,
+                                                             double tol, const double *tol_rel,
+                                                             int64_t max_iter,
                                                              int32_t *live_total) {
     const int seg = blockIdx.x;
     if (!init && B.seg_live[seg] == 0) return;
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(TB) batch_fin_update_kernel(BatchDev B, const 
             const double bn = sqrt(tot[j]);
             s.bnorm = bn;
             s.res = bn;
-            s.tol_abs = tol * bn;
+            s.tol_abs = (tol_rel ? tol_rel[seg * K + j] : tol) * bn;
             s.rz = tot[K + j];
             if (bn < 1e-30) {  // zero right-hand side: x = 0 (linalg.rs:206-214)
                 s.done = 1;
@@ -421,7 +424,7 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
         hipLaunchKernelGGL((batch_update_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, X.p, R.p, P.p,
                            AP.p, Z.p, part.p, sc.p);
         hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p,
-                           sc.p, 0, o.tol, max_iter, live_total.p);
+                           sc.p, 0, o.tol, nullptr, max_iter, live_total.p);
         hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p,
                            0);
     };
@@ -441,7 +444,7 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
     hipLaunchKernelGGL((batch_init_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p, R.p, Z.p,
                        part.p);
     hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p, sc.p,
-                       1, o.tol, max_iter, live_total.p);
+                       1, o.tol, nullptr, max_iter, live_total.p);
     hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p, 1);
     AXG_HIP(hipGetLastError());
     int32_t live = live0;
@@ -525,6 +528,400 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
         throw Error{status, status == AXG_E_NOT_SPD
                                 ? "CG: unerwartetes Null-Pivot — Matrix nicht positiv definit."
                                 : "CG konvergierte nicht (batched solve)."};
+}
+
+// ------------------------------------------------- one mesh, device vectors
+// The block PCG of one mesh runs in node-block space: row 3 I + d is DOF d of
+// block row I (a node with a free DOF); rows of prescribed DOFs carry b = 0,
+// divisor 1 and zero matrix entries, so they stay exactly zero. Vectors are
+// row-major [row][K]. The operator is the FP64 block-sliced-ELL copy of K_ff.
+
+// right-hand side columns (vector-major, n each) -> block rows, zero on
+// prescribed and padding rows and on columns beyond kk
+template <int K>
+__global__ void pack_blocks_kernel(int64_t npad, int64_t rows, int kk,
+                                   const int32_t *__restrict__ fob, int64_t n,
+                                   const double *__restrict__ b, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npad * K) return;
+    const int64_t r = t / K;
+    const int j = (int)(t % K);
+    const int32_t f = r < rows ? fob[r] : -1;
+    out[t] = (f >= 0 && j < kk) ? b[(int64_t)j * n + f] : 0.0;
+}
+
+template <int K>
+__global__ void unpack_blocks_kernel(int64_t n, int kk, const int32_t *__restrict__ bof,
+                                     const double *__restrict__ in, double *__restrict__ x) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kk) return;
+    const int j = (int)(t / n);
+    const int64_t i = t % n;
+    x[t] = in[(int64_t)bof[i] * K + j];
+}
+
+// Jacobi divisor per block row: the free DOF's, 1 on prescribed and padding rows
+__global__ void block_dsel_kernel(int64_t npad, int64_t rows, const int32_t *__restrict__ fob,
+                                  const double *__restrict__ dsel, double *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= npad) return;
+    const int32_t f = r < rows ? fob[r] : -1;
+    out[r] = f >= 0 ? dsel[f] : 1.0;
+}
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// FP64 block-sliced-ELL SpMM: LPR lanes per block row (each takes KL = K / LPR
+// right-hand sides), the 64 block rows of a slice are one workgroup. Block k of
+// a row is five coalesced 16-byte loads (entries 0-7 and entry 8 with the
+// column's bits; the LPR lanes of a row load the same addresses); the 3 KL
+// values of the column block's x that a lane needs are gathered once and
+// shared by the block's three rows. U blocks are loaded before their gathers.
+// With DOT the p.Ap partial sums of the workgroup go to part.
+template <int K, int LPR, int U, bool DOT>
+__global__ void __launch_bounds__(64 * LPR) bsell64_spmm_kernel(
+    int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+    const double *__restrict__ bv, int64_t nsl, const double *__restrict__ P,
+    double *__restrict__ AP, double *__restrict__ part, const int32_t *__restrict__ live) {
+    constexpr int KL = K / LPR;
+    if (DOT && live[0] == 0) return;
+    // XCD-aware order: the grid is a multiple of 8 and workgroup b runs on XCD
+    // b % 8, so XCD x takes the x-th contiguous eighth of the slices and the
+    // neighbouring slices that share x gathers share its L2
+    const int64_t g = gridDim.x, b = blockIdx.x;
+    const int64_t sl = (b % 8) * (g / 8) + b / 8;
+    const int lane = threadIdx.x / LPR, h = threadIdx.x % LPR;
+    const int64_t I = sl * 64 + lane;
+    double dot[KL];
+#pragma unroll
+    for (int k = 0; k < KL; k++) dot[k] = 0.0;
+    if (I < nb) {
+        const d2v *__restrict__ q0 = reinterpret_cast<const d2v *>(bv);
+        const d2v *__restrict__ q1 = reinterpret_cast<const d2v *>(bv + 2 * nsl);
+        const d2v *__restrict__ q2 = reinterpret_cast<const d2v *>(bv + 4 * nsl);
+        const d2v *__restrict__ q3 = reinterpret_cast<const d2v *>(bv + 6 * nsl);
+        const d2v *__restrict__ q4 = reinterpret_cast<const d2v *>(bv + 8 * nsl);
+        const int64_t base = sptr[sl] + lane;
+        const int w = width[sl];
+        double acc[3][KL];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int k = 0; k < KL; k++) acc[d][k] = 0.0;
+        for (int j0 = 0; j0 < w; j0 += U) {
+            d2v a[U][5];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t pos = base + 64 * (int64_t)(j0 + u < w ? j0 + u : j0);
+                a[u][0] = __builtin_nontemporal_load(&q0[pos]);
+                a[u][1] = __builtin_nontemporal_load(&q1[pos]);
+                a[u][2] = __builtin_nontemporal_load(&q2[pos]);
+                a[u][3] = __builtin_nontemporal_load(&q3[pos]);
+                a[u][4] = __builtin_nontemporal_load(&q4[pos]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (j0 + u >= w) break;
+                const int64_t J = (int64_t)__double_as_longlong(a[u][4].y) & 0xffffffffll;
+                const double *xb = P + J * 3 * K + h * KL;
+                double x[3][KL];
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    if constexpr (KL % 2 == 0) {  // KL doubles at a 16-byte aligned offset
+                        const d2v *xp = reinterpret_cast<const d2v *>(xb + e * K);
+#pragma unroll
+                        for (int t = 0; t < KL / 2; t++) {
+                            const d2v v2 = xp[t];
+                            x[e][2 * t] = v2.x;
+                            x[e][2 * t + 1] = v2.y;
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < KL; t++) x[e][t] = xb[e * K + t];
+                    }
+                }
+                const double v[9] = {a[u][0].x, a[u][0].y, a[u][1].x, a[u][1].y, a[u][2].x,
+                                     a[u][2].y, a[u][3].x, a[u][3].y, a[u][4].x};
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+#pragma unroll
+                    for (int k = 0; k < KL; k++)
+                        acc[d][k] += v[3 * d] * x[0][k] + v[3 * d + 1] * x[1][k] + v[3 * d + 2] * x[2][k];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int k = 0; k < KL; k++) {
+                const int64_t i = (3 * I + d) * K + h * KL + k;
+                AP[i] = acc[d][k];
+                if (DOT) dot[k] += P[i] * acc[d][k];
+            }
+    }
+    if (!DOT) return;
+    // per right-hand side: lanes of the same h summed within the wavefront,
+    // then the LPR wavefronts of the workgroup through LDS
+#pragma unroll
+    for (int k = 0; k < KL; k++)
+#pragma unroll
+        for (int off = 32; off >= LPR; off >>= 1) dot[k] += __shfl_down(dot[k], off, 64);
+    __shared__ double sh[LPR][K];
+    const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (wl < LPR)
+#pragma unroll
+        for (int k = 0; k < KL; k++) sh[wv][wl * KL + k] = dot[k];
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double t = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < LPR; w2++) t += sh[w2][threadIdx.x];
+        part[(size_t)blockIdx.x * K + threadIdx.x] = t;
+    }
+}
+
+struct BlockPcg {
+    axg_mesh *m = nullptr;
+    int64_t n = 0, nb = 0, rows = 0, npad = 0, max_iter = 0;
+    int nwg = 0, nsl_wg = 0;
+    int U[4] = {2, 2, 2, 2};  // blocks in flight per lane, tuned per K = 1, 2, 4, 8
+    DevBuf<int64_t> wg_row, seg_end;
+    DevBuf<int32_t> wg_seg, segwg, segwg_mv, seg_live, live_total;
+    DevBuf<double> dsel, Bv, X, R, P, AP, Z, part, tolr;
+    DevBuf<SegScalars> sc;
+    BatchDev B, Bmv;  // Bmv: the SpMM's partial sums (one per slice)
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // K = 1, 2, 4, 8
+    ~BlockPcg() {
+        for (auto &g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+    }
+};
+
+static constexpr int kGraphIters = 32;
+
+template <int K>
+static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x) {
+    axg_mesh *m = bp.m;
+    constexpr int LPR = K == 8 ? 2 : 1;  // 8 right-hand sides: two lanes of 4 per block row
+    const dim3 g(bp.nsl_wg), b(64 * LPR);
+#define AXG_BSELL64(UU, DD)                                                                         \
+    hipLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, bp.nb, m->bs_sptr.p,     \
+                       m->bs_width.p, m->bval64.p, m->bs_nslots, x, bp.AP.p, bp.part.p,            \
+                       bp.seg_live.p)
+    const int u = bp.U[K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3];
+    if (dot) {
+        if (u == 1) AXG_BSELL64(1, true);
+        else if (u == 2) AXG_BSELL64(2, true);
+        else AXG_BSELL64(4, true);
+    } else {
+        if (u == 1) AXG_BSELL64(1, false);
+        else if (u == 2) AXG_BSELL64(2, false);
+        else AXG_BSELL64(4, false);
+    }
+#undef AXG_BSELL64
+}
+
+template <int K>
+static void block_iter(BlockPcg &bp, hipStream_t s) {
+    block_spmv<K>(bp, s, true, bp.P.p);
+    hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(1), dim3(TB), 0, s, bp.Bmv, bp.part.p,
+                       bp.sc.p, bp.live_total.p);
+    hipLaunchKernelGGL((batch_update_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.X.p, bp.R.p,
+                       bp.P.p, bp.AP.p, bp.Z.p, bp.part.p, bp.sc.p);
+    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(TB), 0, s, bp.B, bp.part.p,
+                       bp.sc.p, 0, 0.0, bp.tolr.p, bp.max_iter, bp.live_total.p);
+    hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.P.p, bp.Z.p,
+                       bp.sc.p, 0);
+}
+
+template <int K>
+static void tune_u(BlockPcg &bp, int slot) {
+    axg_ctx *ctx = bp.m->ctx;
+    hipStream_t s = ctx->stream;
+    float best = 1e30f;
+    int best_u = 2;
+    for (int cand : {1, 2, 4}) {
+        bp.U[slot] = cand;
+        block_spmv<K>(bp, s, false, bp.Bv.p);
+        AXG_HIP(hipEventRecord(ctx->ev0, s));
+        for (int r = 0; r < 3; r++) block_spmv<K>(bp, s, false, bp.Bv.p);
+        AXG_HIP(hipEventRecord(ctx->ev1, s));
+        AXG_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ms < best) best = ms, best_u = cand;
+    }
+    bp.U[slot] = best_u;
+}
+
+BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter) {
+    auto bp = std::make_unique<BlockPcg>();
+    axg_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    ensure_bsell64(m);
+    bp->m = m;
+    const int64_t n = m->n_free, nb = m->nb, rows = 3 * nb;
+    const int64_t npad = std::max<int64_t>(((rows + TB - 1) / TB) * TB, TB);
+    AXG_REQUIRE(npad * 8 < INT32_MAX, AXG_E_INVALID, "block PCG exceeds 2^31 vector entries");
+    bp->n = n;
+    bp->nb = nb;
+    bp->rows = rows;
+    bp->npad = npad;
+    bp->max_iter = max_iter > 0 ? max_iter : std::max<int64_t>(4 * n, 200);
+    bp->nwg = (int)(npad / TB);
+    bp->nsl_wg = (int)(((std::max<int64_t>((nb + 63) / 64, 1) + 7) / 8) * 8);  // multiple of 8
+    std::vector<int64_t> h_wg_row(bp->nwg);
+    std::vector<int32_t> h_wg_seg(bp->nwg, 0);
+    for (int w = 0; w < bp->nwg; w++) h_wg_row[w] = (int64_t)w * TB;
+    bp->wg_row.alloc(bp->nwg);
+    bp->wg_seg.alloc(bp->nwg);
+    bp->seg_end.alloc(2);
+    bp->segwg.alloc(2);
+    bp->segwg_mv.alloc(2);
+    bp->seg_live.alloc(1);
+    bp->live_total.alloc(1);
+    bp->dsel.alloc(npad);
+    const int64_t h_end[2] = {0, rows};
+    const int32_t h_segwg[2] = {0, bp->nwg}, h_segwg_mv[2] = {0, bp->nsl_wg};
+    AXG_HIP(hipMemcpyAsync(bp->wg_row.p, h_wg_row.data(), 8 * bp->nwg, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp->wg_seg.p, h_wg_seg.data(), 4 * bp->nwg, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp->seg_end.p, h_end, 16, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp->segwg.p, h_segwg, 8, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp->segwg_mv.p, h_segwg_mv, 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(block_dsel_kernel, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, s,
+                       npad, rows, m->free_of_blk.p, m->dsel.p, bp->dsel.p);
+    const size_t nv = (size_t)npad * 8;
+    bp->Bv.alloc(nv);
+    bp->X.alloc(nv);
+    bp->R.alloc(nv);
+    bp->P.alloc(nv);
+    bp->AP.alloc(nv);
+    bp->Z.alloc(nv);
+    bp->part.alloc((size_t)std::max(bp->nwg * 16, bp->nsl_wg * 8) + 16);
+    bp->tolr.alloc(8);
+    bp->sc.alloc(8);
+    BatchDev &B = bp->B;
+    B.sptr = nullptr;  // the update kernels read only the row structure
+    B.width = nullptr;
+    B.col = nullptr;
+    B.val = nullptr;
+    B.dsel = bp->dsel.p;
+    B.wg_seg = bp->wg_seg.p;
+    B.wg_row = bp->wg_row.p;
+    B.seg_row = bp->seg_end.p;
+    B.seg_wg = bp->segwg.p;
+    B.seg_live = bp->seg_live.p;
+    bp->Bmv = B;
+    bp->Bmv.seg_wg = bp->segwg_mv.p;
+    hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(256), 0, s, (int64_t)nv, bp->Bv.p, 0.0);
+    // blocks in flight (U) tuned on this operator for each width
+    tune_u<1>(*bp, 0);
+    tune_u<2>(*bp, 1);
+    tune_u<4>(*bp, 2);
+    tune_u<8>(*bp, 3);
+    AXG_HIP(hipGetLastError());
+    return bp.release();
+}
+
+void block_pcg_destroy(BlockPcg *bp) { delete bp; }
+
+template <int K>
+static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *tol_rel,
+                        double *d_x, PcgResult &res) {
+    axg_ctx *ctx = bp.m->ctx;
+    hipStream_t s = ctx->stream;
+    const int slot = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
+    if (!bp.graph[slot]) {
+        hipGraph_t g;
+        AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < kGraphIters; i++) block_iter<K>(bp, s);
+        AXG_HIP(hipStreamEndCapture(s, &g));
+        AXG_HIP(hipGraphInstantiate(&bp.graph[slot], g, nullptr, nullptr, 0));
+        AXG_HIP(hipGraphDestroy(g));
+    }
+    double h_tol[8];
+    for (int j = 0; j < K; j++) h_tol[j] = j < kk ? tol_rel[j] : 1.0;
+    std::vector<SegScalars> hs(K);
+    std::memset(hs.data(), 0, sizeof(SegScalars) * K);
+    const int32_t live0 = K;
+    AXG_HIP(hipMemcpyAsync(bp.tolr.p, h_tol, 8 * K, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp.sc.p, hs.data(), sizeof(SegScalars) * K, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp.seg_live.p, &live0, 4, hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bp.live_total.p, &live0, 4, hipMemcpyHostToDevice, s));
+    const int64_t tot = bp.npad * K;
+    hipLaunchKernelGGL((pack_blocks_kernel<K>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       bp.npad, bp.rows, kk, bp.m->free_of_blk.p, bp.n, d_b, bp.Bv.p);
+    AXG_HIP(hipEventRecord(ctx->ev0, s));
+    hipLaunchKernelGGL((batch_init_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.Bv.p, bp.X.p,
+                       bp.R.p, bp.Z.p, bp.part.p);
+    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(TB), 0, s, bp.B, bp.part.p,
+                       bp.sc.p, 1, 0.0, bp.tolr.p, bp.max_iter, bp.live_total.p);
+    hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.P.p,
+                       bp.Z.p, bp.sc.p, 1);
+    AXG_HIP(hipGetLastError());
+    int32_t live = live0;
+    int64_t launched = 0;
+    for (;;) {
+        AXG_HIP(hipMemcpyAsync(&live, bp.live_total.p, 4, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        if (live <= 0 || launched >= bp.max_iter) break;
+        AXG_HIP(hipGraphLaunch(bp.graph[slot], s));
+        launched += kGraphIters;
+    }
+    AXG_HIP(hipEventRecord(ctx->ev1, s));
+    if (bp.n)
+        hipLaunchKernelGGL((unpack_blocks_kernel<K>), dim3((unsigned)((bp.n * kk + 255) / 256)),
+                           dim3(256), 0, s, bp.n, kk, bp.m->blk_of_free.p, bp.X.p, d_x);
+    AXG_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    AXG_HIP(hipMemcpy(hs.data(), bp.sc.p, sizeof(SegScalars) * K, hipMemcpyDeviceToHost));
+    res.solve_ms += ms;
+    for (int j = 0; j < kk; j++) {
+        const SegScalars &r = hs[j];
+        res.iters += r.iters;
+        if (!r.done && res.status == AXG_OK) res.status = AXG_E_NOT_CONVERGED;
+        if (r.status != AXG_OK && res.status == AXG_OK) res.status = r.status;
+        if (r.bnorm > 0 && r.res / r.bnorm >= res.residual / std::max(res.bnorm, 1e-300))
+            res.residual = r.res, res.bnorm = r.bnorm;
+    }
+}
+
+template <int K>
+static double time_block_spmm(BlockPcg &bp, int reps) {
+    axg_ctx *ctx = bp.m->ctx;
+    hipStream_t s = ctx->stream;
+    block_spmv<K>(bp, s, false, bp.Bv.p);
+    AXG_HIP(hipEventRecord(ctx->ev0, s));
+    for (int r = 0; r < reps; r++) block_spmv<K>(bp, s, false, bp.Bv.p);
+    AXG_HIP(hipEventRecord(ctx->ev1, s));
+    AXG_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return (double)ms / reps;
+}
+
+void bench_block_spmm(axg_mesh *m, int nrhs, int reps, double *avg_ms) {
+    std::unique_ptr<BlockPcg> bp(block_pcg_create(m, 0));
+    *avg_ms = nrhs == 8   ? time_block_spmm<8>(*bp, reps)
+              : nrhs == 4 ? time_block_spmm<4>(*bp, reps)
+              : nrhs == 2 ? time_block_spmm<2>(*bp, reps)
+                          : time_block_spmm<1>(*bp, reps);
+}
+
+void block_pcg_solve(BlockPcg *bp, const double *d_b, int nrhs, const double *tol_rel,
+                     double *d_x, PcgResult &res) {
+    int j0 = 0;
+    while (j0 < nrhs) {
+        const int left = nrhs - j0;
+        const int kk = left >= 8 ? 8 : left;
+        const double *b = d_b + (int64_t)j0 * bp->n;
+        double *x = d_x + (int64_t)j0 * bp->n;
+        if (kk > 4) block_group<8>(*bp, b, kk, tol_rel + j0, x, res);
+        else if (kk > 2) block_group<4>(*bp, b, kk, tol_rel + j0, x, res);
+        else if (kk == 2) block_group<2>(*bp, b, kk, tol_rel + j0, x, res);
+        else block_group<1>(*bp, b, kk, tol_rel + j0, x, res);
+        j0 += kk;
+    }
 }
 
 }  // namespace axg

@@ -61,6 +61,12 @@ extern "C" int axg_bench_operator(axg_mesh *m, int32_t which, int32_t nrhs, int3
             bench_operator_mixed(m, nrhs, reps, avg_ms);
             return AXG_OK;
         }
+        if (which == 3) {
+            AXG_REQUIRE(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8, AXG_E_INVALID,
+                        "axg_bench_operator: nrhs must be 1, 2, 4 or 8");
+            bench_block_spmm(m, nrhs, reps, avg_ms);
+            return AXG_OK;
+        }
         CsrView A = which == 0 ? solver_csr(m) : active_csr(m);
         DevBuf<double> x(A.n), y(A.n);
         AXG_HIP(hipMemsetAsync(x.p, 0, 8 * A.n, s));

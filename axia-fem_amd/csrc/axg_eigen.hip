@@ -62,6 +62,13 @@ __global__ void axpy_kernel(int64_t n, double c, const double *__restrict__ b,
     if (i < n) a[i] -= c * b[i];
 }
 
+// y = x / c
+__global__ void div_copy_kernel(int64_t n, double c, const double *__restrict__ x,
+                                double *__restrict__ y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] / c;
+}
+
 __global__ void scale_kernel(int64_t n, double c, double *__restrict__ a) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] /= c;
@@ -127,6 +134,77 @@ static void jacobi_eigen(std::vector<double> &a, int n, std::vector<double> &val
     vals.resize(n);
     for (int i = 0; i < n; i++) vals[i] = a[i * n + i];
 }
+
+// The solves of a subspace sweep: columns R_j (n each) with tolerance tl[j] < 1
+// are packed, solved together (multi-right-hand-side FP64 block PCG for the
+// Jacobi preconditioner, the mixed solver at the tightest of the tolerances
+// for mixed precision, one FP64 PCG per column otherwise) and added to the
+// iterate as Y_j += scl[j] x_j.
+struct ColumnSolver {
+    axg_mesh *m;
+    axg_solve_opts o;
+    double *P, *D;
+    BlockPcg *bp = nullptr;
+    int64_t iters = 0;
+    double ms = 0.0, res = 0.0, rel = 0.0;
+    ColumnSolver(axg_mesh *mesh, const axg_solve_opts &opts, double *pack, double *sol)
+        : m(mesh), o(opts), P(pack), D(sol) {
+        if (o.precision != AXG_PREC_MIXED && o.precond == AXG_PRECOND_JACOBI)
+            bp = block_pcg_create(m, o.max_iter);
+    }
+    ~ColumnSolver() {
+        if (bp) block_pcg_destroy(bp);
+    }
+    ColumnSolver(const ColumnSolver &) = delete;
+    ColumnSolver &operator=(const ColumnSolver &) = delete;
+
+    void solve(const double *R, double *Y, const std::vector<double> &scl,
+               const std::vector<double> &tl) {
+        hipStream_t s = m->ctx->stream;
+        const int64_t n = m->n_free;
+        const unsigned gn = (unsigned)((n + 255) / 256);
+        iters = 0, ms = 0.0, res = 0.0, rel = 0.0;
+        std::vector<int> cols;
+        std::vector<double> ctol;
+        for (int j = 0; j < (int)tl.size(); j++)
+            if (tl[j] < 1.0) {
+                AXG_HIP(hipMemcpyAsync(P + (size_t)cols.size() * n, R + (size_t)j * n, 8 * n,
+                                       hipMemcpyDeviceToDevice, s));
+                cols.push_back(j);
+                ctol.push_back(tl[j]);
+            }
+        const int k = (int)cols.size();
+        if (k == 0) return;
+        if (bp) {
+            PcgResult pr;
+            block_pcg_solve(bp, P, k, ctol.data(), D, pr);
+            if (pr.status != AXG_OK) throw Error{pr.status, pcg_error(pr)};
+            iters = pr.iters, ms = pr.solve_ms, res = pr.residual;
+            rel = pr.residual / std::max(pr.bnorm, 1e-300);
+        } else if (o.precision == AXG_PREC_MIXED) {
+            axg_solve_opts oj = o;
+            oj.tol = *std::min_element(ctol.begin(), ctol.end());
+            axg_solve_info si;
+            const int st = axg_mesh_solve_device(m, P, k, &oj, D, &si);
+            if (st != AXG_OK) throw Error{st, axg_last_error()};
+            iters = si.iters, ms = si.solve_ms, res = si.residual, rel = si.rel_residual;
+        } else {
+            for (int c = 0; c < k; c++) {
+                axg_solve_opts oj = o;
+                oj.tol = ctol[c];
+                axg_solve_info si;
+                const int st = axg_mesh_solve_device(m, P + (size_t)c * n, 1, &oj, D + (size_t)c * n, &si);
+                if (st != AXG_OK) throw Error{st, axg_last_error()};
+                iters += si.iters, ms += si.solve_ms;
+                if (si.rel_residual >= rel) rel = si.rel_residual, res = si.residual;
+            }
+        }
+        for (int c = 0; c < k; c++)
+            hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s, n, -scl[cols[c]],
+                               D + (size_t)c * n, Y + (size_t)cols[c] * n);
+        AXG_HIP(hipGetLastError());
+    }
+};
 
 }  // namespace axg
 
@@ -196,37 +274,73 @@ extern "C" int axg_eigen(axg_mesh *mesh, const double *m_diag, int32_t nmodes,
     m_orthonormalize(X.p);
 
     CsrView A = solver_csr(mesh);
-    // Power-of-two scale of the right-hand sides M X. The PCG stops with "not
-    // positive definite" when |p^T K p| < 1e-30 (an absolute guard); with lumped
-    // masses of order 1e-10 the curvatures of a converging solve fall below it.
-    // Scaling to R^T D^-1 R ~ 1 (D: the Jacobi divisor; an M-orthonormal x gives
-    // R^T D^-1 R ~ mean(m / D)) keeps them far above it, and a power of two
-    // changes no rounding: the iterates are exactly 2^k times the unscaled ones
-    // and the M-orthonormalisation that follows removes the factor exactly.
-    double rhs_scale = 1.0;
+    // 1 / Jacobi divisor, for the right-hand-side scale below
+    DevBuf<double> dinv(n), P(nv), D(nv);
     {
         std::vector<double> hd(n);
         AXG_HIP(hipMemcpyAsync(hd.data(), mesh->dsel.p, 8 * n, hipMemcpyDeviceToHost, s));
         AXG_HIP(hipStreamSynchronize(s));
-        double acc = 0.0;
-        for (int64_t i = 0; i < n; i++) acc += m_diag[i] / hd[i];
-        acc /= (double)n;
-        if (acc > 0.0 && std::isfinite(acc))
-            rhs_scale = std::ldexp(1.0, -(int)std::lround(0.5 * std::log2(acc)));
+        for (int64_t i = 0; i < n; i++) hd[i] = 1.0 / hd[i];
+        AXG_HIP(hipMemcpyAsync(dinv.p, hd.data(), 8 * n, hipMemcpyHostToDevice, s));
     }
+    // The PCG stops with "not positive definite" when |p^T K p| < 1e-30 (an
+    // absolute guard, linalg.rs:231-233); with lumped masses of order 1e-10 the
+    // curvatures of a converging solve fall below it. Each right-hand side is
+    // divided by the power of two nearest sqrt(b^T D^-1 b) (D: the Jacobi
+    // divisor) and its solution multiplied back: a power of two changes no
+    // rounding, so the iterates are exactly those of the unscaled system.
+    auto pow2_scale = [&](double *b) {
+        const double beta = wdot(b, dinv.p, b);
+        if (!(beta > 0.0) || !std::isfinite(beta)) return 1.0;
+        const double sc = std::ldexp(1.0, (int)std::lround(0.5 * std::log2(beta)));
+        hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s, n, sc, b);
+        return sc;
+    };
     std::vector<double> vals(nvec, 0.0), ev, q, kr((size_t)nvec * nvec);
+    std::vector<double> sc(nvec, 1.0), tolj(nvec, o.tol);
     int it = 0;
-    int64_t solve_iters = 0;
+    int64_t solve_iters = 0, skipped = 0;
     double solve_ms = 0.0, worst_res = 0.0, worst_rel = 0.0;
+    ColumnSolver cs(mesh, o, P.p, D.p);
+    // Y_j += sc_j K^-1 R_j for every column with tolj < 1, each to its tolerance
+    auto solve_columns = [&](double *Rb, const std::vector<double> &scl,
+                             const std::vector<double> &tl) {
+        cs.solve(Rb, Y.p, scl, tl);
+        solve_iters += cs.iters;
+        solve_ms += cs.ms;
+        if (cs.rel >= worst_rel) worst_rel = cs.rel, worst_res = cs.res;
+    };
     for (; it < 40; it++) {
         hipLaunchKernelGGL(mass_apply_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s,
-                           n, nvec, dm.p, X.p, rhs_scale, R.p);
-        axg_solve_info si;
-        const int st = axg_mesh_solve_device(mesh, R.p, nvec, &o, Y.p, &si);
-        if (st != AXG_OK) throw Error{st, axg_last_error()};
-        solve_iters += si.iters;
-        solve_ms += si.solve_ms;
-        if (si.rel_residual >= worst_rel) worst_rel = si.rel_residual, worst_res = si.residual;
+                           n, nvec, dm.p, X.p, 1.0, R.p);
+        // Warm start (after the first sweep): x_j is a Ritz vector with Ritz value
+        // l_j, so K^-1 M x_j ~ x_j / l_j. The solve then only corrects that guess:
+        // K d_j = M x_j - K (x_j / l_j), stopped at the same absolute residual,
+        // tol ||M x_j||, as a solve from zero (so at relative tolerance
+        // tol ||M x_j|| / ||M x_j - K x_j / l_j||; none once that reaches 1).
+        for (int j = 0; j < nvec; j++) {
+            double *rj = R.p + (size_t)j * n, *yj = Y.p + (size_t)j * n;
+            const double lam = vals[j];
+            if (it > 0 && lam > 0.0 && std::isfinite(lam)) {
+                const double nb = std::sqrt(wdot(rj, nullptr, rj));
+                hipLaunchKernelGGL(div_copy_kernel, dim3(gn), dim3(256), 0, s, n, lam,
+                                   X.p + (size_t)j * n, yj);
+                spmv_fp64(ctx, A, yj, KY.p);
+                hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s, n, 1.0, KY.p, rj);
+                const double nbp = std::sqrt(wdot(rj, nullptr, rj));
+                tolj[j] = nbp > 0.0 ? o.tol * nb / nbp : HUGE_VAL;
+            } else {
+                AXG_HIP(hipMemsetAsync(yj, 0, 8 * n, s));
+                tolj[j] = o.tol;
+            }
+            if (tolj[j] >= 1.0) {
+                skipped++;
+                continue;
+            }
+            sc[j] = pow2_scale(rj);
+        }
+        AXG_HIP(hipGetLastError());
+        solve_columns(R.p, sc, tolj);
         m_orthonormalize(Y.p);
         // Kr = Y^T K Y, column j from K y_j
         for (int j = 0; j < nvec; j++) {
@@ -288,7 +402,8 @@ extern "C" int axg_eigen(axg_mesh *mesh, const double *m_diag, int32_t nmodes,
         info->residual = worst_res;
         info->rel_residual = worst_rel;
         info->solve_ms = solve_ms;
-        std::snprintf(info->name, sizeof(info->name), "subspace (%d vectors, GPU PCG)", nvec);
+        std::snprintf(info->name, sizeof(info->name), "subspace (%d vectors, GPU PCG, %lld warm)", nvec,
+                      (long long)skipped);
     }
     API_END
 }
@@ -319,7 +434,7 @@ extern "C" int axg_buckle(axg_mesh *mesh, const double *u_full, int32_t nmodes,
 
     const size_t nv = (size_t)n * nvec;
     DevBuf<double> du(std::max<int64_t>(mesh->ndof, 1)), aval(std::max<int64_t>(mesh->nnz_topo, 1)),
-        X(nv), Y(nv), R(nv), T(n), dinv(n);
+        X(nv), Y(nv), R(nv), P(nv), D(nv), T(n), dinv(n);
     AXG_HIP(hipMemcpyAsync(du.p, u_full, 8 * mesh->ndof, hipMemcpyHostToDevice, s));
     geometric_stiffness_device(mesh, du.p, -1.0, aval.p, nullptr);
     int ashape = -1;
@@ -370,24 +485,26 @@ extern "C" int axg_buckle(axg_mesh *mesh, const double *u_full, int32_t nmodes,
     int64_t solve_iters = 0;
     double solve_ms = 0.0, worst_res = 0.0, worst_rel = 0.0;
     double *x = X.p, *y = Y.p;
+    ColumnSolver cs(mesh, o, P.p, D.p);
+    std::vector<double> scl(nvec, 1.0), tl(nvec, o.tol);
     for (int it = 0; it < 30; it++) {
         for (int k = 0; k < nvec; k++) {
             double *r = R.p + (size_t)k * n;
             spmv_fp64(ctx, Aop, x + (size_t)k * n, r);
             // power-of-two scale to r^T D^-1 r ~ 1 (the PCG's absolute breakdown
-            // guard); exact, and removed exactly by the normalisation below
+            // guard); exact, and multiplied back onto the solution
             const double beta = wdot(r, dinv.p, r);
+            scl[k] = 1.0;
             if (beta > 0.0 && std::isfinite(beta)) {
-                const double sc = std::ldexp(1.0, (int)std::lround(0.5 * std::log2(beta)));
-                hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s, n, sc, r);
+                scl[k] = std::ldexp(1.0, (int)std::lround(0.5 * std::log2(beta)));
+                hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s, n, scl[k], r);
             }
         }
-        axg_solve_info si;
-        const int st = axg_mesh_solve_device(mesh, R.p, nvec, &o, y, &si);
-        if (st != AXG_OK) throw Error{st, axg_last_error()};
-        solve_iters += si.iters;
-        solve_ms += si.solve_ms;
-        if (si.rel_residual >= worst_rel) worst_rel = si.rel_residual, worst_res = si.residual;
+        AXG_HIP(hipMemsetAsync(y, 0, 8 * nv, s));
+        cs.solve(R.p, y, scl, tl);
+        solve_iters += cs.iters;
+        solve_ms += cs.ms;
+        if (cs.rel >= worst_rel) worst_rel = cs.rel, worst_res = cs.res;
         // Euclidean Gram-Schmidt (eigen.rs:143-156)
         for (int a = 0; a < nvec; a++) {
             double *ya = y + (size_t)a * n;

@@ -216,6 +216,10 @@ struct axg_mesh {
     axg::DevBuf<int64_t> bs_sptr;
     axg::DevBuf<int32_t> bs_width, bs_col;
     int spmm_shape[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};  // tuned BSR SpMM shape per k
+    // FP64 values in the same block layout (five double2 planes per slot), for
+    // the multi-right-hand-side FP64 block PCG; refreshed after every assembly
+    bool bsell64_fresh = false;
+    axg::DevBuf<double> bval64;
     // host mirror of the active pattern (kept for export)
     std::vector<int32_t> h_rowptr, h_colind;
 };

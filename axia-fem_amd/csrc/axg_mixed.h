@@ -21,6 +21,9 @@ double true_residual(axg_ctx *ctx, const CsrView &A, const double *d_x, const do
 void ensure_block_maps(axg_mesh *m);
 void ensure_block_inverse64(axg_mesh *m);
 void assemble_mixed(axg_mesh *m, double *elapsed_ms);
+// FP64 block-sliced-ELL values of K_ff (bval64) in the FP32 operator's block
+// layout; built on first use after each FP64 assembly.
+void ensure_bsell64(axg_mesh *m);
 void solve_mixed(axg_mesh *m, const double *d_rhs, int32_t nrhs, const axg_solve_opts &o,
                  double *d_x, MixedResult &res);
 

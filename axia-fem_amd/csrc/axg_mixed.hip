@@ -214,6 +214,69 @@ __global__ void col_plane_kernel(int64_t nsl, const int32_t *__restrict__ col,
     if (i < nsl) c2[2 * i + 1] = __int_as_float(col[i]);
 }
 
+__global__ void fill64_kernel(int64_t n, double *__restrict__ a) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0.0;
+}
+
+// FP64 block planes: five double2 per slot, entries (0, 1), (2, 3), (4, 5),
+// (6, 7) and (8, column bits) of the row-major 3x3 block, 80 B per block.
+// Each topological slot's value is its contributions summed in ascending
+// element order (the FP64 K_ff value, exact zeros included), stored at the
+// position the FP32 layout gives it.
+__global__ void gather_bsell64_kernel(const uint32_t *__restrict__ slotptr,
+                                      const uint32_t *__restrict__ contrib,
+                                      const double *__restrict__ ke, int64_t nslot,
+                                      const int32_t *__restrict__ eos, int64_t nsl,
+                                      double *__restrict__ bv) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslot;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (uint32_t c = slotptr[s]; c < slotptr[s + 1]; c++) v += ke[contrib[c]];
+        int64_t e = eos[s], pos;
+        int q;
+        if (e < 4 * nsl) {
+            pos = e >> 2, q = (int)(e & 3);
+        } else if (e < 8 * nsl) {
+            e -= 4 * nsl;
+            pos = e >> 2, q = 4 + (int)(e & 3);
+        } else {
+            pos = (e - 8 * nsl) >> 1, q = 8;
+        }
+        bv[q < 8 ? (q >> 1) * 2 * nsl + 2 * pos + (q & 1) : 8 * nsl + 2 * pos] = v;
+    }
+}
+
+__global__ void col_plane64_kernel(int64_t nsl, const int32_t *__restrict__ col,
+                                   double *__restrict__ plane) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsl) plane[2 * i + 1] = __longlong_as_double((long long)col[i]);
+}
+
+void ensure_bsell64(axg_mesh *m) {
+    if (m->bsell64_fresh) return;
+    AXG_REQUIRE(m->assembled64, AXG_E_INVALID, "mesh not assembled (call axg_assemble)");
+    hipStream_t s = m->ctx->stream;
+    if (m->nnzb == 0) build_bsr(m);
+    const int64_t nsl = m->bs_nslots;
+    m->bval64.ensure(std::max<int64_t>(10 * nsl, 1));
+    if (nsl) {
+        hipLaunchKernelGGL(fill64_kernel, dim3(1024), dim3(256), 0, s, 10 * nsl, m->bval64.p);
+        hipLaunchKernelGGL(col_plane64_kernel, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, s,
+                           nsl, m->bs_col.p, m->bval64.p + 8 * nsl);
+    }
+    const int64_t nslot = m->nnz_topo;
+    if (nslot)
+        hipLaunchKernelGGL(gather_bsell64_kernel,
+                           dim3((unsigned)std::min<int64_t>((nslot + 255) / 256, 1 << 20)), dim3(256),
+                           0, s, m->slotptr_t.p, m->contrib_t.p, m->ke.p, nslot,
+                           m->entry_of_slot.p, nsl, m->bval64.p);
+    AXG_HIP(hipGetLastError());
+    AXG_HIP(hipStreamSynchronize(s));
+    m->bsell64_fresh = true;
+}
+
 void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;

@@ -51,4 +51,19 @@ void diag_fp64(axg_ctx *ctx, const CsrView &A, double *dsel, double *diag_raw);
 void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b, double *x,
               double tol, int64_t max_iter, bool timing, PcgResult &res);
 
+// Multi-right-hand-side FP64 Jacobi PCG of one assembled mesh on device
+// vectors (the batched solver's kernels on the mesh's own sliced-ELL operator,
+// 8, 4, 2 or 1 right-hand sides per pass, one graph per width, reused across
+// calls). Column j of d_b (n_free each) is solved to ||r|| <= tol_rel[j] ||b_j||.
+// The reference's CG error text for a failed solve (linalg.rs:231-257).
+std::string pcg_error(const PcgResult &r);
+
+struct BlockPcg;
+BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter);
+void block_pcg_destroy(BlockPcg *bp);
+void block_pcg_solve(BlockPcg *bp, const double *d_b, int nrhs, const double *tol_rel,
+                     double *d_x, PcgResult &res);
+// Average device time of one block-PCG SpMM launch (nrhs = 1, 2, 4 or 8).
+void bench_block_spmm(axg_mesh *m, int nrhs, int reps, double *avg_ms);
+
 }  // namespace axg

@@ -26,6 +26,11 @@ batch  48 C3D8 cubes m^3, m ~ U{8..48}, 4 top-face load cases each, all in
 simp  C3D8 brick 120x60x60 (432,000 elements), SIMP modulus, point load;
     compliance, dc/drho through the adjoint kernel, central differences
     (step 1e-4) on 64 random elements, and the FP64 CPU adjoint of the oracle.
+freq  *FREQUENCY: the 10 lowest modes of a steel C3D8 brick 48x48x48 (110,592
+    elements, 345,744 free DOFs; E 210000, nu 0.3, rho 7.85e-9, bottom face
+    clamped) by the reference's subspace iteration (14 vectors, at most 40
+    sweeps) over the multi-right-hand-side FP64 block PCG (warm-started from
+    the Ritz vectors). One step = assembly, lumped mass and the eigen solve.
 
 Inputs are resident in HBM when a timed region starts.
 """
@@ -45,7 +50,7 @@ for _p in (ROOT, os.path.join(ROOT, "axia-fem_amd")):
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 METRIC = "DOF·iter/s PCG + elements/s assembly, mixed vs FP64 GPU at 1e-8 rel. error"
-WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp")
+WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq")
 
 
 def log(*a):
@@ -136,6 +141,12 @@ def sell64_bytes(n, nnz, k=1):
     """Algorithmic bytes of one FP64 SpMV: 8 B value + 4 B column per entry, x read and y
     written once per row and right-hand side."""
     return 12 * nnz + 16 * n * k
+
+
+def bsell64_bytes(nb, nnzb, k):
+    """Algorithmic bytes of one FP64 block SpMM launch: 72 B values + 4 B column per 3x3
+    block, x read and y written once per block row and right-hand side (24 B each)."""
+    return 76 * nnzb + 2 * 8 * 3 * nb * k
 
 
 def traffic_for(kernel_key, workload_key):
@@ -522,6 +533,88 @@ def run_batch(a, G, MG, torch, np):
     return out
 
 
+# --------------------------------------------------------------------- freq
+def run_freq(a, G, MG, torch, np):
+    import scipy.sparse as sp
+    N, nm, rho, tol = a.freq_n, a.nmodes, 7.85e-9, a.eig_tol
+    xyz, conn = MG.brick_hex8(N, N, N)
+    ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 2] == 0.0)[0])
+    ne = len(conn)
+    p = MG.Problem(f"brick_c3d8_{N}x{N}x{N}_freq", 8, xyz, conn, np.full(ne, 210000.0),
+                   np.full(ne, 0.3), ind)
+    ctx = G.Context(0)
+    m = G.Mesh(8, xyz, conn, p.e, p.nu, ind, ctx=ctx)
+    m.assemble("fp64")
+    n, nnz = m.n_free, m.info()["nnz"]
+    log(f"freq: n={n} nnz={nnz}")
+    if a.kernels:
+        out = {"workload": p.name, "n_free": n, "nnz": nnz, "reps": a.reps}
+        for k in (1, 2, 4, 8):
+            ms = m.bench_operator(3, k, a.reps)
+            out[f"fp64_block_spmm_k{k}"] = {
+                "ms": ms, "gbs": bsell64_bytes(n // 3, nnz // 9, k) / (ms / 1e3) / 1e9}
+            log(f"fp64 block spmm k={k}: {ms:.4f} ms")
+        print(json.dumps(out))
+        return None
+
+    def step(precision="fp64", vectors=True):
+        m.assemble("mixed" if precision == "mixed" else "fp64")
+        md = m.lumped_mass(rho)[ind >= 0]
+        return md, m.eigen(md, nm, precision=precision, tol=tol, vectors=vectors)
+
+    for _ in range(a.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        md, (vals, vecs, sweeps, info) = step()
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / a.steps
+    freqs = np.sqrt(vals) / (2 * np.pi)
+    log(f"freq: {ms_per_step:.0f} ms/step, sweeps {sweeps}, {info['iters']} column iterations, "
+        f"f = {freqs[:4]}")
+    # size-independent checks: M-orthonormal modes solving K phi = lambda M phi
+    ip, ix, dv = m.csr()
+    K = sp.csr_matrix((dv, ix, ip), shape=(n, n))
+    gram = (vecs * md) @ vecs.T
+    orth = float(np.abs(gram - np.eye(len(vals))).max())
+    resid = [float(np.linalg.norm(K @ v - lam * md * v) / np.linalg.norm(K @ v))
+             for lam, v in zip(vals, vecs)]
+    # the same analysis with the FP32-operator solver
+    _, (vmx, _, sw_mx, info_mx) = step("mixed", vectors=False)
+    mixed_dev = float(np.abs(vmx / vals - 1.0).max())
+    value = n * info["iters"] / (info["solve_ms"] / 1e3)
+    launch_ms = m.bench_operator(3, 8, a.reps)
+    nb, nnzb = n // 3, nnz // 9  # every node of this brick has 3 free DOFs or none
+    out = base_line(a, value, ms_per_step, "f64", {
+        "workload": f"*FREQUENCY, {nm} lowest modes of a steel C3D8 brick {N}^3 (n={n}), "
+                    f"bottom clamped, lumped mass, subspace iteration over the FP64 block PCG "
+                    f"to {tol:g}",
+        "n_free": n, "nnz": nnz, "nmodes": nm, "precond": "jacobi"})
+    out.update({
+        "eigen": {"time_to_solution_ms": ms_per_step, "sweeps": sweeps,
+                  "column_iterations": info["iters"], "solve_ms": info["solve_ms"],
+                  "frequencies_hz": [float(f) for f in freqs],
+                  "m_orthonormality_max_dev": orth, "rel_eigen_residual_max": max(resid),
+                  "mixed_solve_ms": info_mx["solve_ms"], "mixed_sweeps": sw_mx,
+                  "mixed_rel_value_dev_max": mixed_dev, "solver": info["name"]},
+        "roofline": roofline("bsell64_spmm_kernel<K=8> (FP64 block-sliced-ELL SpMM of the "
+                             "block PCG)", bsell64_bytes(nb, nnzb, 8), launch_ms),
+    })
+    if not a.no_cpu:
+        from oracle import oracle as O
+        cores = host_threads()
+        csr = _oracle_csr(O, p)
+        b = np.asarray(md, dtype=np.float64) * 1e6
+        rate = _oracle_pcg_rate(O, csr, b, 60, cores)
+        out["cpu_baseline"] = {
+            "value": rate, "unit": "DOF·iter/s", "cores": cores, "kind": "port",
+            "sample": f"oracle linalg::pcg restatement (Jacobi, FP64 CSR, OpenMP), 60 iterations "
+                      f"on this brick's K_ff (n={csr.n}, nnz={csr.nnz}); the eigen solve's "
+                      f"{info['iters']} column iterations at this rate take "
+                      f"{info['iters'] * n / rate:.1f} s",
+        }
+    return out
+
+
 # --------------------------------------------------------------------- SIMP
 def cpu_adjoint(O, p, u, threads, chunk=20000):
     """FP64 CPU adjoint: dc/dE_e = -u_e^T K1_e u_e with the oracle's element matrices."""
@@ -657,6 +750,9 @@ def main():
     ap.add_argument("--kernels", action="store_true",
                     help="only the operator kernels (repeated launches), for profiling")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--freq-n", type=int, default=48, help="freq brick edge (elements)")
+    ap.add_argument("--nmodes", type=int, default=10)
+    ap.add_argument("--eig-tol", type=float, default=1e-10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -670,7 +766,7 @@ def main():
     from axia_gpu import meshgen as MG
 
     run = {"brick": run_brick, "cantilever": run_cantilever, "c3d20": run_c3d20,
-           "batch": run_batch, "simp": run_simp}[a.workload]
+           "batch": run_batch, "simp": run_simp, "freq": run_freq}[a.workload]
     out = run(a, G, MG, torch, np)
     if out is not None:
         print(json.dumps(out))

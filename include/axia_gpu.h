@@ -287,7 +287,9 @@ int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *meshes,
 int axg_bench_stream(axg_ctx *ctx, int64_t n, int32_t reps, double *copy_gbs);
 /* Average device time of one launch of a solver operator kernel on an
  * assembled mesh: which = 0 FP64 SpMV (solver layout), 1 FP64 CSR SpMV,
- * 2 FP32 block SpMM with nrhs right-hand sides (1, 2, 4 or 8). */
+ * 2 FP32 block SpMM with nrhs right-hand sides (1, 2, 4 or 8), 3 FP64
+ * sliced-ELL SpMM of the multi-right-hand-side block PCG (the eigen and
+ * buckling solves) with nrhs right-hand sides (1, 2, 4 or 8). */
 int axg_bench_operator(axg_mesh *mesh, int32_t which, int32_t nrhs, int32_t reps, double *avg_ms);
 
 /* ------------------------------------------------- host-only helpers */

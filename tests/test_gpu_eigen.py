@@ -222,3 +222,12 @@ def test_deck_buckle_euler(ctx):
     assert abs(res.factors[0] / euler - 1.0) < 2e-2
     assert res.modes.shape == (2, 3 * dk.nnode)
     assert dk.nbuckle == 2 and dk.nmodes == 0
+
+
+def test_block_spmm_timing_entry(ctx):
+    """The block PCG's SpMM is timed through axg_bench_operator(which = 3)."""
+    m, *_ = cantilever_mesh(ctx, 8, 6, 2, 2)
+    for k in (1, 2, 4, 8):
+        assert m.bench_operator(3, k, 3) > 0.0
+    with pytest.raises(G.AxgError):
+        m.bench_operator(3, 3, 3)
