@@ -414,6 +414,26 @@ static void lagrange1d(const double *pts, int n, double x, double *w) {
     }
 }
 
+/* quadratic.rs:208-230: 27 Gauss-point values to the 20 nodes by tensor
+ * Lagrange interpolation through the Gauss abscissae. */
+static void hex20_extrapolate(const double g[27][6], double *out) {
+    for (int a = 0; a < 20; a++) {
+        double lx[3], ly[3], lz[3];
+        lagrange1d(G3, 3, HEX20_XI[a][0], lx);
+        lagrange1d(G3, 3, HEX20_XI[a][1], ly);
+        lagrange1d(G3, 3, HEX20_XI[a][2], lz);
+        for (int c = 0; c < 6; c++) out[a * 6 + c] = 0.0;
+        int idx = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) {
+                    double w = lx[i] * ly[j] * lz[k];
+                    for (int c = 0; c < 6; c++) out[a * 6 + c] += w * g[idx][c];
+                    idx++;
+                }
+    }
+}
+
 /* elem.rs:743-779 (nn = 8) and quadratic.rs:148-172 + 208-230 (nn = 20,
  * 27-point rule, tensor Lagrange extrapolation). */
 int or_hex_nodal_stress(int nn, const double *xyz, const double *ue, double e, double nu,
@@ -454,21 +474,7 @@ int or_hex_nodal_stress(int nn, const double *xyz, const double *ue, double e, d
                     sigma_from_b(b, n, d, ue, gsig[idx]);
                     idx++;
                 }
-        for (int a = 0; a < 20; a++) {
-            double lx[3], ly[3], lz[3];
-            lagrange1d(G3, 3, HEX20_XI[a][0], lx);
-            lagrange1d(G3, 3, HEX20_XI[a][1], ly);
-            lagrange1d(G3, 3, HEX20_XI[a][2], lz);
-            for (int c = 0; c < 6; c++) out[a * 6 + c] = 0.0;
-            idx = 0;
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++)
-                    for (int k = 0; k < 3; k++) {
-                        double w = lx[i] * ly[j] * lz[k];
-                        for (int c = 0; c < 6; c++) out[a * 6 + c] += w * gsig[idx][c];
-                        idx++;
-                    }
-        }
+        hex20_extrapolate(gsig, out);
     } else {
         return OR_E_INVALID;
     }
@@ -710,5 +716,204 @@ int or_chol_solve(int64_t n, double *a, const double *b, double *x) {
         x[i] = s / a[i * n + i];
     }
     free(y);
+    return OR_OK;
+}
+
+
+/* ------------------------------------------------------------- NLGEOM
+ * nlgeom.rs:22-127, 197-273: total-Lagrange St. Venant-Kirchhoff for hex8
+ * (2x2x2, weight det) and hex20 (3x3x3, weight W_i W_j W_k det). */
+static int lam_mu(double e, double nu, double *lam, double *mu) {
+    if (e <= 0.0 || nu >= 0.5 || nu <= -1.0) return OR_E_MATERIAL;
+    *lam = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    *mu = e / (2.0 * (1.0 + nu));
+    return OR_OK;
+}
+
+/* nlgeom.rs:50-64 */
+static void deformation_gradient(const double *dndx, const double *ue, int nn, double f[3][3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) f[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int a = 0; a < nn; a++)
+        for (int i = 0; i < 3; i++) {
+            double u = ue[3 * a + i];
+            for (int j = 0; j < 3; j++) f[i][j] += u * dndx[3 * a + j];
+        }
+}
+
+/* nlgeom.rs:66-70 */
+static double det3(const double a[3][3]) {
+    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+           a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+}
+
+/* nlgeom.rs:34-48 */
+static void green_lagrange(const double f[3][3], double e[3][3]) {
+    double c[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) c[i][j] = f[0][i] * f[0][j] + f[1][i] * f[1][j] + f[2][i] * f[2][j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) e[i][j] = 0.5 * (c[i][j] - (i == j ? 1.0 : 0.0));
+}
+
+/* nlgeom.rs:22-32 */
+static void voigt_s(const double e[3][3], double lam, double mu, double s[6]) {
+    double tr = e[0][0] + e[1][1] + e[2][2];
+    s[0] = lam * tr + 2.0 * mu * e[0][0];
+    s[1] = lam * tr + 2.0 * mu * e[1][1];
+    s[2] = lam * tr + 2.0 * mu * e[2][2];
+    s[3] = 2.0 * mu * e[0][1];
+    s[4] = 2.0 * mu * e[1][2];
+    s[5] = 2.0 * mu * e[2][0];
+}
+
+/* nlgeom.rs:72-99 */
+static int pk2_to_cauchy(const double f[3][3], const double s[6], double out[6]) {
+    double sm[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+    double fs[3][3], c[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) fs[i][j] = f[i][0] * sm[0][j] + f[i][1] * sm[1][j] + f[i][2] * sm[2][j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) c[i][j] = fs[i][0] * f[j][0] + fs[i][1] * f[j][1] + fs[i][2] * f[j][2];
+    double j = det3(f);
+    if (j <= 1e-18) return OR_E_INVERSION;
+    double invj = 1.0 / j;
+    out[0] = c[0][0] * invj;
+    out[1] = c[1][1] * invj;
+    out[2] = c[2][2] * invj;
+    out[3] = c[0][1] * invj;
+    out[4] = c[1][2] * invj;
+    out[5] = c[2][0] * invj;
+    return OR_OK;
+}
+
+/* nlgeom.rs:101-115 */
+static void fill_b_nl(double *b, int nn, const double f[3][3], const double *dndx) {
+    int n = 3 * nn;
+    for (int a = 0; a < nn; a++) {
+        const double *g = dndx + 3 * a;
+        for (int i = 0; i < 3; i++) {
+            int c = 3 * a + i;
+            b[0 * n + c] = f[i][0] * g[0];
+            b[1 * n + c] = f[i][1] * g[1];
+            b[2 * n + c] = f[i][2] * g[2];
+            b[3 * n + c] = f[i][0] * g[1] + f[i][1] * g[0];
+            b[4 * n + c] = f[i][1] * g[2] + f[i][2] * g[1];
+            b[5 * n + c] = f[i][2] * g[0] + f[i][0] * g[2];
+        }
+    }
+}
+
+/* nlgeom.rs:197-238 (assemble_gp) for an isotropic elastic material */
+static int assemble_gp_nl(double *ke, double *fe, double *vol, double cau[6], double gl[6],
+                          int nn, const double *dndx, const double *ue, double lam, double mu,
+                          const double *cmat, double w) {
+    int nd = 3 * nn;
+    double f[3][3], egl[3][3], s[6], cs[6];
+    deformation_gradient(dndx, ue, nn, f);
+    if (det3(f) <= 1e-18) return OR_E_INVERSION;
+    green_lagrange(f, egl);
+    voigt_s(egl, lam, mu, s);
+    double b[6 * 60], tmp[6 * 60];
+    fill_b_nl(b, nn, f, dndx);
+    gemm_bt_d_b(ke, nd, b, 6, cmat, w, tmp);
+    add_continuum_kg(ke, nn, dndx, s, w);
+    for (int j = 0; j < nd; j++) {
+        double q = 0.0;
+        for (int i = 0; i < 6; i++) q += b[i * nd + j] * s[i];
+        fe[j] += q * w;
+    }
+    int st = pk2_to_cauchy(f, s, cs);
+    if (st) return st;
+    double glv[6] = {egl[0][0], egl[1][1], egl[2][2], 2.0 * egl[0][1], 2.0 * egl[1][2],
+                     2.0 * egl[2][0]};
+    for (int i = 0; i < 6; i++) {
+        cau[i] += cs[i] * w;
+        gl[i] += glv[i] * w;
+    }
+    *vol += w;
+    return OR_OK;
+}
+
+/* nlgeom.rs:127-273: tangent ke ((3nn)^2), internal force fe (3nn), volume-
+ * averaged Cauchy and Green-Lagrange (6 each) of one hex at displacements ue. */
+int or_hex_nl(int nn, const double *xyz0, const double *ue, double e, double nu, double *ke,
+              double *fe, double cauchy[6], double gl[6], double *vol_out) {
+    double lam, mu, cmat[36];
+    int st = lam_mu(e, nu, &lam, &mu);
+    if (st) return st;
+    st = or_d_iso_3d(e, nu, cmat);
+    if (st) return st;
+    int nd = 3 * nn;
+    for (int i = 0; i < nd * nd; i++) ke[i] = 0.0;
+    for (int i = 0; i < nd; i++) fe[i] = 0.0;
+    double cau[6] = {0, 0, 0, 0, 0, 0}, g[6] = {0, 0, 0, 0, 0, 0}, vol = 0.0, dndx[60], det;
+    if (nn == 8) {
+        const double pts[2] = {-G2, G2};
+        for (int a = 0; a < 2; a++)
+            for (int c = 0; c < 2; c++)
+                for (int k = 0; k < 2; k++) {
+                    st = or_hex_dndx(8, xyz0, pts[a], pts[c], pts[k], dndx, &det, NULL);
+                    if (st) return st;
+                    if (det <= 0.0) return OR_E_NEG_JACOBIAN;
+                    st = assemble_gp_nl(ke, fe, &vol, cau, g, 8, dndx, ue, lam, mu, cmat, det);
+                    if (st) return st;
+                }
+    } else if (nn == 20) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) {
+                    double w = W3[i] * W3[j] * W3[k];
+                    st = or_hex_dndx(20, xyz0, G3[i], G3[j], G3[k], dndx, &det, NULL);
+                    if (st) return st;
+                    if (det <= 0.0) return OR_E_NEG_JACOBIAN;
+                    st = assemble_gp_nl(ke, fe, &vol, cau, g, 20, dndx, ue, lam, mu, cmat, w * det);
+                    if (st) return st;
+                }
+    } else {
+        return OR_E_INVALID;
+    }
+    /* GpAcc::finish (nlgeom.rs:168-188) */
+    double inv = fabs(vol) > 0.0 ? 1.0 / vol : 0.0;
+    for (int i = 0; i < 6; i++) {
+        cauchy[i] = cau[i] * inv;
+        gl[i] = g[i] * inv;
+    }
+    if (vol_out) *vol_out = vol;
+    return OR_OK;
+}
+
+/* nlgeom.rs:274-308: Cauchy and Green-Lagrange at the 27 points of a hex20,
+ * extrapolated to its 20 nodes (20 x 6 each). */
+int or_hex20_nodal_nl(const double *xyz0, const double *ue, double e, double nu, double *cs,
+                      double *gs) {
+    double lam, mu, dndx[60], det, f[3][3], egl[3][3], s[6];
+    int st = lam_mu(e, nu, &lam, &mu);
+    if (st) return st;
+    double gc[27][6], ge[27][6];
+    int idx = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                st = or_hex_dndx(20, xyz0, G3[i], G3[j], G3[k], dndx, &det, NULL);
+                if (st) return st;
+                if (det <= 0.0) return OR_E_NEG_JACOBIAN;
+                deformation_gradient(dndx, ue, 20, f);
+                if (det3(f) <= 1e-18) return OR_E_INVERSION;
+                green_lagrange(f, egl);
+                voigt_s(egl, lam, mu, s);
+                st = pk2_to_cauchy(f, s, gc[idx]);
+                if (st) return st;
+                ge[idx][0] = egl[0][0];
+                ge[idx][1] = egl[1][1];
+                ge[idx][2] = egl[2][2];
+                ge[idx][3] = 2.0 * egl[0][1];
+                ge[idx][4] = 2.0 * egl[1][2];
+                ge[idx][5] = 2.0 * egl[2][0];
+                idx++;
+            }
+    hex20_extrapolate(gc, cs);
+    hex20_extrapolate(ge, gs);
     return OR_OK;
 }

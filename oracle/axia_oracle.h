@@ -29,7 +29,8 @@ enum {
     OR_E_NOT_CONVERGED = 6,  /* "CG konvergierte nicht ..."                   */
     OR_E_INDEFINITE = 7,     /* "Steifigkeitsmatrix ist indefinit."           */
     OR_E_SINGULAR = 8,       /* "Steifigkeitsmatrix ist singulär ..."         */
-    OR_E_NOMEM = 9
+    OR_E_NOMEM = 9,
+    OR_E_INVERSION = 10      /* "NLGEOM: det(F) <= 0 (Element inversion)."     */
 };
 
 /* elem.rs:41-59 */
@@ -49,6 +50,14 @@ int or_hex_stiffness(int nn, const double *xyz, const double d[36], double *ke, 
 int or_hex_face_pressure(int nn, const double *xyz, int face, double p, double *fe);
 /* elem.rs:743-779, quadratic.rs:148-172,208-244 ; out nn x 6 */
 int or_hex_kg(int nn, const double *xyz, const double stress[6], double *kg);
+/* nlgeom.rs:127-273 (hex8_nl / hex20_nl, St. Venant-Kirchhoff): tangent ke
+ * ((3nn)^2), internal force fe (3nn), volume-averaged Cauchy and
+ * Green-Lagrange (6 each), reference volume. */
+int or_hex_nl(int nn, const double *xyz0, const double *ue, double e, double nu, double *ke,
+              double *fe, double cauchy[6], double gl[6], double *vol);
+/* nlgeom.rs:274-308: hex20 nodal Cauchy and Green-Lagrange (20 x 6 each) */
+int or_hex20_nodal_nl(const double *xyz0, const double *ue, double e, double nu, double *cs,
+                      double *gs);
 int or_hex_nodal_stress(int nn, const double *xyz, const double *ue, double e, double nu,
                         double *out);
 /* elem.rs:879-889 */

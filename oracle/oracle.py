@@ -42,6 +42,7 @@ ERRORS = {
     8: "Steifigkeitsmatrix ist singulär — Randbedingungen unzureichend "
        "(Starrkörperbewegung oder entartete Elemente).",
     9: "out of memory",
+    10: "NLGEOM: det(F) ≤ 0 (Element inversion).",
 }
 
 
@@ -75,6 +76,8 @@ def lib():
         L.or_hex_face_pressure.argtypes = [C.c_int, dp, C.c_int, C.c_double, dp]
         L.or_hex_nodal_stress.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp]
         L.or_hex_kg.argtypes = [C.c_int, dp, dp, dp]
+        L.or_hex_nl.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp, dp, dp, dp, dp]
+        L.or_hex20_nodal_nl.argtypes = [dp, dp, C.c_double, C.c_double, dp, dp]
         L.or_von_mises.argtypes = [dp]
         L.or_von_mises.restype = C.c_double
         L.or_assemble_triplets.argtypes = [C.c_int, i64, dp, i64, ip, dp, dp, lp, lp, dp, lp, lp]
@@ -334,6 +337,11 @@ def _parse_i32(s):
     return int(round(f))
 
 
+def _round_half_away(x):
+    """Rust's f64::round: halfway cases away from zero."""
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
 def _is_int(s):
     return re.fullmatch(r"[+-]?\d+", s.strip()) is not None
 
@@ -382,6 +390,9 @@ class Model:
     densities: dict = field(default_factory=dict)  # material name -> density (*DENSITY)
     nmodes: int = 0  # *FREQUENCY n (0: static step)
     nbuckle: int = 0  # *BUCKLE n (0: no buckling step)
+    nlgeom: bool = False  # *STEP, NLGEOM
+    max_inc: int = 100  # *STEP, INC=
+    ninc: int = 1  # increments of the static step (*STATIC: round(period / dt0), capped)
 
     def node_index(self, nid):
         return self.id_to_index[nid]
@@ -569,6 +580,28 @@ def parse_inp(text):
                 except ValueError:
                     nm = 10
             m.nmodes = max(nm, 1)
+        elif kw == "*STEP":
+            # inp.rs:953-982: NLGEOM (any value) switches the static step to NLGEOM
+            if "NLGEOM" in params:
+                m.nlgeom = True
+                m.ninc = 1
+            if "INC" in params:
+                try:
+                    v = _parse_i32(params["INC"])
+                    if v > 0:
+                        m.max_inc = v
+                except ValueError:
+                    pass
+            i += 1
+        elif kw == "*STATIC":
+            # inp.rs:983-1061 without RIKS: initial increment and period
+            toks, i = collect(i + 1)
+            def pos(k):
+                v = _parse_f64(toks[k]) if len(toks) > k else None
+                return v if v is not None and v > 0.0 else 1.0
+            dt0, period = pos(0), pos(1)
+            n_from_time = max(int(_round_half_away(period / dt0)), 1)
+            m.ninc = max(min(n_from_time, max(m.max_inc, 1)), 1)
         elif kw == "*BUCKLE":
             # inp.rs:1072-1081: n factors, 1 when absent or unreadable
             toks, i = collect(i + 1)
@@ -1146,3 +1179,253 @@ def buckle_deck(text):
     a = Csr(kg.n, kg.indptr, kg.indices, -kg.data)
     vals, vecs = subspace_ab(res.csr, a, max(model.nbuckle, 1))
     return model, res, kg, vals, vecs
+
+
+# ------------------------------------------------------------------- NLGEOM
+def hex_nl(xyz0, ue, e, nu):
+    """nlgeom.rs:127-273 (hex8_nl / hex20_nl): St. Venant-Kirchhoff tangent, internal force,
+    volume-averaged Cauchy and Green-Lagrange, reference volume of one hex."""
+    xyz0 = _f64(xyz0)
+    nn = xyz0.shape[0]
+    ke = np.zeros((3 * nn) ** 2)
+    fe = np.zeros(3 * nn)
+    cau = np.zeros(6)
+    gl = np.zeros(6)
+    vol = np.zeros(1)
+    _check(lib().or_hex_nl(nn, _d(xyz0), _d(_f64(ue)), float(e), float(nu), _d(ke), _d(fe),
+                           _d(cau), _d(gl), _d(vol)))
+    return ke.reshape(3 * nn, 3 * nn), fe, cau, gl, float(vol[0])
+
+
+def hex20_nodal_nl(xyz0, ue, e, nu):
+    """nlgeom.rs:274-308: nodal Cauchy and Green-Lagrange of a hex20 (20 x 6 each)."""
+    cs = np.zeros(120)
+    gs = np.zeros(120)
+    _check(lib().or_hex20_nodal_nl(_d(_f64(xyz0)), _d(_f64(ue)), float(e), float(nu), _d(cs),
+                                   _d(gs)))
+    return cs.reshape(20, 6), gs.reshape(20, 6)
+
+
+@dataclass
+class NlResult:
+    u_full: np.ndarray
+    rf_full: np.ndarray  # f_int - f_ext over all DOFs
+    stress: np.ndarray  # nodal Cauchy (nnode x 6)
+    strain: np.ndarray  # nodal Green-Lagrange (nnode x 6)
+    von_mises: np.ndarray
+    stress_gp: np.ndarray  # element means (nelem x 6)
+    iters: int
+    ninc: int
+    residual: float
+    n_free: int
+    ind_of: np.ndarray
+
+
+def nl_assemble(nn, xyz, conn, e, nu, u_full):
+    """The element loop of solve_continuum_newton (analysis.rs:4474-4545): per element the
+    NL tangent and internal force; f_int summed in element order, tangent triplets (|v| > 0,
+    row-major) in the full DOF space, element Cauchy / Green-Lagrange means."""
+    nelem = conn.shape[0]
+    m = 3 * nn
+    f_int = np.zeros(3 * xyz.shape[0])
+    u3 = u_full.reshape(-1, 3)
+    ti, tj, tv = [], [], []
+    cau = np.zeros((nelem, 6))
+    gl = np.zeros((nelem, 6))
+    for el in range(nelem):
+        nodes = conn[el]
+        ke, fe, c, g, _ = hex_nl(xyz[nodes], u3[nodes].ravel(), e[el], nu[el])
+        gd = (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+        for i in range(m):
+            f_int[gd[i]] += fe[i]
+        ii, jj = np.nonzero(np.abs(ke) > 0.0)
+        ti.append(gd[ii])
+        tj.append(gd[jj])
+        tv.append(ke[ii, jj])
+        cau[el] = c
+        gl[el] = g
+    return (f_int, (np.concatenate(ti).astype(np.int64), np.concatenate(tj).astype(np.int64),
+                    np.concatenate(tv)), cau, gl)
+
+
+def nl_fext(nn, xyz, conn, cloads, dloads, u_full=None):
+    """assemble_fext_u (analysis.rs:3222-3300) for CLOAD and face pressure: nodal loads summed
+    per (node, dof) and added first, then each element's pressure faces (on the deformed
+    coordinates when u_full is given) in element order."""
+    f = np.zeros(3 * xyz.shape[0])
+    acc = {}
+    for ni, d, mag in cloads:
+        if d < 3:
+            acc[(ni, d)] = acc.get((ni, d), 0.0) + mag * 1.0
+    for (ni, d), mag in acc.items():
+        f[3 * ni + d] += mag
+    by_elem = {}
+    for el, face, mag in dloads:
+        by_elem.setdefault(int(el), []).append((face, mag))
+    for el in sorted(by_elem):
+        nodes = conn[el]
+        x = xyz[nodes].copy()
+        if u_full is not None:
+            x = x + u_full.reshape(-1, 3)[nodes]
+        for face, mag in by_elem[el]:
+            fe = hex_face_pressure(x, face, mag)
+            for a in range(nn):
+                for d in range(3):
+                    f[3 * nodes[a] + d] += fe[3 * a + d]
+    return f
+
+
+def _norm_seq(v):
+    s = 0.0
+    for x in v:
+        s += x * x
+    return math.sqrt(s)
+
+
+def nlgeom_deck(text, newton_tol=1e-8, max_newton=25, solve=None):
+    """solve_continuum_newton (analysis.rs:4313-4860) for an all-hex St. Venant-Kirchhoff deck
+    without contact, pretension, rigid bodies, plasticity or amplitudes: increments of
+    1 / ninc (grown by 1.5x after a success, halved on failure), full Newton with the
+    consistent tangent and follower pressure, prescribed values ramped with the load factor,
+    convergence at ||r_free|| < newton_tol (1 + ||f_inc||). solve(csr, rhs) defaults to a
+    dense LAPACK solve (small decks)."""
+    model = parse_inp(text)
+    if not model.nlgeom:
+        raise ValueError("the deck has no *STEP, NLGEOM")
+    nn, xyz, conn, e, nu = model.mesh_arrays()
+    nnode, nelem, ndof = xyz.shape[0], conn.shape[0], 3 * xyz.shape[0]
+    cloads = [(model.node_index(nid), d, mag) for nid, d, mag in model.cloads]
+    eidx = {el[0]: k for k, el in enumerate(model.elements)}
+    dloads = [(eidx[eid], face, mag) for eid, face, mag in model.dloads]
+    prescribed = {}
+    for nid, d, v in model.bcs:
+        if d < 3:
+            prescribed[3 * model.node_index(nid) + d] = float(v)
+    attached = np.zeros(nnode, bool)
+    attached[conn.ravel()] = True
+    for ni in np.nonzero(~attached)[0]:
+        for d in range(3):
+            prescribed.setdefault(3 * int(ni) + d, 0.0)
+    is_ind = np.ones(ndof, bool)
+    u_target = np.zeros(ndof)
+    for d, v in prescribed.items():
+        is_ind[d] = False
+        u_target[d] = v
+    ind_of = np.full(ndof, -1, np.int64)
+    n_free = int(is_ind.sum())
+    ind_of[is_ind] = np.arange(n_free)
+    if n_free == 0:
+        raise ValueError("NLGEOM Kontinuum: keine freien DOF.")
+    if solve is None:
+        def solve(csr, rhs):
+            return np.linalg.solve(csr.to_dense(), rhs)
+    f_tgt = nl_fext(nn, xyz, conn, cloads, dloads)
+    f_0 = np.zeros(ndof)
+    ninc = max(model.ninc, 1)
+    u_full = np.zeros(ndof)
+    lam, dt, ninc_done, retries = 0.0, 1.0 / ninc, 0, 0
+    f_now = f_0.copy()
+    iters, residual = 0, 0.0
+    last_fint = np.zeros(ndof)
+    last_cau = np.zeros((nelem, 6))
+    last_gl = np.zeros((nelem, 6))
+    pres = sorted(prescribed)
+    while lam < 1.0 - 1e-14:
+        lam_new = min(lam + dt, 1.0)
+        u_work = u_full.copy()
+        for d in pres:
+            u_work[d] = (1.0 - lam_new) * 0.0 + lam_new * u_target[d]
+        f_inc = (1.0 - lam_new) * f_0 + lam_new * f_tgt
+        last_du = np.zeros(ndof)
+        have_du, inc_ok, inc_err = False, False, None
+        it = 0
+        while it < max_newton:
+            iters = it + 1
+            if dloads:
+                fd = nl_fext(nn, xyz, conn, cloads, dloads, u_work)
+                f_inc = (1.0 - lam_new) * f_0 + lam_new * fd
+            try:
+                f_int, trips, cau, gl = nl_assemble(nn, xyz, conn, e, nu, u_work)
+            except OracleError as ex:
+                inc_err = str(ex)
+                if ex.code == 10 and have_du:
+                    u_work -= 0.5 * last_du
+                    last_du *= 0.5
+                    for d in pres:
+                        u_work[d] = (1.0 - lam_new) * 0.0 + lam_new * u_target[d]
+                    if _norm_seq(last_du) > 1e-18:
+                        inc_err = None
+                        it += 1
+                        continue
+                break
+            last_fint = f_int
+            last_cau, last_gl = cau, gl
+            r = f_inc - f_int
+            ti, tj, tv = trips
+            oi, oj, ov, rhs = reduce(ind_of, np.zeros(ndof), n_free, ti, tj, tv, r)
+            residual = _norm_seq(rhs)
+            fref = _norm_seq(f_inc)
+            if residual < newton_tol * (1.0 + fref):
+                inc_ok = True
+                break
+            if it + 1 == max_newton:
+                inc_err = f"Newton konvergierte nicht (r={residual:.3e} nach {iters} Iterationen)."
+                break
+            x = solve(csr_from_triplets(n_free, oi, oj, ov), rhs)
+            du_full = np.zeros(ndof)
+            du_full[is_ind] = x[ind_of[is_ind]]
+            u_work = u_work + du_full
+            for d in pres:
+                u_work[d] = (1.0 - lam_new) * 0.0 + lam_new * u_target[d]
+            last_du = du_full
+            have_du = True
+            if _norm_seq(x) < 1e-14:
+                inc_ok = True
+                break
+            it += 1
+        if inc_ok:
+            u_full = u_work
+            lam = lam_new
+            f_now = f_inc
+            ninc_done += 1
+            retries = 0
+            remain = 1.0 - lam
+            if remain > 1e-14:
+                dt = min(dt * 1.5, 1.0 / ninc, remain)
+        else:
+            retries += 1
+            dt *= 0.5
+            if dt < 1e-6 or retries > 16:
+                raise ValueError(f"NLGEOM Inkrement bei λ={lam:.4f} fehlgeschlagen ({inc_err}).")
+    rf = last_fint - f_now
+    # nodal Cauchy / Green-Lagrange (analysis.rs:4745-4799): C3D20 from its 27 points,
+    # C3D8 the element mean on every node; plain mean per node in element order
+    accs = np.zeros((nnode, 6))
+    gacc = np.zeros((nnode, 6))
+    cnt = np.zeros(nnode)
+    stress_gp = np.zeros((nelem, 6))
+    u3 = u_full.reshape(-1, 3)
+    for el in range(nelem):
+        nodes = conn[el]
+        if nn == 20:
+            cs, gs = hex20_nodal_nl(xyz[nodes], u3[nodes].ravel(), e[el], nu[el])
+            mean = np.zeros(6)
+            for a in range(nn):
+                mean += cs[a] / float(nn)
+        else:
+            cs = np.repeat(last_cau[el][None, :], nn, 0)
+            gs = np.repeat(last_gl[el][None, :], nn, 0)
+            mean = last_cau[el]
+        for a in range(nn):
+            accs[nodes[a]] += cs[a]
+            gacc[nodes[a]] += gs[a]
+            cnt[nodes[a]] += 1.0
+        stress_gp[el] = mean
+    stress = np.zeros((nnode, 6))
+    strain = np.zeros((nnode, 6))
+    has = cnt > 0
+    stress[has] = accs[has] / cnt[has, None]
+    strain[has] = gacc[has] / cnt[has, None]
+    vm = np.array([von_mises(sv) for sv in stress])
+    return NlResult(u_full, rf, stress, strain, vm, stress_gp, iters, ninc_done, residual,
+                    n_free, ind_of)

@@ -81,3 +81,49 @@ def column_deck(nn, xyz, conn, fixed_nodes, tip_nodes, p_total=-1.0, e=210000.0,
     out.append(f"*BOUNDARY\nFIX, 1, 3\n*STEP\n*BUCKLE\n{nbuckle}\n"
                f"*CLOAD\nTIP, 1, {p_total / len(tip_nodes)!r}\n*END STEP\n")
     return "".join(out)
+
+
+def nl_bar_deck(nel=4, length=4.0, delta=0.5, e=1000.0, nu=0.0, ninc=4, nn=8):
+    """A unit-section x-bar of nel hexes stretched by a prescribed end displacement delta
+    under *STEP, NLGEOM in ninc increments: x = 0 face ux = 0, the origin node uy = uz = 0,
+    node (0, 1, 0) uz = 0 (no rigid motion, lateral contraction free)."""
+    from axia_gpu import meshgen as MG
+    import numpy as np
+    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(nel, 1, 1)
+    xyz = xyz * np.array([length / nel, 1.0, 1.0])
+    out = _solid_lines(nn, xyz, conn, e, nu, None)
+    x0 = np.nonzero(xyz[:, 0] == 0.0)[0]
+    xl = np.nonzero(xyz[:, 0] == length)[0]
+    org = int(np.nonzero((xyz == 0.0).all(1))[0][0])
+    y1 = int(np.nonzero((xyz[:, 0] == 0.0) & (xyz[:, 1] == 1.0) & (xyz[:, 2] == 0.0))[0][0])
+    out += _nset("X0", x0) + _nset("XL", xl)
+    out.append(f"*STEP, NLGEOM\n*STATIC\n{1.0 / ninc!r}, 1.0\n*BOUNDARY\nX0, 1, 1\n"
+               f"{org + 1}, 2, 3\n{y1 + 1}, 3, 3\nXL, 1, 1, {float(delta)!r}\n*END STEP\n")
+    return "".join(out)
+
+
+def nl_cantilever_deck(nx=6, ny=2, nz=2, dims=(6.0, 1.0, 1.0), p_total=-1.0, e=1000.0,
+                       nu=0.3, ninc=2, nn=8, pressure=None):
+    """A clamped cantilever under *STEP, NLGEOM: p_total spread over the tip nodes in z
+    (dead load), or a pressure on the top face (P2 of the top layer, a follower load)."""
+    from axia_gpu import meshgen as MG
+    import numpy as np
+    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(nx, ny, nz)
+    xyz = xyz * np.array([dims[0] / nx, dims[1] / ny, dims[2] / nz])
+    out = _solid_lines(nn, xyz, conn, e, nu, None)
+    fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
+    tip = np.nonzero(xyz[:, 0] == dims[0])[0]
+    out += _nset("FIX", fixed) + _nset("TIP", tip)
+    top = [k for k in range(len(conn)) if k // (nx * ny) == nz - 1]
+    if pressure is not None:
+        out.append("*ELSET, ELSET=TOP\n")
+        ids = [str(k + 1) for k in top]
+        for i in range(0, len(ids), 16):
+            out.append(", ".join(ids[i:i + 16]) + "\n")
+    out.append(f"*STEP, NLGEOM\n*STATIC\n{1.0 / ninc!r}, 1.0\n*BOUNDARY\nFIX, 1, 3\n")
+    if pressure is not None:
+        out.append(f"*DLOAD\nTOP, P2, {float(pressure)!r}\n")
+    else:
+        out.append(f"*CLOAD\nTIP, 3, {p_total / len(tip)!r}\n")
+    out.append("*END STEP\n")
+    return "".join(out)
