@@ -319,6 +319,16 @@ class Mesh:
         k = nf.value
         return vals[:k], (vecs[:k] if vectors else None), _info_dict(info)
 
+    def nl_assemble(self, u_full):
+        """NLGEOM tangent (into this mesh's K_ff) and internal force at u_full: returns
+        (f_int (3 nnode), element mean Cauchy (nelem, 6), element mean Green-Lagrange)."""
+        u = f64(u_full).ravel()
+        fint = np.zeros(3 * self.nnode)
+        cau = np.zeros((self.nelem, 6))
+        gl = np.zeros((self.nelem, 6))
+        check(_lib.lib().axg_nl_assemble(self._h, dptr(u), dptr(fint), dptr(cau), dptr(gl)))
+        return fint, cau, gl
+
     def compliance_adjoint(self, u_full, f_full):
         """c = f^T u and dc/dE_e = -u_e^T K1_e u_e (K1: Ke at unit modulus)."""
         u = f64(u_full).ravel()

@@ -123,7 +123,11 @@ SIGNATURES = {
     "axg_deck_destroy": (C.c_int, [_vp]),
     "axg_deck_info": (C.c_int, [_vp, C.POINTER(C.c_int32), _lp, _lp, _lp]),
     "axg_deck_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _ip, _dp, _dp, _lp, _dp, _dp]),
-    "axg_deck_nmodes": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "axg_deck_procedure": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "axg_deck_nlgeom": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _dp, _dp, _ip,
+                                  _ip, C.POINTER(SolveInfo)]),
+    "axg_nl_assemble": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "axg_deck_buckle": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _ip,
                                   C.POINTER(SolveInfo)]),
     "axg_geometric_stiffness": (C.c_int, [_vp, _dp, _dp, _dp]),

@@ -66,6 +66,21 @@ class BuckleResult:
     ndof: int
 
 
+@dataclass
+class NlResult:
+    node_ids: np.ndarray     # (nnode,) ascending node ids
+    u_full: np.ndarray       # (3 nnode,) displacements
+    rf: np.ndarray           # (3 nnode,) f_int - f_ext
+    stress: np.ndarray       # (nnode, 6) nodal Cauchy stress
+    von_mises: np.ndarray    # (nnode,)
+    strain: np.ndarray       # (nnode, 6) nodal Green-Lagrange strain
+    ninc: int                # increments taken
+    iters: int               # Newton iterations of the last increment
+    residual: float          # last ||r_free||
+    solver: str
+    pcg_iters: int           # summed PCG iterations of all corrections
+
+
 class Deck:
     """A parsed deck and its linear-static setup (host side of libaxia_gpu)."""
 
@@ -125,19 +140,44 @@ class Deck:
                             self.n_free, ndof)
 
 
+    def procedure(self):
+        """dict nmodes (*FREQUENCY), nbuckle (*BUCKLE), nlgeom (*STEP, NLGEOM), ninc."""
+        v = [C.c_int32() for _ in range(4)]
+        check(_lib.lib().axg_deck_procedure(self._h, *[C.byref(x) for x in v]))
+        return {"nmodes": v[0].value, "nbuckle": v[1].value, "nlgeom": bool(v[2].value),
+                "ninc": v[3].value}
+
     @property
     def nmodes(self):
         """Modes the deck's *FREQUENCY step asks for (0 without one)."""
-        n = C.c_int32()
-        check(_lib.lib().axg_deck_nmodes(self._h, C.byref(n), None))
-        return n.value
+        return self.procedure()["nmodes"]
 
     @property
     def nbuckle(self):
         """Buckling factors the deck's *BUCKLE step asks for (0 without one)."""
-        n = C.c_int32()
-        check(_lib.lib().axg_deck_nmodes(self._h, None, C.byref(n)))
-        return n.value
+        return self.procedure()["nbuckle"]
+
+    def nlgeom(self, ctx=None, precision="fp64", precond="block3", tol=1e-10, max_iter=0,
+               inner_tol=1e-3):
+        """solve_continuum_newton on the device: NlResult (u, rf, nodal Cauchy stress, von
+        Mises, Green-Lagrange strain, increments, Newton iterations of the last increment)."""
+        from . import _info_dict, _opts, default_context
+        ctx = ctx or default_context()
+        if not self.procedure()["nlgeom"]:
+            raise ValueError("the deck has no *STEP, NLGEOM")
+        ndof = 3 * self.nnode
+        u, rf = np.zeros(ndof), np.zeros(ndof)
+        stress, strain = np.zeros((self.nnode, 6)), np.zeros((self.nnode, 6))
+        vm = np.zeros(self.nnode)
+        ninc, its = C.c_int32(), C.c_int32()
+        info = _lib.SolveInfo()
+        o = _opts(precision, precond, tol, max_iter, inner_tol)
+        check(_lib.lib().axg_deck_nlgeom(ctx.handle, self._h, C.byref(o), dptr(u), dptr(rf),
+                                         dptr(stress), dptr(vm), dptr(strain), C.byref(ninc),
+                                         C.byref(its), C.byref(info)))
+        inf = _info_dict(info)
+        return NlResult(self.arrays()["node_ids"], u, rf, stress, vm, strain, ninc.value,
+                        its.value, inf["residual"], inf["name"], inf["iters"])
 
     def buckle(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
                inner_tol=1e-3, modes=True):
@@ -208,3 +248,9 @@ def buckle_deck(text, ctx=None, **kw):
     """Parse a *BUCKLE deck and run it: (Deck, BuckleResult)."""
     d = Deck(text)
     return d, d.buckle(ctx=ctx, **kw)
+
+
+def nlgeom_deck(text, ctx=None, **kw):
+    """Parse a *STEP, NLGEOM deck and run it: (Deck, NlResult)."""
+    d = Deck(text)
+    return d, d.nlgeom(ctx=ctx, **kw)

@@ -134,6 +134,148 @@ void geometric_stiffness_device(axg_mesh *mesh, const double *d_u, double sign, 
 }
 }  // namespace axg
 
+// Drop the slots whose contributions were all exact zeros (the reference
+// never creates them: analysis.rs:664, linalg.rs:11).
+static void compact_pattern(axg_mesh *m) {
+    hipStream_t s = m->ctx->stream;
+    int64_t nslot = m->nnz_topo, n = m->n_free;
+    std::vector<int32_t> rp(n + 1), ci(nslot);
+    std::vector<uint32_t> sp(nslot + 1), cb(m->ncontrib_topo);
+    std::vector<uint8_t> nz(nslot);
+    AXG_HIP(hipMemcpyAsync(rp.data(), m->rowptr_t.p, 4 * (n + 1), hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(ci.data(), m->colind_t.p, 4 * nslot, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(sp.data(), m->slotptr_t.p, 4 * (nslot + 1), hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(cb.data(), m->contrib_t.p, 4 * m->ncontrib_topo,
+                           hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(nz.data(), m->slot_nz.p, nslot, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> rp2(n + 1, 0), ci2;
+    std::vector<uint32_t> sp2, cb2;
+    ci2.reserve(nslot);
+    sp2.reserve(nslot + 1);
+    cb2.reserve(cb.size());
+    for (int64_t i = 0; i < n; i++) {
+        for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
+            if (!nz[k]) continue;
+            ci2.push_back(ci[k]);
+            sp2.push_back((uint32_t)cb2.size());
+            for (uint32_t c = sp[k]; c < sp[k + 1]; c++) cb2.push_back(cb[c]);
+        }
+        rp2[i + 1] = (int32_t)ci2.size();
+    }
+    sp2.push_back((uint32_t)cb2.size());
+    auto up = [&](auto &buf, const auto &v) {
+        buf.alloc(v.size());
+        if (!v.empty())
+            AXG_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice,
+                                   s));
+    };
+    up(m->rowptr_c, rp2);
+    up(m->colind_c, ci2);
+    up(m->slotptr_c, sp2);
+    up(m->contrib_c, cb2);
+    AXG_HIP(hipStreamSynchronize(s));
+    m->compacted = true;
+    m->nnz = (int64_t)ci2.size();
+    m->ncontrib = (int64_t)cb2.size();
+}
+
+// After the topological gather of K_ff from mesh->ke: drop the slots whose
+// contributions were all exact zeros (re-gathering the compacted pattern), or
+// return to the topological pattern when there are none; then the Jacobi
+// divisor and the validity flags of the derived operators.
+static void finish_gather64(axg_mesh *mesh, uint32_t zero_slots) {
+    axg_ctx *ctx = mesh->ctx;
+    hipStream_t s = ctx->stream;
+    if (zero_slots > 0) {
+        compact_pattern(mesh);
+        launch_gather_fp64(mesh, s, false, nullptr);
+        mesh->sell_valid = false;
+    } else if (mesh->compacted) {
+        mesh->compacted = false;
+        mesh->nnz = mesh->nnz_topo;
+        mesh->ncontrib = mesh->ncontrib_topo;
+        mesh->sell_valid = false;
+    }
+    mesh->sell_fresh = false;
+    mesh->bsell64_fresh = false;
+    CsrView A = active_csr(mesh);
+    diag_fp64(ctx, A, mesh->dsel.p, nullptr);
+    AXG_HIP(hipStreamSynchronize(s));
+    mesh->assembled64 = true;
+    mesh->binv_valid = false;
+    mesh->assembled32 = false;
+}
+
+namespace axg {
+// NLGEOM FemError texts (nlgeom.rs:93, 117-125, 240-272)
+static std::string nl_error(int code, int nn, int32_t el, double e, double nu) {
+    char buf[200];
+    switch (code) {
+        case AXG_E_INVERSION:
+            return "NLGEOM: det(F) ≤ 0 (Element inversion).";
+        case AXG_E_NEG_JACOBIAN:
+            snprintf(buf, sizeof buf, "NLGEOM %s: negative Jakobideterminante.", elem_name(nn));
+            return buf;
+        case AXG_E_MATERIAL:
+            snprintf(buf, sizeof buf, "Ungültiges Material E=%g nu=%g", e, nu);
+            return buf;
+        default:
+            return element_error(code, nn, el, nullptr, nullptr);
+    }
+}
+
+// The NLGEOM tangent of every element into mesh->ke, its internal forces (d_fe,
+// nelem x 3nn), Cauchy and Green-Lagrange means (nelem x 6), K_T gathered into
+// the mesh's K_ff (exact-zero slots dropped as for K) and f_int over all DOFs.
+void nl_assemble_device(axg_mesh *mesh, const double *d_u, double *d_fe, double *d_cau,
+                        double *d_gl, double *d_fint) {
+    axg_ctx *ctx = mesh->ctx;
+    hipStream_t s = ctx->stream;
+    Workspace &w = ctx->ws;
+    AXG_HIP(hipMemsetAsync(w.err.p, 0, 4 * sizeof(int32_t), s));
+    AXG_HIP(hipMemsetAsync(w.cnt.p, 0, 4 * sizeof(uint32_t), s));
+    launch_nl(mesh, s, d_u, d_fe, d_cau, d_gl, w.err.p);
+    launch_gather_fp64(mesh, s, true, w.cnt.p);
+    launch_nl_fint(mesh, s, d_fe, d_fint);
+    int32_t err[2];
+    uint32_t zero_slots = 0;
+    AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(&zero_slots, w.cnt.p, 4, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    if (err[0] != 0) {
+        double e = 0.0, nu = 0.0;
+        if (err[1] >= 0 && err[1] < mesh->nelem) {
+            AXG_HIP(hipMemcpy(&e, mesh->e.p + err[1], 8, hipMemcpyDeviceToHost));
+            AXG_HIP(hipMemcpy(&nu, mesh->nu.p + err[1], 8, hipMemcpyDeviceToHost));
+        }
+        throw Error{err[0], nl_error(err[0], mesh->nn, err[1], e, nu)};
+    }
+    finish_gather64(mesh, zero_slots);
+}
+
+// Nodal Cauchy and Green-Lagrange of an NLGEOM state (analysis.rs:4745-4799)
+// and the element means stress_gp; d_cau / d_gl: the element means of the
+// last tangent assembly at the same displacements (C3D8).
+void nl_nodal_device(axg_mesh *mesh, const double *d_u, const double *d_cau, const double *d_gl,
+                     double *d_stress, double *d_vm, double *d_strain, double *d_gp) {
+    axg_ctx *ctx = mesh->ctx;
+    hipStream_t s = ctx->stream;
+    Workspace &w = ctx->ws;
+    const size_t ne = (size_t)std::max<int64_t>(mesh->nelem, 1) * mesh->nn * 6;
+    DevBuf<double> enc(ne), eng(ne), dummy((size_t)std::max<int64_t>(mesh->nnode, 1) * 6);
+    AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
+    launch_nl_nodal(mesh, s, d_u, d_cau, d_gl, enc.p, eng.p, w.err.p);
+    launch_nodal_average(mesh, s, enc.p, d_stress, d_vm);
+    launch_nodal_average(mesh, s, eng.p, d_strain ? d_strain : dummy.p, nullptr);
+    if (d_gp) launch_element_mean(mesh, s, enc.p, d_gp);
+    int32_t err[2];
+    AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    if (err[0]) throw Error{err[0], nl_error(err[0], mesh->nn, err[1], 0.0, 0.0)};
+}
+}  // namespace axg
+
 extern "C" {
 
 const char *axg_last_error(void) { return g_err.c_str(); }
@@ -462,52 +604,6 @@ int axg_mesh_set_modulus(axg_mesh *mesh, const double *e) {
     API_END
 }
 
-// Drop the slots whose contributions were all exact zeros (the reference
-// never creates them: analysis.rs:664, linalg.rs:11).
-static void compact_pattern(axg_mesh *m) {
-    hipStream_t s = m->ctx->stream;
-    int64_t nslot = m->nnz_topo, n = m->n_free;
-    std::vector<int32_t> rp(n + 1), ci(nslot);
-    std::vector<uint32_t> sp(nslot + 1), cb(m->ncontrib_topo);
-    std::vector<uint8_t> nz(nslot);
-    AXG_HIP(hipMemcpyAsync(rp.data(), m->rowptr_t.p, 4 * (n + 1), hipMemcpyDeviceToHost, s));
-    AXG_HIP(hipMemcpyAsync(ci.data(), m->colind_t.p, 4 * nslot, hipMemcpyDeviceToHost, s));
-    AXG_HIP(hipMemcpyAsync(sp.data(), m->slotptr_t.p, 4 * (nslot + 1), hipMemcpyDeviceToHost, s));
-    AXG_HIP(hipMemcpyAsync(cb.data(), m->contrib_t.p, 4 * m->ncontrib_topo,
-                           hipMemcpyDeviceToHost, s));
-    AXG_HIP(hipMemcpyAsync(nz.data(), m->slot_nz.p, nslot, hipMemcpyDeviceToHost, s));
-    AXG_HIP(hipStreamSynchronize(s));
-    std::vector<int32_t> rp2(n + 1, 0), ci2;
-    std::vector<uint32_t> sp2, cb2;
-    ci2.reserve(nslot);
-    sp2.reserve(nslot + 1);
-    cb2.reserve(cb.size());
-    for (int64_t i = 0; i < n; i++) {
-        for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
-            if (!nz[k]) continue;
-            ci2.push_back(ci[k]);
-            sp2.push_back((uint32_t)cb2.size());
-            for (uint32_t c = sp[k]; c < sp[k + 1]; c++) cb2.push_back(cb[c]);
-        }
-        rp2[i + 1] = (int32_t)ci2.size();
-    }
-    sp2.push_back((uint32_t)cb2.size());
-    auto up = [&](auto &buf, const auto &v) {
-        buf.alloc(v.size());
-        if (!v.empty())
-            AXG_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice,
-                                   s));
-    };
-    up(m->rowptr_c, rp2);
-    up(m->colind_c, ci2);
-    up(m->slotptr_c, sp2);
-    up(m->contrib_c, cb2);
-    AXG_HIP(hipStreamSynchronize(s));
-    m->compacted = true;
-    m->nnz = (int64_t)ci2.size();
-    m->ncontrib = (int64_t)cb2.size();
-}
-
 int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
     API_BEGIN
     AXG_REQUIRE(mesh, AXG_E_INVALID, "axg_assemble: NULL mesh");
@@ -540,24 +636,7 @@ int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms) {
         AXG_HIP(hipMemcpy(nu.data(), mesh->nu.p, 8 * mesh->nelem, hipMemcpyDeviceToHost));
         throw Error{err[0], element_error(err[0], mesh->nn, err[1], e.data(), nu.data())};
     }
-    if (zero_slots > 0) {
-        compact_pattern(mesh);
-        launch_gather_fp64(mesh, s, false, nullptr);
-        mesh->sell_valid = false;
-    } else if (mesh->compacted) {
-        mesh->compacted = false;
-        mesh->nnz = mesh->nnz_topo;
-        mesh->ncontrib = mesh->ncontrib_topo;
-        mesh->sell_valid = false;
-    }
-    mesh->sell_fresh = false;
-    mesh->bsell64_fresh = false;
-    CsrView A = active_csr(mesh);
-    diag_fp64(ctx, A, mesh->dsel.p, nullptr);
-    AXG_HIP(hipStreamSynchronize(s));
-    mesh->assembled64 = true;
-    mesh->binv_valid = false;
-    mesh->assembled32 = false;
+    finish_gather64(mesh, zero_slots);
     if (precision == AXG_PREC_MIXED) {
         double ms32 = 0.0;
         assemble_mixed(mesh, &ms32);
@@ -757,6 +836,25 @@ int axg_geometric_stiffness(axg_mesh *mesh, const double *u_full, double *mean, 
         AXG_HIP(hipMemcpyAsync(mean, dmean.p, 8 * (size_t)mesh->nelem * 6, hipMemcpyDeviceToHost, s));
     if (kg_val && mesh->nnz_topo > 0)
         AXG_HIP(hipMemcpyAsync(kg_val, dval.p, 8 * mesh->nnz_topo, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    API_END
+}
+
+int axg_nl_assemble(axg_mesh *mesh, const double *u_full, double *f_int, double *cauchy,
+                    double *gl) {
+    API_BEGIN
+    AXG_REQUIRE(mesh && u_full, AXG_E_INVALID, "axg_nl_assemble: NULL argument");
+    axg_ctx *ctx = mesh->ctx;
+    AXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int64_t ne = std::max<int64_t>(mesh->nelem, 1);
+    DevBuf<double> du(std::max<int64_t>(mesh->ndof, 1)), fe((size_t)ne * mesh->m), cau(6 * ne),
+        glb(6 * ne), fint(std::max<int64_t>(mesh->ndof, 1));
+    AXG_HIP(hipMemcpyAsync(du.p, u_full, 8 * mesh->ndof, hipMemcpyHostToDevice, s));
+    nl_assemble_device(mesh, du.p, fe.p, cau.p, glb.p, fint.p);
+    if (f_int) AXG_HIP(hipMemcpyAsync(f_int, fint.p, 8 * mesh->ndof, hipMemcpyDeviceToHost, s));
+    if (cauchy) AXG_HIP(hipMemcpyAsync(cauchy, cau.p, 8 * 6 * mesh->nelem, hipMemcpyDeviceToHost, s));
+    if (gl) AXG_HIP(hipMemcpyAsync(gl, glb.p, 8 * 6 * mesh->nelem, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
     API_END
 }

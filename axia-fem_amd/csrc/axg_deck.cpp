@@ -188,6 +188,9 @@ struct axg_deck {
     std::map<std::string, double> densities;  // *DENSITY per material
     int32_t nmodes = 0;                       // *FREQUENCY n (0: static step)
     int32_t nbuckle = 0;                      // *BUCKLE n (0: no buckling step)
+    bool nlgeom = false;                      // *STEP, NLGEOM
+    int32_t max_inc = 100;                    // *STEP, INC=
+    int32_t ninc = 1;                         // increments of the static step
     std::map<std::string, std::string> elset_material;
     std::vector<NodeLoad> bcs, cloads, dloads;
     std::vector<Deferred> deferred;
@@ -199,6 +202,9 @@ struct axg_deck {
     std::vector<int32_t> conn;
     std::vector<int64_t> ind_of;
     std::vector<uint8_t> prescribed;
+    // the prescribed set and values before autospc (the NLGEOM driver has none)
+    std::vector<uint8_t> prescribed0;
+    std::vector<double> u00;
 
     std::vector<int32_t> expand_nset(const std::string &name) const {
         if (is_int_token(name)) return {(int32_t)std::strtoll(trim(name).c_str(), nullptr, 10)};
@@ -529,6 +535,29 @@ void parse_deck(axg_deck &m, const std::string &text) {
             int32_t nm = 10;
             if (!toks.empty() && !parse_i32(toks[0], nm)) nm = 10;
             m.nmodes = std::max(nm, 1);
+        } else if (kw == "*STEP") {
+            // inp.rs:953-982: NLGEOM (with any value) makes the step's static
+            // procedure geometrically nonlinear; INC= caps the increments
+            if (params.count("NLGEOM")) {
+                m.nlgeom = true;
+                m.ninc = 1;
+            }
+            if (params.count("INC")) {
+                int32_t v;
+                if (parse_i32(params["INC"], v) && v > 0) m.max_inc = v;
+            }
+            i++;
+        } else if (kw == "*STATIC") {
+            // inp.rs:983-1061 without RIKS: initial increment, period
+            i = collect(i + 1, toks);
+            auto positive = [&](size_t k) {
+                double v;
+                return (toks.size() > k && parse_f64(toks[k], v) && v > 0.0) ? v : 1.0;
+            };
+            const double dt0 = positive(0), period = positive(1);
+            const long long n_from_time = std::max<long long>(std::llround(period / dt0), 1);
+            m.ninc = (int32_t)std::max<long long>(
+                std::min<long long>(n_from_time, std::max<int32_t>(m.max_inc, 1)), 1);
         } else if (kw == "*BUCKLE") {
             // inp.rs:1072-1081: n factors, 1 when absent or unreadable, at least 1
             i = collect(i + 1, toks);
@@ -662,6 +691,8 @@ void linear_setup(axg_deck &m) {
     for (int64_t i = 0; i < m.nnode; i++)
         if (!attached[i])
             for (int d = 0; d < 3; d++) m.prescribed[3 * i + d] = 1;  // u0 keeps its value (0)
+    m.prescribed0 = m.prescribed;
+    m.u00 = m.u0;
     // pressure faces: elements ascending, each element's faces in deck order
     m.f_full.assign(m.ndof, 0.0);
     std::map<int64_t, std::vector<std::pair<int, double>>> by_elem;
@@ -828,11 +859,14 @@ static void static_solve(axg_deck &d, DeckMesh &g, const axg_solve_opts *opts, d
     reconstruct(d, x.data(), u_full);
 }
 
-int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle) {
+int axg_deck_procedure(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle, int32_t *nlgeom,
+                       int32_t *ninc) {
     API_BEGIN
-    AXG_REQUIRE(deck, AXG_E_INVALID, "axg_deck_nmodes: NULL deck");
+    AXG_REQUIRE(deck, AXG_E_INVALID, "axg_deck_procedure: NULL deck");
     if (nmodes) *nmodes = deck->nmodes;
     if (nbuckle) *nbuckle = deck->nbuckle;
+    if (nlgeom) *nlgeom = deck->nlgeom ? 1 : 0;
+    if (ninc) *ninc = deck->ninc;
     API_END
 }
 
@@ -890,6 +924,255 @@ int axg_deck_frequency(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts,
         if (modes_full) reconstruct(d, vecs.data() + (size_t)j * d.n_free, modes_full + (size_t)j * d.ndof);
     }
     *nfound = k;
+    API_END
+}
+
+// assemble_fext_u (analysis.rs:3222-3300) for nodal loads and face pressure:
+// the nodal loads summed per (node, dof) and added first, then each element's
+// pressure faces in element order, on the deformed coordinates x + u when u is
+// given (follower pressure).
+static void nl_fext(const axg_deck &d, const double *u, std::vector<double> &f) {
+    f.assign(d.ndof, 0.0);
+    std::map<int64_t, double> acc;
+    for (auto &cl : d.cloads) {
+        if (cl.k >= 3) continue;
+        auto it = d.id_to_index.find(cl.id);
+        AXG_REQUIRE(it != d.id_to_index.end(), AXG_E_INVALID,
+                    "Unbekannter Knoten " + std::to_string(cl.id));
+        const int64_t dof = 3 * it->second + cl.k;
+        auto a = acc.find(dof);
+        if (a == acc.end())
+            acc.emplace(dof, 0.0 + cl.v * 1.0);
+        else
+            a->second += cl.v * 1.0;
+    }
+    for (auto &kv : acc) f[kv.first] += kv.second;
+    if (d.dloads.empty()) return;
+    std::unordered_map<int32_t, int64_t> eidx;
+    for (int64_t k = 0; k < d.nelem; k++) eidx.emplace(d.elements[k].id, k);
+    std::map<int64_t, std::vector<std::pair<int, double>>> by_elem;
+    for (auto &dl : d.dloads) {
+        auto it = eidx.find(dl.id);
+        AXG_REQUIRE(it != eidx.end(), AXG_E_INVALID,
+                    "DLOAD auf unbekanntes Element " + std::to_string(dl.id));
+        by_elem[it->second].push_back({dl.k, dl.v});
+    }
+    const int nn = d.nn;
+    double exyz[60], fe[60];
+    for (auto &kv : by_elem) {
+        const int32_t *en = &d.conn[kv.first * nn];
+        for (int a = 0; a < nn; a++)
+            for (int q = 0; q < 3; q++) {
+                const int64_t g = 3 * (int64_t)en[a] + q;
+                exyz[3 * a + q] = u ? d.xyz[g] + u[g] : d.xyz[g];
+            }
+        for (auto &fm : kv.second) {
+            int st = face_pressure_host(nn, exyz, fm.first, fm.second, fe);
+            AXG_REQUIRE(st == AXG_OK, AXG_E_INVALID,
+                        std::string("Ungültige ") + (nn == 8 ? "C3D8" : "C3D20") + "-Fläche P" +
+                            std::to_string(fm.first));
+            for (int a = 0; a < nn; a++)
+                for (int q = 0; q < 3; q++) f[3 * (int64_t)en[a] + q] += fe[3 * a + q];
+        }
+    }
+}
+
+static double norm_seq(const std::vector<double> &v) {
+    double s = 0.0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+}
+
+// solve_continuum_newton (analysis.rs:4313-4860) for an all-hex St. Venant-
+// Kirchhoff deck without contact, pretension, rigid bodies, plasticity or
+// amplitudes: load factor steps of 1 / ninc (x1.5 after a success, halved
+// on failure, at most 16 retries, 1e-6 minimum), prescribed values and loads
+// ramped with the factor, follower pressure re-assembled at every iteration,
+// full Newton with the consistent tangent assembled on the device and the
+// GPU PCG of opts, convergence at ||r_free|| < 1e-8 (1 + ||f||) within 25
+// iterations. Results: u_full, rf = f_int - f_ext, nodal Cauchy stress, von
+// Mises and Green-Lagrange strain (each may be NULL).
+int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
+                    double *rf, double *stress, double *vm, double *strain, int32_t *ninc_done,
+                    int32_t *newton_iters, axg_solve_info *info) {
+    API_BEGIN
+    AXG_REQUIRE(ctx && deck && u_full, AXG_E_INVALID, "axg_deck_nlgeom: invalid argument");
+    const axg_deck &d = *deck;
+    AXG_REQUIRE(d.nlgeom, AXG_E_INVALID, "axg_deck_nlgeom: the deck has no *STEP, NLGEOM");
+    const double newton_tol = 1e-8;  // model.rs:1173
+    const int max_newton = 25;       // model.rs:1172
+    const int64_t ndof = d.ndof;
+    // DofMap::build over the step's prescribed DOFs (no autospc on this path)
+    std::vector<int64_t> ind_of(ndof, -1);
+    std::vector<int64_t> pres;
+    int64_t n_free = 0;
+    for (int64_t i = 0; i < ndof; i++) {
+        if (d.prescribed0[i])
+            pres.push_back(i);
+        else
+            ind_of[i] = n_free++;
+    }
+    AXG_REQUIRE(n_free > 0, AXG_E_INVALID, "NLGEOM Kontinuum: keine freien DOF.");
+    DeckMesh g;
+    check_status(axg_mesh_create(ctx, d.nn, d.nnode, d.xyz.data(), d.nelem, d.conn.data(),
+                                 d.e.data(), d.nu.data(), ind_of.data(), &g.m));
+    axg_mesh *mesh = g.m;
+    hipStream_t s = ctx->stream;
+    const int64_t ne = std::max<int64_t>(d.nelem, 1), m = 3 * d.nn;
+    DevBuf<double> du(std::max<int64_t>(ndof, 1)), fe((size_t)ne * m), cau(6 * ne), glb(6 * ne),
+        fint(std::max<int64_t>(ndof, 1));
+    std::vector<double> f_tgt, f_0(ndof, 0.0), f_inc(ndof), fd, f_int(ndof), last_fint(ndof, 0.0);
+    nl_fext(d, nullptr, f_tgt);
+    const bool follower = !d.dloads.empty();
+    const int ninc = std::max<int32_t>(d.ninc, 1);
+    std::vector<double> u(ndof, 0.0), u_work, last_du(ndof), r(ndof), rhs(n_free), x(n_free),
+        dsel_h(n_free);
+    std::vector<double> f_now(f_0);
+    double lam = 0.0, dt = 1.0 / ninc, residual = 0.0;
+    int done_inc = 0, retries = 0, iters = 0;
+    int64_t pcg_iters = 0;
+    double pcg_ms = 0.0;
+    std::string solver_name;
+    auto upload = [&](const std::vector<double> &v) {
+        AXG_HIP(hipMemcpyAsync(du.p, v.data(), 8 * ndof, hipMemcpyHostToDevice, s));
+    };
+    while (lam < 1.0 - 1e-14) {
+        const double lam_new = std::min(lam + dt, 1.0);
+        u_work = u;
+        for (int64_t p : pres) u_work[p] = (1.0 - lam_new) * 0.0 + lam_new * d.u00[p];
+        for (int64_t i = 0; i < ndof; i++) f_inc[i] = (1.0 - lam_new) * f_0[i] + lam_new * f_tgt[i];
+        std::fill(last_du.begin(), last_du.end(), 0.0);
+        bool have_du = false, inc_ok = false;
+        std::string inc_err;
+        for (int it = 0; it < max_newton; it++) {
+            iters = it + 1;
+            if (follower) {
+                nl_fext(d, u_work.data(), fd);
+                for (int64_t i = 0; i < ndof; i++) f_inc[i] = (1.0 - lam_new) * f_0[i] + lam_new * fd[i];
+            }
+            upload(u_work);
+            try {
+                nl_assemble_device(mesh, du.p, fe.p, cau.p, glb.p, fint.p);
+            } catch (const Error &ex) {
+                inc_err = ex.msg;
+                if (ex.code == AXG_E_INVERSION && have_du) {
+                    double nrm = 0.0;
+                    for (int64_t i = 0; i < ndof; i++) {
+                        u_work[i] -= 0.5 * last_du[i];
+                        last_du[i] *= 0.5;
+                    }
+                    for (int64_t p : pres) u_work[p] = (1.0 - lam_new) * 0.0 + lam_new * d.u00[p];
+                    nrm = norm_seq(last_du);
+                    if (nrm > 1e-18) {
+                        inc_err.clear();
+                        continue;
+                    }
+                }
+                break;
+            }
+            AXG_HIP(hipMemcpyAsync(f_int.data(), fint.p, 8 * ndof, hipMemcpyDeviceToHost, s));
+            AXG_HIP(hipStreamSynchronize(s));
+            last_fint = f_int;
+            // DofMap::reduce_inc (constraint.rs:175-205): free rows of f_inc - f_int
+            for (int64_t i = 0; i < ndof; i++) r[i] = f_inc[i] - f_int[i];
+            std::fill(rhs.begin(), rhs.end(), 0.0);
+            for (int64_t i = 0; i < ndof; i++)
+                if (ind_of[i] >= 0 && std::fabs(r[i]) != 0.0) rhs[ind_of[i]] += 1.0 * r[i];
+            residual = norm_seq(rhs);
+            const double fref = norm_seq(f_inc);
+            if (residual < newton_tol * (1.0 + fref)) {
+                inc_ok = true;
+                break;
+            }
+            if (it + 1 == max_newton) {
+                char buf[160];
+                std::snprintf(buf, sizeof buf, "Newton konvergierte nicht (r=%.3e nach %d Iterationen).",
+                              residual, iters);
+                inc_err = buf;
+                break;
+            }
+            // The reference solves K_T du = r directly; the PCG's absolute
+            // breakdown guard (|p^T K p| < 1e-30, linalg.rs:231-233) would stop it
+            // near convergence, where r is small. The right-hand side is divided
+            // by the power of two nearest sqrt(r^T D^-1 r) (D: the Jacobi divisor)
+            // and the solution multiplied back: exact, as a power of two changes no
+            // rounding.
+            AXG_HIP(hipMemcpyAsync(dsel_h.data(), mesh->dsel.p, 8 * n_free, hipMemcpyDeviceToHost, s));
+            AXG_HIP(hipStreamSynchronize(s));
+            double beta = 0.0;
+            for (int64_t k = 0; k < n_free; k++) beta += rhs[k] * rhs[k] / dsel_h[k];
+            double scale = 1.0;
+            if (beta > 0.0 && std::isfinite(beta)) {
+                scale = std::ldexp(1.0, (int)std::lround(0.5 * std::log2(beta)));
+                for (double &v : rhs) v /= scale;
+            }
+            axg_solve_info si;
+            const int st = axg_mesh_solve(mesh, rhs.data(), 1, opts, x.data(), &si);
+            if (st != AXG_OK) {
+                inc_err = axg_last_error();
+                break;
+            }
+            for (double &v : x) v *= scale;
+            solver_name = si.name;
+            pcg_iters += si.iters;
+            pcg_ms += si.solve_ms;
+            for (int64_t i = 0; i < ndof; i++) {
+                const double dv = ind_of[i] >= 0 ? 0.0 + 1.0 * x[ind_of[i]] : 0.0;
+                u_work[i] += dv;
+                last_du[i] = dv;
+            }
+            for (int64_t p : pres) u_work[p] = (1.0 - lam_new) * 0.0 + lam_new * d.u00[p];
+            have_du = true;
+            if (norm_seq(x) < 1e-14) {
+                inc_ok = true;
+                break;
+            }
+        }
+        if (inc_ok) {
+            u = u_work;
+            lam = lam_new;
+            f_now = f_inc;
+            done_inc++;
+            retries = 0;
+            const double remain = 1.0 - lam;
+            if (remain > 1e-14) dt = std::min(std::min(dt * 1.5, 1.0 / ninc), remain);
+        } else {
+            retries++;
+            dt *= 0.5;
+            if (dt < 1e-6 || retries > 16) {
+                char buf[96];
+                std::snprintf(buf, sizeof buf, "NLGEOM Inkrement bei λ=%.4f fehlgeschlagen (", lam);
+                throw Error{AXG_E_NOT_CONVERGED,
+                            std::string(buf) + (inc_err.empty() ? "unbekannt" : inc_err) + ")."};
+            }
+        }
+    }
+    std::copy(u.begin(), u.end(), u_full);
+    if (rf)
+        for (int64_t i = 0; i < ndof; i++) rf[i] = last_fint[i] - f_now[i];
+    if (stress || vm || strain) {
+        const int64_t nn6 = std::max<int64_t>(d.nnode, 1) * 6;
+        DevBuf<double> dst(nn6), dvm(std::max<int64_t>(d.nnode, 1)), dsn(nn6);
+        upload(u);
+        nl_nodal_device(mesh, du.p, cau.p, glb.p, dst.p, dvm.p, dsn.p, nullptr);
+        if (stress) AXG_HIP(hipMemcpyAsync(stress, dst.p, 8 * d.nnode * 6, hipMemcpyDeviceToHost, s));
+        if (vm) AXG_HIP(hipMemcpyAsync(vm, dvm.p, 8 * d.nnode, hipMemcpyDeviceToHost, s));
+        if (strain) AXG_HIP(hipMemcpyAsync(strain, dsn.p, 8 * d.nnode * 6, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+    }
+    if (ninc_done) *ninc_done = done_inc;
+    if (newton_iters) *newton_iters = iters;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iters = pcg_iters;
+        info->outer_iters = iters;
+        info->nrhs = 1;
+        info->residual = residual;
+        info->rel_residual = residual / std::max(norm_seq(f_now), 1e-300);
+        info->solve_ms = pcg_ms;
+        std::snprintf(info->name, sizeof(info->name), "Newton (%.32s, %d inc)",
+                      solver_name.empty() ? "equilibrium" : solver_name.c_str(), done_inc);
+    }
     API_END
 }
 

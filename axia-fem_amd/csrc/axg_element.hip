@@ -704,6 +704,328 @@ __global__ void gather_kg_kernel(const uint32_t *__restrict__ slotptr,
     }
 }
 
+// ------------------------------------------------------------- NLGEOM
+// Total-Lagrange St. Venant-Kirchhoff hex (nlgeom.rs:127-273), one element per
+// workgroup: per Gauss point (the stiffness rule, in order) F = I + sum u (x)
+// grad N, E = (F^T F - I) / 2, S = lam tr(E) I + 2 mu E, B_nl (fill_b_nl),
+// tmp = D B_nl; each thread owns entries (i, j) of the tangent and adds
+// w sum_k B[k][i] tmp[k][j] (gemm_bt_d_b) and then, on equal directions, the
+// geometric term (grad N_a . S . grad N_b) w (add_continuum_kg); f_e[j] +=
+// (sum_i B[i][j] S[i]) w. Cauchy and Green-Lagrange are weighted and summed in
+// point order and divided by the volume (GpAcc::finish). All in the
+// reference's operation order (this file is built without FMA contraction).
+template <int NN, int NG, int NT>
+__global__ void __launch_bounds__(NT) nl_kernel(const double *__restrict__ xyz,
+                                                const int32_t *__restrict__ conn,
+                                                const double *__restrict__ E,
+                                                const double *__restrict__ NU, int64_t nelem,
+                                                const double *__restrict__ u,
+                                                double *__restrict__ ke,
+                                                double *__restrict__ fe_out,
+                                                double *__restrict__ cau_out,
+                                                double *__restrict__ gl_out,
+                                                int32_t *__restrict__ err) {
+    constexpr int M = 3 * NN;
+    constexpr int KPT = (M * M + NT - 1) / NT;
+    __shared__ double s_x[3 * NN], s_u[3 * NN];
+    __shared__ double s_dndx[NG][3 * NN];
+    __shared__ double s_w[NG];
+    __shared__ double s_f[NG][9], s_s[NG][6], s_c[NG][6], s_g[NG][6];
+    __shared__ double s_b[6][M], s_t[6][M];
+    __shared__ double s_mat[3];
+    __shared__ int s_bad;
+    const int64_t e = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (e >= nelem) return;
+    for (int t = tid; t < NN; t += NT) {
+        const int64_t node = conn[e * NN + t];
+        for (int q = 0; q < 3; q++) {
+            s_x[3 * t + q] = xyz[3 * node + q];
+            s_u[3 * t + q] = u[3 * node + q];
+        }
+    }
+    if (tid == 0) {
+        s_bad = 0;
+        // lam_mu (nlgeom.rs:117-125) and d_iso_3d's diagonal
+        const double ev = E[e], nv = NU[e];
+        if (ev <= 0.0 || nv >= 0.5 || nv <= -1.0) {
+            report(err, AXG_E_MATERIAL, e);
+            s_bad = 1;
+            s_mat[0] = s_mat[1] = s_mat[2] = 0.0;
+        } else {
+            const double lam = ev * nv / ((1.0 + nv) * (1.0 - 2.0 * nv));
+            const double mu = ev / (2.0 * (1.0 + nv));
+            s_mat[0] = lam;
+            s_mat[1] = mu;
+            s_mat[2] = lam + 2.0 * mu;
+        }
+    }
+    __syncthreads();
+    for (int g = tid; g < NG; g += NT) {
+        double det;
+        const double(*dn)[3] = (NN == 8) ? c_tab.h8_dn[g] : c_tab.h20_dn[g];
+        const double *dx = s_dndx[g];
+        int st = geometry<NN>(dn, s_x, s_dndx[g], det);
+        if (st != AXG_OK) {
+            report(err, st, e);
+            s_bad = 1;
+            continue;
+        }
+        if (det <= 0.0) {
+            report(err, AXG_E_NEG_JACOBIAN, e);
+            s_bad = 1;
+            continue;
+        }
+        s_w[g] = (NN == 8) ? det : c_tab.h20_w[g] * det;
+        // deformation_gradient (nlgeom.rs:50-64)
+        double f[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+        for (int a = 0; a < NN; a++)
+            for (int i = 0; i < 3; i++) {
+                const double uu = s_u[3 * a + i];
+                for (int j = 0; j < 3; j++) f[i][j] += uu * dx[3 * a + j];
+            }
+        const double jdet = f[0][0] * (f[1][1] * f[2][2] - f[1][2] * f[2][1]) -
+                            f[0][1] * (f[1][0] * f[2][2] - f[1][2] * f[2][0]) +
+                            f[0][2] * (f[1][0] * f[2][1] - f[1][1] * f[2][0]);
+        if (jdet <= 1e-18) {
+            report(err, AXG_E_INVERSION, e);
+            s_bad = 1;
+            continue;
+        }
+        // green_lagrange (nlgeom.rs:34-48), voigt_s (nlgeom.rs:22-32)
+        double eg[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double c = f[0][i] * f[0][j] + f[1][i] * f[1][j] + f[2][i] * f[2][j];
+                eg[i][j] = 0.5 * (c - (i == j ? 1.0 : 0.0));
+            }
+        const double lam = s_mat[0], mu = s_mat[1];
+        const double tr = eg[0][0] + eg[1][1] + eg[2][2];
+        double sv[6];
+        sv[0] = lam * tr + 2.0 * mu * eg[0][0];
+        sv[1] = lam * tr + 2.0 * mu * eg[1][1];
+        sv[2] = lam * tr + 2.0 * mu * eg[2][2];
+        sv[3] = 2.0 * mu * eg[0][1];
+        sv[4] = 2.0 * mu * eg[1][2];
+        sv[5] = 2.0 * mu * eg[2][0];
+        // pk2_to_cauchy (nlgeom.rs:72-99)
+        const double sm[3][3] = {{sv[0], sv[3], sv[5]}, {sv[3], sv[1], sv[4]}, {sv[5], sv[4], sv[2]}};
+        double fs[3][3], cc[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) fs[i][j] = f[i][0] * sm[0][j] + f[i][1] * sm[1][j] + f[i][2] * sm[2][j];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) cc[i][j] = fs[i][0] * f[j][0] + fs[i][1] * f[j][1] + fs[i][2] * f[j][2];
+        const double invj = 1.0 / jdet;
+        s_c[g][0] = cc[0][0] * invj;
+        s_c[g][1] = cc[1][1] * invj;
+        s_c[g][2] = cc[2][2] * invj;
+        s_c[g][3] = cc[0][1] * invj;
+        s_c[g][4] = cc[1][2] * invj;
+        s_c[g][5] = cc[2][0] * invj;
+        s_g[g][0] = eg[0][0];
+        s_g[g][1] = eg[1][1];
+        s_g[g][2] = eg[2][2];
+        s_g[g][3] = 2.0 * eg[0][1];
+        s_g[g][4] = 2.0 * eg[1][2];
+        s_g[g][5] = 2.0 * eg[2][0];
+        for (int q = 0; q < 9; q++) s_f[g][q] = f[q / 3][q % 3];
+        for (int q = 0; q < 6; q++) s_s[g][q] = sv[q];
+    }
+    __syncthreads();
+    if (s_bad) return;  // uniform across the workgroup
+    double kacc[KPT];
+#pragma unroll
+    for (int k = 0; k < KPT; k++) kacc[k] = 0.0;
+    double facc = 0.0;
+    const double lam = s_mat[0], mu = s_mat[1], dg = s_mat[2];
+    for (int g = 0; g < NG; g++) {
+        // fill_b_nl (nlgeom.rs:101-115)
+        for (int t = tid; t < M; t += NT) {
+            const int a = t / 3, i = t % 3;
+            const double *gg = &s_dndx[g][3 * a];
+            const double *f = s_f[g];
+            s_b[0][t] = f[3 * i + 0] * gg[0];
+            s_b[1][t] = f[3 * i + 1] * gg[1];
+            s_b[2][t] = f[3 * i + 2] * gg[2];
+            s_b[3][t] = f[3 * i + 0] * gg[1] + f[3 * i + 1] * gg[0];
+            s_b[4][t] = f[3 * i + 1] * gg[2] + f[3 * i + 2] * gg[1];
+            s_b[5][t] = f[3 * i + 2] * gg[0] + f[3 * i + 0] * gg[2];
+        }
+        __syncthreads();
+        // tmp = D B (gemm_bt_d_b's first product; D's exact zeros skipped, which
+        // leaves every sum unchanged)
+        for (int t = tid; t < M; t += NT) {
+            const double b0 = s_b[0][t], b1 = s_b[1][t], b2 = s_b[2][t];
+            s_t[0][t] = dg * b0 + lam * b1 + lam * b2;
+            s_t[1][t] = lam * b0 + dg * b1 + lam * b2;
+            s_t[2][t] = lam * b0 + lam * b1 + dg * b2;
+            s_t[3][t] = mu * s_b[3][t];
+            s_t[4][t] = mu * s_b[4][t];
+            s_t[5][t] = mu * s_b[5][t];
+        }
+        __syncthreads();
+        const double w = s_w[g];
+        const double sxx = s_s[g][0], syy = s_s[g][1], szz = s_s[g][2];
+        const double sxy = s_s[g][3], syz = s_s[g][4], szx = s_s[g][5];
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int t = tid + k * NT;
+            if (t >= M * M) break;
+            const int i = t / M, j = t % M;
+            double sacc = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; r++) sacc += s_b[r][i] * s_t[r][j];
+            kacc[k] += w * sacc;
+            if (i % 3 == j % 3) {
+                const double *ga = &s_dndx[g][3 * (i / 3)];
+                const double *gb = &s_dndx[g][3 * (j / 3)];
+                const double gtg = ga[0] * (sxx * gb[0] + sxy * gb[1] + szx * gb[2]) +
+                                   ga[1] * (sxy * gb[0] + syy * gb[1] + syz * gb[2]) +
+                                   ga[2] * (szx * gb[0] + syz * gb[1] + szz * gb[2]);
+                kacc[k] += gtg * w;
+            }
+        }
+        if (tid < M) {
+            double q = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; r++) q += s_b[r][tid] * s_s[g][r];
+            facc += q * w;
+        }
+        __syncthreads();
+    }
+    double *out = ke + e * (int64_t)(M * M);
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+        const int t = tid + k * NT;
+        if (t < M * M) out[t] = kacc[k];
+    }
+    if (tid < M) fe_out[e * M + tid] = facc;
+    if (tid < 6) {
+        // GpAcc: weighted sums in point order, then / volume
+        double ca = 0.0, ga = 0.0, vol = 0.0;
+        for (int g = 0; g < NG; g++) {
+            ca += s_c[g][tid] * s_w[g];
+            ga += s_g[g][tid] * s_w[g];
+            vol += s_w[g];
+        }
+        const double inv = fabs(vol) > 0.0 ? 1.0 / vol : 0.0;
+        cau_out[e * 6 + tid] = ca * inv;
+        gl_out[e * 6 + tid] = ga * inv;
+    }
+}
+
+// f_int over all DOFs: the element internal forces summed per node over the
+// incident elements in ascending element order (the element loop's
+// f_int[gdofs[i]] += fe[i]).
+__global__ void nl_fint_kernel(const int64_t *__restrict__ inc_ptr,
+                               const int64_t *__restrict__ inc_slot, int64_t nnode,
+                               const double *__restrict__ fe, double *__restrict__ fint) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnode * 3) return;
+    const int64_t i = t / 3;
+    const int d = (int)(t % 3);
+    double acc = 0.0;
+    for (int64_t k = inc_ptr[i]; k < inc_ptr[i + 1]; k++) acc += fe[inc_slot[k] * 3 + d];
+    fint[t] = acc;
+}
+
+// hex20_nodal_nl (nlgeom.rs:274-308): Cauchy and Green-Lagrange at the 27
+// points, extrapolated to the 20 nodes (quadratic.rs:208-230), one thread per
+// element; out_c / out_g: nelem x 20 x 6.
+__global__ void nl_nodal20_kernel(const double *__restrict__ xyz, const int32_t *__restrict__ conn,
+                                  const double *__restrict__ E, const double *__restrict__ NU,
+                                  int64_t nelem, const double *__restrict__ u,
+                                  double *__restrict__ out_c, double *__restrict__ out_g,
+                                  int32_t *__restrict__ err) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    double x[60], ue[60], dndx[60];
+    for (int a = 0; a < 20; a++) {
+        const int64_t node = conn[e * 20 + a];
+        for (int q = 0; q < 3; q++) {
+            x[3 * a + q] = xyz[3 * node + q];
+            ue[3 * a + q] = u[3 * node + q];
+        }
+    }
+    const double ev = E[e], nv = NU[e];
+    if (ev <= 0.0 || nv >= 0.5 || nv <= -1.0) {
+        report(err, AXG_E_MATERIAL, e);
+        return;
+    }
+    const double lam = ev * nv / ((1.0 + nv) * (1.0 - 2.0 * nv));
+    const double mu = ev / (2.0 * (1.0 + nv));
+    double oc[20][6], og[20][6];
+    for (int a = 0; a < 20; a++)
+        for (int c = 0; c < 6; c++) oc[a][c] = og[a][c] = 0.0;
+    for (int g = 0; g < 27; g++) {
+        double det;
+        int st = geometry<20>(c_tab.h20_dn[g], x, dndx, det);
+        if (st != AXG_OK) {
+            report(err, st, e);
+            return;
+        }
+        if (det <= 0.0) {
+            report(err, AXG_E_NEG_JACOBIAN, e);
+            return;
+        }
+        double f[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+        for (int a = 0; a < 20; a++)
+            for (int i = 0; i < 3; i++) {
+                const double uu = ue[3 * a + i];
+                for (int j = 0; j < 3; j++) f[i][j] += uu * dndx[3 * a + j];
+            }
+        const double jdet = f[0][0] * (f[1][1] * f[2][2] - f[1][2] * f[2][1]) -
+                            f[0][1] * (f[1][0] * f[2][2] - f[1][2] * f[2][0]) +
+                            f[0][2] * (f[1][0] * f[2][1] - f[1][1] * f[2][0]);
+        if (jdet <= 1e-18) {
+            report(err, AXG_E_INVERSION, e);
+            return;
+        }
+        double eg[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double c = f[0][i] * f[0][j] + f[1][i] * f[1][j] + f[2][i] * f[2][j];
+                eg[i][j] = 0.5 * (c - (i == j ? 1.0 : 0.0));
+            }
+        const double tr = eg[0][0] + eg[1][1] + eg[2][2];
+        const double sv[6] = {lam * tr + 2.0 * mu * eg[0][0], lam * tr + 2.0 * mu * eg[1][1],
+                              lam * tr + 2.0 * mu * eg[2][2], 2.0 * mu * eg[0][1],
+                              2.0 * mu * eg[1][2],           2.0 * mu * eg[2][0]};
+        const double sm[3][3] = {{sv[0], sv[3], sv[5]}, {sv[3], sv[1], sv[4]}, {sv[5], sv[4], sv[2]}};
+        double fs[3][3], cc[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) fs[i][j] = f[i][0] * sm[0][j] + f[i][1] * sm[1][j] + f[i][2] * sm[2][j];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) cc[i][j] = fs[i][0] * f[j][0] + fs[i][1] * f[j][1] + fs[i][2] * f[j][2];
+        const double invj = 1.0 / jdet;
+        const double gc[6] = {cc[0][0] * invj, cc[1][1] * invj, cc[2][2] * invj,
+                              cc[0][1] * invj, cc[1][2] * invj, cc[2][0] * invj};
+        const double gg[6] = {eg[0][0], eg[1][1], eg[2][2], 2.0 * eg[0][1], 2.0 * eg[1][2],
+                              2.0 * eg[2][0]};
+        for (int a = 0; a < 20; a++) {
+            const double wv = c_tab.h20_ext[a][g];
+            for (int c = 0; c < 6; c++) {
+                oc[a][c] += wv * gc[c];
+                og[a][c] += wv * gg[c];
+            }
+        }
+    }
+    for (int a = 0; a < 20; a++)
+        for (int c = 0; c < 6; c++) {
+            out_c[(e * 20 + a) * 6 + c] = oc[a][c];
+            out_g[(e * 20 + a) * 6 + c] = og[a][c];
+        }
+}
+
+// every node of an element gets the element's value (nelem x 6 -> nelem x nn x 6)
+__global__ void broadcast_nodal_kernel(int64_t nelem, int nn, const double *__restrict__ v,
+                                       double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nelem * nn * 6) return;
+    const int64_t e = t / (nn * 6);
+    out[t] = v[e * 6 + t % 6];
+}
+
 // Lumped mass of solid elements (analysis.rs:669-717): density x |volume| of each
 // element, 1/nn of it on each translational DOF of each of its nodes, summed per
 // node over the incident elements in ascending element order.
@@ -767,6 +1089,54 @@ void launch_gather_kg(axg_mesh *m, hipStream_t s, const double *d_kg, double sig
     int64_t blocks = std::min<int64_t>((nslot + 255) / 256, 1 << 20);
     hipLaunchKernelGGL(gather_kg_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
                        m->slotptr_t.p, m->contrib_t.p, d_kg, m->nn, nslot, sign, d_val);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_nl(axg_mesh *m, hipStream_t s, const double *d_u, double *d_fe, double *d_cau,
+               double *d_gl, int32_t *d_err) {
+    if (m->nelem == 0) return;
+    const dim3 grid((unsigned)m->nelem);
+    if (m->nn == 8)
+        hipLaunchKernelGGL((nl_kernel<8, 8, 64>), grid, dim3(64), 0, s, m->xyz.p, m->conn.p, m->e.p,
+                           m->nu.p, m->nelem, d_u, m->ke.p, d_fe, d_cau, d_gl, d_err);
+    else
+        hipLaunchKernelGGL((nl_kernel<20, 27, 256>), grid, dim3(256), 0, s, m->xyz.p, m->conn.p,
+                           m->e.p, m->nu.p, m->nelem, d_u, m->ke.p, d_fe, d_cau, d_gl, d_err);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_nl_fint(axg_mesh *m, hipStream_t s, const double *d_fe, double *d_fint) {
+    if (m->nnode == 0) return;
+    const int64_t t = m->nnode * 3;
+    hipLaunchKernelGGL(nl_fint_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s,
+                       m->inc_ptr.p, m->inc_slot.p, m->nnode, d_fe, d_fint);
+    AXG_HIP(hipGetLastError());
+}
+
+// Nodal Cauchy / Green-Lagrange of the NLGEOM result (analysis.rs:4745-4799):
+// C3D20 from its 27 points, C3D8 the element means on every node; en_c, en_g
+// receive the per-element nodal values (nelem x nn x 6).
+void launch_nl_nodal(axg_mesh *m, hipStream_t s, const double *d_u, const double *d_cau,
+                     const double *d_gl, double *en_c, double *en_g, int32_t *d_err) {
+    if (m->nelem == 0) return;
+    if (m->nn == 20) {
+        hipLaunchKernelGGL(nl_nodal20_kernel, dim3((unsigned)((m->nelem + 63) / 64)), dim3(64), 0, s,
+                           m->xyz.p, m->conn.p, m->e.p, m->nu.p, m->nelem, d_u, en_c, en_g, d_err);
+    } else {
+        const int64_t t = m->nelem * m->nn * 6;
+        hipLaunchKernelGGL(broadcast_nodal_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s,
+                           m->nelem, m->nn, d_cau, en_c);
+        hipLaunchKernelGGL(broadcast_nodal_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s,
+                           m->nelem, m->nn, d_gl, en_g);
+    }
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_nodal_average(axg_mesh *m, hipStream_t s, const double *en, double *d_stress,
+                          double *d_vm) {
+    if (m->nnode == 0) return;
+    hipLaunchKernelGGL(nodal_average_kernel, dim3((unsigned)((m->nnode + 255) / 256)), dim3(256), 0,
+                       s, m->inc_ptr.p, m->inc_slot.p, m->nnode, m->nn, en, d_stress, d_vm);
     AXG_HIP(hipGetLastError());
 }
 

@@ -247,6 +247,17 @@ void launch_kg(axg_mesh *m, hipStream_t s, const double *d_mean, double *d_kg, i
 void launch_gather_kg(axg_mesh *m, hipStream_t s, const double *d_kg, double sign, double *d_val);
 void geometric_stiffness_device(axg_mesh *mesh, const double *d_u, double sign, double *d_val,
                                 double *d_mean);
+void launch_nl(axg_mesh *m, hipStream_t s, const double *d_u, double *d_fe, double *d_cau,
+               double *d_gl, int32_t *d_err);
+void launch_nl_fint(axg_mesh *m, hipStream_t s, const double *d_fe, double *d_fint);
+void launch_nl_nodal(axg_mesh *m, hipStream_t s, const double *d_u, const double *d_cau,
+                     const double *d_gl, double *en_c, double *en_g, int32_t *d_err);
+void launch_nodal_average(axg_mesh *m, hipStream_t s, const double *en, double *d_stress,
+                          double *d_vm);
+void nl_assemble_device(axg_mesh *mesh, const double *d_u, double *d_fe, double *d_cau,
+                        double *d_gl, double *d_fint);
+void nl_nodal_device(axg_mesh *mesh, const double *d_u, const double *d_cau, const double *d_gl,
+                     double *d_stress, double *d_vm, double *d_strain, double *d_gp);
 void launch_lumped_mass(axg_mesh *m, hipStream_t s, const double *d_rho, double *d_m_full);
 void launch_row_apply(axg_mesh *m, hipStream_t s, int mode, const double *d_w,
                       const double *d_f, double *d_out);

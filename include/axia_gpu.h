@@ -38,7 +38,8 @@ enum axg_status {
     AXG_E_NOT_CONVERGED = 6,  /* "CG konvergierte nicht (...)" linalg.rs:253-257 */
     AXG_E_SINGULAR = 7,       /* "Steifigkeitsmatrix ist singulär ..." linalg.rs:89-92 */
     AXG_E_DEVICE = 8,         /* HIP runtime error                              */
-    AXG_E_OOM = 9             /* device allocation failed                       */
+    AXG_E_OOM = 9,            /* device allocation failed                       */
+    AXG_E_INVERSION = 10      /* "NLGEOM: det(F) ≤ 0 (Element inversion)." nlgeom.rs:93 */
 };
 
 enum axg_precision {
@@ -227,10 +228,22 @@ int axg_geometric_stiffness(axg_mesh *mesh, const double *u_full, double *mean, 
 int axg_buckle(axg_mesh *mesh, const double *u_full, int32_t nmodes, const axg_solve_opts *opts,
                double *values, double *vectors, int32_t *nfound, axg_solve_info *info);
 
+/* ------------------------------------------------------ NLGEOM (Newton) */
+/* The element loop of solve_continuum_newton (analysis.rs:4474-4545) on the
+ * device: per element the total-Lagrange St. Venant-Kirchhoff tangent and
+ * internal force at u_full (hex8_nl / hex20_nl, nlgeom.rs:127-273), the
+ * tangent gathered into the mesh's K_ff like K (exportable with
+ * axg_csr_export / axg_ke_export, solvable with axg_mesh_solve), f_int
+ * (3*nnode) summed in element order, element means of Cauchy stress and
+ * Green-Lagrange strain (nelem x 6). Any output pointer may be NULL. */
+int axg_nl_assemble(axg_mesh *mesh, const double *u_full, double *f_int, double *cauchy,
+                    double *gl);
+
 /* ------------------------------------------------------- deck ingestion */
 /* Parses an INP deck (the keyword subset of inp.rs:383-1061: *NODE, *ELEMENT
  * TYPE=C3D8|C3D20, *NSET and *ELSET (GENERATE, nested names), *MATERIAL, *ELASTIC,
- * *DENSITY, *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn, *FREQUENCY, *BUCKLE) into the compacted model
+ * *DENSITY, *SOLID SECTION, *BOUNDARY, *CLOAD, *DLOAD Pn, *STEP (NLGEOM, INC=),
+ * *STATIC, *FREQUENCY, *BUCKLE) into the compacted model
  * (model.rs:1234-1309) and solve_linear's setup on the host
  * (analysis.rs:548-946): arrays, prescribed values (last BC wins), isolated
  * nodes pinned, load vector, free-DOF numbering (DofMap::build). text need not
@@ -251,10 +264,13 @@ int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double
 int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
                    double *rf, double *stress, double *vm, axg_solve_info *info);
 
-/* Number of modes the deck's *FREQUENCY step asks for (inp.rs:1062-1071)
- * and of buckling factors its *BUCKLE step asks for (inp.rs:1072-1081); 0
- * for a step that is absent. Either pointer may be NULL. */
-int axg_deck_nmodes(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle);
+/* The deck's procedure: modes of its *FREQUENCY step (inp.rs:1062-1071),
+ * buckling factors of its *BUCKLE step (inp.rs:1072-1081), 0 for a step that
+ * is absent; nlgeom = 1 for *STEP, NLGEOM (inp.rs:953-982) and the static
+ * step's increments (*STATIC round(period / initial increment), capped by
+ * INC=; inp.rs:983-1061). Any pointer may be NULL. */
+int axg_deck_procedure(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle, int32_t *nlgeom,
+                       int32_t *ninc);
 /* The *FREQUENCY branch of solve_linear (analysis.rs:955-1005) without a
  * preload: assembly and autospc as in axg_deck_solve, lumped mass from the
  * materials' *DENSITY, axg_eigen with opts, freqs = sqrt(lambda) / (2 pi)
@@ -271,6 +287,19 @@ int axg_deck_frequency(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts,
  * entries; modes_full (may be NULL): nbuckle x 3*nnode. */
 int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *factors,
                     double *modes_full, double *u_static, int32_t *nfound, axg_solve_info *info);
+
+/* solve_continuum_newton (analysis.rs:4313-4860) for an all-hex St. Venant-
+ * Kirchhoff deck under *STEP, NLGEOM (no contact, pretension, rigid bodies,
+ * plasticity or amplitudes): load steps of 1 / ninc with the reference's
+ * growth and cutback, full Newton with the tangent and internal force
+ * assembled on the device (axg_nl_assemble), follower pressure, the GPU PCG
+ * of opts for each correction. u_full (3*nnode); rf = f_int - f_ext, nodal
+ * Cauchy stress (nnode x 6), von Mises (nnode), nodal Green-Lagrange strain
+ * (nnode x 6) may be NULL; ninc_done, newton_iters (the last increment's
+ * iterations) may be NULL. */
+int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
+                    double *rf, double *stress, double *vm, double *strain, int32_t *ninc_done,
+                    int32_t *newton_iters, axg_solve_info *info);
 
 /* Batched solves: nmesh meshes (assembled), each with nrhs[m] right-hand
  * sides rhs[m] (n_free[m] * nrhs[m], column-major by right-hand side),

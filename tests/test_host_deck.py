@@ -131,3 +131,30 @@ def test_unsupported_and_malformed_decks():
     with pytest.raises(AxgError) as e:
         D.Deck(BRICK.replace("BLK, P2, 3.0", "BLK, BX, 3.0"))
     assert "DLOAD" in str(e.value)
+
+
+def test_procedure_keywords():
+    """*STEP, NLGEOM / INC= and *STATIC increments, *FREQUENCY and *BUCKLE counts
+    (inp.rs:953-1081) as the reader records them, against the oracle's reader."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from eigen_decks import bar_deck, nl_cantilever_deck
+    cases = [
+        nl_cantilever_deck(ninc=4),
+        nl_cantilever_deck(ninc=3).replace("*STEP, NLGEOM", "*STEP, NLGEOM, INC=2"),
+        nl_cantilever_deck(ninc=4).replace("*STATIC\n0.25, 1.0\n", "*STATIC\n0.3, 1.0\n"),
+        nl_cantilever_deck(ninc=4).replace("*STEP, NLGEOM", "*STEP"),
+        bar_deck(3, nmodes=5),
+        bar_deck(3).replace("*FREQUENCY\n3\n", "*FREQUENCY\n"),
+    ]
+    for text in cases:
+        m = O.parse_inp(text)
+        p = D.Deck(text).procedure()
+        assert p["nlgeom"] == m.nlgeom
+        assert p["nmodes"] == m.nmodes and p["nbuckle"] == m.nbuckle
+        if m.nlgeom:
+            assert p["ninc"] == m.ninc
+    assert D.Deck(cases[0]).procedure()["ninc"] == 4
+    assert D.Deck(cases[1]).procedure()["ninc"] == 2   # INC= caps the increments
+    assert D.Deck(cases[2]).procedure()["ninc"] == 3   # round(1 / 0.3)
+    assert D.Deck(cases[5]).procedure()["nmodes"] == 10
