@@ -79,6 +79,7 @@ class NlResult:
     residual: float          # last ||r_free||
     solver: str
     pcg_iters: int           # summed PCG iterations of all corrections
+    solve_ms: float          # their summed device time
 
 
 class Deck:
@@ -177,7 +178,7 @@ class Deck:
                                          C.byref(its), C.byref(info)))
         inf = _info_dict(info)
         return NlResult(self.arrays()["node_ids"], u, rf, stress, vm, strain, ninc.value,
-                        its.value, inf["residual"], inf["name"], inf["iters"])
+                        its.value, inf["residual"], inf["name"], inf["iters"], inf["solve_ms"])
 
     def buckle(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
                inner_tol=1e-3, modes=True):

@@ -223,3 +223,60 @@ def simp_brick(nx=120, ny=60, nz=60, seed=0, nu=0.3, rho_lo=0.3, rho_hi=1.0, emi
     f[3 * node + 2] = -1.0
     return Problem(f"simp_c3d8_{nx}x{ny}x{nz}", 8, xyz, conn, e, np.full(ne, nu), ind, [f],
                    {"rho": rho, "emin": emin, "e0": e0, "load_node": node})
+
+
+# ------------------------------------------------------------- INP writing
+def inp_solid(nn, xyz, conn, e, nu, rho=None):
+    """INP lines of a one-material solid: *NODE, *ELEMENT (ELSET=SOLID), *MATERIAL with
+    *ELASTIC (and *DENSITY when rho is given), *SOLID SECTION. Ids are index + 1."""
+    typ = "C3D8" if nn == 8 else "C3D20"
+    out = ["*NODE\n"]
+    out += [f"{i + 1}, {float(x)!r}, {float(y)!r}, {float(z)!r}\n" for i, (x, y, z) in enumerate(xyz)]
+    out.append(f"*ELEMENT, TYPE={typ}, ELSET=SOLID\n")
+    for k, row in enumerate(conn):
+        ids = [str(int(a) + 1) for a in row]
+        # 16 values per data line at most, continuation lines after a trailing comma
+        cur = [str(k + 1)] + ids
+        while len(cur) > 16:
+            out.append(", ".join(cur[:16]) + ",\n")
+            cur = cur[16:]
+        out.append(", ".join(cur) + "\n")
+    out.append(f"*MATERIAL, NAME=MAT\n*ELASTIC\n{float(e)!r}, {float(nu)!r}\n")
+    if rho is not None:
+        out.append(f"*DENSITY\n{float(rho)!r}\n")
+    out.append("*SOLID SECTION, ELSET=SOLID, MATERIAL=MAT\n")
+    return out
+
+
+def inp_nset(name, nodes):
+    """*NSET lines (ids = index + 1, 16 per line)."""
+    out = [f"*NSET, NSET={name}\n"]
+    ids = [str(int(a) + 1) for a in nodes]
+    for i in range(0, len(ids), 16):
+        out.append(", ".join(ids[i:i + 16]) + "\n")
+    return out
+
+
+def nlgeom_cantilever_deck(nx=6, ny=2, nz=2, dims=(6.0, 1.0, 1.0), p_total=-1.0, e=1000.0,
+                           nu=0.3, ninc=2, nn=8, pressure=None):
+    """A clamped cantilever under *STEP, NLGEOM: p_total spread over the tip nodes in z
+    (dead load), or a pressure on the top face (P2 of the top layer, a follower load)."""
+    xyz, conn = (brick_hex20 if nn == 20 else brick_hex8)(nx, ny, nz)
+    xyz = xyz * np.array([dims[0] / nx, dims[1] / ny, dims[2] / nz])
+    out = inp_solid(nn, xyz, conn, e, nu, None)
+    fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
+    tip = np.nonzero(xyz[:, 0] == dims[0])[0]
+    out += inp_nset("FIX", fixed) + inp_nset("TIP", tip)
+    top = [k for k in range(len(conn)) if k // (nx * ny) == nz - 1]
+    if pressure is not None:
+        out.append("*ELSET, ELSET=TOP\n")
+        ids = [str(k + 1) for k in top]
+        for i in range(0, len(ids), 16):
+            out.append(", ".join(ids[i:i + 16]) + "\n")
+    out.append(f"*STEP, NLGEOM\n*STATIC\n{1.0 / ninc!r}, 1.0\n*BOUNDARY\nFIX, 1, 3\n")
+    if pressure is not None:
+        out.append(f"*DLOAD\nTOP, P2, {float(pressure)!r}\n")
+    else:
+        out.append(f"*CLOAD\nTIP, 3, {p_total / len(tip)!r}\n")
+    out.append("*END STEP\n")
+    return "".join(out)

@@ -31,6 +31,12 @@ freq  *FREQUENCY: the 10 lowest modes of a steel C3D8 brick 48x48x48 (110,592
     clamped) by the reference's subspace iteration (14 vectors, at most 40
     sweeps) over the multi-right-hand-side FP64 block PCG (warm-started from
     the Ritz vectors). One step = assembly, lumped mass and the eigen solve.
+nlgeom  *STEP, NLGEOM: a C3D8 cantilever 200x20x20 (80,000 elements, 264,600
+    free DOFs; 10 x 1 x 1, E 1000, nu 0.3, x = 0 clamped) under a tip load of
+    0.5 in -z (about 20 % of the length in linear theory) in 4 increments: the
+    reference's continuum Newton driver with the St. Venant-Kirchhoff tangent
+    re-assembled on the device at every iteration and block-Jacobi PCG solves.
+    One step = one whole analysis.
 
 Inputs are resident in HBM when a timed region starts.
 """
@@ -50,7 +56,7 @@ for _p in (ROOT, os.path.join(ROOT, "axia-fem_amd")):
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 METRIC = "DOF·iter/s PCG + elements/s assembly, mixed vs FP64 GPU at 1e-8 rel. error"
-WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq")
+WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq", "nlgeom")
 
 
 def log(*a):
@@ -615,6 +621,88 @@ def run_freq(a, G, MG, torch, np):
     return out
 
 
+# ------------------------------------------------------------------- nlgeom
+def nl_flops(nn, ngp):
+    """Algorithmic FP64 flops of one NLGEOM element tangent + internal force: the dense
+    B^T D B of gemm_bt_d_b (72 nd + 14 nd^2 per point, as for Ke), the geometric term on the
+    nd^2 / 3 equal-direction entries (20 flops each) and B^T S (12 nd) per point."""
+    nd = 3 * nn
+    return ngp * (72 * nd + 14 * nd * nd + 20 * nd * nd // 3 + 12 * nd)
+
+
+def run_nlgeom(a, G, MG, torch, np):
+    from axia_gpu import deck as D
+    nx, ny, nz = a.nl_shape
+    text = MG.nlgeom_cantilever_deck(nx, ny, nz, dims=(10.0, 1.0, 1.0), p_total=-0.5, e=1000.0,
+                                     nu=0.3, ninc=4)
+    dk = D.Deck(text)
+    arr = dk.arrays()
+    n = dk.n_free
+    log(f"nlgeom: {dk.nelem} elements, n={n}")
+    for _ in range(a.warmup):
+        dk.nlgeom(tol=a.nl_tol)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = dk.nlgeom(tol=a.nl_tol)
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / a.steps
+    log(f"nlgeom: {ms_per_step:.0f} ms per analysis, {res.ninc} increments, {res.pcg_iters} PCG "
+        f"iterations in {res.solve_ms:.0f} ms, {res.solver}")
+    # the tangent re-assembly alone (tangent + gather + f_int), at the converged state
+    m = G.Mesh(8, arr["xyz"], arr["conn"], arr["e"], arr["nu"], arr["ind_of"])
+    m.nl_assemble(res.u_full)
+    reps = 10
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        m.nl_assemble(res.u_full)
+    asm_ms = 1e3 * (time.perf_counter() - t1) / reps
+    nnz = m.info()["nnz"]
+    spmv_ms = m.bench_operator(0, 1, a.reps)
+    xyz = arr["xyz"]
+    tip = np.nonzero(xyz[:, 0] == 10.0)[0]
+    fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
+    value = n * res.pcg_iters / (res.solve_ms / 1e3)
+    out = base_line(a, value, ms_per_step, "f64", {
+        "workload": f"*STEP, NLGEOM C3D8 cantilever {nx}x{ny}x{nz} (10 x 1 x 1, n={n}), tip "
+                    f"load 0.5, 4 increments, Newton with the device tangent and block-Jacobi "
+                    f"PCG to {a.nl_tol:g}",
+        "n_free": n, "nnz": nnz, "elements": dk.nelem, "precond": "block3"})
+    asm_eps = dk.nelem / (asm_ms / 1e3)
+    out.update({
+        "nlgeom": {"time_per_analysis_ms": ms_per_step, "increments": res.ninc,
+                   "newton_iters_last_increment": res.iters, "pcg_iterations": res.pcg_iters,
+                   "pcg_solve_ms": res.solve_ms, "tip_deflection": float(res.u_full[3 * tip + 2].mean()),
+                   "residual": res.residual,
+                   "reaction_z_sum": float(res.rf[3 * fixed + 2].sum()),
+                   "tangent_reassembly_ms": asm_ms, "tangent_reassembly_elements_per_s": asm_eps,
+                   "tangent_fp64_tflops": nl_flops(8, 8) * asm_eps / 1e12,
+                   "solver": res.solver},
+        "roofline": roofline("sell_spmv_kernel (FP64 SpMV of the tangent in the PCG)",
+                             sell64_bytes(n, nnz), spmv_ms),
+    })
+    if not a.no_cpu:
+        from oracle import oracle as O
+        cores = host_threads()
+        # the oracle's element tangent (single thread) on a sample of elements
+        u3 = res.u_full.reshape(-1, 3)
+        conn, e, nu = arr["conn"], arr["e"], arr["nu"]
+        sample = min(2000, dk.nelem)
+        t2 = time.perf_counter()
+        for el in range(sample):
+            O.hex_nl(xyz[conn[el]], u3[conn[el]].ravel(), e[el], nu[el])
+        cpu_eps = sample / (time.perf_counter() - t2)
+        p = MG.Problem("nlgeom_tangent_sample", 8, xyz, conn, e, nu, arr["ind_of"])
+        csr = _oracle_csr(O, p)
+        rate = _oracle_pcg_rate(O, csr, np.ones(csr.n), 60, cores)
+        out["cpu_baseline"] = {
+            "value": rate, "unit": "DOF·iter/s", "cores": cores, "kind": "port",
+            "sample": f"oracle linalg::pcg restatement (Jacobi, FP64 CSR, OpenMP), 60 iterations "
+                      f"on this cantilever's K_ff (n={csr.n}, nnz={csr.nnz}); the oracle's "
+                      f"NLGEOM element tangent single-thread on {sample} elements",
+            "tangent_elements_per_s_1thread": cpu_eps,
+        }
+    return out
+
+
 # --------------------------------------------------------------------- SIMP
 def cpu_adjoint(O, p, u, threads, chunk=20000):
     """FP64 CPU adjoint: dc/dE_e = -u_e^T K1_e u_e with the oracle's element matrices."""
@@ -753,6 +841,8 @@ def main():
     ap.add_argument("--freq-n", type=int, default=48, help="freq brick edge (elements)")
     ap.add_argument("--nmodes", type=int, default=10)
     ap.add_argument("--eig-tol", type=float, default=1e-10)
+    ap.add_argument("--nl-shape", type=int, nargs=3, default=[200, 20, 20])
+    ap.add_argument("--nl-tol", type=float, default=1e-10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -766,7 +856,8 @@ def main():
     from axia_gpu import meshgen as MG
 
     run = {"brick": run_brick, "cantilever": run_cantilever, "c3d20": run_c3d20,
-           "batch": run_batch, "simp": run_simp, "freq": run_freq}[a.workload]
+           "batch": run_batch, "simp": run_simp, "freq": run_freq,
+           "nlgeom": run_nlgeom}[a.workload]
     out = run(a, G, MG, torch, np)
     if out is not None:
         print(json.dumps(out))

@@ -1,4 +1,8 @@
-"""INP decks with a *FREQUENCY step for the eigen tests (generated, not taken from files)."""
+"""INP decks for the eigen, buckling and NLGEOM tests (generated, not taken from files)."""
+
+from axia_gpu.meshgen import inp_nset as _nset
+from axia_gpu.meshgen import inp_solid as _solid_lines
+from axia_gpu.meshgen import nlgeom_cantilever_deck as nl_cantilever_deck
 
 BAR = """*NODE
 {nodes}*ELEMENT, TYPE=C3D8, ELSET=BAR
@@ -37,34 +41,6 @@ def bar_deck(nel, length=100.0, nmodes=3):
     return BAR.format(nodes="".join(nodes), elems="".join(elems), alln=ids + "\n", nmodes=nmodes)
 
 
-def _solid_lines(nn, xyz, conn, e, nu, rho):
-    typ = "C3D8" if nn == 8 else "C3D20"
-    out = ["*NODE\n"]
-    out += [f"{i + 1}, {float(x)!r}, {float(y)!r}, {float(z)!r}\n" for i, (x, y, z) in enumerate(xyz)]
-    out.append(f"*ELEMENT, TYPE={typ}, ELSET=SOLID\n")
-    for k, row in enumerate(conn):
-        ids = [str(int(a) + 1) for a in row]
-        # 16 values per data line at most, continuation lines after a trailing comma
-        cur = [str(k + 1)] + ids
-        while len(cur) > 16:
-            out.append(", ".join(cur[:16]) + ",\n")
-            cur = cur[16:]
-        out.append(", ".join(cur) + "\n")
-    out.append(f"*MATERIAL, NAME=MAT\n*ELASTIC\n{float(e)!r}, {float(nu)!r}\n")
-    if rho is not None:
-        out.append(f"*DENSITY\n{float(rho)!r}\n")
-    out.append("*SOLID SECTION, ELSET=SOLID, MATERIAL=MAT\n")
-    return out
-
-
-def _nset(name, nodes):
-    out = [f"*NSET, NSET={name}\n"]
-    ids = [str(int(a) + 1) for a in nodes]
-    for i in range(0, len(ids), 16):
-        out.append(", ".join(ids[i:i + 16]) + "\n")
-    return out
-
-
 def mesh_deck(nn, xyz, conn, fixed_nodes, e=210000.0, nu=0.3, rho=7.85e-9, nmodes=4):
     """A solid deck from arrays: one material with density, fixed_nodes clamped, *FREQUENCY."""
     out = _solid_lines(nn, xyz, conn, e, nu, rho) + _nset("FIX", fixed_nodes)
@@ -99,31 +75,4 @@ def nl_bar_deck(nel=4, length=4.0, delta=0.5, e=1000.0, nu=0.0, ninc=4, nn=8):
     out += _nset("X0", x0) + _nset("XL", xl)
     out.append(f"*STEP, NLGEOM\n*STATIC\n{1.0 / ninc!r}, 1.0\n*BOUNDARY\nX0, 1, 1\n"
                f"{org + 1}, 2, 3\n{y1 + 1}, 3, 3\nXL, 1, 1, {float(delta)!r}\n*END STEP\n")
-    return "".join(out)
-
-
-def nl_cantilever_deck(nx=6, ny=2, nz=2, dims=(6.0, 1.0, 1.0), p_total=-1.0, e=1000.0,
-                       nu=0.3, ninc=2, nn=8, pressure=None):
-    """A clamped cantilever under *STEP, NLGEOM: p_total spread over the tip nodes in z
-    (dead load), or a pressure on the top face (P2 of the top layer, a follower load)."""
-    from axia_gpu import meshgen as MG
-    import numpy as np
-    xyz, conn = (MG.brick_hex20 if nn == 20 else MG.brick_hex8)(nx, ny, nz)
-    xyz = xyz * np.array([dims[0] / nx, dims[1] / ny, dims[2] / nz])
-    out = _solid_lines(nn, xyz, conn, e, nu, None)
-    fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
-    tip = np.nonzero(xyz[:, 0] == dims[0])[0]
-    out += _nset("FIX", fixed) + _nset("TIP", tip)
-    top = [k for k in range(len(conn)) if k // (nx * ny) == nz - 1]
-    if pressure is not None:
-        out.append("*ELSET, ELSET=TOP\n")
-        ids = [str(k + 1) for k in top]
-        for i in range(0, len(ids), 16):
-            out.append(", ".join(ids[i:i + 16]) + "\n")
-    out.append(f"*STEP, NLGEOM\n*STATIC\n{1.0 / ninc!r}, 1.0\n*BOUNDARY\nFIX, 1, 3\n")
-    if pressure is not None:
-        out.append(f"*DLOAD\nTOP, P2, {float(pressure)!r}\n")
-    else:
-        out.append(f"*CLOAD\nTIP, 3, {p_total / len(tip)!r}\n")
-    out.append("*END STEP\n")
     return "".join(out)
