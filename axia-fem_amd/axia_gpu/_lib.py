@@ -125,8 +125,10 @@ SIGNATURES = {
     "axg_deck_arrays": (C.c_int, [_vp, _ip, _ip, _dp, _ip, _dp, _dp, _lp, _dp, _dp]),
     "axg_deck_procedure": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
-    "axg_deck_nlgeom": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _dp, _dp, _ip,
-                                  _ip, C.POINTER(SolveInfo)]),
+    "axg_deck_nlgeom": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _dp, _dp, _dp,
+                                  _ip, _ip, C.POINTER(SolveInfo)]),
+    "axg_deck_write_dat": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _i32, C.c_char_p, _i64, _lp]),
+    "axg_deck_write_frd": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_char_p, _i64, _lp]),
     "axg_nl_assemble": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "axg_deck_buckle": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _ip,
                                   C.POINTER(SolveInfo)]),
@@ -135,7 +137,7 @@ SIGNATURES = {
                              C.POINTER(SolveInfo)]),
     "axg_deck_frequency": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _ip, _ip,
                                      C.POINTER(SolveInfo)]),
-    "axg_deck_solve": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _dp,
+    "axg_deck_solve": (C.c_int, [_vp, _vp, C.POINTER(SolveOpts), _dp, _dp, _dp, _dp, _dp,
                                  C.POINTER(SolveInfo)]),
 }
 

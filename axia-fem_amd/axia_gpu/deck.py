@@ -33,6 +33,7 @@ class LinearResult:
     residual: float
     nfree: int
     ndof: int
+    stress_gp: np.ndarray = None  # (nelem, 6) element mean stress
 
     def node_index(self, nid):
         i = int(np.searchsorted(self.node_ids, nid))
@@ -80,6 +81,7 @@ class NlResult:
     solver: str
     pcg_iters: int           # summed PCG iterations of all corrections
     solve_ms: float          # their summed device time
+    stress_gp: np.ndarray = None  # (nelem, 6) element mean Cauchy stress
 
 
 class Deck:
@@ -131,14 +133,16 @@ class Deck:
         rf = np.zeros(ndof)
         stress = np.zeros((self.nnode, 6))
         vm = np.zeros(self.nnode)
+        gp = np.zeros((self.nelem, 6))
         info = _lib.SolveInfo()
         o = _opts(precision, precond, tol, max_iter, inner_tol)
         check(_lib.lib().axg_deck_solve(ctx.handle, self._h, C.byref(o), dptr(u), dptr(rf),
-                                        dptr(stress), dptr(vm), C.byref(info)))
+                                        dptr(stress), dptr(vm), dptr(gp), C.byref(info)))
         inf = _info_dict(info)
         ids = self.arrays()["node_ids"]
+        self._refresh()
         return LinearResult(ids, u, rf, stress, vm, inf["name"], inf["iters"], inf["residual"],
-                            self.n_free, ndof)
+                            self.n_free, ndof, gp)
 
 
     def procedure(self):
@@ -170,15 +174,16 @@ class Deck:
         u, rf = np.zeros(ndof), np.zeros(ndof)
         stress, strain = np.zeros((self.nnode, 6)), np.zeros((self.nnode, 6))
         vm = np.zeros(self.nnode)
+        gp = np.zeros((self.nelem, 6))
         ninc, its = C.c_int32(), C.c_int32()
         info = _lib.SolveInfo()
         o = _opts(precision, precond, tol, max_iter, inner_tol)
         check(_lib.lib().axg_deck_nlgeom(ctx.handle, self._h, C.byref(o), dptr(u), dptr(rf),
-                                         dptr(stress), dptr(vm), dptr(strain), C.byref(ninc),
-                                         C.byref(its), C.byref(info)))
+                                         dptr(stress), dptr(vm), dptr(strain), dptr(gp),
+                                         C.byref(ninc), C.byref(its), C.byref(info)))
         inf = _info_dict(info)
         return NlResult(self.arrays()["node_ids"], u, rf, stress, vm, strain, ninc.value,
-                        its.value, inf["residual"], inf["name"], inf["iters"], inf["solve_ms"])
+                        its.value, inf["residual"], inf["name"], inf["iters"], inf["solve_ms"], gp)
 
     def buckle(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
                inner_tol=1e-3, modes=True):
@@ -225,6 +230,31 @@ class Deck:
         return FrequencyResult(self.arrays()["node_ids"], freqs[:n], vals[:n],
                                shapes[:n] if modes else None, sw.value, inf["name"], inf["iters"],
                                self.n_free, ndof)
+
+    def write_dat(self, u_full, rf, stress_gp=None, nodal_stress=None, freqs=None):
+        """The reference's DAT text for results on this deck (dat.rs), as a str."""
+        keep = [np.ascontiguousarray(a, np.float64) if a is not None else None
+                for a in (u_full, rf, stress_gp, nodal_stress, freqs)]
+        args = [dptr(a) if a is not None else None for a in keep]
+        nf = 0 if freqs is None else len(keep[4])
+        n = C.c_int64()
+        L = _lib.lib()
+        check(L.axg_deck_write_dat(self._h, *args, nf, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(L.axg_deck_write_dat(self._h, *args, nf, buf, n.value + 1, C.byref(n)))
+        return buf.raw[:n.value].decode("utf-8")
+
+    def write_frd(self, u_full, stress=None, rf=None, strain=None):
+        """The reference's FRD text for results on this deck (frd.rs, one frame), as a str."""
+        keep = [np.ascontiguousarray(a, np.float64) if a is not None else None
+                for a in (u_full, stress, rf, strain)]
+        args = [dptr(a) if a is not None else None for a in keep]
+        n = C.c_int64()
+        L = _lib.lib()
+        check(L.axg_deck_write_frd(self._h, *args, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(L.axg_deck_write_frd(self._h, *args, buf, n.value + 1, C.byref(n)))
+        return buf.raw[:n.value].decode("utf-8")
 
     def _refresh(self):
         nn, nnode, nelem, nfree = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()

@@ -25,11 +25,11 @@ COMMON = [
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
 ]
 # Files whose FP64 results must follow the reference's operation order.
-EXACT = {"axg_element.hip", "axg_triplets.hip", "axg_loads.cpp", "axg_deck.cpp"}
+EXACT = {"axg_element.hip", "axg_triplets.hip", "axg_loads.cpp", "axg_deck.cpp", "axg_writers.cpp"}
 SOURCES = [
     "axg_api.cpp", "axg_symbolic.cpp", "axg_loads.cpp", "axg_element.hip", "axg_triplets.hip",
     "axg_pcg.hip", "axg_mixed.hip", "axg_element32.hip", "axg_bench.hip", "axg_batch.hip",
-    "axg_deck.cpp", "axg_eigen.hip",
+    "axg_deck.cpp", "axg_eigen.hip", "axg_writers.cpp",
 ]
 
 

@@ -268,7 +268,14 @@ void nl_nodal_device(axg_mesh *mesh, const double *d_u, const double *d_cau, con
     launch_nl_nodal(mesh, s, d_u, d_cau, d_gl, enc.p, eng.p, w.err.p);
     launch_nodal_average(mesh, s, enc.p, d_stress, d_vm);
     launch_nodal_average(mesh, s, eng.p, d_strain ? d_strain : dummy.p, nullptr);
-    if (d_gp) launch_element_mean(mesh, s, enc.p, d_gp);
+    // stress_gp: C3D20 the mean of its nodal Cauchy values / nn, C3D8 the
+    // element's volume-averaged Cauchy itself (analysis.rs:4766-4783)
+    if (d_gp) {
+        if (mesh->nn == 20)
+            launch_element_mean(mesh, s, enc.p, d_gp);
+        else if (mesh->nelem > 0)
+            AXG_HIP(hipMemcpyAsync(d_gp, d_cau, 8 * 6 * mesh->nelem, hipMemcpyDeviceToDevice, s));
+    }
     int32_t err[2];
     AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));

@@ -8,6 +8,8 @@
 // summed exactly as the reference sums it.
 
 #include <algorithm>
+#include <array>
+#include <functional>
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
@@ -20,6 +22,7 @@
 #include <vector>
 
 #include "axg_internal.h"
+#include "axg_writers.h"
 
 using namespace axg;
 
@@ -173,8 +176,13 @@ void parse_keyword(const std::string &raw, std::string &kw, std::map<std::string
         if (comma == std::string::npos) break;
         start = comma + 1;
     }
-    if (kw == "*SOLIDSECTION") kw = "*SOLID SECTION";
-    if (kw == "*ENDSTEP") kw = "*END STEP";
+    // unglue_keyword (inp.rs:41-69), the aliases on this path
+    static const char *const kAlias[][2] = {
+        {"*SOLIDSECTION", "*SOLID SECTION"}, {"*ENDSTEP", "*END STEP"},
+        {"*ELFILE", "*EL FILE"},             {"*NODEFILE", "*NODE FILE"},
+        {"*NODEOUTPUT", "*NODE OUTPUT"},     {"*ELOUTPUT", "*EL OUTPUT"}};
+    for (auto &a : kAlias)
+        if (kw == a[0]) kw = a[1];
 }
 
 }  // namespace
@@ -189,6 +197,9 @@ struct axg_deck {
     int32_t nmodes = 0;                       // *FREQUENCY n (0: static step)
     int32_t nbuckle = 0;                      // *BUCKLE n (0: no buckling step)
     bool nlgeom = false;                      // *STEP, NLGEOM
+    std::string heading;                      // *HEADING's first line
+    // *NODE FILE / *EL FILE selections (model.rs:1176-1180 defaults)
+    bool output_u = true, output_s = true, output_rf = true, output_e = false, output_nt = false;
     int32_t max_inc = 100;                    // *STEP, INC=
     int32_t ninc = 1;                         // increments of the static step
     std::map<std::string, std::string> elset_material;
@@ -535,6 +546,35 @@ void parse_deck(axg_deck &m, const std::string &text) {
             int32_t nm = 10;
             if (!toks.empty() && !parse_i32(toks[0], nm)) nm = 10;
             m.nmodes = std::max(nm, 1);
+        } else if (kw == "*HEADING") {
+            // inp.rs:410-417: the next non-blank line, trimmed, unless a keyword
+            i++;
+            while (i < n && is_comment_or_blank(lines[i])) i++;
+            if (i < n && !is_keyword(lines[i])) {
+                m.heading = trim(lines[i]);
+                i++;
+            }
+        } else if (kw == "*NODE FILE" || kw == "*NODE OUTPUT") {
+            // inp.rs:1989-2004
+            i = collect(i + 1, toks);
+            m.output_u = m.output_rf = false;
+            for (auto &t : toks) {
+                const std::string u = upper(trim(t));
+                if (u == "U") m.output_u = true;
+                else if (u == "RF") m.output_rf = true;
+                else if (u == "NT") m.output_nt = true;
+            }
+            if (!m.output_u && !m.output_rf) m.output_u = true;
+        } else if (kw == "*EL FILE" || kw == "*ELEMENT OUTPUT") {
+            // inp.rs:2006-2021
+            i = collect(i + 1, toks);
+            m.output_s = m.output_e = false;
+            for (auto &t : toks) {
+                const std::string u = upper(trim(t));
+                if (u == "S") m.output_s = true;
+                else if (u == "E") m.output_e = true;
+            }
+            if (!m.output_s && !m.output_e) m.output_s = true;
         } else if (kw == "*STEP") {
             // inp.rs:953-982: NLGEOM (with any value) makes the step's static
             // procedure geometrically nonlinear; INC= caps the increments
@@ -874,7 +914,8 @@ int axg_deck_procedure(axg_deck *deck, int32_t *nmodes, int32_t *nbuckle, int32_
 // autospc, reduced right-hand side, GPU PCG, reconstruction, reactions and
 // nodal stresses.
 int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
-                   double *rf, double *stress, double *vm, axg_solve_info *info) {
+                   double *rf, double *stress, double *vm, double *stress_gp,
+                   axg_solve_info *info) {
     API_BEGIN
     AXG_REQUIRE(ctx && deck && u_full, AXG_E_INVALID, "axg_deck_solve: invalid argument");
     axg_deck &d = *deck;
@@ -885,7 +926,21 @@ int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, dou
     axg_solve_info si;
     static_solve(d, g, opts, u_full, &si);
     if (rf) check_status(axg_reactions(g.m, u_full, d.f_full.data(), rf));
-    if (stress) check_status(axg_nodal_stress(g.m, u_full, stress, vm, nullptr));
+    if (stress || stress_gp) {
+        std::vector<double> st(stress ? 0 : (size_t)d.nnode * 6), en;
+        if (stress_gp) en.resize((size_t)d.nelem * d.nn * 6);
+        check_status(axg_nodal_stress(g.m, u_full, stress ? stress : st.data(), vm,
+                                      stress_gp ? en.data() : nullptr));
+        // stress_gp (analysis.rs:1145-1151): nodal values / nn summed in node order
+        if (stress_gp)
+            for (int64_t e = 0; e < d.nelem; e++)
+                for (int c = 0; c < 6; c++) {
+                    double acc = 0.0;
+                    for (int a = 0; a < d.nn; a++)
+                        acc += en[((size_t)e * d.nn + a) * 6 + c] / (double)d.nn;
+                    stress_gp[e * 6 + c] = acc;
+                }
+    }
     if (info) *info = si;
     API_END
 }
@@ -993,8 +1048,8 @@ static double norm_seq(const std::vector<double> &v) {
 // iterations. Results: u_full, rf = f_int - f_ext, nodal Cauchy stress, von
 // Mises and Green-Lagrange strain (each may be NULL).
 int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
-                    double *rf, double *stress, double *vm, double *strain, int32_t *ninc_done,
-                    int32_t *newton_iters, axg_solve_info *info) {
+                    double *rf, double *stress, double *vm, double *strain, double *stress_gp,
+                    int32_t *ninc_done, int32_t *newton_iters, axg_solve_info *info) {
     API_BEGIN
     AXG_REQUIRE(ctx && deck && u_full, AXG_E_INVALID, "axg_deck_nlgeom: invalid argument");
     const axg_deck &d = *deck;
@@ -1150,14 +1205,16 @@ int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
     std::copy(u.begin(), u.end(), u_full);
     if (rf)
         for (int64_t i = 0; i < ndof; i++) rf[i] = last_fint[i] - f_now[i];
-    if (stress || vm || strain) {
+    if (stress || vm || strain || stress_gp) {
         const int64_t nn6 = std::max<int64_t>(d.nnode, 1) * 6;
-        DevBuf<double> dst(nn6), dvm(std::max<int64_t>(d.nnode, 1)), dsn(nn6);
+        DevBuf<double> dst(nn6), dvm(std::max<int64_t>(d.nnode, 1)), dsn(nn6), dgp(6 * ne);
         upload(u);
-        nl_nodal_device(mesh, du.p, cau.p, glb.p, dst.p, dvm.p, dsn.p, nullptr);
+        nl_nodal_device(mesh, du.p, cau.p, glb.p, dst.p, dvm.p, dsn.p, dgp.p);
         if (stress) AXG_HIP(hipMemcpyAsync(stress, dst.p, 8 * d.nnode * 6, hipMemcpyDeviceToHost, s));
         if (vm) AXG_HIP(hipMemcpyAsync(vm, dvm.p, 8 * d.nnode, hipMemcpyDeviceToHost, s));
         if (strain) AXG_HIP(hipMemcpyAsync(strain, dsn.p, 8 * d.nnode * 6, hipMemcpyDeviceToHost, s));
+        if (stress_gp)
+            AXG_HIP(hipMemcpyAsync(stress_gp, dgp.p, 8 * d.nelem * 6, hipMemcpyDeviceToHost, s));
         AXG_HIP(hipStreamSynchronize(s));
     }
     if (ninc_done) *ninc_done = done_inc;
@@ -1205,6 +1262,261 @@ int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
         if (modes_full) reconstruct(d, vecs.data() + (size_t)j * d.n_free, modes_full + (size_t)j * d.ndof);
     }
     *nfound = k;
+    API_END
+}
+
+// ------------------------------------------------------------- writers
+static void copy_out(const std::string &o, char *buf, int64_t cap, int64_t *len) {
+    *len = (int64_t)o.size();
+    if (!buf) return;
+    AXG_REQUIRE(cap >= (int64_t)o.size(), AXG_E_INVALID, "output buffer too small");
+    std::memcpy(buf, o.data(), o.size());
+}
+
+// dat.rs:3-94 for a static or frequency result: displacements of every node,
+// forces where |fx| + |fy| + |fz| > 1e-14, element stresses (stress_gp, one
+// point per element), then the nodal stresses and the frequencies when given.
+int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
+                       const double *stress_gp, const double *nodal_stress, const double *freqs,
+                       int32_t nfreq, char *buf, int64_t cap, int64_t *len) {
+    API_BEGIN
+    AXG_REQUIRE(deck && u_full && rf && len, AXG_E_INVALID, "axg_deck_write_dat: NULL argument");
+    const axg_deck &d = *deck;
+    std::string o;
+    auto time0 = [&](std::string &s) { rust_exp(s, 0.0, 7, 14); };
+    auto e14 = [](std::string &s, double v) {
+        s += ' ';
+        rust_exp(s, v, 6, 14);
+    };
+    o += "\n displacements (vx,vy,vz) for set NALL and time  ";
+    time0(o);
+    o += "\n\n";
+    o += format_rows(d.nnode, [&](int64_t k, std::string &s) {
+        rust_int(s, d.node_ids[k], 10);
+        for (int c = 0; c < 3; c++) e14(s, u_full[3 * k + c]);
+        s += '\n';
+    });
+    o += "\n forces (fx,fy,fz) for set NALL and time  ";
+    time0(o);
+    o += "\n\n";
+    o += format_rows(d.nnode, [&](int64_t k, std::string &s) {
+        const double *r = rf + 3 * k;
+        if (std::fabs(r[0]) + std::fabs(r[1]) + std::fabs(r[2]) > 1e-14) {
+            rust_int(s, d.node_ids[k], 10);
+            for (int c = 0; c < 3; c++) e14(s, r[c]);
+            s += '\n';
+        }
+    });
+    o += "\n stresses (elem, integ.pnt.,sxx,syy,szz,sxy,syz,szx) for set EALL and time  ";
+    time0(o);
+    o += "\n\n";
+    if (stress_gp)
+        o += format_rows(d.nelem, [&](int64_t e, std::string &s) {
+            rust_int(s, d.elements[e].id, 10);
+            s += ' ';
+            rust_int(s, 1, 4);
+            for (int c = 0; c < 6; c++) e14(s, stress_gp[6 * e + c]);
+            s += '\n';
+        });
+    o += '\n';
+    if (nodal_stress) {  // append_nodal_stress, dat.rs:70-83
+        o += "\n stresses (node,sxx,syy,szz,sxy,syz,szx) for set NALL\n\n";
+        o += format_rows(d.nnode, [&](int64_t k, std::string &s) {
+            rust_int(s, d.node_ids[k], 10);
+            for (int c = 0; c < 6; c++) e14(s, nodal_stress[6 * k + c]);
+            s += '\n';
+        });
+        o += '\n';
+    }
+    if (freqs && nfreq > 0) {  // append_frequencies, dat.rs:85-94
+        o += "\n EIGENVALUE OUTPUT\n\n MODE  FREQUENCY (CYCLES/TIME)\n\n";
+        for (int32_t i = 0; i < nfreq; i++) {
+            rust_int(o, i + 1, 10);
+            e14(o, freqs[i]);
+            o += '\n';
+        }
+        o += '\n';
+    }
+    copy_out(o, buf, cap, len);
+    API_END
+}
+
+// frd.rs:42-92: an integer right-aligned in [start, start + width) (the last
+// width characters when longer)
+static void put_int(std::string &line, size_t start, size_t width, int64_t v) {
+    std::string t;
+    rust_int(t, v, (int)width);
+    const size_t n = std::min(t.size(), width);
+    line.replace(start + width - n, n, t, t.size() - n, n);
+}
+
+// frd.rs:173-452 for one static frame (time 1, increment 1): the heading and
+// fixed header records, the nodes used by elements (2C) and the elements (3C,
+// C3D20 midsides in FRD order), then DISP, STRESS, FORC and TOSTRAIN blocks
+// as the deck's *NODE FILE / *EL FILE select them, each with its 1PSTEP and
+// 100CL records. stress, rf, strain may be NULL (zeros); without a strain the
+// linear path's rule applies: strain from the nodal stress with the first
+// material's E, nu (analysis.rs:1163-1174).
+int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stress,
+                       const double *rf, const double *strain, char *buf, int64_t cap,
+                       int64_t *len) {
+    API_BEGIN
+    AXG_REQUIRE(deck && u_full && len, AXG_E_INVALID, "axg_deck_write_frd: NULL argument");
+    const axg_deck &d = *deck;
+    std::string o;
+    o.reserve(1 << 16);
+    // the heading's first 66 characters (UTF-8 code points)
+    std::string h = d.heading.empty() ? std::string("Axia") : d.heading;
+    {
+        size_t cut = 0, count = 0;
+        while (cut < h.size() && count < 66) {
+            const unsigned char c = (unsigned char)h[cut];
+            cut += c < 0x80 ? 1 : c < 0xE0 ? 2 : c < 0xF0 ? 3 : 4;
+            count++;
+        }
+        h = h.substr(0, std::min(cut, h.size()));
+    }
+    o += "    1C" + h + "\n";
+    o += "    1UDATE              24.September.2026\n";
+    o += "    1UTIME              21:00:00\n";
+    o += "    1UHOST              axia\n";
+    o += "    1UPGM               Axia FEM\n";
+    o += "    1UVERSION           1.31.10\n";
+    o += "    1UCODE              CalculiX-compatible Axia FEM\n";
+    // frd_output_nodes (frd.rs:134-170): the nodes elements use, in node order
+    std::vector<int64_t> out;
+    {
+        std::vector<uint8_t> used(d.nnode, 0);
+        for (int32_t c : d.conn) used[c] = 1;
+        for (int64_t i = 0; i < d.nnode; i++)
+            if (used[i]) out.push_back(i);
+        if (out.empty())
+            for (int64_t i = 0; i < d.nnode; i++) out.push_back(i);
+    }
+    const int64_t nout = (int64_t)out.size();
+    auto block_header = [&](char kind, int64_t n) {  // frd.rs:52-55
+        o += "    ";
+        o += kind;
+        o += 'C';
+        o.append(18, ' ');
+        rust_int(o, n, 12);
+        rust_int(o, 1, 38);
+        o += '\n';
+    };
+    auto e12 = [](std::string &s, double v) {
+        char t[12];
+        frd_e12(v, t);
+        s.append(t, 12);
+    };
+    block_header('2', nout);
+    o += format_rows(nout, [&](int64_t k, std::string &s) {
+        const int64_t i = out[k];
+        s += " -1";
+        rust_int(s, d.node_ids[i], 10);
+        for (int q = 0; q < 3; q++) e12(s, d.coords[3 * i + q]);
+        s += '\n';
+    });
+    o += " -3\n";
+    block_header('3', d.nelem);
+    const int ty = d.nn == 8 ? 1 : 4;  // ElemKind::frd_type (model.rs:364-369)
+    o += format_rows(d.nelem, [&](int64_t e, std::string &s) {
+        const DeckElem &el = d.elements[e];
+        s += " -1";
+        rust_int(s, el.id, 10);
+        rust_int(s, ty, 5);
+        rust_int(s, 0, 5);
+        rust_int(s, 1, 5);
+        s += '\n';
+        // frd_nodes (frd.rs:110-132): C3D20 INP 1-12, 17-20, 13-16
+        static const int k20[20] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 16, 17, 18, 19, 12, 13, 14, 15};
+        for (int a0 = 0; a0 < d.nn; a0 += 10) {  // write_minus2, 10 per line
+            s += " -2";
+            for (int a = a0; a < std::min(d.nn, a0 + 10); a++)
+                rust_int(s, el.nodes[d.nn == 20 ? k20[a] : a], 10);
+            s += '\n';
+        }
+    });
+    o += " -3\n";
+    int counter = 0;
+    const double time = 1.0;
+    auto result_block = [&](const char *name, int ncomp, const std::vector<std::array<int, 3>> &meta,
+                            const std::vector<const char *> &cnames, bool last_all,
+                            const std::function<double(int64_t, int)> &val) {
+        counter++;
+        std::string p(70, ' ');  // 1PSTEP (frd.rs:57-64)
+        p.replace(0, 10, "    1PSTEP");
+        put_int(p, 24, 12, counter);
+        put_int(p, 36, 12, 1);
+        put_int(p, 48, 12, 1);
+        o += p + "\n";
+        std::string c(75, ' ');  // 100CL (frd.rs:81-92)
+        c.replace(0, 7, "  100CL");
+        put_int(c, 7, 5, 101);
+        char t[12];
+        frd_e12(time, t);
+        c.replace(12, 12, t, 12);
+        put_int(c, 24, 12, nout);
+        c[57] = '0';
+        put_int(c, 58, 5, 1);
+        c[74] = '1';
+        o += c + "\n";
+        std::string nm(name);
+        o += " -4  " + nm + std::string(nm.size() < 8 ? 8 - nm.size() : 0, ' ');
+        rust_int(o, ncomp, 5);
+        o += "    1\n";
+        for (size_t i = 0; i < meta.size(); i++) {
+            std::string cn(cnames[i]);
+            o += " -5  " + cn + std::string(cn.size() < 8 ? 8 - cn.size() : 0, ' ') + "    1";
+            rust_int(o, meta[i][0], 5);
+            rust_int(o, meta[i][1], 5);
+            rust_int(o, meta[i][2], 5);
+            if (last_all && i + 1 == meta.size()) o += "    1ALL";
+            o += '\n';
+        }
+        const int nvals = last_all ? ncomp - 1 : ncomp;
+        o += format_rows(nout, [&](int64_t k, std::string &s) {
+            const int64_t i = out[k];
+            s += " -1";
+            rust_int(s, d.node_ids[i], 10);
+            for (int q = 0; q < nvals; q++) e12(s, val(i, q));
+            s += '\n';
+        });
+        o += " -3\n";
+    };
+    if (d.output_u)
+        result_block("DISP", 4, {{2, 1, 0}, {2, 2, 0}, {2, 3, 0}, {2, 0, 0}},
+                     {"D1", "D2", "D3", "ALL"}, true,
+                     [&](int64_t i, int q) { return u_full[3 * i + q]; });
+    if (d.output_s)
+        result_block("STRESS", 6, {{4, 1, 1}, {4, 2, 2}, {4, 3, 3}, {4, 1, 2}, {4, 2, 3}, {4, 3, 1}},
+                     {"SXX", "SYY", "SZZ", "SXY", "SYZ", "SZX"}, false,
+                     [&](int64_t i, int q) { return stress ? stress[6 * i + q] : 0.0; });
+    if (d.output_rf)
+        result_block("FORC", 4, {{2, 1, 0}, {2, 2, 0}, {2, 3, 0}, {2, 0, 0}},
+                     {"F1", "F2", "F3", "ALL"}, true,
+                     [&](int64_t i, int q) { return rf ? rf[3 * i + q] : 0.0; });
+    if (d.output_e) {
+        std::vector<double> e_lin;
+        const double *ev = strain;
+        if (!ev && stress) {
+            AXG_REQUIRE(!d.materials.empty(), AXG_E_INVALID, "Kein Material");
+            const double e = d.materials.begin()->second.first, nu = d.materials.begin()->second.second;
+            e_lin.resize((size_t)d.nnode * 6);
+            for (int64_t i = 0; i < d.nnode; i++) {
+                const double *sv = stress + 6 * i;
+                const double tr = sv[0] + sv[1] + sv[2];
+                const double g2 = e / (1.0 + nu);
+                for (int c = 0; c < 3; c++) e_lin[6 * i + c] = ((1.0 + nu) * sv[c] - nu * tr) / e;
+                for (int c = 3; c < 6; c++) e_lin[6 * i + c] = sv[c] / g2;
+            }
+            ev = e_lin.data();
+        }
+        result_block("TOSTRAIN", 6, {{4, 1, 1}, {4, 2, 2}, {4, 3, 3}, {4, 1, 2}, {4, 2, 3}, {4, 3, 1}},
+                     {"EXX", "EYY", "EZZ", "EXY", "EYZ", "EZX"}, false,
+                     [&](int64_t i, int q) { return ev ? ev[6 * i + q] : 0.0; });
+    }
+    o += " 9999\n";
+    copy_out(o, buf, cap, len);
     API_END
 }
 

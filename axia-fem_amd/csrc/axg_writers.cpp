@@ -1,0 +1,118 @@
+// Number formats of the reference's writers and a threaded row formatter.
+// Built without FMA contraction: frd_e12 follows frd.rs's arithmetic.
+
+#include "axg_writers.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace axg {
+
+// 10^e the way f64::powi computes it on this target: LLVM lowers powi with a
+// variable exponent to the runtime's repeated squaring (binary
+// exponentiation, then one reciprocal for a negative exponent).
+static double powi_rt(double a, int b) {
+    const bool recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+void frd_e12(double v, char out[12]) {
+    if (!std::isfinite(v) || v == 0.0) {
+        std::memcpy(out, " 0.00000E+00", 12);
+        return;
+    }
+    const char sign = std::signbit(v) ? '-' : ' ';
+    const double av = std::fabs(v);
+    int ex = (int)std::floor(std::log10(av));
+    double mant = av / powi_rt(10.0, ex);
+    if (mant >= 10.0) {
+        mant /= 10.0;
+        ex += 1;
+    } else if (mant < 1.0) {
+        mant *= 10.0;
+        ex -= 1;
+    }
+    double scaled = std::round(mant * 1e5);  // halfway cases away from zero, as f64::round
+    if (scaled >= 1e6) {
+        scaled /= 10.0;
+        ex += 1;
+    }
+    char buf[64];
+    const int len = std::snprintf(buf, sizeof buf, "%c%.5fE%+03d", sign, scaled / 1e5, ex);
+    if (len >= 12) {
+        std::memcpy(out, buf, 12);  // a three-digit exponent is cut, as the reference does
+    } else {
+        const int pad = 12 - len;
+        std::memset(out, ' ', pad);
+        std::memcpy(out + pad, buf, len);
+    }
+}
+
+void rust_exp(std::string &s, double v, int prec, int width) {
+    char body[64];
+    int len;
+    if (std::isnan(v)) {
+        len = std::snprintf(body, sizeof body, "NaN");
+    } else if (std::isinf(v)) {
+        len = std::snprintf(body, sizeof body, v > 0 ? "inf" : "-inf");
+    } else {
+        // C prints "d.ddddddE+dd"; Rust prints the exponent as a plain integer
+        char c[64];
+        std::snprintf(c, sizeof c, "%.*E", prec, v);
+        char *e = std::strchr(c, 'E');
+        const bool neg = e[1] == '-';
+        const long ev = std::strtol(e + 2, nullptr, 10);
+        *e = '\0';
+        len = std::snprintf(body, sizeof body, "%sE%s%ld", c, neg && ev != 0 ? "-" : "", ev);
+    }
+    if (len < width) s.append((size_t)(width - len), ' ');
+    s.append(body, (size_t)len);
+}
+
+void rust_int(std::string &s, int64_t v, int width) {
+    char b[32];
+    const int len = std::snprintf(b, sizeof b, "%lld", (long long)v);
+    if (len < width) s.append((size_t)(width - len), ' ');
+    s.append(b, (size_t)len);
+}
+
+std::string format_rows(int64_t n, const std::function<void(int64_t, std::string &)> &row,
+                        int threads) {
+    if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    const int64_t min_chunk = 4096;
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n / min_chunk));
+    std::vector<std::string> part(nt);
+    auto work = [&](int t) {
+        const int64_t a = n * t / nt, b = n * (t + 1) / nt;
+        std::string &o = part[t];
+        o.reserve((size_t)(b - a) * 96);
+        for (int64_t i = a; i < b; i++) row(i, o);
+    };
+    if (nt == 1) {
+        work(0);
+        return std::move(part[0]);
+    }
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    size_t total = 0;
+    for (auto &p : part) total += p.size();
+    std::string out;
+    out.reserve(total);
+    for (auto &p : part) out += p;
+    return out;
+}
+
+}  // namespace axg

@@ -661,6 +661,12 @@ def run_nlgeom(a, G, MG, torch, np):
     tip = np.nonzero(xyz[:, 0] == 10.0)[0]
     fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
     value = n * res.pcg_iters / (res.solve_ms / 1e3)
+    # the result files (frd.rs / dat.rs text), formatted on the host's threads
+    t3 = time.perf_counter()
+    frd = dk.write_frd(res.u_full, res.stress, res.rf, res.strain)
+    dat = dk.write_dat(res.u_full, res.rf, res.stress_gp)
+    write_ms = 1e3 * (time.perf_counter() - t3)
+    write_mb = (len(frd) + len(dat)) / 1e6
     out = base_line(a, value, ms_per_step, "f64", {
         "workload": f"*STEP, NLGEOM C3D8 cantilever {nx}x{ny}x{nz} (10 x 1 x 1, n={n}), tip "
                     f"load 0.5, 4 increments, Newton with the device tangent and block-Jacobi "
@@ -675,6 +681,7 @@ def run_nlgeom(a, G, MG, torch, np):
                    "reaction_z_sum": float(res.rf[3 * fixed + 2].sum()),
                    "tangent_reassembly_ms": asm_ms, "tangent_reassembly_elements_per_s": asm_eps,
                    "tangent_fp64_tflops": nl_flops(8, 8) * asm_eps / 1e12,
+                   "frd_dat_write_ms": write_ms, "frd_dat_mb": write_mb,
                    "solver": res.solver},
         "roofline": roofline("sell_spmv_kernel (FP64 SpMV of the tangent in the PCG)",
                              sell64_bytes(n, nnz), spmv_ms),

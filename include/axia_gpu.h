@@ -260,9 +260,11 @@ int axg_deck_arrays(axg_deck *deck, int32_t *node_ids, int32_t *elem_ids, double
 /* solve_linear (analysis.rs:548-1162) on the device: assembly, autospc of
  * null diagonals (analysis.rs:2164-2228), reduced rhs, PCG (opts), u_full
  * (3*nnode), reactions rf (3*nnode, may be NULL), nodal stress (nnode x 6)
- * and von Mises (nnode) (both may be NULL). */
+ * and von Mises (nnode) (both may be NULL), element mean stresses stress_gp
+ * (nelem x 6, analysis.rs:1145-1151, may be NULL). */
 int axg_deck_solve(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
-                   double *rf, double *stress, double *vm, axg_solve_info *info);
+                   double *rf, double *stress, double *vm, double *stress_gp,
+                   axg_solve_info *info);
 
 /* The deck's procedure: modes of its *FREQUENCY step (inp.rs:1062-1071),
  * buckling factors of its *BUCKLE step (inp.rs:1072-1081), 0 for a step that
@@ -295,11 +297,30 @@ int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
  * assembled on the device (axg_nl_assemble), follower pressure, the GPU PCG
  * of opts for each correction. u_full (3*nnode); rf = f_int - f_ext, nodal
  * Cauchy stress (nnode x 6), von Mises (nnode), nodal Green-Lagrange strain
- * (nnode x 6) may be NULL; ninc_done, newton_iters (the last increment's
- * iterations) may be NULL. */
+ * (nnode x 6), element mean Cauchy stress_gp (nelem x 6) may be NULL;
+ * ninc_done, newton_iters (the last increment's iterations) may be NULL. */
 int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, double *u_full,
-                    double *rf, double *stress, double *vm, double *strain, int32_t *ninc_done,
-                    int32_t *newton_iters, axg_solve_info *info);
+                    double *rf, double *stress, double *vm, double *strain, double *stress_gp,
+                    int32_t *ninc_done, int32_t *newton_iters, axg_solve_info *info);
+
+/* ------------------------------------------------------------- writers */
+/* The reference's DAT text (dat.rs:3-94) for a deck's results, byte for byte:
+ * displacements, forces (|fx|+|fy|+|fz| > 1e-14), element stresses stress_gp
+ * (nelem x 6, may be NULL), nodal stresses (nnode x 6, may be NULL) and
+ * frequencies (nfreq, may be 0). Two-call protocol: with buf == NULL only
+ * *len (bytes, no terminating NUL) is set; then pass a buffer of cap >= *len. */
+int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
+                       const double *stress_gp, const double *nodal_stress, const double *freqs,
+                       int32_t nfreq, char *buf, int64_t cap, int64_t *len);
+/* The reference's FRD text (frd.rs:173-452, one static frame), byte for byte:
+ * heading and header records, nodes used by elements, elements (C3D20 in FRD
+ * node order), then DISP / STRESS / FORC / TOSTRAIN as the deck's *NODE FILE
+ * and *EL FILE select them. stress, rf, strain may be NULL (zeros; without a
+ * strain, TOSTRAIN follows the linear path's rule from the nodal stress).
+ * Same two-call protocol. */
+int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stress,
+                       const double *rf, const double *strain, char *buf, int64_t cap,
+                       int64_t *len);
 
 /* Batched solves: nmesh meshes (assembled), each with nrhs[m] right-hand
  * sides rhs[m] (n_free[m] * nrhs[m], column-major by right-hand side),

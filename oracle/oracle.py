@@ -369,7 +369,9 @@ def _keyword(line):
             params[k.strip()] = v.strip()
         else:
             params[p] = ""
-    kw = {"*SOLIDSECTION": "*SOLID SECTION", "*ENDSTEP": "*END STEP"}.get(kw, kw)
+    kw = {"*SOLIDSECTION": "*SOLID SECTION", "*ENDSTEP": "*END STEP", "*ELFILE": "*EL FILE",
+          "*NODEFILE": "*NODE FILE", "*NODEOUTPUT": "*NODE OUTPUT",
+          "*ELOUTPUT": "*EL OUTPUT"}.get(kw, kw)
     return kw, params
 
 
@@ -390,6 +392,9 @@ class Model:
     densities: dict = field(default_factory=dict)  # material name -> density (*DENSITY)
     nmodes: int = 0  # *FREQUENCY n (0: static step)
     nbuckle: int = 0  # *BUCKLE n (0: no buckling step)
+    heading: str = ""  # *HEADING
+    output: dict = field(default_factory=lambda: {"u": True, "s": True, "rf": True, "e": False,
+                                                  "nt": False})  # *NODE FILE / *EL FILE
     nlgeom: bool = False  # *STEP, NLGEOM
     max_inc: int = 100  # *STEP, INC=
     ninc: int = 1  # increments of the static step (*STATIC: round(period / dt0), capped)
@@ -580,6 +585,40 @@ def parse_inp(text):
                 except ValueError:
                     nm = 10
             m.nmodes = max(nm, 1)
+        elif kw == "*HEADING":
+            # inp.rs:410-417
+            i += 1
+            while i < n and (not lines[i].strip() or lines[i].strip().startswith("**")):
+                i += 1
+            if i < n and not _is_kw(lines[i]):
+                m.heading = lines[i].strip()
+                i += 1
+        elif kw in ("*NODE FILE", "*NODE OUTPUT"):
+            # inp.rs:1989-2004
+            toks, i = collect(i + 1)
+            m.output["u"] = m.output["rf"] = False
+            for t in toks:
+                t = t.strip().upper()
+                if t == "U":
+                    m.output["u"] = True
+                elif t == "RF":
+                    m.output["rf"] = True
+                elif t == "NT":
+                    m.output["nt"] = True
+            if not m.output["u"] and not m.output["rf"]:
+                m.output["u"] = True
+        elif kw in ("*EL FILE", "*ELEMENT OUTPUT"):
+            # inp.rs:2006-2021
+            toks, i = collect(i + 1)
+            m.output["s"] = m.output["e"] = False
+            for t in toks:
+                t = t.strip().upper()
+                if t == "S":
+                    m.output["s"] = True
+                elif t == "E":
+                    m.output["e"] = True
+            if not m.output["s"] and not m.output["e"]:
+                m.output["s"] = True
         elif kw == "*STEP":
             # inp.rs:953-982: NLGEOM (any value) switches the static step to NLGEOM
             if "NLGEOM" in params:
@@ -1429,3 +1468,199 @@ def nlgeom_deck(text, newton_tol=1e-8, max_newton=25, solve=None):
     vm = np.array([von_mises(sv) for sv in stress])
     return NlResult(u_full, rf, stress, strain, vm, stress_gp, iters, ninc_done, residual,
                     n_free, ind_of)
+
+
+# ------------------------------------------------------------------ writers
+def rust_exp(v, prec, width):
+    """Rust's "{:W.PE}" for an f64: the exponent as a plain integer (no '+', no padding)."""
+    if math.isnan(v):
+        body = "NaN"
+    elif math.isinf(v):
+        body = "inf" if v > 0 else "-inf"
+    else:
+        mant, ex = format(v, f".{prec}E").split("E")
+        body = f"{mant}E{int(ex)}"
+    return body.rjust(width)
+
+
+def _powi(a, b):
+    """f64::powi with a variable exponent: binary exponentiation, one reciprocal."""
+    recip = b < 0
+    r = 1.0
+    while True:
+        if b & 1:
+            r *= a
+        b = int(b / 2)
+        if b == 0:
+            break
+        a *= a
+    return 1.0 / r if recip else r
+
+
+def frd_e12(v):
+    """frd.rs:7-40: Fortran ES12.5."""
+    if not math.isfinite(v) or v == 0.0:
+        return " 0.00000E+00"
+    sign = "-" if math.copysign(1.0, v) < 0 else " "
+    av = abs(v)
+    ex = math.floor(math.log10(av))
+    p10 = _powi(10.0, ex)
+    mant = av / p10 if p10 != 0.0 else math.inf  # IEEE: a subnormal over 10^-324 (= 0)
+    if mant >= 10.0:
+        mant /= 10.0
+        ex += 1
+    elif mant < 1.0:
+        mant *= 10.0
+        ex -= 1
+    x = mant * 1e5
+    scaled = x if math.isinf(x) else float(math.floor(x + 0.5))  # f64::round, x >= 0
+    if scaled >= 1e6:
+        scaled /= 10.0
+        ex += 1
+    s = f"{sign}{scaled / 1e5:.5f}E{ex:+03d}"
+    return s[:12] if len(s) >= 12 else s.rjust(12)
+
+
+def write_dat(model, u_full, rf, stress_gp=None, nodal_stress=None, freqs=None):
+    """dat.rs:3-94 (static / frequency result of a solid deck)."""
+    ids = model.node_ids
+    u3 = np.asarray(u_full).reshape(-1, 3)
+    r3 = np.asarray(rf).reshape(-1, 3)
+    t0 = rust_exp(0.0, 7, 14)
+    o = [f"\n displacements (vx,vy,vz) for set NALL and time  {t0}\n\n"]
+    for k, nid in enumerate(ids):
+        o.append(f"{nid:10d}" + "".join(" " + rust_exp(float(x), 6, 14) for x in u3[k]) + "\n")
+    o.append(f"\n forces (fx,fy,fz) for set NALL and time  {t0}\n\n")
+    for k, nid in enumerate(ids):
+        r = r3[k]
+        if abs(r[0]) + abs(r[1]) + abs(r[2]) > 1e-14:
+            o.append(f"{nid:10d}" + "".join(" " + rust_exp(float(x), 6, 14) for x in r) + "\n")
+    o.append("\n stresses (elem, integ.pnt.,sxx,syy,szz,sxy,syz,szx) for set EALL and time  "
+             f"{t0}\n\n")
+    if stress_gp is not None:
+        for k, el in enumerate(model.elements):
+            o.append(f"{el[0]:10d} {1:4d}" +
+                     "".join(" " + rust_exp(float(x), 6, 14) for x in stress_gp[k]) + "\n")
+    o.append("\n")
+    if nodal_stress is not None:
+        o.append("\n stresses (node,sxx,syy,szz,sxy,syz,szx) for set NALL\n\n")
+        for k, nid in enumerate(ids):
+            o.append(f"{nid:10d}" +
+                     "".join(" " + rust_exp(float(x), 6, 14) for x in nodal_stress[k]) + "\n")
+        o.append("\n")
+    if freqs is not None and len(freqs):
+        o.append("\n EIGENVALUE OUTPUT\n\n MODE  FREQUENCY (CYCLES/TIME)\n\n")
+        for i, f in enumerate(freqs):
+            o.append(f"{i + 1:10d} {rust_exp(float(f), 6, 14)}\n")
+        o.append("\n")
+    return "".join(o)
+
+
+def _put_int(line, start, width, v):
+    t = f"{v:>{width}}"
+    t = t[len(t) - width:] if len(t) > width else t
+    return line[:start + width - len(t)] + t + line[start + width:]
+
+
+def frd_1pstep(counter, iinc, istep):
+    """frd.rs:57-64."""
+    t = "    1PSTEP" + " " * 60
+    t = _put_int(t, 24, 12, counter)
+    t = _put_int(t, 36, 12, iinc)
+    return _put_int(t, 48, 12, istep)
+
+
+def frd_100cl(iinc, time, nout):
+    """frd.rs:81-92."""
+    t = "  100CL" + " " * 68
+    t = _put_int(t, 7, 5, 100 + iinc)
+    t = t[:12] + frd_e12(time) + t[24:]
+    t = _put_int(t, 24, 12, nout)
+    t = t[:57] + "0" + t[58:]
+    t = _put_int(t, 58, 5, iinc)
+    return t[:74] + "1"
+
+
+_FRD20 = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 16, 17, 18, 19, 12, 13, 14, 15]
+
+
+def write_frd(model, u_full, stress=None, rf=None, strain=None):
+    """frd.rs:173-452 for one static frame of a solid deck."""
+    h = model.heading if model.heading else "Axia"
+    o = [f"    1C{h[:66]}\n",
+         "    1UDATE              24.September.2026\n", "    1UTIME              21:00:00\n",
+         "    1UHOST              axia\n", "    1UPGM               Axia FEM\n",
+         "    1UVERSION           1.31.10\n",
+         "    1UCODE              CalculiX-compatible Axia FEM\n"]
+    used = set()
+    for el in model.elements:
+        used.update(el[2])
+    out = [(nid, k) for k, nid in enumerate(model.node_ids) if nid in used]
+    if not out:
+        out = list(enumerate(model.node_ids))
+        out = [(nid, k) for k, nid in out]
+    nn = model.elements[0][1]
+
+    def header(kind, n):
+        return f"    {kind}C{'':18}{n:>12}{1:>38}\n"
+
+    o.append(header("2", len(out)))
+    for nid, k in out:
+        c = model.coords[k]
+        o.append(f" -1{nid:10d}{frd_e12(c[0])}{frd_e12(c[1])}{frd_e12(c[2])}\n")
+    o.append(" -3\n")
+    o.append(header("3", len(model.elements)))
+    ty = 1 if nn == 8 else 4
+    for el in model.elements:
+        o.append(f" -1{el[0]:10d}{ty:5d}{0:5d}{1:5d}\n")
+        nodes = list(el[2])
+        if nn == 20:
+            nodes = [nodes[a] for a in _FRD20]
+        for a0 in range(0, len(nodes), 10):
+            o.append(" -2" + "".join(f"{x:10d}" for x in nodes[a0:a0 + 10]) + "\n")
+    o.append(" -3\n")
+    nnode = len(model.node_ids)
+    u3 = np.asarray(u_full).reshape(-1, 3)
+    s6 = np.zeros((nnode, 6)) if stress is None else np.asarray(stress).reshape(-1, 6)
+    r3 = np.zeros((nnode, 3)) if rf is None else np.asarray(rf).reshape(-1, 3)
+    counter = [0]
+
+    def block(name, ncomp, comps, last_all, vals):
+        counter[0] += 1
+        o.append(frd_1pstep(counter[0], 1, 1) + "\n")
+        o.append(frd_100cl(1, 1.0, len(out)) + "\n")
+        o.append(f" -4  {name:<8}{ncomp:5d}    1\n")
+        for i, (cn, typ, a, b) in enumerate(comps):
+            tail = "    1ALL" if last_all and i == len(comps) - 1 else ""
+            o.append(f" -5  {cn:<8}    1{typ:5d}{a:5d}{b:5d}{tail}\n")
+        nv = ncomp - 1 if last_all else ncomp
+        for nid, k in out:
+            o.append(f" -1{nid:10d}" + "".join(frd_e12(float(vals[k][q])) for q in range(nv)) + "\n")
+        o.append(" -3\n")
+
+    if model.output["u"]:
+        block("DISP", 4, [("D1", 2, 1, 0), ("D2", 2, 2, 0), ("D3", 2, 3, 0), ("ALL", 2, 0, 0)],
+              True, u3)
+    if model.output["s"]:
+        block("STRESS", 6, [("SXX", 4, 1, 1), ("SYY", 4, 2, 2), ("SZZ", 4, 3, 3),
+                            ("SXY", 4, 1, 2), ("SYZ", 4, 2, 3), ("SZX", 4, 3, 1)], False, s6)
+    if model.output["rf"]:
+        block("FORC", 4, [("F1", 2, 1, 0), ("F2", 2, 2, 0), ("F3", 2, 3, 0), ("ALL", 2, 0, 0)],
+              True, r3)
+    if model.output["e"]:
+        if strain is None:
+            e, nu = model.materials[sorted(model.materials)[0]]
+            strain = np.zeros((nnode, 6))
+            for i in range(nnode):
+                sv = s6[i]
+                tr = sv[0] + sv[1] + sv[2]
+                g2 = e / (1.0 + nu)
+                for c in range(3):
+                    strain[i, c] = ((1.0 + nu) * sv[c] - nu * tr) / e
+                for c in range(3, 6):
+                    strain[i, c] = sv[c] / g2
+        block("TOSTRAIN", 6, [("EXX", 4, 1, 1), ("EYY", 4, 2, 2), ("EZZ", 4, 3, 3),
+                              ("EXY", 4, 1, 2), ("EYZ", 4, 2, 3), ("EZX", 4, 3, 1)], False,
+              np.asarray(strain).reshape(-1, 6))
+    o.append(" 9999\n")
+    return "".join(o)
