@@ -129,6 +129,33 @@ __global__ void __launch_bounds__(TB) batch_fin_pap_kernel(BatchDev B, const dou
         }
 }
 
+// The vector kernels work on a workgroup's TB rows x K right-hand sides as one
+// flat range: element t = (row - row0) K + j, thread tid takes t = tid + q TB
+// (q < K). Consecutive lanes touch consecutive doubles, and since TB is a
+// multiple of K every element of a thread belongs to right-hand side
+// j = tid % K, so a thread accumulates two sums and the per-right-hand-side
+// totals are shuffles over the lanes of equal j.
+template <int K>
+__device__ inline void flat_sums_out(double rr, double rz, double *__restrict__ part) {
+    for (int off = 32; off >= K; off >>= 1) {
+        rr += __shfl_down(rr, off, 64);
+        rz += __shfl_down(rz, off, 64);
+    }
+    __shared__ double sh[TB / 64][2 * K];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < K) {
+        sh[w][lane] = rr;
+        sh[w][K + lane] = rz;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * K) {
+        double t = 0.0;
+#pragma unroll
+        for (int v = 0; v < TB / 64; v++) t += sh[v][threadIdx.x];
+        part[(size_t)blockIdx.x * 2 * K + threadIdx.x] = t;
+    }
+}
+
 // x += alpha p, r -= alpha Ap, z = r / d, ||r||^2, r.z (per right-hand side).
 template <int K>
 __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__restrict__ X,
@@ -138,34 +165,33 @@ __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__
                                                          double *__restrict__ Z,
                                                          double *__restrict__ part,
                                                          const SegScalars *sc) {
+    static_assert(TB % K == 0, "flat layout needs TB % K == 0");
     const int seg = B.wg_seg[blockIdx.x];
     if (B.seg_live[seg] == 0) return;
-    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
-    double acc[2 * K];
+    const int64_t row0 = B.wg_row[blockIdx.x], rend = B.seg_row[seg + 1];
+    const int j = threadIdx.x % K;
+    const SegScalars &s = sc[seg * K + j];
+    const bool live = !s.done;
+    const double alpha = s.alpha;
+    double rr = 0.0, rz = 0.0;
 #pragma unroll
-    for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
-    if (row < B.seg_row[seg + 1]) {
-        const double d = B.dsel[row];
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const SegScalars &s = sc[seg * K + j];
-            const int64_t i = row * K + j;
-            double r = R[i];
-            if (!s.done) {
-                X[i] += s.alpha * P[i];
-                r -= s.alpha * AP[i];
-                R[i] = r;
-            }
-            const double z = r / d;
-            Z[i] = z;
-            acc[j] += r * r;
-            acc[K + j] += r * z;
+    for (int q = 0; q < K; q++) {
+        const int t = threadIdx.x + q * TB;
+        const int64_t row = row0 + t / K;
+        if (row >= rend) break;  // t / K grows with q
+        const int64_t i = row0 * K + t;
+        double r = R[i];
+        if (live) {
+            X[i] += alpha * P[i];
+            r -= alpha * AP[i];
+            R[i] = r;
         }
+        const double z = r / B.dsel[row];
+        Z[i] = z;
+        rr += r * r;
+        rz += r * z;
     }
-    group_sums<2 * K, TB>(acc);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
+    flat_sums_out<K>(rr, rz, part);
 }
 
 template <int K>
@@ -221,16 +247,17 @@ __global__ void __launch_bounds__(TB) batch_update_p_kernel(BatchDev B, double *
                                                            const SegScalars *sc, int init) {
     const int seg = B.wg_seg[blockIdx.x];
     if (!init && B.seg_live[seg] == 0) return;
-    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
-    if (row >= B.seg_row[seg + 1]) return;
+    const int64_t row0 = B.wg_row[blockIdx.x], rend = B.seg_row[seg + 1];
+    const int j = threadIdx.x % K;
+    const SegScalars &s = sc[seg * K + j];
+    if (!init && s.done) return;
+    const double beta = s.beta;
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const SegScalars &s = sc[seg * K + j];
-        const int64_t i = row * K + j;
-        if (init)
-            P[i] = Z[i];
-        else if (!s.done)
-            P[i] = Z[i] + s.beta * P[i];
+    for (int q = 0; q < K; q++) {
+        const int t = threadIdx.x + q * TB;
+        if (row0 + t / K >= rend) break;
+        const int64_t i = row0 * K + t;
+        P[i] = init ? Z[i] : Z[i] + beta * P[i];
     }
 }
 
@@ -242,28 +269,23 @@ __global__ void __launch_bounds__(TB) batch_init_kernel(BatchDev B, const double
                                                        double *__restrict__ Z,
                                                        double *__restrict__ part) {
     const int seg = B.wg_seg[blockIdx.x];
-    const int64_t row = B.wg_row[blockIdx.x] + threadIdx.x;
-    double acc[2 * K];
+    const int64_t row0 = B.wg_row[blockIdx.x], rend = B.seg_row[seg + 1];
+    double rr = 0.0, rz = 0.0;
 #pragma unroll
-    for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
-    if (row < B.seg_row[seg + 1]) {
-        const double d = B.dsel[row];
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int64_t i = row * K + j;
-            const double b = Bv[i];
-            X[i] = 0.0;
-            R[i] = b;
-            const double z = b / d;
-            Z[i] = z;
-            acc[j] += b * b;
-            acc[K + j] += b * z;
-        }
+    for (int q = 0; q < K; q++) {
+        const int t = threadIdx.x + q * TB;
+        const int64_t row = row0 + t / K;
+        if (row >= rend) break;
+        const int64_t i = row0 * K + t;
+        const double b = Bv[i];
+        X[i] = 0.0;
+        R[i] = b;
+        const double z = b / B.dsel[row];
+        Z[i] = z;
+        rr += b * b;
+        rz += b * z;
     }
-    group_sums<2 * K, TB>(acc);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
+    flat_sums_out<K>(rr, rz, part);
 }
 
 // Rebase the column indices of one mesh's sliced-ELL copy into the batch.
