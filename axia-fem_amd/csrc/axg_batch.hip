@@ -103,30 +103,37 @@ __global__ void __launch_bounds__(TB) batch_spmv_kernel(BatchDev B, const double
 template <int NC>
 __device__ inline void seg_sums(const BatchDev &B, const double *part, int seg, double (&out)[NC]) {
     const int w0 = B.seg_wg[seg], w1 = B.seg_wg[seg + 1];
-    sum_partials<NC, TB>(part + (size_t)w0 * NC, w1 - w0, out);
+    fin_sums<NC>(part + (size_t)w0 * NC, w1 - w0, out);
 }
 
+// The finalize kernels load the scalars they need before the partials and
+// test the stop flags after the sum, so the reads overlap instead of costing
+// one memory round trip each.
 template <int K>
-__global__ void __launch_bounds__(TB) batch_fin_pap_kernel(BatchDev B, const double *part,
+__global__ void __launch_bounds__(FIN_TB) batch_fin_pap_kernel(BatchDev B, const double *part,
                                                           SegScalars *sc, int32_t *live_total) {
     const int seg = blockIdx.x;
-    if (B.seg_live[seg] == 0) return;
+    const int live = B.seg_live[seg];
+    int done[K];
+    double rz[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) done[j] = sc[seg * K + j].done, rz[j] = sc[seg * K + j].rz;
     double pap[K];
     seg_sums<K>(B, part, seg, pap);
-    if (threadIdx.x == 0)
-        for (int j = 0; j < K; j++) {
-            SegScalars &s = sc[seg * K + j];
-            if (s.done) continue;
-            if (fabs(pap[j]) < 1e-30) {  // linalg.rs:231-233
-                s.status = AXG_E_NOT_SPD;
-                s.done = 1;
-                s.t_done = now_ticks();
-                B.seg_live[seg]--;
-                atomicSub(live_total, 1);
-            } else {
-                s.alpha = s.rz / pap[j];
-            }
+    if (live == 0 || threadIdx.x != 0) return;
+    for (int j = 0; j < K; j++) {
+        SegScalars &s = sc[seg * K + j];
+        if (done[j]) continue;
+        if (fabs(pap[j]) < 1e-30) {  // linalg.rs:231-233
+            s.status = AXG_E_NOT_SPD;
+            s.done = 1;
+            s.t_done = now_ticks();
+            B.seg_live[seg]--;
+            atomicSub(live_total, 1);
+        } else {
+            s.alpha = rz[j] / pap[j];
         }
+    }
 }
 
 // The vector kernels work on a workgroup's TB rows x K right-hand sides as one
@@ -195,16 +202,24 @@ __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__
 }
 
 template <int K>
-__global__ void __launch_bounds__(TB) batch_fin_update_kernel(BatchDev B, const double *part,
+__global__ void __launch_bounds__(FIN_TB) batch_fin_update_kernel(BatchDev B, const double *part,
                                                              SegScalars *sc, int init,
                                                              double tol, const double *tol_rel,
                                                              int64_t max_iter,
                                                              int32_t *live_total) {
     const int seg = blockIdx.x;
-    if (!init && B.seg_live[seg] == 0) return;
+    const int live = B.seg_live[seg];
+    int done[K];
+    int64_t iters[K];
+    double rz[K], tol_abs[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const SegScalars &s = sc[seg * K + j];
+        done[j] = s.done, iters[j] = s.iters, rz[j] = s.rz, tol_abs[j] = s.tol_abs;
+    }
     double tot[2 * K];
     seg_sums<2 * K>(B, part, seg, tot);
-    if (threadIdx.x != 0) return;
+    if ((!init && live == 0) || threadIdx.x != 0) return;
     int finished = 0;
     for (int j = 0; j < K; j++) {
         SegScalars &s = sc[seg * K + j];
@@ -221,17 +236,17 @@ __global__ void __launch_bounds__(TB) batch_fin_update_kernel(BatchDev B, const 
             }
             continue;
         }
-        if (s.done) continue;
+        if (done[j]) continue;
         const double res = sqrt(tot[j]);
         s.res = res;
-        s.iters += 1;
-        if (res <= s.tol_abs || s.iters >= max_iter) {
+        s.iters = iters[j] + 1;
+        if (res <= tol_abs[j] || iters[j] + 1 >= max_iter) {
             s.done = 1;
-            if (res > s.tol_abs && res > 10.0 * s.tol_abs) s.status = AXG_E_NOT_CONVERGED;
+            if (res > tol_abs[j] && res > 10.0 * tol_abs[j]) s.status = AXG_E_NOT_CONVERGED;
             s.t_done = now_ticks();
             finished++;
         } else {
-            s.beta = tot[K + j] / s.rz;
+            s.beta = tot[K + j] / rz[j];
             s.rz = tot[K + j];
         }
     }
@@ -441,11 +456,11 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
     };
     auto one_iter = [&]() {
         spmv_dot();
-        hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p, sc.p,
+        hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p, sc.p,
                            live_total.p);
         hipLaunchKernelGGL((batch_update_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, X.p, R.p, P.p,
                            AP.p, Z.p, part.p, sc.p);
-        hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p,
+        hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p,
                            sc.p, 0, o.tol, nullptr, max_iter, live_total.p);
         hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p,
                            0);
@@ -465,7 +480,7 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, t0.p);  // start of the solves
     hipLaunchKernelGGL((batch_init_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p, R.p, Z.p,
                        part.p);
-    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(TB), 0, s, B, part.p, sc.p,
+    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p, sc.p,
                        1, o.tol, nullptr, max_iter, live_total.p);
     hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p, 1);
     AXG_HIP(hipGetLastError());
@@ -745,11 +760,11 @@ static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x) {
 template <int K>
 static void block_iter(BlockPcg &bp, hipStream_t s) {
     block_spmv<K>(bp, s, true, bp.P.p);
-    hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(1), dim3(TB), 0, s, bp.Bmv, bp.part.p,
+    hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.Bmv, bp.part.p,
                        bp.sc.p, bp.live_total.p);
     hipLaunchKernelGGL((batch_update_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.X.p, bp.R.p,
                        bp.P.p, bp.AP.p, bp.Z.p, bp.part.p, bp.sc.p);
-    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(TB), 0, s, bp.B, bp.part.p,
+    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.B, bp.part.p,
                        bp.sc.p, 0, 0.0, bp.tolr.p, bp.max_iter, bp.live_total.p);
     hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.P.p, bp.Z.p,
                        bp.sc.p, 0);
@@ -875,7 +890,7 @@ static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *t
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     hipLaunchKernelGGL((batch_init_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.Bv.p, bp.X.p,
                        bp.R.p, bp.Z.p, bp.part.p);
-    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(TB), 0, s, bp.B, bp.part.p,
+    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.B, bp.part.p,
                        bp.sc.p, 1, 0.0, bp.tolr.p, bp.max_iter, bp.live_total.p);
     hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.P.p,
                        bp.Z.p, bp.sc.p, 1);

@@ -460,58 +460,70 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
 
 // ------------------------------------------------------ finalize kernels
 // One workgroup each: sum the partials of the kernel launched just before
-// (fixed order) and update the per-right-hand-side scalars.
+// (fixed order) and update the per-right-hand-side scalars. The scalars are
+// loaded before the partials and the stop flag is tested after the sum, so
+// the reads overlap instead of costing one memory round trip each.
 template <int K>
-__global__ void __launch_bounds__(MBS) fin_spmm_kernel(const double *__restrict__ part, int nblk,
+__global__ void __launch_bounds__(FIN_TB) fin_spmm_kernel(const double *__restrict__ part, int nblk,
                                                       InnerScalars *sc) {
-    if (sc->all_done) return;
+    const int all_done = sc->all_done;
+    int done[K];
+    double rz[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) done[j] = sc->done[j], rz[j] = sc->rz[j];
     double pap[K];
-    sum_partials<K, MBS>(part, nblk, pap);
-    if (threadIdx.x == 0) {
-        for (int j = 0; j < K; j++) {
-            if (sc->done[j]) continue;
-            if (!(pap[j] > 1e-300)) {
-                sc->done[j] = 1;  // breakdown: this sweep ends, refinement goes on
-                sc->alpha[j] = 0.0;
-            } else {
-                sc->alpha[j] = sc->rz[j] / pap[j];
-            }
+    fin_sums<K>(part, nblk, pap);
+    if (all_done || threadIdx.x != 0) return;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        if (done[j]) continue;
+        if (!(pap[j] > 1e-300)) {
+            sc->done[j] = 1;  // breakdown: this sweep ends, refinement goes on
+            sc->alpha[j] = 0.0;
+        } else {
+            sc->alpha[j] = rz[j] / pap[j];
         }
     }
 }
 
 template <int K>
-__global__ void __launch_bounds__(MBS) fin_inner_update_kernel(const double *__restrict__ part,
+__global__ void __launch_bounds__(FIN_TB) fin_inner_update_kernel(const double *__restrict__ part,
                                                               int nblk, InnerScalars *sc) {
-    if (sc->all_done) return;
+    const int all_done = sc->all_done, max_iter = sc->max_iter;
+    int done[K], iters[K];
+    double rz[K], tol_abs[K];
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        done[j] = sc->done[j], iters[j] = sc->iters[j], rz[j] = sc->rz[j], tol_abs[j] = sc->tol_abs[j];
     double tot[2 * K];
-    sum_partials<2 * K, MBS>(part, nblk, tot);
-    if (threadIdx.x == 0) {
-        int all = 1;
-        for (int j = 0; j < K; j++) {
-            if (!sc->done[j]) {
-                double res = sqrt(tot[j]);
-                sc->res[j] = res;
-                sc->iters[j] += 1;
-                if (res <= sc->tol_abs[j] || sc->iters[j] >= sc->max_iter) {
-                    sc->done[j] = 1;
-                } else {
-                    sc->beta[j] = tot[K + j] / sc->rz[j];
-                    sc->rz[j] = tot[K + j];
-                }
+    fin_sums<2 * K>(part, nblk, tot);
+    if (all_done || threadIdx.x != 0) return;
+    int all = 1;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        if (!done[j]) {
+            const double res = sqrt(tot[j]);
+            sc->res[j] = res;
+            sc->iters[j] = iters[j] + 1;
+            if (res <= tol_abs[j] || iters[j] + 1 >= max_iter) {
+                done[j] = 1;
+                sc->done[j] = 1;
+            } else {
+                sc->beta[j] = tot[K + j] / rz[j];
+                sc->rz[j] = tot[K + j];
             }
-            all &= sc->done[j];
         }
-        sc->all_done = all;
+        all &= done[j];
     }
+    sc->all_done = all;
 }
 
 template <int K>
-__global__ void __launch_bounds__(MBS) fin_inner_init_kernel(const double *__restrict__ part,
+__global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__restrict__ part,
                                                             int nblk, InnerScalars *sc,
                                                             double inner_tol) {
     double tot[2 * K];
-    sum_partials<2 * K, MBS>(part, nblk, tot);
+    fin_sums<2 * K>(part, nblk, tot);
     if (threadIdx.x == 0) {
         int all = 1;
         for (int j = 0; j < K; j++) {
@@ -910,7 +922,7 @@ struct MixedSolver {
                            w.f3.p, w.f4.p, w.part.p, dsc.p)
             AXG_SPMM_SWITCH(shp, AXG_SPMM)
 #undef AXG_SPMM
-            hipLaunchKernelGGL((fin_spmm_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p, grid_mv,
+            hipLaunchKernelGGL((fin_spmm_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_mv,
                                dsc.p);
         };
         auto launch_update = [&]() {
@@ -922,7 +934,7 @@ struct MixedSolver {
                 hipLaunchKernelGGL((inner_update_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
                                    nb, w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p,
                                    m->dinv32.p, w.part.p, dsc.p);
-            hipLaunchKernelGGL((fin_inner_update_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p,
+            hipLaunchKernelGGL((fin_inner_update_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
                                grid_v, dsc.p);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
                                w.f3.p, Z.p, dsc.p, 0);
@@ -1001,7 +1013,7 @@ struct MixedSolver {
                 hipLaunchKernelGGL((inner_init_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
                                    nb, w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p,
                                    w.part.p, dsc.p, o.inner_tol);
-            hipLaunchKernelGGL((fin_inner_init_kernel<K>), dim3(1), dim3(MBS), 0, s, w.part.p,
+            hipLaunchKernelGGL((fin_inner_init_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
                                grid_v, dsc.p, o.inner_tol);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
                                w.f3.p, Z.p, dsc.p, 1);

@@ -409,62 +409,65 @@ __global__ void __launch_bounds__(BS) update_xr_kernel(int64_t n, double *__rest
 
 // ----------------------------------------------------- finalize kernels
 // One workgroup: sum the partials of the kernel just before (fixed order)
-// and update the scalars.
-__global__ void __launch_bounds__(BS) fin_pap_kernel(const double *__restrict__ part, int nblk,
+// and update the scalars. The scalars a kernel needs are loaded before the
+// partials and the stop flag is tested after the sum, so the two reads
+// overlap instead of costing one memory round trip each.
+__global__ void __launch_bounds__(FIN_TB) fin_pap_kernel(const double *__restrict__ part, int nblk,
                                                     PcgScalars *sc) {
-    if (sc->done) return;
+    const int done = sc->done;
+    const double rz = sc->rz;
     double pap[1];
-    sum_partials<1, BS>(part, nblk, pap);
-    if (threadIdx.x == 0) {
-        sc->pap = pap[0];
-        if (fabs(pap[0]) < 1e-30) {  // linalg.rs:231-233
-            sc->status = AXG_E_NOT_SPD;
-            sc->done = 1;
-        } else {
-            sc->alpha = sc->rz / pap[0];
-        }
+    fin_sums<1>(part, nblk, pap);
+    if (done || threadIdx.x != 0) return;
+    sc->pap = pap[0];
+    if (fabs(pap[0]) < 1e-30) {  // linalg.rs:231-233
+        sc->status = AXG_E_NOT_SPD;
+        sc->done = 1;
+    } else {
+        sc->alpha = rz / pap[0];
     }
 }
 
 // ||r||, iteration count, stop decision (linalg.rs:241-245); with FUSED also
 // beta = r.z_new / r.z.
 template <bool FUSED>
-__global__ void __launch_bounds__(BS) fin_update_kernel(const double *__restrict__ part, int nblk,
+__global__ void __launch_bounds__(FIN_TB) fin_update_kernel(const double *__restrict__ part, int nblk,
                                                        PcgScalars *sc) {
-    if (sc->done) return;
+    const int done = sc->done;
+    const double rz = sc->rz, tol_abs = sc->tol_abs;
+    const int64_t iters = sc->iters + 1, max_iter = sc->max_iter;
     double tot[2];
-    sum_partials<2, BS>(part, nblk, tot);
-    if (threadIdx.x == 0) {
-        double res = sqrt(tot[0]);
-        sc->residual = res;
-        sc->iters += 1;
-        if (res <= sc->tol_abs) {
-            sc->done = 1;
-        } else if (sc->iters >= sc->max_iter) {
-            sc->done = 1;
-            sc->status = AXG_E_NOT_CONVERGED;
-        } else if (FUSED) {
-            sc->beta = tot[1] / sc->rz;
-            sc->rz = tot[1];
-        }
+    fin_sums<2>(part, nblk, tot);
+    if (done || threadIdx.x != 0) return;
+    const double res = sqrt(tot[0]);
+    sc->residual = res;
+    sc->iters = iters;
+    if (res <= tol_abs) {
+        sc->done = 1;
+    } else if (iters >= max_iter) {
+        sc->done = 1;
+        sc->status = AXG_E_NOT_CONVERGED;
+    } else if (FUSED) {
+        sc->beta = tot[1] / rz;
+        sc->rz = tot[1];
     }
 }
 
-__global__ void __launch_bounds__(BS) fin_rz_kernel(const double *__restrict__ part, int nblk,
+__global__ void __launch_bounds__(FIN_TB) fin_rz_kernel(const double *__restrict__ part, int nblk,
                                                    PcgScalars *sc, int init) {
-    if (sc->done) return;
+    const int done = sc->done;
+    const double rz = sc->rz;
     double tot[1];
-    sum_partials<1, BS>(part, nblk, tot);
-    if (threadIdx.x == 0) {
-        if (!init) sc->beta = tot[0] / sc->rz;
-        sc->rz = tot[0];
-    }
+    fin_sums<1>(part, nblk, tot);
+    if (done || threadIdx.x != 0) return;
+    if (!init) sc->beta = tot[0] / rz;
+    sc->rz = tot[0];
 }
 
-__global__ void __launch_bounds__(BS) fin_init_kernel(const double *__restrict__ part, int nblk,
+__global__ void __launch_bounds__(FIN_TB) fin_init_kernel(const double *__restrict__ part, int nblk,
                                                      PcgScalars *sc, double tol) {
     double tot[1];
-    sum_partials<1, BS>(part, nblk, tot);
+    fin_sums<1>(part, nblk, tot);
     if (threadIdx.x == 0) {
         double bn = sqrt(tot[0]);
         sc->bnorm = bn;
@@ -568,10 +571,10 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     hipLaunchKernelGGL(init_kernel, dim3(grid_v), dim3(BS), 0, s, n, b, x, w.r.p, w.part.p, w.sc.p,
                        tol);
-    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_v, w.sc.p, tol);
+    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_v, w.sc.p, tol);
     hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.r.p, w.z.p, M,
                        w.part.p, w.sc.p, 1);
-    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_v, w.sc.p, 1);
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_v, w.sc.p, 1);
     hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p, 1);
     AXG_HIP(hipGetLastError());
 
@@ -599,20 +602,20 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
             AXG_SHAPE_SWITCH(shape, AXG_L)
 #undef AXG_L
         }
-        hipLaunchKernelGGL(fin_pap_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_mv, w.sc.p);
+        hipLaunchKernelGGL(fin_pap_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_mv, w.sc.p);
         if (fused) {
             hipLaunchKernelGGL((update_xr_kernel<true>), dim3(grid_v), dim3(BS), 0, s, n, x, w.r.p,
                                w.p.p, w.ap.p, w.z.p, M.dsel, w.part.p, w.sc.p);
-            hipLaunchKernelGGL((fin_update_kernel<true>), dim3(1), dim3(BS), 0, s, w.part.p,
+            hipLaunchKernelGGL((fin_update_kernel<true>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
                                grid_v, w.sc.p);
         } else {
             hipLaunchKernelGGL((update_xr_kernel<false>), dim3(grid_v), dim3(BS), 0, s, n, x,
                                w.r.p, w.p.p, w.ap.p, w.z.p, M.dsel, w.part.p, w.sc.p);
-            hipLaunchKernelGGL((fin_update_kernel<false>), dim3(1), dim3(BS), 0, s, w.part.p,
+            hipLaunchKernelGGL((fin_update_kernel<false>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
                                grid_v, w.sc.p);
             hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.r.p, w.z.p,
                                M, w.part.p, w.sc.p, 0);
-            hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(BS), 0, s, w.part.p, grid_v, w.sc.p,
+            hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_v, w.sc.p,
                                0);
         }
         hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p,

@@ -2,7 +2,7 @@
 //
 // A kernel reduces its values to one partial per workgroup (fixed tree) and
 // stores it; a one-workgroup finalize kernel launched right after sums the
-// partials with a fixed tree and updates the solver scalars. The kernel
+// partials in a fixed order and updates the solver scalars. The kernel
 // boundary orders the partial stores before the finalize reads (no
 // per-workgroup device-scope fences, which on gfx950 write back the L2 of the
 // XCD). Results do not depend on scheduling: same inputs and launch shape,
@@ -72,29 +72,48 @@ __device__ inline void group_sums(double (&v)[NC]) {
         }
 }
 
-// Sum over b < nblk of part[b * NC + j] for every j, by the whole workgroup
-// (fixed strided order, then a fixed tree); result valid in every thread.
-template <int NC, int BSZ>
-__device__ inline void sum_partials(const double *part, int nblk, double (&out)[NC]) {
-    __shared__ double t[NC][BSZ];
-    double s[NC];
+// Threads of a finalize kernel (one or a few workgroups that sum partials).
+constexpr int FIN_TB = 1024;
+
+// Sum over b < nblk of part[b * NC + j] for every j, by a workgroup of FIN_TB
+// threads; result valid in every thread. The records are read as one flat
+// array: thread t takes elements t, t + FIN_TB, ... (consecutive lanes on
+// consecutive doubles, eight loads in flight per thread), all of which belong
+// to column t % NC since FIN_TB is a multiple of NC; lanes of equal column are
+// then folded by shuffles and the waves through LDS. The order is fixed for a
+// given nblk. The partials were written by other XCDs, so the reads miss this
+// XCD's L2: the cost is a few round trips, not bandwidth.
+template <int NC>
+__device__ inline void fin_sums(const double *__restrict__ part, int nblk, double (&out)[NC]) {
+    static_assert(NC >= 1 && NC <= 64 && (NC & (NC - 1)) == 0, "NC must be a power of two");
+    const int64_t nflat = (int64_t)nblk * NC;
+    double a[8];
 #pragma unroll
-    for (int j = 0; j < NC; j++) s[j] = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += BSZ)
+    for (int u = 0; u < 8; u++) a[u] = 0.0;
+    int64_t f = threadIdx.x;
+    for (; f + 7 * FIN_TB < nflat; f += 8 * FIN_TB)
 #pragma unroll
-        for (int j = 0; j < NC; j++) s[j] += part[(size_t)i * NC + j];
+        for (int u = 0; u < 8; u++) a[u] += part[f + u * FIN_TB];
 #pragma unroll
-    for (int j = 0; j < NC; j++) t[j][threadIdx.x] = s[j];
+    for (int u = 0; u < 7; u++)
+        if (f + u * FIN_TB < nflat) a[u] += part[f + u * FIN_TB];
+    double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+#pragma unroll
+    for (int off = 32; off >= NC; off >>= 1) s += __shfl_down(s, off, 64);
+    __shared__ double sh[FIN_TB / 64][NC];
+    __shared__ double tot[NC];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < NC) sh[w][lane] = s;
     __syncthreads();
-    for (int off = BSZ / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
+    if (threadIdx.x < NC) {
+        double t = 0.0;
 #pragma unroll
-            for (int j = 0; j < NC; j++) t[j][threadIdx.x] += t[j][threadIdx.x + off];
-        __syncthreads();
+        for (int v = 0; v < FIN_TB / 64; v++) t += sh[v][threadIdx.x];
+        tot[threadIdx.x] = t;
     }
-#pragma unroll
-    for (int j = 0; j < NC; j++) out[j] = t[j][0];
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NC; j++) out[j] = tot[j];
 }
 
 }  // namespace axg

@@ -118,7 +118,7 @@ def test_pressure_loads_bitexact_vs_oracle(nn):
         G.pressure_loads(nn, xyz, conn[:1], np.array([7], np.int32), np.array([1.0]), f)
 
 
-def test_meshgen_counts_match_survey_configs():
+def test_meshgen_counts_of_the_benchmark_shapes():
     """Exact node / element / DOF counts of the benchmark shapes."""
     xyz, conn = MG.brick_hex20(96, 24, 24)
     assert len(xyz) == 237025 and len(conn) == 55296
