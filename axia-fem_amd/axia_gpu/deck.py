@@ -162,7 +162,7 @@ class Deck:
         """Buckling factors the deck's *BUCKLE step asks for (0 without one)."""
         return self.procedure()["nbuckle"]
 
-    def nlgeom(self, ctx=None, precision="fp64", precond="block3", tol=1e-10, max_iter=0,
+    def nlgeom(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
                inner_tol=1e-3):
         """solve_continuum_newton on the device: NlResult (u, rf, nodal Cauchy stress, von
         Mises, Green-Lagrange strain, increments, Newton iterations of the last increment)."""

@@ -16,6 +16,8 @@
 #include <memory>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "axg_internal.h"
 #include "axg_mixed.h"
 #include "axg_pcg.h"
@@ -727,39 +729,58 @@ struct BlockPcg {
     DevBuf<SegScalars> sc;
     BatchDev B, Bmv;  // Bmv: the SpMM's partial sums (one per slice)
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // K = 1, 2, 4, 8
-    ~BlockPcg() {
+    const double *graph_bv = nullptr;  // operator values the graphs were captured with
+    void drop_graphs() {
         for (auto &g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
+            if (g) (void)hipGraphExecDestroy(g), g = nullptr;
     }
+    ~BlockPcg() { drop_graphs(); }
 };
 
 static constexpr int kGraphIters = 32;
 
+// With events, the launch itself stamps e0 / e1 (the SpMM's device time).
 template <int K>
-static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x) {
+static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     axg_mesh *m = bp.m;
     constexpr int LPR = K == 8 ? 2 : 1;  // 8 right-hand sides: two lanes of 4 per block row
     const dim3 g(bp.nsl_wg), b(64 * LPR);
 #define AXG_BSELL64(UU, DD)                                                                         \
-    hipLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, bp.nb, m->bs_sptr.p,     \
-                       m->bs_width.p, m->bval64.p, m->bs_nslots, x, bp.AP.p, bp.part.p,            \
-                       bp.seg_live.p)
+    if (e0)                                                                                         \
+        hipExtLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, e0, e1, 0, bp.nb, \
+                              (const int64_t *)m->bs_sptr.p, (const int32_t *)m->bs_width.p,      \
+                              (const double *)m->bval64.p, m->bs_nslots, x, bp.AP.p, bp.part.p,   \
+                              (const int32_t *)bp.seg_live.p);                                    \
+    else                                                                                            \
+        hipLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, bp.nb, m->bs_sptr.p, \
+                           m->bs_width.p, m->bval64.p, m->bs_nslots, x, bp.AP.p, bp.part.p,        \
+                           bp.seg_live.p)
     const int u = bp.U[K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3];
     if (dot) {
-        if (u == 1) AXG_BSELL64(1, true);
-        else if (u == 2) AXG_BSELL64(2, true);
-        else AXG_BSELL64(4, true);
+        if (u == 1) {
+            AXG_BSELL64(1, true);
+        } else if (u == 2) {
+            AXG_BSELL64(2, true);
+        } else {
+            AXG_BSELL64(4, true);
+        }
     } else {
-        if (u == 1) AXG_BSELL64(1, false);
-        else if (u == 2) AXG_BSELL64(2, false);
-        else AXG_BSELL64(4, false);
+        if (u == 1) {
+            AXG_BSELL64(1, false);
+        } else if (u == 2) {
+            AXG_BSELL64(2, false);
+        } else {
+            AXG_BSELL64(4, false);
+        }
     }
 #undef AXG_BSELL64
 }
 
 template <int K>
-static void block_iter(BlockPcg &bp, hipStream_t s) {
-    block_spmv<K>(bp, s, true, bp.P.p);
+static void block_iter(BlockPcg &bp, hipStream_t s, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr) {
+    block_spmv<K>(bp, s, true, bp.P.p, e0, e1);
     hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.Bmv, bp.part.p,
                        bp.sc.p, bp.live_total.p);
     hipLaunchKernelGGL((batch_update_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.X.p, bp.R.p,
@@ -861,20 +882,30 @@ BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter) {
 
 void block_pcg_destroy(BlockPcg *bp) { delete bp; }
 
+// With timing, the iterations are launched one by one (no graph) and every
+// SpMM stamps a pair of events; the launches that did work are counted from
+// the iteration counters (the SpMM returns at once when no column is live).
 template <int K>
 static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *tol_rel,
-                        double *d_x, PcgResult &res) {
+                        double *d_x, PcgResult &res, bool timing) {
     axg_ctx *ctx = bp.m->ctx;
     hipStream_t s = ctx->stream;
     const int slot = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
-    if (!bp.graph[slot]) {
+    if (!timing && !bp.graph[slot]) {
         hipGraph_t g;
         AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < kGraphIters; i++) block_iter<K>(bp, s);
         AXG_HIP(hipStreamEndCapture(s, &g));
         AXG_HIP(hipGraphInstantiate(&bp.graph[slot], g, nullptr, nullptr, 0));
         AXG_HIP(hipGraphDestroy(g));
+        bp.graph_bv = bp.m->bval64.p;
     }
+    if (timing)
+        while (ctx->timing_events.size() < 2 * (size_t)kGraphIters) {
+            hipEvent_t e;
+            AXG_HIP(hipEventCreate(&e));
+            ctx->timing_events.push_back(e);
+        }
     double h_tol[8];
     for (int j = 0; j < K; j++) h_tol[j] = j < kk ? tol_rel[j] : 1.0;
     std::vector<SegScalars> hs(K);
@@ -896,12 +927,30 @@ static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *t
                        bp.Z.p, bp.sc.p, 1);
     AXG_HIP(hipGetLastError());
     int32_t live = live0;
-    int64_t launched = 0;
+    int64_t launched = 0, done_iters = 0;
     for (;;) {
         AXG_HIP(hipMemcpyAsync(&live, bp.live_total.p, 4, hipMemcpyDeviceToHost, s));
         AXG_HIP(hipStreamSynchronize(s));
         if (live <= 0 || launched >= bp.max_iter) break;
-        AXG_HIP(hipGraphLaunch(bp.graph[slot], s));
+        if (timing) {
+            for (int i = 0; i < kGraphIters; i++)
+                block_iter<K>(bp, s, ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1]);
+            AXG_HIP(hipMemcpyAsync(hs.data(), bp.sc.p, sizeof(SegScalars) * K,
+                                   hipMemcpyDeviceToHost, s));
+            AXG_HIP(hipStreamSynchronize(s));
+            int64_t most = 0;
+            for (int j = 0; j < kk; j++) most = std::max<int64_t>(most, hs[j].iters);
+            for (int64_t i = 0; i < most - done_iters && i < kGraphIters; i++) {
+                float ms = 0.f;
+                AXG_HIP(hipEventElapsedTime(&ms, ctx->timing_events[2 * i],
+                                            ctx->timing_events[2 * i + 1]));
+                res.spmv_ms += ms;
+                res.spmv_launches++;
+            }
+            done_iters = most;
+        } else {
+            AXG_HIP(hipGraphLaunch(bp.graph[slot], s));
+        }
         launched += kGraphIters;
     }
     AXG_HIP(hipEventRecord(ctx->ev1, s));
@@ -916,8 +965,12 @@ static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *t
     for (int j = 0; j < kk; j++) {
         const SegScalars &r = hs[j];
         res.iters += r.iters;
-        if (!r.done && res.status == AXG_OK) res.status = AXG_E_NOT_CONVERGED;
-        if (r.status != AXG_OK && res.status == AXG_OK) res.status = r.status;
+        const int st = r.status != AXG_OK ? r.status : r.done ? AXG_OK : AXG_E_NOT_CONVERGED;
+        if (st != AXG_OK && res.status == AXG_OK) {
+            res.status = st;
+            res.fail_iters = r.iters;
+            res.fail_residual = r.res;
+        }
         if (r.bnorm > 0 && r.res / r.bnorm >= res.residual / std::max(res.bnorm, 1e-300))
             res.residual = r.res, res.bnorm = r.bnorm;
     }
@@ -945,18 +998,31 @@ void bench_block_spmm(axg_mesh *m, int nrhs, int reps, double *avg_ms) {
                           : time_block_spmm<1>(*bp, reps);
 }
 
+void block_pcg_refresh(BlockPcg *bp, int64_t max_iter) {
+    axg_mesh *m = bp->m;
+    hipStream_t s = m->ctx->stream;
+    ensure_bsell64(m);
+    const int64_t mi = max_iter > 0 ? max_iter : std::max<int64_t>(4 * bp->n, 200);
+    // the graphs hold max_iter and the operator's address as kernel arguments
+    if (mi != bp->max_iter || m->bval64.p != bp->graph_bv) bp->drop_graphs();
+    bp->max_iter = mi;
+    hipLaunchKernelGGL(block_dsel_kernel, dim3((unsigned)((bp->npad + 255) / 256)), dim3(256), 0, s,
+                       bp->npad, bp->rows, m->free_of_blk.p, m->dsel.p, bp->dsel.p);
+    AXG_HIP(hipGetLastError());
+}
+
 void block_pcg_solve(BlockPcg *bp, const double *d_b, int nrhs, const double *tol_rel,
-                     double *d_x, PcgResult &res) {
+                     double *d_x, PcgResult &res, bool timing) {
     int j0 = 0;
     while (j0 < nrhs) {
         const int left = nrhs - j0;
         const int kk = left >= 8 ? 8 : left;
         const double *b = d_b + (int64_t)j0 * bp->n;
         double *x = d_x + (int64_t)j0 * bp->n;
-        if (kk > 4) block_group<8>(*bp, b, kk, tol_rel + j0, x, res);
-        else if (kk > 2) block_group<4>(*bp, b, kk, tol_rel + j0, x, res);
-        else if (kk == 2) block_group<2>(*bp, b, kk, tol_rel + j0, x, res);
-        else block_group<1>(*bp, b, kk, tol_rel + j0, x, res);
+        if (kk > 4) block_group<8>(*bp, b, kk, tol_rel + j0, x, res, timing);
+        else if (kk > 2) block_group<4>(*bp, b, kk, tol_rel + j0, x, res, timing);
+        else if (kk == 2) block_group<2>(*bp, b, kk, tol_rel + j0, x, res, timing);
+        else block_group<1>(*bp, b, kk, tol_rel + j0, x, res, timing);
         j0 += kk;
     }
 }

@@ -144,16 +144,16 @@ struct ColumnSolver {
     axg_mesh *m;
     axg_solve_opts o;
     double *P, *D;
-    BlockPcg *bp = nullptr;
+    BlockPcg *bp = nullptr;  // the mesh's own block PCG (it outlives this solver)
     int64_t iters = 0;
     double ms = 0.0, res = 0.0, rel = 0.0;
     ColumnSolver(axg_mesh *mesh, const axg_solve_opts &opts, double *pack, double *sol)
         : m(mesh), o(opts), P(pack), D(sol) {
-        if (o.precision != AXG_PREC_MIXED && o.precond == AXG_PRECOND_JACOBI)
-            bp = block_pcg_create(m, o.max_iter);
-    }
-    ~ColumnSolver() {
-        if (bp) block_pcg_destroy(bp);
+        if (o.precision != AXG_PREC_MIXED && o.precond == AXG_PRECOND_JACOBI) {
+            if (!m->bpcg) m->bpcg = block_pcg_create(m, o.max_iter);
+            block_pcg_refresh(m->bpcg, o.max_iter);
+            bp = m->bpcg;
+        }
     }
     ColumnSolver(const ColumnSolver &) = delete;
     ColumnSolver &operator=(const ColumnSolver &) = delete;

@@ -10,11 +10,37 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/axia_gpu.h"
 
 namespace axg {
+
+// ------------------------------------------------------- host threading
+// Threads for n items in chunks of at least min_chunk: at most 16 (and the
+// hardware's count), at least 1.
+inline int host_threads(int64_t n, int64_t min_chunk) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t cap = hw == 0 ? 1 : (hw < 16 ? (int64_t)hw : 16);
+    const int64_t by_size = n / (min_chunk > 0 ? min_chunk : 1);
+    return (int)(by_size < 1 ? 1 : (by_size < cap ? by_size : cap));
+}
+
+// fn(t, lo, hi) over nt contiguous chunks of [0, n), chunk 0 on the calling
+// thread. fn must not throw (report failures through per-chunk flags).
+template <class F>
+inline void parallel_chunks(int64_t n, int nt, F &&fn) {
+    if (nt <= 1) {
+        fn(0, (int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (int t = 1; t < nt; t++) pool.emplace_back([&fn, n, nt, t] { fn(t, n * t / nt, n * (t + 1) / nt); });
+    fn(0, (int64_t)0, n / nt);
+    for (auto &th : pool) th.join();
+}
 
 // ---------------------------------------------------------------- errors
 void set_error(const std::string &msg);
@@ -160,6 +186,10 @@ struct axg_ctx {
     std::string arch;
 };
 
+namespace axg {
+struct BlockPcg;
+}
+
 struct axg_mesh {
     axg_ctx *ctx = nullptr;
     int nn = 8, m = 24;
@@ -220,6 +250,9 @@ struct axg_mesh {
     // the multi-right-hand-side FP64 block PCG; refreshed after every assembly
     bool bsell64_fresh = false;
     axg::DevBuf<double> bval64;
+    // the FP64 Jacobi block PCG of this mesh (created at the first such solve,
+    // destroyed with the mesh)
+    axg::BlockPcg *bpcg = nullptr;
     // host mirror of the active pattern (kept for export)
     std::vector<int32_t> h_rowptr, h_colind;
 };

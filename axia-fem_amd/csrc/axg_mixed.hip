@@ -141,23 +141,35 @@ static void build_bsr(axg_mesh *m) {
     AXG_HIP(hipMemcpyAsync(fob.data(), m->free_of_blk.p, 4 * 3 * m->nb, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
     const int64_t nb = m->nb;
-    std::vector<int32_t> brp(nb + 1, 0), bci;
-    bci.reserve(nnz / 3 + 16);
     // block columns of block row I = block rows of the columns of I's first free row
-    // (all free rows of a node share the same node neighbourhood)
-    std::vector<int32_t> row;
-    for (int64_t I = 0; I < nb; I++) {
-        int32_t r = -1;
-        for (int d = 0; d < 3 && r < 0; d++) r = fob[3 * I + d];
-        row.clear();
-        for (int32_t k = rp[r]; k < rp[r + 1]; k++) row.push_back(bof[ci[k]] / 3);
-        std::sort(row.begin(), row.end());  // any free-DOF numbering
-        row.erase(std::unique(row.begin(), row.end()), row.end());
-        AXG_REQUIRE(std::binary_search(row.begin(), row.end(), (int32_t)I), AXG_E_INVALID,
-                    "block row without its diagonal block");
-        bci.insert(bci.end(), row.begin(), row.end());
-        brp[I + 1] = (int32_t)bci.size();
-    }
+    // (all free rows of a node share the same node neighbourhood); rows are split over
+    // host threads, each chunk building its rows' lists, then joined in row order
+    const int nt = host_threads(nb, 4096);
+    std::vector<std::vector<int32_t>> part_ci(nt);
+    std::vector<int32_t> brp(nb + 1, 0);
+    std::vector<int> bad(nt, 0);
+    parallel_chunks(nb, nt, [&](int t, int64_t lo, int64_t hi) {
+        std::vector<int32_t> row;
+        auto &out = part_ci[t];
+        for (int64_t I = lo; I < hi; I++) {
+            int32_t r = -1;
+            for (int d = 0; d < 3 && r < 0; d++) r = fob[3 * I + d];
+            row.clear();
+            for (int32_t k = rp[r]; k < rp[r + 1]; k++) row.push_back(bof[ci[k]] / 3);
+            if (!std::is_sorted(row.begin(), row.end()))
+                std::sort(row.begin(), row.end());  // any free-DOF numbering
+            row.erase(std::unique(row.begin(), row.end()), row.end());
+            if (!std::binary_search(row.begin(), row.end(), (int32_t)I)) bad[t] = 1;
+            out.insert(out.end(), row.begin(), row.end());
+            brp[I + 1] = (int32_t)row.size();
+        }
+    });
+    for (int t = 0; t < nt; t++)
+        AXG_REQUIRE(!bad[t], AXG_E_INVALID, "block row without its diagonal block");
+    for (int64_t I = 0; I < nb; I++) brp[I + 1] += brp[I];
+    std::vector<int32_t> bci;
+    bci.reserve(brp[nb]);
+    for (auto &v : part_ci) bci.insert(bci.end(), v.begin(), v.end()), std::vector<int32_t>().swap(v);
     const int64_t nnzb = (int64_t)bci.size();
     // block-sliced-ELL layout: 64 block rows per slice; row I's diagonal block
     // first, then its other blocks by ascending column, position k of row I at
@@ -175,22 +187,37 @@ static void build_bsr(axg_mesh *m) {
     const int64_t nsl = L.nslots;
     AXG_REQUIRE(10 * nsl < INT32_MAX, AXG_E_INVALID, "FP32 operator exceeds 2^31 values");
     std::vector<int32_t> bcol(nsl, 0);
-    for (int64_t I = 0; I < nb; I++) {
-        const int64_t base = L.sptr[I >> 6] + (I & 63);
-        for (int32_t k = brp[I]; k < brp[I + 1]; k++) bcol[base + 64 * pos_in_row(I, bci[k])] = bci[k];
-        for (int64_t k = blen[I]; k < L.width[I >> 6]; k++) bcol[base + 64 * k] = (int32_t)I;
-    }
-    std::vector<int32_t> eos(nnz);
-    for (int64_t i = 0; i < n; i++) {
-        int32_t I = bof[i] / 3, d = bof[i] % 3;
-        const int64_t base = L.sptr[I >> 6] + (I & 63);
-        for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
-            int32_t J = bof[ci[k]] / 3, e = bof[ci[k]] % 3;
-            int64_t pos = base + 64 * pos_in_row(I, J);
-            int q = 3 * d + e;  // planes: entries 0-3, 4-7 as float4; entry 8 beside the column
-            eos[k] = (int32_t)(q < 4 ? 4 * pos + q : q < 8 ? 4 * nsl + 4 * pos + (q - 4) : 8 * nsl + 2 * pos);
+    parallel_chunks(nb, nt, [&](int, int64_t lo, int64_t hi) {
+        for (int64_t I = lo; I < hi; I++) {
+            const int64_t base = L.sptr[I >> 6] + (I & 63);
+            for (int32_t k = brp[I]; k < brp[I + 1]; k++)
+                bcol[base + 64 * pos_in_row(I, bci[k])] = bci[k];
+            for (int64_t k = blen[I]; k < L.width[I >> 6]; k++) bcol[base + 64 * k] = (int32_t)I;
         }
-    }
+    });
+    // plane position of every scalar slot: the block columns of a row are met in
+    // ascending order when the free numbering follows the nodes, so a cursor
+    // walks the row's block list (a search from the start when a column goes back)
+    std::vector<int32_t> eos(nnz);
+    parallel_chunks(n, host_threads(n, 4096), [&](int, int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) {
+            const int32_t I = bof[i] / 3, d = bof[i] % 3;
+            const int64_t base = L.sptr[I >> 6] + (I & 63);
+            const int32_t *b0 = bci.data() + brp[I], *b1 = bci.data() + brp[I + 1];
+            const int64_t kd = std::lower_bound(b0, b1, I) - b0;
+            const int32_t *q = b0;
+            for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
+                const int32_t J = bof[ci[k]] / 3, e = bof[ci[k]] % 3;
+                if (q == b1 || *q > J) q = b0;
+                q = std::lower_bound(q, b1, J);
+                const int64_t kb = q - b0;
+                const int64_t pos = base + 64 * (J == I ? 0 : (kb < kd ? kb + 1 : kb));
+                const int qq = 3 * d + e;  // planes: entries 0-3, 4-7 as float4; entry 8 beside the column
+                eos[k] = (int32_t)(qq < 4 ? 4 * pos + qq
+                                   : qq < 8 ? 4 * nsl + 4 * pos + (qq - 4) : 8 * nsl + 2 * pos);
+            }
+        }
+    });
     m->nnzb = nnzb;
     m->bs_nslots = nsl;
     m->bs_sptr.alloc(L.sptr.size());

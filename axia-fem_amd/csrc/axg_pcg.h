@@ -44,6 +44,9 @@ struct PcgResult {
     double solve_ms = 0.0, spmv_ms = 0.0;
     int64_t spmv_launches = 0;
     int spmv_shape_L = 0, spmv_shape_U = 0;
+    // multi-right-hand-side solves: the first failing column's count and residual
+    int64_t fail_iters = -1;
+    double fail_residual = 0.0;
 };
 
 void spmv_fp64(axg_ctx *ctx, const CsrView &A, const double *x, double *y);
@@ -51,18 +54,22 @@ void diag_fp64(axg_ctx *ctx, const CsrView &A, double *dsel, double *diag_raw);
 void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b, double *x,
               double tol, int64_t max_iter, bool timing, PcgResult &res);
 
-// Multi-right-hand-side FP64 Jacobi PCG of one assembled mesh on device
-// vectors (the batched solver's kernels on the mesh's own sliced-ELL operator,
-// 8, 4, 2 or 1 right-hand sides per pass, one graph per width, reused across
-// calls). Column j of d_b (n_free each) is solved to ||r|| <= tol_rel[j] ||b_j||.
 // The reference's CG error text for a failed solve (linalg.rs:231-257).
 std::string pcg_error(const PcgResult &r);
 
+// Multi-right-hand-side FP64 Jacobi PCG of one assembled mesh on device
+// vectors, in node-block space on the FP64 block-sliced-ELL operator (the
+// batched solver's vector kernels, 8, 4, 2 or 1 right-hand sides per pass, one
+// graph per width, reused across calls). Column j of d_b (n_free each) is
+// solved to ||r|| <= tol_rel[j] ||b_j||. block_pcg_refresh re-reads the
+// mesh's operator and Jacobi divisor after an assembly and sets max_iter
+// (0: max(4 n, 200)); with timing every SpMM is timed by events.
 struct BlockPcg;
 BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter);
 void block_pcg_destroy(BlockPcg *bp);
+void block_pcg_refresh(BlockPcg *bp, int64_t max_iter);
 void block_pcg_solve(BlockPcg *bp, const double *d_b, int nrhs, const double *tol_rel,
-                     double *d_x, PcgResult &res);
+                     double *d_x, PcgResult &res, bool timing = false);
 // Average device time of one block-PCG SpMM launch (nrhs = 1, 2, 4 or 8).
 void bench_block_spmm(axg_mesh *m, int nrhs, int reps, double *avg_ms);
 

@@ -272,12 +272,16 @@ def run_brick(a, G, MG, torch, np):
         torch.cuda.synchronize()
         wall64 = time.perf_counter() - t1
         fp64 = {
+            "solver": r64["name"],
             "iters_total": r64["iters"], "solve_ms": r64["solve_ms"], "wall_ms": 1e3 * wall64,
             "dof_iter_per_s": n * r64["iters"] / (r64["solve_ms"] / 1e3),
             "spmv_ms_avg": avg_launch(r64),
             "rel_err_vs_ref": rel_err(X64), "rel_residual": r64["rel_residual"],
         }
-        fp64["spmv_gbs"] = sell64_bytes(n, nnz) / (fp64["spmv_ms_avg"] / 1e3) / 1e9
+        # the node-block operator takes all k right-hand sides in one pass
+        op_bytes = (bsell64_bytes(n // 3, nnz // 9, k) if "node blocks" in r64["name"]
+                    else sell64_bytes(n, nnz))
+        fp64["spmv_gbs"] = op_bytes / (fp64["spmv_ms_avg"] / 1e3) / 1e9
         fp64["spmv_frac"] = fp64["spmv_gbs"] / HBM_PEAK_GBS
         mesh.assemble("mixed")
         log(f"fp64: {fp64['solve_ms']:.0f} ms, iters {r64['iters']}, err {fp64['rel_err_vs_ref']:.2e}")
@@ -353,8 +357,11 @@ def run_cantilever(a, G, MG, torch, np):
         "fp64": {"iters": iters, "solve_ms": solve_ms, "assembly_ms": asm_ms,
                  "elements_per_s": prob.nelem / (asm_ms / 1e3),
                  "rel_residual": res[-1][1]["rel_residual"]},
-        "roofline": roofline("sell_spmv_dot_kernel (FP64 sliced-ELL SpMV)", sell64_bytes(n, nnz),
-                             spmv),
+        "roofline": (roofline("bsell64_spmm_kernel<K=1> (FP64 node-block sliced-ELL SpMV)",
+                              bsell64_bytes(n // 3, nnz // 9, 1), spmv)
+                     if "node blocks" in res[-1][1]["name"] else
+                     roofline("sell_spmv_dot_kernel (FP64 sliced-ELL SpMV)", sell64_bytes(n, nnz),
+                              spmv)),
     })
     if not a.no_cpu:
         from oracle import oracle as O
@@ -640,10 +647,10 @@ def run_nlgeom(a, G, MG, torch, np):
     n = dk.n_free
     log(f"nlgeom: {dk.nelem} elements, n={n}")
     for _ in range(a.warmup):
-        dk.nlgeom(tol=a.nl_tol)
+        dk.nlgeom(tol=a.nl_tol, precond=a.nl_precond)
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        res = dk.nlgeom(tol=a.nl_tol)
+        res = dk.nlgeom(tol=a.nl_tol, precond=a.nl_precond)
     ms_per_step = 1e3 * (time.perf_counter() - t0) / a.steps
     log(f"nlgeom: {ms_per_step:.0f} ms per analysis, {res.ninc} increments, {res.pcg_iters} PCG "
         f"iterations in {res.solve_ms:.0f} ms, {res.solver}")
@@ -656,7 +663,8 @@ def run_nlgeom(a, G, MG, torch, np):
         m.nl_assemble(res.u_full)
     asm_ms = 1e3 * (time.perf_counter() - t1) / reps
     nnz = m.info()["nnz"]
-    spmv_ms = m.bench_operator(0, 1, a.reps)
+    block_op = "node blocks" in res.solver  # every free node keeps its 3 DOFs here
+    spmv_ms = m.bench_operator(3 if block_op else 0, 1, a.reps)
     xyz = arr["xyz"]
     tip = np.nonzero(xyz[:, 0] == 10.0)[0]
     fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
@@ -669,9 +677,9 @@ def run_nlgeom(a, G, MG, torch, np):
     write_mb = (len(frd) + len(dat)) / 1e6
     out = base_line(a, value, ms_per_step, "f64", {
         "workload": f"*STEP, NLGEOM C3D8 cantilever {nx}x{ny}x{nz} (10 x 1 x 1, n={n}), tip "
-                    f"load 0.5, 4 increments, Newton with the device tangent and block-Jacobi "
-                    f"PCG to {a.nl_tol:g}",
-        "n_free": n, "nnz": nnz, "elements": dk.nelem, "precond": "block3"})
+                    f"load 0.5, 4 increments, Newton with the device tangent and "
+                    f"{'Jacobi' if a.nl_precond == 'jacobi' else 'block-Jacobi'} PCG to {a.nl_tol:g}",
+        "n_free": n, "nnz": nnz, "elements": dk.nelem, "precond": a.nl_precond})
     asm_eps = dk.nelem / (asm_ms / 1e3)
     out.update({
         "nlgeom": {"time_per_analysis_ms": ms_per_step, "increments": res.ninc,
@@ -683,8 +691,10 @@ def run_nlgeom(a, G, MG, torch, np):
                    "tangent_fp64_tflops": nl_flops(8, 8) * asm_eps / 1e12,
                    "frd_dat_write_ms": write_ms, "frd_dat_mb": write_mb,
                    "solver": res.solver},
-        "roofline": roofline("sell_spmv_kernel (FP64 SpMV of the tangent in the PCG)",
-                             sell64_bytes(n, nnz), spmv_ms),
+        "roofline": (roofline("bsell64_spmm_kernel<K=1> (FP64 node-block SpMV of the tangent)",
+                              bsell64_bytes(n // 3, nnz // 9, 1), spmv_ms) if block_op else
+                     roofline("sell_spmv_kernel (FP64 SpMV of the tangent in the PCG)",
+                              sell64_bytes(n, nnz), spmv_ms)),
     })
     if not a.no_cpu:
         from oracle import oracle as O
@@ -745,7 +755,7 @@ def run_simp(a, G, MG, torch, np):
     def forward(e):
         mesh.set_modulus(e)
         ms_asm = mesh.assemble("fp64")
-        x, inf = mesh.solve(prob.rhs(), precision="fp64", precond="block3", tol=tol)
+        x, inf = mesh.solve(prob.rhs(), precision="fp64", precond="jacobi", tol=tol)
         return ms_asm, prob.expand(x), inf
 
     def step():
@@ -802,7 +812,7 @@ def run_simp(a, G, MG, torch, np):
     out = base_line(a, n * iters / (solve_ms / 1e3), ms_per_step, "f64", {
         "workload": "SIMP C3D8 brick 120x60x60, E = 1e-3 + rho^3, rho ~ U(0.3, 1) (seed 0), "
                     "left face clamped, unit point load; compliance and dc/drho via the adjoint",
-        "elements": prob.nelem, "n_free": n, "nnz": nnz, "precond": "block3", "tol": tol})
+        "elements": prob.nelem, "n_free": n, "nnz": nnz, "precond": "jacobi", "tol": tol})
     out.update({
         "compliance": c0,
         "forward": {"assembly_ms": ms_asm, "solve_ms": solve_ms, "iters": iters},
@@ -850,6 +860,7 @@ def main():
     ap.add_argument("--eig-tol", type=float, default=1e-10)
     ap.add_argument("--nl-shape", type=int, nargs=3, default=[200, 20, 20])
     ap.add_argument("--nl-tol", type=float, default=1e-10)
+    ap.add_argument("--nl-precond", default="jacobi", choices=("jacobi", "block3"))
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "

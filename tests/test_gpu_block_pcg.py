@@ -1,0 +1,97 @@
+"""The mesh's FP64 Jacobi PCG on the node-block operator (linalg::pcg, linalg.rs:194-259, every
+right-hand side at once) against the CSR Jacobi PCG of the solve_kff drop-in on the same K_ff,
+the oracle's direct solve (golden x), and itself across right-hand-side widths, re-assembly and
+iteration limits."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import axia_gpu as G
+from axia_gpu import meshgen as MG
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def csr_solve(mesh, rhs, ctx, tol):
+    """The same K_ff through axg_solve_kff (triplets -> CSR -> CSR Jacobi PCG)."""
+    ip, ix, dv = mesh.csr()
+    n = len(ip) - 1
+    rows = np.repeat(np.arange(n), np.diff(ip))
+    return G.solve_kff(n, (rows, ix, dv), rhs, ctx=ctx, tol=tol)
+
+
+def test_matches_the_csr_pcg_and_the_golden(ctx):
+    say("case: node-block vs CSR Jacobi PCG, cantilever 20x4x4")
+    g = np.load(os.path.join(GOLD, "cantilever_20x4x4.npz"))
+    p = MG.cantilever(20, 4, 4)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble()
+    x, info = mesh.solve(g["rhs"], tol=1e-10)
+    assert "node blocks" in info["name"]
+    ref = csr_solve(mesh, g["rhs"], ctx, 1e-10)
+    say("  iters", info["iters"], ref.iters, "rel vs CSR", rel(x, ref.x), "vs golden", rel(x, g["x"]))
+    # the same iteration in exact arithmetic: counts within the rounding of the stop test
+    assert abs(info["iters"] - ref.iters) <= 2
+    assert rel(x, ref.x) < 1e-8
+    assert rel(x, g["x"]) < 1e-8
+    assert info["rel_residual"] < 1e-9
+
+
+@pytest.mark.parametrize("nrhs", [2, 3, 6, 11])
+def test_every_width_matches_single_solves(ctx, nrhs):
+    """nrhs columns go through the 8 / 4 / 2 / 1 wide kernels with masked padding columns;
+    each column is the single right-hand-side solve."""
+    say("case: widths", nrhs)
+    p = MG.brick_lognormal(10, seed=4)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble()
+    rng = np.random.default_rng(nrhs)
+    B = rng.standard_normal((nrhs, mesh.n_free))
+    B[1] = 0.0  # a zero column: x = 0 without iterations (linalg.rs:206-214)
+    X, info = mesh.solve(B, tol=1e-10)
+    total = 0
+    for j in range(nrhs):
+        xj, ij = mesh.solve(B[j], tol=1e-10)
+        total += ij["iters"]
+        if j == 1:
+            assert np.all(X[j] == 0.0) and ij["iters"] == 0
+        else:
+            assert rel(X[j], xj) < 1e-8, j
+    assert abs(info["iters"] - total) <= 2 * nrhs
+
+
+def test_refreshes_after_reassembly_and_limits(ctx):
+    """The cached solver re-reads the operator after an NLGEOM re-assembly (new values, same
+    pattern) and re-captures its graphs when max_iter changes."""
+    say("case: refresh")
+    xyz, conn = MG.brick_hex8(6, 3, 3)
+    ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 0] == 0.0)[0])
+    mesh = G.Mesh(8, xyz, conn, np.full(len(conn), 1000.0), np.full(len(conn), 0.3), ind, ctx=ctx)
+    mesh.assemble()
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(mesh.n_free)
+    x0, _ = mesh.solve(b, tol=1e-11)
+    u = rng.uniform(-0.05, 0.05, 3 * len(xyz))
+    u[ind < 0] = 0.0
+    mesh.nl_assemble(u)  # K_T replaces K
+    x1, i1 = mesh.solve(b, tol=1e-11)
+    ref = csr_solve(mesh, b, ctx, 1e-11)
+    say("  rel vs CSR after re-assembly", rel(x1, ref.x), "change", rel(x1, x0))
+    assert rel(x1, ref.x) < 1e-8 and rel(x1, x0) > 1e-3
+    with pytest.raises(G.AxgError) as ei:
+        mesh.solve(b, tol=1e-11, max_iter=3)
+    assert ei.value.code == 6 and "nach 3 Iterationen" in str(ei.value)
+    x2, i2 = mesh.solve(b, tol=1e-11)
+    assert np.array_equal(x2, x1) and i2["iters"] == i1["iters"]
