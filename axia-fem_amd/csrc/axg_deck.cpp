@@ -1227,7 +1227,7 @@ int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
         info->residual = residual;
         info->rel_residual = residual / std::max(norm_seq(f_now), 1e-300);
         info->solve_ms = pcg_ms;
-        std::snprintf(info->name, sizeof(info->name), "Newton (%.32s, %d inc)",
+        std::snprintf(info->name, sizeof(info->name), "Newton (%.44s, %d inc)",
                       solver_name.empty() ? "equilibrium" : solver_name.c_str(), done_inc);
     }
     API_END
