@@ -723,6 +723,7 @@ struct BlockPcg {
     int64_t n = 0, nb = 0, rows = 0, npad = 0, max_iter = 0;
     int nwg = 0, nsl_wg = 0;
     int U[4] = {2, 2, 2, 2};  // blocks in flight per lane, tuned per K = 1, 2, 4, 8
+    int L[4] = {1, 1, 1, 2};  // lanes per block row, tuned per K = 2, 4, 8
     DevBuf<int64_t> wg_row, seg_end;
     DevBuf<int32_t> wg_seg, segwg, segwg_mv, seg_live, live_total;
     DevBuf<double> dsel, Bv, X, R, P, AP, Z, part, tolr;
@@ -740,11 +741,10 @@ struct BlockPcg {
 static constexpr int kGraphIters = 32;
 
 // With events, the launch itself stamps e0 / e1 (the SpMM's device time).
-template <int K>
-static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x,
-                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+template <int K, int LPR>
+static void launch_bsell64(BlockPcg &bp, hipStream_t s, bool dot, int u, const double *x,
+                           hipEvent_t e0, hipEvent_t e1) {
     axg_mesh *m = bp.m;
-    constexpr int LPR = K == 8 ? 2 : 1;  // 8 right-hand sides: two lanes of 4 per block row
     const dim3 g(bp.nsl_wg), b(64 * LPR);
 #define AXG_BSELL64(UU, DD)                                                                         \
     if (e0)                                                                                         \
@@ -756,7 +756,6 @@ static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x,
         hipLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, bp.nb, m->bs_sptr.p, \
                            m->bs_width.p, m->bval64.p, m->bs_nslots, x, bp.AP.p, bp.part.p,        \
                            bp.seg_live.p)
-    const int u = bp.U[K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3];
     if (dot) {
         if (u == 1) {
             AXG_BSELL64(1, true);
@@ -777,6 +776,25 @@ static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x,
 #undef AXG_BSELL64
 }
 
+// Lanes per block row: one for a single right-hand side; 1 or 2 for 2 and 4
+// (two lanes halve each lane's x gathers and registers, the block's values are
+// loaded by both from the same lines); 2 or 4 for 8.
+template <int K>
+static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const int slot = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
+    const int u = bp.U[slot], l = bp.L[slot];
+    if constexpr (K == 1) {
+        launch_bsell64<1, 1>(bp, s, dot, u, x, e0, e1);
+    } else if constexpr (K == 8) {
+        if (l == 4) launch_bsell64<8, 4>(bp, s, dot, u, x, e0, e1);
+        else launch_bsell64<8, 2>(bp, s, dot, u, x, e0, e1);
+    } else {
+        if (l == 2) launch_bsell64<K, 2>(bp, s, dot, u, x, e0, e1);
+        else launch_bsell64<K, 1>(bp, s, dot, u, x, e0, e1);
+    }
+}
+
 template <int K>
 static void block_iter(BlockPcg &bp, hipStream_t s, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
@@ -791,24 +809,32 @@ static void block_iter(BlockPcg &bp, hipStream_t s, hipEvent_t e0 = nullptr,
                        bp.sc.p, 0);
 }
 
+// Lanes per block row and blocks in flight per lane, timed on this operator.
 template <int K>
 static void tune_u(BlockPcg &bp, int slot) {
     axg_ctx *ctx = bp.m->ctx;
     hipStream_t s = ctx->stream;
+    std::vector<int> lanes;
+    if (K == 1) lanes = {1};
+    else if (K == 8) lanes = {2, 4};
+    else lanes = {1, 2};
     float best = 1e30f;
-    int best_u = 2;
-    for (int cand : {1, 2, 4}) {
-        bp.U[slot] = cand;
-        block_spmv<K>(bp, s, false, bp.Bv.p);
-        AXG_HIP(hipEventRecord(ctx->ev0, s));
-        for (int r = 0; r < 3; r++) block_spmv<K>(bp, s, false, bp.Bv.p);
-        AXG_HIP(hipEventRecord(ctx->ev1, s));
-        AXG_HIP(hipEventSynchronize(ctx->ev1));
-        float ms = 0.f;
-        AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        if (ms < best) best = ms, best_u = cand;
-    }
+    int best_u = 2, best_l = lanes[0];
+    for (int l : lanes)
+        for (int cand : {1, 2, 4}) {
+            bp.U[slot] = cand;
+            bp.L[slot] = l;
+            block_spmv<K>(bp, s, false, bp.Bv.p);
+            AXG_HIP(hipEventRecord(ctx->ev0, s));
+            for (int r = 0; r < 3; r++) block_spmv<K>(bp, s, false, bp.Bv.p);
+            AXG_HIP(hipEventRecord(ctx->ev1, s));
+            AXG_HIP(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) best = ms, best_u = cand, best_l = l;
+        }
     bp.U[slot] = best_u;
+    bp.L[slot] = best_l;
 }
 
 BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter) {
