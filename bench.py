@@ -513,8 +513,9 @@ def run_batch(a, G, MG, torch, np):
         _, inf = m.solve(b, precision="fp64", tol=tol)
         seq_ms += inf["solve_ms"]
     seq_wall = 1e3 * (time.perf_counter() - t2)
-    # concatenated operator, 4 right-hand sides per row (row-major)
-    nbytes = int(12 * nnzs.sum() + 2 * 8 * 4 * ns.sum())
+    # concatenated node-block operator, 4 right-hand sides per row (every free node of these
+    # bottom-clamped cubes keeps its 3 DOFs: n / 3 block rows, nnz / 9 blocks)
+    nbytes = int(sum(bsell64_bytes(n // 3, z // 9, 4) for n, z in zip(ns, nnzs)))
     out = base_line(a, dof_iters / (solve_ms / 1e3), ms_per_step, "f64", {
         "workload": "48 C3D8 cubes m^3, m ~ U{8..48} (seed 0), E ~ U(5e4, 2.5e5), 4 Gaussian "
                     f"top-face load cases of 1 kN each, one batched Jacobi PCG to {tol:g}",
@@ -529,7 +530,7 @@ def run_batch(a, G, MG, torch, np):
                   "rel_err_vs_fp64_ref_max": err, "rel_residual_max": info["rel_residual"]},
         "sequential": {"solve_ms_sum": seq_ms, "wall_ms": seq_wall,
                        "speedup_batch_vs_sequential": seq_ms / solve_ms},
-        "roofline": roofline("batch_spmv_kernel<K=4> (FP64 concatenated sliced-ELL SpMM)",
+        "roofline": roofline("bsell64_spmm_kernel<K=4> (FP64 concatenated node-block SpMM)",
                              nbytes, avg_launch(info)),
     })
     if not a.no_cpu:

@@ -95,3 +95,44 @@ def test_refreshes_after_reassembly_and_limits(ctx):
     assert ei.value.code == 6 and "nach 3 Iterationen" in str(ei.value)
     x2, i2 = mesh.solve(b, tol=1e-11)
     assert np.array_equal(x2, x1) and i2["iters"] == i1["iters"]
+
+
+def _partly_fixed(nx, ny, nz, seed):
+    """A brick whose bottom nodes keep some DOFs: z fixed on the whole bottom face, x on its
+    x = 0 edge, y at one corner, so block rows hold 1, 2 or 3 free DOFs."""
+    xyz, conn = MG.brick_hex8(nx, ny, nz)
+    ind = np.zeros(3 * len(xyz), np.int64)
+    bot = xyz[:, 2] == 0.0
+    ind[3 * np.nonzero(bot)[0] + 2] = -1
+    ind[3 * np.nonzero(bot & (xyz[:, 0] == 0.0))[0]] = -1
+    ind[3 * np.nonzero(bot & (xyz[:, 0] == 0.0) & (xyz[:, 1] == 0.0))[0] + 1] = -1
+    free = ind >= 0
+    ind[free] = np.arange(int(free.sum()))
+    rng = np.random.default_rng(seed)
+    e = rng.uniform(800.0, 1200.0, len(conn))
+    return xyz, conn, e, ind
+
+
+def test_batch_and_block_solves_with_partly_fixed_nodes(ctx):
+    """Block rows with prescribed DOFs (zero rows and columns, divisor 1) inside the batch's
+    concatenated node-block operator and the single-mesh block PCG, against the CSR PCG."""
+    say("case: partly fixed nodes")
+    meshes, rhs = [], []
+    for k, (shape, seed) in enumerate((((5, 4, 3), 1), ((3, 3, 6), 2), ((7, 2, 2), 3))):
+        xyz, conn, e, ind = _partly_fixed(*shape, seed)
+        me = G.Mesh(8, xyz, conn, e, np.full(len(conn), 0.3), ind, ctx=ctx)
+        me.assemble("fp64")
+        meshes.append(me)
+        rng = np.random.default_rng(10 + k)
+        rhs.append(rng.standard_normal((k + 1, me.n_free)))
+    xs, iters, lat, info = G.solve_batch(meshes, rhs, tol=1e-11)
+    assert "node blocks" in info["name"]
+    q = 0
+    for me, B, X in zip(meshes, rhs, xs):
+        for k in range(B.shape[0]):
+            ref = csr_solve(me, B[k], ctx, 1e-11)
+            x1, i1 = me.solve(B[k], tol=1e-11)
+            say("  mesh", me.n_free, "rhs", k, "iters", int(iters[q]), i1["iters"], ref.iters)
+            assert rel(X[k], ref.x) < 1e-8 and rel(x1, ref.x) < 1e-8
+            assert abs(int(iters[q]) - ref.iters) <= 2 and abs(i1["iters"] - ref.iters) <= 2
+            q += 1
