@@ -11,6 +11,7 @@
 #include <array>
 #include <functional>
 #include <cctype>
+#include <charconv>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -18,6 +19,8 @@
 #include <memory>
 #include <set>
 #include <string>
+#include <string_view>
+#include <system_error>
 #include <unordered_map>
 #include <vector>
 
@@ -31,7 +34,7 @@ namespace {
 struct DeckElem {
     int32_t id;
     int nn;
-    std::vector<int32_t> nodes;  // node ids
+    std::array<int32_t, 20> nodes;  // node ids (nn of them)
     std::string elset;
 };
 
@@ -48,34 +51,45 @@ struct NodeLoad {
     double v;
 };
 
-std::string trim(const std::string &s) {
+using sv = std::string_view;
+
+sv trim_v(sv s) {
     size_t a = 0, b = s.size();
     while (a < b && std::isspace((unsigned char)s[a])) a++;
     while (b > a && std::isspace((unsigned char)s[b - 1])) b--;
     return s.substr(a, b - a);
 }
 
-std::string upper(std::string s) {
-    for (char &c : s) c = (char)std::toupper((unsigned char)c);
-    return s;
+std::string trim(const std::string &s) { return std::string(trim_v(s)); }
+
+std::string upper(sv s) {
+    std::string r(s);
+    for (char &c : r) c = (char)std::toupper((unsigned char)c);
+    return r;
 }
 
-std::vector<std::string> split_tokens(const std::string &line) {
-    std::vector<std::string> out;
-    std::string cur;
-    for (char c : line) {
-        if (c == ',' || std::isspace((unsigned char)c)) {
-            if (!cur.empty()) out.push_back(cur), cur.clear();
-        } else {
-            cur.push_back(c);
-        }
+// Tokens of a data line (separated by commas and blanks), appended to out as
+// views into the line.
+void split_tokens(sv line, std::vector<sv> &out) {
+    size_t a = 0;
+    const size_t n = line.size();
+    while (a < n) {
+        while (a < n && (line[a] == ',' || std::isspace((unsigned char)line[a]))) a++;
+        size_t b = a;
+        while (b < n && line[b] != ',' && !std::isspace((unsigned char)line[b])) b++;
+        if (b > a) out.push_back(line.substr(a, b - a));
+        a = b;
     }
-    if (!cur.empty()) out.push_back(cur);
+}
+
+std::vector<sv> split_tokens(sv line) {
+    std::vector<sv> out;
+    split_tokens(line, out);
     return out;
 }
 
-bool is_int_token(const std::string &s) {
-    std::string t = trim(s);
+bool is_int_token(sv s) {
+    sv t = trim_v(s);
     size_t i = 0;
     if (i < t.size() && (t[i] == '+' || t[i] == '-')) i++;
     if (i == t.size()) return false;
@@ -97,14 +111,33 @@ bool full_strtod(const std::string &s, double &out) {
     return true;
 }
 
-// inp.rs:100-141: Fortran reals (1.d0, 1.0D-3, 1., .5, 1.0+3).
-bool parse_f64(const std::string &tok, double &out) {
-    std::string raw = trim(tok);
+// inp.rs:100-141: Fortran reals (1.d0, 1.0D-3, 1., .5, 1.0+3). A token made of
+// digits, '.', 'e', 'E' and '-' only that from_chars reads to its end has the
+// value strtod gives it (both round correctly; from_chars takes no '+' sign and
+// no inf / nan / hex spellings): that is the common case and needs no copy.
+// Everything else takes the reference's full rules.
+bool parse_f64(sv tok, double &out) {
+    const sv raw = trim_v(tok);
     if (raw.empty()) return false;
-    std::string stripped = raw;
+    bool plain = raw.size() < 64;
+    for (char c : raw)
+        if (!(std::isdigit((unsigned char)c) || c == '.' || c == 'e' || c == 'E' || c == '-')) {
+            plain = false;
+            break;
+        }
+    if (plain) {
+        double v;
+        const auto r = std::from_chars(raw.data(), raw.data() + raw.size(), v,
+                                       std::chars_format::general);
+        if (r.ec == std::errc() && r.ptr == raw.data() + raw.size()) {
+            out = v;
+            return true;
+        }
+    }
+    std::string stripped(raw);
     while (!stripped.empty() && stripped.back() == '.') stripped.pop_back();
-    if (full_strtod(stripped, out) || full_strtod(raw, out)) return true;
-    std::string b = raw;
+    if (full_strtod(stripped, out) || full_strtod(std::string(raw), out)) return true;
+    std::string b(raw);
     for (char &c : b)
         if (c == 'd' || c == 'D') c = 'e';
     while (!b.empty() && b.back() == '.') b.pop_back();
@@ -118,16 +151,29 @@ bool parse_f64(const std::string &tok, double &out) {
     return false;
 }
 
-double f64_or0(const std::string &tok) {
+double f64_or0(sv tok) {
     double v = 0.0;
     if (!parse_f64(tok, v)) return 0.0;
     return v == 0.0 ? 0.0 : v;
 }
 
-bool parse_i32(const std::string &tok, int32_t &out) {
-    std::string t = trim(tok);
+// An integer token: [-]digits (at most 9, so no overflow) directly; otherwise
+// the reference's rules (an optional sign and digits read as a 64-bit integer
+// and narrowed, or a real rounded to the nearest integer).
+bool parse_i32(sv tok, int32_t &out) {
+    const sv t = trim_v(tok);
+    size_t i = t.size() && t[0] == '-' ? 1 : 0;
+    if (t.size() > i && t.size() - i <= 9) {
+        int64_t v = 0;
+        size_t k = i;
+        for (; k < t.size() && std::isdigit((unsigned char)t[k]); k++) v = 10 * v + (t[k] - '0');
+        if (k == t.size()) {
+            out = (int32_t)(i ? -v : v);
+            return true;
+        }
+    }
     if (is_int_token(t)) {
-        out = (int32_t)std::strtoll(t.c_str(), nullptr, 10);
+        out = (int32_t)std::strtoll(std::string(t).c_str(), nullptr, 10);
         return true;
     }
     double v;
@@ -136,44 +182,64 @@ bool parse_i32(const std::string &tok, int32_t &out) {
     return true;
 }
 
-int32_t need_i32(const std::string &tok) {
+int32_t need_i32(sv tok) {
     int32_t v;
-    AXG_REQUIRE(parse_i32(tok, v), AXG_E_INVALID, "Keine ganze Zahl: " + tok);
+    AXG_REQUIRE(parse_i32(tok, v), AXG_E_INVALID, "Keine ganze Zahl: " + std::string(tok));
     return v;
 }
 
-bool is_comment_or_blank(const std::string &raw) {
-    std::string t = trim(raw);
+bool is_comment_or_blank(sv raw) {
+    const sv t = trim_v(raw);
     return t.empty() || t.compare(0, 2, "**") == 0;
 }
 
-bool is_keyword(const std::string &raw) {
-    std::string t = trim(raw);
+bool is_keyword(sv raw) {
+    const sv t = trim_v(raw);
     return !t.empty() && t[0] == '*' && t.compare(0, 2, "**") != 0;
 }
 
-void parse_keyword(const std::string &raw, std::string &kw, std::map<std::string, std::string> &params) {
-    std::string t = trim(raw);
+// A keyword line's parameters (NAME=VALUE or bare NAME), upper case; a later
+// repetition of a name replaces the earlier value. A handful per line, so a
+// flat list.
+struct KeywordParams {
+    std::vector<std::pair<std::string, std::string>> kv;
+    void clear() { kv.clear(); }
+    const std::string *get(const std::string &k) const {
+        for (auto &p : kv)
+            if (p.first == k) return &p.second;
+        return nullptr;
+    }
+    size_t count(const std::string &k) const { return get(k) ? 1 : 0; }
+    std::string &operator[](const std::string &k) {
+        for (auto &p : kv)
+            if (p.first == k) return p.second;
+        kv.emplace_back(k, std::string());
+        return kv.back().second;
+    }
+};
+
+void parse_keyword(sv raw, std::string &kw, KeywordParams &params) {
+    std::string t(trim_v(raw));
     if (t.compare(0, 3, "\xEF\xBB\xBF") == 0) t = t.substr(3);
     t = upper(t);
     params.clear();
+    const sv tv(t);
     size_t start = 0;
     bool first = true;
-    while (start <= t.size()) {
-        size_t comma = t.find(',', start);
-        std::string part = trim(t.substr(start, comma == std::string::npos ? std::string::npos
-                                                                           : comma - start));
+    while (start <= tv.size()) {
+        size_t comma = tv.find(',', start);
+        const sv part = trim_v(tv.substr(start, comma == sv::npos ? sv::npos : comma - start));
         if (first) {
-            kw = part;
+            kw.assign(part);
             first = false;
         } else if (!part.empty()) {
             size_t eq = part.find('=');
-            if (eq == std::string::npos)
-                params[part] = "";
+            if (eq == sv::npos)
+                params[std::string(part)] = "";
             else
-                params[trim(part.substr(0, eq))] = trim(part.substr(eq + 1));
+                params[std::string(trim_v(part.substr(0, eq)))] = std::string(trim_v(part.substr(eq + 1)));
         }
-        if (comma == std::string::npos) break;
+        if (comma == sv::npos) break;
         start = comma + 1;
     }
     // unglue_keyword (inp.rs:41-69), the aliases on this path
@@ -191,9 +257,12 @@ struct axg_deck {
     std::vector<int32_t> node_ids;
     std::vector<double> coords;  // 3 per node
     std::vector<DeckElem> elements;
-    std::map<std::string, std::vector<int32_t>> nsets, elsets;
-    std::map<std::string, std::pair<double, double>> materials;
-    std::map<std::string, double> densities;  // *DENSITY per material
+    // hash maps: decks with one set, material and section per element have
+    // hundreds of thousands of names; the two places that need an order
+    // (section rebinding in compact, the strain output's material) sort
+    std::unordered_map<std::string, std::vector<int32_t>> nsets, elsets;
+    std::unordered_map<std::string, std::pair<double, double>> materials;
+    std::unordered_map<std::string, double> densities;  // *DENSITY per material
     int32_t nmodes = 0;                       // *FREQUENCY n (0: static step)
     int32_t nbuckle = 0;                      // *BUCKLE n (0: no buckling step)
     bool nlgeom = false;                      // *STEP, NLGEOM
@@ -202,7 +271,7 @@ struct axg_deck {
     bool output_u = true, output_s = true, output_rf = true, output_e = false, output_nt = false;
     int32_t max_inc = 100;                    // *STEP, INC=
     int32_t ninc = 1;                         // increments of the static step
-    std::map<std::string, std::string> elset_material;
+    std::unordered_map<std::string, std::string> elset_material;
     std::vector<NodeLoad> bcs, cloads, dloads;
     std::vector<Deferred> deferred;
     std::unordered_map<int32_t, int64_t> id_to_index;
@@ -241,6 +310,13 @@ struct axg_deck {
     std::pair<double, double> material_for(const std::string &elset) const {
         return materials.at(material_name_for(elset));
     }
+    // the first material by name (the strain output's E and nu)
+    const std::pair<double, double> &first_material() const {
+        auto best = materials.begin();
+        for (auto it = materials.begin(); it != materials.end(); ++it)
+            if (it->first < best->first) best = it;
+        return best->second;
+    }
     double density_for(const std::string &elset) const {
         auto it = densities.find(material_name_for(elset));
         return it == densities.end() ? 0.0 : it->second;
@@ -250,14 +326,14 @@ struct axg_deck {
 namespace {
 
 // inp.rs:204-230: "set, first[, last][, value]"
-bool boundary_card(const std::vector<std::string> &t, std::string &name, int &first, int &last,
+bool boundary_card(const std::vector<sv> &t, std::string &name, int &first, int &last,
                    double &val) {
     if (t.size() < 2) return false;
     name = upper(t[0]);
     first = need_i32(t[1]);
-    auto dof_id = [](const std::string &s, int &d) {
+    auto dof_id = [](sv s, int &d) {
         if (!is_int_token(s)) return false;
-        long v = std::strtol(trim(s).c_str(), nullptr, 10);
+        long v = std::strtol(std::string(trim_v(s)).c_str(), nullptr, 10);
         if ((v >= 1 && v <= 6) || v == 11) {
             d = (int)v;
             return true;
@@ -337,10 +413,15 @@ void compact(axg_deck &m) {
     m.elsets["EALL"] = eall;
     if (!m.elset_material.empty()) {
         std::unordered_map<int32_t, std::string> id_to_set;
-        for (auto &kv : m.elset_material) {  // sectioned set names in sorted order
-            auto it = m.elsets.find(kv.first);
+        std::vector<const std::string *> names;  // sectioned set names in sorted order
+        names.reserve(m.elset_material.size());
+        for (auto &kv : m.elset_material) names.push_back(&kv.first);
+        std::sort(names.begin(), names.end(),
+                  [](const std::string *a, const std::string *b) { return *a < *b; });
+        for (const std::string *name : names) {
+            auto it = m.elsets.find(*name);
             if (it == m.elsets.end()) continue;
-            for (int32_t eid : it->second) id_to_set.emplace(eid, kv.first);
+            for (int32_t eid : it->second) id_to_set.emplace(eid, *name);
         }
         for (auto &el : m.elements) {
             if (m.elset_material.count(el.elset)) continue;
@@ -379,15 +460,17 @@ void expand_deferred(axg_deck &m) {
 }
 
 void parse_deck(axg_deck &m, const std::string &text) {
-    std::vector<std::string> lines;
+    // lines and tokens are views into text, which outlives the parse
+    std::vector<sv> lines;
     {
+        const sv all(text);
         size_t a = 0;
-        while (a <= text.size()) {
-            size_t b = text.find('\n', a);
-            std::string line = text.substr(a, b == std::string::npos ? std::string::npos : b - a);
-            if (!line.empty() && line.back() == '\r') line.pop_back();
+        while (a <= all.size()) {
+            size_t b = all.find('\n', a);
+            sv line = all.substr(a, b == sv::npos ? sv::npos : b - a);
+            if (!line.empty() && line.back() == '\r') line.remove_suffix(1);
             lines.push_back(line);
-            if (b == std::string::npos) break;
+            if (b == sv::npos) break;
             a = b + 1;
         }
         if (!lines.empty() && lines.back().empty()) lines.pop_back();
@@ -396,34 +479,33 @@ void parse_deck(axg_deck &m, const std::string &text) {
     size_t i = 0;
     std::string cur_mat;
     bool have_mat = false;
-    auto collect = [&](size_t j, std::vector<std::string> &toks) {
+    auto collect = [&](size_t j, std::vector<sv> &toks) {
         toks.clear();
         while (j < n) {
-            const std::string &raw = lines[j];
+            const sv raw = lines[j];
             if (is_comment_or_blank(raw)) {
                 j++;
                 continue;
             }
             if (is_keyword(raw)) break;
-            auto t = split_tokens(raw);
-            toks.insert(toks.end(), t.begin(), t.end());
+            split_tokens(raw, toks);
             j++;
         }
         return j;
     };
-    std::vector<std::string> toks;
-    std::map<std::string, std::string> params;
+    std::vector<sv> toks, line_toks;
+    KeywordParams params;
     std::string kw;
     while (i < n) {
-        const std::string &raw = lines[i];
+        const sv raw = lines[i];
         if (is_comment_or_blank(raw) || !is_keyword(raw)) {
             i++;
             continue;
         }
         parse_keyword(raw, kw, params);
         auto param = [&](const char *key, const std::string &dflt) {
-            auto it = params.find(key);
-            return it == params.end() ? dflt : it->second;
+            const std::string *v = params.get(key);
+            return v ? *v : dflt;
         };
         if (kw == "*NODE") {
             const std::string nset = param("NSET", "");
@@ -431,13 +513,15 @@ void parse_deck(axg_deck &m, const std::string &text) {
             i++;
             std::vector<int32_t> added;
             while (i < n) {
-                const std::string &l = lines[i];
+                const sv l = lines[i];
                 if (is_comment_or_blank(l)) {
                     i++;
                     continue;
                 }
                 if (is_keyword(l)) break;
-                auto t = split_tokens(l);
+                auto &t = line_toks;
+                t.clear();
+                split_tokens(l, t);
                 i++;
                 if (t.empty()) continue;
                 const int32_t nid = need_i32(t[0]);
@@ -465,7 +549,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
                 DeckElem el;
                 el.id = need_i32(toks[k]);
                 el.nn = nn;
-                for (int a = 0; a < nn; a++) el.nodes.push_back(need_i32(toks[k + 1 + a]));
+                for (int a = 0; a < nn; a++) el.nodes[a] = need_i32(toks[k + 1 + a]);
                 el.elset = elset;
                 k += 1 + nn;
                 added.push_back(el.id);
@@ -536,7 +620,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
             i = collect(i + 1, toks);
             AXG_REQUIRE(!toks.empty(), AXG_E_INVALID, "*DENSITY ohne Wert");
             double rho = 0.0;
-            AXG_REQUIRE(parse_f64(toks[0], rho), AXG_E_INVALID, "Keine Zahl: " + toks[0]);
+            AXG_REQUIRE(parse_f64(toks[0], rho), AXG_E_INVALID, "Keine Zahl: " + std::string(toks[0]));
             const std::string name = have_mat && !cur_mat.empty() ? cur_mat : "MATERIAL-1";
             m.materials.emplace(name, std::make_pair(0.0, 0.0));
             m.densities[name] = rho;
@@ -551,7 +635,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
             i++;
             while (i < n && is_comment_or_blank(lines[i])) i++;
             if (i < n && !is_keyword(lines[i])) {
-                m.heading = trim(lines[i]);
+                m.heading = std::string(trim_v(lines[i]));
                 i++;
             }
         } else if (kw == "*NODE FILE" || kw == "*NODE OUTPUT") {
@@ -559,7 +643,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
             i = collect(i + 1, toks);
             m.output_u = m.output_rf = false;
             for (auto &t : toks) {
-                const std::string u = upper(trim(t));
+                const std::string u = upper(trim_v(t));
                 if (u == "U") m.output_u = true;
                 else if (u == "RF") m.output_rf = true;
                 else if (u == "NT") m.output_nt = true;
@@ -570,7 +654,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
             i = collect(i + 1, toks);
             m.output_s = m.output_e = false;
             for (auto &t : toks) {
-                const std::string u = upper(trim(t));
+                const std::string u = upper(trim_v(t));
                 if (u == "S") m.output_s = true;
                 else if (u == "E") m.output_e = true;
             }
@@ -611,7 +695,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
         } else if (kw == "*BOUNDARY") {
             i++;
             while (i < n) {
-                const std::string &l = lines[i];
+                const sv l = lines[i];
                 if (is_comment_or_blank(l)) {
                     i++;
                     continue;
@@ -1500,7 +1584,7 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
         const double *ev = strain;
         if (!ev && stress) {
             AXG_REQUIRE(!d.materials.empty(), AXG_E_INVALID, "Kein Material");
-            const double e = d.materials.begin()->second.first, nu = d.materials.begin()->second.second;
+            const double e = d.first_material().first, nu = d.first_material().second;
             e_lin.resize((size_t)d.nnode * 6);
             for (int64_t i = 0; i < d.nnode; i++) {
                 const double *sv = stress + 6 * i;

@@ -158,3 +158,30 @@ def test_procedure_keywords():
     assert D.Deck(cases[1]).procedure()["ninc"] == 2   # INC= caps the increments
     assert D.Deck(cases[2]).procedure()["ninc"] == 3   # round(1 / 0.3)
     assert D.Deck(cases[5]).procedure()["nmodes"] == 10
+
+
+def test_number_spellings_match_the_oracle():
+    """Node ids and coordinates in every spelling the reader treats differently (plain, signed,
+    exponent forms, Fortran D and sign-only exponents, trailing dots, leading zeros, huge and
+    tiny magnitudes, negative zero): the native reader's fast and full paths against the
+    oracle's restatement of parse_f64_inner / parse_usize (inp.rs:100-141). The corners of
+    the element get spellings of 0 and 1; nodes no element uses carry the extremes."""
+    zero = ["0", "-0", "-0.0", "0.", "+0.0", "0e5", "0D0", "0.0+0", "1.5e-320", "-1.5e-320",
+            "000", ".0"]
+    one = ["1.", "+1.0", "1.0e+0", "1E0", "10e-1", "1.d0", "1.0D+0", ".1e1", "1.0+0",
+           "0.99999999999999999", "9.999999999999999e-1", "1.0000000000000002", "10D-1", "1..",
+           "+.1E+1"]
+    ids = ["1", "+2", "003", "4.0", "5", "6", "7", "8"]
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    lines = ["*NODE\n"]
+    for k, c in enumerate(corners):
+        xs = [(one if c[q] else zero)[(3 * k + q) % len(one if c[q] else zero)] for q in range(3)]
+        lines.append(f"{ids[k]}, {xs[0]}, {xs[1]}, {xs[2]}\n")
+    lines += ["9, 2.5e300, -1e-400, 123456789.123456789\n", "10, 5d-1, 1.0-1, -.25\n",
+              "11, 1e400, 0.30000000000000004, 7.0D+2\n"]
+    text = "".join(lines) + (
+        "*ELEMENT, TYPE=C3D8, ELSET=B\n1, 1, 2, 3, 4, 5, 6, 7, 8\n*MATERIAL, NAME=M\n"
+        "*ELASTIC\n2.1d5, .3\n*SOLID SECTION, ELSET=B, MATERIAL=M\n*BOUNDARY\n1, 1, 3\n")
+    d, a, s = assert_same_setup(text)
+    assert a["e"][0] == 210000.0 and a["nu"][0] == 0.3
+    assert a["xyz"][8, 0] == 2.5e300 and a["xyz"][9, 1] == 0.1 and np.isinf(a["xyz"][10, 0])
