@@ -34,6 +34,7 @@ class LinearResult:
     nfree: int
     ndof: int
     stress_gp: np.ndarray = None  # (nelem, 6) element mean stress
+    solve_ms: float = 0.0         # device time of the PCG
 
     def node_index(self, nid):
         i = int(np.searchsorted(self.node_ids, nid))
@@ -142,7 +143,7 @@ class Deck:
         ids = self.arrays()["node_ids"]
         self._refresh()
         return LinearResult(ids, u, rf, stress, vm, inf["name"], inf["iters"], inf["residual"],
-                            self.n_free, ndof, gp)
+                            self.n_free, ndof, gp, inf["solve_ms"])
 
 
     def procedure(self):

@@ -280,3 +280,25 @@ def nlgeom_cantilever_deck(nx=6, ny=2, nz=2, dims=(6.0, 1.0, 1.0), p_total=-1.0,
         out.append(f"*CLOAD\nTIP, 3, {p_total / len(tip)!r}\n")
     out.append("*END STEP\n")
     return "".join(out)
+
+
+def lognormal_brick_deck(n=80, seed=0, nu=0.3, pressure=1.0):
+    """brick_lognormal(n, seed) as an INP deck the way a per-element modulus has to be written
+    for the reference (its material is per element set, model.rs:1391-1407): one *ELSET,
+    *MATERIAL and *SOLID SECTION per element; bottom face clamped (*BOUNDARY on NSET BOT),
+    P2 = pressure on the top layer (*DLOAD on ELSET TOP). Returns (text, problem)."""
+    p = brick_lognormal(n, seed=seed, nu=nu, pressure=pressure)
+    out = [f"*HEADING\nC3D8 brick {n}^3, one material per element\n*NODE\n"]
+    out += [f"{i + 1}, {x!r}, {y!r}, {z!r}\n" for i, (x, y, z) in enumerate(p.xyz.tolist())]
+    out.append("*ELEMENT, TYPE=C3D8\n")
+    out += [f"{k + 1}, {r[0]}, {r[1]}, {r[2]}, {r[3]}, {r[4]}, {r[5]}, {r[6]}, {r[7]}\n"
+            for k, r in enumerate((p.conn + 1).tolist())]
+    out += [f"*ELSET, ELSET=E{k + 1}\n{k + 1}\n*MATERIAL, NAME=M{k + 1}\n*ELASTIC\n{e!r}, {nu!r}\n"
+            f"*SOLID SECTION, ELSET=E{k + 1}, MATERIAL=M{k + 1}\n" for k, e in enumerate(p.e.tolist())]
+    out += inp_nset("BOT", np.nonzero(p.xyz[:, 2] == 0.0)[0])
+    top = np.nonzero(p.xyz[p.conn[:, 4], 2] == float(n))[0] + 1  # the top layer's elements
+    out.append("*ELSET, ELSET=TOP\n")
+    out += [", ".join(str(int(v)) for v in top[i:i + 16]) + "\n" for i in range(0, len(top), 16)]
+    out.append(f"*STEP\n*STATIC\n*BOUNDARY\nBOT, 1, 3\n*DLOAD\nTOP, P2, {float(pressure)!r}\n"
+               "*END STEP\n")
+    return "".join(out), p

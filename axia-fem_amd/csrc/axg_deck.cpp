@@ -219,9 +219,11 @@ struct KeywordParams {
 };
 
 void parse_keyword(sv raw, std::string &kw, KeywordParams &params) {
-    std::string t(trim_v(raw));
-    if (t.compare(0, 3, "\xEF\xBB\xBF") == 0) t = t.substr(3);
-    t = upper(t);
+    sv r = trim_v(raw);
+    if (r.compare(0, 3, "\xEF\xBB\xBF") == 0) r.remove_prefix(3);
+    thread_local std::string t;  // reused: no allocation per keyword line
+    t.assign(r);
+    for (char &c : t) c = (char)std::toupper((unsigned char)c);
     params.clear();
     const sv tv(t);
     size_t start = 0;
@@ -413,16 +415,18 @@ void compact(axg_deck &m) {
     m.elsets["EALL"] = eall;
     if (!m.elset_material.empty()) {
         std::unordered_map<int32_t, std::string> id_to_set;
-        std::vector<const std::string *> names;  // sectioned set names in sorted order
-        names.reserve(m.elset_material.size());
-        for (auto &kv : m.elset_material) names.push_back(&kv.first);
-        std::sort(names.begin(), names.end(),
-                  [](const std::string *a, const std::string *b) { return *a < *b; });
-        for (const std::string *name : names) {
-            auto it = m.elsets.find(*name);
+        // each element's first sectioned set by name that lists it: the smallest name
+        std::unordered_map<int32_t, const std::string *> first_set;
+        first_set.reserve(m.elements.size());
+        for (auto &kv : m.elset_material) {
+            auto it = m.elsets.find(kv.first);
             if (it == m.elsets.end()) continue;
-            for (int32_t eid : it->second) id_to_set.emplace(eid, *name);
+            for (int32_t eid : it->second) {
+                auto ins = first_set.emplace(eid, &kv.first);
+                if (!ins.second && kv.first < *ins.first->second) ins.first->second = &kv.first;
+            }
         }
+        for (auto &kv : first_set) id_to_set.emplace(kv.first, *kv.second);
         for (auto &el : m.elements) {
             if (m.elset_material.count(el.elset)) continue;
             auto it = id_to_set.find(el.id);
@@ -476,6 +480,13 @@ void parse_deck(axg_deck &m, const std::string &text) {
         if (!lines.empty() && lines.back().empty()) lines.pop_back();
     }
     const size_t n = lines.size();
+    {
+        size_t nkw = 0;  // decks with a set, material and section per element have many
+        for (const sv l : lines) nkw += !l.empty() && l[0] == '*';
+        m.elsets.reserve(nkw / 4 + 16);
+        m.materials.reserve(nkw / 4 + 16);
+        m.elset_material.reserve(nkw / 4 + 16);
+    }
     size_t i = 0;
     std::string cur_mat;
     bool have_mat = false;

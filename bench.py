@@ -56,7 +56,7 @@ for _p in (ROOT, os.path.join(ROOT, "axia-fem_amd")):
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 METRIC = "DOF·iter/s PCG + elements/s assembly, mixed vs FP64 GPU at 1e-8 rel. error"
-WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq", "nlgeom")
+WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq", "nlgeom", "deck")
 
 
 def log(*a):
@@ -742,6 +742,79 @@ def cpu_adjoint(O, p, u, threads, chunk=20000):
     return out
 
 
+def run_deck(a, G, MG, torch, np):
+    """A deck end to end: parse + setup (the native INP reader), device assembly + autospc +
+    FP64 Jacobi PCG to 1e-8 (linalg::pcg's defaults) + reconstruction + reactions + stresses,
+    then the DAT and FRD files; the deck writes the lognormal brick the way the reference needs a
+    per-element modulus written: one element set, material and section per element."""
+    from axia_gpu import deck as D
+    n = a.deck_n
+    t0 = time.perf_counter()
+    text, prob = MG.lognormal_brick_deck(n, seed=0)
+    mb = len(text) / 1e6
+    log(f"deck: {mb:.1f} MB, {prob.nelem} elements, generated in {time.perf_counter() - t0:.1f} s")
+
+    def step():
+        t1 = time.perf_counter()
+        dk = D.Deck(text)
+        t2 = time.perf_counter()
+        res = dk.solve()
+        t3 = time.perf_counter()
+        frd = dk.write_frd(res.u_full, res.stress, res.rf)
+        dat = dk.write_dat(res.u_full, res.rf, res.stress_gp, res.stress)
+        t4 = time.perf_counter()
+        return dk, res, (t2 - t1, t3 - t2, t4 - t3), len(frd) + len(dat)
+
+    for _ in range(a.warmup):
+        step()
+    t5 = time.perf_counter()
+    runs = [step() for _ in range(a.steps)]
+    ms_per_step = 1e3 * (time.perf_counter() - t5) / a.steps
+    dk, res, _, out_bytes = runs[-1]
+    parse_s, solve_s, write_s = (float(np.mean([r[2][k] for r in runs])) for k in range(3))
+    log(f"deck: parse {parse_s:.2f} s, solve {solve_s:.2f} s ({res.iters} it., PCG "
+        f"{res.solve_ms:.0f} ms), files {write_s:.2f} s")
+    arr = dk.arrays()
+    nf = dk.n_free
+    bot = np.nonzero(arr["xyz"][:, 2] == 0.0)[0]
+    reaction_z = float(res.rf[3 * bot + 2].sum())
+    # the deck's operator on a mesh of the same arrays: the block SpMV the PCG runs
+    m = G.Mesh(8, arr["xyz"], arr["conn"], arr["e"], arr["nu"], arr["ind_of"])
+    m.assemble("fp64")
+    nnz = m.info()["nnz"]
+    spmv_ms = m.bench_operator(3, 1, a.reps)
+    out = base_line(a, nf * res.iters / (res.solve_ms / 1e3), ms_per_step, "f64", {
+        "workload": f"INP deck of the C3D8 brick {n}^3 with lognormal E, one *ELSET, *MATERIAL "
+                    f"and *SOLID SECTION per element ({mb:.0f} MB), bottom clamped, P2 = 1 on the "
+                    "top layer: parse, FP64 Jacobi PCG to 1e-8, reactions, stresses, DAT + FRD",
+        "deck_mb": mb, "elements": prob.nelem, "n_free": nf, "nnz": nnz, "precond": "jacobi"})
+    out.update({
+        "deck": {"time_per_deck_ms": ms_per_step, "parse_and_setup_s": parse_s,
+                 "parse_mb_per_s": mb / parse_s, "solve_s": solve_s, "pcg_ms": res.solve_ms,
+                 "pcg_iterations": res.iters, "files_s": write_s, "files_mb": out_bytes / 1e6,
+                 "reaction_z_sum": reaction_z, "applied_load": float(n * n),
+                 "solver": res.solver},
+        "roofline": roofline("bsell64_spmm_kernel<K=1> (FP64 node-block SpMV of the deck's K_ff)",
+                             bsell64_bytes(nf // 3, nnz // 9, 1), spmv_ms),
+    })
+    m.close()
+    if not a.no_cpu:
+        from oracle import oracle as O
+        # the oracle's reader and setup (a restatement of inp.rs / analysis.rs:548-946 in
+        # Python, one thread) on the same kind of deck, 16^3
+        small, _ = MG.lognormal_brick_deck(16, seed=0)
+        t6 = time.perf_counter()
+        O.deck_setup(O.parse_inp(small))
+        t_small = time.perf_counter() - t6
+        out["cpu_baseline"] = {
+            "value": len(small) / 1e6 / t_small, "unit": "MB/s (deck parse + setup)", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle parse_inp + deck_setup on the same deck form at 16^3 "
+                      f"({len(small) / 1e6:.1f} MB); the native reader did "
+                      f"{mb / parse_s:.1f} MB/s on the {n}^3 deck"}
+    return out
+
+
 def run_simp(a, G, MG, torch, np):
     t0 = time.perf_counter()
     prob = MG.simp_brick(120, 60, 60, seed=0)
@@ -862,6 +935,7 @@ def main():
     ap.add_argument("--nl-shape", type=int, nargs=3, default=[200, 20, 20])
     ap.add_argument("--nl-tol", type=float, default=1e-10)
     ap.add_argument("--nl-precond", default="jacobi", choices=("jacobi", "block3"))
+    ap.add_argument("--deck-n", type=int, default=80, help="deck brick edge (elements)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -876,7 +950,7 @@ def main():
 
     run = {"brick": run_brick, "cantilever": run_cantilever, "c3d20": run_c3d20,
            "batch": run_batch, "simp": run_simp, "freq": run_freq,
-           "nlgeom": run_nlgeom}[a.workload]
+           "nlgeom": run_nlgeom, "deck": run_deck}[a.workload]
     out = run(a, G, MG, torch, np)
     if out is not None:
         print(json.dumps(out))

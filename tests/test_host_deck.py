@@ -185,3 +185,16 @@ def test_number_spellings_match_the_oracle():
     d, a, s = assert_same_setup(text)
     assert a["e"][0] == 210000.0 and a["nu"][0] == 0.3
     assert a["xyz"][8, 0] == 2.5e300 and a["xyz"][9, 1] == 0.1 and np.isinf(a["xyz"][10, 0])
+
+
+def test_lognormal_brick_deck_is_the_brick():
+    """The per-element-material deck of the deck benchmark describes brick_lognormal exactly:
+    coordinates, connectivity, moduli (one material per element), DOF numbering and the top
+    pressure load, through the native reader; the oracle's reader agrees on the same text."""
+    from axia_gpu import meshgen as MG
+    text, p = MG.lognormal_brick_deck(5, seed=2)
+    d, a, s = assert_same_setup(text)
+    assert np.array_equal(a["xyz"], p.xyz) and np.array_equal(a["conn"], p.conn)
+    assert np.array_equal(a["e"], p.e) and np.array_equal(a["ind_of"], p.ind_of)
+    assert np.allclose(a["f_full"], p.f_full[0], rtol=0, atol=1e-15)
+    assert abs(a["f_full"][2::3].sum() + 25.0) < 1e-12
