@@ -287,6 +287,13 @@ struct axg_deck {
     // the prescribed set and values before autospc (the NLGEOM driver has none)
     std::vector<uint8_t> prescribed0;
     std::vector<double> u00;
+    // the text a writer's size query formatted, kept for the fill call with the
+    // same arguments and unchanged inputs, so the two-call protocol formats once
+    struct WriterCache {
+        uint64_t key = 0;
+        bool valid = false;
+        std::string text;
+    } wcache;
 
     std::vector<int32_t> expand_nset(const std::string &name) const {
         if (is_int_token(name)) return {(int32_t)std::strtoll(trim(name).c_str(), nullptr, 10)};
@@ -1361,9 +1368,51 @@ int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
 }
 
 // ------------------------------------------------------------- writers
-static void copy_out(const std::string &o, char *buf, int64_t cap, int64_t *len) {
+// Key of a writer call: the writer, a count, the input addresses and their
+// values (FNV-1a over the 64-bit words).
+static uint64_t writer_key(int kind, int64_t extra,
+                           std::initializer_list<std::pair<const double *, int64_t>> in) {
+    uint64_t h = 14695981039346656037ull;
+    auto mix = [&h](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    mix((uint64_t)kind);
+    mix((uint64_t)extra);
+    for (auto &a : in) {
+        mix((uint64_t)(uintptr_t)a.first);
+        if (!a.first) continue;
+        for (int64_t i = 0; i < a.second; i++) {
+            uint64_t w;
+            std::memcpy(&w, a.first + i, 8);
+            mix(w);
+        }
+    }
+    return h;
+}
+
+// The size query (buf == NULL) keeps the formatted text; the fill call with
+// the same key takes it instead of formatting again.
+static bool serve_cached(axg_deck &d, uint64_t key, char *buf, int64_t cap, int64_t *len) {
+    if (!d.wcache.valid || d.wcache.key != key) return false;
+    *len = (int64_t)d.wcache.text.size();
+    if (!buf) return true;
+    AXG_REQUIRE(cap >= *len, AXG_E_INVALID, "output buffer too small");
+    std::memcpy(buf, d.wcache.text.data(), d.wcache.text.size());
+    d.wcache.valid = false;
+    std::string().swap(d.wcache.text);
+    return true;
+}
+
+static void copy_out(axg_deck &d, uint64_t key, std::string &o, char *buf, int64_t cap,
+                     int64_t *len) {
     *len = (int64_t)o.size();
-    if (!buf) return;
+    if (!buf) {
+        d.wcache.key = key;
+        d.wcache.valid = true;
+        d.wcache.text = std::move(o);
+        return;
+    }
     AXG_REQUIRE(cap >= (int64_t)o.size(), AXG_E_INVALID, "output buffer too small");
     std::memcpy(buf, o.data(), o.size());
 }
@@ -1377,6 +1426,10 @@ int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
     API_BEGIN
     AXG_REQUIRE(deck && u_full && rf && len, AXG_E_INVALID, "axg_deck_write_dat: NULL argument");
     const axg_deck &d = *deck;
+    const uint64_t key = writer_key(1, nfreq, {{u_full, d.ndof}, {rf, d.ndof},
+                                               {stress_gp, 6 * d.nelem}, {nodal_stress, 6 * d.nnode},
+                                               {freqs, nfreq}});
+    if (serve_cached(*deck, key, buf, cap, len)) return AXG_OK;
     std::string o;
     auto time0 = [&](std::string &s) { rust_exp(s, 0.0, 7, 14); };
     auto e14 = [](std::string &s, double v) {
@@ -1432,7 +1485,7 @@ int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
         }
         o += '\n';
     }
-    copy_out(o, buf, cap, len);
+    copy_out(*deck, key, o, buf, cap, len);
     API_END
 }
 
@@ -1458,6 +1511,9 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
     API_BEGIN
     AXG_REQUIRE(deck && u_full && len, AXG_E_INVALID, "axg_deck_write_frd: NULL argument");
     const axg_deck &d = *deck;
+    const uint64_t key = writer_key(2, 0, {{u_full, d.ndof}, {stress, 6 * d.nnode}, {rf, d.ndof},
+                                          {strain, 6 * d.nnode}});
+    if (serve_cached(*deck, key, buf, cap, len)) return AXG_OK;
     std::string o;
     o.reserve(1 << 16);
     // the heading's first 66 characters (UTF-8 code points)
@@ -1611,7 +1667,7 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
                      [&](int64_t i, int q) { return ev ? ev[6 * i + q] : 0.0; });
     }
     o += " 9999\n";
-    copy_out(o, buf, cap, len);
+    copy_out(*deck, key, o, buf, cap, len);
     API_END
 }
 

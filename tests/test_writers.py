@@ -179,3 +179,23 @@ def test_output_selection_and_heading():
     frd = D.Deck(base.replace("*STEP\n", "*STEP\n*NODE FILE\nRF\n*EL FILE\nE\n")).write_frd(
         np.zeros(24), np.ones((8, 6)), np.ones(24))
     assert "DISP" not in frd and "STRESS" not in frd and "FORC" in frd and "TOSTRAIN" in frd
+
+
+def test_size_query_text_is_not_served_stale():
+    """The size query keeps its text for the fill call only while the inputs are unchanged:
+    values changed in place between the two calls are formatted anew."""
+    import ctypes as C
+    from axia_gpu import _lib
+    dk = D.Deck(FLACHEISEN)
+    n = len(dk.arrays()["node_ids"])
+    u = np.zeros(3 * n)
+    L = _lib.lib()
+    ln = C.c_int64()
+    assert L.axg_deck_write_frd(dk._h, u.ctypes.data_as(C.POINTER(C.c_double)), None, None, None,
+                                None, 0, C.byref(ln)) == 0
+    u[0] = 0.125  # same address, new value
+    buf = C.create_string_buffer(ln.value + 64)
+    assert L.axg_deck_write_frd(dk._h, u.ctypes.data_as(C.POINTER(C.c_double)), None, None, None,
+                                buf, ln.value + 64, C.byref(ln)) == 0
+    assert buf.raw[:ln.value].decode() == dk.write_frd(u)
+    assert " 1.25000E-01" in buf.raw[:ln.value].decode()
