@@ -66,6 +66,7 @@ class BuckleResult:
     iters: int
     nfree: int
     ndof: int
+    solve_ms: float = 0.0    # device time of the solves (the preload's and the sweeps')
 
 
 @dataclass
@@ -207,7 +208,7 @@ class Deck:
         self._refresh()
         n = nf.value
         return BuckleResult(self.arrays()["node_ids"], fac[:n], shapes[:n] if modes else None, u,
-                            inf["name"], inf["iters"], self.n_free, ndof)
+                            inf["name"], inf["iters"], self.n_free, ndof, inf["solve_ms"])
 
     def frequency(self, ctx=None, precision="fp64", precond="jacobi", tol=1e-10, max_iter=0,
                   inner_tol=1e-3, modes=True):

@@ -302,3 +302,17 @@ def lognormal_brick_deck(n=80, seed=0, nu=0.3, pressure=1.0):
     out.append(f"*STEP\n*STATIC\n*BOUNDARY\nBOT, 1, 3\n*DLOAD\nTOP, P2, {float(pressure)!r}\n"
                "*END STEP\n")
     return "".join(out), p
+
+
+def buckling_column_deck(nx=320, ny=16, nz=16, p_total=-1.0, e=210000.0, nu=0.3, nbuckle=4):
+    """A C3D8 column of unit elements along x (length nx, section ny x nz), the x = 0 face
+    clamped, p_total spread equally over the x = nx face nodes in x (the *CLOAD magnitude
+    applies to each node of the set), and a *BUCKLE step for nbuckle factors."""
+    xyz, conn = brick_hex8(nx, ny, nz)
+    out = inp_solid(8, xyz, conn, e, nu, None)
+    fixed = np.nonzero(xyz[:, 0] == 0.0)[0]
+    tip = np.nonzero(xyz[:, 0] == float(nx))[0]
+    out += inp_nset("FIX", fixed) + inp_nset("TIP", tip)
+    out.append(f"*BOUNDARY\nFIX, 1, 3\n*STEP\n*BUCKLE\n{int(nbuckle)}\n"
+               f"*CLOAD\nTIP, 1, {p_total / len(tip)!r}\n*END STEP\n")
+    return "".join(out)

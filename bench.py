@@ -56,7 +56,8 @@ for _p in (ROOT, os.path.join(ROOT, "axia-fem_amd")):
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 METRIC = "DOF·iter/s PCG + elements/s assembly, mixed vs FP64 GPU at 1e-8 rel. error"
-WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq", "nlgeom", "deck")
+WORKLOADS = ("brick", "cantilever", "c3d20", "batch", "simp", "freq", "nlgeom", "deck",
+             "buckle")
 
 
 def log(*a):
@@ -629,6 +630,50 @@ def run_freq(a, G, MG, torch, np):
     return out
 
 
+def run_buckle(a, G, MG, torch, np):
+    """*BUCKLE on a clamped C3D8 column under an end load: the preload solve, the geometric
+    stiffness on the device, and the reference's subspace iteration (subspace_ab) over the FP64
+    block PCG; the lowest factor against Euler's fixed-free column, pi^2 E I / (4 L^2)."""
+    from axia_gpu import deck as D
+    nx, ny, nz = a.buckle_shape
+    E, P = 210000.0, -1.0
+    text = MG.buckling_column_deck(nx, ny, nz, p_total=P, e=E, nbuckle=a.nbuckle)
+    dk = D.Deck(text)
+    n = dk.n_free
+    log(f"buckle: {dk.nelem} elements, n={n}")
+    for _ in range(a.warmup):
+        dk.buckle(tol=a.eig_tol)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = dk.buckle(tol=a.eig_tol)
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / a.steps
+    inertia = min(ny * nz ** 3, nz * ny ** 3) / 12.0
+    euler = np.pi ** 2 * E * inertia / (4.0 * nx ** 2) / abs(P)
+    log(f"buckle: {ms_per_step:.0f} ms per analysis, factors {res.factors}, Euler {euler:.1f}, "
+        f"{res.iters} column iterations in {res.solve_ms:.0f} ms")
+    arr = dk.arrays()
+    m = G.Mesh(8, arr["xyz"], arr["conn"], arr["e"], arr["nu"], arr["ind_of"])
+    m.assemble("fp64")
+    nnz = m.info()["nnz"]
+    launch_ms = m.bench_operator(3, 8, a.reps)
+    m.close()
+    out = base_line(a, n * res.iters / (res.solve_ms / 1e3), ms_per_step, "f64", {
+        "workload": f"*BUCKLE, {a.nbuckle} factors of a C3D8 column {nx}x{ny}x{nz} (n={n}), "
+                    f"x = 0 clamped, end load {P:g} in x; subspace iteration over the FP64 "
+                    f"block PCG to {a.eig_tol:g}",
+        "n_free": n, "nnz": nnz, "elements": dk.nelem, "nbuckle": a.nbuckle, "precond": "jacobi"})
+    out.update({
+        "buckle": {"time_to_solution_ms": ms_per_step, "factors": [float(f) for f in res.factors],
+                   "euler_fixed_free": euler,
+                   "lowest_over_euler": float(abs(res.factors[0]) / euler),
+                   "column_iterations": res.iters, "solve_ms": res.solve_ms,
+                   "solver": res.solver},
+        "roofline": roofline("bsell64_spmm_kernel<K=8> (FP64 block-sliced-ELL SpMM of the "
+                             "block PCG)", bsell64_bytes(n // 3, nnz // 9, 8), launch_ms),
+    })
+    return out
+
+
 # ------------------------------------------------------------------- nlgeom
 def nl_flops(nn, ngp):
     """Algorithmic FP64 flops of one NLGEOM element tangent + internal force: the dense
@@ -936,6 +981,8 @@ def main():
     ap.add_argument("--nl-tol", type=float, default=1e-10)
     ap.add_argument("--nl-precond", default="jacobi", choices=("jacobi", "block3"))
     ap.add_argument("--deck-n", type=int, default=80, help="deck brick edge (elements)")
+    ap.add_argument("--buckle-shape", type=int, nargs=3, default=[320, 16, 16])
+    ap.add_argument("--nbuckle", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -950,7 +997,7 @@ def main():
 
     run = {"brick": run_brick, "cantilever": run_cantilever, "c3d20": run_c3d20,
            "batch": run_batch, "simp": run_simp, "freq": run_freq,
-           "nlgeom": run_nlgeom, "deck": run_deck}[a.workload]
+           "nlgeom": run_nlgeom, "deck": run_deck, "buckle": run_buckle}[a.workload]
     out = run(a, G, MG, torch, np)
     if out is not None:
         print(json.dumps(out))
