@@ -19,7 +19,7 @@ from . import Mesh
 
 class Compliance(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, e, mesh: Mesh, f_full, tol=1e-12, precond="jacobi"):
+    def forward(ctx, e, mesh: Mesh, f_full, tol=1e-12, precond="block3"):
         e_np = e.detach().to("cpu", torch.float64).numpy()
         mesh.set_modulus(e_np)
         mesh.assemble("fp64")
@@ -40,7 +40,7 @@ class Compliance(torch.autograd.Function):
         return grad_c * dc_de, None, None, None, None
 
 
-def compliance(e, mesh, f_full, tol=1e-12, precond="jacobi"):
+def compliance(e, mesh, f_full, tol=1e-12, precond="block3"):
     """c(E) with gradient dc/dE (see Compliance)."""
     return Compliance.apply(e, mesh, np.ascontiguousarray(f_full, dtype=np.float64), tol, precond)
 

@@ -591,6 +591,7 @@ int axg_mesh_destroy(axg_mesh *mesh) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (mesh->bpcg) block_pcg_destroy(mesh->bpcg);
+    if (mesh->bpcg3) block_pcg_destroy(mesh->bpcg3);
     delete mesh;
     ctx_release(ctx);
     return AXG_OK;
@@ -756,14 +757,17 @@ static void mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
                   mr.spmv_ms, mr.spmv_launches, nrhs);
         return;
     }
-    if (o.precond == AXG_PRECOND_JACOBI && block_pcg_fits(mesh)) {
-        // the reference's Jacobi PCG, every right-hand side at once on the
-        // node-block operator (9 values and one column per 3x3 block)
-        if (!mesh->bpcg) mesh->bpcg = block_pcg_create(mesh, max_iter);
-        block_pcg_refresh(mesh->bpcg, max_iter);
+    if (block_pcg_fits(mesh)) {
+        // the PCG (the reference's Jacobi, or block-Jacobi), every right-hand
+        // side at once on the node-block operator (9 values and one column per
+        // 3x3 block)
+        const bool blk3 = o.precond == AXG_PRECOND_BLOCK3;
+        BlockPcg *&bp = blk3 ? mesh->bpcg3 : mesh->bpcg;
+        if (!bp) bp = block_pcg_create(mesh, max_iter, blk3);
+        block_pcg_refresh(bp, max_iter);
         std::vector<double> tolr(nrhs, o.tol);
         PcgResult r;
-        block_pcg_solve(mesh->bpcg, d_rhs, nrhs, tolr.data(), d_x, r, o.timing != 0);
+        block_pcg_solve(bp, d_rhs, nrhs, tolr.data(), d_x, r, o.timing != 0);
         if (r.status != AXG_OK) throw Error{r.status, pcg_error(r)};
         double worst = 0.0, worst_b = 0.0;
         for (int32_t k = 0; k < nrhs; k++) {
@@ -774,8 +778,9 @@ static void mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
                 worst_b = bn;
             }
         }
-        fill_info(info, "GPU PCG (FP64 node blocks, Jacobi)", r.iters, 1, worst, worst_b,
-                  r.solve_ms, r.spmv_ms, r.spmv_launches, nrhs);
+        fill_info(info, blk3 ? "GPU PCG (FP64 node blocks, block-Jacobi)"
+                             : "GPU PCG (FP64 node blocks, Jacobi)",
+                  r.iters, 1, worst, worst_b, r.solve_ms, r.spmv_ms, r.spmv_launches, nrhs);
         return;
     }
     Precond64Args pa;

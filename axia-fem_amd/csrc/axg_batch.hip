@@ -36,6 +36,8 @@ struct SegScalars {
 
 // Row structure of a system of segments, for the vector and finalize kernels.
 struct BatchDev {
+    const double *binv;     // 3x3 inverse per block row (block-Jacobi), or null (Jacobi)
+    int32_t rows_wg;        // rows per vector workgroup: TB, or 255 (85 whole nodes) with binv
     const double *dsel;     // Jacobi divisor per row (1 on padding rows)
     const int32_t *wg_seg;  // segment of each workgroup
     const int64_t *wg_row;  // first row of each workgroup
@@ -112,8 +114,23 @@ __device__ inline void flat_sums_out(double rr, double rz, double *__restrict__ 
     }
 }
 
-// x += alpha p, r -= alpha Ap, z = r / d, ||r||^2, r.z (per right-hand side).
-template <int K>
+// z for the elements of one workgroup from its residual slab rl (rows row0 ..,
+// [row][K] in LDS): Jacobi z = r / d, or block-Jacobi, row d of the node's 3x3
+// inverse times the node's three residuals (the slab holds whole nodes).
+template <int K, bool BLK>
+__device__ inline double precond_z(const BatchDev &B, const double *rl, int t, int64_t row,
+                                   int64_t row0, double r) {
+    if (!BLK) return r / B.dsel[row];
+    const int lr = t / K, j = t % K, d = (int)(row % 3);
+    const double *Mi = B.binv + 9 * (row / 3) + 3 * d;
+    const double *rn = rl + (lr - d) * K + j;  // the node's first row
+    return Mi[0] * rn[0] + Mi[1] * rn[K] + Mi[2] * rn[2 * K];
+}
+
+// x += alpha p, r -= alpha Ap, z = M^-1 r, ||r||^2, r.z (per right-hand side).
+// With BLK the updated residuals go through LDS first: z of a row needs the
+// other rows of its node, which other threads hold.
+template <int K, bool BLK>
 __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__restrict__ X,
                                                          double *__restrict__ R,
                                                          const double *__restrict__ P,
@@ -122,19 +139,23 @@ __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__
                                                          double *__restrict__ part,
                                                          const SegScalars *sc) {
     static_assert(TB % K == 0, "flat layout needs TB % K == 0");
+    __shared__ double rl[BLK ? TB * K : 1];
     const int seg = B.wg_seg[blockIdx.x];
     if (B.seg_live[seg] == 0) return;
     const int64_t row0 = B.wg_row[blockIdx.x], rend = B.seg_row[seg + 1];
+    const int tend = B.rows_wg * K;
     const int j = threadIdx.x % K;
     const SegScalars &s = sc[seg * K + j];
     const bool live = !s.done;
     const double alpha = s.alpha;
     double rr = 0.0, rz = 0.0;
+    double rk[K];
 #pragma unroll
     for (int q = 0; q < K; q++) {
         const int t = threadIdx.x + q * TB;
         const int64_t row = row0 + t / K;
-        if (row >= rend) break;  // t / K grows with q
+        rk[q] = 0.0;
+        if (t >= tend || row >= rend) break;  // t / K grows with q
         const int64_t i = row0 * K + t;
         double r = R[i];
         if (live) {
@@ -142,10 +163,27 @@ __global__ void __launch_bounds__(TB) batch_update_kernel(BatchDev B, double *__
             r -= alpha * AP[i];
             R[i] = r;
         }
-        const double z = r / B.dsel[row];
-        Z[i] = z;
+        rk[q] = r;
         rr += r * r;
-        rz += r * z;
+        if (BLK) {
+            rl[t] = r;
+        } else {
+            const double z = precond_z<K, false>(B, nullptr, t, row, row0, r);
+            Z[i] = z;
+            rz += r * z;
+        }
+    }
+    if (BLK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int t = threadIdx.x + q * TB;
+            const int64_t row = row0 + t / K;
+            if (t >= tend || row >= rend) break;
+            const double z = precond_z<K, true>(B, rl, t, row, row0, rk[q]);
+            Z[row0 * K + t] = z;
+            rz += rk[q] * z;
+        }
     }
     flat_sums_out<K>(rr, rz, part);
 }
@@ -219,40 +257,61 @@ __global__ void __launch_bounds__(TB) batch_update_p_kernel(BatchDev B, double *
 #pragma unroll
     for (int q = 0; q < K; q++) {
         const int t = threadIdx.x + q * TB;
-        if (row0 + t / K >= rend) break;
+        if (t >= B.rows_wg * K || row0 + t / K >= rend) break;
         const int64_t i = row0 * K + t;
         P[i] = init ? Z[i] : Z[i] + beta * P[i];
     }
 }
 
-// x = 0, r = b, z = r / d and the sums for ||b|| and r.z.
-template <int K>
+// x = 0, r = b, z = M^-1 r and the sums for ||b|| and r.z.
+template <int K, bool BLK>
 __global__ void __launch_bounds__(TB) batch_init_kernel(BatchDev B, const double *__restrict__ Bv,
                                                        double *__restrict__ X,
                                                        double *__restrict__ R,
                                                        double *__restrict__ Z,
                                                        double *__restrict__ part) {
+    __shared__ double rl[BLK ? TB * K : 1];
     const int seg = B.wg_seg[blockIdx.x];
     const int64_t row0 = B.wg_row[blockIdx.x], rend = B.seg_row[seg + 1];
+    const int tend = B.rows_wg * K;
     double rr = 0.0, rz = 0.0;
+    double bk[K];
 #pragma unroll
     for (int q = 0; q < K; q++) {
         const int t = threadIdx.x + q * TB;
         const int64_t row = row0 + t / K;
-        if (row >= rend) break;
+        bk[q] = 0.0;
+        if (t >= tend || row >= rend) break;
         const int64_t i = row0 * K + t;
         const double b = Bv[i];
         X[i] = 0.0;
         R[i] = b;
-        const double z = b / B.dsel[row];
-        Z[i] = z;
+        bk[q] = b;
         rr += b * b;
-        rz += b * z;
+        if (BLK) {
+            rl[t] = b;
+        } else {
+            const double z = precond_z<K, false>(B, nullptr, t, row, row0, b);
+            Z[i] = z;
+            rz += b * z;
+        }
+    }
+    if (BLK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int t = threadIdx.x + q * TB;
+            const int64_t row = row0 + t / K;
+            if (t >= tend || row >= rend) break;
+            const double z = precond_z<K, true>(B, rl, t, row, row0, bk[q]);
+            Z[row0 * K + t] = z;
+            rz += bk[q] * z;
+        }
     }
     flat_sums_out<K>(rr, rz, part);
 }
 
-// Rebase the column indices of one mesh's sliced-ELL copy into the batch.
+// a[i] = v
 __global__ void fill_kernel(int64_t n, double *__restrict__ a, double v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -422,7 +481,9 @@ struct BlockPcg {
     DevBuf<SegScalars> sc;
     BatchDev B, Bmv;  // Bmv: the SpMM's partial sums (one per slice)
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // K = 1, 2, 4, 8
+    bool blk3 = false;                 // block-Jacobi (3x3 node inverses) instead of Jacobi
     const double *graph_bv = nullptr;  // operator values the graphs were captured with
+    const double *graph_binv = nullptr;
     void drop_graphs() {
         for (auto &g : graph)
             if (g) (void)hipGraphExecDestroy(g), g = nullptr;
@@ -505,8 +566,8 @@ static void block_spmv(BlockPcg &bp, hipStream_t s, bool dot, const double *x,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const int slot = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
     axg_mesh *m = bp.m;
-    const Bsell64Args a{bp.nb,  m->bs_sptr.p, m->bs_width.p, m->bval64.p, m->bs_nslots,
-                        bp.nsl_wg, bp.AP.p,   bp.part.p,     bp.seg_live.p, nullptr};
+    const Bsell64Args a{bp.nb,     m->bs_sptr.p, m->bs_width.p, m->bval64.p,   m->bs_nslots,
+                        bp.nsl_wg, bp.AP.p,      bp.part.p,     bp.seg_live.p, nullptr};
     bsell64_spmm<K>(a, s, dot, bp.U[slot], bp.L[slot], x, e0, e1);
 }
 
@@ -516,8 +577,12 @@ static void block_iter(BlockPcg &bp, hipStream_t s, hipEvent_t e0 = nullptr,
     block_spmv<K>(bp, s, true, bp.P.p, e0, e1);
     hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.Bmv, bp.part.p,
                        bp.sc.p, bp.live_total.p);
-    hipLaunchKernelGGL((batch_update_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.X.p, bp.R.p,
-                       bp.P.p, bp.AP.p, bp.Z.p, bp.part.p, bp.sc.p);
+    if (bp.blk3)
+        hipLaunchKernelGGL((batch_update_kernel<K, true>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.X.p,
+                           bp.R.p, bp.P.p, bp.AP.p, bp.Z.p, bp.part.p, bp.sc.p);
+    else
+        hipLaunchKernelGGL((batch_update_kernel<K, false>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.X.p,
+                           bp.R.p, bp.P.p, bp.AP.p, bp.Z.p, bp.part.p, bp.sc.p);
     hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.B, bp.part.p,
                        bp.sc.p, 0, 0.0, bp.tolr.p, bp.max_iter, bp.live_total.p);
     hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.P.p, bp.Z.p,
@@ -549,25 +614,29 @@ static void tune_u(BlockPcg &bp, int slot) {
     bp.L[slot] = best_l;
 }
 
-BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter) {
+BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter, bool blk3) {
     auto bp = std::make_unique<BlockPcg>();
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     ensure_bsell64(m);
+    if (blk3) ensure_block_inverse64(m);
     bp->m = m;
+    bp->blk3 = blk3;
     const int64_t n = m->n_free, nb = m->nb, rows = 3 * nb;
     const int64_t npad = std::max<int64_t>(((rows + TB - 1) / TB) * TB, TB);
     AXG_REQUIRE(npad * 8 < INT32_MAX, AXG_E_INVALID, "block PCG exceeds 2^31 vector entries");
+    // block-Jacobi needs whole nodes in a vector workgroup: 255 rows
+    const int rows_wg = blk3 ? 255 : TB;
     bp->n = n;
     bp->nb = nb;
     bp->rows = rows;
     bp->npad = npad;
     bp->max_iter = max_iter > 0 ? max_iter : std::max<int64_t>(4 * n, 200);
-    bp->nwg = (int)(npad / TB);
+    bp->nwg = (int)std::max<int64_t>((rows + rows_wg - 1) / rows_wg, 1);
     bp->nsl_wg = (int)(((std::max<int64_t>((nb + 63) / 64, 1) + 7) / 8) * 8);  // multiple of 8
     std::vector<int64_t> h_wg_row(bp->nwg);
     std::vector<int32_t> h_wg_seg(bp->nwg, 0);
-    for (int w = 0; w < bp->nwg; w++) h_wg_row[w] = (int64_t)w * TB;
+    for (int w = 0; w < bp->nwg; w++) h_wg_row[w] = (int64_t)w * rows_wg;
     bp->wg_row.alloc(bp->nwg);
     bp->wg_seg.alloc(bp->nwg);
     bp->seg_end.alloc(2);
@@ -596,6 +665,8 @@ BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter) {
     bp->tolr.alloc(8);
     bp->sc.alloc(8);
     BatchDev &B = bp->B;
+    B.binv = blk3 ? m->binv64.p : nullptr;
+    B.rows_wg = rows_wg;
     B.dsel = bp->dsel.p;
     B.wg_seg = bp->wg_seg.p;
     B.wg_row = bp->wg_row.p;
@@ -633,6 +704,7 @@ static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *t
         AXG_HIP(hipGraphInstantiate(&bp.graph[slot], g, nullptr, nullptr, 0));
         AXG_HIP(hipGraphDestroy(g));
         bp.graph_bv = bp.m->bval64.p;
+        bp.graph_binv = bp.B.binv;
     }
     if (timing)
         while (ctx->timing_events.size() < 2 * (size_t)kGraphIters) {
@@ -653,8 +725,12 @@ static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *t
     hipLaunchKernelGGL((pack_blocks_kernel<K>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                        bp.npad, bp.rows, kk, bp.m->free_of_blk.p, bp.n, d_b, bp.Bv.p);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
-    hipLaunchKernelGGL((batch_init_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.Bv.p, bp.X.p,
-                       bp.R.p, bp.Z.p, bp.part.p);
+    if (bp.blk3)
+        hipLaunchKernelGGL((batch_init_kernel<K, true>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.Bv.p,
+                           bp.X.p, bp.R.p, bp.Z.p, bp.part.p);
+    else
+        hipLaunchKernelGGL((batch_init_kernel<K, false>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.Bv.p,
+                           bp.X.p, bp.R.p, bp.Z.p, bp.part.p);
     hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, bp.B, bp.part.p,
                        bp.sc.p, 1, 0.0, bp.tolr.p, bp.max_iter, bp.live_total.p);
     hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(bp.nwg), dim3(TB), 0, s, bp.B, bp.P.p,
@@ -815,6 +891,8 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
     AXG_HIP(hipMemcpyAsync(segsl.p, h_segsl.data(), 4 * (nmesh + 1), hipMemcpyHostToDevice, s));
     AXG_HIP(hipMemcpyAsync(seg_live.p, h_live.data(), 4 * nmesh, hipMemcpyHostToDevice, s));
     BatchDev B;
+    B.binv = nullptr;
+    B.rows_wg = TB;
     B.dsel = dsel.p;
     B.wg_seg = wg_seg.p;
     B.wg_row = wg_row.p;
@@ -847,8 +925,8 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
     int32_t live0 = nmesh * K;
     AXG_HIP(hipMemcpyAsync(live_total.p, &live0, 4, hipMemcpyHostToDevice, s));
     const int64_t max_iter = o.max_iter > 0 ? o.max_iter : std::max<int64_t>(4 * max_n, 200);
-    const Bsell64Args op{nrow / 3, sptr.p, width.p, bv.p,           nslot,
-                         (int)nsl_wg, AP.p, part.p, seg_live.p, slice_seg.p};
+    const Bsell64Args op{nrow / 3,    sptr.p, width.p, bv.p,       nslot,
+                         (int)nsl_wg, AP.p,   part.p,  seg_live.p, slice_seg.p};
     // lanes per block row and blocks in flight, timed on this operator
     int U = 2, L = bsell64_lanes(K)[0];
     {
@@ -869,7 +947,7 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
         bsell64_spmm<K>(op, s, true, U, L, P.p);
         hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, Bmv, part.p,
                            sc.p, live_total.p);
-        hipLaunchKernelGGL((batch_update_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, X.p, R.p, P.p,
+        hipLaunchKernelGGL((batch_update_kernel<K, false>), dim3(nwg), dim3(TB), 0, s, B, X.p, R.p, P.p,
                            AP.p, Z.p, part.p, sc.p);
         hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p,
                            sc.p, 0, o.tol, nullptr, max_iter, live_total.p);
@@ -889,7 +967,7 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
     DevBuf<int64_t> t0(1);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, t0.p);  // start of the solves
-    hipLaunchKernelGGL((batch_init_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p, R.p, Z.p,
+    hipLaunchKernelGGL((batch_init_kernel<K, false>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p, R.p, Z.p,
                        part.p);
     hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p, sc.p,
                        1, o.tol, nullptr, max_iter, live_total.p);
@@ -994,9 +1072,15 @@ void block_pcg_refresh(BlockPcg *bp, int64_t max_iter) {
     axg_mesh *m = bp->m;
     hipStream_t s = m->ctx->stream;
     ensure_bsell64(m);
+    if (bp->blk3) {
+        ensure_block_inverse64(m);
+        bp->B.binv = bp->Bmv.binv = m->binv64.p;
+    }
     const int64_t mi = max_iter > 0 ? max_iter : std::max<int64_t>(4 * bp->n, 200);
-    // the graphs hold max_iter and the operator's address as kernel arguments
-    if (mi != bp->max_iter || m->bval64.p != bp->graph_bv) bp->drop_graphs();
+    // the graphs hold max_iter and the operator's and inverses' addresses as
+    // kernel arguments
+    if (mi != bp->max_iter || m->bval64.p != bp->graph_bv || bp->B.binv != bp->graph_binv)
+        bp->drop_graphs();
     bp->max_iter = mi;
     hipLaunchKernelGGL(block_dsel_kernel, dim3((unsigned)((bp->npad + 255) / 256)), dim3(256), 0, s,
                        bp->npad, bp->rows, m->free_of_blk.p, m->dsel.p, bp->dsel.p);

@@ -149,10 +149,12 @@ struct ColumnSolver {
     double ms = 0.0, res = 0.0, rel = 0.0;
     ColumnSolver(axg_mesh *mesh, const axg_solve_opts &opts, double *pack, double *sol)
         : m(mesh), o(opts), P(pack), D(sol) {
-        if (o.precision != AXG_PREC_MIXED && o.precond == AXG_PRECOND_JACOBI) {
-            if (!m->bpcg) m->bpcg = block_pcg_create(m, o.max_iter);
-            block_pcg_refresh(m->bpcg, o.max_iter);
-            bp = m->bpcg;
+        if (o.precision != AXG_PREC_MIXED) {
+            const bool blk3 = o.precond == AXG_PRECOND_BLOCK3;
+            BlockPcg *&mine = blk3 ? m->bpcg3 : m->bpcg;
+            if (!mine) mine = block_pcg_create(m, o.max_iter, blk3);
+            block_pcg_refresh(mine, o.max_iter);
+            bp = mine;
         }
     }
     ColumnSolver(const ColumnSolver &) = delete;

@@ -250,9 +250,10 @@ struct axg_mesh {
     // the multi-right-hand-side FP64 block PCG; refreshed after every assembly
     bool bsell64_fresh = false;
     axg::DevBuf<double> bval64;
-    // the FP64 Jacobi block PCG of this mesh (created at the first such solve,
-    // destroyed with the mesh)
+    // the FP64 block PCGs of this mesh, Jacobi and block-Jacobi (each created at
+    // the first such solve, destroyed with the mesh)
     axg::BlockPcg *bpcg = nullptr;
+    axg::BlockPcg *bpcg3 = nullptr;
     // host mirror of the active pattern (kept for export)
     std::vector<int32_t> h_rowptr, h_colind;
 };

@@ -57,15 +57,16 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
 // The reference's CG error text for a failed solve (linalg.rs:231-257).
 std::string pcg_error(const PcgResult &r);
 
-// Multi-right-hand-side FP64 Jacobi PCG of one assembled mesh on device
-// vectors, in node-block space on the FP64 block-sliced-ELL operator (the
-// batched solver's vector kernels, 8, 4, 2 or 1 right-hand sides per pass, one
-// graph per width, reused across calls). Column j of d_b (n_free each) is
-// solved to ||r|| <= tol_rel[j] ||b_j||. block_pcg_refresh re-reads the
-// mesh's operator and Jacobi divisor after an assembly and sets max_iter
-// (0: max(4 n, 200)); with timing every SpMM is timed by events.
+// Multi-right-hand-side FP64 PCG of one assembled mesh on device vectors, in
+// node-block space on the FP64 block-sliced-ELL operator, preconditioned by
+// Jacobi or (blk3) the 3x3 node-block inverses (the batched solver's vector
+// kernels, 8, 4, 2 or 1 right-hand sides per pass, one graph per width,
+// reused across calls). Column j of d_b (n_free each) is solved to
+// ||r|| <= tol_rel[j] ||b_j||. block_pcg_refresh re-reads the mesh's operator
+// and preconditioner after an assembly and sets max_iter (0: max(4 n, 200));
+// with timing every SpMM is timed by events.
 struct BlockPcg;
-BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter);
+BlockPcg *block_pcg_create(axg_mesh *m, int64_t max_iter, bool blk3 = false);
 void block_pcg_destroy(BlockPcg *bp);
 void block_pcg_refresh(BlockPcg *bp, int64_t max_iter);
 void block_pcg_solve(BlockPcg *bp, const double *d_b, int nrhs, const double *tol_rel,

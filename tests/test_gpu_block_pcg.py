@@ -49,6 +49,35 @@ def test_matches_the_csr_pcg_and_the_golden(ctx):
     assert info["rel_residual"] < 1e-9
 
 
+@pytest.mark.parametrize("nn", [8, 20])
+def test_block_jacobi_on_the_node_blocks(ctx, nn):
+    """Block-Jacobi (the 3x3 node inverses applied through the workgroup's residual slab in
+    LDS, 255-row workgroups of whole nodes) on the node-block operator: the solution of the
+    Jacobi solve and of the direct reference, for one and for several right-hand sides."""
+    say("case: block-Jacobi", nn)
+    if nn == 8:
+        p = MG.brick_lognormal(11, seed=5)
+        mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+        b = p.rhs()
+    else:
+        xyz, conn = MG.brick_hex20(6, 3, 3)
+        ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 0] == 0.0)[0])
+        mesh = G.Mesh(20, xyz, conn, np.full(len(conn), 70000.0), np.full(len(conn), 0.3), ind,
+                      ctx=ctx)
+        b = np.random.default_rng(1).standard_normal(int((ind >= 0).sum()))
+    mesh.assemble()
+    xj, ij = mesh.solve(b, tol=1e-12)
+    xb, ib = mesh.solve(b, precond="block3", tol=1e-12)
+    ref = csr_solve(mesh, b, ctx, 1e-12)
+    say("  iters jacobi", ij["iters"], "block", ib["iters"], "rel", rel(xb, xj), rel(xb, ref.x))
+    assert "node blocks, block-Jacobi" in ib["name"]
+    assert rel(xb, xj) < 1e-9 and rel(xb, ref.x) < 1e-9
+    B = np.stack([b, 0.5 * b + 1.0, -b])
+    XB, info = mesh.solve(B, precond="block3", tol=1e-12)
+    for k in range(3):
+        assert rel(XB[k], mesh.solve(B[k], precond="block3", tol=1e-12)[0]) < 1e-9
+
+
 @pytest.mark.parametrize("nrhs", [2, 3, 6, 11])
 def test_every_width_matches_single_solves(ctx, nrhs):
     """nrhs columns go through the 8 / 4 / 2 / 1 wide kernels with masked padding columns;
