@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/axia_gpu.h"
@@ -260,9 +261,32 @@ struct axg_mesh {
 
 namespace axg {
 // symbolic phase (host)
+// Allocator whose resize leaves new elements uninitialised (for large host
+// arrays that are written in full by parallel passes: no serial zero fill).
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <class U>
+    DefaultInitAlloc(const DefaultInitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using HostVec = std::vector<T, DefaultInitAlloc<T>>;
+
 struct Pattern {
-    std::vector<int32_t> rowptr, colind;
-    std::vector<uint32_t> slotptr, contrib;
+    HostVec<int32_t> rowptr, colind;
+    HostVec<uint32_t> slotptr, contrib;
 };
 void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
                    const int64_t *ind_of, int64_t n_free, Pattern &out);

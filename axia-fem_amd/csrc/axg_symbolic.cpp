@@ -132,9 +132,11 @@ void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
         if (r != n_free) throw Error{AXG_E_INVALID, "ind_of does not match n_free"};
     }
     const int64_t nnz = out.rowptr[n_free];
-    out.colind.resize(nnz);
-    // per-slot contribution counts, then the contributions themselves
-    std::vector<uint32_t> slot_cnt(nnz, 0);
+    out.colind.resize(nnz);  // written in full by the count pass
+    // per-slot contribution counts, then the contributions themselves; each
+    // row's slots belong to the thread of its node, which zeroes its counts and
+    // later sets its fill cursors (no serial pass over the slots)
+    HostVec<uint32_t> slot_cnt(nnz);
     auto for_rows = [&](bool fill) {
         parallel_for(nnode, [&](int64_t a0, int64_t a1) {
             // per-thread map neighbour node -> offset of its first free DOF in the row
@@ -158,6 +160,10 @@ void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
                     if (ind_of[3 * a + p] < 0) continue;
                     int64_t r = r0 + pr++;
                     int64_t base = out.rowptr[r];
+                    if (!fill)
+                        std::fill(slot_cnt.begin() + base, slot_cnt.begin() + out.rowptr[r + 1], 0u);
+                    else
+                        for (int64_t q = base; q < out.rowptr[r + 1]; q++) slot_cnt[q] = out.slotptr[q];
                     if (!fill) {
                         // columns of this row
                         int64_t c = base;
@@ -194,20 +200,29 @@ void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
         });
     };
     for_rows(false);
-    out.slotptr.assign(nnz + 1, 0);
+    // slot pointers: exclusive prefix sum of the counts, one chunk per thread
+    out.slotptr.resize(nnz + 1);
     {
-        uint64_t acc = 0;
-        for (int64_t s = 0; s < nnz; s++) {
-            out.slotptr[s] = (uint32_t)acc;
-            acc += slot_cnt[s];
-            if (acc > UINT32_MAX) throw Error{AXG_E_INVALID, "scatter map exceeds 2^32 entries"};
-        }
-        out.slotptr[nnz] = (uint32_t)acc;
-        out.contrib.resize(acc);
+        const int nt = host_threads(nnz, 1 << 16);
+        std::vector<uint64_t> csum(nt + 1, 0);
+        parallel_chunks(nnz, nt, [&](int t, int64_t lo, int64_t hi) {
+            uint64_t acc = 0;
+            for (int64_t s = lo; s < hi; s++) acc += slot_cnt[s];
+            csum[t + 1] = acc;
+        });
+        for (int t = 0; t < nt; t++) csum[t + 1] += csum[t];
+        if (csum[nt] > UINT32_MAX) throw Error{AXG_E_INVALID, "scatter map exceeds 2^32 entries"};
+        parallel_chunks(nnz, nt, [&](int t, int64_t lo, int64_t hi) {
+            uint64_t acc = csum[t];
+            for (int64_t s = lo; s < hi; s++) {
+                out.slotptr[s] = (uint32_t)acc;
+                acc += slot_cnt[s];
+            }
+        });
+        out.slotptr[nnz] = (uint32_t)csum[nt];
+        out.contrib.resize(csum[nt]);  // written in full by the fill pass
     }
-    // reuse slot_cnt as the fill cursor
-    for (int64_t s = 0; s < nnz; s++) slot_cnt[s] = out.slotptr[s];
-    for_rows(true);
+    for_rows(true);  // slot_cnt is each row's fill cursor, set from slotptr
 }
 
 }  // namespace axg
