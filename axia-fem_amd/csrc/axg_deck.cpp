@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <set>
@@ -253,6 +254,47 @@ void parse_keyword(sv raw, std::string &kw, KeywordParams &params) {
         if (kw == a[0]) kw = a[1];
 }
 
+// Id -> position for node and element ids: a dense table when the ids are
+// compact (their span at most 4 n + 1024), else a hash map. The first position
+// of a repeated id wins. find returns -1 for an unknown id.
+struct IdIndex {
+    int32_t lo = 0;
+    bool is_dense = true;
+    std::vector<int64_t> dense;
+    std::unordered_map<int32_t, int64_t> sparse;
+
+    template <class Id>
+    void build(size_t n, Id id_at) {
+        dense.clear();
+        sparse.clear();
+        is_dense = true;
+        if (n == 0) return;
+        int32_t mn = id_at(0), mx = mn;
+        for (size_t k = 1; k < n; k++) {
+            mn = std::min(mn, id_at(k));
+            mx = std::max(mx, id_at(k));
+        }
+        lo = mn;
+        const int64_t span = (int64_t)mx - mn + 1;
+        is_dense = span <= 4 * (int64_t)n + 1024;
+        if (is_dense) {
+            dense.assign((size_t)span, -1);
+            for (size_t k = n; k-- > 0;) dense[(size_t)((int64_t)id_at(k) - lo)] = (int64_t)k;
+        } else {
+            sparse.reserve(n);
+            for (size_t k = 0; k < n; k++) sparse.emplace(id_at(k), (int64_t)k);
+        }
+    }
+    int64_t find(int32_t id) const {
+        if (is_dense) {
+            const int64_t o = (int64_t)id - lo;
+            return o >= 0 && o < (int64_t)dense.size() ? dense[(size_t)o] : -1;
+        }
+        auto it = sparse.find(id);
+        return it == sparse.end() ? -1 : it->second;
+    }
+};
+
 }  // namespace
 
 struct axg_deck {
@@ -276,7 +318,7 @@ struct axg_deck {
     std::unordered_map<std::string, std::string> elset_material;
     std::vector<NodeLoad> bcs, cloads, dloads;
     std::vector<Deferred> deferred;
-    std::unordered_map<int32_t, int64_t> id_to_index;
+    IdIndex id_to_index;  // node id -> node index (after compact)
     // linear-static setup (full DOF space)
     int nn = 0;
     int64_t nnode = 0, nelem = 0, ndof = 0, n_free = 0;
@@ -305,16 +347,20 @@ struct axg_deck {
     }
 
     // model.rs:1391-1407: the section's material, else the only one, else STEEL
-    const std::string &material_name_for(const std::string &elset) const {
+    using MaterialIt = std::unordered_map<std::string, std::pair<double, double>>::const_iterator;
+    MaterialIt material_entry_for(const std::string &elset) const {
         auto m = elset_material.find(elset);
         if (m != elset_material.end()) {
             auto it = materials.find(m->second);
-            if (it != materials.end()) return it->first;
+            if (it != materials.end()) return it;
         }
-        if (materials.size() == 1) return materials.begin()->first;
+        if (materials.size() == 1) return materials.begin();
         auto st = materials.find("STEEL");
-        if (st != materials.end()) return st->first;
+        if (st != materials.end()) return st;
         throw Error{AXG_E_INVALID, "Kein Material für ELSET=" + elset};
+    }
+    const std::string &material_name_for(const std::string &elset) const {
+        return material_entry_for(elset)->first;
     }
     std::pair<double, double> material_for(const std::string &elset) const {
         return materials.at(material_name_for(elset));
@@ -396,48 +442,66 @@ void push_boundary(axg_deck &m, const std::string &name, int first, int last, do
 // sectioned set (by name) that lists them.
 void compact(axg_deck &m) {
     const size_t n = m.node_ids.size();
-    std::vector<size_t> order(n);
-    for (size_t i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return m.node_ids[a] < m.node_ids[b]; });
-    std::vector<int32_t> ids;
-    std::vector<double> coords;
-    ids.reserve(n);
-    coords.reserve(3 * n);
-    for (size_t k = 0; k < n; k++) {
-        size_t i = order[k];
-        if (!ids.empty() && ids.back() == m.node_ids[i]) continue;
-        ids.push_back(m.node_ids[i]);
-        for (int q = 0; q < 3; q++) coords.push_back(m.coords[3 * i + q]);
+    bool strictly_sorted = true;  // then sorting and de-duplication change nothing
+    for (size_t i = 1; i < n && strictly_sorted; i++) strictly_sorted = m.node_ids[i - 1] < m.node_ids[i];
+    if (!strictly_sorted) {
+        std::vector<size_t> order(n);
+        for (size_t i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return m.node_ids[a] < m.node_ids[b]; });
+        std::vector<int32_t> ids;
+        std::vector<double> coords;
+        ids.reserve(n);
+        coords.reserve(3 * n);
+        for (size_t k = 0; k < n; k++) {
+            size_t i = order[k];
+            if (!ids.empty() && ids.back() == m.node_ids[i]) continue;
+            ids.push_back(m.node_ids[i]);
+            for (int q = 0; q < 3; q++) coords.push_back(m.coords[3 * i + q]);
+        }
+        m.node_ids.swap(ids);
+        m.coords.swap(coords);
     }
-    m.node_ids.swap(ids);
-    m.coords.swap(coords);
-    m.id_to_index.clear();
-    for (size_t k = 0; k < m.node_ids.size(); k++) m.id_to_index[m.node_ids[k]] = (int64_t)k;
-    std::stable_sort(m.elements.begin(), m.elements.end(),
-                     [](const DeckElem &a, const DeckElem &b) { return a.id < b.id; });
+    m.id_to_index.build(m.node_ids.size(), [&](size_t k) { return m.node_ids[k]; });
+    auto by_id = [](const DeckElem &a, const DeckElem &b) { return a.id < b.id; };
+    if (!std::is_sorted(m.elements.begin(), m.elements.end(), by_id))
+        std::stable_sort(m.elements.begin(), m.elements.end(), by_id);
     m.nsets["NALL"] = m.node_ids;
     std::vector<int32_t> eall;
+    eall.reserve(m.elements.size());
     for (auto &el : m.elements) eall.push_back(el.id);
     m.elsets["EALL"] = eall;
     if (!m.elset_material.empty()) {
-        std::unordered_map<int32_t, std::string> id_to_set;
-        // each element's first sectioned set by name that lists it: the smallest name
-        std::unordered_map<int32_t, const std::string *> first_set;
-        first_set.reserve(m.elements.size());
+        // each element's first sectioned set by name that lists it: the smallest
+        // name, by element position
+        const size_t ne = m.elements.size();
+        IdIndex epos;
+        epos.build(ne, [&](size_t k) { return m.elements[k].id; });
+        std::vector<const std::string *> first_set(ne, nullptr);
         for (auto &kv : m.elset_material) {
             auto it = m.elsets.find(kv.first);
             if (it == m.elsets.end()) continue;
             for (int32_t eid : it->second) {
-                auto ins = first_set.emplace(eid, &kv.first);
-                if (!ins.second && kv.first < *ins.first->second) ins.first->second = &kv.first;
+                const int64_t k = epos.find(eid);
+                if (k < 0) continue;
+                // every element of a repeated id takes the set (first position found here)
+                for (int64_t q = k; q < (int64_t)ne && m.elements[q].id == eid; q++)
+                    if (!first_set[q] || kv.first < *first_set[q]) first_set[q] = &kv.first;
             }
         }
-        for (auto &kv : first_set) id_to_set.emplace(kv.first, *kv.second);
-        for (auto &el : m.elements) {
-            if (m.elset_material.count(el.elset)) continue;
-            auto it = id_to_set.find(el.id);
-            if (it != id_to_set.end()) el.elset = it->second;
+        // an element whose own set has a section keeps it; consecutive elements
+        // usually share their set, so its lookup is reused
+        std::string last;
+        bool have_last = false, last_sectioned = false;
+        for (size_t k = 0; k < ne; k++) {
+            DeckElem &el = m.elements[k];
+            if (!have_last || el.elset != last) {
+                last_sectioned = m.elset_material.count(el.elset) != 0;
+                last = el.elset;
+                have_last = true;
+            }
+            if (last_sectioned || !first_set[k]) continue;
+            el.elset = *first_set[k];
         }
     }
 }
@@ -497,6 +561,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
     size_t i = 0;
     std::string cur_mat;
     bool have_mat = false;
+    std::pair<double, double> *cur_mat_val = nullptr;  // node-based map: stays valid
     auto collect = [&](size_t j, std::vector<sv> &toks) {
         toks.clear();
         while (j < n) {
@@ -511,7 +576,12 @@ void parse_deck(axg_deck &m, const std::string &text) {
         }
         return j;
     };
-    std::vector<sv> toks, line_toks;
+    // the first line at or after j that ends a keyword's data lines
+    auto block_end = [&](size_t j) {
+        while (j < n && (is_comment_or_blank(lines[j]) || !is_keyword(lines[j]))) j++;
+        return j;
+    };
+    std::vector<sv> toks;
     KeywordParams params;
     std::string kw;
     while (i < n) {
@@ -527,31 +597,43 @@ void parse_deck(axg_deck &m, const std::string &text) {
         };
         if (kw == "*NODE") {
             const std::string nset = param("NSET", "");
-            const bool has_set = !nset.empty();
-            i++;
-            std::vector<int32_t> added;
-            while (i < n) {
-                const sv l = lines[i];
-                if (is_comment_or_blank(l)) {
-                    i++;
-                    continue;
+            const size_t j = block_end(i + 1);
+            // data lines in parallel chunks, appended in line order
+            const int64_t nl = (int64_t)(j - i - 1);
+            const int nt = host_threads(nl, 16384);
+            std::vector<std::vector<int32_t>> cid(nt);
+            std::vector<std::vector<double>> cxyz(nt);
+            std::vector<std::exception_ptr> cerr(nt);
+            parallel_chunks(nl, nt, [&](int t, int64_t lo, int64_t hi) {
+                try {
+                    std::vector<sv> tk;
+                    for (int64_t q = lo; q < hi; q++) {
+                        const sv l = lines[i + 1 + q];
+                        if (is_comment_or_blank(l)) continue;
+                        tk.clear();
+                        split_tokens(l, tk);
+                        if (tk.empty()) continue;
+                        cid[t].push_back(need_i32(tk[0]));
+                        for (size_t c = 1; c <= 3; c++)
+                            cxyz[t].push_back(tk.size() > c ? f64_or0(tk[c]) : 0.0);
+                    }
+                } catch (...) {
+                    cerr[t] = std::current_exception();
                 }
-                if (is_keyword(l)) break;
-                auto &t = line_toks;
-                t.clear();
-                split_tokens(l, t);
-                i++;
-                if (t.empty()) continue;
-                const int32_t nid = need_i32(t[0]);
-                m.node_ids.push_back(nid);
-                for (int q = 1; q <= 3; q++)
-                    m.coords.push_back(t.size() > (size_t)q ? f64_or0(t[q]) : 0.0);
-                added.push_back(nid);
+            });
+            for (auto &e : cerr)
+                if (e) std::rethrow_exception(e);
+            size_t added = 0;
+            for (int t = 0; t < nt; t++) {
+                m.node_ids.insert(m.node_ids.end(), cid[t].begin(), cid[t].end());
+                m.coords.insert(m.coords.end(), cxyz[t].begin(), cxyz[t].end());
+                added += cid[t].size();
             }
-            if (has_set) {
+            if (!nset.empty()) {
                 auto &dst = m.nsets[nset];
-                dst.insert(dst.end(), added.begin(), added.end());
+                dst.insert(dst.end(), m.node_ids.end() - added, m.node_ids.end());
             }
+            i = j;
         } else if (kw == "*ELEMENT") {
             const std::string typ = param("TYPE", "");
             int nn = 0;
@@ -560,21 +642,44 @@ void parse_deck(axg_deck &m, const std::string &text) {
             AXG_REQUIRE(nn > 0, AXG_E_INVALID,
                         "Nicht unterstützter Elementtyp " + typ + " (this path: C3D8, C3D20).");
             const std::string elset = param("ELSET", "EALL");
-            i = collect(i + 1, toks);
-            std::vector<int32_t> added;
-            size_t k = 0;
-            while (k + nn < toks.size()) {
-                DeckElem el;
-                el.id = need_i32(toks[k]);
-                el.nn = nn;
-                for (int a = 0; a < nn; a++) el.nodes[a] = need_i32(toks[k + 1 + a]);
-                el.elset = elset;
-                k += 1 + nn;
-                added.push_back(el.id);
-                m.elements.push_back(std::move(el));
-            }
+            const size_t j = block_end(i + 1);
+            // tokens of the data lines (an element may continue on the next
+            // line), tokenised in parallel chunks and joined in line order; then
+            // every complete group of 1 + nn tokens is an element
+            const int64_t nl = (int64_t)(j - i - 1);
+            const int nt = host_threads(nl, 16384);
+            std::vector<std::vector<sv>> ctok(nt);
+            parallel_chunks(nl, nt, [&](int t, int64_t lo, int64_t hi) {
+                for (int64_t q = lo; q < hi; q++) {
+                    const sv l = lines[i + 1 + q];
+                    if (!is_comment_or_blank(l)) split_tokens(l, ctok[t]);
+                }
+            });
+            toks.clear();
+            for (auto &c : ctok) toks.insert(toks.end(), c.begin(), c.end());
+            const size_t ne = toks.size() / (size_t)(nn + 1), e0 = m.elements.size();
+            m.elements.resize(e0 + ne);
+            const int ntp = host_threads((int64_t)ne, 8192);
+            std::vector<std::exception_ptr> cerr(ntp);
+            parallel_chunks((int64_t)ne, ntp, [&](int t, int64_t lo, int64_t hi) {
+                try {
+                    for (int64_t q = lo; q < hi; q++) {
+                        const size_t k = (size_t)q * (nn + 1);
+                        DeckElem &el = m.elements[e0 + q];
+                        el.id = need_i32(toks[k]);
+                        el.nn = nn;
+                        for (int a = 0; a < nn; a++) el.nodes[a] = need_i32(toks[k + 1 + a]);
+                        el.elset = elset;
+                    }
+                } catch (...) {
+                    cerr[t] = std::current_exception();
+                }
+            });
+            for (auto &e : cerr)
+                if (e) std::rethrow_exception(e);
             auto &dst = m.elsets[elset];
-            dst.insert(dst.end(), added.begin(), added.end());
+            for (size_t q = 0; q < ne; q++) dst.push_back(m.elements[e0 + q].id);
+            i = j;
         } else if (kw == "*NSET" || kw == "*ELSET") {
             const bool nsetkw = kw == "*NSET";
             const char *key = nsetkw ? "NSET" : "ELSET";
@@ -613,7 +718,7 @@ void parse_deck(axg_deck &m, const std::string &text) {
         } else if (kw == "*MATERIAL") {
             cur_mat = param("NAME", "MATERIAL-1");
             have_mat = true;
-            m.materials.emplace(cur_mat, std::make_pair(0.0, 0.0));
+            cur_mat_val = &m.materials.emplace(cur_mat, std::make_pair(0.0, 0.0)).first->second;
             i++;
         } else if (kw == "*ELASTIC") {
             i = collect(i + 1, toks);
@@ -632,7 +737,10 @@ void parse_deck(axg_deck &m, const std::string &text) {
                 }
             }
             nu = std::min(std::max(nu, -0.49), 0.49);
-            m.materials[have_mat && !cur_mat.empty() ? cur_mat : "MATERIAL-1"] = {e, nu};
+            if (have_mat && !cur_mat.empty())
+                *cur_mat_val = {e, nu};
+            else
+                m.materials["MATERIAL-1"] = {e, nu};
         } else if (kw == "*DENSITY") {
             // inp.rs:675-686
             i = collect(i + 1, toks);
@@ -796,27 +904,35 @@ void linear_setup(axg_deck &m) {
     m.e.assign(m.nelem, 0.0);
     m.nu.assign(m.nelem, 0.0);
     m.rho.assign(m.nelem, 0.0);
-    std::unordered_map<int32_t, int64_t> eidx;
+    IdIndex eidx;
+    eidx.build(m.elements.size(), [&](size_t k) { return m.elements[k].id; });
+    const std::string *last_set = nullptr;
+    std::pair<double, double> mat{0.0, 0.0};
+    double rho = 0.0;
     for (int64_t k = 0; k < m.nelem; k++) {
         const DeckElem &el = m.elements[k];
-        eidx.emplace(el.id, k);
         for (int a = 0; a < nn; a++) {
-            auto it = m.id_to_index.find(el.nodes[a]);
-            AXG_REQUIRE(it != m.id_to_index.end(), AXG_E_INVALID,
+            const int64_t idx = m.id_to_index.find(el.nodes[a]);
+            AXG_REQUIRE(idx >= 0, AXG_E_INVALID,
                         "Element " + std::to_string(el.id) + ": unbekannter Knoten " +
                             std::to_string(el.nodes[a]));
-            m.conn[k * nn + a] = (int32_t)it->second;
+            m.conn[k * nn + a] = (int32_t)idx;
         }
-        auto mat = m.material_for(el.elset);
+        if (!last_set || el.elset != *last_set) {  // consecutive elements often share a set
+            auto mit = m.material_entry_for(el.elset);
+            mat = mit->second;
+            auto dit = m.densities.empty() ? m.densities.end() : m.densities.find(mit->first);
+            rho = dit == m.densities.end() ? 0.0 : dit->second;
+            last_set = &el.elset;
+        }
         m.e[k] = mat.first;
         m.nu[k] = mat.second;
-        m.rho[k] = m.density_for(el.elset);
+        m.rho[k] = rho;
     }
     auto node_index = [&](int32_t nid) {
-        auto it = m.id_to_index.find(nid);
-        AXG_REQUIRE(it != m.id_to_index.end(), AXG_E_INVALID,
-                    "Unbekannter Knoten " + std::to_string(nid));
-        return it->second;
+        const int64_t idx = m.id_to_index.find(nid);
+        AXG_REQUIRE(idx >= 0, AXG_E_INVALID, "Unbekannter Knoten " + std::to_string(nid));
+        return idx;
     };
     // prescribed values, last BC wins (analysis.rs:555-563)
     m.prescribed.assign(m.ndof, 0);
@@ -839,10 +955,9 @@ void linear_setup(axg_deck &m) {
     m.f_full.assign(m.ndof, 0.0);
     std::map<int64_t, std::vector<std::pair<int, double>>> by_elem;
     for (auto &dl : m.dloads) {
-        auto it = eidx.find(dl.id);
-        AXG_REQUIRE(it != eidx.end(), AXG_E_INVALID,
-                    "DLOAD auf unbekanntes Element " + std::to_string(dl.id));
-        by_elem[it->second].push_back({dl.k, dl.v});
+        const int64_t k = eidx.find(dl.id);
+        AXG_REQUIRE(k >= 0, AXG_E_INVALID, "DLOAD auf unbekanntes Element " + std::to_string(dl.id));
+        by_elem[k].push_back({dl.k, dl.v});
     }
     double exyz[60], fe[60];
     for (auto &kv : by_elem) {
@@ -1093,10 +1208,9 @@ static void nl_fext(const axg_deck &d, const double *u, std::vector<double> &f) 
     std::map<int64_t, double> acc;
     for (auto &cl : d.cloads) {
         if (cl.k >= 3) continue;
-        auto it = d.id_to_index.find(cl.id);
-        AXG_REQUIRE(it != d.id_to_index.end(), AXG_E_INVALID,
-                    "Unbekannter Knoten " + std::to_string(cl.id));
-        const int64_t dof = 3 * it->second + cl.k;
+        const int64_t idx = d.id_to_index.find(cl.id);
+        AXG_REQUIRE(idx >= 0, AXG_E_INVALID, "Unbekannter Knoten " + std::to_string(cl.id));
+        const int64_t dof = 3 * idx + cl.k;
         auto a = acc.find(dof);
         if (a == acc.end())
             acc.emplace(dof, 0.0 + cl.v * 1.0);

@@ -198,3 +198,44 @@ def test_lognormal_brick_deck_is_the_brick():
     assert np.array_equal(a["e"], p.e) and np.array_equal(a["ind_of"], p.ind_of)
     assert np.allclose(a["f_full"], p.f_full[0], rtol=0, atol=1e-15)
     assert abs(a["f_full"][2::3].sum() + 25.0) < 1e-12
+
+
+def test_large_deck_with_sparse_unsorted_ids():
+    """A C3D20 deck with more than 32k *NODE and *ELEMENT data lines (the reader splits them
+    over threads; every element continues on a second line), node ids sparse (1000 i + 7,
+    so the id lookup is the hash map), listed out of order with a repeated id (first
+    definition wins), element ids descending: the native setup equals the oracle's bit for
+    bit."""
+    from axia_gpu import meshgen as MG
+    xyz, conn = MG.brick_hex20(40, 20, 21)
+    nnode, ne = len(xyz), len(conn)
+    assert nnode > 32768 and 2 * ne > 32768
+    nid = 1000 * np.arange(nnode) + 7
+    order = np.random.default_rng(3).permutation(nnode)
+    out = ["*NODE\n"]
+    out += [f"{nid[i]}, {float(xyz[i, 0])!r}, {float(xyz[i, 1])!r}, {float(xyz[i, 2])!r}\n"
+            for i in order]
+    out.append(f"{nid[order[0]]}, 99.0, 99.0, 99.0\n")  # a repeated id: ignored
+    out.append("*ELEMENT, TYPE=C3D20, ELSET=SOLID\n")
+    for k in range(ne):
+        ids = [str(2 * (ne - k) + 1)] + [str(nid[a]) for a in conn[k]]
+        out.append(", ".join(ids[:16]) + ",\n" + ", ".join(ids[16:]) + "\n")
+    out.append("*MATERIAL, NAME=MAT\n*ELASTIC\n70000., 0.3\n*SOLID SECTION, ELSET=SOLID, MATERIAL=MAT\n")
+    fix = nid[xyz[:, 0] == 0.0]
+    out.append("*NSET, NSET=FIX\n")
+    out += [", ".join(str(v) for v in fix[j:j + 16]) + "\n" for j in range(0, len(fix), 16)]
+    out.append("*STEP\n*STATIC\n*BOUNDARY\nFIX, 1, 3\n")
+    out.append(f"*CLOAD\n{nid[np.argmax(xyz[:, 0])]}, 3, -2.5\n")
+    out.append(f"*DLOAD\n{2 * ne + 1}, P2, 1.5\n3, P1, 0.5\n*END STEP\n")
+    text = "".join(out)
+    d, a, s = assert_same_setup(text)
+    assert np.array_equal(a["node_ids"], np.sort(nid))
+    assert np.array_equal(a["elem_ids"], 2 * np.arange(ne) + 3)
+    # a bad id deep inside the threaded blocks raises the reader's error for that token
+    bad = text.replace(f"\n{nid[order[40000]]}, ", "\nX12, ", 1)
+    with pytest.raises(AxgError, match="Keine ganze Zahl: X12"):
+        D.Deck(bad)
+    bad = text.replace(f", {nid[conn[ne - 100, 18]]}, ", ", 12Y, ", 1)
+    assert bad != text
+    with pytest.raises(AxgError, match="Keine ganze Zahl: 12Y"):
+        D.Deck(bad)
