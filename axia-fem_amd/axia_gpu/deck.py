@@ -86,6 +86,15 @@ class NlResult:
     stress_gp: np.ndarray = None  # (nelem, 6) element mean Cauchy stress
 
 
+def _text_out(fill, n):
+    """The text of a writer's fill call (size n from its size query): written into an
+    uninitialised buffer and decoded from it in place (no zero fill, no intermediate copy)."""
+    buf = np.empty(n + 1, np.uint8)
+    m = C.c_int64()
+    check(fill(buf.ctypes.data_as(C.c_char_p), n + 1, C.byref(m)))
+    return str(memoryview(buf)[:m.value], "utf-8")
+
+
 class Deck:
     """A parsed deck and its linear-static setup (host side of libaxia_gpu)."""
 
@@ -242,9 +251,8 @@ class Deck:
         n = C.c_int64()
         L = _lib.lib()
         check(L.axg_deck_write_dat(self._h, *args, nf, None, 0, C.byref(n)))
-        buf = C.create_string_buffer(n.value + 1)
-        check(L.axg_deck_write_dat(self._h, *args, nf, buf, n.value + 1, C.byref(n)))
-        return buf.raw[:n.value].decode("utf-8")
+        return _text_out(lambda buf, cap, m: L.axg_deck_write_dat(self._h, *args, nf, buf, cap, m),
+                         n.value)
 
     def write_frd(self, u_full, stress=None, rf=None, strain=None):
         """The reference's FRD text for results on this deck (frd.rs, one frame), as a str."""
@@ -254,9 +262,8 @@ class Deck:
         n = C.c_int64()
         L = _lib.lib()
         check(L.axg_deck_write_frd(self._h, *args, None, 0, C.byref(n)))
-        buf = C.create_string_buffer(n.value + 1)
-        check(L.axg_deck_write_frd(self._h, *args, buf, n.value + 1, C.byref(n)))
-        return buf.raw[:n.value].decode("utf-8")
+        return _text_out(lambda buf, cap, m: L.axg_deck_write_frd(self._h, *args, buf, cap, m),
+                         n.value)
 
     def _refresh(self):
         nn, nnode, nelem, nfree = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()

@@ -4,6 +4,7 @@
 #include "axg_writers.h"
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -48,14 +49,33 @@ void frd_e12(double v, char out[12]) {
         scaled /= 10.0;
         ex += 1;
     }
-    char buf[64];
-    const int len = std::snprintf(buf, sizeof buf, "%c%.5fE%+03d", sign, scaled / 1e5, ex);
-    if (len >= 12) {
-        std::memcpy(out, buf, 12);  // a three-digit exponent is cut, as the reference does
-    } else {
-        const int pad = 12 - len;
-        std::memset(out, ' ', pad);
-        std::memcpy(out + pad, buf, len);
+    // "%c%.5fE%+03d" of (sign, scaled / 1e5, ex), cut to 12 characters: scaled is
+    // an integer in [1e5, 1e6), so its six digits are the printed ones (the
+    // division's rounding error is far below the fifth decimal)
+    if (!(scaled < 1e6)) {
+        // a subnormal v: powi's 10^ex overflowed, the mantissa is infinite and
+        // the reference prints it as such ("    infE-307")
+        char buf[64];
+        const int len = std::snprintf(buf, sizeof buf, "%c%.5fE%+03d", sign, scaled / 1e5, ex);
+        const int pad = len < 12 ? 12 - len : 0;
+        std::memset(out, ' ', (size_t)pad);
+        std::memcpy(out + pad, buf, (size_t)(12 - pad));
+        return;
+    }
+    int64_t k = (int64_t)scaled;
+    out[0] = sign;
+    for (int i = 7; i >= 3; i--) out[i] = (char)('0' + k % 10), k /= 10;
+    out[1] = (char)('0' + k);
+    out[2] = '.';
+    out[8] = 'E';
+    out[9] = ex < 0 ? '-' : '+';
+    const int ae = ex < 0 ? -ex : ex;
+    if (ae < 100) {
+        out[10] = (char)('0' + ae / 10);
+        out[11] = (char)('0' + ae % 10);
+    } else {  // a three-digit exponent loses its last digit, as the reference's cut
+        out[10] = (char)('0' + ae / 100);
+        out[11] = (char)('0' + ae / 10 % 10);
     }
 }
 
@@ -67,14 +87,19 @@ void rust_exp(std::string &s, double v, int prec, int width) {
     } else if (std::isinf(v)) {
         len = std::snprintf(body, sizeof body, v > 0 ? "inf" : "-inf");
     } else {
-        // C prints "d.ddddddE+dd"; Rust prints the exponent as a plain integer
+        // to_chars with a precision rounds exactly, as printf's %.*e; Rust prints
+        // the exponent as a plain integer ("1.500000E-3", "1.000000E0")
         char c[64];
-        std::snprintf(c, sizeof c, "%.*E", prec, v);
-        char *e = std::strchr(c, 'E');
+        const auto r = std::to_chars(c, c + sizeof c, v, std::chars_format::scientific, prec);
+        char *e = std::find(c, r.ptr, 'e');
         const bool neg = e[1] == '-';
-        const long ev = std::strtol(e + 2, nullptr, 10);
-        *e = '\0';
-        len = std::snprintf(body, sizeof body, "%sE%s%ld", c, neg && ev != 0 ? "-" : "", ev);
+        int ev = 0;
+        std::from_chars(e + 2, r.ptr, ev);
+        len = (int)(e - c);
+        std::memcpy(body, c, (size_t)len);
+        body[len++] = 'E';
+        if (neg && ev != 0) body[len++] = '-';
+        len = (int)(std::to_chars(body + len, body + sizeof body, ev).ptr - body);
     }
     if (len < width) s.append((size_t)(width - len), ' ');
     s.append(body, (size_t)len);
@@ -82,7 +107,7 @@ void rust_exp(std::string &s, double v, int prec, int width) {
 
 void rust_int(std::string &s, int64_t v, int width) {
     char b[32];
-    const int len = std::snprintf(b, sizeof b, "%lld", (long long)v);
+    const int len = (int)(std::to_chars(b, b + sizeof b, v).ptr - b);
     if (len < width) s.append((size_t)(width - len), ' ');
     s.append(b, (size_t)len);
 }
