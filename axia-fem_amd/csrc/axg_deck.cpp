@@ -1553,7 +1553,7 @@ int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
     o += "\n displacements (vx,vy,vz) for set NALL and time  ";
     time0(o);
     o += "\n\n";
-    o += format_rows(d.nnode, [&](int64_t k, std::string &s) {
+    format_rows(o, d.nnode, [&](int64_t k, std::string &s) {
         rust_int(s, d.node_ids[k], 10);
         for (int c = 0; c < 3; c++) e14(s, u_full[3 * k + c]);
         s += '\n';
@@ -1561,7 +1561,7 @@ int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
     o += "\n forces (fx,fy,fz) for set NALL and time  ";
     time0(o);
     o += "\n\n";
-    o += format_rows(d.nnode, [&](int64_t k, std::string &s) {
+    format_rows(o, d.nnode, [&](int64_t k, std::string &s) {
         const double *r = rf + 3 * k;
         if (std::fabs(r[0]) + std::fabs(r[1]) + std::fabs(r[2]) > 1e-14) {
             rust_int(s, d.node_ids[k], 10);
@@ -1573,7 +1573,7 @@ int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
     time0(o);
     o += "\n\n";
     if (stress_gp)
-        o += format_rows(d.nelem, [&](int64_t e, std::string &s) {
+        format_rows(o, d.nelem, [&](int64_t e, std::string &s) {
             rust_int(s, d.elements[e].id, 10);
             s += ' ';
             rust_int(s, 1, 4);
@@ -1583,7 +1583,7 @@ int axg_deck_write_dat(axg_deck *deck, const double *u_full, const double *rf,
     o += '\n';
     if (nodal_stress) {  // append_nodal_stress, dat.rs:70-83
         o += "\n stresses (node,sxx,syy,szz,sxy,syz,szx) for set NALL\n\n";
-        o += format_rows(d.nnode, [&](int64_t k, std::string &s) {
+        format_rows(o, d.nnode, [&](int64_t k, std::string &s) {
             rust_int(s, d.node_ids[k], 10);
             for (int c = 0; c < 6; c++) e14(s, nodal_stress[6 * k + c]);
             s += '\n';
@@ -1674,7 +1674,7 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
         s.append(t, 12);
     };
     block_header('2', nout);
-    o += format_rows(nout, [&](int64_t k, std::string &s) {
+    format_rows(o, nout, [&](int64_t k, std::string &s) {
         const int64_t i = out[k];
         s += " -1";
         rust_int(s, d.node_ids[i], 10);
@@ -1684,7 +1684,7 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
     o += " -3\n";
     block_header('3', d.nelem);
     const int ty = d.nn == 8 ? 1 : 4;  // ElemKind::frd_type (model.rs:364-369)
-    o += format_rows(d.nelem, [&](int64_t e, std::string &s) {
+    format_rows(o, d.nelem, [&](int64_t e, std::string &s) {
         const DeckElem &el = d.elements[e];
         s += " -1";
         rust_int(s, el.id, 10);
@@ -1739,7 +1739,7 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
             o += '\n';
         }
         const int nvals = last_all ? ncomp - 1 : ncomp;
-        o += format_rows(nout, [&](int64_t k, std::string &s) {
+        format_rows(o, nout, [&](int64_t k, std::string &s) {
             const int64_t i = out[k];
             s += " -1";
             rust_int(s, d.node_ids[i], 10);

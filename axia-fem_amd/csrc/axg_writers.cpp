@@ -112,32 +112,30 @@ void rust_int(std::string &s, int64_t v, int width) {
     s.append(b, (size_t)len);
 }
 
-std::string format_rows(int64_t n, const std::function<void(int64_t, std::string &)> &row,
-                        int threads) {
+void format_rows(std::string &o, int64_t n,
+                 const std::function<void(int64_t, std::string &)> &row, int threads) {
     if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
     const int64_t min_chunk = 4096;
     const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n / min_chunk));
+    if (nt == 1) {
+        for (int64_t i = 0; i < n; i++) row(i, o);
+        return;
+    }
     std::vector<std::string> part(nt);
     auto work = [&](int t) {
         const int64_t a = n * t / nt, b = n * (t + 1) / nt;
-        std::string &o = part[t];
-        o.reserve((size_t)(b - a) * 96);
-        for (int64_t i = a; i < b; i++) row(i, o);
+        std::string &p = part[t];
+        p.reserve((size_t)(b - a) * 96);
+        for (int64_t i = a; i < b; i++) row(i, p);
     };
-    if (nt == 1) {
-        work(0);
-        return std::move(part[0]);
-    }
     std::vector<std::thread> pool;
     for (int t = 1; t < nt; t++) pool.emplace_back(work, t);
     work(0);
     for (auto &th : pool) th.join();
-    size_t total = 0;
+    size_t total = o.size();
     for (auto &p : part) total += p.size();
-    std::string out;
-    out.reserve(total);
-    for (auto &p : part) out += p;
-    return out;
+    o.reserve(total);
+    for (auto &p : part) o += p;
 }
 
 }  // namespace axg

@@ -19,10 +19,10 @@ void rust_exp(std::string &s, double v, int prec, int width);
 // Rust's "{:W}" for an integer: right-aligned to width W. Appends to s.
 void rust_int(std::string &s, int64_t v, int width);
 
-// Rows 0..n-1 formatted by row(i, out) (appending to out), in row order; the
-// rows are split into contiguous chunks formatted on up to `threads` host
-// threads (0: the hardware's count, at most 16).
-std::string format_rows(int64_t n, const std::function<void(int64_t, std::string &)> &row,
-                        int threads = 0);
+// Rows 0..n-1 formatted by row(i, out) (appending to out), in row order,
+// appended to o; the rows are split into contiguous chunks formatted on up to
+// `threads` host threads (0: the hardware's count, at most 16).
+void format_rows(std::string &o, int64_t n,
+                 const std::function<void(int64_t, std::string &)> &row, int threads = 0);
 
 }  // namespace axg
