@@ -134,7 +134,8 @@ static void build_bsr(axg_mesh *m) {
     ensure_block_maps(m);
     hipStream_t s = m->ctx->stream;
     const int64_t n = m->n_free, nnz = m->nnz_topo;
-    std::vector<int32_t> rp(n + 1), ci(nnz), bof(n), fob(3 * m->nb);
+    HostVec<int32_t> rp(n + 1), ci(nnz);  // filled by the downloads: no zero fill
+    std::vector<int32_t> bof(n), fob(3 * m->nb);
     AXG_HIP(hipMemcpyAsync(rp.data(), m->rowptr_t.p, 4 * (n + 1), hipMemcpyDeviceToHost, s));
     AXG_HIP(hipMemcpyAsync(ci.data(), m->colind_t.p, 4 * nnz, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipMemcpyAsync(bof.data(), m->blk_of_free.p, 4 * n, hipMemcpyDeviceToHost, s));
@@ -198,7 +199,7 @@ static void build_bsr(axg_mesh *m) {
     // plane position of every scalar slot: the block columns of a row are met in
     // ascending order when the free numbering follows the nodes, so a cursor
     // walks the row's block list (a search from the start when a column goes back)
-    std::vector<int32_t> eos(nnz);
+    HostVec<int32_t> eos(nnz);  // every entry written below (rows 0..n-1 cover them)
     parallel_chunks(n, host_threads(n, 4096), [&](int, int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; i++) {
             const int32_t I = bof[i] / 3, d = bof[i] % 3;
