@@ -199,3 +199,23 @@ def test_size_query_text_is_not_served_stale():
                                 buf, ln.value + 64, C.byref(ln)) == 0
     assert buf.raw[:ln.value].decode() == dk.write_frd(u)
     assert " 1.25000E-01" in buf.raw[:ln.value].decode()
+
+
+def test_threaded_rows_byte_identical_on_a_large_deck():
+    """More than 8,192 node and element rows, so the rows are formatted in chunks on several
+    host threads and joined in order: both files equal the oracle's byte for byte."""
+    from axia_gpu import meshgen as MG
+    xyz, conn = MG.brick_hex8(24, 24, 24)
+    text = "".join(MG.inp_solid(8, xyz, conn, 210000.0, 0.3)) + (
+        "*STEP\n*STATIC\n*CLOAD\n1, 3, 1.0\n*END STEP\n")
+    m = O.parse_inp(text)
+    n, ne = len(m.node_ids), len(m.elements)
+    assert n > 2 * 4096 and ne > 2 * 4096
+    rng = np.random.default_rng(11)
+    u = _edge_values(rng, 3 * n)
+    s = _edge_values(rng, 6 * n).reshape(n, 6)
+    r = _edge_values(rng, 3 * n)
+    gp = _edge_values(rng, 6 * ne).reshape(ne, 6)
+    dk = D.Deck(text)
+    assert dk.write_frd(u, s, r) == O.write_frd(m, u, s, r)
+    assert dk.write_dat(u, r, gp, s) == O.write_dat(m, u, r, gp, s)
