@@ -560,6 +560,8 @@ int axg_mesh_create(axg_ctx *ctx, int32_t nn, int64_t nnode, const double *xyz, 
     up(mesh->colind_t, pat.colind.data(), pat.colind.size());
     up(mesh->slotptr_t, pat.slotptr.data(), pat.slotptr.size());
     up(mesh->contrib_t, pat.contrib.data(), pat.contrib.size());
+    mesh->h_rowptr_t = std::move(pat.rowptr);
+    mesh->h_colind_t = std::move(pat.colind);
     // node -> (element, local) incidence in ascending element order, for stresses
     {
         std::vector<int64_t> ptr(nnode + 1, 0);

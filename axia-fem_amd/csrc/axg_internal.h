@@ -145,6 +145,29 @@ struct Workspace {
     DevBuf<int32_t> err;
     DevBuf<uint32_t> cnt;
 };
+// Allocator whose resize leaves new elements uninitialised (for large host
+// arrays that are written in full by parallel passes: no serial zero fill).
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <class U>
+    DefaultInitAlloc(const DefaultInitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using HostVec = std::vector<T, DefaultInitAlloc<T>>;
+
 }  // namespace axg
 
 namespace axg {
@@ -205,6 +228,9 @@ struct axg_mesh {
     // topological pattern and scatter map (device)
     axg::DevBuf<int32_t> rowptr_t, colind_t;
     axg::DevBuf<uint32_t> slotptr_t, contrib_t;
+    // host copy of rowptr_t / colind_t (written once, at creation), read by the
+    // block layout build instead of downloading the pattern again
+    axg::HostVec<int32_t> h_rowptr_t, h_colind_t;
     // active pattern (== topological unless exact-zero slots were dropped)
     bool compacted = false;
     axg::DevBuf<int32_t> rowptr_c, colind_c;
@@ -255,34 +281,10 @@ struct axg_mesh {
     // the first such solve, destroyed with the mesh)
     axg::BlockPcg *bpcg = nullptr;
     axg::BlockPcg *bpcg3 = nullptr;
-    // host mirror of the active pattern (kept for export)
-    std::vector<int32_t> h_rowptr, h_colind;
 };
 
 namespace axg {
 // symbolic phase (host)
-// Allocator whose resize leaves new elements uninitialised (for large host
-// arrays that are written in full by parallel passes: no serial zero fill).
-template <class T>
-struct DefaultInitAlloc : std::allocator<T> {
-    template <class U>
-    struct rebind {
-        using other = DefaultInitAlloc<U>;
-    };
-    DefaultInitAlloc() = default;
-    template <class U>
-    DefaultInitAlloc(const DefaultInitAlloc<U> &) {}
-    template <class U>
-    void construct(U *p) noexcept {
-        ::new (static_cast<void *>(p)) U;
-    }
-    template <class U, class... A>
-    void construct(U *p, A &&...a) {
-        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
-    }
-};
-template <class T>
-using HostVec = std::vector<T, DefaultInitAlloc<T>>;
 
 struct Pattern {
     HostVec<int32_t> rowptr, colind;

@@ -134,10 +134,18 @@ static void build_bsr(axg_mesh *m) {
     ensure_block_maps(m);
     hipStream_t s = m->ctx->stream;
     const int64_t n = m->n_free, nnz = m->nnz_topo;
-    HostVec<int32_t> rp(n + 1), ci(nnz);  // filled by the downloads: no zero fill
+    // the topology pattern: the host copy kept at creation, else downloaded
+    HostVec<int32_t> rp_dl, ci_dl;
+    const int32_t *rp = m->h_rowptr_t.data(), *ci = m->h_colind_t.data();
+    if ((int64_t)m->h_rowptr_t.size() != n + 1 || (int64_t)m->h_colind_t.size() != nnz) {
+        rp_dl.resize(n + 1);
+        ci_dl.resize(nnz);
+        AXG_HIP(hipMemcpyAsync(rp_dl.data(), m->rowptr_t.p, 4 * (n + 1), hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipMemcpyAsync(ci_dl.data(), m->colind_t.p, 4 * nnz, hipMemcpyDeviceToHost, s));
+        rp = rp_dl.data();
+        ci = ci_dl.data();
+    }
     std::vector<int32_t> bof(n), fob(3 * m->nb);
-    AXG_HIP(hipMemcpyAsync(rp.data(), m->rowptr_t.p, 4 * (n + 1), hipMemcpyDeviceToHost, s));
-    AXG_HIP(hipMemcpyAsync(ci.data(), m->colind_t.p, 4 * nnz, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipMemcpyAsync(bof.data(), m->blk_of_free.p, 4 * n, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipMemcpyAsync(fob.data(), m->free_of_blk.p, 4 * 3 * m->nb, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
