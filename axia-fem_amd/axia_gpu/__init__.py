@@ -24,7 +24,7 @@ from ._lib import (AxgError, PREC_FP64, PREC_MIXED, PRECOND_BLOCK3, PRECOND_JACO
                    REFINE_PLAIN, check, dptr, f64, i32, i64, iptr, lptr)
 
 __all__ = [
-    "AxgError", "Context", "Mesh", "SparseResult", "solve_kff", "csr_from_triplets",
+    "AxgError", "Context", "Mesh", "SparseResult", "solve_kff", "solve_csr", "csr_from_triplets",
     "pattern_host", "pressure_loads", "default_context", "PREC_FP64", "PREC_MIXED",
     "PRECOND_JACOBI", "PRECOND_BLOCK3", "REFINE_FCG", "REFINE_PLAIN",
 ]
@@ -52,6 +52,14 @@ def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inn
     o.timing = 1 if timing else 0
     o.refinement = {"fcg": REFINE_FCG, "plain": REFINE_PLAIN}.get(refinement, refinement)
     return o
+
+
+def _need(a, size, what):
+    """Host-side size check before a buffer crosses the C ABI (the library reads and
+    writes exactly the sizes the mesh implies)."""
+    if a.size != size:
+        raise ValueError(f"{what}: expected {size} values, got {a.size}")
+    return a
 
 
 def _info_dict(info):
@@ -119,11 +127,13 @@ class Mesh:
         self.e = f64(np.broadcast_to(e, (self.nelem,)))
         self.nu = f64(np.broadcast_to(nu, (self.nelem,)))
         self.ind_of = i64(ind_of)
+        _need(self.ind_of, 3 * self.nnode, "ind_of")
         h = C.c_void_p()
         check(_lib.lib().axg_mesh_create(self.ctx.handle, self.nn, self.nnode, dptr(self.xyz),
                                          self.nelem, iptr(self.conn), dptr(self.e), dptr(self.nu),
                                          lptr(self.ind_of), C.byref(h)))
         self._h = h
+        self._n_free = self.info()["n_free"]
 
     def close(self):
         if getattr(self, "_h", None):
@@ -146,7 +156,10 @@ class Mesh:
 
     @property
     def n_free(self):
-        return self.info()["n_free"]
+        return self._n_free
+
+    def _u(self, u_full, what="u_full"):
+        return _need(f64(u_full).ravel(), 3 * self.nnode, what)
 
     def set_modulus(self, e):
         self.e = f64(np.broadcast_to(e, (self.nelem,)))
@@ -202,6 +215,9 @@ class Mesh:
         b = f64(rhs)
         single = b.ndim == 1
         b2 = b.reshape(1, -1) if single else b
+        if b2.ndim != 2 or b2.shape[1] != self.n_free or b2.shape[0] < 1:
+            raise ValueError(f"rhs: expected (n_free,) or (nrhs, n_free) with n_free = "
+                             f"{self.n_free}, got shape {b.shape}")
         nrhs = b2.shape[0]
         x = np.zeros_like(b2)
         info = _lib.SolveInfo()
@@ -226,7 +242,7 @@ class Mesh:
         return ms.value
 
     def nodal_stress(self, u_full):
-        u = f64(u_full).ravel()
+        u = self._u(u_full)
         stress = np.zeros((self.nnode, 6))
         vm = np.zeros(self.nnode)
         en = np.zeros((self.nelem, self.nn, 6))
@@ -236,15 +252,15 @@ class Mesh:
     def reactions(self, u_full, f_full):
         """rf = K_full u_full - f_full over all DOFs (analysis.rs:1027-1035)."""
         rf = np.zeros(3 * self.nnode)
-        check(_lib.lib().axg_reactions(self._h, dptr(f64(u_full).ravel()),
-                                       dptr(f64(f_full).ravel()), dptr(rf)))
+        u, f = self._u(u_full), self._u(f_full, "f_full")
+        check(_lib.lib().axg_reactions(self._h, dptr(u), dptr(f), dptr(rf)))
         return rf
 
     def reduce_rhs(self, f_full, u0):
         """DofMap::reduce of the load vector with prescribed values u0 (constraint.rs:138-173)."""
         rhs = np.zeros(max(self.n_free, 1))
-        check(_lib.lib().axg_reduce_rhs(self._h, dptr(f64(f_full).ravel()), dptr(f64(u0).ravel()),
-                                        dptr(rhs)))
+        f, u = self._u(f_full, "f_full"), self._u(u0, "u0")
+        check(_lib.lib().axg_reduce_rhs(self._h, dptr(f), dptr(u), dptr(rhs)))
         return rhs[:self.n_free]
 
     def full_diagonal(self):
@@ -286,7 +302,7 @@ class Mesh:
 
     def geometric_stiffness(self, u_full):
         """(element mean stresses (nelem, 6), Kg_ff values on the topological pattern)."""
-        u = f64(u_full).ravel()
+        u = self._u(u_full)
         mean = np.zeros((self.nelem, 6))
         nnz_t = self._nnz_topo()
         kg = np.zeros(max(nnz_t, 1))
@@ -307,7 +323,7 @@ class Mesh:
                inner_tol=1e-3, vectors=True):
         """Buckling factors of K_ff phi = lambda (-Kg_ff) phi for the displacements u_full
         (eigen.rs:113-179). Returns (values, vectors (k, n_free) or None, info)."""
-        u = f64(u_full).ravel()
+        u = self._u(u_full)
         vals = np.zeros(nmodes)
         vecs = np.zeros((nmodes, self.n_free)) if vectors else None
         nf = C.c_int32()
@@ -322,7 +338,7 @@ class Mesh:
     def nl_assemble(self, u_full):
         """NLGEOM tangent (into this mesh's K_ff) and internal force at u_full: returns
         (f_int (3 nnode), element mean Cauchy (nelem, 6), element mean Green-Lagrange)."""
-        u = f64(u_full).ravel()
+        u = self._u(u_full)
         fint = np.zeros(3 * self.nnode)
         cau = np.zeros((self.nelem, 6))
         gl = np.zeros((self.nelem, 6))
@@ -331,8 +347,8 @@ class Mesh:
 
     def compliance_adjoint(self, u_full, f_full):
         """c = f^T u and dc/dE_e = -u_e^T K1_e u_e (K1: Ke at unit modulus)."""
-        u = f64(u_full).ravel()
-        f = f64(f_full).ravel()
+        u = self._u(u_full)
+        f = self._u(f_full, "f_full")
         c = C.c_double()
         dc = np.zeros(self.nelem)
         check(_lib.lib().axg_compliance_adjoint(self._h, dptr(u), dptr(f), C.byref(c), dptr(dc)))
@@ -350,7 +366,13 @@ def solve_batch(meshes, rhs_list, tol=1e-8, max_iter=0, ctx=None):
     """
     ctx = ctx or meshes[0].ctx
     nm = len(meshes)
+    if len(rhs_list) != nm:
+        raise ValueError("one right-hand-side block per mesh")
     bs = [f64(np.atleast_2d(b)) for b in rhs_list]
+    for m, b in zip(meshes, bs):
+        if b.ndim != 2 or b.shape[1] != m.n_free or not 1 <= b.shape[0] <= 8:
+            raise ValueError(f"rhs block of shape {b.shape} for a mesh with n_free = {m.n_free}"
+                             " (1 to 8 load cases)")
     xs = [np.zeros_like(b) for b in bs]
     nrhs = np.array([b.shape[0] for b in bs], np.int32)
     H = (C.c_void_p * nm)(*[m._h for m in meshes])
@@ -390,18 +412,42 @@ def csr_from_triplets(n, ti, tj, tv, ctx=None):
     return indptr, indices[:nnz], data[:nnz]
 
 
-def solve_kff(n, trips, rhs, ctx=None, tol=1e-8, max_iter=0, timing=False):
-    """linalg::solve_kff drop-in: trips = (ti, tj, tv) arrays; returns SparseResult."""
+def solve_kff(n, trips, rhs, ctx=None, tol=1e-8, max_iter=0, timing=False, precision="fp64",
+              precond="jacobi", inner_tol=1e-3, refinement="fcg"):
+    """linalg::solve_kff drop-in: trips = (ti, tj, tv) arrays; returns SparseResult.
+
+    precision "fp64" with precond "jacobi" is the reference's linalg::pcg on the scalar
+    rows; "block3" (3x3 block-Jacobi over rows 3i..3i+2) and "mixed" (FP32 inner PCG under
+    FP64 refinement) solve the operator in node-block form."""
     ctx = ctx or default_context()
     ti, tj, tv = (i64(trips[0]), i64(trips[1]), f64(trips[2]))
-    b = f64(rhs)
+    if not len(ti) == len(tj) == len(tv):
+        raise ValueError("triplet arrays differ in length")
+    b = _need(f64(rhs), n, "rhs")
     x = np.zeros(n)
     info = _lib.SolveInfo()
-    o = _opts(PREC_FP64, PRECOND_JACOBI, tol, max_iter, timing=timing)
+    o = _opts(precision, precond, tol, max_iter, inner_tol, timing=timing, refinement=refinement)
     check(_lib.lib().axg_solve_kff(ctx.handle, n, len(ti), lptr(ti), lptr(tj), dptr(tv), dptr(b),
                                    C.byref(o), dptr(x), C.byref(info)))
     d = _info_dict(info)
     return SparseResult(x, d["name"], d["iters"], d["residual"])
+
+
+def solve_csr(indptr, indices, data, rhs, ctx=None, tol=1e-8, max_iter=0, timing=False,
+              precision="fp64", precond="jacobi", inner_tol=1e-3, refinement="fcg"):
+    """Solve K x = rhs for a host CSR matrix (axg_solve_csr). Returns (x, info)."""
+    ctx = ctx or default_context()
+    indptr, indices, data = i64(indptr), i64(indices), f64(data)
+    n = len(indptr) - 1
+    if n < 0 or len(indices) != len(data) or len(indices) < int(indptr[-1]):
+        raise ValueError("inconsistent CSR arrays")
+    b = _need(f64(rhs), n, "rhs")
+    x = np.zeros(n)
+    info = _lib.SolveInfo()
+    o = _opts(precision, precond, tol, max_iter, inner_tol, timing=timing, refinement=refinement)
+    check(_lib.lib().axg_solve_csr(ctx.handle, n, lptr(indptr), lptr(indices), dptr(data),
+                                   dptr(b), C.byref(o), dptr(x), C.byref(info)))
+    return x, _info_dict(info)
 
 
 def pattern_host(nn, nnode, conn, ind_of):

@@ -1,9 +1,12 @@
 """torch.autograd binding of the compliance adjoint over the C ABI.
 
 forward:  E (per-element modulus, float64 tensor) -> assemble K(E) on the
-          device, solve K u = f (FP64 PCG), return c = f^T u.
+          device, solve K u = f (FP64 PCG), return c = f^T u; keeps u.
 backward: dc/dE_e = -u_e^T K1_e u_e (self-adjoint system: the adjoint
-          solution is -u, so no second solve), from axg_compliance_adjoint.
+          solution is -u, so no second solve), from axg_compliance_adjoint on
+          the kept u. K1_e depends on the geometry and nu only, so the mesh's
+          modulus may change between forward and backward. A forward without
+          a backward (no gradient needed) never runs the adjoint kernel.
 
 The modulus enters K linearly (K = sum_e E_e K1_e with K1_e the element
 stiffness at unit modulus), so any parametrisation E(rho) composes through
@@ -29,15 +32,17 @@ class Compliance(torch.autograd.Function):
         x, _ = mesh.solve(b, precision="fp64", precond=precond, tol=tol)
         u = np.zeros(3 * mesh.nnode)
         u[free] = x[mesh.ind_of[free]]
-        c, dc_de = mesh.compliance_adjoint(u, f_full)
-        ctx.save_for_backward(torch.from_numpy(dc_de).to(e.device))
+        c = float(np.dot(f_full, u))
+        ctx.mesh = mesh
         ctx.u = u
+        ctx.f_full = f_full
         return torch.tensor(c, dtype=torch.float64, device=e.device)
 
     @staticmethod
     def backward(ctx, grad_c):
-        (dc_de,) = ctx.saved_tensors
-        return grad_c * dc_de, None, None, None, None
+        _, dc_de = ctx.mesh.compliance_adjoint(ctx.u, ctx.f_full)
+        g = torch.from_numpy(dc_de).to(device=grad_c.device, dtype=torch.float64)
+        return grad_c * g, None, None, None, None
 
 
 def compliance(e, mesh, f_full, tol=1e-12, precond="block3"):

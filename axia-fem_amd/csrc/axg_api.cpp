@@ -423,17 +423,36 @@ int axg_csr_from_triplets(axg_ctx *ctx, int64_t n, int64_t ntrip, const int64_t 
     API_END
 }
 
-static void solve_device_csr(axg_ctx *ctx, const CsrView &A, const double *d_b, double *d_x,
-                             const axg_solve_opts &o, axg_solve_info *info, const char *what) {
-    DevBuf<double> dsel(A.n);
-    diag_fp64(ctx, A, dsel.p, nullptr);
-    Precond64Args pa;
-    pa.dsel = dsel.p;
-    PcgResult r;
-    pcg_fp64(ctx, A, pa, d_b, d_x, o.tol, default_max_iter(o, A.n), o.timing != 0, r);
-    if (r.status != AXG_OK) throw Error{r.status, pcg_error(r)};
-    double res = true_residual(ctx, A, d_x, d_b);
-    fill_info(info, what, r.iters, 1, res, r.bnorm, r.solve_ms, r.spmv_ms, r.spmv_launches, 1);
+static void csr_operator_solve(axg_ctx *ctx, int64_t n, const int32_t *rp, const int32_t *ci,
+                               const double *d_val, const double *d_b, double *d_x,
+                               const axg_solve_opts &o, axg_solve_info *info);
+
+static void check_solve_opts(const axg_solve_opts &o, const char *what) {
+    AXG_REQUIRE(o.precision == AXG_PREC_FP64 || o.precision == AXG_PREC_MIXED, AXG_E_INVALID,
+                std::string(what) + ": unknown precision");
+    AXG_REQUIRE(o.precond == AXG_PRECOND_JACOBI || o.precond == AXG_PRECOND_BLOCK3, AXG_E_INVALID,
+                std::string(what) + ": unknown preconditioner");
+    AXG_REQUIRE(o.refinement == AXG_REFINE_FCG || o.refinement == AXG_REFINE_PLAIN, AXG_E_INVALID,
+                std::string(what) + ": unknown refinement");
+    AXG_REQUIRE(o.tol > 0.0 && std::isfinite(o.tol), AXG_E_INVALID,
+                std::string(what) + ": tol must be positive");
+}
+
+// The solve of a device CSR (int32 row pointer and columns; rp, ci: host copies
+// of the pattern), through an operator mesh (csr_operator_solve): FP64 + Jacobi
+// is the reference's Jacobi PCG (linalg::pcg) on the scalar rows, one right-hand
+// side, on the sliced-ELL copy of the rows; block-Jacobi and the mixed FP32 /
+// FP64 solve take the operator in node-block form (rows 3i, 3i+1, 3i+2 as block
+// row i) and solve it as a mesh's K_ff is.
+static void solve_device_csr(axg_ctx *ctx, const CsrView &A, const int32_t *rp, const int32_t *ci,
+                             const double *d_b, double *d_x, const axg_solve_opts &o,
+                             axg_solve_info *info) {
+    const char *what = "GPU PCG (FP64, Jacobi)";
+    if (A.n == 0) {
+        fill_info(info, what, 0, 1, 0.0, 0.0, 0.0, 0.0, 0, 1);
+        return;
+    }
+    csr_operator_solve(ctx, A.n, rp, ci, A.val, d_b, d_x, o, info);
 }
 
 int axg_solve_kff(axg_ctx *ctx, int64_t n, int64_t ntrip, const int64_t *ti, const int64_t *tj,
@@ -446,6 +465,7 @@ int axg_solve_kff(axg_ctx *ctx, int64_t n, int64_t ntrip, const int64_t *ti, con
     axg_solve_opts o;
     axg_default_opts(&o);
     if (opts) o = *opts;
+    check_solve_opts(o, "axg_solve_kff");
     AXG_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     DevBuf<int64_t> dti(ntrip), dtj(ntrip);
@@ -458,15 +478,21 @@ int axg_solve_kff(axg_ctx *ctx, int64_t n, int64_t ntrip, const int64_t *ti, con
     DeviceCsr64 csr;
     csr_from_triplets_device(ctx, n, ntrip, dti.p, dtj.p, dtv.p, csr);
     dti.release(), dtj.release(), dtv.release();
+    AXG_REQUIRE(csr.nnz < INT32_MAX, AXG_E_INVALID, "axg_solve_kff: nnz exceeds 2^31");
     CsrView A;
     A.n = n;
     A.nnz = csr.nnz;
     A.rowptr = csr.rowptr32.p;
     A.colind = csr.colind32.p;
     A.val = csr.data.p;
+    std::vector<int32_t> rp(n + 1), ci(std::max<int64_t>(csr.nnz, 1));
+    AXG_HIP(hipMemcpyAsync(rp.data(), A.rowptr, 4 * (n + 1), hipMemcpyDeviceToHost, s));
+    if (csr.nnz)
+        AXG_HIP(hipMemcpyAsync(ci.data(), A.colind, 4 * csr.nnz, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
     DevBuf<double> db(n), dx(n);
     AXG_HIP(hipMemcpyAsync(db.p, rhs, 8 * n, hipMemcpyHostToDevice, s));
-    solve_device_csr(ctx, A, db.p, dx.p, o, info, "GPU PCG (FP64, Jacobi)");
+    solve_device_csr(ctx, A, rp.data(), ci.data(), db.p, dx.p, o, info);
     AXG_HIP(hipMemcpyAsync(x_out, dx.p, 8 * n, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
     API_END
@@ -481,17 +507,25 @@ int axg_solve_csr(axg_ctx *ctx, int64_t n, const int64_t *indptr, const int64_t 
     axg_solve_opts o;
     axg_default_opts(&o);
     if (opts) o = *opts;
-    AXG_HIP(hipSetDevice(ctx->device));
-    hipStream_t s = ctx->stream;
+    check_solve_opts(o, "axg_solve_csr");
+    AXG_REQUIRE(indptr[0] == 0, AXG_E_INVALID, "axg_solve_csr: indptr[0] must be 0");
+    for (int64_t i = 0; i < n; i++)
+        AXG_REQUIRE(indptr[i + 1] >= indptr[i], AXG_E_INVALID,
+                    "axg_solve_csr: indptr must be non-decreasing");
     int64_t nnz = indptr[n];
     AXG_REQUIRE(nnz >= 0 && nnz < INT32_MAX, AXG_E_INVALID, "axg_solve_csr: nnz out of range");
-    std::vector<int32_t> rp(n + 1), ci(nnz);
+    AXG_REQUIRE(nnz == 0 || (indices && data), AXG_E_INVALID,
+                "axg_solve_csr: NULL indices or data");
+    std::vector<int32_t> rp(n + 1), ci(std::max<int64_t>(nnz, 1));
     for (int64_t i = 0; i <= n; i++) rp[i] = (int32_t)indptr[i];
     for (int64_t k = 0; k < nnz; k++) {
         AXG_REQUIRE(indices[k] >= 0 && indices[k] < n, AXG_E_INVALID,
                     "axg_solve_csr: column index out of range");
         ci[k] = (int32_t)indices[k];
     }
+    // every argument is checked before the context and the device are touched
+    AXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
     DevBuf<int32_t> drp(n + 1), dci(nnz);
     DevBuf<double> dv(nnz), db(n), dx(n);
     AXG_HIP(hipMemcpyAsync(drp.p, rp.data(), 4 * (n + 1), hipMemcpyHostToDevice, s));
@@ -506,7 +540,7 @@ int axg_solve_csr(axg_ctx *ctx, int64_t n, const int64_t *indptr, const int64_t 
     A.rowptr = drp.p;
     A.colind = dci.p;
     A.val = dv.p;
-    solve_device_csr(ctx, A, db.p, dx.p, o, info, "GPU PCG (FP64, Jacobi)");
+    solve_device_csr(ctx, A, rp.data(), ci.data(), db.p, dx.p, o, info);
     AXG_HIP(hipMemcpyAsync(x_out, dx.p, 8 * n, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
     API_END
@@ -845,6 +879,77 @@ int axg_mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
     mesh_solve_device(mesh, d_rhs, nrhs, o, d_x, info);
     AXG_HIP(hipStreamSynchronize(mesh->ctx->stream));
     API_END
+}
+
+// An axg_mesh without elements whose K_ff is a caller's CSR (n rows, int32
+// pattern rp / ci on the host, values d_val on the device): block row i holds
+// rows 3i .. 3i+2 (the last one padded with absent rows), every stored value is
+// the single contribution of its slot (identity scatter map, ke = the values),
+// so the block layouts, the 3x3 inverses, the FP32 operator and the solvers of
+// a mesh apply unchanged.
+static void csr_operator_solve(axg_ctx *ctx, int64_t n, const int32_t *rp, const int32_t *ci,
+                               const double *d_val, const double *d_b, double *d_x,
+                               const axg_solve_opts &o, axg_solve_info *info) {
+    hipStream_t s = ctx->stream;
+    const int64_t nnz = rp[n];
+    AXG_REQUIRE(n > 0, AXG_E_INVALID, "empty operator");
+    auto mesh = std::make_unique<axg_mesh>();
+    mesh->ctx = ctx;
+    mesh->csr_operator = true;
+    mesh->nnode = (n + 2) / 3;
+    mesh->nelem = 0;
+    mesh->ndof = 3 * mesh->nnode;
+    mesh->n_free = n;
+    mesh->nnz = mesh->nnz_topo = nnz;
+    mesh->ncontrib = mesh->ncontrib_topo = nnz;
+    const bool plain = o.precision == AXG_PREC_FP64 && o.precond == AXG_PRECOND_JACOBI;
+    std::vector<int32_t> ind(mesh->ndof);
+    for (int64_t d = 0; d < mesh->ndof; d++) ind[d] = d < n ? (int32_t)d : -1;
+    mesh->h_rowptr_t.assign(rp, rp + n + 1);
+    mesh->h_colind_t.assign(ci, ci + nnz);
+    auto up = [&](auto &buf, const auto *src, size_t count) {
+        buf.alloc(count);
+        if (count)
+            AXG_HIP(hipMemcpyAsync(buf.p, src, count * sizeof(*src), hipMemcpyHostToDevice, s));
+    };
+    up(mesh->ind_of, ind.data(), ind.size());
+    up(mesh->rowptr_t, rp, (size_t)n + 1);
+    up(mesh->colind_t, ci, (size_t)nnz);
+    mesh->val.alloc(std::max<int64_t>(nnz, 1));
+    if (nnz) AXG_HIP(hipMemcpyAsync(mesh->val.p, d_val, 8 * nnz, hipMemcpyDeviceToDevice, s));
+    if (!plain) {
+        // the node-block layouts gather slot values through the scatter map
+        std::vector<uint32_t> ident(nnz + 1);
+        for (int64_t k = 0; k <= nnz; k++) ident[k] = (uint32_t)k;
+        up(mesh->slotptr_t, ident.data(), ident.size());
+        up(mesh->contrib_t, ident.data(), (size_t)nnz);
+        mesh->ke.alloc(std::max<int64_t>(nnz, 1));
+        if (nnz) AXG_HIP(hipMemcpyAsync(mesh->ke.p, d_val, 8 * nnz, hipMemcpyDeviceToDevice, s));
+    }
+    mesh->dsel.alloc(n);
+    CsrView A = active_csr(mesh.get());
+    diag_fp64(ctx, A, mesh->dsel.p, nullptr);
+    AXG_HIP(hipStreamSynchronize(s));
+    mesh->assembled64 = true;
+    ctx->refs.fetch_add(1);
+    struct Drop {
+        axg_mesh *m;
+        ~Drop() { axg_mesh_destroy(m); }
+    } drop{mesh.release()};
+    if (!plain) {
+        mesh_solve_device(drop.m, d_b, 1, o, d_x, info);
+        return;
+    }
+    // linalg::pcg: Jacobi, the scalar rows (sliced-ELL copy), one right-hand side
+    CsrView As = solver_csr(drop.m);
+    Precond64Args pa;
+    pa.dsel = drop.m->dsel.p;
+    PcgResult r;
+    pcg_fp64(ctx, As, pa, d_b, d_x, o.tol, default_max_iter(o, n), o.timing != 0, r);
+    if (r.status != AXG_OK) throw Error{r.status, pcg_error(r)};
+    const double res = true_residual(ctx, As, d_x, d_b);
+    fill_info(info, "GPU PCG (FP64, Jacobi)", r.iters, 1, res, r.bnorm, r.solve_ms, r.spmv_ms,
+              r.spmv_launches, 1);
 }
 
 // ------------------------------------------------------ stress, adjoint

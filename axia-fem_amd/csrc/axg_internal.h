@@ -216,6 +216,10 @@ struct BlockPcg;
 
 struct axg_mesh {
     axg_ctx *ctx = nullptr;
+    // an operator built from a caller's CSR instead of elements (no elements;
+    // three consecutive rows form a block row; each stored value is the one
+    // contribution of its own slot, ke holds the values)
+    bool csr_operator = false;
     int nn = 8, m = 24;
     int64_t nnode = 0, nelem = 0, ndof = 0, n_free = 0;
     int64_t nnz = 0, ncontrib = 0;          // of the current (possibly compacted) pattern

@@ -151,20 +151,27 @@ static void build_bsr(axg_mesh *m) {
     AXG_HIP(hipStreamSynchronize(s));
     const int64_t nb = m->nb;
     // block columns of block row I = block rows of the columns of I's first free row
-    // (all free rows of a node share the same node neighbourhood); rows are split over
-    // host threads, each chunk building its rows' lists, then joined in row order
+    // (all free rows of a node share the same node neighbourhood; an operator taken
+    // from a caller's CSR has no such guarantee and takes the union of its rows); rows
+    // are split over host threads, each chunk building its rows' lists, then joined in
+    // row order
     const int nt = host_threads(nb, 4096);
     std::vector<std::vector<int32_t>> part_ci(nt);
     std::vector<int32_t> brp(nb + 1, 0);
     std::vector<int> bad(nt, 0);
+    const bool all_rows = m->csr_operator;
     parallel_chunks(nb, nt, [&](int t, int64_t lo, int64_t hi) {
         std::vector<int32_t> row;
         auto &out = part_ci[t];
         for (int64_t I = lo; I < hi; I++) {
-            int32_t r = -1;
-            for (int d = 0; d < 3 && r < 0; d++) r = fob[3 * I + d];
             row.clear();
-            for (int32_t k = rp[r]; k < rp[r + 1]; k++) row.push_back(bof[ci[k]] / 3);
+            bool first = true;
+            for (int d = 0; d < 3; d++) {
+                const int32_t r = fob[3 * I + d];
+                if (r < 0 || (!first && !all_rows)) continue;
+                first = false;
+                for (int32_t k = rp[r]; k < rp[r + 1]; k++) row.push_back(bof[ci[k]] / 3);
+            }
             if (!std::is_sorted(row.begin(), row.end()))
                 std::sort(row.begin(), row.end());  // any free-DOF numbering
             row.erase(std::unique(row.begin(), row.end()), row.end());
@@ -313,12 +320,20 @@ void ensure_bsell64(axg_mesh *m) {
     m->bsell64_fresh = true;
 }
 
+// FP32 copy of an operator's stored values (the contributions of a CSR operator).
+__global__ void to_fp32_kernel(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     if (m->nnzb == 0) build_bsr(m);
     ensure_block_inverse64(m);
-    m->ke32.ensure((size_t)m->nelem * m->m * m->m);
+    m->ke32.ensure(m->csr_operator ? (size_t)std::max<int64_t>(m->ncontrib, 1)
+                                   : (size_t)m->nelem * m->m * m->m);
     m->bval.ensure(std::max<int64_t>(10 * m->bs_nslots, 1));
     Workspace &w = ctx->ws;
     AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
@@ -327,7 +342,13 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
         hipLaunchKernelGGL(col_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)), dim3(256),
                            0, s, m->bs_nslots, m->bs_col.p, m->bval.p + 8 * m->bs_nslots);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
-    launch_ke_fp32(m, s, w.err.p);
+    if (m->csr_operator) {
+        if (m->ncontrib)
+            hipLaunchKernelGGL(to_fp32_kernel, dim3(1024), dim3(256), 0, s, m->ncontrib, m->ke.p,
+                               m->ke32.p);
+    } else {
+        launch_ke_fp32(m, s, w.err.p);
+    }
     AXG_HIP(hipEventRecord(ctx->evA, s));
     launch_gather_bsr32(m, s, m->entry_of_slot.p);
     AXG_HIP(hipEventRecord(ctx->ev1, s));

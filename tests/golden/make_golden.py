@@ -9,7 +9,7 @@ plus the synthetic cantilever of the small benchmark config (20x4x4 and
 20x9x9 C3D8). Expected outputs come from the CPU oracle (oracle/oracle.py),
 whose pinning is documented there; the reference's own known answers for
 these decks (patch u_x = +-0.01, sigma_xx = +-210, LE10 sigma_yy(D) =
--5.51 / -5.50) are asserted in tests/test_oracle_decks.py.
+-5.51 / -5.50) are asserted in tests/test_oracle_linalg_decks.py.
 
 Run from the repo root: python tests/golden/make_golden.py
 (reads /root/reference; the outputs are committed, the GPU box never reads

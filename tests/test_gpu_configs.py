@@ -1,0 +1,208 @@
+"""The benchmark workloads against the oracle (GPU, through the C ABI).
+
+Small instances of each workload are solved by the product in every mode it
+offers for that workload and compared with the oracle's direct FP64 solve
+(reduced triplets -> csr_from_triplets -> scipy SuperLU as the faer stand-in,
+then the oracle's reconstruction and stress recovery): displacements and nodal
+stresses within 1e-8 relative. The load vectors are checked against the oracle's
+own (face pressure bit-exact). Full-size instances are checked through
+size-independent properties: the relative FP64 residual ||f - K u|| / ||f|| of
+every solve on the host from the exported K_ff, and the energy identity
+u^T K u = f^T u.
+
+Workloads (meshgen): the lognormal C3D8 brick with a P2 top pressure and three
+more top-face load cases (elem.rs:892-950, analysis.rs:1365-1450), the jittered
+C3D20 brick with a tip shear (quadratic.rs:100-146), and batches of uneven C3D8
+cubes with four Gaussian top-face load cases each.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import axia_gpu as G
+from axia_gpu import meshgen as MG
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def oracle_cases(p, dloads0=()):
+    """The oracle's linear solve of every load case of a meshgen problem: the setup with
+    the problem's clamped DOFs (and, for case 0, the given face pressures), then per case
+    reduce -> csr_from_triplets -> direct solve -> reconstruction -> stresses."""
+    fixed = np.nonzero(p.ind_of < 0)[0]
+    bcs = [(int(d // 3), int(d % 3), 0.0) for d in fixed]
+    s0 = O.linear_setup(p.nn, p.xyz, p.conn, p.e, p.nu, bcs, [], list(dloads0))
+    assert np.array_equal(s0.ind_of, p.ind_of)
+    out = []
+    for k, f in enumerate(p.f_full):
+        if k == 0 and dloads0:
+            assert np.array_equal(s0.f_full, f), "face-pressure loads differ from the oracle"
+        s = dataclasses.replace(s0, f_full=np.array(f, np.float64))
+        out.append(O.linear_solve(s, solver="superlu"))
+    return out
+
+
+def check_against_oracle(mesh, p, refs, X, what, tol=1e-8):
+    for k, ref in enumerate(refs):
+        ex = rel(X[k], ref.x)
+        st, _, _ = mesh.nodal_stress(p.expand(X[k]))
+        es = rel(st, ref.stress)
+        say(f"  {what} case {k}: u err {ex:.2e}, stress err {es:.2e}")
+        assert ex < tol and es < tol
+
+
+def host_properties(mesh, B, X, tol_res):
+    """Relative FP64 residual of every column on the host and the energy identity."""
+    indptr, indices, data = mesh.csr()
+    n = len(indptr) - 1
+    K = sp.csr_matrix((data, indices, indptr), shape=(n, n))
+    worst = 0.0
+    for k in range(B.shape[0]):
+        Ku = K @ X[k]
+        r = np.linalg.norm(B[k] - Ku) / np.linalg.norm(B[k])
+        energy = abs(X[k] @ Ku - B[k] @ X[k]) / abs(B[k] @ X[k])
+        worst = max(worst, r)
+        assert r <= tol_res, (k, r)
+        assert energy < 1e-9, (k, energy)
+    return worst
+
+
+# ------------------------------------------------------------------ C3D8 lognormal brick
+@pytest.fixture(scope="module")
+def brick12(ctx):
+    p = MG.brick_lognormal(12, seed=5, nloads=4)
+    top = np.nonzero(p.xyz[p.conn[:, 4], 2] == 12.0)[0]
+    refs = oracle_cases(p, [(int(el), 2, 1.0) for el in top])
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble("mixed")
+    yield p, refs, mesh
+    mesh.close()
+
+
+@pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("fp64", "block3"),
+                                               ("mixed", "block3"), ("mixed", "jacobi")])
+def test_lognormal_brick_vs_oracle(brick12, precision, precond):
+    """C3D8 12^3 lognormal brick, P2 = 1 MPa top pressure plus x/y shear and a random
+    normal top load: all four load cases in one solve."""
+    say("case: lognormal brick 12^3", precision, precond)
+    p, refs, mesh = brick12
+    B = p.rhs_all()
+    X, info = mesh.solve(B, precision=precision, precond=precond, tol=1e-12)
+    say(f"  {info['name']}: iters {info['iters']}, rel.res {info['rel_residual']:.2e}")
+    assert info["nrhs"] == 4 and info["rel_residual"] <= 1e-11
+    check_against_oracle(mesh, p, refs, X, f"{precision}/{precond}")
+
+
+def test_lognormal_brick80_mixed_properties(ctx):
+    """The benchmark's 80^3 brick (1.57M DOFs), mixed precision, four load cases at once:
+    host residual from the exported K_ff and the energy identity for every case."""
+    say("case: lognormal brick 80^3 mixed, 4 load cases")
+    p = MG.brick_lognormal(80, seed=0, nloads=4)
+    mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble("mixed")
+    B = p.rhs_all()
+    X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-11)
+    say(f"  {info['name']}: inner {info['iters']}, sweeps {info['outer_iters']}, "
+        f"{info['solve_ms']:.0f} ms, rel.res {info['rel_residual']:.2e}")
+    worst = host_properties(mesh, B, X, 1e-10)
+    say(f"  host residual (worst case) {worst:.2e}")
+    # the FP64 solve of the same operator agrees
+    X64, _ = mesh.solve(B, precision="fp64", tol=1e-11)
+    for k in range(4):
+        assert rel(X[k], X64[k]) < 1e-8
+    mesh.close()
+
+
+# ------------------------------------------------------------------ C3D20 jittered brick
+@pytest.fixture(scope="module")
+def c3d20_small(ctx):
+    p = MG.brick_c3d20_jitter(8, 2, 2, seed=11)
+    refs = oracle_cases(p)
+    mesh = G.Mesh(20, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble("mixed")
+    yield p, refs, mesh
+    mesh.close()
+
+
+@pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("fp64", "block3"),
+                                               ("mixed", "block3"), ("mixed", "jacobi")])
+def test_jittered_c3d20_vs_oracle(c3d20_small, precision, precond):
+    """C3D20 8 x 2 x 2, interior corner nodes jittered by U(-0.15, 0.15) h, tip shear."""
+    say("case: jittered C3D20 8x2x2", precision, precond)
+    p, refs, mesh = c3d20_small
+    x, info = mesh.solve(p.rhs(), precision=precision, precond=precond, tol=1e-12)
+    say(f"  {info['name']}: iters {info['iters']}, rel.res {info['rel_residual']:.2e}")
+    check_against_oracle(mesh, p, refs, [x], f"{precision}/{precond}")
+
+
+def test_jittered_c3d20_full_properties(ctx):
+    """The benchmark's C3D20 96 x 24 x 24 (705,600 DOFs): FP64 block-Jacobi and mixed
+    block-Jacobi, host residual and energy identity, and the two agree."""
+    say("case: jittered C3D20 96x24x24")
+    p = MG.brick_c3d20_jitter(96, 24, 24, seed=0)
+    mesh = G.Mesh(20, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+    mesh.assemble("mixed")
+    B = p.rhs_all()
+    X64, i64 = mesh.solve(B, precision="fp64", precond="block3", tol=1e-9)
+    X32, i32 = mesh.solve(B, precision="mixed", precond="block3", tol=1e-9)
+    say(f"  fp64 {i64['iters']} it {i64['solve_ms']:.0f} ms; mixed {i32['iters']} inner "
+        f"{i32['solve_ms']:.0f} ms")
+    host_properties(mesh, B, X64, 1e-8)
+    host_properties(mesh, B, X32, 1e-8)
+    assert rel(X32[0], X64[0]) < 1e-7
+    mesh.close()
+
+
+# ------------------------------------------------------------------ batched cubes
+def test_batch_small_vs_oracle(ctx):
+    """Six uneven cubes (4^3 .. 8^3), four Gaussian top-face load cases each, one batched
+    solve; every (mesh, load case) against the oracle's direct solve."""
+    say("case: batch of 6 cubes vs oracle")
+    probs = MG.batch_meshes(count=6, lo=4, hi=8, nloads=4, seed=21)
+    meshes = []
+    for p in probs:
+        m = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+        m.assemble("fp64")
+        meshes.append(m)
+    xs, iters, lat, info = G.solve_batch(meshes, [p.rhs_all() for p in probs], tol=1e-12)
+    say(f"  {info['name']}: {len(iters)} solves, {info['solve_ms']:.1f} ms")
+    for p, m, X in zip(probs, meshes, xs):
+        check_against_oracle(m, p, oracle_cases(p), X, p.name)
+    assert (iters > 0).all() and (lat > 0).all()
+    for m in meshes:
+        m.close()
+
+
+def test_batch48_full_properties(ctx):
+    """The benchmark's 48 cubes (8^3 .. 48^3, 5.5M DOFs, 192 solves) in one batched solve:
+    host residual and energy identity of every solve; p50 <= p99 latency."""
+    say("case: batch of 48 cubes")
+    probs = MG.batch_meshes(count=48, lo=8, hi=48, nloads=4, seed=0)
+    meshes = []
+    for p in probs:
+        m = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+        m.assemble("fp64")
+        meshes.append(m)
+    say(f"  meshes built: {sum(p.n_free for p in probs)} DOFs")
+    rhs = [p.rhs_all() for p in probs]
+    xs, iters, lat, info = G.solve_batch(meshes, rhs, tol=1e-10)
+    say(f"  {len(iters)} solves in {info['solve_ms']:.0f} ms, p50 {np.percentile(lat, 50):.0f} ms,"
+        f" p99 {np.percentile(lat, 99):.0f} ms")
+    worst = 0.0
+    for k, (m, B, X) in enumerate(zip(meshes, rhs, xs)):
+        worst = max(worst, host_properties(m, B, X, 1e-9))
+        m.close()
+    say(f"  host residual (worst solve) {worst:.2e}")

@@ -169,6 +169,48 @@ def test_solve_kff_dropin(ctx):
     assert r.iters > 0 and r.residual < 1e-10 * np.linalg.norm(rhs)
 
 
+@pytest.mark.parametrize("precision,precond", [("fp64", "block3"), ("mixed", "jacobi"),
+                                               ("mixed", "block3")])
+def test_solve_kff_dropin_options(ctx, precision, precond):
+    """The drop-in honours precision and preconditioner: LE10's reduced system (partly
+    constrained nodes, so row triples straddle nodes) against the oracle's direct solve."""
+    say("case: solve_kff", precision, precond)
+    m, s = deck("le10_c3d20")
+    oi, oj, ov, rhs = O.reduce(s.ind_of, s.u0, s.n_free, *s.trips, s.f_full)
+    g = np.load(os.path.join(GOLD, "le10_c3d20.npz"))
+    r = G.solve_kff(s.n_free, (oi, oj, ov), rhs, ctx=ctx, tol=1e-12, precision=precision,
+                    precond=precond)
+    say(f"  {r.name}: iters {r.iters}, err {rel(r.x, g['x']):.2e}")
+    assert ("FP32" in r.name) == (precision == "mixed")
+    assert ("block-Jacobi" in r.name) == (precond == "block3")
+    assert rel(r.x, g["x"]) < 1e-8
+    assert r.residual <= 1e-11 * np.linalg.norm(rhs)
+
+
+@pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("fp64", "block3"),
+                                               ("mixed", "block3")])
+def test_solve_csr_generic_spd(ctx, precision, precond):
+    """A caller's CSR without node structure (n = 301, not a multiple of 3, rows of
+    different patterns) through axg_solve_csr in each mode, against LAPACK."""
+    say("case: solve_csr", precision, precond)
+    rng = np.random.default_rng(12)
+    n = 301
+    a = np.zeros((n, n))
+    for off in (1, 2, 5, 17):
+        v = rng.uniform(-1.0, 1.0, n - off) * (rng.random(n - off) < 0.7)
+        a += np.diag(v, off) + np.diag(v, -off)
+    a += np.diag(np.abs(a).sum(1) + rng.uniform(0.5, 2.0, n))
+    import scipy.sparse as sps
+    csr = sps.csr_matrix(a)
+    b = rng.standard_normal(n)
+    x, info = G.solve_csr(csr.indptr, csr.indices, csr.data, b, ctx=ctx, tol=1e-12,
+                          precision=precision, precond=precond)
+    ref = np.linalg.solve(a, b)
+    say(f"  {info['name']}: iters {info['iters']}, err {rel(x, ref):.2e}")
+    assert rel(x, ref) < 1e-10
+    assert info["rel_residual"] <= 1e-11
+
+
 def test_csr_from_triplets_device_bitexact(ctx):
     say("case: csr_from_triplets")
     rng = np.random.default_rng(3)

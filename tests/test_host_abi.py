@@ -131,3 +131,121 @@ def test_meshgen_counts_of_the_benchmark_shapes():
     p = MG.cantilever(20, 9, 9)
     assert p.nnode == 2100 and p.n_free == 6000
     assert abs(p.f_full.__getitem__(0).sum() + 1000.0) < 1e-9
+
+
+# ------------------------------------------------- struct layouts across the ABI
+RUST_SIZES = {"i32": (4, 4), "i64": (8, 8), "f64": (8, 8), "[c_char; 64]": (64, 1)}
+
+
+def rust_struct_fields(name):
+    """Fields of a #[repr(C)] struct of the Rust binding in INTEGRATION.md."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = re.search(r"pub struct " + name + r" \{(.*?)\n\}", text, re.S).group(1)
+    return re.findall(r"pub (\w+): ([^,]+),", body)
+
+
+def repr_c_layout(fields):
+    """Offsets and size by the C layout rules #[repr(C)] follows."""
+    off, align, out = 0, 1, {}
+    for f, t in fields:
+        size, al = RUST_SIZES[t.strip()]
+        off = (off + al - 1) // al * al
+        out[f] = off
+        off += size
+        align = max(align, al)
+    return out, (off + align - 1) // align * align
+
+
+def c_layout(struct, fields):
+    """sizeof / offsetof of the C header's struct, from a program compiled with gcc."""
+    import subprocess
+    import tempfile
+    lines = [f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    src = ("#include <stddef.h>\n#include <stdio.h>\n#include \"axia_gpu.h\"\nint main(void) {\n"
+           + "\n".join(lines) + f'\nprintf("%zu\\n", sizeof({struct}));\nreturn 0;\n}}\n')
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe],
+                       check=True)
+        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                                check=True).stdout.split()]
+    return dict(zip(fields, vals[:-1])), vals[-1]
+
+
+@pytest.mark.parametrize("c_name,rust_name,py", [("axg_solve_opts", "AxgSolveOpts", "SolveOpts"),
+                                                 ("axg_solve_info", "AxgSolveInfo", "SolveInfo")])
+def test_struct_layout_matches_rust_and_ctypes_mirrors(c_name, rust_name, py):
+    """The C header's structs, the Rust #[repr(C)] mirror a maintainer adds and the ctypes
+    mirror of the Python binding agree field for field in order, offset and size."""
+    rust = rust_struct_fields(rust_name)
+    names = [f for f, _ in rust]
+    cls = getattr(_lib, py)
+    assert names == [f for f, _ in cls._fields_]
+    c_off, c_size = c_layout(c_name, names)
+    r_off, r_size = repr_c_layout(rust)
+    assert c_off == r_off and c_size == r_size
+    import ctypes
+    assert {f: getattr(cls, f).offset for f in names} == c_off
+    assert ctypes.sizeof(cls) == c_size
+
+
+# ------------------------------------------------- argument checks before the device
+def test_solve_csr_rejects_malformed_csr_before_any_device_work():
+    """axg_solve_csr validates the host CSR and the options before it touches the device:
+    the handle below is never dereferenced on these paths."""
+    import ctypes as C
+    L = _lib.lib()
+    fake = C.c_void_p(1)
+    o = _lib.SolveOpts()
+    L.axg_default_opts(C.byref(o))
+    x = np.zeros(3)
+    rhs = np.ones(3)
+    info = _lib.SolveInfo()
+
+    def call(indptr, indices, data, opts=o):
+        ip = np.asarray(indptr, np.int64)
+        ix = None if indices is None else np.asarray(indices, np.int64)
+        dv = None if data is None else np.asarray(data, np.float64)
+        return L.axg_solve_csr(fake, 3, _lib.lptr(ip), _lib.lptr(ix) if ix is not None else None,
+                               _lib.dptr(dv) if dv is not None else None, _lib.dptr(rhs),
+                               C.byref(opts), _lib.dptr(x), C.byref(info))
+
+    cases = [([1, 1, 2, 3], [0, 1, 2], [1.0, 1.0, 1.0], "indptr[0]"),
+             ([0, 2, 1, 3], [0, 1, 2], [1.0, 1.0, 1.0], "non-decreasing"),
+             ([0, 1, 2, 3], None, None, "NULL indices"),
+             ([0, 1, 2, 3], [0, 1, 5], [1.0, 1.0, 1.0], "column index")]
+    for ip, ix, dv, msg in cases:
+        assert call(ip, ix, dv) == _lib.AXG_E_INVALID
+        assert msg in L.axg_last_error().decode()
+    bad = _lib.SolveOpts()
+    L.axg_default_opts(C.byref(bad))
+    bad.precision = 7
+    assert call([0, 1, 2, 3], [0, 1, 2], [1.0, 1.0, 1.0], bad) == _lib.AXG_E_INVALID
+    assert "unknown precision" in L.axg_last_error().decode()
+    bad.precision, bad.precond = 0, 3
+    assert call([0, 1, 2, 3], [0, 1, 2], [1.0, 1.0, 1.0], bad) == _lib.AXG_E_INVALID
+    assert "unknown preconditioner" in L.axg_last_error().decode()
+
+
+def test_python_binding_checks_sizes_before_the_call():
+    """Mesh methods raise ValueError for arrays of the wrong length instead of letting the
+    library read or write past them (the checks run before any library call)."""
+    m = G.Mesh.__new__(G.Mesh)
+    m._h, m.nn, m.nnode, m.nelem, m._n_free = None, 8, 10, 2, 27
+    with pytest.raises(ValueError):
+        m.solve(np.zeros(26))
+    with pytest.raises(ValueError):
+        m.solve(np.zeros((2, 28)))
+    for call in (lambda: m.nodal_stress(np.zeros(29)),
+                 lambda: m.reactions(np.zeros(30), np.zeros(31)),
+                 lambda: m.reduce_rhs(np.zeros(30), np.zeros(29)),
+                 lambda: m.compliance_adjoint(np.zeros(30), np.zeros(3)),
+                 lambda: m.nl_assemble(np.zeros(3)),
+                 lambda: m.geometric_stiffness(np.zeros(31))):
+        with pytest.raises(ValueError):
+            call()
+    with pytest.raises(ValueError):
+        G.solve_kff(3, ([0], [0], [1.0]), np.ones(4), ctx=object())
+    with pytest.raises(ValueError):
+        G.solve_batch([m], [np.zeros((1, 26))], ctx=object())
