@@ -21,12 +21,12 @@ import numpy as np
 
 from . import _lib
 from ._lib import (AxgError, PREC_FP64, PREC_MIXED, PRECOND_BLOCK3, PRECOND_JACOBI, REFINE_FCG,
-                   REFINE_PLAIN, check, dptr, f64, i32, i64, iptr, lptr)
+                   REFINE_PLAIN, REFINE_RELIABLE, check, dptr, f64, i32, i64, iptr, lptr)
 
 __all__ = [
     "AxgError", "Context", "Mesh", "SparseResult", "solve_kff", "solve_csr", "csr_from_triplets",
     "pattern_host", "pressure_loads", "default_context", "PREC_FP64", "PREC_MIXED",
-    "PRECOND_JACOBI", "PRECOND_BLOCK3", "REFINE_FCG", "REFINE_PLAIN",
+    "PRECOND_JACOBI", "PRECOND_BLOCK3", "REFINE_FCG", "REFINE_PLAIN", "REFINE_RELIABLE",
 ]
 
 
@@ -50,7 +50,8 @@ def _opts(precision=PREC_FP64, precond=PRECOND_JACOBI, tol=1e-8, max_iter=0, inn
     o.inner_tol = inner_tol
     o.max_outer = max_outer
     o.timing = 1 if timing else 0
-    o.refinement = {"fcg": REFINE_FCG, "plain": REFINE_PLAIN}.get(refinement, refinement)
+    o.refinement = {"fcg": REFINE_FCG, "plain": REFINE_PLAIN,
+                    "reliable": REFINE_RELIABLE}.get(refinement, refinement)
     return o
 
 

@@ -31,6 +31,7 @@ PRECOND_JACOBI = 0
 PRECOND_BLOCK3 = 1
 REFINE_FCG = 0
 REFINE_PLAIN = 1
+REFINE_RELIABLE = 2
 
 
 class SolveOpts(C.Structure):

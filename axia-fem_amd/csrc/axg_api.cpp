@@ -432,7 +432,9 @@ static void check_solve_opts(const axg_solve_opts &o, const char *what) {
                 std::string(what) + ": unknown precision");
     AXG_REQUIRE(o.precond == AXG_PRECOND_JACOBI || o.precond == AXG_PRECOND_BLOCK3, AXG_E_INVALID,
                 std::string(what) + ": unknown preconditioner");
-    AXG_REQUIRE(o.refinement == AXG_REFINE_FCG || o.refinement == AXG_REFINE_PLAIN, AXG_E_INVALID,
+    AXG_REQUIRE(o.refinement == AXG_REFINE_FCG || o.refinement == AXG_REFINE_PLAIN ||
+                    o.refinement == AXG_REFINE_RELIABLE,
+                AXG_E_INVALID,
                 std::string(what) + ": unknown refinement");
     AXG_REQUIRE(o.tol > 0.0 && std::isfinite(o.tol), AXG_E_INVALID,
                 std::string(what) + ": tol must be positive");
@@ -787,7 +789,9 @@ static void mesh_solve_device(axg_mesh *mesh, const double *d_rhs, int32_t nrhs,
         }
         char name[64];
         std::snprintf(name, sizeof name, "GPU PCG (FP32 BSR-3 + FP64 %s, %s)",
-                      o.refinement == AXG_REFINE_PLAIN ? "IR" : "FCG",
+                      o.refinement == AXG_REFINE_PLAIN      ? "IR"
+                      : o.refinement == AXG_REFINE_RELIABLE ? "RU"
+                                                            : "FCG",
                       o.precond == AXG_PRECOND_BLOCK3 ? "block-Jacobi" : "Jacobi");
         fill_info(info, name, mr.inner_iters, mr.outer_iters, mr.residual, mr.bnorm, mr.solve_ms,
                   mr.spmv_ms, mr.spmv_launches, nrhs);

@@ -12,6 +12,8 @@
 // so consecutive lanes read consecutive 16-byte words.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -409,7 +411,7 @@ struct InnerScalars {
 // wavefront): block j of every row is one coalesced load of two float4 planes
 // and one float plane; x of the column block (3 K floats) is gathered through
 // the caches. U blocks per lane are loaded before their x gathers.
-template <int K, int U, bool DOT>
+template <int K, int U, bool DOT, bool GF = false>
 __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
     int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
     const float *__restrict__ bval, int64_t nsl, const float *__restrict__ P,
@@ -481,17 +483,64 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
             }
         };
         if (nfull > 0) load_batch(0, va, vb, vc, J);
-        for (int b = 0; b < nfull; b++) {
-            f4v na[U], nbv[U];
-            float nc[U];
-            int32_t nj[U];
-            const bool more = b + 1 < nfull;
-            if (more) load_batch(b + 1, na, nbv, nc, nj);
+        if (GF) {
+            // the batch's x gathers are issued before the next batch's values:
+            // loads return in issue order, so waiting for the gathers then does
+            // not wait for the value stream behind them
+            auto gather = [&](float (&xg)[U][3][K]) {
 #pragma unroll
-            for (int u = 0; u < U; u++) fma_block(va[u], vb[u], vc[u], J[u]);
-            if (more) {
+                for (int u = 0; u < U; u++) {
+                    const float *x = P + (int64_t)3 * J[u] * K;
+                    load_k<K>(x, xg[u][0]);
+                    load_k<K>(x + K, xg[u][1]);
+                    load_k<K>(x + 2 * K, xg[u][2]);
+                }
+            };
+            auto fma_x = [&](const float (&xg)[U][3][K]) {
+#pragma unroll
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int j = 0; j < K; j++) {
+                        acc[0][j] += va[u].x * xg[u][0][j] + va[u].y * xg[u][1][j] +
+                                     va[u].z * xg[u][2][j];
+                        acc[1][j] += va[u].w * xg[u][0][j] + vb[u].x * xg[u][1][j] +
+                                     vb[u].y * xg[u][2][j];
+                        acc[2][j] += vb[u].z * xg[u][0][j] + vb[u].w * xg[u][1][j] +
+                                     vc[u] * xg[u][2][j];
+                    }
+            };
+            for (int b = 0; b + 1 < nfull; b++) {
+                float xg[U][3][K];
+                gather(xg);
+                __builtin_amdgcn_sched_barrier(0);
+                f4v na[U], nbv[U];
+                float nc[U];
+                int32_t nj[U];
+                load_batch(b + 1, na, nbv, nc, nj);
+                __builtin_amdgcn_sched_barrier(0);
+                fma_x(xg);
 #pragma unroll
                 for (int u = 0; u < U; u++) va[u] = na[u], vb[u] = nbv[u], vc[u] = nc[u], J[u] = nj[u];
+            }
+            if (nfull > 0) {
+                float xg[U][3][K];
+                gather(xg);
+                fma_x(xg);
+            }
+        } else {
+            for (int b = 0; b < nfull; b++) {
+                f4v na[U], nbv[U];
+                float nc[U];
+                int32_t nj[U];
+                const bool more = b + 1 < nfull;
+                if (more) load_batch(b + 1, na, nbv, nc, nj);
+#pragma unroll
+                for (int u = 0; u < U; u++) fma_block(va[u], vb[u], vc[u], J[u]);
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        va[u] = na[u], vb[u] = nbv[u], vc[u] = nc[u], J[u] = nj[u];
+                }
             }
         }
         int jb = 1 + nfull * U;
@@ -575,10 +624,14 @@ __global__ void __launch_bounds__(FIN_TB) fin_inner_update_kernel(const double *
     sc->all_done = all;
 }
 
+// rz_prev (may be null): a restart that keeps the search direction (reliable
+// update) passes the last r.z of the previous interval in the new scaling, and
+// beta_j = r.z / rz_prev_j joins the new residual to the kept direction.
 template <int K>
 __global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__restrict__ part,
                                                             int nblk, InnerScalars *sc,
-                                                            double inner_tol) {
+                                                            double inner_tol,
+                                                            const double *__restrict__ rz_prev) {
     double tot[2 * K];
     fin_sums<2 * K>(part, nblk, tot);
     if (threadIdx.x == 0) {
@@ -588,6 +641,7 @@ __global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__
             sc->bnorm[j] = bn;
             sc->res[j] = bn;
             sc->tol_abs[j] = inner_tol * bn;
+            if (rz_prev) sc->beta[j] = rz_prev[j] > 0.0 ? tot[K + j] / rz_prev[j] : 0.0;
             sc->rz[j] = tot[K + j];
             sc->iters[j] = 0;
             sc->done[j] = (bn < 1e-30) ? 1 : 0;
@@ -677,16 +731,22 @@ __global__ void __launch_bounds__(MBS) inner_update_p_kernel(int64_t nv, float *
     }
 }
 
-// Inner start: X = 0, R = B, Z = M R, norms; the last workgroup sets
-// tol_abs = inner_tol |B_j|, rz and the zero-rhs exits.
+// Inner start: X = 0, R = B, Z = M R, norms; the finalize sets
+// tol_abs = inner_tol |B_j|, rz and the zero-rhs exits. With ratio (a restart
+// that keeps the search direction) P is rescaled to the new residual scale
+// (P *= ratio_j, and set to 0 where ratio_j is 0).
 template <int K, bool BLOCK>
 __global__ void __launch_bounds__(MBS) inner_init_kernel(
     int64_t nb, const float *__restrict__ B, float *__restrict__ X, float *__restrict__ R,
     float *__restrict__ Z, const float *__restrict__ binv, const float *__restrict__ dinv,
-    double *__restrict__ part, InnerScalars *sc, double inner_tol) {
+    double *__restrict__ part, InnerScalars *sc, double inner_tol, float *__restrict__ P,
+    const double *__restrict__ ratio) {
     double acc[2 * K];
 #pragma unroll
     for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
+    float rat[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) rat[j] = ratio ? (float)ratio[j] : 0.f;
     for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
          b += (int64_t)gridDim.x * MBS) {
         float r[3][K], zero[K];
@@ -697,6 +757,13 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
             load_k<K>(B + (3 * b + d) * K, r[d]);
             store_k<K>(R + (3 * b + d) * K, r[d]);
             store_k<K>(X + (3 * b + d) * K, zero);
+            if (ratio) {
+                float p[K];
+                load_k<K>(P + (3 * b + d) * K, p);
+#pragma unroll
+                for (int j = 0; j < K; j++) p[j] = rat[j] > 0.f ? p[j] * rat[j] : 0.f;
+                store_k<K>(P + (3 * b + d) * K, p);
+            }
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -749,14 +816,19 @@ __global__ void unpack_add_kernel(int64_t n, const float *__restrict__ D,
 // ----------------------------------------------------- SpMM shapes
 // U (blocks loaded per lane before their gathers) is picked per operator and
 // right-hand-side count by timing the candidates.
-static const int kSpmmU[] = {1, 2, 3, 4};
-static const int kNumSpmm = 4;
-#define AXG_SPMM_SWITCH(IDX, MACRO) \
-    switch (IDX) {                  \
-        case 0: MACRO(1); break;    \
-        case 1: MACRO(2); break;    \
-        case 2: MACRO(3); break;    \
-        default: MACRO(4); break;   \
+// Shapes 0-3: U = 1..4 blocks per batch, gathers after the next batch's
+// values; shapes 4-7: the same with the gathers issued first.
+static const int kNumSpmm = 8;
+#define AXG_SPMM_SWITCH(IDX, MACRO)     \
+    switch (IDX) {                      \
+        case 0: MACRO(1, false); break; \
+        case 1: MACRO(2, false); break; \
+        case 2: MACRO(3, false); break; \
+        case 3: MACRO(4, false); break; \
+        case 4: MACRO(1, true); break;  \
+        case 5: MACRO(2, true); break;  \
+        case 6: MACRO(3, true); break;  \
+        default: MACRO(4, true); break; \
     }
 
 // Workgroups of one SpMM launch: the same number on every CU (4, or fewer
@@ -766,8 +838,8 @@ static const int kNumSpmm = 4;
 template <int K, bool DOT>
 static int spmm_grid(axg_ctx *ctx, int64_t nb, int shape) {
     int per_cu = 1;
-#define AXG_OCC(UU) \
-    per_cu = resident_per_cu(reinterpret_cast<const void *>(bsell_spmm_kernel<K, UU, DOT>), MBS)
+#define AXG_OCC(UU, GG) \
+    per_cu = resident_per_cu(reinterpret_cast<const void *>(bsell_spmm_kernel<K, UU, DOT, GG>), MBS)
     AXG_SPMM_SWITCH(shape, AXG_OCC)
 #undef AXG_OCC
     return std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
@@ -784,8 +856,8 @@ static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
         float best_ms = 1e30f;
         for (int c = 0; c < kNumSpmm; c++) {
             const int grid = spmm_grid<K, false>(ctx, m->nb, c);
-#define AXG_T(UU)                                                                                \
-    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
+#define AXG_T(UU, GG)                                                                            \
+    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false, GG>), dim3(grid), dim3(MBS), 0, s, m->nb, \
                        m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots, P, AP,  \
                        nullptr, nullptr)
             AXG_SPMM_SWITCH(c, AXG_T)
@@ -798,6 +870,8 @@ static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
             AXG_HIP(hipEventSynchronize(ctx->ev1));
             float ms = 0.f;
             AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (std::getenv("AXG_TUNE_LOG"))
+                std::fprintf(stderr, "spmm K=%d shape %d: %.4f ms\n", K, c, ms / 5);
             if (ms < best_ms) best_ms = ms, best = c;
         }
         AXG_HIP(hipGetLastError());
@@ -897,8 +971,8 @@ static void bench_spmm_k(axg_mesh *m, int reps, double *avg_ms) {
     AXG_HIP(hipMemsetAsync(P.p, 0, 4 * nv * K, s));
     const int shp = tune_spmm<K>(m, P.p, AP.p);
     const int grid = spmm_grid<K, false>(ctx, m->nb, shp);
-#define AXG_T(UU)                                                                                \
-    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false>), dim3(grid), dim3(MBS), 0, s, m->nb,     \
+#define AXG_T(UU, GG)                                                                            \
+    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false, GG>), dim3(grid), dim3(MBS), 0, s, m->nb, \
                        m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots, P.p,    \
                        AP.p, nullptr, nullptr)
     AXG_SPMM_SWITCH(shp, AXG_T)
@@ -947,7 +1021,7 @@ struct MixedSolver {
         w.f4.ensure(nv * K);  // AP
         DevBuf<float> Z(nv * K);
         w.t0.ensure(n * K);   // r64
-        w.t1.ensure(32);      // scales, alpha, beta
+        w.t1.ensure(48);      // scales, alpha, beta, restart ratio and r.z
         const int shp = tune_spmm<K>(m, w.f3.p, w.f4.p);
         const int grid_mv = spmm_grid<K, true>(ctx, nb, shp);
         const int grid_v = std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
@@ -967,14 +1041,14 @@ struct MixedSolver {
         // timing: the events are stamped by the dispatch itself (kernel start
         // and end), not by separate stream markers
         auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1) {
-#define AXG_SPMM(UU)                                                                            \
+#define AXG_SPMM(UU, GG)                                                                        \
     if (e0)                                                                                     \
-        hipExtLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s,  \
+        hipExtLaunchKernelGGL((bsell_spmm_kernel<K, UU, true, GG>), dim3(grid_mv), dim3(MBS), 0, s, \
                               e0, e1, 0, nb, (const int64_t *)m->bs_sptr.p,                     \
                               (const int32_t *)m->bs_width.p, (const float *)m->bval.p,          \
                               m->bs_nslots, (const float *)w.f3.p, w.f4.p, w.part.p, dsc.p);    \
     else                                                                                        \
-        hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true>), dim3(grid_mv), dim3(MBS), 0, s, nb,  \
+        hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true, GG>), dim3(grid_mv), dim3(MBS), 0, s, nb, \
                            m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots,    \
                            w.f3.p, w.f4.p, w.part.p, dsc.p)
             AXG_SPMM_SWITCH(shp, AXG_SPMM)
@@ -1054,26 +1128,40 @@ struct MixedSolver {
                 ctx->timing_events.push_back(e);
             }
         }
-        // FP32 inner PCG on A32 X = R / |R| (columns with scale 0 stay X = 0)
-        auto inner_solve = [&](const double (&scale)[K]) {
+        // FP32 inner PCG on A32 X = R / |R| (columns with scale 0 stay X = 0).
+        // warm: the search direction of the previous interval runs on (its scale
+        // was prev_scale), with beta from the previous interval's last r.z.
+        double prev_scale[K], prev_rz[K];
+        auto inner_solve = [&](const double (&scale)[K], bool warm) {
             put(8, scale);
             hipLaunchKernelGGL((pack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.t0.p,
                                m->blk_of_free.p, w.t1.p + 8, w.f0.p);
+            if (warm) {
+                double ratio[K], rzs[K];
+                for (int j = 0; j < K; j++) {
+                    ratio[j] = (prev_scale[j] > 0.0 && scale[j] > 0.0) ? prev_scale[j] / scale[j]
+                                                                       : 0.0;
+                    rzs[j] = prev_rz[j] * ratio[j] * ratio[j];
+                }
+                put(32, ratio);
+                put(40, rzs);
+            }
             std::memset(h, 0, sizeof(InnerScalars));
             h->max_iter = (int32_t)std::min<int64_t>(inner_cap - total_inner, INT32_MAX);
             AXG_HIP(hipMemcpyAsync(dsc.p, h, sizeof(InnerScalars), hipMemcpyHostToDevice, s));
+            const double *ratio_p = warm ? w.t1.p + 32 : nullptr;
             if (block)
                 hipLaunchKernelGGL((inner_init_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
                                    w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p, w.part.p,
-                                   dsc.p, o.inner_tol);
+                                   dsc.p, o.inner_tol, w.f3.p, ratio_p);
             else
                 hipLaunchKernelGGL((inner_init_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
                                    nb, w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p,
-                                   w.part.p, dsc.p, o.inner_tol);
+                                   w.part.p, dsc.p, o.inner_tol, w.f3.p, ratio_p);
             hipLaunchKernelGGL((fin_inner_init_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
-                               grid_v, dsc.p, o.inner_tol);
+                               grid_v, dsc.p, o.inner_tol, warm ? w.t1.p + 40 : nullptr);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
-                               w.f3.p, Z.p, dsc.p, 1);
+                               w.f3.p, Z.p, dsc.p, warm ? 0 : 1);
             AXG_HIP(hipGetLastError());
             int32_t prev_it[K];
             for (int j = 0; j < K; j++) prev_it[j] = 0;
@@ -1106,6 +1194,7 @@ struct MixedSolver {
             int32_t mx = 0;
             for (int j = 0; j < K; j++) mx = std::max(mx, h->iters[j]);
             total_inner += mx;
+            for (int j = 0; j < K; j++) prev_scale[j] = scale[j], prev_rz[j] = h->rz[j];
         };
         // the FP64 residual of a stiff problem has a round-off floor: stop once
         // two true residuals in a row improved no column by half
@@ -1142,7 +1231,27 @@ struct MixedSolver {
                     break;
                 }
                 sweeps++;
-                inner_solve(scale);
+                inner_solve(scale, false);
+                hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
+                                   w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u);
+                true_residual_check();
+            }
+        } else if (o.refinement == AXG_REFINE_RELIABLE) {
+            // one FP32 CG across the FP64 updates: each interval runs until its
+            // residual has dropped by inner_tol, then u += |r| X in FP64, the FP32
+            // residual is replaced by (f - K u) / |f - K u| and the CG continues
+            // with its search direction rescaled instead of restarting
+            bool warm = false;
+            for (;;) {
+                double scale[K];
+                if (converged(scale)) break;
+                if (sweeps >= o.max_outer || total_inner >= inner_cap || stalls >= 2) {
+                    status = AXG_E_NOT_CONVERGED;
+                    break;
+                }
+                sweeps++;
+                inner_solve(scale, warm);
+                warm = true;
                 hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
                                    w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u);
                 true_residual_check();
@@ -1168,7 +1277,7 @@ struct MixedSolver {
                     break;
                 }
                 sweeps++;
-                inner_solve(scale);
+                inner_solve(scale, false);
                 hipLaunchKernelGGL((unpack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.f1.p,
                                    m->blk_of_free.p, w.t1.p + 8, Zd.p);
                 double beta[K];

@@ -251,7 +251,7 @@ def run_brick(a, G, MG, torch, np):
         t = mesh.assembly_times()
         inf = mesh.solve_device(B.data_ptr(), k, X.data_ptr(), precision="mixed",
                                 precond=a.precond, tol=a.tol, inner_tol=a.inner_tol,
-                                timing=timing)
+                                timing=timing, refinement=a.refinement)
         return ms_asm, t, inf
 
     for _ in range(a.warmup):
@@ -283,7 +283,7 @@ def run_brick(a, G, MG, torch, np):
     nelem = prob.nelem
     # the same solve replayed as hipGraphs without per-launch events (untimed)
     g = mesh.solve_device(B.data_ptr(), k, X.data_ptr(), precision="mixed", precond=a.precond,
-                          tol=a.tol, inner_tol=a.inner_tol)
+                          tol=a.tol, inner_tol=a.inner_tol, refinement=a.refinement)
 
     fp64 = plain = None
     indptr = indices = data = None
@@ -346,7 +346,7 @@ def run_brick(a, G, MG, torch, np):
         "workload": f"C3D8 brick {a.n}^3, lognormal E, bottom clamped, {k} top-face load "
                     f"cases, mixed-precision multi-RHS PCG to rel. residual {a.tol:g}",
         "elements": nelem, "n_free": n, "nnz": nnz, "nrhs": k, "precond": a.precond,
-        "inner_tol": a.inner_tol,
+        "inner_tol": a.inner_tol, "refinement": a.refinement,
     })
     out.update({
         "mixed": {
@@ -1027,6 +1027,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-11)
     ap.add_argument("--precond", default="block3")
     ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--refinement", default="fcg", choices=("fcg", "plain", "reliable"))
     ap.add_argument("--fd-elements", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

@@ -54,8 +54,12 @@ enum axg_precond {
 
 /* Mixed precision: how the FP64 outer loop uses the FP32 inner solves. */
 enum axg_refinement {
-    AXG_REFINE_FCG = 0,   /* flexible CG in FP64, the inner solve as preconditioner  */
-    AXG_REFINE_PLAIN = 1  /* classical refinement u += M32(f - K u)                  */
+    AXG_REFINE_FCG = 0,      /* flexible CG in FP64, the inner solve as preconditioner */
+    AXG_REFINE_PLAIN = 1,    /* classical refinement u += M32(f - K u)                 */
+    AXG_REFINE_RELIABLE = 2  /* one FP32 CG whose search direction runs on across the
+                                FP64 updates: u += correction, the FP32 residual
+                                replaced by f - K u (FP64) each time it has dropped by
+                                inner_tol ("reliable updates")                        */
 };
 
 typedef struct axg_solve_opts {

@@ -344,9 +344,9 @@ def test_mixed_multi_rhs_matches_fp64(ctx):
         assert rel(X[k], X64[k]) < 1e-9
 
 
-@pytest.mark.parametrize("refinement", ["fcg", "plain"])
+@pytest.mark.parametrize("refinement", ["fcg", "plain", "reliable"])
 def test_mixed_refinement_modes_match_fp64(ctx, refinement):
-    """Flexible-CG and classical refinement both reach the FP64 solution."""
+    """Flexible-CG, classical refinement and reliable updates all reach the FP64 solution."""
     say("case: refinement", refinement)
     p = MG.brick_lognormal(10, seed=8, nloads=2)
     mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
@@ -354,7 +354,7 @@ def test_mixed_refinement_modes_match_fp64(ctx, refinement):
     B = p.rhs_all()
     X64, _ = mesh.solve(B, precision="fp64", precond="block3", tol=1e-13)
     X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-12, refinement=refinement)
-    assert ("FCG" if refinement == "fcg" else "IR") in info["name"]
+    assert {"fcg": "FCG", "plain": "IR", "reliable": "RU"}[refinement] in info["name"]
     for k in range(2):
         assert rel(X[k], X64[k]) < 1e-9
     assert info["rel_residual"] <= 1e-12
