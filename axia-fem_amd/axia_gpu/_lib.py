@@ -91,6 +91,8 @@ SIGNATURES = {
     "axg_csr_from_triplets": (C.c_int, [_vp, _i64, _i64, _lp, _lp, _dp, _lp, _lp, _dp]),
     "axg_solve_csr": (C.c_int, [_vp, _i64, _lp, _lp, _dp, _dp, C.POINTER(SolveOpts), _dp,
                                 C.POINTER(SolveInfo)]),
+    "axg_solve_csr_device_noalloc": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp,
+                                               C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "axg_mesh_create": (C.c_int, [_vp, _i32, _i64, _dp, _i64, _ip, _dp, _dp, _lp, C.POINTER(_vp)]),
     "axg_mesh_destroy": (C.c_int, [_vp]),
     "axg_mesh_info": (C.c_int, [_vp, _lp, _lp, _lp, _dp]),

@@ -29,7 +29,7 @@ EXACT = {"axg_element.hip", "axg_triplets.hip", "axg_loads.cpp", "axg_deck.cpp",
 SOURCES = [
     "axg_api.cpp", "axg_symbolic.cpp", "axg_loads.cpp", "axg_element.hip", "axg_triplets.hip",
     "axg_pcg.hip", "axg_mixed.hip", "axg_element32.hip", "axg_bench.hip", "axg_batch.hip",
-    "axg_deck.cpp", "axg_eigen.hip", "axg_writers.cpp",
+    "axg_deck.cpp", "axg_eigen.hip", "axg_writers.cpp", "axg_noalloc.hip",
 ]
 
 

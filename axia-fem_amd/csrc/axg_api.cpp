@@ -341,6 +341,7 @@ int axg_create(int device, axg_ctx **out) {
         c->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         c->arch = prop.gcnArchName;
         ensure_tables(device);
+        noalloc_setup(device);
         c->ws.err.ensure(4);
         c->ws.cnt.ensure(4);
     } catch (...) {

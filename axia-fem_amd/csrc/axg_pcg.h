@@ -54,6 +54,19 @@ void diag_fp64(axg_ctx *ctx, const CsrView &A, double *dsel, double *diag_raw);
 void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b, double *x,
               double tol, int64_t max_iter, bool timing, PcgResult &res);
 
+// Partials a solve's reductions need (2 per workgroup of the largest grid, + 2).
+size_t pcg_part_count(axg_ctx *ctx);
+size_t pcg_noalloc_doubles(axg_ctx *ctx, int64_t n);
+// linalg::pcg (Jacobi) on a device CSR without allocating: every intermediate in
+// scratch (pcg_noalloc_doubles(ctx, n) doubles), plain launches (no graph),
+// the CSR SpMV shape from the mean row length. Returns ||b - A x||_2 (FP64).
+double pcg_fp64_noalloc(axg_ctx *ctx, const CsrView &A, const double *b, double *x, double tol,
+                        int64_t max_iter, double *scratch, PcgResult &res);
+
+// Looks up this device's copy of the no-allocation solve's fixed scratch array
+// (axg_noalloc.hip); called once per device by axg_create, device current.
+void noalloc_setup(int device);
+
 // The reference's CG error text for a failed solve (linalg.rs:231-257).
 std::string pcg_error(const PcgResult &r);
 

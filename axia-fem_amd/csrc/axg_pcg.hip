@@ -520,6 +520,27 @@ __global__ void __launch_bounds__(BS) init_kernel(int64_t n, const double *__res
     (void)sc, (void)tol;
 }
 
+// Per-workgroup partial sums of (b - y)^2.
+__global__ void __launch_bounds__(BS) residual_part_kernel(int64_t n, const double *__restrict__ b,
+                                                          const double *__restrict__ y,
+                                                          double *__restrict__ part) {
+    double acc[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+        const double d = b[i] - y[i];
+        acc[0] += d * d;
+    }
+    group_sums<1, BS>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// sqrt of the partials' fixed-order sum, into sc->pad[0].
+__global__ void __launch_bounds__(FIN_TB) fin_residual_kernel(const double *__restrict__ part,
+                                                         int nblk, PcgScalars *sc) {
+    double tot[1];
+    fin_sums<1>(part, nblk, tot);
+    if (threadIdx.x == 0) sc->pad[0] = sqrt(tot[0]);
+}
+
 // dsel[i] = first (i, i) entry of row i if |d| > 1e-30, else 1.0 (Csr::diag,
 // linalg.rs:52-63, and the divisor rule of linalg.rs:213-216).
 __global__ void diag_kernel(int64_t n, const int32_t *__restrict__ rowptr,
@@ -545,37 +566,105 @@ void diag_fp64(axg_ctx *ctx, const CsrView &A, double *dsel, double *diag_raw) {
 }
 
 // ------------------------------------------------------------ host driver
+// Buffers of one solve: n-vectors r, p, ap, z; part holds 2 max(grid) + 2
+// partials (pcg_part_count); sc one PcgScalars; host the host copy of the
+// scalars (pinned, or any host memory when graph is false).
+struct PcgBuffers {
+    double *r = nullptr, *p = nullptr, *ap = nullptr, *z = nullptr, *part = nullptr;
+    PcgScalars *sc = nullptr, *host = nullptr;
+    bool graph = true;  // replay captured batches of iterations (false: plain launches)
+    int shape = -1;     // CSR SpMV shape (-1: tuned on first use and cached in A.shape)
+};
+
+static void pcg_fp64_core(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b,
+                          double *x, double tol, int64_t max_iter, bool timing, PcgResult &res,
+                          const PcgBuffers &bf);
+
 void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b,
               double *x, double tol, int64_t max_iter, bool timing, PcgResult &res) {
-    hipStream_t s = ctx->stream;
     const int64_t n = A.n;
     Workspace &w = ctx->ws;
     w.r.ensure(n);
     w.p.ensure(n);
     w.ap.ensure(n);
     w.z.ensure(n);
+    w.part.ensure(pcg_part_count(ctx));
+    w.sc.ensure(1);
+    ctx->pinned.ensure(sizeof(PcgScalars) * 2);
+    PcgBuffers bf;
+    bf.r = w.r.p, bf.p = w.p.p, bf.ap = w.ap.p, bf.z = w.z.p, bf.part = w.part.p;
+    bf.sc = w.sc.p;
+    bf.host = (PcgScalars *)ctx->pinned.p;
+    pcg_fp64_core(ctx, A, pa, b, x, tol, max_iter, timing, res, bf);
+}
+
+size_t pcg_part_count(axg_ctx *ctx) { return 2 * (size_t)ctx->cu_count * 8 + 2; }
+
+static size_t round32(size_t v) { return (v + 31) / 32 * 32; }
+
+size_t pcg_noalloc_doubles(axg_ctx *ctx, int64_t n) {
+    return round32(5 * (size_t)n) + round32(pcg_part_count(ctx)) +
+           round32(sizeof(PcgScalars) / sizeof(double));
+}
+
+// linalg::pcg with Jacobi on a device CSR (int32 row pointer and columns)
+// into caller buffers: no allocation, no graph, no autotuning (the SpMV shape
+// follows the mean row length). scratch (pcg_noalloc_doubles(ctx, n) doubles,
+// 256-byte aligned): Jacobi divisor, r, p, ap, z, partials, scalars. Returns
+// the FP64 ||b - A x||.
+double pcg_fp64_noalloc(axg_ctx *ctx, const CsrView &A, const double *b, double *x, double tol,
+                        int64_t max_iter, double *scratch, PcgResult &res) {
+    const int64_t n = A.n;
+    double *dsel = scratch;
+    PcgBuffers bf;
+    bf.r = scratch + n, bf.p = scratch + 2 * n, bf.ap = scratch + 3 * n, bf.z = scratch + 4 * n;
+    bf.part = scratch + round32(5 * (size_t)n);
+    bf.sc = reinterpret_cast<PcgScalars *>(bf.part + round32(pcg_part_count(ctx)));
+    PcgScalars host;
+    bf.host = &host;
+    bf.graph = false;
+    bf.shape = default_shape(A);
+    diag_fp64(ctx, A, dsel, nullptr);
+    Precond64Args pa;
+    pa.dsel = dsel;
+    pcg_fp64_core(ctx, A, pa, b, x, tol, max_iter, false, res, bf);
+    // ||b - A x||: A x into ap, squared differences summed by workgroup, then in
+    // fixed order by one workgroup into the scalars' pad slot
+    hipStream_t s = ctx->stream;
+    launch_spmv_shape(ctx, bf.shape, A, x, bf.ap);
+    const int grid_v = grid_for(ctx, n, 4);
+    hipLaunchKernelGGL(residual_part_kernel, dim3(grid_v), dim3(BS), 0, s, n, b, bf.ap, bf.part);
+    hipLaunchKernelGGL(fin_residual_kernel, dim3(1), dim3(FIN_TB), 0, s, bf.part, grid_v, bf.sc);
+    AXG_HIP(hipGetLastError());
+    AXG_HIP(hipMemcpyAsync(&host, bf.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    return host.pad[0];
+}
+
+static void pcg_fp64_core(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const double *b,
+                          double *x, double tol, int64_t max_iter, bool timing, PcgResult &res,
+                          const PcgBuffers &bf) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = A.n;
     const bool sell = A.sell_val != nullptr;
-    const int shape = sell ? 0 : shape_of(ctx, A);
+    const int shape = sell ? 0 : bf.shape >= 0 ? bf.shape : shape_of(ctx, A);
     const int grid_mv = sell ? grid_for(ctx, n, 4) : grid_for(ctx, n * kShapes[shape].L, 8);
     const int grid_v = grid_for(ctx, n, 4);
-    w.part.ensure(2 * (size_t)std::max(grid_mv, grid_v) + 2);
-    w.sc.ensure(1);
     Precond64 M{pa.dsel, pa.binv, pa.blk_of_free, pa.free_of_blk};
     const bool fused = pa.binv == nullptr;
 
-    ctx->pinned.ensure(sizeof(PcgScalars) * 2);
-    PcgScalars *h = (PcgScalars *)ctx->pinned.p;
+    PcgScalars *h = bf.host;
     std::memset((void *)h, 0, sizeof(PcgScalars));
     h->max_iter = max_iter;
-    AXG_HIP(hipMemcpyAsync(w.sc.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+    AXG_HIP(hipMemcpyAsync(bf.sc, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
     AXG_HIP(hipEventRecord(ctx->ev0, s));
-    hipLaunchKernelGGL(init_kernel, dim3(grid_v), dim3(BS), 0, s, n, b, x, w.r.p, w.part.p, w.sc.p,
+    hipLaunchKernelGGL(init_kernel, dim3(grid_v), dim3(BS), 0, s, n, b, x, bf.r, bf.part, bf.sc,
                        tol);
-    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_v, w.sc.p, tol);
-    hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.r.p, w.z.p, M,
-                       w.part.p, w.sc.p, 1);
-    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_v, w.sc.p, 1);
-    hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p, 1);
+    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(FIN_TB), 0, s, bf.part, grid_v, bf.sc, tol);
+    hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, bf.r, bf.z, M,
+                       bf.part, bf.sc, 1);
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(FIN_TB), 0, s, bf.part, grid_v, bf.sc, 1);
+    hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, bf.p, bf.z, bf.sc, 1);
     AXG_HIP(hipGetLastError());
 
     // timing: the events are stamped by the SpMV dispatch itself
@@ -584,41 +673,41 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
             hipExtLaunchKernelGGL((sell_spmv_dot_kernel<4>), dim3(grid_mv), dim3(BS), 0, s, e0, e1,
                                   0, n, (const int64_t *)A.sell_sptr, (const int32_t *)A.sell_width,
                                   (const int32_t *)A.sell_col, (const double *)A.sell_val,
-                                  (const double *)w.p.p, w.ap.p, w.part.p, w.sc.p);
+                                  (const double *)bf.p, bf.ap, bf.part, bf.sc);
         } else if (sell) {
             hipLaunchKernelGGL((sell_spmv_dot_kernel<4>), dim3(grid_mv), dim3(BS), 0, s, n,
-                               A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, w.p.p, w.ap.p,
-                               w.part.p, w.sc.p);
+                               A.sell_sptr, A.sell_width, A.sell_col, A.sell_val, bf.p, bf.ap,
+                               bf.part, bf.sc);
         } else {
 #define AXG_L(LL, UU)                                                                            \
     if (e0)                                                                                      \
         hipExtLaunchKernelGGL((spmv_dot_kernel<LL, UU>), dim3(grid_mv), dim3(BS), 0, s, e0, e1, 0, \
                               n, (const int32_t *)A.rowptr, (const int32_t *)A.colind,           \
-                              (const double *)A.val, (const double *)w.p.p, w.ap.p, w.part.p,   \
-                              w.sc.p);                                                           \
+                              (const double *)A.val, (const double *)bf.p, bf.ap, bf.part,   \
+                              bf.sc);                                                           \
     else                                                                                         \
         hipLaunchKernelGGL((spmv_dot_kernel<LL, UU>), dim3(grid_mv), dim3(BS), 0, s, n, A.rowptr, \
-                           A.colind, A.val, w.p.p, w.ap.p, w.part.p, w.sc.p)
+                           A.colind, A.val, bf.p, bf.ap, bf.part, bf.sc)
             AXG_SHAPE_SWITCH(shape, AXG_L)
 #undef AXG_L
         }
-        hipLaunchKernelGGL(fin_pap_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_mv, w.sc.p);
+        hipLaunchKernelGGL(fin_pap_kernel, dim3(1), dim3(FIN_TB), 0, s, bf.part, grid_mv, bf.sc);
         if (fused) {
-            hipLaunchKernelGGL((update_xr_kernel<true>), dim3(grid_v), dim3(BS), 0, s, n, x, w.r.p,
-                               w.p.p, w.ap.p, w.z.p, M.dsel, w.part.p, w.sc.p);
-            hipLaunchKernelGGL((fin_update_kernel<true>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
-                               grid_v, w.sc.p);
+            hipLaunchKernelGGL((update_xr_kernel<true>), dim3(grid_v), dim3(BS), 0, s, n, x, bf.r,
+                               bf.p, bf.ap, bf.z, M.dsel, bf.part, bf.sc);
+            hipLaunchKernelGGL((fin_update_kernel<true>), dim3(1), dim3(FIN_TB), 0, s, bf.part,
+                               grid_v, bf.sc);
         } else {
             hipLaunchKernelGGL((update_xr_kernel<false>), dim3(grid_v), dim3(BS), 0, s, n, x,
-                               w.r.p, w.p.p, w.ap.p, w.z.p, M.dsel, w.part.p, w.sc.p);
-            hipLaunchKernelGGL((fin_update_kernel<false>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
-                               grid_v, w.sc.p);
-            hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.r.p, w.z.p,
-                               M, w.part.p, w.sc.p, 0);
-            hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_v, w.sc.p,
+                               bf.r, bf.p, bf.ap, bf.z, M.dsel, bf.part, bf.sc);
+            hipLaunchKernelGGL((fin_update_kernel<false>), dim3(1), dim3(FIN_TB), 0, s, bf.part,
+                               grid_v, bf.sc);
+            hipLaunchKernelGGL(precond_dot_kernel, dim3(grid_v), dim3(BS), 0, s, n, bf.r, bf.z,
+                               M, bf.part, bf.sc, 0);
+            hipLaunchKernelGGL(fin_rz_kernel, dim3(1), dim3(FIN_TB), 0, s, bf.part, grid_v, bf.sc,
                                0);
         }
-        hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, w.p.p, w.z.p, w.sc.p,
+        hipLaunchKernelGGL(update_p_kernel, dim3(grid_v), dim3(BS), 0, s, n, bf.p, bf.z, bf.sc,
                            0);
     };
 
@@ -626,14 +715,14 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
     res.spmv_launches = 0;
     const int batch = n > 200000 ? 16 : 64;
     hipGraphExec_t gexec = nullptr;
-    if (!timing) {
+    if (!timing && bf.graph) {
         hipGraph_t graph;
         AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < batch; i++) one_iter(nullptr, nullptr);
         AXG_HIP(hipStreamEndCapture(s, &graph));
         AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         AXG_HIP(hipGraphDestroy(graph));
-    } else {
+    } else if (timing) {
         while (ctx->timing_events.size() < 2 * (size_t)batch) {
             hipEvent_t e;
             AXG_HIP(hipEventCreate(&e));
@@ -642,14 +731,14 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
     }
     int64_t launched = 0;
     for (;;) {
-        AXG_HIP(hipMemcpyAsync(h, w.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipMemcpyAsync(h, bf.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         AXG_HIP(hipStreamSynchronize(s));
         if (h->done || launched >= max_iter) break;
         if (timing) {
             int64_t before = h->iters;
             for (int i = 0; i < batch; i++)
                 one_iter(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1]);
-            AXG_HIP(hipMemcpyAsync(h, w.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+            AXG_HIP(hipMemcpyAsync(h, bf.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
             AXG_HIP(hipStreamSynchronize(s));
             // SpMV launches that did work: the iterations this batch counted
             int64_t did = h->iters - before;
@@ -660,8 +749,11 @@ void pcg_fp64(axg_ctx *ctx, const CsrView &A, const Precond64Args &pa, const dou
                 res.spmv_ms += ms;
                 res.spmv_launches++;
             }
-        } else {
+        } else if (gexec) {
             AXG_HIP(hipGraphLaunch(gexec, s));
+        } else {
+            for (int i = 0; i < batch; i++) one_iter(nullptr, nullptr);
+            AXG_HIP(hipGetLastError());
         }
         launched += batch;
     }

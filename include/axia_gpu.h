@@ -124,6 +124,19 @@ int axg_solve_csr(axg_ctx *ctx, int64_t n, const int64_t *indptr, const int64_t 
                   const double *data, const double *rhs, const axg_solve_opts *opts,
                   double *x_out, axg_solve_info *info);
 
+/* No counterpart in the reference. The solve of axg_solve_csr with FP64 +
+ * Jacobi (linalg::pcg) for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * the CSR (int32 row pointer of n+1 and columns, FP64 values), rhs and x are
+ * device pointers owned by the caller; every intermediate lives in a fixed
+ * library array of 40 MiB per device, set up by axg_create, so n is at most
+ * 1048576 (larger n: AXG_E_INVALID). Other precisions or preconditioners:
+ * AXG_E_INVALID. Calls that overlap on one device race on that array; the
+ * array keeps the last call's contents. Synchronous on the context's stream. */
+int axg_solve_csr_device_noalloc(axg_ctx *ctx, int64_t n, const int32_t *d_rowptr,
+                                 const int32_t *d_colind, const double *d_val, const double *d_rhs,
+                                 const axg_solve_opts *opts, double *d_x, axg_solve_info *info);
+
 /* ------------------------------------------------------------ assembly */
 /* An all-C3D8 (nn = 8) or all-C3D20 (nn = 20) isotropic mesh.
  * xyz: nnode x 3; conn: nelem x nn, 0-based node indices in the reference's

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# torch (device buffers in a few GPU tests) loads its own HIP runtime; importing it
+# before the library is first loaded keeps one runtime in the process, as bench.py does
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "axia-fem_amd")):
     if p not in sys.path:
