@@ -564,104 +564,37 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
         for (int j = 0; j < K; j++) part[(size_t)rb.id * K + j] = dot[j];
 }
 
-// ------------------------------------------------------ finalize kernels
-// One workgroup each: sum the partials of the kernel launched just before
-// (fixed order) and update the per-right-hand-side scalars. The scalars are
-// loaded before the partials and the stop flag is tested after the sum, so
-// the reads overlap instead of costing one memory round trip each.
-template <int K>
-__global__ void __launch_bounds__(FIN_TB) fin_spmm_kernel(const double *__restrict__ part, int nblk,
-                                                      InnerScalars *sc) {
-    const int all_done = sc->all_done;
+// ------------------------------------------------------ inner iteration
+// One inner CG iteration is three kernels: the SpMM (Ap and p.Ap partials),
+// inner_step (alpha, x, r, z, and the ||r||^2, r.z partials) and inner_dir
+// (stop tests, beta, p). Each of the last two sums the previous kernel's
+// partials in every workgroup, in the same fixed order, so all workgroups form
+// the same scalars and no separate finalize kernel runs. The per-column scalars
+// are double-buffered by iteration parity: inner_dir reads `cur` and its
+// workgroup 0 writes `next`, which the next iteration's kernels read.
+
+// x += alpha p, r -= alpha Ap, z = M r (3x3 block or diagonal), partials of
+// ||r||^2 and r.z; workgroup 0 keeps p.Ap for inner_dir's breakdown test.
+template <int K, bool BLOCK>
+__global__ void __launch_bounds__(MBS) inner_step_kernel(
+    int64_t nb, float *__restrict__ X, float *__restrict__ R, const float *__restrict__ P,
+    const float *__restrict__ AP, float *__restrict__ Z, const float *__restrict__ binv,
+    const float *__restrict__ dinv, const double *__restrict__ part_s, int nblk_s,
+    double *__restrict__ part_u, double *__restrict__ pap_out, const InnerScalars *sc) {
+    if (sc->all_done) return;
     int done[K];
     double rz[K];
 #pragma unroll
     for (int j = 0; j < K; j++) done[j] = sc->done[j], rz[j] = sc->rz[j];
     double pap[K];
-    fin_sums<K>(part, nblk, pap);
-    if (all_done || threadIdx.x != 0) return;
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        if (done[j]) continue;
-        if (!(pap[j] > 1e-300)) {
-            sc->done[j] = 1;  // breakdown: this sweep ends, refinement goes on
-            sc->alpha[j] = 0.0;
-        } else {
-            sc->alpha[j] = rz[j] / pap[j];
-        }
-    }
-}
-
-template <int K>
-__global__ void __launch_bounds__(FIN_TB) fin_inner_update_kernel(const double *__restrict__ part,
-                                                              int nblk, InnerScalars *sc) {
-    const int all_done = sc->all_done, max_iter = sc->max_iter;
-    int done[K], iters[K];
-    double rz[K], tol_abs[K];
-#pragma unroll
-    for (int j = 0; j < K; j++)
-        done[j] = sc->done[j], iters[j] = sc->iters[j], rz[j] = sc->rz[j], tol_abs[j] = sc->tol_abs[j];
-    double tot[2 * K];
-    fin_sums<2 * K>(part, nblk, tot);
-    if (all_done || threadIdx.x != 0) return;
-    int all = 1;
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        if (!done[j]) {
-            const double res = sqrt(tot[j]);
-            sc->res[j] = res;
-            sc->iters[j] = iters[j] + 1;
-            if (res <= tol_abs[j] || iters[j] + 1 >= max_iter) {
-                done[j] = 1;
-                sc->done[j] = 1;
-            } else {
-                sc->beta[j] = tot[K + j] / rz[j];
-                sc->rz[j] = tot[K + j];
-            }
-        }
-        all &= done[j];
-    }
-    sc->all_done = all;
-}
-
-// rz_prev (may be null): a restart that keeps the search direction (reliable
-// update) passes the last r.z of the previous interval in the new scaling, and
-// beta_j = r.z / rz_prev_j joins the new residual to the kept direction.
-template <int K>
-__global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__restrict__ part,
-                                                            int nblk, InnerScalars *sc,
-                                                            double inner_tol,
-                                                            const double *__restrict__ rz_prev) {
-    double tot[2 * K];
-    fin_sums<2 * K>(part, nblk, tot);
-    if (threadIdx.x == 0) {
-        int all = 1;
-        for (int j = 0; j < K; j++) {
-            double bn = sqrt(tot[j]);
-            sc->bnorm[j] = bn;
-            sc->res[j] = bn;
-            sc->tol_abs[j] = inner_tol * bn;
-            if (rz_prev) sc->beta[j] = rz_prev[j] > 0.0 ? tot[K + j] / rz_prev[j] : 0.0;
-            sc->rz[j] = tot[K + j];
-            sc->iters[j] = 0;
-            sc->done[j] = (bn < 1e-30) ? 1 : 0;
-            all &= sc->done[j];
-        }
-        sc->all_done = all;
-    }
-}
-
-// x += alpha p, r -= alpha Ap, z = M r (3x3 block or diagonal), ||r||^2, r.z;
-// the last workgroup decides per right-hand side. One thread per block row.
-template <int K, bool BLOCK>
-__global__ void __launch_bounds__(MBS) inner_update_kernel(
-    int64_t nb, float *__restrict__ X, float *__restrict__ R, const float *__restrict__ P,
-    const float *__restrict__ AP, float *__restrict__ Z, const float *__restrict__ binv,
-    const float *__restrict__ dinv, double *__restrict__ part, InnerScalars *sc) {
-    if (sc->all_done) return;
+    fin_sums<K, MBS>(part_s, nblk_s, pap);
     float alpha[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) alpha[j] = sc->done[j] ? 0.f : (float)sc->alpha[j];
+    for (int j = 0; j < K; j++)
+        alpha[j] = (done[j] || !(pap[j] > 1e-300)) ? 0.f : (float)(rz[j] / pap[j]);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < K; j++) pap_out[j] = pap[j];
     double acc[2 * K];  // ||r||^2 then r.z
 #pragma unroll
     for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
@@ -705,9 +638,108 @@ __global__ void __launch_bounds__(MBS) inner_update_kernel(
     group_sums<2 * K, MBS>(acc);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int j = 0; j < 2 * K; j++) part[(size_t)blockIdx.x * 2 * K + j] = acc[j];
+        for (int j = 0; j < 2 * K; j++) part_u[(size_t)blockIdx.x * 2 * K + j] = acc[j];
 }
 
+// Per column: a breakdown (p.Ap <= 0) or ||r|| <= tol_abs or the iteration cap
+// ends it; otherwise beta = r.z_new / r.z and p = z + beta p.
+template <int K>
+__global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__restrict__ P,
+                                                       const float *__restrict__ Z,
+                                                       const double *__restrict__ part_u,
+                                                       int nblk_u,
+                                                       const double *__restrict__ pap_in,
+                                                       const InnerScalars *cur,
+                                                       InnerScalars *next) {
+    if (cur->all_done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *next = *cur;
+        return;
+    }
+    const int max_iter = cur->max_iter;
+    int done[K], iters[K];
+    double rz[K], tol_abs[K], pap[K];
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        done[j] = cur->done[j], iters[j] = cur->iters[j], rz[j] = cur->rz[j],
+        tol_abs[j] = cur->tol_abs[j], pap[j] = pap_in[j];
+    double tot[2 * K];
+    fin_sums<2 * K, MBS>(part_u, nblk_u, tot);
+    int ndone[K];
+    float beta[K];
+    double res[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        ndone[j] = done[j];
+        beta[j] = 0.f;
+        res[j] = cur->res[j];
+        if (done[j]) continue;
+        if (!(pap[j] > 1e-300)) {  // breakdown: this interval ends, refinement goes on
+            ndone[j] = 1;
+            continue;
+        }
+        res[j] = sqrt(tot[j]);
+        if (res[j] <= tol_abs[j] || iters[j] + 1 >= max_iter) {
+            ndone[j] = 1;
+        } else {
+            beta[j] = (float)(tot[K + j] / rz[j]);
+        }
+    }
+    for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < nv;
+         i += (int64_t)gridDim.x * MBS) {
+        float p[K], z[K];
+        load_k<K>(P + i * K, p);
+        load_k<K>(Z + i * K, z);
+#pragma unroll
+        for (int j = 0; j < K; j++) p[j] = ndone[j] ? p[j] : z[j] + beta[j] * p[j];
+        store_k<K>(P + i * K, p);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        InnerScalars nx = *cur;
+        int all = 1;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (!done[j] && pap[j] > 1e-300) {
+                nx.res[j] = res[j];
+                nx.iters[j] = iters[j] + 1;
+                if (!ndone[j]) nx.beta[j] = beta[j], nx.rz[j] = tot[K + j];
+            }
+            nx.done[j] = ndone[j];
+            all &= ndone[j];
+        }
+        nx.all_done = all;
+        *next = nx;
+    }
+}
+
+// rz_prev (may be null): a restart that keeps the search direction (reliable
+// update) passes the last r.z of the previous interval in the new scaling, and
+// beta_j = r.z / rz_prev_j joins the new residual to the kept direction.
+template <int K>
+__global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__restrict__ part,
+                                                            int nblk, InnerScalars *sc,
+                                                            double inner_tol,
+                                                            const double *__restrict__ rz_prev) {
+    double tot[2 * K];
+    fin_sums<2 * K>(part, nblk, tot);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int j = 0; j < K; j++) {
+            double bn = sqrt(tot[j]);
+            sc->bnorm[j] = bn;
+            sc->res[j] = bn;
+            sc->tol_abs[j] = inner_tol * bn;
+            if (rz_prev) sc->beta[j] = rz_prev[j] > 0.0 ? tot[K + j] / rz_prev[j] : 0.0;
+            sc->rz[j] = tot[K + j];
+            sc->iters[j] = 0;
+            sc->done[j] = (bn < 1e-30) ? 1 : 0;
+            all &= sc->done[j];
+        }
+        sc->all_done = all;
+    }
+}
+
+// The first direction p = z of an interval (init) or p = z + beta p after a
+// restart that keeps the direction (beta from fin_inner_init_kernel).
 template <int K>
 __global__ void __launch_bounds__(MBS) inner_update_p_kernel(int64_t nv, float *__restrict__ P,
                                                             const float *__restrict__ Z,
@@ -1029,7 +1061,8 @@ struct MixedSolver {
         const int grid_r = std::max(1, (int)std::min<int64_t>((n + MBS - 1) / MBS,
                                                               (int64_t)ctx->cu_count * 4));
         w.part.ensure((size_t)std::max({grid_mv, grid_v, grid_r}) * 2 * K + 8);
-        DevBuf<InnerScalars> dsc(1);
+        DevBuf<InnerScalars> dsc(2);  // by iteration parity
+        DevBuf<double> part_u((size_t)grid_v * 2 * K + 8), pap(8);
         ctx->pinned.ensure(sizeof(InnerScalars) + 256);
         InnerScalars *h = (InnerScalars *)ctx->pinned.p;
         double *hn = (double *)((char *)ctx->pinned.p + sizeof(InnerScalars));
@@ -1040,35 +1073,33 @@ struct MixedSolver {
 
         // timing: the events are stamped by the dispatch itself (kernel start
         // and end), not by separate stream markers
-        auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1) {
+        auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1, int par) {
+            InnerScalars *cur = dsc.p + par;
 #define AXG_SPMM(UU, GG)                                                                        \
     if (e0)                                                                                     \
         hipExtLaunchKernelGGL((bsell_spmm_kernel<K, UU, true, GG>), dim3(grid_mv), dim3(MBS), 0, s, \
                               e0, e1, 0, nb, (const int64_t *)m->bs_sptr.p,                     \
                               (const int32_t *)m->bs_width.p, (const float *)m->bval.p,          \
-                              m->bs_nslots, (const float *)w.f3.p, w.f4.p, w.part.p, dsc.p);    \
+                              m->bs_nslots, (const float *)w.f3.p, w.f4.p, w.part.p, cur);      \
     else                                                                                        \
         hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true, GG>), dim3(grid_mv), dim3(MBS), 0, s, nb, \
                            m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots,    \
-                           w.f3.p, w.f4.p, w.part.p, dsc.p)
+                           w.f3.p, w.f4.p, w.part.p, cur)
             AXG_SPMM_SWITCH(shp, AXG_SPMM)
 #undef AXG_SPMM
-            hipLaunchKernelGGL((fin_spmm_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p, grid_mv,
-                               dsc.p);
         };
-        auto launch_update = [&]() {
+        auto launch_update = [&](int par) {
+            InnerScalars *cur = dsc.p + par, *next = dsc.p + (par ^ 1);
             if (block)
-                hipLaunchKernelGGL((inner_update_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s,
-                                   nb, w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p,
-                                   m->dinv32.p, w.part.p, dsc.p);
+                hipLaunchKernelGGL((inner_step_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
+                                   w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p,
+                                   w.part.p, grid_mv, part_u.p, pap.p, cur);
             else
-                hipLaunchKernelGGL((inner_update_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
-                                   nb, w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p,
-                                   m->dinv32.p, w.part.p, dsc.p);
-            hipLaunchKernelGGL((fin_inner_update_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
-                               grid_v, dsc.p);
-            hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
-                               w.f3.p, Z.p, dsc.p, 0);
+                hipLaunchKernelGGL((inner_step_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s, nb,
+                                   w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p,
+                                   w.part.p, grid_mv, part_u.p, pap.p, cur);
+            hipLaunchKernelGGL((inner_dir_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv, w.f3.p,
+                               Z.p, part_u.p, grid_v, pap.p, cur, next);
         };
         const int grid_n = (int)((n + 255) / 256);
         std::vector<double> hpart((size_t)grid_v * K);
@@ -1114,9 +1145,9 @@ struct MixedSolver {
         if (!o.timing) {
             hipGraph_t graph;
             AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < inner_batch; i++) {
-                launch_spmm(nullptr, nullptr);
-                launch_update();
+            for (int i = 0; i < inner_batch; i++) {  // inner_batch is even: state back in dsc[0]
+                launch_spmm(nullptr, nullptr, i & 1);
+                launch_update(i & 1);
             }
             AXG_HIP(hipStreamEndCapture(s, &graph));
             AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -1171,8 +1202,8 @@ struct MixedSolver {
                 if (h->all_done) break;
                 if (o.timing) {
                     for (int i = 0; i < inner_batch; i++) {
-                        launch_spmm(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1]);
-                        launch_update();
+                        launch_spmm(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1], i & 1);
+                        launch_update(i & 1);
                     }
                     AXG_HIP(hipMemcpyAsync(h, dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost,
                                            s));

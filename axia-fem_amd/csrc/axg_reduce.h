@@ -83,24 +83,28 @@ constexpr int FIN_TB = 1024;
 // then folded by shuffles and the waves through LDS. The order is fixed for a
 // given nblk. The partials were written by other XCDs, so the reads miss this
 // XCD's L2: the cost is a few round trips, not bandwidth.
-template <int NC>
+// TB: threads of the workgroup that sums (FIN_TB for the finalize kernels;
+// a vector kernel's own workgroup size when every workgroup sums the partials
+// of the previous kernel itself, all of them in the same order).
+template <int NC, int TB = FIN_TB>
 __device__ inline void fin_sums(const double *__restrict__ part, int nblk, double (&out)[NC]) {
     static_assert(NC >= 1 && NC <= 64 && (NC & (NC - 1)) == 0, "NC must be a power of two");
+    static_assert(TB % 64 == 0 && TB >= 64, "whole wavefronts");
     const int64_t nflat = (int64_t)nblk * NC;
     double a[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) a[u] = 0.0;
     int64_t f = threadIdx.x;
-    for (; f + 7 * FIN_TB < nflat; f += 8 * FIN_TB)
+    for (; f + 7 * TB < nflat; f += 8 * TB)
 #pragma unroll
-        for (int u = 0; u < 8; u++) a[u] += part[f + u * FIN_TB];
+        for (int u = 0; u < 8; u++) a[u] += part[f + u * TB];
 #pragma unroll
     for (int u = 0; u < 7; u++)
-        if (f + u * FIN_TB < nflat) a[u] += part[f + u * FIN_TB];
+        if (f + u * TB < nflat) a[u] += part[f + u * TB];
     double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 #pragma unroll
     for (int off = 32; off >= NC; off >>= 1) s += __shfl_down(s, off, 64);
-    __shared__ double sh[FIN_TB / 64][NC];
+    __shared__ double sh[TB / 64][NC];
     __shared__ double tot[NC];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane < NC) sh[w][lane] = s;
@@ -108,7 +112,7 @@ __device__ inline void fin_sums(const double *__restrict__ part, int nblk, doubl
     if (threadIdx.x < NC) {
         double t = 0.0;
 #pragma unroll
-        for (int v = 0; v < FIN_TB / 64; v++) t += sh[v][threadIdx.x];
+        for (int v = 0; v < TB / 64; v++) t += sh[v][threadIdx.x];
         tot[threadIdx.x] = t;
     }
     __syncthreads();
