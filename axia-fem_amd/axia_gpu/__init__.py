@@ -242,6 +242,13 @@ class Mesh:
         check(_lib.lib().axg_bench_operator(self._h, which, nrhs, reps, C.byref(ms)))
         return ms.value
 
+    def spmm_shape(self, nrhs, shape=-1):
+        """FP32 SpMM variant for nrhs right-hand sides: pin it with shape >= 0; returns the
+        active one (-1 before the first mixed solve tunes it)."""
+        v = C.c_int32(shape)
+        check(_lib.lib().axg_mesh_spmm_shape(self._h, nrhs, C.byref(v)))
+        return v.value
+
     def nodal_stress(self, u_full):
         u = self._u(u_full)
         stress = np.zeros((self.nnode, 6))

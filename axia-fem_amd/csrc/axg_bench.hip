@@ -48,6 +48,24 @@ extern "C" int axg_bench_stream(axg_ctx *ctx, int64_t n, int32_t reps, double *c
 
 // which: 0 = FP64 SpMV of the solver (sliced ELL), 1 = FP64 CSR SpMV,
 // 2 = FP32 block-sliced-ELL SpMM with nrhs right-hand sides.
+extern "C" int axg_mesh_spmm_shape(axg_mesh *m, int32_t nrhs, int32_t *shape) {
+    try {
+        clear_error();
+        AXG_REQUIRE(m && shape && nrhs >= 1 && nrhs <= 8, AXG_E_INVALID,
+                    "axg_mesh_spmm_shape: invalid argument");
+        if (*shape >= 0) {
+            AXG_REQUIRE(spmm_shape_ok(*shape, nrhs), AXG_E_INVALID,
+                        "axg_mesh_spmm_shape: no such shape for this many right-hand sides");
+            m->spmm_shape[nrhs] = *shape;
+        }
+        *shape = m->spmm_shape[nrhs];
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    return AXG_OK;
+}
+
 extern "C" int axg_bench_operator(axg_mesh *m, int32_t which, int32_t nrhs, int32_t reps,
                                   double *avg_ms) {
     try {

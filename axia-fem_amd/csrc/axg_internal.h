@@ -277,6 +277,13 @@ struct axg_mesh {
     axg::DevBuf<int64_t> bs_sptr;
     axg::DevBuf<int32_t> bs_width, bs_col;
     int spmm_shape[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};  // tuned BSR SpMM shape per k
+    // groups of 256 block rows for the LDS-staged FP32 SpMM: the sorted column
+    // blocks of each group (at most a fixed count; the rest are read from
+    // global memory), per slot the column's place in its group's list (or the
+    // column with the top bit set), and the third value plane with that index
+    // in place of the column
+    axg::DevBuf<int32_t> grp_ptr, grp_col, bs_loc;
+    axg::DevBuf<float> bval_loc;
     // FP64 values in the same block layout (five double2 planes per slot), for
     // the multi-right-hand-side FP64 block PCG; refreshed after every assembly
     bool bsell64_fresh = false;

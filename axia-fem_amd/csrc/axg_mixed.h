@@ -24,6 +24,10 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms);
 // FP64 block-sliced-ELL values of K_ff (bval64) in the FP32 operator's block
 // layout; built on first use after each FP64 assembly.
 void ensure_bsell64(axg_mesh *m);
+// Number of FP32 SpMM shapes the autotuner chooses from (mesh->spmm_shape).
+int spmm_shape_count();
+// Whether SpMM shape `shape` exists for K right-hand sides per pass.
+bool spmm_shape_ok(int shape, int K);
 void solve_mixed(axg_mesh *m, const double *d_rhs, int32_t nrhs, const axg_solve_opts &o,
                  double *d_x, MixedResult &res);
 

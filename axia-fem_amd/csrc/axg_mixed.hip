@@ -5,11 +5,16 @@
 // until |f - K u| <= tol |f| for every right-hand side.
 //
 // Vectors of the inner solve live in block space: 3 entries per node that
-// keeps a free DOF (constrained entries stay zero), k right-hand sides
-// interleaved (entry (3 b + d) * K + j). BSR values are stored as three
-// planes: a float4 plane with entries (0,0) (0,1) (0,2) (1,0) of every block,
-// a float4 plane with (1,1) (1,2) (2,0) (2,1), and a float plane with (2,2),
-// so consecutive lanes read consecutive 16-byte words.
+// keeps a free DOF (constrained entries stay zero), stored as three planes by
+// DOF direction d with the k right-hand sides interleaved: entry
+// d * nb * K + b * K + j. A gather of one direction of a column block is then
+// 4 K contiguous bytes, and the 64 lanes of a wavefront (64 neighbouring block
+// rows) read neighbouring chunks: 3 x 16 B loads per block touch a third of
+// the cache lines a node-interleaved [b][d][j] layout would. BSR values are
+// stored as three planes: a float4 plane with entries (0,0) (0,1) (0,2) (1,0)
+// of every block, a float4 plane with (1,1) (1,2) (2,0) (2,1), and a float2
+// plane with (2,2) and the block's column, so consecutive lanes read
+// consecutive 16-byte words.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -27,6 +32,10 @@ namespace axg {
 
 
 constexpr int MBS = 256;
+// LDS-staged SpMM: block rows per group (one workgroup step) and the most
+// column blocks of a group whose x is staged (3 K floats each: 65,280 B at K = 4)
+constexpr int kGrpRows = 256;
+constexpr int kGrpMaxCols = 1360;
 
 // ---------------------------------------------------------- block maps
 void ensure_block_maps(axg_mesh *m) {
@@ -248,7 +257,50 @@ static void build_bsr(axg_mesh *m) {
     if (nsl) AXG_HIP(hipMemcpyAsync(m->bs_col.p, bcol.data(), 4 * nsl, hipMemcpyHostToDevice, s));
     if (nnz)
         AXG_HIP(hipMemcpyAsync(m->entry_of_slot.p, eos.data(), 4 * nnz, hipMemcpyHostToDevice, s));
+    // groups of kGrpRows block rows (4 slices): the sorted distinct columns of
+    // their slots (padding slots included), the first kGrpMaxCols of them staged
+    const int64_t nslice = (int64_t)L.width.size();
+    const int64_t ngrp = (nb + kGrpRows - 1) / kGrpRows;
+    std::vector<std::vector<int32_t>> gcols(ngrp);
+    HostVec<int32_t> loc(std::max<int64_t>(nsl, 1));
+    parallel_chunks(ngrp, host_threads(ngrp, 64), [&](int, int64_t lo, int64_t hi) {
+        for (int64_t g = lo; g < hi; g++) {
+            const int64_t s0 = L.sptr[4 * g], s1 = L.sptr[std::min(4 * g + 4, nslice)];
+            std::vector<int32_t> cols(bcol.begin() + s0, bcol.begin() + s1);
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            if ((int64_t)cols.size() > kGrpMaxCols) cols.resize(kGrpMaxCols);
+            for (int64_t q = s0; q < s1; q++) {
+                auto it = std::lower_bound(cols.begin(), cols.end(), bcol[q]);
+                loc[q] = (it != cols.end() && *it == bcol[q]) ? (int32_t)(it - cols.begin())
+                                                               : (int32_t)(bcol[q] | 0x80000000u);
+            }
+            gcols[g] = std::move(cols);
+        }
+    });
+    std::vector<int32_t> gptr(ngrp + 1, 0), gcol;
+    for (int64_t g = 0; g < ngrp; g++) gptr[g + 1] = gptr[g] + (int32_t)gcols[g].size();
+    gcol.reserve(gptr[ngrp]);
+    for (auto &v : gcols) gcol.insert(gcol.end(), v.begin(), v.end());
+    m->grp_ptr.alloc(ngrp + 1);
+    m->grp_col.alloc(std::max<size_t>(gcol.size(), 1));
+    m->bs_loc.alloc(std::max<int64_t>(nsl, 1));
+    AXG_HIP(hipMemcpyAsync(m->grp_ptr.p, gptr.data(), 4 * (ngrp + 1), hipMemcpyHostToDevice, s));
+    if (!gcol.empty())
+        AXG_HIP(hipMemcpyAsync(m->grp_col.p, gcol.data(), 4 * gcol.size(), hipMemcpyHostToDevice, s));
+    if (nsl) AXG_HIP(hipMemcpyAsync(m->bs_loc.p, loc.data(), 4 * nsl, hipMemcpyHostToDevice, s));
     AXG_HIP(hipStreamSynchronize(s));
+}
+
+// The third plane of the LDS-staged SpMM: (entry 8, the column's index in its
+// group's staged list, or the column with the top bit set).
+__global__ void loc_plane_kernel(int64_t nsl, const float *__restrict__ c2,
+                                 const int32_t *__restrict__ loc, float *__restrict__ l2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsl) {
+        l2[2 * i] = c2[2 * i];
+        l2[2 * i + 1] = __int_as_float(loc[i]);
+    }
 }
 
 // The third plane holds (entry 8, column) pairs: the column's bits go in
@@ -353,6 +405,11 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     }
     AXG_HIP(hipEventRecord(ctx->evA, s));
     launch_gather_bsr32(m, s, m->entry_of_slot.p);
+    m->bval_loc.ensure(std::max<int64_t>(2 * m->bs_nslots, 2));
+    if (m->bs_nslots)
+        hipLaunchKernelGGL(loc_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)), dim3(256),
+                           0, s, m->bs_nslots, m->bval.p + 8 * m->bs_nslots, m->bs_loc.p,
+                           m->bval_loc.p);
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     int32_t err[2];
     AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
@@ -411,7 +468,7 @@ struct InnerScalars {
 // wavefront): block j of every row is one coalesced load of two float4 planes
 // and one float plane; x of the column block (3 K floats) is gathered through
 // the caches. U blocks per lane are loaded before their x gathers.
-template <int K, int U, bool DOT, bool GF = false>
+template <int K, int U, bool DOT, bool GF = false, bool SP = false>
 __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
     int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
     const float *__restrict__ bval, int64_t nsl, const float *__restrict__ P,
@@ -421,6 +478,7 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
     const f4v *__restrict__ pb = reinterpret_cast<const f4v *>(bval + 4 * nsl);
     typedef float f2v __attribute__((ext_vector_type(2)));
     const f2v *__restrict__ pc = reinterpret_cast<const f2v *>(bval + 8 * nsl);
+    const int64_t ps = nb * K;  // plane stride of the vectors
     double dot[K];
 #pragma unroll
     for (int j = 0; j < K; j++) dot[j] = 0.0;
@@ -435,10 +493,10 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
             for (int j = 0; j < K; j++) acc[d][j] = 0.f;
         auto fma_block = [&](const f4v &va, const f4v &vb, float vc, int32_t J) {
             float x0[K], x1[K], x2[K];
-            const float *x = P + (int64_t)3 * J * K;
+            const float *x = P + (int64_t)J * K;
             load_k<K>(x, x0);
-            load_k<K>(x + K, x1);
-            load_k<K>(x + 2 * K, x2);
+            load_k<K>(x + ps, x1);
+            load_k<K>(x + 2 * ps, x2);
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 acc[0][j] += va.x * x0[j] + va.y * x1[j] + va.z * x2[j];
@@ -450,10 +508,10 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
         float pv[3][K];
         {
             const int64_t p = base;
-            const float *x = P + (int64_t)3 * I * K;
+            const float *x = P + I * K;
             load_k<K>(x, pv[0]);
-            load_k<K>(x + K, pv[1]);
-            load_k<K>(x + 2 * K, pv[2]);
+            load_k<K>(x + ps, pv[1]);
+            load_k<K>(x + 2 * ps, pv[2]);
             const f4v va = __builtin_nontemporal_load(&pa[p]);
             const f4v vb = __builtin_nontemporal_load(&pb[p]);
             const float vc = __builtin_nontemporal_load(&pc[p]).x;
@@ -483,17 +541,67 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
             }
         };
         if (nfull > 0) load_batch(0, va, vb, vc, J);
-        if (GF) {
+        if (SP) {
+            // software pipeline over two slots: at step t the x of batch t + 1 is
+            // gathered (its columns arrived during step t - 1), batch t is
+            // multiplied, then the values of batch t + 2 are loaded into the slot
+            // batch t used, so each gather has a whole step to arrive
+            struct Vals {
+                f4v a[U], b[U];
+                float c[U];
+                int32_t j[U];
+            };
+            struct Xs {
+                float x[U][3][K];
+            };
+            Vals v0, v1;
+            Xs x0, x1;
+#pragma unroll
+            for (int u = 0; u < U; u++) v0.a[u] = va[u], v0.b[u] = vb[u], v0.c[u] = vc[u], v0.j[u] = J[u];
+            auto gather_v = [&](const Vals &v, Xs &xs) {
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const float *x = P + (int64_t)v.j[u] * K;
+                    load_k<K>(x, xs.x[u][0]);
+                    load_k<K>(x + ps, xs.x[u][1]);
+                    load_k<K>(x + 2 * ps, xs.x[u][2]);
+                }
+            };
+            auto fma_v = [&](const Vals &v, const Xs &xs) {
+#pragma unroll
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int j = 0; j < K; j++) {
+                        acc[0][j] += v.a[u].x * xs.x[u][0][j] + v.a[u].y * xs.x[u][1][j] +
+                                     v.a[u].z * xs.x[u][2][j];
+                        acc[1][j] += v.a[u].w * xs.x[u][0][j] + v.b[u].x * xs.x[u][1][j] +
+                                     v.b[u].y * xs.x[u][2][j];
+                        acc[2][j] += v.b[u].z * xs.x[u][0][j] + v.b[u].w * xs.x[u][1][j] +
+                                     v.c[u] * xs.x[u][2][j];
+                    }
+            };
+            if (nfull > 0) gather_v(v0, x0);
+            if (nfull > 1) load_batch(1, v1.a, v1.b, v1.c, v1.j);
+            for (int t = 0; t < nfull; t += 2) {
+                if (t + 1 < nfull) gather_v(v1, x1);
+                fma_v(v0, x0);
+                if (t + 2 < nfull) load_batch(t + 2, v0.a, v0.b, v0.c, v0.j);
+                if (t + 1 >= nfull) break;
+                if (t + 2 < nfull) gather_v(v0, x0);
+                fma_v(v1, x1);
+                if (t + 3 < nfull) load_batch(t + 3, v1.a, v1.b, v1.c, v1.j);
+            }
+        } else if (GF) {
             // the batch's x gathers are issued before the next batch's values:
             // loads return in issue order, so waiting for the gathers then does
             // not wait for the value stream behind them
             auto gather = [&](float (&xg)[U][3][K]) {
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const float *x = P + (int64_t)3 * J[u] * K;
+                    const float *x = P + (int64_t)J[u] * K;
                     load_k<K>(x, xg[u][0]);
-                    load_k<K>(x + K, xg[u][1]);
-                    load_k<K>(x + 2 * K, xg[u][2]);
+                    load_k<K>(x + ps, xg[u][1]);
+                    load_k<K>(x + 2 * ps, xg[u][2]);
                 }
             };
             auto fma_x = [&](const float (&xg)[U][3][K]) {
@@ -551,11 +659,178 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            store_k<K>(AP + (3 * I + d) * K, acc[d]);
+            store_k<K>(AP + d * ps + I * K, acc[d]);
             if (DOT)
 #pragma unroll
                 for (int j = 0; j < K; j++) dot[j] += (double)pv[d][j] * (double)acc[d][j];
         }
+    }
+    if (!DOT) return;
+    group_sums<K, MBS>(dot);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < K; j++) part[(size_t)rb.id * K + j] = dot[j];
+}
+
+// ------------------------------------- LDS-staged block SpMM (FP32, K <= 4)
+// The same operator walk, but each workgroup step (256 block rows = 4 slices,
+// one per wavefront) first stages x of its group's column blocks in LDS with
+// coalesced loads (the sorted list built with the block pattern), then reads
+// x from LDS: the only global loads of the multiply phase are the value
+// stream, so its waits never queue behind dependent gathers. Columns beyond
+// the staged list (top bit set in the third plane) are read from global memory.
+template <int K>
+constexpr int lds_floats() {
+    return 3 * kGrpMaxCols * (K <= 4 ? K : 1);
+}
+
+// (two workgroups per CU, the LDS limit: the register budget is kept to two
+// wavefronts per SIMD)
+template <int K, int U, bool DOT>
+__global__ void __launch_bounds__(MBS, 2) bsell_spmm_lds_kernel(
+    int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+    const float *__restrict__ bval, const float *__restrict__ bloc, int64_t nsl,
+    const int32_t *__restrict__ grp_ptr, const int32_t *__restrict__ grp_col,
+    const float *__restrict__ P, float *__restrict__ AP, double *__restrict__ part,
+    InnerScalars *sc) {
+    if (DOT && sc->all_done) return;
+    __shared__ float xs[lds_floats<K>()];
+    constexpr int plane = kGrpMaxCols * K;  // LDS plane stride
+    const f4v *__restrict__ pa = reinterpret_cast<const f4v *>(bval);
+    const f4v *__restrict__ pb = reinterpret_cast<const f4v *>(bval + 4 * nsl);
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v *__restrict__ pc = reinterpret_cast<const f2v *>(bloc);
+    const int64_t ps = nb * K;
+    double dot[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) dot[j] = 0.0;
+    const RowBand rb = row_band(nb, MBS);
+    for (int64_t row0 = rb.first - threadIdx.x; row0 < nb; row0 += rb.step) {
+        const int64_t I = row0 + threadIdx.x;
+        const bool live = I < nb;
+        const int64_t sl = live ? I >> 6 : 0, base = live ? sptr[sl] + (I & 63) : 0;
+        const int w = live ? width[sl] : 0;
+        // the diagonal block's and the first batch's values are loaded before
+        // the staging loads, so their latencies overlap
+        f4v dva = {0.f, 0.f, 0.f, 0.f}, dvb = {0.f, 0.f, 0.f, 0.f};
+        f2v dcj = {0.f, 0.f};
+        if (live) {
+            dva = __builtin_nontemporal_load(&pa[base]);
+            dvb = __builtin_nontemporal_load(&pb[base]);
+            dcj = __builtin_nontemporal_load(&pc[base]);
+        }
+        const int nfull = live ? (w - 1) / U : 0;
+        f4v va[U], vb[U];
+        float vc[U];
+        int32_t lcs[U];
+        auto load_batch = [&](int b, f4v (&ra)[U], f4v (&rb)[U], float (&rc)[U],
+                              int32_t (&rj)[U]) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t p = base + 64 * (int64_t)(1 + b * U + u);
+                const f2v cj = __builtin_nontemporal_load(&pc[p]);
+                ra[u] = __builtin_nontemporal_load(&pa[p]);
+                rb[u] = __builtin_nontemporal_load(&pb[p]);
+                rc[u] = cj.x;
+                rj[u] = __float_as_int(cj.y);
+            }
+        };
+        if (nfull > 0) load_batch(0, va, vb, vc, lcs);
+        // stage x of the group's columns: every load of a thread issued before
+        // its LDS stores
+        {
+            constexpr int kIt = (kGrpMaxCols + MBS - 1) / MBS;
+            const int64_t grp = row0 / kGrpRows;
+            const int c0 = grp_ptr[grp], mcols = grp_ptr[grp + 1] - c0;
+            float sv[kIt][3][K];
+#pragma unroll
+            for (int it = 0; it < kIt; it++) {
+                const int t = threadIdx.x + it * MBS;
+                if (t < mcols) {
+                    const int64_t J = grp_col[c0 + t];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) load_k<K>(P + d * ps + J * K, sv[it][d]);
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < kIt; it++) {
+                const int t = threadIdx.x + it * MBS;
+                if (t < mcols)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) store_k<K>(xs + d * plane + t * K, sv[it][d]);
+            }
+        }
+        __syncthreads();
+        if (live) {
+            float acc[3][K];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int j = 0; j < K; j++) acc[d][j] = 0.f;
+            auto xload = [&](int32_t lc, float (&x0)[K], float (&x1)[K], float (&x2)[K]) {
+                if (lc >= 0) {
+                    const float *x = xs + lc * K;
+                    load_k<K>(x, x0);
+                    load_k<K>(x + plane, x1);
+                    load_k<K>(x + 2 * plane, x2);
+                } else {
+                    const float *x = P + (int64_t)(lc & 0x7fffffff) * K;
+                    load_k<K>(x, x0);
+                    load_k<K>(x + ps, x1);
+                    load_k<K>(x + 2 * ps, x2);
+                }
+            };
+            auto fma_block = [&](const f4v &va, const f4v &vb, float vc, int32_t lc) {
+                float x0[K], x1[K], x2[K];
+                xload(lc, x0, x1, x2);
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    acc[0][j] += va.x * x0[j] + va.y * x1[j] + va.z * x2[j];
+                    acc[1][j] += va.w * x0[j] + vb.x * x1[j] + vb.y * x2[j];
+                    acc[2][j] += vb.z * x0[j] + vb.w * x1[j] + vc * x2[j];
+                }
+            };
+            // block 0 of every row is its diagonal block: x there is this row's p
+            float pv[3][K];
+            {
+                const f4v va = dva, vb = dvb;
+                const f2v cj = dcj;
+                xload(__float_as_int(cj.y), pv[0], pv[1], pv[2]);
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    acc[0][j] += va.x * pv[0][j] + va.y * pv[1][j] + va.z * pv[2][j];
+                    acc[1][j] += va.w * pv[0][j] + vb.x * pv[1][j] + vb.y * pv[2][j];
+                    acc[2][j] += vb.z * pv[0][j] + vb.w * pv[1][j] + cj.x * pv[2][j];
+                }
+            }
+            for (int b = 0; b < nfull; b++) {
+                f4v na[U], nbv[U];
+                float nc[U];
+                int32_t nj[U];
+                const bool more = b + 1 < nfull;
+                if (more) load_batch(b + 1, na, nbv, nc, nj);
+#pragma unroll
+                for (int u = 0; u < U; u++) fma_block(va[u], vb[u], vc[u], lcs[u]);
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        va[u] = na[u], vb[u] = nbv[u], vc[u] = nc[u], lcs[u] = nj[u];
+                }
+            }
+            for (int jb = 1 + nfull * U; jb < w; jb++) {
+                const int64_t p = base + 64 * (int64_t)jb;
+                const f2v cj = pc[p];
+                fma_block(pa[p], pb[p], cj.x, __float_as_int(cj.y));
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                store_k<K>(AP + d * ps + I * K, acc[d]);
+                if (DOT)
+#pragma unroll
+                    for (int j = 0; j < K; j++) dot[j] += (double)pv[d][j] * (double)acc[d][j];
+            }
+        }
+        __syncthreads();  // the next group's staging overwrites xs
     }
     if (!DOT) return;
     group_sums<K, MBS>(dot);
@@ -598,12 +873,13 @@ __global__ void __launch_bounds__(MBS) inner_step_kernel(
     double acc[2 * K];  // ||r||^2 then r.z
 #pragma unroll
     for (int j = 0; j < 2 * K; j++) acc[j] = 0.0;
+    const int64_t ps = nb * K;
     for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
          b += (int64_t)gridDim.x * MBS) {
         float r[3][K];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const int64_t o = (3 * b + d) * K;
+            const int64_t o = d * ps + b * K;
             float x[K], p[K], ap[K];
             load_k<K>(R + o, r[d]);
             load_k<K>(X + o, x);
@@ -632,7 +908,7 @@ __global__ void __launch_bounds__(MBS) inner_step_kernel(
                 acc[j] += (double)r[d][j] * r[d][j];
                 acc[K + j] += (double)r[d][j] * z[j];
             }
-            store_k<K>(Z + (3 * b + d) * K, z);
+            store_k<K>(Z + d * ps + b * K, z);
         }
     }
     group_sums<2 * K, MBS>(acc);
@@ -779,6 +1055,7 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
     float rat[K];
 #pragma unroll
     for (int j = 0; j < K; j++) rat[j] = ratio ? (float)ratio[j] : 0.f;
+    const int64_t ps = nb * K;
     for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
          b += (int64_t)gridDim.x * MBS) {
         float r[3][K], zero[K];
@@ -786,15 +1063,16 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
         for (int j = 0; j < K; j++) zero[j] = 0.f;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            load_k<K>(B + (3 * b + d) * K, r[d]);
-            store_k<K>(R + (3 * b + d) * K, r[d]);
-            store_k<K>(X + (3 * b + d) * K, zero);
+            const int64_t o = d * ps + b * K;
+            load_k<K>(B + o, r[d]);
+            store_k<K>(R + o, r[d]);
+            store_k<K>(X + o, zero);
             if (ratio) {
                 float p[K];
-                load_k<K>(P + (3 * b + d) * K, p);
+                load_k<K>(P + o, p);
 #pragma unroll
                 for (int j = 0; j < K; j++) p[j] = rat[j] > 0.f ? p[j] * rat[j] : 0.f;
-                store_k<K>(P + (3 * b + d) * K, p);
+                store_k<K>(P + o, p);
             }
         }
 #pragma unroll
@@ -808,7 +1086,7 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
                 acc[j] += (double)r[d][j] * r[d][j];
                 acc[K + j] += (double)r[d][j] * z[j];
             }
-            store_k<K>(Z + (3 * b + d) * K, z);
+            store_k<K>(Z + d * ps + b * K, z);
         }
     }
     group_sums<2 * K, MBS>(acc);
@@ -819,17 +1097,23 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
 }
 
 // FP64 free-space residual -> FP32 block space, scaled by 1/|r_j|.
+// Offset of block-space row bi = 3 b + d in the planar layout.
+template <int K>
+__device__ inline int64_t plane_off(int64_t bi, int64_t nb) {
+    return (bi % 3) * nb * K + (bi / 3) * K;
+}
+
 template <int K>
 __global__ void pack_kernel(int64_t n, const double *__restrict__ Y,
                             const int32_t *__restrict__ blk_of_free,
-                            const double *__restrict__ scale, float *__restrict__ B) {
+                            const double *__restrict__ scale, float *__restrict__ B, int64_t nb) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int64_t bi = blk_of_free[i];
+    const int64_t o = plane_off<K>(blk_of_free[i], nb);
 #pragma unroll
     for (int j = 0; j < K; j++) {
         double s = scale[j];
-        B[bi * K + j] = s > 0.0 ? (float)(Y[i + j * n] / s) : 0.f;
+        B[o + j] = s > 0.0 ? (float)(Y[i + j * n] / s) : 0.f;
     }
 }
 
@@ -837,31 +1121,109 @@ __global__ void pack_kernel(int64_t n, const double *__restrict__ Y,
 template <int K>
 __global__ void unpack_add_kernel(int64_t n, const float *__restrict__ D,
                                   const int32_t *__restrict__ blk_of_free,
-                                  const double *__restrict__ scale, double *__restrict__ U) {
+                                  const double *__restrict__ scale, double *__restrict__ U,
+                                  int64_t nb) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int64_t bi = blk_of_free[i];
+    const int64_t o = plane_off<K>(blk_of_free[i], nb);
 #pragma unroll
-    for (int j = 0; j < K; j++) U[i + j * n] += scale[j] * (double)D[bi * K + j];
+    for (int j = 0; j < K; j++) U[i + j * n] += scale[j] * (double)D[o + j];
 }
 
 // ----------------------------------------------------- SpMM shapes
 // U (blocks loaded per lane before their gathers) is picked per operator and
 // right-hand-side count by timing the candidates.
 // Shapes 0-3: U = 1..4 blocks per batch, gathers after the next batch's
-// values; shapes 4-7: the same with the gathers issued first.
-static const int kNumSpmm = 8;
-#define AXG_SPMM_SWITCH(IDX, MACRO)     \
-    switch (IDX) {                      \
-        case 0: MACRO(1, false); break; \
-        case 1: MACRO(2, false); break; \
-        case 2: MACRO(3, false); break; \
-        case 3: MACRO(4, false); break; \
-        case 4: MACRO(1, true); break;  \
-        case 5: MACRO(2, true); break;  \
-        case 6: MACRO(3, true); break;  \
-        default: MACRO(4, true); break; \
+// values; shapes 4-7: the same with the gathers issued first; shapes 8-10:
+// U = 1..3 software-pipelined (the gathers of the next batch issued before
+// the current batch is multiplied); shapes 11-12: U = 6, 8 (one wavefront
+// per SIMD, more values in flight per wavefront); shapes 13-16: U = 1..4 with
+// x staged in LDS per group of 256 block rows (K <= 4 only).
+static const int kNumSpmm = 17;
+int spmm_shape_count() { return kNumSpmm; }
+bool spmm_shape_ok(int shape, int K) { return shape >= 0 && shape < kNumSpmm && (shape < 13 || K <= 4); }
+
+template <class F>
+static void spmm_shape_switch(int shape, F &&f) {
+    switch (shape) {
+        case 0: f.template run<1, false, false, false>(); break;
+        case 1: f.template run<2, false, false, false>(); break;
+        case 2: f.template run<3, false, false, false>(); break;
+        case 3: f.template run<4, false, false, false>(); break;
+        case 4: f.template run<1, true, false, false>(); break;
+        case 5: f.template run<2, true, false, false>(); break;
+        case 6: f.template run<3, true, false, false>(); break;
+        case 7: f.template run<4, true, false, false>(); break;
+        case 8: f.template run<1, false, true, false>(); break;
+        case 9: f.template run<2, false, true, false>(); break;
+        case 10: f.template run<3, false, true, false>(); break;
+        case 11: f.template run<6, false, false, false>(); break;
+        case 12: f.template run<8, false, false, false>(); break;
+        case 13: f.template run<1, false, false, true>(); break;
+        case 14: f.template run<2, false, false, true>(); break;
+        case 15: f.template run<3, false, false, true>(); break;
+        default: f.template run<4, false, false, true>(); break;
     }
+}
+
+// One launch of a shape of the FP32 block SpMM (events: stamped by the dispatch).
+template <int K, bool DOT>
+struct SpmmLaunch {
+    axg_mesh *m;
+    int grid;
+    hipStream_t s;
+    const float *P;
+    float *AP;
+    double *part;
+    InnerScalars *sc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    template <int U, bool GF, bool SP, bool LDS>
+    void run() {
+        if constexpr (LDS) {
+            if constexpr (K <= 4) {
+                auto k = bsell_spmm_lds_kernel<K, U, DOT>;
+                if (e0)
+                    hipExtLaunchKernelGGL(k, dim3(grid), dim3(MBS), 0, s, e0, e1, 0, m->nb,
+                                          (const int64_t *)m->bs_sptr.p, (const int32_t *)m->bs_width.p,
+                                          (const float *)m->bval.p, (const float *)m->bval_loc.p,
+                                          m->bs_nslots, (const int32_t *)m->grp_ptr.p,
+                                          (const int32_t *)m->grp_col.p, P, AP, part, sc);
+                else
+                    hipLaunchKernelGGL(k, dim3(grid), dim3(MBS), 0, s, m->nb,
+                                       (const int64_t *)m->bs_sptr.p, (const int32_t *)m->bs_width.p,
+                                       (const float *)m->bval.p, (const float *)m->bval_loc.p,
+                                       m->bs_nslots, (const int32_t *)m->grp_ptr.p,
+                                       (const int32_t *)m->grp_col.p, P, AP, part, sc);
+            }
+        } else {
+            auto k = bsell_spmm_kernel<K, U, DOT, GF, SP>;
+            if (e0)
+                hipExtLaunchKernelGGL(k, dim3(grid), dim3(MBS), 0, s, e0, e1, 0, m->nb,
+                                      (const int64_t *)m->bs_sptr.p, (const int32_t *)m->bs_width.p,
+                                      (const float *)m->bval.p, m->bs_nslots, P, AP, part, sc);
+            else
+                hipLaunchKernelGGL(k, dim3(grid), dim3(MBS), 0, s, m->nb,
+                                   (const int64_t *)m->bs_sptr.p, (const int32_t *)m->bs_width.p,
+                                   (const float *)m->bval.p, m->bs_nslots, P, AP, part, sc);
+        }
+    }
+};
+
+template <int K, bool DOT>
+struct SpmmOccupancy {
+    int per_cu = 1;
+    template <int U, bool GF, bool SP, bool LDS>
+    void run() {
+        if constexpr (LDS) {
+            if constexpr (K <= 4)
+                per_cu = resident_per_cu(
+                    reinterpret_cast<const void *>(bsell_spmm_lds_kernel<K, U, DOT>), MBS);
+        } else {
+            per_cu = resident_per_cu(
+                reinterpret_cast<const void *>(bsell_spmm_kernel<K, U, DOT, GF, SP>), MBS);
+        }
+    }
+};
 
 // Workgroups of one SpMM launch: the same number on every CU (4, or fewer
 // when the variant's occupancy is lower). Measured on the 80^3 brick, an even
@@ -869,13 +1231,19 @@ static const int kNumSpmm = 8;
 // workgroups beat 1032 (a few CUs with one extra) by 25 %, 2048 by 7 %.
 template <int K, bool DOT>
 static int spmm_grid(axg_ctx *ctx, int64_t nb, int shape) {
-    int per_cu = 1;
-#define AXG_OCC(UU, GG) \
-    per_cu = resident_per_cu(reinterpret_cast<const void *>(bsell_spmm_kernel<K, UU, DOT, GG>), MBS)
-    AXG_SPMM_SWITCH(shape, AXG_OCC)
-#undef AXG_OCC
+    SpmmOccupancy<K, DOT> occ;
+    spmm_shape_switch(shape, occ);
     return std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
-                                              (int64_t)ctx->cu_count * std::min(per_cu, 4)));
+                                              (int64_t)ctx->cu_count * std::min(occ.per_cu, 4)));
+}
+
+template <int K, bool DOT>
+static void launch_spmm_shape(axg_mesh *m, int shape, int grid, const float *P, float *AP,
+                              double *part, InnerScalars *sc, hipEvent_t e0 = nullptr,
+                              hipEvent_t e1 = nullptr) {
+    AXG_REQUIRE(spmm_shape_ok(shape, K), AXG_E_INVALID, "SpMM shape not available for this width");
+    SpmmLaunch<K, DOT> l{m, grid, m->ctx->stream, P, AP, part, sc, e0, e1};
+    spmm_shape_switch(shape, l);
 }
 
 template <int K>
@@ -887,17 +1255,11 @@ static int tune_spmm(axg_mesh *m, const float *P, float *AP) {
     if (m->nnzb > 200000) {
         float best_ms = 1e30f;
         for (int c = 0; c < kNumSpmm; c++) {
+            if (!spmm_shape_ok(c, K)) continue;
             const int grid = spmm_grid<K, false>(ctx, m->nb, c);
-#define AXG_T(UU, GG)                                                                            \
-    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false, GG>), dim3(grid), dim3(MBS), 0, s, m->nb, \
-                       m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots, P, AP,  \
-                       nullptr, nullptr)
-            AXG_SPMM_SWITCH(c, AXG_T)
+            launch_spmm_shape<K, false>(m, c, grid, P, AP, nullptr, nullptr);
             AXG_HIP(hipEventRecord(ctx->ev0, s));
-            for (int r = 0; r < 5; r++) {
-                AXG_SPMM_SWITCH(c, AXG_T)
-            }
-#undef AXG_T
+            for (int r = 0; r < 5; r++) launch_spmm_shape<K, false>(m, c, grid, P, AP, nullptr, nullptr);
             AXG_HIP(hipEventRecord(ctx->ev1, s));
             AXG_HIP(hipEventSynchronize(ctx->ev1));
             float ms = 0.f;
@@ -960,12 +1322,13 @@ __global__ void __launch_bounds__(MBS) col_dot_kernel(int64_t n, const double *_
 template <int K>
 __global__ void unpack_kernel(int64_t n, const float *__restrict__ D,
                               const int32_t *__restrict__ blk_of_free,
-                              const double *__restrict__ scale, double *__restrict__ Z) {
+                              const double *__restrict__ scale, double *__restrict__ Z,
+                              int64_t nb) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int64_t bi = blk_of_free[i];
+    const int64_t o = plane_off<K>(blk_of_free[i], nb);
 #pragma unroll
-    for (int j = 0; j < K; j++) Z[i + j * n] = scale[j] * (double)D[bi * K + j];
+    for (int j = 0; j < K; j++) Z[i + j * n] = scale[j] * (double)D[o + j];
 }
 
 // Outer direction: P = Z (first) or Z + beta_j P.
@@ -1003,16 +1366,9 @@ static void bench_spmm_k(axg_mesh *m, int reps, double *avg_ms) {
     AXG_HIP(hipMemsetAsync(P.p, 0, 4 * nv * K, s));
     const int shp = tune_spmm<K>(m, P.p, AP.p);
     const int grid = spmm_grid<K, false>(ctx, m->nb, shp);
-#define AXG_T(UU, GG)                                                                            \
-    hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, false, GG>), dim3(grid), dim3(MBS), 0, s, m->nb, \
-                       m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots, P.p,    \
-                       AP.p, nullptr, nullptr)
-    AXG_SPMM_SWITCH(shp, AXG_T)
+    launch_spmm_shape<K, false>(m, shp, grid, P.p, AP.p, nullptr, nullptr);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
-    for (int r = 0; r < reps; r++) {
-        AXG_SPMM_SWITCH(shp, AXG_T)
-    }
-#undef AXG_T
+    for (int r = 0; r < reps; r++) launch_spmm_shape<K, false>(m, shp, grid, P.p, AP.p, nullptr, nullptr);
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     AXG_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
@@ -1074,19 +1430,8 @@ struct MixedSolver {
         // timing: the events are stamped by the dispatch itself (kernel start
         // and end), not by separate stream markers
         auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1, int par) {
-            InnerScalars *cur = dsc.p + par;
-#define AXG_SPMM(UU, GG)                                                                        \
-    if (e0)                                                                                     \
-        hipExtLaunchKernelGGL((bsell_spmm_kernel<K, UU, true, GG>), dim3(grid_mv), dim3(MBS), 0, s, \
-                              e0, e1, 0, nb, (const int64_t *)m->bs_sptr.p,                     \
-                              (const int32_t *)m->bs_width.p, (const float *)m->bval.p,          \
-                              m->bs_nslots, (const float *)w.f3.p, w.f4.p, w.part.p, cur);      \
-    else                                                                                        \
-        hipLaunchKernelGGL((bsell_spmm_kernel<K, UU, true, GG>), dim3(grid_mv), dim3(MBS), 0, s, nb, \
-                           m->bs_sptr.p, m->bs_width.p, m->bval.p, m->bs_nslots,    \
-                           w.f3.p, w.f4.p, w.part.p, cur)
-            AXG_SPMM_SWITCH(shp, AXG_SPMM)
-#undef AXG_SPMM
+            launch_spmm_shape<K, true>(m, shp, grid_mv, w.f3.p, w.f4.p, w.part.p, dsc.p + par, e0,
+                                       e1);
         };
         auto launch_update = [&](int par) {
             InnerScalars *cur = dsc.p + par, *next = dsc.p + (par ^ 1);
@@ -1166,7 +1511,7 @@ struct MixedSolver {
         auto inner_solve = [&](const double (&scale)[K], bool warm) {
             put(8, scale);
             hipLaunchKernelGGL((pack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.t0.p,
-                               m->blk_of_free.p, w.t1.p + 8, w.f0.p);
+                               m->blk_of_free.p, w.t1.p + 8, w.f0.p, nb);
             if (warm) {
                 double ratio[K], rzs[K];
                 for (int j = 0; j < K; j++) {
@@ -1264,7 +1609,7 @@ struct MixedSolver {
                 sweeps++;
                 inner_solve(scale, false);
                 hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
-                                   w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u);
+                                   w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u, nb);
                 true_residual_check();
             }
         } else if (o.refinement == AXG_REFINE_RELIABLE) {
@@ -1284,7 +1629,7 @@ struct MixedSolver {
                 inner_solve(scale, warm);
                 warm = true;
                 hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
-                                   w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u);
+                                   w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u, nb);
                 true_residual_check();
             }
         } else {
@@ -1310,7 +1655,7 @@ struct MixedSolver {
                 sweeps++;
                 inner_solve(scale, false);
                 hipLaunchKernelGGL((unpack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.f1.p,
-                                   m->blk_of_free.p, w.t1.p + 8, Zd.p);
+                                   m->blk_of_free.p, w.t1.p + 8, Zd.p, nb);
                 double beta[K];
                 if (first) {
                     for (int j = 0; j < K; j++) beta[j] = 0.0;

@@ -358,6 +358,10 @@ int axg_bench_stream(axg_ctx *ctx, int64_t n, int32_t reps, double *copy_gbs);
  * sliced-ELL SpMM of the multi-right-hand-side block PCG (the eigen and
  * buckling solves) with nrhs right-hand sides (1, 2, 4 or 8). */
 int axg_bench_operator(axg_mesh *mesh, int32_t which, int32_t nrhs, int32_t reps, double *avg_ms);
+/* The FP32 block SpMM variant of a mesh's mixed solves with nrhs (1..8)
+ * right-hand sides: *shape >= 0 pins it (instead of the autotuner's pick at
+ * the first solve), then *shape receives the active one (-1: not tuned yet). */
+int axg_mesh_spmm_shape(axg_mesh *mesh, int32_t nrhs, int32_t *shape);
 
 /* ------------------------------------------------- host-only helpers */
 /* The symbolic phase alone, on the host (no device): K_ff pattern and scatter

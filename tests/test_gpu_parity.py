@@ -360,6 +360,29 @@ def test_mixed_refinement_modes_match_fp64(ctx, refinement):
     assert info["rel_residual"] <= 1e-12
 
 
+def test_mixed_every_spmm_shape():
+    """Every FP32 block SpMM variant the autotuner can pick (pinned one after the other)
+    gives a mixed solve within 1e-9 of the FP64 solve, at 4 and 1 right-hand sides."""
+    say("case: every SpMM shape")
+    c = G.Context(0)
+    p = MG.brick_lognormal(11, seed=9, nloads=4)
+    B = p.rhs_all()
+    ref = None
+    for shape in range(17):
+        mesh = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=c)
+        mesh.assemble("mixed")
+        if ref is None:
+            ref, _ = mesh.solve(B, precision="fp64", tol=1e-13)
+        assert mesh.spmm_shape(4, shape) == shape and mesh.spmm_shape(1, shape) == shape
+        X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-12)
+        x1, _ = mesh.solve(B[2], precision="mixed", precond="jacobi", tol=1e-12)
+        err = max(rel(X[k], ref[k]) for k in range(4))
+        say(f"  shape {shape}: inner {info['iters']}, err {err:.2e}, single {rel(x1, ref[2]):.2e}")
+        assert err < 1e-9 and rel(x1, ref[2]) < 1e-9
+        mesh.close()
+    c.close()
+
+
 def test_mixed_c3d20_stiff_beam_floor():
     """A slender jittered C3D20 beam: its FP64 residual has a round-off floor near 1e-10
     relative. The mixed solve reaches 1e-9 (displacements within 1e-8 of FP64), and an
