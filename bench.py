@@ -179,6 +179,16 @@ def bsell64_bytes(nb, nnzb, k):
     return 76 * nnzb + 2 * 8 * 3 * nb * k
 
 
+def spmm_kernel_name(k, shape):
+    """The kernel behind an FP32 SpMM shape of the autotuner (axg_mixed.hip's shape table),
+    as rocprofv3 names it (the solve's launches carry the p.Ap partials: DOT = true)."""
+    if shape >= 13:
+        return f"axg::bsell_spmm_lds_kernel<{k}, {shape - 12}, true>"
+    u = {0: 1, 1: 2, 2: 3, 3: 4, 4: 1, 5: 2, 6: 3, 7: 4, 8: 1, 9: 2, 10: 3, 11: 6, 12: 8}[shape]
+    return f"axg::bsell_spmm_kernel<{k}, {u}, true, {str(4 <= shape <= 7).lower()}, " \
+           f"{str(8 <= shape <= 10).lower()}>"
+
+
 def traffic_for(kernel_key, workload_key):
     try:
         prof = json.load(open(os.path.join(ROOT, "profiles", "hbm_traffic_brick80.json")))
@@ -370,9 +380,9 @@ def run_brick(a, G, MG, torch, np):
         },
         "speedup_mixed_vs_fp64_plain": (plain["solve_ms"] / solve_ms) if plain else None,
         "speedup_mixed_vs_fp64_best": (fp64["solve_ms"] / solve_ms) if fp64 else None,
-        "roofline": roofline(f"bsell_spmm_kernel<K={k}> (FP32 block-sliced-ELL SpMM)",
-                             spmm_bytes(nb, nnzb, k), spmv_ms,
-                             traffic_for(f"axg::bsell_spmm_kernel<{k}, U, true>", f"brick{a.n}")),
+        "roofline": roofline(spmm_kernel_name(k, mesh.spmm_shape(k)) +
+                             " (FP32 block-sliced-ELL SpMM)", spmm_bytes(nb, nnzb, k), spmv_ms,
+                             traffic_for(spmm_kernel_name(k, mesh.spmm_shape(k)), f"brick{a.n}")),
         "reference_fp64_iters": ref["iters"],
     })
     if not a.no_cpu:
