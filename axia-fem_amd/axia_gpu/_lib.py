@@ -12,7 +12,10 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_PKG), "lib", "libaxia_gpu.so")
+# AXIA_GPU_LIB selects another build of the same library (the sanitizer builds of
+# sanitize.py); by default the in-tree build
+LIB_PATH = os.environ.get("AXIA_GPU_LIB") or os.path.join(os.path.dirname(_PKG), "lib",
+                                                           "libaxia_gpu.so")
 
 AXG_OK = 0
 AXG_E_INVALID = 1

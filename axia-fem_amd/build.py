@@ -40,9 +40,9 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def compile_one(src):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    flags = list(COMMON)
+def compile_one(src, obj_dir=OBJ, extra=()):
+    obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+    flags = list(COMMON) + list(extra)
     if src in EXACT:
         flags.append("-ffp-contract=off")
     if src.endswith(".cpp"):
@@ -54,21 +54,24 @@ def compile_one(src):
     return obj
 
 
-def build(jobs=None, verbose=True):
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+def build(jobs=None, verbose=True, lib=LIB, obj_dir=OBJ, extra=(), link_extra=()):
+    """Compile every source and link lib. extra / link_extra: additional compile / link
+    flags (the sanitizer builds of sanitize.py use them)."""
+    os.makedirs(obj_dir, exist_ok=True)
+    os.makedirs(os.path.dirname(lib), exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
+        objs = list(ex.map(lambda src: compile_one(src, obj_dir, extra), SOURCES))
+    tmp = lib + ".tmp"
+    cmd = ([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs +
+           ["-lpthread"] + list(link_extra))
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {lib}")
+    return lib
 
 
 if __name__ == "__main__":

@@ -70,6 +70,29 @@ def test_symbolic_pattern_and_scatter_map_vs_oracle(maker):
         assert np.all(np.diff(els) > 0)
 
 
+def test_symbolic_pattern_on_host_threads():
+    """A mesh large enough for the symbolic phase to split its rows over host threads
+    (C3D8 28^3, 24,389 nodes): the pattern equals the oracle's csr_from_triplets pattern,
+    and every slot's contributions are its (element, i, j) entries in ascending element
+    order, each naming the slot's row and column DOFs."""
+    p = MG.brick_lognormal(28, seed=2)
+    csr = oracle_csr(p)
+    rp, ci, sp, cb = G.pattern_host(p.nn, p.nnode, p.conn, p.ind_of)
+    assert np.array_equal(rp, csr.indptr) and np.array_equal(ci, csr.indices)
+    m = 3 * p.nn
+    el, rest = cb // (m * m), cb % (m * m)
+    li, lj = rest // m, rest % m
+    dof_i = 3 * p.conn[el, li // 3].astype(np.int64) + li % 3
+    dof_j = 3 * p.conn[el, lj // 3].astype(np.int64) + lj % 3
+    slot = np.repeat(np.arange(len(ci)), np.diff(sp))
+    row = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    assert np.array_equal(p.ind_of[dof_i], row[slot])
+    assert np.array_equal(p.ind_of[dof_j], ci[slot])
+    first = np.ones(len(cb), bool)
+    first[sp[:-1]] = False  # the first contribution of each slot has no predecessor
+    assert np.all(np.diff(el)[first[1:]] > 0)
+
+
 def test_symbolic_partial_constraints():
     """Nodes with only some DOFs prescribed (LE10-style YS,2,2) keep their other rows."""
     xyz, conn = MG.brick_hex8(3, 2, 2)
