@@ -174,7 +174,8 @@ class Mesh:
         return ms.value
 
     def assembly_times(self):
-        """Device ms of the last assembly: ke64, gather64, ke32, gather32."""
+        """Device ms of the last assembly: ke64, gather64, ke32 (the fused FP32 integration
+        and block assembly), gather32 (0: no separate FP32 gather)."""
         v = [C.c_double() for _ in range(4)]
         check(_lib.lib().axg_assembly_times(self._h, *[C.byref(x) for x in v]))
         return {k: x.value for k, x in zip(("ke64", "gather64", "ke32", "gather32"), v)}
@@ -202,13 +203,21 @@ class Mesh:
         check(_lib.lib().axg_ke_export(self._h, first, count, dptr(out)))
         return out.reshape(count, m, m)
 
-    def ke32(self, first=0, count=None):
-        count = self.nelem - first if count is None else count
-        m = 3 * self.nn
-        out = np.zeros(count * m * m, np.float32)
-        check(_lib.lib().axg_ke32_export(self._h, first, count,
-                                         out.ctypes.data_as(C.POINTER(C.c_float))))
-        return out.reshape(count, m, m)
+    def op32(self):
+        """The FP32 inner operator of the mixed solve as a scipy CSR matrix over the
+        free DOFs (after assemble("mixed"))."""
+        import scipy.sparse as sp
+        lib = _lib.lib()
+        cnt = C.c_int64(0)
+        check(lib.axg_op32_export(self._h, C.byref(cnt), None, None, None))
+        n = cnt.value
+        rows, cols = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        vals = np.zeros(n, np.float32)
+        ip = C.POINTER(C.c_int32)
+        check(lib.axg_op32_export(self._h, C.byref(cnt), rows.ctypes.data_as(ip),
+                                  cols.ctypes.data_as(ip),
+                                  vals.ctypes.data_as(C.POINTER(C.c_float))))
+        return sp.csr_matrix((vals, (rows, cols)), shape=(self.n_free, self.n_free))
 
     def solve(self, rhs, precision="fp64", precond="jacobi", tol=1e-8, max_iter=0,
               inner_tol=1e-3, max_outer=30, timing=False, refinement="fcg"):

@@ -105,7 +105,8 @@ SIGNATURES = {
     "axg_csr_export": (C.c_int, [_vp, _lp, _lp, _dp]),
     "axg_scatter_map_export": (C.c_int, [_vp, _lp, _lp]),
     "axg_ke_export": (C.c_int, [_vp, _i64, _i64, _dp]),
-    "axg_ke32_export": (C.c_int, [_vp, _i64, _i64, C.POINTER(C.c_float)]),
+    "axg_op32_export": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "axg_mesh_solve": (C.c_int, [_vp, _dp, _i32, C.POINTER(SolveOpts), _dp, C.POINTER(SolveInfo)]),
     "axg_mesh_solve_device": (C.c_int, [_vp, _vp, _i32, C.POINTER(SolveOpts), _vp,
                                         C.POINTER(SolveInfo)]),

@@ -756,13 +756,46 @@ int axg_ke_export(axg_mesh *mesh, int64_t first, int64_t count, double *ke) {
     API_END
 }
 
-int axg_ke32_export(axg_mesh *mesh, int64_t first, int64_t count, float *ke) {
+int axg_op32_export(axg_mesh *mesh, int64_t *count, int32_t *rows, int32_t *cols, float *vals) {
     API_BEGIN
-    AXG_REQUIRE(mesh && ke && first >= 0 && count >= 0 && first + count <= mesh->nelem,
-                AXG_E_INVALID, "axg_ke32_export: invalid range");
-    AXG_REQUIRE(mesh->assembled32, AXG_E_INVALID, "axg_ke32_export: mixed operator not assembled");
-    size_t mm = (size_t)mesh->m * mesh->m;
-    AXG_HIP(hipMemcpy(ke, mesh->ke32.p + first * mm, 4 * mm * count, hipMemcpyDeviceToHost));
+    AXG_REQUIRE(mesh && count, AXG_E_INVALID, "axg_op32_export: NULL argument");
+    AXG_REQUIRE(mesh->assembled32, AXG_E_INVALID, "axg_op32_export: mixed operator not assembled");
+    AXG_REQUIRE(!rows == !cols && !rows == !vals, AXG_E_INVALID,
+                "axg_op32_export: rows, cols and vals are all given or all NULL");
+    const int64_t nb = mesh->nb, nsl = mesh->bs_nslots, ns = (nb + 63) / 64;
+    hipStream_t s = mesh->ctx->stream;
+    std::vector<int64_t> sptr(ns + 1);
+    std::vector<int32_t> width(ns), bcol(nsl), fob(3 * nb);
+    std::vector<float> bv(10 * nsl);
+    AXG_HIP(hipMemcpyAsync(sptr.data(), mesh->bs_sptr.p, 8 * (ns + 1), hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(width.data(), mesh->bs_width.p, 4 * ns, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(bcol.data(), mesh->bs_col.p, 4 * nsl, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(fob.data(), mesh->free_of_blk.p, 4 * 3 * nb, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipMemcpyAsync(bv.data(), mesh->bval.p, 4 * 10 * nsl, hipMemcpyDeviceToHost, s));
+    AXG_HIP(hipStreamSynchronize(s));
+    const int64_t cap = *count;
+    int64_t n = 0;
+    for (int64_t I = 0; I < nb; I++) {
+        const int64_t base = sptr[I >> 6] + (I & 63);
+        for (int64_t k = 0; k < width[I >> 6]; k++) {
+            const int64_t pos = base + 64 * k;
+            const int32_t J = bcol[pos];
+            if (k > 0 && J == (int32_t)I) continue;  // padding slot
+            for (int q = 0; q < 9; q++) {
+                const int32_t fi = fob[3 * I + q / 3], fj = fob[3 * (int64_t)J + q % 3];
+                if (fi < 0 || fj < 0) continue;
+                if (rows) {
+                    AXG_REQUIRE(n < cap, AXG_E_INVALID, "axg_op32_export: arrays too small");
+                    rows[n] = fi;
+                    cols[n] = fj;
+                    vals[n] = q < 4 ? bv[4 * pos + q] : q < 8 ? bv[4 * nsl + 4 * pos + q - 4]
+                                                              : bv[8 * nsl + 2 * pos];
+                }
+                n++;
+            }
+        }
+    }
+    *count = n;
     API_END
 }
 

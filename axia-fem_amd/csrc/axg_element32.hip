@@ -1,13 +1,31 @@
-// FP32 element integration for the mixed-precision operator, and the gather
-// of FP32 element matrices into the FP32 BSR-3 blocks.
+// FP32 assembly of the mixed-precision operator straight into the BSR-3
+// planes, fused with the element integration (no element matrix is stored).
 //
-// Ke = sum_g w_g B_g^T D B_g is evaluated as one dense contraction per
-// element, Ke = Bs^T Cs, where Bs stacks the strain-displacement matrices of
-// all integration points (6 NG x 3 NN) and Cs = blockdiag(w_g D) Bs. That
-// product runs on the matrix cores: v_mfma_f32_32x32x2_f32, one wavefront per
-// element, 32x32 output tiles (1 tile for C3D8, 2x2 tiles for C3D20), K
-// walked two strain rows at a time. Lane l supplies A = Bs^T[row][k] and
-// B = Cs[k][col] with row/col = tile offset + l % 32 and k = 2 t + l / 32.
+// For isotropic linear elasticity the 3x3 block of B^T D B that couples the
+// local nodes a and b of an element is, summed over its Gauss points g,
+//   K_ab = lam P + mu P^T + mu tr(P) 1,
+//   P    = sum_g s_g^2 grad N_a(g) grad N_b(g)^T,   s_g^2 = w_g det J_g,
+// (lam, mu the Lame constants of the element's E, nu). This is B^T D B of the
+// element formulation (elem.rs:90-112, 199-219; quadratic.rs:122-146) with the
+// zero structure of B and D multiplied out: nine FMAs per (a, b, g) instead of
+// the dense 6 x 3NN contraction.
+//
+// The kernel is node-centric: one wavefront owns one block row I (node i) and
+// builds it from the elements incident to i, in chunks of CH elements:
+//   1. element-local coordinates of the chunk (FP64 differences to the
+//      element's first node, rounded to FP32) into LDS;
+//   2. lane per (element, Gauss point): J, its inverse and s_g, then the
+//      scaled gradients s_g grad N_b of every local node b into LDS;
+//   3. lane per (element, b): P over the Gauss points and K_ab, and the slot
+//      of b's node in row I (binary search of the row's sorted columns);
+//   4. lane per slot: the chunk's K_ab added in ascending element order.
+// Every block is summed in the same fixed order (incident elements ascending)
+// on every run, and written once, with its column, to the three value planes
+// and to the third plane of the LDS-staged SpMM. Constrained DOFs give zero
+// entries. Workgroups are mapped XCD-aware: the XCD that runs workgroup b
+// (b % 8) takes a contiguous eighth of the rows, so the 16-byte block writes
+// of neighbouring rows meet in one L2.
+#include <climits>
 #include <cmath>
 
 #include "axg_internal.h"
@@ -33,190 +51,399 @@ __device__ inline void report32(int32_t *err, int32_t code, int64_t el) {
     if (atomicCAS(&err[0], 0, code) == 0) err[1] = (int32_t)el;
 }
 
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-
-// Strain row k (Voigt xx yy zz xy yz zx) of B for DOF component p: which
-// dN/dx component it holds (0..2) or 3 for an exact zero; 2 bits per (k, p).
-__device__ inline int bcomp(int k, int p) {
-    // k = 0: (0,3,3)  1: (3,1,3)  2: (3,3,2)  3: (1,0,3)  4: (3,2,1)  5: (2,3,0)
-    constexpr unsigned long long tab =
-        (0ull << 0) | (3ull << 2) | (3ull << 4) |      // k = 0
-        (3ull << 6) | (1ull << 8) | (3ull << 10) |     // k = 1
-        (3ull << 12) | (3ull << 14) | (2ull << 16) |   // k = 2
-        (1ull << 18) | (0ull << 20) | (3ull << 22) |   // k = 3
-        (3ull << 24) | (2ull << 26) | (1ull << 28) |   // k = 4
-        (2ull << 30) | (3ull << 32) | (0ull << 34);    // k = 5
-    return (int)((tab >> (2 * (3 * k + p))) & 3ull);
+// LDS written by some lanes of a wavefront and read by others: the wave's LDS
+// operations complete in order, the fences keep the compiler from moving them.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront (one workgroup) per element, grid-stride over elements.
-// Phase 1 (lanes < NG): J, its inverse and w det at every point, in FP32 on
-// element-local coordinates (node 0 subtracted in FP64). Phase 2 (all lanes):
-// dN/dx of every (point, node). Phase 3: Ke = Bs^T Cs on the matrix cores.
-template <int NN, int NG>
-__global__ void __launch_bounds__(64) ke_fp32_mfma_kernel(const double *__restrict__ xyz,
-                                                          const int32_t *__restrict__ conn,
-                                                          const double *__restrict__ E,
-                                                          const double *__restrict__ NU,
-                                                          int64_t nelem, float *__restrict__ ke,
-                                                          int32_t *__restrict__ err) {
-    constexpr int M = 3 * NN;         // 24 or 60
-    constexpr int T = (M + 31) / 32;  // output tiles per dimension
-    constexpr int KTOT = 6 * NG;      // 48 or 162
-    __shared__ float s_x[3 * NN];
-    __shared__ float s_g[NG][10];     // inverse J (row-major) and w det
-    __shared__ float s_dndx[NG][3 * NN];
+// Scaled gradients of every element: grad[(e NN + b) NGP + 3 g + i] =
+// s_g d N_b / d x_i at Gauss point g (NGP = 3 NG rounded up to a multiple of
+// 12, the tail zero), and the element's (lam, mu). One wavefront takes 64 / NG
+// elements, one lane per (element, Gauss point): J from the element-local
+// coordinates (FP64 differences to the element's first node, rounded to FP32)
+// and the reference-element derivative table, then s_g J^-1 applied to every
+// node's derivatives.
+template <int NN, int NG, int NGP>
+__global__ void __launch_bounds__(64)
+    elem_grad32_kernel(int64_t nelem, const int32_t *__restrict__ conn,
+                       const double *__restrict__ xyz, const double *__restrict__ E,
+                       const double *__restrict__ NU, float *__restrict__ grad,
+                       float2 *__restrict__ lm, int32_t *__restrict__ err) {
+    constexpr int EPW = 64 / NG;
+    __shared__ float s_dn[NG][3 * NN + 1];
+    __shared__ float s_w[NG];
+    __shared__ float s_x[EPW][3 * NN + 1];
     const int lane = threadIdx.x;
-    const int h = lane >> 5, c32 = lane & 31;
-    int ca[T], cp[T];
+    for (int t = lane; t < NG * NN * 3; t += 64)
+        s_dn[t / (3 * NN)][t % (3 * NN)] =
+            NN == 8 ? (&c_tab32.h8_dn[0][0][0])[t] : (&c_tab32.h20_dn[0][0][0])[t];
+    for (int t = lane; t < NG; t += 64) s_w[t] = NN == 8 ? 1.f : c_tab32.h20_w[t];
+    const int ce = lane / NG, g = lane - NG * (lane / NG);
+    for (int64_t e0 = (int64_t)blockIdx.x * EPW; e0 < nelem; e0 += (int64_t)gridDim.x * EPW) {
+        const int ne = (int)(nelem - e0 < EPW ? nelem - e0 : EPW);
+        for (int t = lane; t < ne * NN; t += 64) {
+            const int c = t / NN, b = t - NN * (t / NN);
+            const int64_t e = e0 + c;
+            const int64_t n0 = conn[e * NN], nd = conn[e * NN + b];
 #pragma unroll
-    for (int ti = 0; ti < T; ti++) {
-        int c = 32 * ti + c32;
-        ca[ti] = c < M ? c / 3 : 0;
-        cp[ti] = c < M ? c % 3 : 0;
-    }
-    for (int64_t e = blockIdx.x; e < nelem; e += gridDim.x) {
-        const int64_t n0 = conn[e * NN];
-        for (int t = lane; t < 3 * NN; t += 64) {
-            int a = t / 3, q = t - 3 * a;
-            s_x[t] = (float)(xyz[3 * (int64_t)conn[e * NN + a] + q] - xyz[3 * n0 + q]);
+            for (int q = 0; q < 3; q++) s_x[c][3 * b + q] = (float)(xyz[3 * nd + q] - xyz[3 * n0 + q]);
+        }
+        if (lane < ne) {
+            const double ed = E[e0 + lane], nud = NU[e0 + lane];
+            lm[e0 + lane] = make_float2((float)(ed * nud / ((1.0 + nud) * (1.0 - 2.0 * nud))),
+                                        (float)(ed / (2.0 * (1.0 + nud))));
         }
         __syncthreads();
-        if (lane < NG) {
-            const float(*dn)[3] = (NN == 8) ? c_tab32.h8_dn[lane] : c_tab32.h20_dn[lane];
+        if (ce < ne) {
             float j[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-            for (int a = 0; a < NN; a++)
+            for (int b = 0; b < NN; b++) {
+                const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
 #pragma unroll
-                for (int pp = 0; pp < 3; pp++)
-#pragma unroll
-                    for (int q = 0; q < 3; q++) j[q][pp] += dn[a][pp] * s_x[3 * a + q];
+                for (int q = 0; q < 3; q++) {
+                    const float xq = s_x[ce][3 * b + q];
+                    j[q][0] += d0 * xq;
+                    j[q][1] += d1 * xq;
+                    j[q][2] += d2 * xq;
+                }
+            }
             float det = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
                         j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
                         j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
             if (!(det > 0.f)) {
-                report32(err, AXG_E_NEG_JACOBIAN, e);
+                report32(err, AXG_E_NEG_JACOBIAN, e0 + ce);
                 det = 1.f;
             }
-            float id = 1.f / det;
-            float *g = s_g[lane];
-            g[0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * id;
-            g[1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) * id;
-            g[2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * id;
-            g[3] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) * id;
-            g[4] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * id;
-            g[5] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) * id;
-            g[6] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * id;
-            g[7] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) * id;
-            g[8] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * id;
-            g[9] = ((NN == 8) ? 1.f : c_tab32.h20_w[lane]) * det;
-        }
-        __syncthreads();
-        for (int t = lane; t < NG * NN; t += 64) {
-            const int g = t / NN, a = t - NN * (t / NN);
-            const float(*dn)[3] = (NN == 8) ? c_tab32.h8_dn[g] : c_tab32.h20_dn[g];
-            const float d0 = dn[a][0], d1 = dn[a][1], d2 = dn[a][2];
-            const float *iv = s_g[g];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                s_dndx[g][3 * a + i] = iv[i] * d0 + iv[3 + i] * d1 + iv[6 + i] * d2;
-        }
-        __syncthreads();
-        const double ed = E[e], nud = NU[e];
-        const float lam = (float)(ed * nud / ((1.0 + nud) * (1.0 - 2.0 * nud)));
-        const float mu = (float)(ed / (2.0 * (1.0 + nud)));
-        const float diag = lam + 2.f * mu;
-        floatx16 acc[T][T];
-#pragma unroll
-        for (int ti = 0; ti < T; ti++)
-#pragma unroll
-            for (int tj = 0; tj < T; tj++)
-#pragma unroll
-                for (int r = 0; r < 16; r++) acc[ti][tj][r] = 0.f;
-        int g = 0, k = h;  // stacked strain row 2 t + h = 6 g + k
-        for (int t = 0; t < KTOT / 2; t++) {
-            const float w = s_g[g][9];
-            float av[T], bv[T];
-#pragma unroll
-            for (int ti = 0; ti < T; ti++) {
-                const float *dd = &s_dndx[g][3 * ca[ti]];
-                const int cmp = bcomp(k, cp[ti]);
-                const float b = cmp < 3 ? dd[cmp] : 0.f;
-                // (D B)[k][c]: normal rows D[k][p] dN_p, shear rows mu B
-                const float db = k < 3 ? ((k == cp[ti]) ? diag : lam) * dd[cp[ti]] : mu * b;
-                const bool ok = 32 * ti + c32 < M;
-                av[ti] = ok ? b : 0.f;
-                bv[ti] = ok ? w * db : 0.f;
+            const float id = __builtin_amdgcn_sqrtf(s_w[g] * det) * __builtin_amdgcn_rcpf(det);
+            // iv[p][i] = s_g (J^-1)[p][i]
+            float iv[9];
+            iv[0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * id;
+            iv[1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) * id;
+            iv[2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * id;
+            iv[3] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) * id;
+            iv[4] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * id;
+            iv[5] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) * id;
+            iv[6] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * id;
+            iv[7] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) * id;
+            iv[8] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * id;
+            float *out = grad + (e0 + ce) * (int64_t)(NN * NGP);
+            for (int b = 0; b < NN; b++) {
+                const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
+                out[b * NGP + 3 * g] = iv[0] * d0 + iv[3] * d1 + iv[6] * d2;
+                out[b * NGP + 3 * g + 1] = iv[1] * d0 + iv[4] * d1 + iv[7] * d2;
+                out[b * NGP + 3 * g + 2] = iv[2] * d0 + iv[5] * d1 + iv[8] * d2;
             }
-#pragma unroll
-            for (int ti = 0; ti < T; ti++)
-#pragma unroll
-                for (int tj = 0; tj < T; tj++)
-                    acc[ti][tj] =
-                        __builtin_amdgcn_mfma_f32_32x32x2f32(av[ti], bv[tj], acc[ti][tj], 0, 0, 0);
-            k += 2;
-            if (k >= 6) k -= 6, g++;
+            if (NGP > 3 * NG && g < NGP - 3 * NG)  // the zero tail
+                for (int b = 0; b < NN; b++) out[b * NGP + 3 * NG + g] = 0.f;
         }
-        // output layout of 32x32x2: register r of lane l holds row
-        // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
-        float *out = ke + e * (int64_t)(M * M);
-#pragma unroll
-        for (int ti = 0; ti < T; ti++)
-#pragma unroll
-            for (int tj = 0; tj < T; tj++)
-#pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int row = 32 * ti + 8 * (r >> 2) + 4 * h + (r & 3);
-                    const int col = 32 * tj + c32;
-                    if (row < M && col < M) out[row * M + col] = acc[ti][tj][r];
-                }
         __syncthreads();
     }
 }
 
-void launch_ke_fp32(axg_mesh *m, hipStream_t s, int32_t *d_err) {
-    if (m->nelem == 0) return;
-    int64_t blocks = std::min<int64_t>(m->nelem, (int64_t)m->ctx->cu_count * 64);
-    if (m->nn == 8)
-        hipLaunchKernelGGL((ke_fp32_mfma_kernel<8, 8>), dim3((unsigned)blocks), dim3(64), 0, s,
-                           m->xyz.p, m->conn.p, m->e.p, m->nu.p, m->nelem, m->ke32.p, d_err);
-    else
-        hipLaunchKernelGGL((ke_fp32_mfma_kernel<20, 27>), dim3((unsigned)blocks), dim3(64), 0, s,
-                           m->xyz.p, m->conn.p, m->e.p, m->nu.p, m->nelem, m->ke32.p, d_err);
+// Per-row LDS of the block-row assembly: CH elements per chunk, SPR slots.
+template <int NN, int CH, int SPR, int NQA>
+struct RowLds {
+    float4 c[CH][NN + 1][3];  // K_ab of the chunk's elements (9 values, 3 unused); b = NN: zero
+    float4 ga[CH][NQA];       // gradient row of the block row's own node in chunk element ce
+    uint8_t src[SPR][CH];     // local node b of chunk element ce at slot k, NN for none
+};
+
+// Block rows: RPW rows per wavefront (two when every row has at most 32
+// blocks), KPL x 64 slot lanes; slot lane (h, k) owns slot k of row h. Per
+// chunk of CH incident elements, task (h, ce, b) forms K_ab of row h's
+// element ce with its local node b; then every slot lane adds the chunk's
+// contributions to its block in ascending element order.
+template <int NN, int NGP, int CH, int KPL, int RPW, int WPB, int MINW>
+__global__ void __launch_bounds__(64 * WPB, MINW)
+    row_asm32_kernel(int64_t nb, int64_t per_xcd, int incw, const int32_t *__restrict__ rinc,
+                     const float *__restrict__ grad,
+                     const float2 *__restrict__ lm, const uint8_t *__restrict__ eslot,
+                     const uint8_t *__restrict__ smask, const int64_t *__restrict__ sptr,
+                     const int32_t *__restrict__ width,
+                     const int32_t *__restrict__ bcol, const int32_t *__restrict__ bloc,
+                     int64_t nsl, float *__restrict__ bval, float *__restrict__ lval) {
+    constexpr int SPR = 64 * KPL / RPW;
+    constexpr int TPR = CH * NN;  // tasks per row and chunk
+    constexpr int NIT = (RPW * TPR + 63) / 64;
+    // small elements: every pass's gradient rows are held in registers at once and
+    // the row node's rows reach the other lanes through LDS
+    constexpr bool STAGED = NIT * NGP <= 48;
+    constexpr int NQ = NGP / 4;
+    using Lds = RowLds<NN, CH, SPR, STAGED ? NQ : 1>;
+    static_assert(CH % 4 == 0, "src rows are read as words");
+    __shared__ Lds s_row[WPB][RPW];
+    const int64_t grp = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t I0 = (grp * WPB + w) * RPW;
+    if (grp >= per_xcd * 8 || I0 >= nb) return;
+    // rows of the wavefront: slot count and first slot (wave-uniform loads)
+    int rw[RPW];
+    int64_t rbase[RPW];
+#pragma unroll
+    for (int h = 0; h < RPW; h++) {
+        const int64_t I = I0 + h < nb ? I0 + h : I0;
+        rw[h] = I0 + h < nb ? width[I >> 6] : 0;
+        rbase[h] = sptr[I >> 6] + (I & 63);
+    }
+    // slot lanes: row, slot, position, column, DOF mask (the column, mask and
+    // LDS index loads are first used at the end)
+    int sh[KPL], sk[KPL];
+    bool sv[KPL];
+    int64_t spos[KPL];
+    int32_t ccol[KPL], cloc[KPL];
+    unsigned mask[KPL];
+    float acc[KPL][9];
+#pragma unroll
+    for (int j = 0; j < KPL; j++) {
+        const int idx = lane + 64 * j;
+        const int h = idx / SPR, k = idx - SPR * (idx / SPR);
+        int W = rw[0];
+        int64_t base = rbase[0];
+#pragma unroll
+        for (int hh = 1; hh < RPW; hh++)
+            if (h == hh) W = rw[hh], base = rbase[hh];
+        sh[j] = h, sk[j] = k, mask[j] = 0u, ccol[j] = cloc[j] = 0;
+        sv[j] = h < RPW && k < W;
+        spos[j] = base + 64 * (int64_t)k;
+#pragma unroll
+        for (int r = 0; r < 9; r++) acc[j][r] = 0.f;
+        if (sv[j]) {
+            ccol[j] = bcol[spos[j]];
+            cloc[j] = bloc[spos[j]];
+            mask[j] = smask[spos[j]];
+        }
+    }
+    const int nch = (incw + CH - 1) / CH;
+    for (int t = lane; t < RPW * CH * 3; t += 64)  // the zero blocks (never written again)
+        s_row[w][t / (CH * 3)].c[(t / 3) % CH][NN][t % 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = 0; c < nch; c++) {
+        // the tasks' incident pairs first, then their gradient rows
+        int32_t sl[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int t = lane + 64 * it;
+            const int h = t / TPR, ce = (t - TPR * (t / TPR)) / NN;
+            sl[it] = -1;
+            if (t < RPW * TPR && I0 + h < nb && c * CH + ce < incw)
+                sl[it] = rinc[(I0 + h) * incw + c * CH + ce];
+        }
+        for (int t = lane; t < RPW * CH * SPR / 4; t += 64)
+            reinterpret_cast<uint32_t *>(&s_row[w][t / (CH * SPR / 4)].src[0][0])[t % (CH * SPR / 4)] =
+                0x01010101u * NN;
+        if constexpr (STAGED) {
+            float4 gb[NIT][NQ];
+            unsigned kk[NIT];
+            float2 lmv[NIT];
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int t = lane + 64 * it;
+                const int b = (t - TPR * (t / TPR)) % NN;
+                if (sl[it] < 0) continue;
+                const int64_t e = sl[it] / NN;
+                const float4 *pb = reinterpret_cast<const float4 *>(grad + (e * NN + b) * NGP);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) gb[it][q] = pb[q];
+                kk[it] = eslot[(int64_t)sl[it] * NN + b];
+                lmv[it] = lm[e];
+            }
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int t = lane + 64 * it;
+                const int h = t / TPR, u = t - TPR * (t / TPR);
+                const int ce = u / NN, b = u - NN * (u / NN);
+                if (sl[it] >= 0 && b == sl[it] % NN) {
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) s_row[w][h].ga[ce][q] = gb[it][q];
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int t = lane + 64 * it;
+                const int h = t / TPR, u = t - TPR * (t / TPR);
+                const int ce = u / NN, b = u - NN * (u / NN);
+                if (sl[it] < 0) continue;
+                Lds &R = s_row[w][h];
+                float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+                for (int q = 0; q < NQ / 3; q++) {  // four Gauss points per step
+                    const float4 a0 = R.ga[ce][3 * q], a1 = R.ga[ce][3 * q + 1], a2 = R.ga[ce][3 * q + 2];
+                    const float4 b0 = gb[it][3 * q], b1 = gb[it][3 * q + 1], b2 = gb[it][3 * q + 2];
+                    const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
+                                          a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+                    const float bv[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y,
+                                          b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+#pragma unroll
+                    for (int gg = 0; gg < 4; gg++)
+#pragma unroll
+                        for (int r = 0; r < 3; r++)
+#pragma unroll
+                            for (int cc = 0; cc < 3; cc++) p[r][cc] += av[3 * gg + r] * bv[3 * gg + cc];
+                }
+                const float lam = lmv[it].x, mu = lmv[it].y;
+                const float tr = mu * (p[0][0] + p[1][1] + p[2][2]);
+                float kv[9];
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+#pragma unroll
+                    for (int cc = 0; cc < 3; cc++)
+                        kv[3 * r + cc] = lam * p[r][cc] + mu * p[cc][r] + (r == cc ? tr : 0.f);
+                R.c[ce][b][0] = make_float4(kv[0], kv[1], kv[2], kv[3]);
+                R.c[ce][b][1] = make_float4(kv[4], kv[5], kv[6], kv[7]);
+                R.c[ce][b][2] = make_float4(kv[8], 0.f, 0.f, 0.f);
+                if (kk[it] < (unsigned)SPR) R.src[kk[it]][ce] = (uint8_t)b;
+            }
+        } else {
+        wave_sync();
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int t = lane + 64 * it;
+            const int h = t / TPR, u = t - TPR * (t / TPR);
+            const int ce = u / NN, b = u - NN * (u / NN);
+            if (sl[it] < 0) continue;
+            const int64_t e = sl[it] / NN;
+            const float4 *pa = reinterpret_cast<const float4 *>(grad + (int64_t)sl[it] * NGP);
+            const float4 *pb = reinterpret_cast<const float4 *>(grad + (e * NN + b) * NGP);
+            const unsigned k = eslot[(int64_t)sl[it] * NN + b];
+            const float2 lmv = lm[e];
+            float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+            for (int q = 0; q < NGP / 12; q++) {  // four Gauss points per step
+                const float4 a0 = pa[3 * q], a1 = pa[3 * q + 1], a2 = pa[3 * q + 2];
+                const float4 b0 = pb[3 * q], b1 = pb[3 * q + 1], b2 = pb[3 * q + 2];
+                const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
+                                      a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+                const float bv[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y,
+                                      b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+#pragma unroll
+                for (int gg = 0; gg < 4; gg++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+#pragma unroll
+                        for (int cc = 0; cc < 3; cc++) p[r][cc] += av[3 * gg + r] * bv[3 * gg + cc];
+            }
+            const float lam = lmv.x, mu = lmv.y;
+            const float tr = mu * (p[0][0] + p[1][1] + p[2][2]);
+            float kv[9];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++)
+                    kv[3 * r + cc] = lam * p[r][cc] + mu * p[cc][r] + (r == cc ? tr : 0.f);
+            Lds &R = s_row[w][h];
+            R.c[ce][b][0] = make_float4(kv[0], kv[1], kv[2], kv[3]);
+            R.c[ce][b][1] = make_float4(kv[4], kv[5], kv[6], kv[7]);
+            R.c[ce][b][2] = make_float4(kv[8], 0.f, 0.f, 0.f);
+            if (k < (unsigned)SPR) R.src[k][ce] = (uint8_t)b;
+        }
+        }
+        wave_sync();
+        // per slot, the chunk's contributions in ascending element order
+#pragma unroll
+        for (int jj = 0; jj < KPL; jj++) {
+            if (!sv[jj]) continue;
+            const Lds &R = s_row[w][sh[jj]];
+            uint32_t srcw[CH / 4];
+#pragma unroll
+            for (int q = 0; q < CH / 4; q++)
+                srcw[q] = reinterpret_cast<const uint32_t *>(&R.src[sk[jj]][0])[q];
+#pragma unroll
+            for (int ce = 0; ce < CH; ce++) {
+                const float4 *cv = R.c[ce][(srcw[ce / 4] >> (8 * (ce % 4))) & 0xffu];
+                const float4 v0 = cv[0], v1 = cv[1], v2 = cv[2];
+                const float vv[9] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x};
+#pragma unroll
+                for (int r = 0; r < 9; r++) acc[jj][r] += vv[r];
+            }
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int jj = 0; jj < KPL; jj++) {
+        if (!sv[jj]) continue;
+        float v[9];
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+            v[r] = ((mask[jj] >> (r / 3)) & (mask[jj] >> (4 + r % 3)) & 1u) ? acc[jj][r] : 0.f;
+        const int64_t pos = spos[jj];
+        *reinterpret_cast<float4 *>(bval + 4 * pos) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(bval + 4 * nsl + 4 * pos) = make_float4(v[4], v[5], v[6], v[7]);
+        *reinterpret_cast<float2 *>(bval + 8 * nsl + 2 * pos) = make_float2(v[8], __int_as_float(ccol[jj]));
+        *reinterpret_cast<float2 *>(lval + 2 * pos) = make_float2(v[8], __int_as_float(cloc[jj]));
+    }
+}
+
+template <int NN, int NG, int NGP>
+static void launch_grad(axg_mesh *m, hipStream_t s, int32_t *d_err) {
+    constexpr int EPW = 64 / NG;
+    const int64_t blocks = std::min<int64_t>((m->nelem + EPW - 1) / EPW, (int64_t)1 << 24);
+    hipLaunchKernelGGL((elem_grad32_kernel<NN, NG, NGP>), dim3((unsigned)blocks), dim3(64), 0, s,
+                       m->nelem, m->conn.p, m->xyz.p, m->e.p, m->nu.p, m->grad32.p,
+                       reinterpret_cast<float2 *>(m->lm32.p), d_err);
     AXG_HIP(hipGetLastError());
 }
 
-// FP32 sums of the scalar slots' contributions (same scatter map, ascending
-// element order, FP32 accumulation) written to their BSR-3 positions.
-__global__ void gather_bsr32_kernel(const uint32_t *__restrict__ slotptr,
+template <int NN, int NGP, int CH, int KPL, int RPW, int WPB, int MINW = 1>
+static void launch_rows(axg_mesh *m, hipStream_t s) {
+    const int64_t ngrp = (m->nb + WPB * RPW - 1) / (WPB * RPW);
+    const int64_t per_xcd = (ngrp + 7) / 8;
+    AXG_REQUIRE(8 * per_xcd < INT32_MAX, AXG_E_INVALID, "FP32 assembly: too many block rows");
+    hipLaunchKernelGGL((row_asm32_kernel<NN, NGP, CH, KPL, RPW, WPB, MINW>), dim3((unsigned)(8 * per_xcd)),
+                       dim3(64 * WPB), 0, s, m->nb, per_xcd, m->b_incw, m->b_inc.p,
+                       m->grad32.p, reinterpret_cast<const float2 *>(m->lm32.p), m->eslot.p,
+                       m->bs_mask.p, m->bs_sptr.p, m->bs_width.p, m->bs_col.p,
+                       m->bs_loc.p, m->bs_nslots, m->bval.p, m->bval_loc.p);
+    AXG_HIP(hipGetLastError());
+}
+
+constexpr int ngp_of(int ng) { return (3 * ng + 11) / 12 * 12; }
+
+void launch_row_asm32(axg_mesh *m, hipStream_t s, int32_t *d_err) {
+    if (m->nelem == 0 || m->nb == 0) return;
+    const int wmax = m->bs_maxw;
+    AXG_REQUIRE(wmax <= 255, AXG_E_INVALID,
+                "FP32 assembly: a block row couples more than 255 nodes (" + std::to_string(wmax) +
+                    ")");
+    const int nn = m->nn, ngp = nn == 8 ? ngp_of(8) : ngp_of(27);
+    m->grad32.ensure((size_t)m->nelem * nn * ngp);
+    m->lm32.ensure(2 * (size_t)m->nelem);
+    if (nn == 8) {
+        launch_grad<8, 8, ngp_of(8)>(m, s, d_err);
+        if (wmax <= 32) launch_rows<8, ngp_of(8), 8, 1, 2, 4, 4>(m, s);
+        else if (wmax <= 64) launch_rows<8, ngp_of(8), 8, 1, 1, 4>(m, s);
+        else launch_rows<8, ngp_of(8), 8, 4, 1, 4>(m, s);
+    } else {
+        launch_grad<20, 27, ngp_of(27)>(m, s, d_err);
+        if (wmax <= 128) launch_rows<20, ngp_of(27), 8, 2, 1, 4>(m, s);
+        else launch_rows<20, ngp_of(27), 8, 4, 1, 4>(m, s);
+    }
+}
+
+// FP32 values of an operator taken from a caller's CSR: each topological
+// slot's FP64 value (its contributions summed in order) rounded once and
+// stored at its BSR-3 plane position.
+__global__ void csr_to_bsr32_kernel(const uint32_t *__restrict__ slotptr,
                                     const uint32_t *__restrict__ contrib,
-                                    const float *__restrict__ ke32,
+                                    const double *__restrict__ ke,
                                     const int32_t *__restrict__ entry_of_slot, int64_t nslot,
                                     float *__restrict__ bval) {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslot;
          s += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t c0 = slotptr[s], c1 = slotptr[s + 1];
-        float v = 0.f;
-        for (uint32_t c = c0; c < c1; c += 8) {
-            uint32_t idx[8];
-            float kv[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) idx[u] = c + u < c1 ? contrib[c + u] : 0u;
-#pragma unroll
-            for (int u = 0; u < 8; u++) kv[u] = c + u < c1 ? ke32[idx[u]] : 0.f;
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (c + u < c1) v += kv[u];
-        }
-        bval[entry_of_slot[s]] = v;
+        double v = 0.0;
+        for (uint32_t c = slotptr[s]; c < slotptr[s + 1]; c++) v += ke[contrib[c]];
+        bval[entry_of_slot[s]] = (float)v;
     }
 }
 
-void launch_gather_bsr32(axg_mesh *m, hipStream_t s, const int32_t *entry_of_slot) {
-    int64_t nslot = m->nnz_topo;
+void launch_csr_to_bsr32(axg_mesh *m, hipStream_t s) {
+    const int64_t nslot = m->nnz_topo;
     if (nslot == 0) return;
-    int64_t blocks = std::min<int64_t>((nslot + 255) / 256, 1 << 20);
-    hipLaunchKernelGGL(gather_bsr32_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
-                       m->slotptr_t.p, m->contrib_t.p, m->ke32.p, entry_of_slot, nslot, m->bval.p);
+    const int64_t blocks = std::min<int64_t>((nslot + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(csr_to_bsr32_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       m->slotptr_t.p, m->contrib_t.p, m->ke.p, m->entry_of_slot.p, nslot,
+                       m->bval.p);
     AXG_HIP(hipGetLastError());
 }
 

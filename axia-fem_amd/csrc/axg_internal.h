@@ -251,14 +251,16 @@ struct axg_mesh {
     bool assembled32 = false;
     int64_t nb = 0;                 // block rows (nodes with a free DOF)
     int64_t nnzb = 0;               // 3x3 blocks
-    axg::DevBuf<int32_t> b_rowptr, b_colind;  // block pattern
     axg::DevBuf<int32_t> b_node;              // node of each block row
     axg::DevBuf<int32_t> free_of_blk;         // 3*nb: free index or -1
     axg::DevBuf<int32_t> blk_of_free;         // n_free: 3*blockrow + d
-    axg::DevBuf<uint32_t> b_slotptr;          // per block entry (9*nnzb + 1)
-    axg::DevBuf<uint32_t> b_contrib;          // FP32 Ke offsets
-    axg::DevBuf<float> ke32;                  // FP32 element matrices
-    axg::DevBuf<float> bval;                  // 9*nnzb block values (row-major 3x3)
+    axg::DevBuf<float> bval;                  // block planes, 10 floats per slot
+    axg::DevBuf<float> grad32;                // FP32 scaled shape gradients per element
+    axg::DevBuf<float> lm32;                  // FP32 (lam, mu) per element
+    axg::DevBuf<uint8_t> eslot;               // (e, a, b) -> slot of b's node in a's block row
+    axg::DevBuf<int32_t> b_inc;               // b_incw per block row: e * nn + a ascending, -1
+    int32_t b_incw = 0;                       // most elements at one block row's node
+    axg::DevBuf<uint8_t> bs_mask;             // per slot: free DOFs of row (bits 0-2), column (4-6)
     axg::DevBuf<float> binv32;                // 9*nb block-Jacobi inverse (FP32 copy)
     axg::DevBuf<float> dinv32;                // 3*nb scalar Jacobi inverse
     axg::DevBuf<double> binv64;               // 9*nb FP64 block inverse over free DOFs
@@ -276,6 +278,7 @@ struct axg_mesh {
     int64_t bs_nslots = 0;
     axg::DevBuf<int64_t> bs_sptr;
     axg::DevBuf<int32_t> bs_width, bs_col;
+    int32_t bs_maxw = 0;  // widest slice (most blocks in a block row)
     int spmm_shape[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};  // tuned BSR SpMM shape per k
     // groups of 256 block rows for the LDS-staged FP32 SpMM: the sorted column
     // blocks of each group (at most a fixed count; the rest are read from
@@ -307,8 +310,8 @@ void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
 // kernels (launch wrappers), axg_element.hip
 void launch_ke_fp64(axg_mesh *m, hipStream_t s, int32_t *d_err);
 void launch_gather_fp64(axg_mesh *m, hipStream_t s, bool topo, uint32_t *d_zero_count);
-void launch_ke_fp32(axg_mesh *m, hipStream_t s, int32_t *d_err);
-void launch_gather_bsr32(axg_mesh *m, hipStream_t s, const int32_t *entry_of_slot);
+void launch_row_asm32(axg_mesh *m, hipStream_t s, int32_t *d_err);
+void launch_csr_to_bsr32(axg_mesh *m, hipStream_t s);
 void upload_shape_tables32(const double *h8_dn, const double *h20_dn, const double *h20_w);
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,
                          double *d_stress, double *d_vm, int32_t *d_err);

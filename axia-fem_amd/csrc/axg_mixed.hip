@@ -212,6 +212,7 @@ static void build_bsr(axg_mesh *m) {
     SellHost L;
     sell_layout(blen, L);
     const int64_t nsl = L.nslots;
+    m->bs_maxw = L.width.empty() ? 0 : *std::max_element(L.width.begin(), L.width.end());
     AXG_REQUIRE(10 * nsl < INT32_MAX, AXG_E_INVALID, "FP32 operator exceeds 2^31 values");
     std::vector<int32_t> bcol(nsl, 0);
     parallel_chunks(nb, nt, [&](int, int64_t lo, int64_t hi) {
@@ -222,6 +223,75 @@ static void build_bsr(axg_mesh *m) {
             for (int64_t k = blen[I]; k < L.width[I >> 6]; k++) bcol[base + 64 * k] = (int32_t)I;
         }
     });
+    // for the fused FP32 assembly: the slot of local node b's block in the block
+    // row of local node a, for every (element, a, b) (255 where a node keeps no
+    // free DOF)
+    if (!m->csr_operator && m->nelem > 0) {
+        const int nn = m->nn;
+        const int64_t ne = m->nelem;
+        std::vector<int32_t> conn((size_t)ne * nn), bnode(nb), bon(m->nnode, -1);
+        AXG_HIP(hipMemcpyAsync(conn.data(), m->conn.p, 4 * conn.size(), hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipMemcpyAsync(bnode.data(), m->b_node.p, 4 * nb, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipStreamSynchronize(s));
+        for (int64_t I = 0; I < nb; I++) bon[bnode[I]] = (int32_t)I;
+        std::vector<uint8_t> es((size_t)ne * nn * nn);
+        std::vector<int> miss(host_threads(ne, 4096), 0);
+        parallel_chunks(ne, (int)miss.size(), [&](int t, int64_t lo, int64_t hi) {
+            for (int64_t e = lo; e < hi; e++)
+                for (int a = 0; a < nn; a++) {
+                    const int32_t I = bon[conn[e * nn + a]];
+                    for (int b = 0; b < nn; b++) {
+                        const int32_t J = bon[conn[e * nn + b]];
+                        uint8_t v = 255;
+                        if (I >= 0 && J >= 0) {
+                            const auto b0 = bci.begin() + brp[I], b1 = bci.begin() + brp[I + 1];
+                            if (!std::binary_search(b0, b1, J)) miss[t] = 1;
+                            else v = (uint8_t)std::min<int64_t>(pos_in_row(I, J), 255);
+                        }
+                        es[(e * nn + a) * nn + b] = v;
+                    }
+                }
+        });
+        for (int f : miss)
+            AXG_REQUIRE(!f, AXG_E_INVALID, "element coupling missing from the block pattern");
+        // the incident (element, local node) pairs e * nn + a of every block row,
+        // ascending element, in rows of the largest count (-1 after a row's last)
+        AXG_REQUIRE(ne * nn < INT32_MAX, AXG_E_INVALID, "FP32 assembly: too many elements");
+        std::vector<int32_t> cnt(nb, 0);
+        for (int64_t q = 0; q < ne * nn; q++)
+            if (bon[conn[q]] >= 0) cnt[bon[conn[q]]]++;
+        const int32_t mi = nb ? *std::max_element(cnt.begin(), cnt.end()) : 0;
+        std::vector<int32_t> rinc(std::max<int64_t>(nb * mi, 1), -1);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int64_t q = 0; q < ne * nn; q++) {
+            const int32_t I = bon[conn[q]];
+            if (I >= 0) rinc[(int64_t)I * mi + cnt[I]++] = (int32_t)q;
+        }
+        m->b_incw = mi;
+        // per slot: bits 0-2 the row node's free DOFs, bits 4-6 the column node's
+        // (0 for a padding slot)
+        std::vector<uint8_t> smask(std::max<int64_t>(nsl, 1), 0);
+        parallel_chunks(nb, host_threads(nb, 4096), [&](int, int64_t lo, int64_t hi) {
+            for (int64_t I = lo; I < hi; I++) {
+                const int64_t base = L.sptr[I >> 6] + (I & 63);
+                unsigned rm = 0;
+                for (int d = 0; d < 3; d++) rm |= fob[3 * I + d] >= 0 ? 1u << d : 0u;
+                for (int32_t k = 0; k < (int32_t)blen[I]; k++) {
+                    const int32_t J = bcol[base + 64 * k];
+                    unsigned cm = 0;
+                    for (int d = 0; d < 3; d++) cm |= fob[3 * (int64_t)J + d] >= 0 ? 16u << d : 0u;
+                    smask[base + 64 * k] = (uint8_t)(rm | cm);
+                }
+            }
+        });
+        m->bs_mask.alloc(smask.size());
+        AXG_HIP(hipMemcpyAsync(m->bs_mask.p, smask.data(), smask.size(), hipMemcpyHostToDevice, s));
+        m->eslot.alloc(es.size());
+        m->b_inc.alloc(rinc.size());
+        AXG_HIP(hipMemcpyAsync(m->eslot.p, es.data(), es.size(), hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(m->b_inc.p, rinc.data(), 4 * rinc.size(), hipMemcpyHostToDevice, s));
+        AXG_HIP(hipStreamSynchronize(s));
+    }
     // plane position of every scalar slot: the block columns of a row are met in
     // ascending order when the free numbering follows the nodes, so a cursor
     // walks the row's block list (a search from the start when a column goes back)
@@ -374,53 +444,46 @@ void ensure_bsell64(axg_mesh *m) {
     m->bsell64_fresh = true;
 }
 
-// FP32 copy of an operator's stored values (the contributions of a CSR operator).
-__global__ void to_fp32_kernel(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = (float)a[i];
-}
-
+// The FP32 operator: for a mesh, the fused integration + block-row assembly
+// (every plane written by the kernel); for an operator taken from a caller's
+// CSR, its FP64 values rounded into zeroed planes, then the column and LDS
+// planes. t_ke32 is the device time of that; t_gather32 stays 0 (there is no
+// separate gather).
 void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     if (m->nnzb == 0) build_bsr(m);
     ensure_block_inverse64(m);
-    m->ke32.ensure(m->csr_operator ? (size_t)std::max<int64_t>(m->ncontrib, 1)
-                                   : (size_t)m->nelem * m->m * m->m);
     m->bval.ensure(std::max<int64_t>(10 * m->bs_nslots, 1));
+    m->bval_loc.ensure(std::max<int64_t>(2 * m->bs_nslots, 2));
     Workspace &w = ctx->ws;
     AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
-    AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 10 * m->bs_nslots, s));
-    if (m->bs_nslots)
-        hipLaunchKernelGGL(col_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)), dim3(256),
-                           0, s, m->bs_nslots, m->bs_col.p, m->bval.p + 8 * m->bs_nslots);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     if (m->csr_operator) {
-        if (m->ncontrib)
-            hipLaunchKernelGGL(to_fp32_kernel, dim3(1024), dim3(256), 0, s, m->ncontrib, m->ke.p,
-                               m->ke32.p);
+        AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 10 * m->bs_nslots, s));
+        if (m->bs_nslots)
+            hipLaunchKernelGGL(col_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)),
+                               dim3(256), 0, s, m->bs_nslots, m->bs_col.p,
+                               m->bval.p + 8 * m->bs_nslots);
+        launch_csr_to_bsr32(m, s);
+        if (m->bs_nslots)
+            hipLaunchKernelGGL(loc_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)),
+                               dim3(256), 0, s, m->bs_nslots, m->bval.p + 8 * m->bs_nslots,
+                               m->bs_loc.p, m->bval_loc.p);
+        AXG_HIP(hipGetLastError());
     } else {
-        launch_ke_fp32(m, s, w.err.p);
+        launch_row_asm32(m, s, w.err.p);
     }
-    AXG_HIP(hipEventRecord(ctx->evA, s));
-    launch_gather_bsr32(m, s, m->entry_of_slot.p);
-    m->bval_loc.ensure(std::max<int64_t>(2 * m->bs_nslots, 2));
-    if (m->bs_nslots)
-        hipLaunchKernelGGL(loc_plane_kernel, dim3((unsigned)((m->bs_nslots + 255) / 256)), dim3(256),
-                           0, s, m->bs_nslots, m->bval.p + 8 * m->bs_nslots, m->bs_loc.p,
-                           m->bval_loc.p);
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     int32_t err[2];
     AXG_HIP(hipMemcpyAsync(err, w.err.p, 8, hipMemcpyDeviceToHost, s));
     AXG_HIP(hipStreamSynchronize(s));
     if (err[0]) throw Error{err[0], "FP32 integration: negative Jakobideterminante (element index " +
                                         std::to_string(err[1]) + ")"};
-    float ms = 0.f, ms_a = 0.f;
+    float ms = 0.f;
     AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    AXG_HIP(hipEventElapsedTime(&ms_a, ctx->ev0, ctx->evA));
-    m->t_ke32 = ms_a;
-    m->t_gather32 = ms - ms_a;
+    m->t_ke32 = ms;
+    m->t_gather32 = 0.0;
     if (elapsed_ms) *elapsed_ms = ms;
     m->assembled32 = true;
 }

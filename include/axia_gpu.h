@@ -156,11 +156,12 @@ int axg_mesh_set_modulus(axg_mesh *mesh, const double *e);
 /* Numeric assembly on the device: Ke of every element and the gather into the
  * CSR slots, summed in ascending element order (csr_from_triplets order).
  * precision AXG_PREC_FP64 builds the FP64 CSR; AXG_PREC_MIXED also builds the
- * FP32 BSR-3 inner operator from FP32 (MFMA) integration. elapsed_ms (may be
+ * FP32 BSR-3 inner operator, integrated in FP32 and assembled block row by
+ * block row in one fused kernel. elapsed_ms (may be
  * NULL) receives the device time of the numeric assembly. */
 int axg_assemble(axg_mesh *mesh, int32_t precision, double *elapsed_ms);
 /* Device times (ms) of the last assembly: FP64 integration, FP64 gather,
- * FP32 (MFMA) integration, FP32 BSR-3 gather. */
+ * FP32 fused integration + BSR-3 assembly, and 0 (no separate FP32 gather). */
 int axg_assembly_times(axg_mesh *mesh, double *ke64_ms, double *gather64_ms, double *ke32_ms,
                        double *gather32_ms);
 /* Copy the FP64 K_ff CSR to the host (indptr n+1, indices nnz, data nnz). */
@@ -171,9 +172,13 @@ int axg_scatter_map_export(axg_mesh *mesh, int64_t *slot_ptr, int64_t *contrib);
 /* Copy the FP64 element matrices of elements [first, first+count) (row-major). */
 int axg_ke_export(axg_mesh *mesh, int64_t first, int64_t count, double *ke);
 
-/* Copy the FP32 element matrices of the mixed operator (after
- * axg_assemble(..., AXG_PREC_MIXED)) of elements [first, first+count). */
-int axg_ke32_export(axg_mesh *mesh, int64_t first, int64_t count, float *ke);
+/* The FP32 operator of the mixed solve (after axg_assemble(..., AXG_PREC_MIXED))
+ * as triplets over free DOFs: every entry of every stored 3x3 block whose row
+ * and column DOF are free. With rows, cols and vals NULL, *count receives the
+ * number of triplets; otherwise *count gives their capacity on entry and the
+ * number written on return. No reference counterpart (a test and inspection
+ * export of the inner operator). */
+int axg_op32_export(axg_mesh *mesh, int64_t *count, int32_t *rows, int32_t *cols, float *vals);
 
 /* Solve K_ff x = rhs for nrhs right-hand sides (column k at rhs + k*n_free). */
 int axg_mesh_solve(axg_mesh *mesh, const double *rhs, int32_t nrhs, const axg_solve_opts *opts,

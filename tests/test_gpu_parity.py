@@ -290,26 +290,38 @@ def test_brick_properties_40(ctx):
 
 
 # ------------------------------------------------------------ mixed precision
-@pytest.mark.parametrize("nn,seed,offset", [(8, 4, 0.0), (20, 5, 0.0), (8, 6, 1e5), (20, 7, 1e5)])
-def test_mixed_ke32_mfma_vs_fp64(ctx, nn, seed, offset):
-    """FP32 MFMA integration agrees with the FP64 element matrices to FP32 accuracy,
-    also far from the origin (coordinates are taken relative to each element's first node)."""
-    say("case: Ke32", nn, seed, offset)
+@pytest.mark.parametrize("nn,seed,offset,dup", [(8, 4, 0.0, 1), (20, 5, 0.0, 1), (8, 6, 1e5, 1),
+                                                (20, 7, 1e5, 1), (8, 8, 0.0, 3)])
+def test_mixed_op32_vs_fp64(ctx, nn, seed, offset, dup):
+    """The FP32 operator (fused FP32 integration + block-row assembly) agrees with the
+    FP64 K_ff entry by entry to FP32 accuracy (relative to its row's largest entry), also
+    far from the origin (coordinates are taken relative to each element's first node) and
+    with every element listed three times (24 elements at an interior node: several
+    element chunks per block row); the constrained DOFs are absent from it."""
+    say("case: op32", nn, seed, offset, dup)
     if nn == 8:
         xyz, conn = MG.brick_hex8(3, 3, 2)
     else:
         xyz, conn = MG.brick_hex20(2, 2, 2)
     rng = np.random.default_rng(seed)
     xyz = xyz + rng.uniform(-0.1, 0.1, xyz.shape) + offset
+    conn = np.concatenate([conn] * dup)
     ne = len(conn)
     e = rng.uniform(1e3, 3e5, ne)
     mesh = G.Mesh(nn, xyz, conn, e, 0.3, MG.free_map(len(xyz), [0, 1]), ctx=ctx)
     mesh.assemble("mixed")
-    k64 = mesh.ke()
-    k32 = mesh.ke32().astype(np.float64)
-    for el in range(ne):
-        scale = np.abs(k64[el]).max()
-        assert np.abs(k32[el] - k64[el]).max() < 2e-6 * scale, f"element {el}"
+    import scipy.sparse as sp
+    indptr, indices, data = mesh.csr()
+    n = len(indptr) - 1
+    k64 = sp.csr_matrix((data, indices, indptr), shape=(n, n)).toarray()
+    k32 = mesh.op32().toarray().astype(np.float64)
+    rowmax = np.abs(k64).max(axis=1, keepdims=True)
+    err = np.abs(k32 - k64) / rowmax
+    say(f"  n={n}, worst entry error / row max {err.max():.2e}")
+    assert err.max() < 3e-6
+    # every coupling of K_ff is stored; nothing outside the block pattern
+    assert np.all((k32 != 0) | (k64 == 0))
+    assert np.abs(k32 - k32.T).max() < 3e-6 * rowmax.max()
 
 
 @pytest.mark.parametrize("name,precond", [("le10_c3d20", "block3"), ("le10_c3d20_fine", "jacobi"),
