@@ -61,91 +61,101 @@ __device__ inline void wave_sync() {
 
 // Scaled gradients of every element: grad[(e NN + b) NGP + 3 g + i] =
 // s_g d N_b / d x_i at Gauss point g (NGP = 3 NG rounded up to a multiple of
-// 12, the tail zero), and the element's (lam, mu). One wavefront takes 64 / NG
-// elements, one lane per (element, Gauss point): J from the element-local
-// coordinates (FP64 differences to the element's first node, rounded to FP32)
-// and the reference-element derivative table, then s_g J^-1 applied to every
-// node's derivatives.
-template <int NN, int NG, int NGP>
-__global__ void __launch_bounds__(64)
+// 12, the tail zero), and the element's (lam, mu). Each wavefront of a
+// workgroup takes its own 64 / NG consecutive elements, one lane per (element,
+// Gauss point): J from the element-local coordinates (FP64 differences to the
+// element's first node, rounded to FP32) and the reference-element derivative
+// table, then s_g J^-1 applied to every node's derivatives. The wavefront's
+// output (contiguous in memory) is staged in LDS and written with 16-byte
+// stores.
+template <int NN, int NG, int NGP, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
     elem_grad32_kernel(int64_t nelem, const int32_t *__restrict__ conn,
                        const double *__restrict__ xyz, const double *__restrict__ E,
                        const double *__restrict__ NU, float *__restrict__ grad,
                        float2 *__restrict__ lm, int32_t *__restrict__ err) {
     constexpr int EPW = 64 / NG;
+    constexpr int OUT = EPW * NN * NGP;  // floats of one wavefront's elements
+    static_assert(OUT % 4 == 0, "16-byte stores");
     __shared__ float s_dn[NG][3 * NN + 1];
     __shared__ float s_w[NG];
-    __shared__ float s_x[EPW][3 * NN + 1];
-    const int lane = threadIdx.x;
-    for (int t = lane; t < NG * NN * 3; t += 64)
+    __shared__ float s_x[WPB][EPW][3 * NN + 1];
+    __shared__ float4 s_out[WPB][OUT / 4];
+    for (int t = threadIdx.x; t < NG * NN * 3; t += blockDim.x)
         s_dn[t / (3 * NN)][t % (3 * NN)] =
             NN == 8 ? (&c_tab32.h8_dn[0][0][0])[t] : (&c_tab32.h20_dn[0][0][0])[t];
-    for (int t = lane; t < NG; t += 64) s_w[t] = NN == 8 ? 1.f : c_tab32.h20_w[t];
-    const int ce = lane / NG, g = lane - NG * (lane / NG);
-    for (int64_t e0 = (int64_t)blockIdx.x * EPW; e0 < nelem; e0 += (int64_t)gridDim.x * EPW) {
-        const int ne = (int)(nelem - e0 < EPW ? nelem - e0 : EPW);
-        for (int t = lane; t < ne * NN; t += 64) {
-            const int c = t / NN, b = t - NN * (t / NN);
-            const int64_t e = e0 + c;
-            const int64_t n0 = conn[e * NN], nd = conn[e * NN + b];
+    for (int t = threadIdx.x; t < NG; t += blockDim.x) s_w[t] = NN == 8 ? 1.f : c_tab32.h20_w[t];
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t e0 = ((int64_t)blockIdx.x * WPB + w) * EPW;
+    if (e0 >= nelem) return;
+    const int ne = (int)(nelem - e0 < EPW ? nelem - e0 : EPW);
+    float *so = reinterpret_cast<float *>(s_out[w]);
+    for (int t = lane; t < ne * NN; t += 64) {
+        const int c = t / NN, b = t - NN * (t / NN);
+        const int64_t e = e0 + c;
+        const int64_t n0 = conn[e * NN], nd = conn[e * NN + b];
 #pragma unroll
-            for (int q = 0; q < 3; q++) s_x[c][3 * b + q] = (float)(xyz[3 * nd + q] - xyz[3 * n0 + q]);
-        }
-        if (lane < ne) {
-            const double ed = E[e0 + lane], nud = NU[e0 + lane];
-            lm[e0 + lane] = make_float2((float)(ed * nud / ((1.0 + nud) * (1.0 - 2.0 * nud))),
-                                        (float)(ed / (2.0 * (1.0 + nud))));
-        }
-        __syncthreads();
-        if (ce < ne) {
-            float j[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-            for (int b = 0; b < NN; b++) {
-                const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    const float xq = s_x[ce][3 * b + q];
-                    j[q][0] += d0 * xq;
-                    j[q][1] += d1 * xq;
-                    j[q][2] += d2 * xq;
-                }
-            }
-            float det = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
-                        j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
-                        j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
-            if (!(det > 0.f)) {
-                report32(err, AXG_E_NEG_JACOBIAN, e0 + ce);
-                det = 1.f;
-            }
-            const float id = __builtin_amdgcn_sqrtf(s_w[g] * det) * __builtin_amdgcn_rcpf(det);
-            // iv[p][i] = s_g (J^-1)[p][i]
-            float iv[9];
-            iv[0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * id;
-            iv[1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) * id;
-            iv[2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * id;
-            iv[3] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) * id;
-            iv[4] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * id;
-            iv[5] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) * id;
-            iv[6] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * id;
-            iv[7] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) * id;
-            iv[8] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * id;
-            float *out = grad + (e0 + ce) * (int64_t)(NN * NGP);
-            for (int b = 0; b < NN; b++) {
-                const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
-                out[b * NGP + 3 * g] = iv[0] * d0 + iv[3] * d1 + iv[6] * d2;
-                out[b * NGP + 3 * g + 1] = iv[1] * d0 + iv[4] * d1 + iv[7] * d2;
-                out[b * NGP + 3 * g + 2] = iv[2] * d0 + iv[5] * d1 + iv[8] * d2;
-            }
-            if (NGP > 3 * NG && g < NGP - 3 * NG)  // the zero tail
-                for (int b = 0; b < NN; b++) out[b * NGP + 3 * NG + g] = 0.f;
-        }
-        __syncthreads();
+        for (int q = 0; q < 3; q++) s_x[w][c][3 * b + q] = (float)(xyz[3 * nd + q] - xyz[3 * n0 + q]);
     }
+    if (lane < ne) {
+        const double ed = E[e0 + lane], nud = NU[e0 + lane];
+        lm[e0 + lane] = make_float2((float)(ed * nud / ((1.0 + nud) * (1.0 - 2.0 * nud))),
+                                    (float)(ed / (2.0 * (1.0 + nud))));
+    }
+    wave_sync();
+    const int ce = lane / NG, g = lane - NG * (lane / NG);
+    if (ce < ne) {
+        float j[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        for (int b = 0; b < NN; b++) {
+            const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const float xq = s_x[w][ce][3 * b + q];
+                j[q][0] += d0 * xq;
+                j[q][1] += d1 * xq;
+                j[q][2] += d2 * xq;
+            }
+        }
+        float det = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
+                    j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
+                    j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
+        if (!(det > 0.f)) {
+            report32(err, AXG_E_NEG_JACOBIAN, e0 + ce);
+            det = 1.f;
+        }
+        const float id = __builtin_amdgcn_sqrtf(s_w[g] * det) * __builtin_amdgcn_rcpf(det);
+        // iv[p][i] = s_g (J^-1)[p][i]
+        float iv[9];
+        iv[0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * id;
+        iv[1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) * id;
+        iv[2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * id;
+        iv[3] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) * id;
+        iv[4] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * id;
+        iv[5] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) * id;
+        iv[6] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * id;
+        iv[7] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) * id;
+        iv[8] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * id;
+        float *out = so + ce * (NN * NGP);
+        for (int b = 0; b < NN; b++) {
+            const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
+            out[b * NGP + 3 * g] = iv[0] * d0 + iv[3] * d1 + iv[6] * d2;
+            out[b * NGP + 3 * g + 1] = iv[1] * d0 + iv[4] * d1 + iv[7] * d2;
+            out[b * NGP + 3 * g + 2] = iv[2] * d0 + iv[5] * d1 + iv[8] * d2;
+        }
+        if (NGP > 3 * NG && g < NGP - 3 * NG)  // the zero tail
+            for (int b = 0; b < NN; b++) out[b * NGP + 3 * NG + g] = 0.f;
+    }
+    wave_sync();
+    float4 *dst = reinterpret_cast<float4 *>(grad + e0 * (int64_t)(NN * NGP));
+    for (int t = lane; t < ne * (NN * NGP / 4); t += 64) dst[t] = s_out[w][t];
 }
 
 // Per-row LDS of the block-row assembly: CH elements per chunk, SPR slots.
 template <int NN, int CH, int SPR, int NQA>
 struct RowLds {
-    float4 c[CH][NN + 1][3];  // K_ab of the chunk's elements (9 values, 3 unused); b = NN: zero
+    float4 c[CH][NN + 1][2];  // K_ab of the chunk's elements, entries 0-7; b = NN: zero
+    float c8[CH][NN + 1];     // and entry 8
     float4 ga[CH][NQA];       // gradient row of the block row's own node in chunk element ce
     uint8_t src[SPR][CH];     // local node b of chunk element ce at slot k, NN for none
 };
@@ -175,7 +185,8 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
     static_assert(CH % 4 == 0, "src rows are read as words");
     __shared__ Lds s_row[WPB][RPW];
     const int64_t grp = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index is wave-uniform: row metadata comes from scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t I0 = (grp * WPB + w) * RPW;
     if (grp >= per_xcd * 8 || I0 >= nb) return;
     // rows of the wavefront: slot count and first slot (wave-uniform loads)
@@ -206,18 +217,18 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
             if (h == hh) W = rw[hh], base = rbase[hh];
         sh[j] = h, sk[j] = k, mask[j] = 0u, ccol[j] = cloc[j] = 0;
         sv[j] = h < RPW && k < W;
-        spos[j] = base + 64 * (int64_t)k;
+        // unconditional loads from a clamped position: no branch splits the wait counting
+        spos[j] = base + 64 * (int64_t)(sv[j] ? k : 0);
 #pragma unroll
         for (int r = 0; r < 9; r++) acc[j][r] = 0.f;
-        if (sv[j]) {
-            ccol[j] = bcol[spos[j]];
-            cloc[j] = bloc[spos[j]];
-            mask[j] = smask[spos[j]];
-        }
+        ccol[j] = bcol[spos[j]];
+        cloc[j] = bloc[spos[j]];
+        mask[j] = smask[spos[j]];
     }
     const int nch = (incw + CH - 1) / CH;
     for (int t = lane; t < RPW * CH * 3; t += 64)  // the zero blocks (never written again)
-        s_row[w][t / (CH * 3)].c[(t / 3) % CH][NN][t % 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t % 3 < 2) s_row[w][t / (CH * 3)].c[(t / 3) % CH][NN][t % 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+        else s_row[w][t / (CH * 3)].c8[(t / 3) % CH][NN] = 0.f;
     for (int c = 0; c < nch; c++) {
         // the tasks' incident pairs first, then their gradient rows
         int32_t sl[NIT];
@@ -225,9 +236,9 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
         for (int it = 0; it < NIT; it++) {
             const int t = lane + 64 * it;
             const int h = t / TPR, ce = (t - TPR * (t / TPR)) / NN;
-            sl[it] = -1;
-            if (t < RPW * TPR && I0 + h < nb && c * CH + ce < incw)
-                sl[it] = rinc[(I0 + h) * incw + c * CH + ce];
+            const bool ok = t < RPW * TPR && I0 + h < nb && c * CH + ce < incw;
+            const int32_t v = rinc[ok ? (I0 + h) * incw + c * CH + ce : 0];
+            sl[it] = ok ? v : -1;
         }
         for (int t = lane; t < RPW * CH * SPR / 4; t += 64)
             reinterpret_cast<uint32_t *>(&s_row[w][t / (CH * SPR / 4)].src[0][0])[t % (CH * SPR / 4)] =
@@ -240,12 +251,12 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
             for (int it = 0; it < NIT; it++) {
                 const int t = lane + 64 * it;
                 const int b = (t - TPR * (t / TPR)) % NN;
-                if (sl[it] < 0) continue;
-                const int64_t e = sl[it] / NN;
+                const int64_t sc = sl[it] < 0 ? 0 : sl[it];
+                const int64_t e = sc / NN;
                 const float4 *pb = reinterpret_cast<const float4 *>(grad + (e * NN + b) * NGP);
 #pragma unroll
                 for (int q = 0; q < NQ; q++) gb[it][q] = pb[q];
-                kk[it] = eslot[(int64_t)sl[it] * NN + b];
+                kk[it] = eslot[sc * NN + b];
                 lmv[it] = lm[e];
             }
 #pragma unroll
@@ -292,7 +303,7 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
                         kv[3 * r + cc] = lam * p[r][cc] + mu * p[cc][r] + (r == cc ? tr : 0.f);
                 R.c[ce][b][0] = make_float4(kv[0], kv[1], kv[2], kv[3]);
                 R.c[ce][b][1] = make_float4(kv[4], kv[5], kv[6], kv[7]);
-                R.c[ce][b][2] = make_float4(kv[8], 0.f, 0.f, 0.f);
+                R.c8[ce][b] = kv[8];
                 if (kk[it] < (unsigned)SPR) R.src[kk[it]][ce] = (uint8_t)b;
             }
         } else {
@@ -335,7 +346,7 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
             Lds &R = s_row[w][h];
             R.c[ce][b][0] = make_float4(kv[0], kv[1], kv[2], kv[3]);
             R.c[ce][b][1] = make_float4(kv[4], kv[5], kv[6], kv[7]);
-            R.c[ce][b][2] = make_float4(kv[8], 0.f, 0.f, 0.f);
+            R.c8[ce][b] = kv[8];
             if (k < (unsigned)SPR) R.src[k][ce] = (uint8_t)b;
         }
         }
@@ -351,9 +362,9 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
                 srcw[q] = reinterpret_cast<const uint32_t *>(&R.src[sk[jj]][0])[q];
 #pragma unroll
             for (int ce = 0; ce < CH; ce++) {
-                const float4 *cv = R.c[ce][(srcw[ce / 4] >> (8 * (ce % 4))) & 0xffu];
-                const float4 v0 = cv[0], v1 = cv[1], v2 = cv[2];
-                const float vv[9] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x};
+                const unsigned bb = (srcw[ce / 4] >> (8 * (ce % 4))) & 0xffu;
+                const float4 v0 = R.c[ce][bb][0], v1 = R.c[ce][bb][1];
+                const float vv[9] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, R.c8[ce][bb]};
 #pragma unroll
                 for (int r = 0; r < 9; r++) acc[jj][r] += vv[r];
             }
@@ -375,13 +386,14 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
     }
 }
 
-template <int NN, int NG, int NGP>
+template <int NN, int NG, int NGP, int WPB>
 static void launch_grad(axg_mesh *m, hipStream_t s, int32_t *d_err) {
     constexpr int EPW = 64 / NG;
-    const int64_t blocks = std::min<int64_t>((m->nelem + EPW - 1) / EPW, (int64_t)1 << 24);
-    hipLaunchKernelGGL((elem_grad32_kernel<NN, NG, NGP>), dim3((unsigned)blocks), dim3(64), 0, s,
-                       m->nelem, m->conn.p, m->xyz.p, m->e.p, m->nu.p, m->grad32.p,
-                       reinterpret_cast<float2 *>(m->lm32.p), d_err);
+    const int64_t blocks = (m->nelem + EPW * WPB - 1) / (EPW * WPB);
+    AXG_REQUIRE(blocks < INT32_MAX, AXG_E_INVALID, "FP32 assembly: too many elements");
+    hipLaunchKernelGGL((elem_grad32_kernel<NN, NG, NGP, WPB>), dim3((unsigned)blocks),
+                       dim3(64 * WPB), 0, s, m->nelem, m->conn.p, m->xyz.p, m->e.p, m->nu.p,
+                       m->grad32.p, reinterpret_cast<float2 *>(m->lm32.p), d_err);
     AXG_HIP(hipGetLastError());
 }
 
@@ -410,12 +422,12 @@ void launch_row_asm32(axg_mesh *m, hipStream_t s, int32_t *d_err) {
     m->grad32.ensure((size_t)m->nelem * nn * ngp);
     m->lm32.ensure(2 * (size_t)m->nelem);
     if (nn == 8) {
-        launch_grad<8, 8, ngp_of(8)>(m, s, d_err);
+        launch_grad<8, 8, ngp_of(8), 4>(m, s, d_err);
         if (wmax <= 32) launch_rows<8, ngp_of(8), 8, 1, 2, 4, 4>(m, s);
         else if (wmax <= 64) launch_rows<8, ngp_of(8), 8, 1, 1, 4>(m, s);
         else launch_rows<8, ngp_of(8), 8, 4, 1, 4>(m, s);
     } else {
-        launch_grad<20, 27, ngp_of(27)>(m, s, d_err);
+        launch_grad<20, 27, ngp_of(27), 1>(m, s, d_err);
         if (wmax <= 128) launch_rows<20, ngp_of(27), 8, 2, 1, 4>(m, s);
         else launch_rows<20, ngp_of(27), 8, 4, 1, 4>(m, s);
     }

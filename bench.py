@@ -7,7 +7,8 @@ brick (default)  C3D8 brick 80x80x80 (512,000 elements, 531,441 nodes,
     nu = 0.3, bottom face clamped, four load cases on the top face (1 MPa
     pressure P2, x and y shear of the same total, a seeded random normal
     load). One step = numeric assembly (FP64 CSR for the true residual +
-    FP32 MFMA integration into the FP32 BSR-3 operator) followed by the
+    the fused FP32 integration and block-row assembly of the FP32 BSR-3
+    operator) followed by the
     mixed-precision multi-right-hand-side PCG (FP32 inner, FP64 refinement)
     of all load cases to ||f - K u|| <= tol ||f||. After the timed steps two
     FP64 GPU solves of the same load cases to the same tolerance are timed:
@@ -351,12 +352,14 @@ def run_brick(a, G, MG, torch, np):
         mesh.assemble("mixed")
 
     nb, nnzb = n // 3, nnz // 9  # every free node keeps all 3 DOFs in this workload
+    # the reference's dense element count (B^T D B per Gauss point)
     flops_el = 8 * (72 * 24 + 14 * 24 * 24)
-    ke32_tflops = flops_el * nelem / (asm["ke32"] / 1e3) / 1e12
-    # assembly (integration + scatter into the operator) against the compute floor of the
-    # reference's dense element count
-    asm32_tflops = flops_el * nelem / ((asm["ke32"] + asm["gather32"]) / 1e3) / 1e12
+    asm32_ms = asm["ke32"] + asm["gather32"]
+    asm32_tflops = flops_el * nelem / (asm32_ms / 1e3) / 1e12
     asm64_tflops = flops_el * nelem / ((asm["ke64"] + asm["gather64"]) / 1e3) / 1e12
+    # FP32 assembly against HBM: the operator it must write (three value planes and the
+    # LDS-staged SpMM's third plane, 48 B per block) and the mesh it must read
+    asm32_bytes = 48 * nnzb + 24 * prob.nnode + (4 * 8 + 16) * nelem
     out = base_line(a, mixed_dof_iter_s, ms_per_step, "f32 inner PCG + f64 refinement", {
         "workload": f"C3D8 brick {a.n}^3, lognormal E, bottom clamped, {k} top-face load "
                     f"cases, mixed-precision multi-RHS PCG to rel. residual {a.tol:g}",
@@ -378,8 +381,9 @@ def run_brick(a, G, MG, torch, np):
             "ke64_ms": asm["ke64"], "gather64_ms": asm["gather64"], "ke32_ms": asm["ke32"],
             "gather32_ms": asm["gather32"],
             "elements_per_s_fp64": nelem / ((asm["ke64"] + asm["gather64"]) / 1e3),
-            "elements_per_s_fp32": nelem / ((asm["ke32"] + asm["gather32"]) / 1e3),
-            "ke32_mfma_tflops": ke32_tflops, "ke32_mfma_frac": ke32_tflops / FP32_PEAK_TFLOPS,
+            "elements_per_s_fp32": nelem / (asm32_ms / 1e3),
+            "assembly_fp32_bytes": asm32_bytes,
+            "assembly_fp32_hbm_frac": asm32_bytes / (asm32_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
             "assembly_fp32_roofline_frac": asm32_tflops / FP32_PEAK_TFLOPS,
             "assembly_fp64_roofline_frac": asm64_tflops / FP64_PEAK_TFLOPS,
         },
@@ -516,8 +520,6 @@ def run_c3d20(a, G, MG, torch, np):
     # C3D20's nodes all keep 3 DOFs here (the clamped face is fully fixed): n / 3 block rows
     v64["spmv_gbs"] = bsell64_bytes(n // 3, nnz // 9, 1) / (v64["spmv_ms_avg"] / 1e3) / 1e9
     ne = prob.nelem
-    # dense 60x60 per Gauss point on the matrix cores: 64x64 tiles, 6 strain rows
-    flops_el = 27 * 2 * 64 * 64 * 6
     out = base_line(a, n * inner / (solve_ms / 1e3), ms_per_step, "f32 inner PCG + f64 refinement", {
         "workload": "C3D20 brick 96x24x24, jittered interior corners +-0.15 h (seed 0), x=0 "
                     f"clamped, tip shear 1 N/node; mixed block-Jacobi PCG to {tol:g}",
@@ -531,7 +533,6 @@ def run_c3d20(a, G, MG, torch, np):
             "gather32_ms": asm["gather32"],
             "elements_per_s_fp64": ne / ((asm["ke64"] + asm["gather64"]) / 1e3),
             "elements_per_s_fp32": ne / ((asm["ke32"] + asm["gather32"]) / 1e3),
-            "ke32_mfma_tflops": flops_el * ne / (asm["ke32"] / 1e3) / 1e12,
         },
         "roofline": roofline("bsell_spmm_kernel<K=1> (FP32 block-sliced-ELL SpMV)",
                              spmm_bytes(n // 3, nnz // 9, 1), spmm),
