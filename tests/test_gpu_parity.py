@@ -309,8 +309,15 @@ def test_mixed_op32_vs_fp64(ctx, nn, seed, offset, dup):
     ne = len(conn)
     e = rng.uniform(1e3, 3e5, ne)
     mesh = G.Mesh(nn, xyz, conn, e, 0.3, MG.free_map(len(xyz), [0, 1]), ctx=ctx)
-    mesh.assemble("mixed")
+    check_op32(mesh)
+    mesh.close()
+
+
+def check_op32(mesh):
+    """The FP32 operator entry by entry against the FP64 K_ff (relative to the row's largest
+    entry), its pattern covering K_ff's, and its symmetry."""
     import scipy.sparse as sp
+    mesh.assemble("mixed")
     indptr, indices, data = mesh.csr()
     n = len(indptr) - 1
     k64 = sp.csr_matrix((data, indices, indptr), shape=(n, n)).toarray()
@@ -322,6 +329,46 @@ def test_mixed_op32_vs_fp64(ctx, nn, seed, offset, dup):
     # every coupling of K_ff is stored; nothing outside the block pattern
     assert np.all((k32 != 0) | (k64 == 0))
     assert np.abs(k32 - k32.T).max() < 3e-6 * rowmax.max()
+
+
+def fan_hex8(nsec, layers):
+    """nsec kite-shaped hexahedra around the z axis per layer: every node on the axis is shared
+    by 2 nsec elements (nsec in the end layers) and couples with 6 nsec + 3 nodes."""
+    per = 1 + 2 * nsec
+    pts = []
+    for z in range(layers + 1):
+        pts.append((0.0, 0.0, float(z)))
+        for i in range(nsec):
+            t = 2 * np.pi * i / nsec
+            pts.append((np.cos(t), np.sin(t), float(z)))
+        for i in range(nsec):
+            t = 2 * np.pi * (i + 0.5) / nsec
+            pts.append((1.3 * np.cos(t), 1.3 * np.sin(t), float(z)))
+    conn = []
+    for z in range(layers):
+        for i in range(nsec):
+            q = [0, 1 + i, 1 + nsec + i, 1 + (i + 1) % nsec]
+            conn.append([z * per + v for v in q] + [(z + 1) * per + v for v in q])
+    return np.array(pts), np.array(conn, np.int64)
+
+
+@pytest.mark.parametrize("nsec", [6, 16])
+def test_mixed_op32_wide_block_rows(ctx, nsec):
+    """Block rows wider than the two-rows-per-wavefront form (the axis nodes of a fan of
+    hexahedra: 39 and 99 blocks, 12 and 32 incident elements), partially constrained nodes (the
+    outer top nodes keep only y and z free): the FP32 operator against the FP64 K_ff."""
+    say("case: op32 fan", nsec)
+    xyz, conn = fan_hex8(nsec, 2)
+    per = 1 + 2 * nsec
+    ind = np.full(3 * len(xyz), -1, np.int64)
+    pres = np.zeros(3 * len(xyz), bool)
+    pres[:3 * per] = True                                    # bottom layer clamped
+    pres[3 * (2 * per + 1 + nsec + np.arange(nsec))] = True  # outer top nodes: x prescribed
+    ind[~pres] = np.arange(int((~pres).sum()))
+    rng = np.random.default_rng(nsec)
+    mesh = G.Mesh(8, xyz, conn, rng.uniform(1e3, 3e5, len(conn)), 0.3, ind, ctx=ctx)
+    check_op32(mesh)
+    mesh.close()
 
 
 @pytest.mark.parametrize("name,precond", [("le10_c3d20", "block3"), ("le10_c3d20_fine", "jacobi"),
