@@ -1050,13 +1050,14 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__res
     }
 }
 
-// rz_prev (may be null): a restart that keeps the search direction (reliable
-// update) passes the last r.z of the previous interval in the new scaling, and
-// beta_j = r.z / rz_prev_j joins the new residual to the kept direction.
+// tol: the relative inner tolerance of every column. rz_prev (may be null): a
+// restart that keeps the search direction (reliable update) passes the last r.z
+// of the previous interval in the new scaling, and beta_j = r.z / rz_prev_j
+// joins the new residual to the kept direction.
 template <int K>
 __global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__restrict__ part,
                                                             int nblk, InnerScalars *sc,
-                                                            double inner_tol,
+                                                            const double *__restrict__ tol,
                                                             const double *__restrict__ rz_prev) {
     double tot[2 * K];
     fin_sums<2 * K>(part, nblk, tot);
@@ -1066,7 +1067,7 @@ __global__ void __launch_bounds__(FIN_TB) fin_inner_init_kernel(const double *__
             double bn = sqrt(tot[j]);
             sc->bnorm[j] = bn;
             sc->res[j] = bn;
-            sc->tol_abs[j] = inner_tol * bn;
+            sc->tol_abs[j] = tol[j] * bn;
             if (rz_prev) sc->beta[j] = rz_prev[j] > 0.0 ? tot[K + j] / rz_prev[j] : 0.0;
             sc->rz[j] = tot[K + j];
             sc->iters[j] = 0;
@@ -1472,7 +1473,7 @@ struct MixedSolver {
         w.f4.ensure(nv * K);  // AP
         DevBuf<float> Z(nv * K);
         w.t0.ensure(n * K);   // r64
-        w.t1.ensure(48);      // scales, alpha, beta, restart ratio and r.z
+        w.t1.ensure(56);      // scales, alpha, beta, restart ratio and r.z, inner tolerances
         const int shp = tune_spmm<K>(m, w.f3.p, w.f4.p);
         const int grid_mv = spmm_grid<K, true>(ctx, nb, shp);
         const int grid_v = std::max(1, (int)std::min<int64_t>((nb + MBS - 1) / MBS,
@@ -1571,8 +1572,18 @@ struct MixedSolver {
         // warm: the search direction of the previous interval runs on (its scale
         // was prev_scale), with beta from the previous interval's last r.z.
         double prev_scale[K], prev_rz[K];
-        auto inner_solve = [&](const double (&scale)[K], bool warm) {
+        // the inner tolerance of a sweep: inner_tol, loosened for a column whose
+        // residual is within 1 / inner_tol of the target to a third of the
+        // reduction it still needs (the last sweep stops near what it needs)
+        auto sweep_tol = [&](double (&tc)[K]) {
+            for (int j = 0; j < K; j++) {
+                const double need = rnorm[j] > 0.0 ? o.tol * fnorm[j] / rnorm[j] : 1.0;
+                tc[j] = std::max(o.inner_tol, std::min(0.5, need / 3.0));
+            }
+        };
+        auto inner_solve = [&](const double (&scale)[K], bool warm, const double (&tc)[K]) {
             put(8, scale);
+            put(48, tc);
             hipLaunchKernelGGL((pack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.t0.p,
                                m->blk_of_free.p, w.t1.p + 8, w.f0.p, nb);
             if (warm) {
@@ -1598,7 +1609,7 @@ struct MixedSolver {
                                    nb, w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p,
                                    w.part.p, dsc.p, o.inner_tol, w.f3.p, ratio_p);
             hipLaunchKernelGGL((fin_inner_init_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
-                               grid_v, dsc.p, o.inner_tol, warm ? w.t1.p + 40 : nullptr);
+                               grid_v, dsc.p, w.t1.p + 48, warm ? w.t1.p + 40 : nullptr);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
                                w.f3.p, Z.p, dsc.p, warm ? 0 : 1);
             AXG_HIP(hipGetLastError());
@@ -1670,7 +1681,9 @@ struct MixedSolver {
                     break;
                 }
                 sweeps++;
-                inner_solve(scale, false);
+                double tc[K];
+                sweep_tol(tc);
+                inner_solve(scale, false, tc);
                 hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
                                    w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u, nb);
                 true_residual_check();
@@ -1689,7 +1702,9 @@ struct MixedSolver {
                     break;
                 }
                 sweeps++;
-                inner_solve(scale, warm);
+                double tc[K];
+                for (int j = 0; j < K; j++) tc[j] = o.inner_tol;
+                inner_solve(scale, warm, tc);
                 warm = true;
                 hipLaunchKernelGGL((unpack_add_kernel<K>), dim3(grid_n), dim3(256), 0, s, n,
                                    w.f1.p, m->blk_of_free.p, w.t1.p + 8, d_u, nb);
@@ -1716,7 +1731,9 @@ struct MixedSolver {
                     break;
                 }
                 sweeps++;
-                inner_solve(scale, false);
+                double tc[K];
+                sweep_tol(tc);
+                inner_solve(scale, false, tc);
                 hipLaunchKernelGGL((unpack_kernel<K>), dim3(grid_n), dim3(256), 0, s, n, w.f1.p,
                                    m->blk_of_free.p, w.t1.p + 8, Zd.p, nb);
                 double beta[K];
