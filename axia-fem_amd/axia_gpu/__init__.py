@@ -258,6 +258,13 @@ class Mesh:
         check(_lib.lib().axg_mesh_spmm_shape(self._h, nrhs, C.byref(v)))
         return v.value
 
+    def asm32_mode(self, mode=-1):
+        """FP32 operator assembly form: 0 fused integration + block rows, 1 element blocks on
+        the matrix cores + block-row gather; pin it with mode >= 0; returns the active one."""
+        v = C.c_int32(mode)
+        check(_lib.lib().axg_mesh_asm32_mode(self._h, C.byref(v)))
+        return v.value
+
     def nodal_stress(self, u_full):
         u = self._u(u_full)
         stress = np.zeros((self.nnode, 6))

@@ -117,6 +117,7 @@ SIGNATURES = {
                                   C.POINTER(SolveInfo)]),
     "axg_bench_stream": (C.c_int, [_vp, _i64, _i32, _dp]),
     "axg_mesh_spmm_shape": (C.c_int, [_vp, _i32, C.POINTER(_i32)]),
+    "axg_mesh_asm32_mode": (C.c_int, [_vp, C.POINTER(_i32)]),
     "axg_bench_operator": (C.c_int, [_vp, _i32, _i32, _i32, _dp]),
     "axg_pattern_host": (C.c_int, [_i32, _i64, _i64, _ip, _lp, _lp, _lp, _lp, _lp, _lp, _lp, _lp]),
     "axg_pressure_loads": (C.c_int, [_i32, _i64, _dp, _i64, _ip, _ip, _dp, _dp]),

@@ -66,6 +66,19 @@ extern "C" int axg_mesh_spmm_shape(axg_mesh *m, int32_t nrhs, int32_t *shape) {
     return AXG_OK;
 }
 
+extern "C" int axg_mesh_asm32_mode(axg_mesh *m, int32_t *mode) {
+    try {
+        clear_error();
+        AXG_REQUIRE(m && mode && *mode <= 1, AXG_E_INVALID, "axg_mesh_asm32_mode: invalid argument");
+        if (*mode >= 0) m->asm32_mode = *mode;
+        *mode = asm32_mode_of(m);
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    return AXG_OK;
+}
+
 extern "C" int axg_bench_operator(axg_mesh *m, int32_t which, int32_t nrhs, int32_t reps,
                                   double *avg_ms) {
     try {

@@ -257,6 +257,8 @@ struct axg_mesh {
     axg::DevBuf<float> bval;                  // block planes, 10 floats per slot
     axg::DevBuf<float> grad32;                // FP32 scaled shape gradients per element
     axg::DevBuf<float> lm32;                  // FP32 (lam, mu) per element
+    axg::DevBuf<float> kb32;                  // FP32 element records, blocks a <= b (3 planes)
+    int asm32_mode = -1;                      // FP32 assembly: 0 fused rows, 1 MFMA blocks, -1 default
     axg::DevBuf<uint8_t> eslot;               // (e, a, b) -> slot of b's node in a's block row
     axg::DevBuf<int32_t> b_inc;               // b_incw per block row: e * nn + a ascending, -1
     int32_t b_incw = 0;                       // most elements at one block row's node
@@ -310,7 +312,10 @@ void build_pattern(int nn, int64_t nnode, int64_t nelem, const int32_t *conn,
 // kernels (launch wrappers), axg_element.hip
 void launch_ke_fp64(axg_mesh *m, hipStream_t s, int32_t *d_err);
 void launch_gather_fp64(axg_mesh *m, hipStream_t s, bool topo, uint32_t *d_zero_count);
-void launch_row_asm32(axg_mesh *m, hipStream_t s, int32_t *d_err);
+// FP32 operator of a mesh; true when it ran as two kernels (element blocks,
+// then the block-row gather), with `mid` recorded between them
+bool launch_row_asm32(axg_mesh *m, hipStream_t s, int32_t *d_err, hipEvent_t mid);
+int asm32_mode_of(const axg_mesh *m);
 void launch_csr_to_bsr32(axg_mesh *m, hipStream_t s);
 void upload_shape_tables32(const double *h8_dn, const double *h20_dn, const double *h20_w);
 void launch_nodal_stress(axg_mesh *m, hipStream_t s, const double *d_u, double *d_elem_nodal,

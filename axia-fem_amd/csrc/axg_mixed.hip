@@ -444,11 +444,12 @@ void ensure_bsell64(axg_mesh *m) {
     m->bsell64_fresh = true;
 }
 
-// The FP32 operator: for a mesh, the fused integration + block-row assembly
-// (every plane written by the kernel); for an operator taken from a caller's
-// CSR, its FP64 values rounded into zeroed planes, then the column and LDS
-// planes. t_ke32 is the device time of that; t_gather32 stays 0 (there is no
-// separate gather).
+// The FP32 operator: for a mesh, either the fused integration + block-row
+// assembly (every plane written by one kernel) or the element blocks on the
+// matrix cores followed by the block-row gather; for an operator taken from a
+// caller's CSR, its FP64 values rounded into zeroed planes, then the column and
+// LDS planes. t_ke32 is the device time of the element kernel (of everything
+// when there is one kernel), t_gather32 that of the gather (else 0).
 void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
@@ -459,6 +460,7 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     Workspace &w = ctx->ws;
     AXG_HIP(hipMemsetAsync(w.err.p, 0, 8, s));
     AXG_HIP(hipEventRecord(ctx->ev0, s));
+    bool split = false;
     if (m->csr_operator) {
         AXG_HIP(hipMemsetAsync(m->bval.p, 0, sizeof(float) * 10 * m->bs_nslots, s));
         if (m->bs_nslots)
@@ -472,7 +474,7 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
                                m->bs_loc.p, m->bval_loc.p);
         AXG_HIP(hipGetLastError());
     } else {
-        launch_row_asm32(m, s, w.err.p);
+        split = launch_row_asm32(m, s, w.err.p, ctx->evA);
     }
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     int32_t err[2];
@@ -480,10 +482,11 @@ void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     AXG_HIP(hipStreamSynchronize(s));
     if (err[0]) throw Error{err[0], "FP32 integration: negative Jakobideterminante (element index " +
                                         std::to_string(err[1]) + ")"};
-    float ms = 0.f;
+    float ms = 0.f, ms_a = 0.f;
     AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    m->t_ke32 = ms;
-    m->t_gather32 = 0.0;
+    if (split) AXG_HIP(hipEventElapsedTime(&ms_a, ctx->ev0, ctx->evA));
+    m->t_ke32 = split ? ms_a : ms;
+    m->t_gather32 = split ? ms - ms_a : 0.0;
     if (elapsed_ms) *elapsed_ms = ms;
     m->assembled32 = true;
 }

@@ -211,6 +211,29 @@ def avg_launch(info):
     return info["spmv_ms"] / max(1, info["spmv_launches"])
 
 
+def asm32_forms(a, mesh, nelem, reps=None):
+    """Device time of both FP32 assembly forms on a mesh (the form in use is restored):
+    0 integration fused with the block-row assembly, 1 element blocks on MFMA + gather."""
+    import numpy as np
+    keep = mesh.asm32_mode()
+    out = {}
+    for mode in (0, 1):
+        mesh.asm32_mode(mode)
+        mesh.assemble("mixed")
+        ts = []
+        for _ in range(reps or max(a.reps, 3)):
+            mesh.assemble("mixed")
+            ts.append(mesh.assembly_times())
+        ke = float(np.mean([t["ke32"] for t in ts]))
+        ga = float(np.mean([t["gather32"] for t in ts]))
+        out[("fused_rows", "mfma_blocks")[mode]] = {
+            "element_ms": ke, "gather_ms": ga, "elements_per_s": nelem / ((ke + ga) / 1e3)}
+        log(f"FP32 assembly form {mode}: {ke:.3f} + {ga:.3f} ms")
+    mesh.asm32_mode(keep)
+    mesh.assemble("mixed")
+    return out
+
+
 # ----------------------------------------------------------- brick (default)
 def kernels_only(a, G, mesh, prob, n, nnz):
     """Repeated launches of the dominant kernels (input for rocprofv3 runs)."""
@@ -244,6 +267,7 @@ def run_brick(a, G, MG, torch, np):
     prob = MG.brick_lognormal(a.n, seed=0, nloads=a.nrhs)
     ctx = G.Context(0)
     mesh = G.Mesh(8, prob.xyz, prob.conn, prob.e, prob.nu, prob.ind_of, ctx=ctx)
+    mesh.asm32_mode(a.asm32_mode)
     info = mesh.info()
     n, nnz = info["n_free"], info["nnz"]
     log(f"mesh {prob.name}: n={n} nnz={nnz} contributions={info['ncontrib']} "
@@ -297,6 +321,7 @@ def run_brick(a, G, MG, torch, np):
     spmv_ms = float(np.mean([avg_launch(s[2]) for s in steps]))
     asm = {kk: float(np.mean([s[1][kk] for s in steps])) for kk in steps[0][1]}
     nelem = prob.nelem
+    forms = asm32_forms(a, mesh, nelem)
     # the same solve replayed as hipGraphs without per-launch events (untimed)
     g = mesh.solve_device(B.data_ptr(), k, X.data_ptr(), precision="mixed", precond=a.precond,
                           tol=a.tol, inner_tol=a.inner_tol, refinement=a.refinement)
@@ -386,6 +411,7 @@ def run_brick(a, G, MG, torch, np):
             "assembly_fp32_hbm_frac": asm32_bytes / (asm32_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
             "assembly_fp32_roofline_frac": asm32_tflops / FP32_PEAK_TFLOPS,
             "assembly_fp64_roofline_frac": asm64_tflops / FP64_PEAK_TFLOPS,
+            "fp32_forms": forms,
         },
         "speedup_mixed_vs_fp64_plain": (plain["solve_ms"] / solve_ms) if plain else None,
         "speedup_mixed_vs_fp64_best": (fp64["solve_ms"] / solve_ms) if fp64 else None,
@@ -465,9 +491,15 @@ def run_c3d20(a, G, MG, torch, np):
     t0 = time.perf_counter()
     prob = MG.brick_c3d20_jitter(96, 24, 24, seed=0)
     mesh = G.Mesh(20, prob.xyz, prob.conn, prob.e, prob.nu, prob.ind_of)
+    mesh.asm32_mode(a.asm32_mode)
     info = mesh.info()
     n, nnz = info["n_free"], info["nnz"]
     log(f"mesh {prob.name}: n={n} nnz={nnz} setup {time.perf_counter() - t0:.1f}s")
+    forms = asm32_forms(a, mesh, prob.nelem)
+    if a.kernels:
+        print(json.dumps({"workload": prob.name, "n_free": n, "nnz": nnz, "reps": a.reps,
+                          "assembly_fp32_forms": forms}))
+        return None
     b = torch.tensor(prob.rhs(), dtype=torch.float64, device="cuda:0")
     x = torch.zeros_like(b)
     xref = torch.zeros_like(b)
@@ -533,6 +565,7 @@ def run_c3d20(a, G, MG, torch, np):
             "gather32_ms": asm["gather32"],
             "elements_per_s_fp64": ne / ((asm["ke64"] + asm["gather64"]) / 1e3),
             "elements_per_s_fp32": ne / ((asm["ke32"] + asm["gather32"]) / 1e3),
+            "fp32_forms": forms,
         },
         "roofline": roofline("bsell_spmm_kernel<K=1> (FP32 block-sliced-ELL SpMV)",
                              spmm_bytes(n // 3, nnz // 9, 1), spmm),
@@ -1044,6 +1077,9 @@ def main():
     ap.add_argument("--precond", default="block3")
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--refinement", default="fcg", choices=("fcg", "plain", "reliable"))
+    ap.add_argument("--asm32-mode", type=int, default=-1, choices=(-1, 0, 1),
+                    help="FP32 assembly form: 0 fused rows, 1 MFMA element blocks + gather, "
+                         "-1 the library's default for the element type")
     ap.add_argument("--fd-elements", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

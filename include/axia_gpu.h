@@ -367,6 +367,15 @@ int axg_bench_operator(axg_mesh *mesh, int32_t which, int32_t nrhs, int32_t reps
  * right-hand sides: *shape >= 0 pins it (instead of the autotuner's pick at
  * the first solve), then *shape receives the active one (-1: not tuned yet). */
 int axg_mesh_spmm_shape(axg_mesh *mesh, int32_t nrhs, int32_t *shape);
+/* The FP32 operator's assembly form of a mesh's mixed solves (the element
+ * integration of elem.rs:199-219 / quadratic.rs:122-146 and the scatter of
+ * analysis.rs:651-668, in FP32): 0 = integration fused with the block-row
+ * assembly (no element matrix stored), 1 = element blocks on the matrix cores
+ * (P = W W^T by MFMA, the upper 3x3 blocks of each element stored), then the
+ * block rows gathered from them. *mode >= 0 pins it; *mode then receives the
+ * active one (the default: 0 for C3D8, 1 for C3D20). Both give the same
+ * operator up to FP32 rounding; the next mixed assembly uses it. */
+int axg_mesh_asm32_mode(axg_mesh *mesh, int32_t *mode);
 
 /* ------------------------------------------------- host-only helpers */
 /* The symbolic phase alone, on the host (no device): K_ff pattern and scatter

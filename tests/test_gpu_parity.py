@@ -290,15 +290,17 @@ def test_brick_properties_40(ctx):
 
 
 # ------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("nn,seed,offset,dup", [(8, 4, 0.0, 1), (20, 5, 0.0, 1), (8, 6, 1e5, 1),
-                                                (20, 7, 1e5, 1), (8, 8, 0.0, 3)])
-def test_mixed_op32_vs_fp64(ctx, nn, seed, offset, dup):
-    """The FP32 operator (fused FP32 integration + block-row assembly) agrees with the
-    FP64 K_ff entry by entry to FP32 accuracy (relative to its row's largest entry), also
-    far from the origin (coordinates are taken relative to each element's first node) and
-    with every element listed three times (24 elements at an interior node: several
+                                                (20, 7, 1e5, 1), (8, 8, 0.0, 3), (20, 9, 0.0, 3)])
+def test_mixed_op32_vs_fp64(ctx, nn, seed, offset, dup, mode):
+    """The FP32 operator agrees with the FP64 K_ff entry by entry to FP32 accuracy (relative
+    to its row's largest entry), in both assembly forms (0: integration fused with the
+    block-row assembly; 1: element blocks on the matrix cores, then the block-row gather),
+    also far from the origin (coordinates are taken relative to each element's first node)
+    and with every element listed three times (24 elements at an interior node: several
     element chunks per block row); the constrained DOFs are absent from it."""
-    say("case: op32", nn, seed, offset, dup)
+    say("case: op32", nn, seed, offset, dup, "mode", mode)
     if nn == 8:
         xyz, conn = MG.brick_hex8(3, 3, 2)
     else:
@@ -309,6 +311,7 @@ def test_mixed_op32_vs_fp64(ctx, nn, seed, offset, dup):
     ne = len(conn)
     e = rng.uniform(1e3, 3e5, ne)
     mesh = G.Mesh(nn, xyz, conn, e, 0.3, MG.free_map(len(xyz), [0, 1]), ctx=ctx)
+    assert mesh.asm32_mode(mode) == mode
     check_op32(mesh)
     mesh.close()
 
@@ -352,12 +355,14 @@ def fan_hex8(nsec, layers):
     return np.array(pts), np.array(conn, np.int64)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("nsec", [6, 16])
-def test_mixed_op32_wide_block_rows(ctx, nsec):
+def test_mixed_op32_wide_block_rows(ctx, nsec, mode):
     """Block rows wider than the two-rows-per-wavefront form (the axis nodes of a fan of
     hexahedra: 39 and 99 blocks, 12 and 32 incident elements), partially constrained nodes (the
-    outer top nodes keep only y and z free): the FP32 operator against the FP64 K_ff."""
-    say("case: op32 fan", nsec)
+    outer top nodes keep only y and z free): the FP32 operator against the FP64 K_ff, both
+    assembly forms."""
+    say("case: op32 fan", nsec, "mode", mode)
     xyz, conn = fan_hex8(nsec, 2)
     per = 1 + 2 * nsec
     ind = np.full(3 * len(xyz), -1, np.int64)
@@ -367,6 +372,7 @@ def test_mixed_op32_wide_block_rows(ctx, nsec):
     ind[~pres] = np.arange(int((~pres).sum()))
     rng = np.random.default_rng(nsec)
     mesh = G.Mesh(8, xyz, conn, rng.uniform(1e3, 3e5, len(conn)), 0.3, ind, ctx=ctx)
+    mesh.asm32_mode(mode)
     check_op32(mesh)
     mesh.close()
 
