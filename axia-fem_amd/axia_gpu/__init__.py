@@ -379,14 +379,16 @@ class Mesh:
         return c.value, dc
 
 
-def solve_batch(meshes, rhs_list, tol=1e-8, max_iter=0, ctx=None):
-    """Solve all meshes' load cases in one batched launch sequence.
+def solve_batch(meshes, rhs_list, tol=1e-8, max_iter=0, ctx=None, groups=0):
+    """Solve all meshes' load cases in one batched call.
 
     meshes: assembled Mesh objects on one context; rhs_list[m]: (k_m, n_free_m)
-    array (k_m <= 8). Returns (xs, iters, latency_ms, info): xs[m] has the
-    shape of rhs_list[m]; iters and latency_ms list every (mesh, load case)
-    solve in order, latency measured from the start of the batch to that
-    solve's convergence on the device clock.
+    array (k_m <= 8). groups: 1 advances every mesh in one launch sequence, G > 1
+    splits the meshes by size into G groups solved one after another, smallest
+    first (at most 16), 0 lets the library choose (one group per 6 meshes). Returns (xs, iters,
+    latency_ms, info): xs[m] has the shape of rhs_list[m]; iters and latency_ms
+    list every (mesh, load case) solve in order, latency measured from the start
+    of the batch to that solve's convergence on the device clock.
     """
     ctx = ctx or meshes[0].ctx
     nm = len(meshes)
@@ -407,6 +409,7 @@ def solve_batch(meshes, rhs_list, tol=1e-8, max_iter=0, ctx=None):
     lat = np.zeros(total)
     info = _lib.SolveInfo()
     o = _opts(PREC_FP64, PRECOND_JACOBI, tol, max_iter)
+    o.batch_groups = int(groups)
     check(_lib.lib().axg_solve_batch(ctx.handle, nm, H, R, iptr(nrhs), C.byref(o), X, lptr(iters),
                                      dptr(lat), C.byref(info)))
     return xs, iters, lat, _info_dict(info)

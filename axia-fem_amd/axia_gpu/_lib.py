@@ -47,7 +47,7 @@ class SolveOpts(C.Structure):
         ("max_outer", C.c_int32),
         ("timing", C.c_int32),
         ("refinement", C.c_int32),
-        ("reserved", C.c_int32),
+        ("batch_groups", C.c_int32),
     ]
 
 

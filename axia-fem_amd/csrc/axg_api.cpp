@@ -315,7 +315,7 @@ void axg_default_opts(axg_solve_opts *o) {
     o->max_outer = 30;
     o->timing = 0;
     o->refinement = AXG_REFINE_FCG;
-    o->reserved = 0;
+    o->batch_groups = 0;
 }
 
 static void ctx_free(axg_ctx *ctx);

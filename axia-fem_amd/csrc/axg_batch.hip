@@ -794,6 +794,8 @@ static void block_group(BlockPcg &bp, const double *d_b, int kk, const double *t
 // side by side with their columns shifted to the segment's first block row;
 // a slice of a mesh whose right-hand sides have all stopped does nothing.
 constexpr int64_t kSegRows = 768;
+constexpr int kMaxGroups = 16;      // size groups of one batched solve
+constexpr int kMeshesPerGroup = 6;  // group size when the caller leaves the choice
 
 // column bits of plane 4 (the low 32 bits of entry 8's neighbour) + off
 __global__ void rebase_block_cols_kernel(int64_t nslots, long long *__restrict__ plane4,
@@ -805,226 +807,322 @@ __global__ void rebase_block_cols_kernel(int64_t nslots, long long *__restrict__
     }
 }
 
+// One group of meshes: the segments of its meshes, its vectors and scalars,
+// and the captured iteration sequence.
+template <int K>
+struct BatchGroup {
+    hipStream_t s = nullptr;
+    std::vector<int> mesh;  // indices into the caller's mesh list
+    std::vector<int64_t> row0;
+    std::vector<std::vector<int32_t>> bof;
+    int64_t nrow = 0, max_iter = 0, launched = 0;
+    int nseg = 0, nwg = 0;
+    DevBuf<int64_t> sptr, wg_row, seg_end;
+    DevBuf<int32_t> width, slice_seg, wg_seg, segwg, segsl, seg_live, live_total;
+    DevBuf<double> bv, dsel, Bv, X, R, P, AP, Z, part;
+    DevBuf<SegScalars> sc;
+    BatchDev B, Bmv;
+    Bsell64Args op{};
+    int U = 2, L = 1;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev = nullptr;  // the live count of the last replay has arrived
+    bool running = false;
+
+    ~BatchGroup() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (ev) (void)hipEventDestroy(ev);
+    }
+
+    void one_iter() {
+        bsell64_spmm<K>(op, s, true, U, L, P.p);
+        hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(nseg), dim3(FIN_TB), 0, s, Bmv, part.p,
+                           sc.p, live_total.p);
+        hipLaunchKernelGGL((batch_update_kernel<K, false>), dim3(nwg), dim3(TB), 0, s, B, X.p,
+                           R.p, P.p, AP.p, Z.p, part.p, sc.p);
+        hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nseg), dim3(FIN_TB), 0, s, B, part.p,
+                           sc.p, 0, tol, nullptr, max_iter, live_total.p);
+        hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p,
+                           sc.p, 0);
+    }
+    double tol = 1e-8;
+
+    // segments, operator planes, right-hand sides, tuning and the captured
+    // iteration sequence (nothing iterates yet)
+    void setup(axg_ctx *ctx, axg_mesh *const *meshes, const double *const *rhs,
+               const int32_t *nrhs, const axg_solve_opts &o) {
+        nseg = (int)mesh.size();
+        tol = o.tol;
+        row0.assign(nseg + 1, 0);
+        std::vector<int64_t> slot0(nseg + 1, 0), slice0(nseg + 1, 0);
+        std::vector<int32_t> seg_wg(nseg + 1, 0);
+        int64_t max_n = 0;
+        for (int g = 0; g < nseg; g++) {
+            axg_mesh *me = meshes[mesh[g]];
+            if (me->n_free > 0) {  // a mesh without free DOFs is an empty segment
+                ensure_block_maps(me);
+                ensure_bsell64(me);
+            }
+            const int64_t rows = 3 * me->nb;
+            const int64_t rpad =
+                std::max<int64_t>(kSegRows, (rows + kSegRows - 1) / kSegRows * kSegRows);
+            max_n = std::max(max_n, me->n_free);
+            row0[g + 1] = row0[g] + rpad;
+            slice0[g + 1] = slice0[g] + rpad / 192;  // 64 block rows per slice
+            slot0[g + 1] = slot0[g] + me->bs_nslots;
+            seg_wg[g + 1] = seg_wg[g] + (int32_t)(rpad / TB);
+        }
+        nrow = row0[nseg];
+        const int64_t nslot = slot0[nseg];
+        const int64_t nslice = slice0[nseg], nsl_wg = (nslice + 7) / 8 * 8;
+        nwg = seg_wg[nseg];
+        AXG_REQUIRE(nrow * K < INT32_MAX && nslot < INT32_MAX, AXG_E_INVALID,
+                    "batch exceeds 2^31 vector entries or slots");
+        sptr.alloc(nsl_wg + 1), wg_row.alloc(nwg), width.alloc(nsl_wg), slice_seg.alloc(nsl_wg);
+        wg_seg.alloc(nwg), segwg.alloc(nseg + 1), segsl.alloc(nseg + 1), seg_live.alloc(nseg);
+        bv.alloc(std::max<int64_t>(10 * nslot, 2)), dsel.alloc(nrow);
+        std::vector<int64_t> h_sptr(nsl_wg + 1, nslot), h_wg_row(nwg);
+        std::vector<int32_t> h_width(nsl_wg, 0), h_slice_seg(nsl_wg, nseg - 1), h_wg_seg(nwg),
+            h_live(nseg, K), h_segsl(nseg + 1);
+        bof.assign(nseg, {});
+        for (int g = 0; g < nseg; g++) {
+            axg_mesh *me = meshes[mesh[g]];
+            const int64_t ns = (me->nb + 63) / 64, nsl = me->bs_nslots;
+            std::vector<int64_t> sp(ns + 1, 0);
+            std::vector<int32_t> wd(ns);
+            if (ns) {
+                AXG_HIP(hipMemcpy(sp.data(), me->bs_sptr.p, 8 * (ns + 1), hipMemcpyDeviceToHost));
+                AXG_HIP(hipMemcpy(wd.data(), me->bs_width.p, 4 * ns, hipMemcpyDeviceToHost));
+            }
+            bof[g].resize(me->n_free);
+            if (me->n_free)
+                AXG_HIP(hipMemcpy(bof[g].data(), me->blk_of_free.p, 4 * me->n_free,
+                                  hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < slice0[g + 1] - slice0[g]; k++) {
+                const int64_t q = slice0[g] + k;
+                h_width[q] = k < ns ? wd[k] : 0;
+                h_sptr[q] = slot0[g] + (k < ns ? sp[k] : sp[ns]);
+                h_slice_seg[q] = g;
+            }
+            h_segsl[g] = (int32_t)slice0[g];
+            for (int32_t wgi = seg_wg[g]; wgi < seg_wg[g + 1]; wgi++) {
+                h_wg_seg[wgi] = g;
+                h_wg_row[wgi] = row0[g] + (int64_t)(wgi - seg_wg[g]) * TB;
+            }
+            // the five value planes side by side, columns shifted to this segment
+            if (nsl) {
+                for (int q = 0; q < 5; q++)
+                    AXG_HIP(hipMemcpyAsync(bv.p + 2 * q * nslot + 2 * slot0[g],
+                                           me->bval64.p + 2 * q * nsl, 16 * nsl,
+                                           hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(rebase_block_cols_kernel, dim3(1024), dim3(256), 0, s, nsl,
+                                   reinterpret_cast<long long *>(bv.p + 8 * nslot + 2 * slot0[g]),
+                                   row0[g] / 3);
+            }
+            const int64_t rpad = row0[g + 1] - row0[g];
+            hipLaunchKernelGGL(block_dsel_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256),
+                               0, s, rpad, 3 * me->nb, me->free_of_blk.p, me->dsel.p,
+                               dsel.p + row0[g]);
+        }
+        h_segsl[nseg] = (int32_t)nslice;
+        seg_end.alloc(nseg + 1);
+        std::vector<int64_t> h_end(nseg + 1, 0);
+        for (int g = 0; g < nseg; g++) h_end[g + 1] = row0[g] + 3 * meshes[mesh[g]]->nb;
+        AXG_HIP(hipMemcpyAsync(sptr.p, h_sptr.data(), 8 * (nsl_wg + 1), hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(width.p, h_width.data(), 4 * nsl_wg, hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(slice_seg.p, h_slice_seg.data(), 4 * nsl_wg, hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(wg_seg.p, h_wg_seg.data(), 4 * nwg, hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(wg_row.p, h_wg_row.data(), 8 * nwg, hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(seg_end.p, h_end.data(), 8 * (nseg + 1), hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(segwg.p, seg_wg.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(segsl.p, h_segsl.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, s));
+        AXG_HIP(hipMemcpyAsync(seg_live.p, h_live.data(), 4 * nseg, hipMemcpyHostToDevice, s));
+        B.binv = nullptr;
+        B.rows_wg = TB;
+        B.dsel = dsel.p;
+        B.wg_seg = wg_seg.p;
+        B.wg_row = wg_row.p;
+        B.seg_row = seg_end.p;
+        B.seg_wg = segwg.p;
+        B.seg_live = seg_live.p;
+        Bmv = B;  // the SpMM's partials: one per slice
+        Bmv.seg_wg = segsl.p;
+        // right-hand sides in block rows: row-major [row][K], zero on prescribed,
+        // padding rows and missing columns
+        std::vector<double> hb((size_t)nrow * K, 0.0);
+        for (int g = 0; g < nseg; g++) {
+            const int m = mesh[g];
+            const int64_t n = meshes[m]->n_free;
+            for (int j = 0; j < std::min(K, (int)nrhs[m]); j++)
+                for (int64_t i = 0; i < n; i++) hb[(row0[g] + bof[g][i]) * K + j] = rhs[m][j * n + i];
+        }
+        Bv.alloc(nrow * K), X.alloc(nrow * K), R.alloc(nrow * K), P.alloc(nrow * K);
+        AP.alloc(nrow * K), Z.alloc(nrow * K);
+        part.alloc((size_t)std::max<int64_t>(nwg * 2 * K, nsl_wg * K) + 8);
+        sc.alloc(nseg * K);
+        live_total.alloc(1);
+        // padding rows are read by the SpMM's p.Ap (times zero rows) and never
+        // written by the vector kernels: they must hold zeros, not stale bits
+        for (double *v : {X.p, R.p, P.p, AP.p, Z.p})
+            hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, nrow * K, v, 0.0);
+        AXG_HIP(hipMemcpyAsync(Bv.p, hb.data(), 8 * hb.size(), hipMemcpyHostToDevice, s));
+        std::vector<SegScalars> hs(nseg * K);
+        std::memset(hs.data(), 0, sizeof(SegScalars) * hs.size());
+        AXG_HIP(hipMemcpyAsync(sc.p, hs.data(), sizeof(SegScalars) * hs.size(),
+                               hipMemcpyHostToDevice, s));
+        const int32_t live0 = nseg * K;
+        AXG_HIP(hipMemcpyAsync(live_total.p, &live0, 4, hipMemcpyHostToDevice, s));
+        max_iter = o.max_iter > 0 ? o.max_iter : std::max<int64_t>(4 * max_n, 200);
+        op = Bsell64Args{nrow / 3, sptr.p, width.p,     bv.p,          nslot,
+                         (int)nsl_wg, AP.p, part.p, seg_live.p, slice_seg.p};
+        // lanes per block row and blocks in flight, timed on this operator
+        L = bsell64_lanes(K)[0];
+        {
+            float best = 1e30f;
+            for (int l : bsell64_lanes(K))
+                for (int cand : {1, 2, 4}) {
+                    bsell64_spmm<K>(op, s, false, cand, l, Bv.p);
+                    AXG_HIP(hipEventRecord(ctx->ev0, s));
+                    for (int r = 0; r < 3; r++) bsell64_spmm<K>(op, s, false, cand, l, Bv.p);
+                    AXG_HIP(hipEventRecord(ctx->ev1, s));
+                    AXG_HIP(hipEventSynchronize(ctx->ev1));
+                    float ms = 0.f;
+                    AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                    if (ms < best) best = ms, U = cand, L = l;
+                }
+        }
+        hipGraph_t graph;
+        AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < kReplay; i++) one_iter();
+        AXG_HIP(hipStreamEndCapture(s, &graph));
+        AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        AXG_HIP(hipGraphDestroy(graph));
+        AXG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        AXG_HIP(hipStreamSynchronize(s));
+    }
+
+    // x = 0, r = b, z = M^-1 r, first direction; then the live count
+    void start(int32_t *h_live) {
+        hipLaunchKernelGGL((batch_init_kernel<K, false>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p,
+                           R.p, Z.p, part.p);
+        hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nseg), dim3(FIN_TB), 0, s, B, part.p,
+                           sc.p, 1, tol, nullptr, max_iter, live_total.p);
+        hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p,
+                           sc.p, 1);
+        AXG_HIP(hipGetLastError());
+        post(h_live);
+        running = true;
+    }
+    void post(int32_t *h_live) {
+        AXG_HIP(hipMemcpyAsync(h_live, live_total.p, 4, hipMemcpyDeviceToHost, s));
+        AXG_HIP(hipEventRecord(ev, s));
+    }
+    // after the last live count has arrived: another replay while any of the
+    // group's right-hand sides iterates; false when the group is finished
+    bool step(int32_t *h_live) {
+        if (!running) return false;
+        AXG_HIP(hipEventSynchronize(ev));
+        if (*h_live <= 0 || launched >= max_iter) {
+            running = false;
+            return false;
+        }
+        AXG_HIP(hipGraphLaunch(gexec, s));
+        launched += kReplay;
+        post(h_live);
+        return true;
+    }
+    static constexpr int kReplay = 32;  // iterations per graph replay
+};
+
+// Many meshes in one call, in G groups of meshes by size, each a
+// block-diagonal system of segments with its own captured iteration sequence.
+// The groups run one after another, smallest meshes first: a small mesh
+// iterates at the pace of its own group's launches instead of the largest
+// meshes' (launches of concurrent streams wait behind the large SpMMs), and
+// the largest group, which sets the total time, starts after the small groups'
+// few milliseconds. G = 1 advances every mesh in one sequence.
 template <int K>
 static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const double *const *rhs,
                       const int32_t *nrhs, const axg_solve_opts &o, double *const *x,
                       int64_t *iters_out, double *ms_out, axg_solve_info *info) {
+    int G = o.batch_groups > 0 ? o.batch_groups : std::max(1, nmesh / kMeshesPerGroup);
+    G = std::max(1, std::min({G, kMaxGroups, nmesh}));
+    std::vector<int> order(nmesh);
+    for (int m = 0; m < nmesh; m++) order[m] = m;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return meshes[a]->n_free < meshes[b]->n_free; });
+    std::vector<std::unique_ptr<BatchGroup<K>>> grp;
+    for (int g = 0; g < G; g++) {
+        auto bg = std::make_unique<BatchGroup<K>>();
+        bg->s = ctx->stream;
+        // meshes in ascending size, split into G runs of (nearly) equal count
+        for (int r = (int)((int64_t)nmesh * g / G); r < (int)((int64_t)nmesh * (g + 1) / G); r++)
+            bg->mesh.push_back(order[r]);
+        bg->setup(ctx, meshes, rhs, nrhs, o);
+        grp.push_back(std::move(bg));
+    }
     hipStream_t s = ctx->stream;
-    std::vector<int64_t> row0(nmesh + 1, 0), slot0(nmesh + 1, 0), slice0(nmesh + 1, 0);
-    std::vector<int32_t> seg_wg(nmesh + 1, 0);
-    int64_t max_n = 0;
-    for (int m = 0; m < nmesh; m++) {
-        axg_mesh *me = meshes[m];
-        if (me->n_free > 0) {  // a mesh without free DOFs is an empty segment
-            ensure_block_maps(me);
-            ensure_bsell64(me);
-        }
-        const int64_t rows = 3 * me->nb;
-        const int64_t rpad = std::max<int64_t>(kSegRows, (rows + kSegRows - 1) / kSegRows * kSegRows);
-        max_n = std::max(max_n, me->n_free);
-        row0[m + 1] = row0[m] + rpad;
-        slice0[m + 1] = slice0[m] + rpad / 192;  // 64 block rows per slice
-        slot0[m + 1] = slot0[m] + me->bs_nslots;
-        seg_wg[m + 1] = seg_wg[m] + (int32_t)(rpad / TB);
-    }
-    const int64_t nrow = row0[nmesh], nslot = slot0[nmesh];
-    const int64_t nslice = slice0[nmesh], nsl_wg = (nslice + 7) / 8 * 8;
-    const int nwg = seg_wg[nmesh];
-    AXG_REQUIRE(nrow * K < INT32_MAX && nslot < INT32_MAX, AXG_E_INVALID,
-                "batch exceeds 2^31 vector entries or slots");
-    DevBuf<int64_t> sptr(nsl_wg + 1), wg_row(nwg);
-    DevBuf<int32_t> width(nsl_wg), slice_seg(nsl_wg), wg_seg(nwg), segwg(nmesh + 1),
-        segsl(nmesh + 1), seg_live(nmesh);
-    DevBuf<double> bv(std::max<int64_t>(10 * nslot, 2)), dsel(nrow);
-    std::vector<int64_t> h_sptr(nsl_wg + 1, nslot), h_wg_row(nwg);
-    std::vector<int32_t> h_width(nsl_wg, 0), h_slice_seg(nsl_wg, nmesh - 1), h_wg_seg(nwg),
-        h_live(nmesh, K), h_segsl(nmesh + 1);
-    std::vector<std::vector<int32_t>> bof(nmesh);
-    for (int m = 0; m < nmesh; m++) {
-        axg_mesh *me = meshes[m];
-        const int64_t ns = (me->nb + 63) / 64, nsl = me->bs_nslots;
-        std::vector<int64_t> sp(ns + 1, 0);
-        std::vector<int32_t> wd(ns);
-        if (ns) {
-            AXG_HIP(hipMemcpy(sp.data(), me->bs_sptr.p, 8 * (ns + 1), hipMemcpyDeviceToHost));
-            AXG_HIP(hipMemcpy(wd.data(), me->bs_width.p, 4 * ns, hipMemcpyDeviceToHost));
-        }
-        bof[m].resize(me->n_free);
-        if (me->n_free)
-            AXG_HIP(hipMemcpy(bof[m].data(), me->blk_of_free.p, 4 * me->n_free,
-                              hipMemcpyDeviceToHost));
-        for (int64_t k = 0; k < slice0[m + 1] - slice0[m]; k++) {
-            const int64_t g = slice0[m] + k;
-            h_width[g] = k < ns ? wd[k] : 0;
-            h_sptr[g] = slot0[m] + (k < ns ? sp[k] : sp[ns]);
-            h_slice_seg[g] = m;
-        }
-        h_segsl[m] = (int32_t)slice0[m];
-        for (int32_t wgi = seg_wg[m]; wgi < seg_wg[m + 1]; wgi++) {
-            h_wg_seg[wgi] = m;
-            h_wg_row[wgi] = row0[m] + (int64_t)(wgi - seg_wg[m]) * TB;
-        }
-        // the five value planes side by side, columns shifted to this segment
-        if (nsl) {
-            for (int q = 0; q < 5; q++)
-                AXG_HIP(hipMemcpyAsync(bv.p + 2 * q * nslot + 2 * slot0[m], me->bval64.p + 2 * q * nsl,
-                                       16 * nsl, hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(rebase_block_cols_kernel, dim3(1024), dim3(256), 0, s, nsl,
-                               reinterpret_cast<long long *>(bv.p + 8 * nslot + 2 * slot0[m]),
-                               row0[m] / 3);
-        }
-        const int64_t rpad = row0[m + 1] - row0[m];
-        hipLaunchKernelGGL(block_dsel_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, s,
-                           rpad, 3 * me->nb, me->free_of_blk.p, me->dsel.p, dsel.p + row0[m]);
-    }
-    h_segsl[nmesh] = (int32_t)nslice;
-    DevBuf<int64_t> seg_end(nmesh + 1);
-    std::vector<int64_t> h_end(nmesh + 1, 0);
-    for (int m = 0; m < nmesh; m++) h_end[m + 1] = row0[m] + 3 * meshes[m]->nb;
-    AXG_HIP(hipMemcpyAsync(sptr.p, h_sptr.data(), 8 * (nsl_wg + 1), hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(width.p, h_width.data(), 4 * nsl_wg, hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(slice_seg.p, h_slice_seg.data(), 4 * nsl_wg, hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(wg_seg.p, h_wg_seg.data(), 4 * nwg, hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(wg_row.p, h_wg_row.data(), 8 * nwg, hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(seg_end.p, h_end.data(), 8 * (nmesh + 1), hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(segwg.p, seg_wg.data(), 4 * (nmesh + 1), hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(segsl.p, h_segsl.data(), 4 * (nmesh + 1), hipMemcpyHostToDevice, s));
-    AXG_HIP(hipMemcpyAsync(seg_live.p, h_live.data(), 4 * nmesh, hipMemcpyHostToDevice, s));
-    BatchDev B;
-    B.binv = nullptr;
-    B.rows_wg = TB;
-    B.dsel = dsel.p;
-    B.wg_seg = wg_seg.p;
-    B.wg_row = wg_row.p;
-    B.seg_row = seg_end.p;
-    B.seg_wg = segwg.p;
-    B.seg_live = seg_live.p;
-    BatchDev Bmv = B;  // the SpMM's partials: one per slice
-    Bmv.seg_wg = segsl.p;
-    // right-hand sides in block rows: row-major [row][K], zero on prescribed,
-    // padding rows and missing columns
-    std::vector<double> hb((size_t)nrow * K, 0.0);
-    for (int m = 0; m < nmesh; m++) {
-        const int64_t n = meshes[m]->n_free;
-        for (int j = 0; j < std::min(K, (int)nrhs[m]); j++)
-            for (int64_t i = 0; i < n; i++) hb[(row0[m] + bof[m][i]) * K + j] = rhs[m][j * n + i];
-    }
-    DevBuf<double> Bv(nrow * K), X(nrow * K), R(nrow * K), P(nrow * K), AP(nrow * K),
-        Z(nrow * K), part((size_t)std::max<int64_t>(nwg * 2 * K, nsl_wg * K) + 8);
-    DevBuf<SegScalars> sc(nmesh * K);
-    DevBuf<int32_t> live_total(1);
-    // padding rows are read by the SpMM's p.Ap (times zero rows) and never
-    // written by the vector kernels: they must hold zeros, not stale bits
-    for (double *v : {X.p, R.p, P.p, AP.p, Z.p})
-        hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, nrow * K, v, 0.0);
-    AXG_HIP(hipMemcpyAsync(Bv.p, hb.data(), 8 * hb.size(), hipMemcpyHostToDevice, s));
-    std::vector<SegScalars> hs(nmesh * K);
-    std::memset(hs.data(), 0, sizeof(SegScalars) * hs.size());
-    AXG_HIP(hipMemcpyAsync(sc.p, hs.data(), sizeof(SegScalars) * hs.size(), hipMemcpyHostToDevice,
-                           s));
-    int32_t live0 = nmesh * K;
-    AXG_HIP(hipMemcpyAsync(live_total.p, &live0, 4, hipMemcpyHostToDevice, s));
-    const int64_t max_iter = o.max_iter > 0 ? o.max_iter : std::max<int64_t>(4 * max_n, 200);
-    const Bsell64Args op{nrow / 3,    sptr.p, width.p, bv.p,       nslot,
-                         (int)nsl_wg, AP.p,   part.p,  seg_live.p, slice_seg.p};
-    // lanes per block row and blocks in flight, timed on this operator
-    int U = 2, L = bsell64_lanes(K)[0];
-    {
-        float best = 1e30f;
-        for (int l : bsell64_lanes(K))
-            for (int cand : {1, 2, 4}) {
-                bsell64_spmm<K>(op, s, false, cand, l, Bv.p);
-                AXG_HIP(hipEventRecord(ctx->ev0, s));
-                for (int r = 0; r < 3; r++) bsell64_spmm<K>(op, s, false, cand, l, Bv.p);
-                AXG_HIP(hipEventRecord(ctx->ev1, s));
-                AXG_HIP(hipEventSynchronize(ctx->ev1));
-                float ms = 0.f;
-                AXG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-                if (ms < best) best = ms, U = cand, L = l;
-            }
-    }
-    auto one_iter = [&]() {
-        bsell64_spmm<K>(op, s, true, U, L, P.p);
-        hipLaunchKernelGGL((batch_fin_pap_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, Bmv, part.p,
-                           sc.p, live_total.p);
-        hipLaunchKernelGGL((batch_update_kernel<K, false>), dim3(nwg), dim3(TB), 0, s, B, X.p, R.p, P.p,
-                           AP.p, Z.p, part.p, sc.p);
-        hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p,
-                           sc.p, 0, o.tol, nullptr, max_iter, live_total.p);
-        hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p,
-                           0);
-    };
-    const int batch = 32;
-    hipGraph_t graph;
-    hipGraphExec_t gexec;
-    AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < batch; i++) one_iter();
-    AXG_HIP(hipStreamEndCapture(s, &graph));
-    AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    AXG_HIP(hipGraphDestroy(graph));
-
-    AXG_HIP(hipStreamSynchronize(s));
+    ctx->pinned.ensure(sizeof(int32_t));
+    int32_t *h_live = (int32_t *)ctx->pinned.p;
+    *h_live = 0;
     DevBuf<int64_t> t0(1);
     AXG_HIP(hipEventRecord(ctx->ev0, s));
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, t0.p);  // start of the solves
-    hipLaunchKernelGGL((batch_init_kernel<K, false>), dim3(nwg), dim3(TB), 0, s, B, Bv.p, X.p, R.p, Z.p,
-                       part.p);
-    hipLaunchKernelGGL((batch_fin_update_kernel<K>), dim3(nmesh), dim3(FIN_TB), 0, s, B, part.p, sc.p,
-                       1, o.tol, nullptr, max_iter, live_total.p);
-    hipLaunchKernelGGL((batch_update_p_kernel<K>), dim3(nwg), dim3(TB), 0, s, B, P.p, Z.p, sc.p, 1);
-    AXG_HIP(hipGetLastError());
-    int32_t live = live0;
-    int64_t launched = 0;
-    for (;;) {
-        AXG_HIP(hipMemcpyAsync(&live, live_total.p, 4, hipMemcpyDeviceToHost, s));
-        AXG_HIP(hipStreamSynchronize(s));
-        if (live <= 0 || launched >= max_iter) break;
-        AXG_HIP(hipGraphLaunch(gexec, s));
-        launched += batch;
+    for (int g = 0; g < G; g++) {
+        grp[g]->start(h_live);
+        while (grp[g]->step(h_live)) {
+        }
     }
     AXG_HIP(hipEventRecord(ctx->ev1, s));
     AXG_HIP(hipEventSynchronize(ctx->ev1));
-    AXG_HIP(hipGraphExecDestroy(gexec));
     float total_ms = 0.f;
     AXG_HIP(hipEventElapsedTime(&total_ms, ctx->ev0, ctx->ev1));
-    // one launch of the concatenated operator, timed on the solve's buffers
-    // (every segment active) right after the solve
-    float op_ms = 0.f;
-    {
-        const int reps = 10;
-        AXG_HIP(hipEventRecord(ctx->ev0, s));
-        for (int r = 0; r < reps; r++) bsell64_spmm<K>(op, s, false, U, L, P.p);
-        AXG_HIP(hipEventRecord(ctx->ev1, s));
-        AXG_HIP(hipEventSynchronize(ctx->ev1));
-        AXG_HIP(hipEventElapsedTime(&op_ms, ctx->ev0, ctx->ev1));
-        op_ms /= reps;
-    }
     int64_t h_t0 = 0;
     AXG_HIP(hipMemcpy(&h_t0, t0.p, 8, hipMemcpyDeviceToHost));
-    AXG_HIP(hipMemcpy(hs.data(), sc.p, sizeof(SegScalars) * hs.size(), hipMemcpyDeviceToHost));
-    std::vector<double> hx((size_t)nrow * K);
-    AXG_HIP(hipMemcpy(hx.data(), X.p, 8 * hx.size(), hipMemcpyDeviceToHost));
+    // per-solve results in the caller's order
+    std::vector<int64_t> qbase(nmesh + 1, 0);
+    for (int m = 0; m < nmesh; m++) qbase[m + 1] = qbase[m] + nrhs[m];
     int status = AXG_OK;
-    int64_t total_iters = 0, q = 0, most_iters = 0;
-    double worst = 0.0, worst_b = 1.0;
-    for (int m = 0; m < nmesh; m++) {
-        const int64_t n = meshes[m]->n_free;
-        for (int j = 0; j < nrhs[m]; j++, q++) {
-            if (j >= K) {  // more right-hand sides than this call handles
-                status = AXG_E_INVALID;
-                continue;
+    int64_t total_iters = 0, launches = 0;
+    double worst = 0.0, worst_b = 1.0, spmv_ms = 0.0;
+    for (int g = 0; g < G; g++) {
+        BatchGroup<K> &bg = *grp[g];
+        std::vector<SegScalars> hs(bg.nseg * K);
+        AXG_HIP(hipMemcpy(hs.data(), bg.sc.p, sizeof(SegScalars) * hs.size(), hipMemcpyDeviceToHost));
+        std::vector<double> hx((size_t)bg.nrow * K);
+        AXG_HIP(hipMemcpy(hx.data(), bg.X.p, 8 * hx.size(), hipMemcpyDeviceToHost));
+        int64_t most_iters = 0;
+        for (int sg = 0; sg < bg.nseg; sg++) {
+            const int m = bg.mesh[sg];
+            const int64_t n = meshes[m]->n_free;
+            for (int j = 0; j < nrhs[m]; j++) {
+                const int64_t q = qbase[m] + j;
+                if (j >= K) {  // more right-hand sides than this call handles
+                    status = AXG_E_INVALID;
+                    continue;
+                }
+                const SegScalars &r = hs[sg * K + j];
+                for (int64_t i = 0; i < n; i++) x[m][j * n + i] = hx[(bg.row0[sg] + bg.bof[sg][i]) * K + j];
+                if (iters_out) iters_out[q] = r.iters;
+                // s_memrealtime runs at 100 MHz
+                if (ms_out) ms_out[q] = r.done ? (double)(r.t_done - h_t0) / 1e5 : (double)total_ms;
+                total_iters += r.iters;
+                most_iters = std::max<int64_t>(most_iters, r.iters);
+                if (!r.done) status = AXG_E_NOT_CONVERGED;
+                else if (r.status != AXG_OK && status == AXG_OK) status = r.status;
+                if (r.bnorm > 0 && r.res / r.bnorm > worst / worst_b) worst = r.res, worst_b = r.bnorm;
             }
-            const SegScalars &r = hs[m * K + j];
-            for (int64_t i = 0; i < n; i++) x[m][j * n + i] = hx[(row0[m] + bof[m][i]) * K + j];
-            if (iters_out) iters_out[q] = r.iters;
-            // s_memrealtime runs at 100 MHz
-            if (ms_out) ms_out[q] = r.done ? (double)(r.t_done - h_t0) / 1e5 : (double)total_ms;
-            total_iters += r.iters;
-            most_iters = std::max<int64_t>(most_iters, r.iters);
-            if (!r.done) status = AXG_E_NOT_CONVERGED;
-            else if (r.status != AXG_OK && status == AXG_OK) status = r.status;
-            if (r.bnorm > 0 && r.res / r.bnorm > worst / worst_b) worst = r.res, worst_b = r.bnorm;
         }
+        // one launch of the group's operator, timed on the solve's buffers (every
+        // segment active) right after the solve
+        const int reps = 10;
+        AXG_HIP(hipEventRecord(ctx->ev0, bg.s));
+        for (int r = 0; r < reps; r++) bsell64_spmm<K>(bg.op, bg.s, false, bg.U, bg.L, bg.P.p);
+        AXG_HIP(hipEventRecord(ctx->ev1, bg.s));
+        AXG_HIP(hipEventSynchronize(ctx->ev1));
+        float op_ms = 0.f;
+        AXG_HIP(hipEventElapsedTime(&op_ms, ctx->ev0, ctx->ev1));
+        spmv_ms += (double)op_ms / reps * (double)most_iters;
+        launches += most_iters;
     }
     if (info) {
         std::memset(info, 0, sizeof(*info));
@@ -1034,11 +1132,11 @@ static void run_batch(axg_ctx *ctx, int nmesh, axg_mesh *const *meshes, const do
         info->residual = worst;
         info->rel_residual = worst / worst_b;
         info->solve_ms = total_ms;
-        // SpMM launches that did work (the longest solve) at the timed launch cost
-        info->spmv_launches = most_iters;
-        info->spmv_ms = (double)op_ms * (double)most_iters;
+        // SpMM launches that did work (each group's longest solve) at the timed launch cost
+        info->spmv_launches = launches;
+        info->spmv_ms = spmv_ms;
         std::snprintf(info->name, sizeof(info->name),
-                      "GPU batched PCG (FP64 node blocks, Jacobi, %d meshes)", nmesh);
+                      "GPU batched PCG (FP64 node blocks, Jacobi, %d meshes, %d groups)", nmesh, G);
     }
     if (status != AXG_OK)
         throw Error{status, status == AXG_E_NOT_SPD

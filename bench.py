@@ -605,11 +605,22 @@ def run_batch(a, G, MG, torch, np):
     refs = [m.solve(b, precision="fp64", precond="block3", tol=1e-13)[0] for m, b in
             zip(meshes, rhs)]
     for _ in range(a.warmup):
-        G.solve_batch(meshes, rhs, tol=tol)
+        G.solve_batch(meshes, rhs, tol=tol, groups=a.batch_groups)
     t1 = time.perf_counter()
-    res = [G.solve_batch(meshes, rhs, tol=tol) for _ in range(a.steps)]
+    res = [G.solve_batch(meshes, rhs, tol=tol, groups=a.batch_groups) for _ in range(a.steps)]
     ms_per_step = 1e3 * (time.perf_counter() - t1) / a.steps
     xs, iters, lat, info = res[-1]
+    # every mesh in one launch sequence (lockstep), for comparison and for the roofline of
+    # the SpMM over the whole concatenated operator
+    ls = [G.solve_batch(meshes, rhs, tol=tol, groups=1) for _ in range(max(1, a.steps // 2))]
+    ls_lat = np.concatenate([r[2] for r in ls])
+    ls_info = ls[-1][3]
+    lockstep = {"solve_ms": float(np.mean([r[3]["solve_ms"] for r in ls])),
+                "latency_p50_ms": float(np.percentile(ls_lat, 50)),
+                "latency_p99_ms": float(np.percentile(ls_lat, 99))}
+    log(f"batch: {info['name']}: {np.mean([r[3]['solve_ms'] for r in res]):.0f} ms, p50 "
+        f"{np.percentile(np.concatenate([r[2] for r in res]), 50):.0f} ms; lockstep "
+        f"{lockstep['solve_ms']:.0f} ms, p50 {lockstep['latency_p50_ms']:.0f} ms")
     err = max(float(np.linalg.norm(X[k] - R[k]) / np.linalg.norm(R[k]))
               for X, R in zip(xs, refs) for k in range(X.shape[0]))
     dof_iters = float(sum(ns[m] * iters[4 * m + k] for m in range(48) for k in range(4)))
@@ -637,10 +648,12 @@ def run_batch(a, G, MG, torch, np):
                   "latency_max_ms": float(lat_all.max()),
                   "iters_min": int(iters.min()), "iters_max": int(iters.max()),
                   "rel_err_vs_fp64_ref_max": err, "rel_residual_max": info["rel_residual"]},
+        "solver": info["name"],
+        "lockstep": lockstep,
         "sequential": {"solve_ms_sum": seq_ms, "wall_ms": seq_wall,
                        "speedup_batch_vs_sequential": seq_ms / solve_ms},
-        "roofline": roofline("bsell64_spmm_kernel<K=4> (FP64 concatenated node-block SpMM)",
-                             nbytes, avg_launch(info)),
+        "roofline": roofline("bsell64_spmm_kernel<K=4> (FP64 concatenated node-block SpMM, "
+                             "every mesh in one launch)", nbytes, avg_launch(ls_info)),
     })
     if not a.no_cpu:
         from oracle import oracle as O
@@ -1077,6 +1090,8 @@ def main():
     ap.add_argument("--precond", default="block3")
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--refinement", default="fcg", choices=("fcg", "plain", "reliable"))
+    ap.add_argument("--batch-groups", type=int, default=0,
+                    help="batch workload: size groups solved smallest first (0: library default)")
     ap.add_argument("--asm32-mode", type=int, default=-1, choices=(-1, 0, 1),
                     help="FP32 assembly form: 0 fused rows, 1 MFMA element blocks + gather, "
                          "-1 the library's default for the element type")

@@ -71,7 +71,9 @@ typedef struct axg_solve_opts {
     int32_t max_outer;   /* mixed: refinement sweeps / inner solves (default 30)     */
     int32_t timing;      /* 1: record per-kernel HIP events (SpMV durations in info) */
     int32_t refinement;  /* mixed: enum axg_refinement (default FCG)                 */
-    int32_t reserved;    /* 0                                                        */
+    int32_t batch_groups;/* axg_solve_batch: G > 1 groups of meshes by size, solved
+                            one after another, smallest first (<= 16); 1 every mesh
+                            in one launch sequence; 0 one group per 6 meshes      */
 } axg_solve_opts;
 
 typedef struct axg_solve_info {

@@ -167,17 +167,20 @@ def test_jittered_c3d20_full_properties(ctx):
 
 
 # ------------------------------------------------------------------ batched cubes
-def test_batch_small_vs_oracle(ctx):
+@pytest.mark.parametrize("groups", [1, 3])
+def test_batch_small_vs_oracle(ctx, groups):
     """Six uneven cubes (4^3 .. 8^3), four Gaussian top-face load cases each, one batched
-    solve; every (mesh, load case) against the oracle's direct solve."""
-    say("case: batch of 6 cubes vs oracle")
+    solve (one launch sequence, or three size groups solved smallest first); every (mesh,
+    load case) against the oracle's direct solve."""
+    say("case: batch of 6 cubes vs oracle, groups", groups)
     probs = MG.batch_meshes(count=6, lo=4, hi=8, nloads=4, seed=21)
     meshes = []
     for p in probs:
         m = G.Mesh(8, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
         m.assemble("fp64")
         meshes.append(m)
-    xs, iters, lat, info = G.solve_batch(meshes, [p.rhs_all() for p in probs], tol=1e-12)
+    xs, iters, lat, info = G.solve_batch(meshes, [p.rhs_all() for p in probs], tol=1e-12,
+                                         groups=groups)
     say(f"  {info['name']}: {len(iters)} solves, {info['solve_ms']:.1f} ms")
     for p, m, X in zip(probs, meshes, xs):
         check_against_oracle(m, p, oracle_cases(p), X, p.name)

@@ -496,10 +496,12 @@ def test_mixed_after_fp64_multi_rhs_on_fresh_context():
 
 
 # ------------------------------------------------------------------ batched
-def test_batched_multi_mesh_matches_single_solves(ctx):
-    """Uneven meshes, several load cases each, one batched launch sequence; every
-    solve matches the single-mesh FP64 PCG and the reference's stopping rule."""
-    say("case: batch")
+@pytest.mark.parametrize("groups", [1, 2, 4])
+def test_batched_multi_mesh_matches_single_solves(ctx, groups):
+    """Uneven meshes, several load cases each, in one batched call (one launch sequence, or
+    size groups solved smallest first); every solve matches the single-mesh FP64 PCG and
+    the reference's stopping rule, in the caller's order."""
+    say("case: batch, groups", groups)
     probs = [MG.batch_meshes(count=1, lo=m, hi=m, nloads=k, seed=m)[0]
              for m, k in ((4, 4), (7, 2), (5, 3), (9, 1))]
     meshes = []
@@ -508,7 +510,8 @@ def test_batched_multi_mesh_matches_single_solves(ctx):
         me.assemble("fp64")
         meshes.append(me)
     rhs = [p.rhs_all() for p in probs]
-    xs, iters, lat, info = G.solve_batch(meshes, rhs, tol=1e-12)
+    xs, iters, lat, info = G.solve_batch(meshes, rhs, tol=1e-12, groups=groups)
+    assert f"{groups} groups" in info["name"]
     q = 0
     for p, me, B, X in zip(probs, meshes, rhs, xs):
         for k in range(B.shape[0]):
