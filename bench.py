@@ -24,9 +24,10 @@ c3d20  C3D20 brick 96x24x24 (55,296 elements, 237,025 nodes, 705,600 free
     DOFs), jittered interior corners (+-0.15 h), x = 0 clamped, tip shear;
     FP64 vs FP32+FP64-refinement integration and solve, Jacobi vs 3x3
     block-Jacobi.
-batch  48 C3D8 cubes m^3, m ~ U{8..48}, 4 top-face load cases each, all in
-    one batched graph-replayed PCG; p50/p99 per-solve latency and aggregate
-    throughput, against the meshes solved one after another.
+batch  48 C3D8 cubes m^3, m ~ U{8..48}, 4 top-face load cases each, in one
+    batched call (size groups of graph-replayed PCG, smallest first); p50/p99
+    per-solve latency and aggregate throughput, against every mesh in one
+    launch sequence and against the meshes solved one after another.
 simp  C3D8 brick 120x60x60 (432,000 elements), SIMP modulus, point load;
     compliance, dc/drho through the adjoint kernel, central differences
     (step 1e-4) on 64 random elements, and the FP64 CPU adjoint of the oracle.
@@ -41,6 +42,10 @@ nlgeom  *STEP, NLGEOM: a C3D8 cantilever 200x20x20 (80,000 elements, 264,600
     reference's continuum Newton driver with the St. Venant-Kirchhoff tangent
     re-assembled on the device at every iteration and block-Jacobi PCG solves.
     One step = one whole analysis.
+
+The default (brick) line also carries `other_configs`: the cantilever, C3D20,
+batch and SIMP workloads run once each after a warm-up, without their CPU legs
+(--no-configs skips them).
 
 Inputs are resident in HBM when a timed region starts.
 """
@@ -1098,6 +1103,8 @@ def main():
     ap.add_argument("--fd-elements", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="brick: skip the summaries of the other benchmark configurations")
     ap.add_argument("--kernels", action="store_true",
                     help="only the operator kernels (repeated launches), for profiling")
     ap.add_argument("--reps", type=int, default=20)
@@ -1126,8 +1133,33 @@ def main():
            "batch": run_batch, "simp": run_simp, "freq": run_freq,
            "nlgeom": run_nlgeom, "deck": run_deck, "buckle": run_buckle}[a.workload]
     out = run(a, G, MG, torch, np)
+    if out is not None and a.workload == "brick" and not a.kernels and not a.no_configs:
+        out["other_configs"] = other_configs(a, G, MG, torch, np)
     if out is not None:
         print(json.dumps(out))
+
+
+def other_configs(a, G, MG, torch, np):
+    """The other benchmark configurations, each once after a warm-up (no CPU legs, 8
+    finite-difference elements for the adjoint), summarised beside the headline line:
+    the cantilever (FP64 vs the oracle's Cholesky is the parity test; here the GPU solve),
+    the jittered C3D20 brick, the 48-mesh batch and the SIMP adjoint."""
+    keep_out = {"metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling",
+                "vs_baseline", "data"}
+    res = {}
+    for name, fn in (("cantilever_20x9x9", run_cantilever), ("c3d20_96x24x24", run_c3d20),
+                     ("batch48", run_batch), ("simp_120x60x60", run_simp)):
+        b = argparse.Namespace(**vars(a))
+        b.steps, b.warmup, b.no_cpu, b.fd_elements, b.reps = 1, 1, True, 8, 3
+        t0 = time.perf_counter()
+        try:
+            o = fn(b, G, MG, torch, np)
+            res[name] = {k: v for k, v in o.items() if k not in keep_out}
+        except Exception as e:  # a failed companion must not hide the headline line
+            res[name] = {"error": f"{type(e).__name__}: {e}"}
+        res[name]["wall_s"] = time.perf_counter() - t0
+        log(f"{name}: {res[name]['wall_s']:.1f} s")
+    return res
 
 
 if __name__ == "__main__":

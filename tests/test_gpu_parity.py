@@ -448,6 +448,29 @@ def test_mixed_every_spmm_shape():
     c.close()
 
 
+@pytest.mark.parametrize("shape", [13, 15])
+def test_mixed_lds_shapes_wide_groups(ctx, shape):
+    """C3D20 rows couple up to 81 nodes: on a 24 x 24 x 2 brick a group of 256 block rows
+    has up to 2,080 distinct column blocks, more than the LDS-staged SpMM stages (1,360), so
+    the rest come from global memory (the column rides in the third plane); mixed solves
+    with LDS shapes pinned match the FP64 solve."""
+    say("case: LDS shapes, wide groups", shape)
+    xyz, conn = MG.brick_hex20(24, 24, 2)
+    rng = np.random.default_rng(shape)
+    ind = MG.free_map(len(xyz), np.nonzero(xyz[:, 2] == 0.0)[0])
+    mesh = G.Mesh(20, xyz, conn, rng.uniform(5e4, 2.5e5, len(conn)), 0.3, ind, ctx=ctx)
+    mesh.assemble("mixed")
+    n = mesh.n_free
+    B = rng.standard_normal((4, n))
+    ref, _ = mesh.solve(B, precision="fp64", tol=1e-13)
+    assert mesh.spmm_shape(4, shape) == shape
+    X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-11)
+    err = max(rel(X[k], ref[k]) for k in range(4))
+    say(f"  n={n}, inner {info['iters']}, err {err:.2e}")
+    assert err < 1e-8
+    mesh.close()
+
+
 def test_mixed_c3d20_stiff_beam_floor():
     """A slender jittered C3D20 beam: its FP64 residual has a round-off floor near 1e-10
     relative. The mixed solve reaches 1e-9 (displacements within 1e-8 of FP64), and an
