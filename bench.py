@@ -123,19 +123,34 @@ def host_cpu_model():
     return "unknown"
 
 
-def cpu_baseline_brick(indptr, indices, data, b, prob, layers=8, iters_c=60, iters_1=20):
+def cpu_baseline_brick(indptr, indices, data, b, prob, layers=8, tol=1e-13, iters_1=20):
     """The oracle (C restatement of the reference's CPU path) on this host, on the benchmarked
     brick itself: linalg::pcg on the exported K_ff (bit-equal to the oracle's own assembly of
-    it) for a bounded number of iterations, multi-threaded and single-threaded, and the
-    reference's single-threaded element loop + triplets + csr_from_triplets on the bottom
-    `layers` element layers of the same brick (same elements, same moduli)."""
+    it) solving load case 0 to ||r|| <= tol ||b|| on all host threads (the rate and the time to
+    solution; the solution is the oracle's answer the GPU results are checked against), the
+    same PCG single-threaded for a bounded number of iterations, and the reference's
+    single-threaded element loop + triplets + csr_from_triplets on the bottom `layers` element
+    layers of the same brick (same elements, same moduli). Returns (summary, oracle x)."""
+    import numpy as np
     from axia_gpu import meshgen as MG
     from oracle import oracle as O
 
     cores = host_threads()
     n = len(indptr) - 1
     csr = O.Csr(n, indptr, indices, data)
-    rc = _oracle_pcg_rate(O, csr, b, iters_c, cores, min_s=3.0)
+    L = O.lib()
+    xo = np.zeros(n)
+    it = np.zeros(1, np.int64)
+    res = np.zeros(1)
+    L.or_set_threads(cores)
+    t0 = time.perf_counter()
+    st = L.or_pcg_omp(n, O._l(csr.indptr), O._l(csr.indices), O._d(csr.data), O._d(b), tol,
+                      max(4 * n, 200), 0, O._d(xo), O._l(it), O._d(res))
+    t_solve = time.perf_counter() - t0
+    L.or_set_threads(0)
+    if st != 0:
+        raise RuntimeError(f"oracle PCG failed (status {st})")
+    rc = n * int(it[0]) / t_solve
     r1 = _oracle_pcg_rate(O, csr, b, iters_1, 1, min_s=3.0)
     nx = round(prob.nelem ** (1.0 / 3.0))
     xyz, conn = MG.brick_hex8(nx, nx, layers)
@@ -148,13 +163,15 @@ def cpu_baseline_brick(indptr, indices, data, b, prob, layers=8, iters_c=60, ite
     return {
         "value": rc, "unit": "DOF·iter/s", "cores": cores, "kind": "port",
         "sample": (f"oracle linalg::pcg restatement (Jacobi, FP64 CSR, OpenMP) on the "
-                   f"benchmarked K_ff itself (n={n}, nnz={csr.nnz}), runs of {iters_c} "
-                   f"iterations for >= 3 s; single-thread {iters_1}-iteration runs; assembly on "
-                   f"the brick's bottom {layers} element layers ({ne} elements), single-thread"),
+                   f"benchmarked K_ff itself (n={n}, nnz={csr.nnz}), load case 0 solved to "
+                   f"{tol:g}; single-thread {iters_1}-iteration runs; assembly on the brick's "
+                   f"bottom {layers} element layers ({ne} elements), single-thread"),
         "cpu_model": host_cpu_model(),
+        "oracle_solve_s": t_solve, "oracle_iters": int(it[0]),
+        "oracle_rel_residual": float(res[0]) / float(np.linalg.norm(b)),
         "pcg_dof_iter_per_s_1thread": r1,
         "assembly_elements_per_s_1thread": asm_eps,
-    }
+    }, xo
 
 
 # ------------------------------------------------------------------ helpers
@@ -429,7 +446,21 @@ def run_brick(a, G, MG, torch, np):
         log("cpu baseline ...")
         if indptr is None:
             indptr, indices, data = mesh.csr()
-        out["cpu_baseline"] = cpu_baseline_brick(indptr, indices, data, prob.rhs(0), prob)
+        out["cpu_baseline"], xo = cpu_baseline_brick(indptr, indices, data, prob.rhs(0), prob)
+
+        def err_vs_oracle(v):
+            return float(np.linalg.norm(v - xo) / np.linalg.norm(xo))
+
+        # load case 0 of every GPU solution against the oracle's own solve of it
+        out["mixed"]["rel_err_vs_oracle_case0"] = err_vs_oracle(X[0].cpu().numpy())
+        if plain:
+            out["fp64_plain"]["rel_err_vs_oracle_case0"] = err_vs_oracle(Xp[0])
+            out["fp64_best"]["rel_err_vs_oracle_case0"] = err_vs_oracle(X64[0].cpu().numpy())
+        out["cpu_baseline"]["time_to_solution_vs_mixed_per_rhs"] = (
+            1e3 * out["cpu_baseline"]["oracle_solve_s"] / out["mixed"]["time_to_solution_per_rhs_ms"])
+        log(f"oracle: {out['cpu_baseline']['oracle_iters']} iterations "
+            f"{out['cpu_baseline']['oracle_solve_s']:.1f} s; mixed vs oracle "
+            f"{out['mixed']['rel_err_vs_oracle_case0']:.2e}")
     return out
 
 
