@@ -364,9 +364,12 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 // a row is five coalesced 16-byte loads (entries 0-7 and entry 8 with the
 // column's bits; the LPR lanes of a row load the same addresses); the 3 KL
 // values of the column block's x that a lane needs are gathered once and
-// shared by the block's three rows. U blocks are loaded before their gathers.
-// With DOT the p.Ap partial sums of the workgroup go to part.
-template <int K, int LPR, int U, bool DOT>
+// shared by the block's three rows. U blocks are loaded before their gathers;
+// with GF a batch's x gathers are issued before the next batch's values (loads
+// return in issue order, so the multiply waits for the gathers only, and the
+// next values arrive meanwhile). With DOT the p.Ap partial sums of the
+// workgroup go to part.
+template <int K, int LPR, int U, bool DOT, bool GF = false>
 __global__ void __launch_bounds__(64 * LPR) bsell64_spmm_kernel(
     int64_t nb, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
     const double *__restrict__ bv, int64_t nsl, const double *__restrict__ P,
@@ -398,8 +401,9 @@ __global__ void __launch_bounds__(64 * LPR) bsell64_spmm_kernel(
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int k = 0; k < KL; k++) acc[d][k] = 0.0;
-        for (int j0 = 0; j0 < w; j0 += U) {
-            d2v a[U][5];
+        // the U blocks from block j0 (clamped to the row's last block), their
+        // column blocks' x, and their products
+        auto load_vals = [&](int j0, d2v (&a)[U][5]) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t pos = base + 64 * (int64_t)(j0 + u < w ? j0 + u : j0);
@@ -409,12 +413,12 @@ __global__ void __launch_bounds__(64 * LPR) bsell64_spmm_kernel(
                 a[u][3] = __builtin_nontemporal_load(&q3[pos]);
                 a[u][4] = __builtin_nontemporal_load(&q4[pos]);
             }
+        };
+        auto gather = [&](const d2v (&a)[U][5], double (&x)[U][3][KL]) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (j0 + u >= w) break;
                 const int64_t J = (int64_t)__double_as_longlong(a[u][4].y) & 0xffffffffll;
                 const double *xb = P + J * 3 * K + h * KL;
-                double x[3][KL];
 #pragma unroll
                 for (int e = 0; e < 3; e++) {
                     if constexpr (KL % 2 == 0) {  // KL doubles at a 16-byte aligned offset
@@ -422,21 +426,56 @@ __global__ void __launch_bounds__(64 * LPR) bsell64_spmm_kernel(
 #pragma unroll
                         for (int t = 0; t < KL / 2; t++) {
                             const d2v v2 = xp[t];
-                            x[e][2 * t] = v2.x;
-                            x[e][2 * t + 1] = v2.y;
+                            x[u][e][2 * t] = v2.x;
+                            x[u][e][2 * t + 1] = v2.y;
                         }
                     } else {
 #pragma unroll
-                        for (int t = 0; t < KL; t++) x[e][t] = xb[e * K + t];
+                        for (int t = 0; t < KL; t++) x[u][e][t] = xb[e * K + t];
                     }
                 }
+            }
+        };
+        auto fma = [&](int j0, const d2v (&a)[U][5], const double (&x)[U][3][KL]) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (j0 + u >= w) break;
                 const double v[9] = {a[u][0].x, a[u][0].y, a[u][1].x, a[u][1].y, a[u][2].x,
                                      a[u][2].y, a[u][3].x, a[u][3].y, a[u][4].x};
 #pragma unroll
                 for (int d = 0; d < 3; d++)
 #pragma unroll
                     for (int k = 0; k < KL; k++)
-                        acc[d][k] += v[3 * d] * x[0][k] + v[3 * d + 1] * x[1][k] + v[3 * d + 2] * x[2][k];
+                        acc[d][k] += v[3 * d] * x[u][0][k] + v[3 * d + 1] * x[u][1][k] +
+                                     v[3 * d + 2] * x[u][2][k];
+            }
+        };
+        if constexpr (GF) {
+            d2v a[U][5];
+            if (w > 0) load_vals(0, a);
+            for (int j0 = 0; j0 < w; j0 += U) {
+                double x[U][3][KL];
+                gather(a, x);
+                __builtin_amdgcn_sched_barrier(0);
+                d2v an[U][5];
+                const bool more = j0 + U < w;
+                if (more) load_vals(j0 + U, an);
+                __builtin_amdgcn_sched_barrier(0);
+                fma(j0, a, x);
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int q = 0; q < 5; q++) a[u][q] = an[u][q];
+                }
+            }
+        } else {
+            for (int j0 = 0; j0 < w; j0 += U) {
+                d2v a[U][5];
+                load_vals(j0, a);
+                double x[U][3][KL];
+                gather(a, x);
+                fma(j0, a, x);
             }
         }
 #pragma unroll
@@ -493,6 +532,11 @@ struct BlockPcg {
 
 static constexpr int kGraphIters = 32;
 
+// The form code of the FP64 block SpMM: blocks per batch (1, 2, 4), plus
+// kBsellGF for the gathers-first form; the tuners try every one.
+constexpr int kBsellGF = 10;
+static const int kBsellForms[] = {1, 2, 4, 1 + kBsellGF, 2 + kBsellGF, 4 + kBsellGF};
+
 // One launch of the FP64 block SpMM over nsl_wg slices (a multiple of 8).
 // With events, the launch itself stamps e0 / e1 (the SpMM's device time).
 struct Bsell64Args {
@@ -510,28 +554,31 @@ template <int K, int LPR>
 static void launch_bsell64(const Bsell64Args &a, hipStream_t s, bool dot, int u, const double *x,
                            hipEvent_t e0, hipEvent_t e1) {
     const dim3 g(a.nsl_wg), b(64 * LPR);
-#define AXG_BSELL64(UU, DD)                                                                         \
-    if (e0)                                                                                         \
-        hipExtLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, e0, e1, 0, a.nb,  \
-                              a.sptr, a.width, a.bv, a.nsl, x, a.ap, a.part, a.live, a.slice_seg); \
-    else                                                                                            \
-        hipLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD>), g, b, 0, s, a.nb, a.sptr,        \
+#define AXG_BSELL64(UU, DD, GG)                                                                     \
+    if (e0)                                                                                             \
+        hipExtLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD, GG>), g, b, 0, s, e0, e1, 0, a.nb,  \
+                              a.sptr, a.width, a.bv, a.nsl, x, a.ap, a.part, a.live, a.slice_seg);     \
+    else                                                                                                \
+        hipLaunchKernelGGL((bsell64_spmm_kernel<K, LPR, UU, DD, GG>), g, b, 0, s, a.nb, a.sptr,        \
                            a.width, a.bv, a.nsl, x, a.ap, a.part, a.live, a.slice_seg)
+    // u: blocks per batch (1, 2, 4), plus kBsellGF for the gathers-first form
+    const bool gf = u >= kBsellGF;
+    const int uu = gf ? u - kBsellGF : u;
     if (dot) {
-        if (u == 1) {
-            AXG_BSELL64(1, true);
-        } else if (u == 2) {
-            AXG_BSELL64(2, true);
+        if (uu == 1) {
+            if (gf) AXG_BSELL64(1, true, true); else AXG_BSELL64(1, true, false);
+        } else if (uu == 2) {
+            if (gf) AXG_BSELL64(2, true, true); else AXG_BSELL64(2, true, false);
         } else {
-            AXG_BSELL64(4, true);
+            if (gf) AXG_BSELL64(4, true, true); else AXG_BSELL64(4, true, false);
         }
     } else {
-        if (u == 1) {
-            AXG_BSELL64(1, false);
-        } else if (u == 2) {
-            AXG_BSELL64(2, false);
+        if (uu == 1) {
+            if (gf) AXG_BSELL64(1, false, true); else AXG_BSELL64(1, false, false);
+        } else if (uu == 2) {
+            if (gf) AXG_BSELL64(2, false, true); else AXG_BSELL64(2, false, false);
         } else {
-            AXG_BSELL64(4, false);
+            if (gf) AXG_BSELL64(4, false, true); else AXG_BSELL64(4, false, false);
         }
     }
 #undef AXG_BSELL64
@@ -598,7 +645,7 @@ static void tune_u(BlockPcg &bp, int slot) {
     float best = 1e30f;
     int best_u = 2, best_l = lanes[0];
     for (int l : lanes)
-        for (int cand : {1, 2, 4}) {
+        for (int cand : kBsellForms) {
             bp.U[slot] = cand;
             bp.L[slot] = l;
             block_spmv<K>(bp, s, false, bp.Bv.p);
@@ -979,7 +1026,7 @@ struct BatchGroup {
         {
             float best = 1e30f;
             for (int l : bsell64_lanes(K))
-                for (int cand : {1, 2, 4}) {
+                for (int cand : kBsellForms) {
                     bsell64_spmm<K>(op, s, false, cand, l, Bv.p);
                     AXG_HIP(hipEventRecord(ctx->ev0, s));
                     for (int r = 0; r < 3; r++) bsell64_spmm<K>(op, s, false, cand, l, Bv.p);
