@@ -1073,7 +1073,7 @@ def run_simp(a, G, MG, torch, np):
             rhs[prob.ind_of[free]] = rhs_full[free]
             mesh.set_modulus(e)
             mesh.assemble("mixed")
-            du, _ = mesh.solve(rhs, precision="mixed", precond="block3", tol=1e-9)
+            du, _ = mesh.solve(rhs, precision="mixed", precond="block3", tol=1e-12)
             dc.append(float(b_free @ du))
         fd.append((dc[0] - dc[1]) / (2 * h))
         mesh.set_modulus(prob.e)
