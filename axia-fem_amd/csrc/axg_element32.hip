@@ -13,15 +13,19 @@
 // zero structure of B and D multiplied out: nine FMAs per (a, b, g) instead of
 // the dense 6 x 3NN contraction.
 //
-// The fused kernel is node-centric: one wavefront owns one block row I (node i) and
-// builds it from the elements incident to i, in chunks of CH elements:
-//   1. element-local coordinates of the chunk (FP64 differences to the
-//      element's first node, rounded to FP32) into LDS;
-//   2. lane per (element, Gauss point): J, its inverse and s_g, then the
-//      scaled gradients s_g grad N_b of every local node b into LDS;
-//   3. lane per (element, b): P over the Gauss points and K_ab, and the slot
-//      of b's node in row I (binary search of the row's sorted columns);
-//   4. lane per slot: the chunk's K_ab added in ascending element order.
+// The fused form has two kernels:
+//   1. elem_grad32_kernel, lane per (element, Gauss point): J from the
+//      element-local coordinates (FP64 differences to the element's first node,
+//      rounded to FP32), s_g J^-1, and the element's (lam, mu). For hex8 it
+//      stores the eight s_g J^-1 (288 B per element); for hex20 the scaled
+//      gradient rows s_g grad N_b (6.7 KB per element).
+//   2. row_asm32_kernel, node-centric: a wavefront owns two block rows (one or
+//      more slot lanes per block) and builds each from its incident elements
+//      in chunks of eight: task (element, b) forms node b's gradient row (hex8:
+//      from the element's s_g J^-1 and the reference derivatives), P with the
+//      row node's row and K_ab, and the block's slot from a table built with the
+//      block layout; then each slot lane adds the chunk's K_ab in ascending
+//      element order.
 // Every block is summed in the same fixed order (incident elements ascending)
 // on every run, and written once, with its column, to the three value planes
 // and to the third plane of the LDS-staged SpMM. Constrained DOFs give zero
@@ -180,14 +184,21 @@ __device__ inline bool scaled_inverse(const float *dn, const float *x, float wg,
 // table, then s_g J^-1 applied to every node's derivatives. The wavefront's
 // output (contiguous in memory) is staged in LDS and written with 16-byte
 // stores.
-template <int NN, int NG, int NGP, int WPB>
+//
+// IVO (hex8): instead of the gradients, the element's eight s_g J_g^-1 (72
+// floats, 288 B against 768 B), laid out per half h of the Gauss points as
+// grad[e 72 + 36 h + 4 k + g % 4] = iv_k at point g = 4 h + g % 4; the block
+// rows form the gradients from them.
+template <int NN, int NG, int NGP, int WPB, bool IVO = false>
 __global__ void __launch_bounds__(64 * WPB)
     elem_grad32_kernel(int64_t nelem, const int32_t *__restrict__ conn,
                        const double *__restrict__ xyz, const double *__restrict__ E,
                        const double *__restrict__ NU, float *__restrict__ grad,
                        float2 *__restrict__ lm, int32_t *__restrict__ err) {
+    static_assert(!IVO || (NN == 8 && NG == 8), "inverse output: hex8");
     constexpr int EPW = 64 / NG;
-    constexpr int OUT = EPW * NN * NGP;  // floats of one wavefront's elements
+    constexpr int PER = IVO ? 9 * NG : NN * NGP;  // floats per element
+    constexpr int OUT = EPW * PER;                // floats of one wavefront's elements
     static_assert(OUT % 4 == 0, "16-byte stores");
     __shared__ float s_dn[NG][3 * NN + 1];
     __shared__ float s_w[NG];
@@ -221,7 +232,11 @@ __global__ void __launch_bounds__(64 * WPB)
         float iv[9];
         if (!scaled_inverse<NN>(s_dn[g], s_x[w][ce], s_w[g], iv))
             report32(err, AXG_E_NEG_JACOBIAN, e0 + ce);
-        float *out = so + ce * (NN * NGP);
+        float *out = so + ce * PER;
+        if constexpr (IVO) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) out[36 * (g >> 2) + 4 * k + (g & 3)] = iv[k];
+        } else {
         for (int b = 0; b < NN; b++) {
             const float d0 = s_dn[g][3 * b], d1 = s_dn[g][3 * b + 1], d2 = s_dn[g][3 * b + 2];
             out[b * NGP + 3 * g] = iv[0] * d0 + iv[3] * d1 + iv[6] * d2;
@@ -230,10 +245,35 @@ __global__ void __launch_bounds__(64 * WPB)
         }
         if (NGP > 3 * NG && g < NGP - 3 * NG)  // the zero tail
             for (int b = 0; b < NN; b++) out[b * NGP + 3 * NG + g] = 0.f;
+        }
     }
     wave_sync();
-    float4 *dst = reinterpret_cast<float4 *>(grad + e0 * (int64_t)(NN * NGP));
-    for (int t = lane; t < ne * (NN * NGP / 4); t += 64) dst[t] = s_out[w][t];
+    float4 *dst = reinterpret_cast<float4 *>(grad + e0 * (int64_t)PER);
+    for (int t = lane; t < ne * (PER / 4); t += 64) dst[t] = s_out[w][t];
+}
+
+// P += sum over four Gauss points of a(g) b(g)^T; a and b hold the points'
+// gradient rows (3 floats each) in three float4.
+__device__ inline void p_step(float (&p)[3][3], float4 a0, float4 a1, float4 a2, float4 b0, float4 b1,
+                              float4 b2) {
+    const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+    const float bv[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+#pragma unroll
+    for (int gg = 0; gg < 4; gg++)
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) p[r][cc] += av[3 * gg + r] * bv[3 * gg + cc];
+}
+
+// K_ab = lam P + mu P^T + mu tr(P) 1, row-major.
+__device__ inline void k_of_p(const float (&p)[3][3], float2 lm, float (&kv)[9]) {
+    const float lam = lm.x, mu = lm.y;
+    const float tr = mu * (p[0][0] + p[1][1] + p[2][2]);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) kv[3 * r + cc] = lam * p[r][cc] + mu * p[cc][r] + (r == cc ? tr : 0.f);
 }
 
 // Per-row LDS of the block-row assembly: CH elements per chunk, SPR slots.
@@ -250,7 +290,16 @@ struct RowLds {
 // chunk of CH incident elements, task (h, ce, b) forms K_ab of row h's
 // element ce with its local node b; then every slot lane adds the chunk's
 // contributions to its block in ascending element order.
-template <int NN, int NGP, int CH, int KPL, int RPW, int WPB, int MINW>
+//
+// IVG (hex8, chunks of eight elements): the element data are the eight
+// s_g J_g^-1 (elem_grad32_kernel's IVO layout). The eight tasks of a chunk
+// element copy its 288 B into LDS (over the chunk's block buffer, which is
+// written only after they are read); task (h, ce, b) forms its gradient row
+// s_g grad N_b(g) = (s_g J_g^-1)^T dN_b(g) from them and node b's reference
+// derivatives, one row h at a time, the row node's row reaching the other
+// tasks through LDS. Otherwise (hex20) the element data are the gradient rows,
+// read by each task from global memory.
+template <int NN, int NGP, int CH, int KPL, int RPW, int WPB, int MINW, bool IVG>
 __global__ void __launch_bounds__(64 * WPB, MINW)
     row_asm32_kernel(int64_t nb, int64_t per_xcd, int incw, const int32_t *__restrict__ rinc,
                      const float *__restrict__ grad,
@@ -262,13 +311,19 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
     constexpr int SPR = 64 * KPL / RPW;
     constexpr int TPR = CH * NN;  // tasks per row and chunk
     constexpr int NIT = (RPW * TPR + 63) / 64;
-    // small elements: every pass's gradient rows are held in registers at once and
-    // the row node's rows reach the other lanes through LDS
-    constexpr bool STAGED = NIT * NGP <= 48;
     constexpr int NQ = NGP / 4;
-    using Lds = RowLds<NN, CH, SPR, STAGED ? NQ : 1>;
+    using Lds = RowLds<NN, CH, SPR, IVG ? NQ : 1>;
     static_assert(CH % 4 == 0, "src rows are read as words");
+    static_assert(!IVG || (NN == 8 && CH == 8 && NGP == 24 && NIT == RPW), "inverse input: hex8 chunks of 8");
+    static_assert(!IVG || sizeof(Lds::c) >= CH * 18 * sizeof(float4), "inverse data fit the block buffer");
     __shared__ Lds s_row[WPB][RPW];
+    // IVG: reference derivatives, dnt[b][3 g + p] = dN_b/dxi_p at Gauss point g
+    __shared__ float4 s_dnt[IVG ? 8 : 1][6];
+    if constexpr (IVG) {
+        for (int t = threadIdx.x; t < 8 * 24; t += blockDim.x)
+            reinterpret_cast<float *>(s_dnt[t / 24])[t % 24] = c_tab32.h8_dn[(t % 24) / 3][t / 24][t % 3];
+        __syncthreads();
+    }
     const int64_t grp = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     // the wave index is wave-uniform: row metadata comes from scalar loads
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -311,9 +366,12 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
         mask[j] = smask[spos[j]];
     }
     const int nch = (incw + CH - 1) / CH;
-    for (int t = lane; t < RPW * CH * 3; t += 64)  // the zero blocks (never written again)
-        if (t % 3 < 2) s_row[w][t / (CH * 3)].c[(t / 3) % CH][NN][t % 3] = make_float4(0.f, 0.f, 0.f, 0.f);
-        else s_row[w][t / (CH * 3)].c8[(t / 3) % CH][NN] = 0.f;
+    auto zero_blocks = [&]() {
+        for (int t = lane; t < RPW * CH * 3; t += 64)
+            if (t % 3 < 2) s_row[w][t / (CH * 3)].c[(t / 3) % CH][NN][t % 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+            else s_row[w][t / (CH * 3)].c8[(t / 3) % CH][NN] = 0.f;
+    };
+    if constexpr (!IVG) zero_blocks();  // never written again
     for (int c = 0; c < nch; c++) {
         // the tasks' incident pairs first, then their gradient rows
         int32_t sl[NIT];
@@ -328,67 +386,75 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
         for (int t = lane; t < RPW * CH * SPR / 4; t += 64)
             reinterpret_cast<uint32_t *>(&s_row[w][t / (CH * SPR / 4)].src[0][0])[t % (CH * SPR / 4)] =
                 0x01010101u * NN;
-        if constexpr (STAGED) {
-            float4 gb[NIT][NQ];
+        if constexpr (IVG) {
+            // task (h = it, ce = lane / 8, b = lane % 8): the element's 18 float4
+            // copied by its eight tasks; then per row its gradient rows, the row
+            // node's through LDS, and K_ab; the blocks go over the inverse data
+            // once every row's are formed
+            const int ce = lane >> 3, b = lane & 7;
             unsigned kk[NIT];
             float2 lmv[NIT];
+            float kv[NIT][9];
 #pragma unroll
             for (int it = 0; it < NIT; it++) {
-                const int t = lane + 64 * it;
-                const int b = (t - TPR * (t / TPR)) % NN;
                 const int64_t sc = sl[it] < 0 ? 0 : sl[it];
                 const int64_t e = sc / NN;
-                const float4 *pb = reinterpret_cast<const float4 *>(grad + (e * NN + b) * NGP);
-#pragma unroll
-                for (int q = 0; q < NQ; q++) gb[it][q] = pb[q];
+                const float4 *pv = reinterpret_cast<const float4 *>(grad) + e * 18;
+                float4 *ivs = &s_row[w][it].c[0][0][0] + ce * 18;
+                ivs[b] = pv[b];
+                ivs[b + 8] = pv[b + 8];
+                if (b < 2) ivs[b + 16] = pv[b + 16];
                 kk[it] = eslot[sc * NN + b];
                 lmv[it] = lm[e];
-            }
-#pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                const int t = lane + 64 * it;
-                const int h = t / TPR, u = t - TPR * (t / TPR);
-                const int ce = u / NN, b = u - NN * (u / NN);
-                if (sl[it] >= 0 && b == sl[it] % NN) {
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) s_row[w][h].ga[ce][q] = gb[it][q];
-                }
             }
             wave_sync();
 #pragma unroll
             for (int it = 0; it < NIT; it++) {
-                const int t = lane + 64 * it;
-                const int h = t / TPR, u = t - TPR * (t / TPR);
-                const int ce = u / NN, b = u - NN * (u / NN);
-                if (sl[it] < 0) continue;
-                Lds &R = s_row[w][h];
+                float4 gb[NQ];
+#pragma unroll
+                for (int hf = 0; hf < 2; hf++) {  // four Gauss points at a time
+                    const float4 q0 = s_dnt[b][3 * hf], q1 = s_dnt[b][3 * hf + 1], q2 = s_dnt[b][3 * hf + 2];
+                    const float dd[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                    const float4 *ivs = &s_row[w][it].c[0][0][0] + ce * 18 + 9 * hf;
+                    float gv[12];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        const float4 v0 = ivs[i], v1 = ivs[3 + i], v2 = ivs[6 + i];
+                        const float a0[4] = {v0.x, v0.y, v0.z, v0.w}, a1[4] = {v1.x, v1.y, v1.z, v1.w},
+                                    a2[4] = {v2.x, v2.y, v2.z, v2.w};
+#pragma unroll
+                        for (int g4 = 0; g4 < 4; g4++)
+                            gv[3 * g4 + i] = a0[g4] * dd[3 * g4] + a1[g4] * dd[3 * g4 + 1] + a2[g4] * dd[3 * g4 + 2];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        gb[3 * hf + q] = make_float4(gv[4 * q], gv[4 * q + 1], gv[4 * q + 2], gv[4 * q + 3]);
+                }
+                Lds &R = s_row[w][it];
+                if (sl[it] >= 0 && b == sl[it] % NN) {
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) R.ga[ce][q] = gb[q];
+                }
+                wave_sync();
                 float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
 #pragma unroll
-                for (int q = 0; q < NQ / 3; q++) {  // four Gauss points per step
-                    const float4 a0 = R.ga[ce][3 * q], a1 = R.ga[ce][3 * q + 1], a2 = R.ga[ce][3 * q + 2];
-                    const float4 b0 = gb[it][3 * q], b1 = gb[it][3 * q + 1], b2 = gb[it][3 * q + 2];
-                    const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
-                                          a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
-                    const float bv[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y,
-                                          b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+                for (int q = 0; q < NQ / 3; q++)
+                    p_step(p, R.ga[ce][3 * q], R.ga[ce][3 * q + 1], R.ga[ce][3 * q + 2], gb[3 * q], gb[3 * q + 1],
+                           gb[3 * q + 2]);
+                k_of_p(p, lmv[it], kv[it]);
+                // formed here: sunk to the stores below, both rows' operands would stay live
 #pragma unroll
-                    for (int gg = 0; gg < 4; gg++)
+                for (int r = 0; r < 9; r++) asm volatile("" : "+v"(kv[it][r]));
+            }
+            wave_sync();
+            zero_blocks();  // over the inverse data, all read
 #pragma unroll
-                        for (int r = 0; r < 3; r++)
-#pragma unroll
-                            for (int cc = 0; cc < 3; cc++) p[r][cc] += av[3 * gg + r] * bv[3 * gg + cc];
-                }
-                const float lam = lmv[it].x, mu = lmv[it].y;
-                const float tr = mu * (p[0][0] + p[1][1] + p[2][2]);
-                float kv[9];
-#pragma unroll
-                for (int r = 0; r < 3; r++)
-#pragma unroll
-                    for (int cc = 0; cc < 3; cc++)
-                        kv[3 * r + cc] = lam * p[r][cc] + mu * p[cc][r] + (r == cc ? tr : 0.f);
-                R.c[ce][b][0] = make_float4(kv[0], kv[1], kv[2], kv[3]);
-                R.c[ce][b][1] = make_float4(kv[4], kv[5], kv[6], kv[7]);
-                R.c8[ce][b] = kv[8];
+            for (int it = 0; it < NIT; it++) {
+                if (sl[it] < 0) continue;
+                Lds &R = s_row[w][it];
+                R.c[ce][b][0] = make_float4(kv[it][0], kv[it][1], kv[it][2], kv[it][3]);
+                R.c[ce][b][1] = make_float4(kv[it][4], kv[it][5], kv[it][6], kv[it][7]);
+                R.c8[ce][b] = kv[it][8];
                 if (kk[it] < (unsigned)SPR) R.src[kk[it]][ce] = (uint8_t)b;
             }
         } else {
@@ -406,28 +472,10 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
             const float2 lmv = lm[e];
             float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
 #pragma unroll
-            for (int q = 0; q < NGP / 12; q++) {  // four Gauss points per step
-                const float4 a0 = pa[3 * q], a1 = pa[3 * q + 1], a2 = pa[3 * q + 2];
-                const float4 b0 = pb[3 * q], b1 = pb[3 * q + 1], b2 = pb[3 * q + 2];
-                const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
-                                      a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
-                const float bv[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y,
-                                      b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
-#pragma unroll
-                for (int gg = 0; gg < 4; gg++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-#pragma unroll
-                        for (int cc = 0; cc < 3; cc++) p[r][cc] += av[3 * gg + r] * bv[3 * gg + cc];
-            }
-            const float lam = lmv.x, mu = lmv.y;
-            const float tr = mu * (p[0][0] + p[1][1] + p[2][2]);
+            for (int q = 0; q < NGP / 12; q++)  // four Gauss points per step
+                p_step(p, pa[3 * q], pa[3 * q + 1], pa[3 * q + 2], pb[3 * q], pb[3 * q + 1], pb[3 * q + 2]);
             float kv[9];
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-#pragma unroll
-                for (int cc = 0; cc < 3; cc++)
-                    kv[3 * r + cc] = lam * p[r][cc] + mu * p[cc][r] + (r == cc ? tr : 0.f);
+            k_of_p(p, lmv, kv);
             Lds &R = s_row[w][h];
             R.c[ce][b][0] = make_float4(kv[0], kv[1], kv[2], kv[3]);
             R.c[ce][b][1] = make_float4(kv[4], kv[5], kv[6], kv[7]);
@@ -755,23 +803,23 @@ __global__ void __launch_bounds__(64 * WPB, MINW)
     }
 }
 
-template <int NN, int NG, int NGP, int WPB>
+template <int NN, int NG, int NGP, int WPB, bool IVO = false>
 static void launch_grad(axg_mesh *m, hipStream_t s, int32_t *d_err) {
     constexpr int EPW = 64 / NG;
     const int64_t blocks = (m->nelem + EPW * WPB - 1) / (EPW * WPB);
     AXG_REQUIRE(blocks < INT32_MAX, AXG_E_INVALID, "FP32 assembly: too many elements");
-    hipLaunchKernelGGL((elem_grad32_kernel<NN, NG, NGP, WPB>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((elem_grad32_kernel<NN, NG, NGP, WPB, IVO>), dim3((unsigned)blocks),
                        dim3(64 * WPB), 0, s, m->nelem, m->conn.p, m->xyz.p, m->e.p, m->nu.p,
                        m->grad32.p, reinterpret_cast<float2 *>(m->lm32.p), d_err);
     AXG_HIP(hipGetLastError());
 }
 
-template <int NN, int NGP, int CH, int KPL, int RPW, int WPB, int MINW = 1>
+template <int NN, int NGP, int CH, int KPL, int RPW, int WPB, int MINW = 1, bool IVG = NN == 8>
 static void launch_rows(axg_mesh *m, hipStream_t s) {
     const int64_t ngrp = (m->nb + WPB * RPW - 1) / (WPB * RPW);
     const int64_t per_xcd = (ngrp + 7) / 8;
     AXG_REQUIRE(8 * per_xcd < INT32_MAX, AXG_E_INVALID, "FP32 assembly: too many block rows");
-    hipLaunchKernelGGL((row_asm32_kernel<NN, NGP, CH, KPL, RPW, WPB, MINW>), dim3((unsigned)(8 * per_xcd)),
+    hipLaunchKernelGGL((row_asm32_kernel<NN, NGP, CH, KPL, RPW, WPB, MINW, IVG>), dim3((unsigned)(8 * per_xcd)),
                        dim3(64 * WPB), 0, s, m->nb, per_xcd, m->b_incw, m->b_inc.p,
                        m->grad32.p, reinterpret_cast<const float2 *>(m->lm32.p), m->eslot.p,
                        m->bs_mask.p, m->bs_sptr.p, m->bs_width.p, m->bs_col.p,
@@ -831,20 +879,22 @@ bool launch_row_asm32(axg_mesh *m, hipStream_t s, int32_t *d_err, hipEvent_t mid
         }
         return true;
     }
-    const int ngp = nn == 8 ? ngp_of(8) : ngp_of(27);
-    m->grad32.ensure((size_t)m->nelem * nn * ngp);
+    // hex8: the eight s_g J_g^-1 per element (72 floats); hex20: the gradient rows
+    m->grad32.ensure((size_t)m->nelem * (nn == 8 ? 72 : nn * ngp_of(27)));
     m->lm32.ensure(2 * (size_t)m->nelem);
     if (nn == 8) {
-        launch_grad<8, 8, ngp_of(8), 4>(m, s, d_err);
+        launch_grad<8, 8, ngp_of(8), 4, true>(m, s, d_err);
+        AXG_HIP(hipEventRecord(mid, s));
         if (wmax <= 32) launch_rows<8, ngp_of(8), 8, 1, 2, 4, 4>(m, s);
         else if (wmax <= 64) launch_rows<8, ngp_of(8), 8, 1, 1, 4>(m, s);
         else launch_rows<8, ngp_of(8), 8, 4, 1, 4>(m, s);
     } else {
         launch_grad<20, 27, ngp_of(27), 1>(m, s, d_err);
+        AXG_HIP(hipEventRecord(mid, s));
         if (wmax <= 128) launch_rows<20, ngp_of(27), 8, 2, 1, 4>(m, s);
         else launch_rows<20, ngp_of(27), 8, 4, 1, 4>(m, s);
     }
-    return false;
+    return true;
 }
 
 // FP32 values of an operator taken from a caller's CSR: each topological

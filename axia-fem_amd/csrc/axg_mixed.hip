@@ -448,8 +448,9 @@ void ensure_bsell64(axg_mesh *m) {
 // assembly (every plane written by one kernel) or the element blocks on the
 // matrix cores followed by the block-row gather; for an operator taken from a
 // caller's CSR, its FP64 values rounded into zeroed planes, then the column and
-// LDS planes. t_ke32 is the device time of the element kernel (of everything
-// when there is one kernel), t_gather32 that of the gather (else 0).
+// LDS planes. t_ke32 is the device time of the element kernel (gradients or
+// element blocks; for a caller's CSR, of everything), t_gather32 that of the
+// block-row kernel (0 for a caller's CSR).
 void assemble_mixed(axg_mesh *m, double *elapsed_ms) {
     axg_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
