@@ -945,6 +945,12 @@ static void csr_operator_solve(axg_ctx *ctx, int64_t n, const int32_t *rp, const
     for (int64_t d = 0; d < mesh->ndof; d++) ind[d] = d < n ? (int32_t)d : -1;
     mesh->h_rowptr_t.assign(rp, rp + n + 1);
     mesh->h_colind_t.assign(ci, ci + nnz);
+    for (int64_t i = 0; i < n && mesh->cols_sorted; i++)
+        for (int32_t k = rp[i] + 1; k < rp[i + 1]; k++)
+            if (ci[k] <= ci[k - 1]) {
+                mesh->cols_sorted = false;
+                break;
+            }
     auto up = [&](auto &buf, const auto *src, size_t count) {
         buf.alloc(count);
         if (count)

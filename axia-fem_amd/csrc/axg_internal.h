@@ -268,6 +268,7 @@ struct axg_mesh {
     axg::DevBuf<double> binv64;               // 9*nb FP64 block inverse over free DOFs
     axg::DevBuf<int32_t> entry_of_slot;       // topological slot -> BSR plane position
     bool binv_valid = false;                  // block inverses match the current values
+    bool cols_sorted = true;                  // every CSR row's columns strictly increasing
     int spmv_shape = -1;                      // tuned FP64 SpMV shape
     // sliced-ELL copy of the active FP64 K_ff (solver operator)
     bool sell_valid = false;  // layout matches the active pattern

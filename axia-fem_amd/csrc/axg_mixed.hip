@@ -68,7 +68,11 @@ void ensure_block_maps(axg_mesh *m) {
 // Inverse of the free-DOF part of each node's 3x3 diagonal block of K_ff,
 // formed in FP64 (cofactors). Constrained positions are identity in the
 // block and zero in the inverse. A block with |det| <= 1e-300 falls back to
-// Jacobi on its diagonal.
+// Jacobi on its diagonal. SORTED (every row's columns strictly increasing):
+// each row's entries are found by a binary search for the node's lowest free
+// DOF and a scan up to its highest; otherwise the whole row is scanned and
+// the last of duplicate entries counts.
+template <bool SORTED>
 __global__ void block_inverse_kernel(int64_t nb, const int32_t *__restrict__ rowptr,
                                      const int32_t *__restrict__ colind,
                                      const double *__restrict__ val,
@@ -78,14 +82,28 @@ __global__ void block_inverse_kernel(int64_t nb, const int32_t *__restrict__ row
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     int32_t f[3] = {free_of_blk[3 * b], free_of_blk[3 * b + 1], free_of_blk[3 * b + 2]};
+    int32_t lo = INT32_MAX, hi = -1;
+    for (int e = 0; e < 3; e++)
+        if (f[e] >= 0) lo = min(lo, f[e]), hi = max(hi, f[e]);
     double a[3][3];
     for (int d = 0; d < 3; d++)
         for (int e = 0; e < 3; e++) a[d][e] = (d == e) ? 1.0 : 0.0;
     for (int d = 0; d < 3; d++) {
         if (f[d] < 0) continue;
         a[d][d] = 0.0;
-        for (int32_t k = rowptr[f[d]]; k < rowptr[f[d] + 1]; k++) {
+        int32_t k = rowptr[f[d]];
+        const int32_t kend = rowptr[f[d] + 1];
+        if (SORTED) {  // first entry with column >= lo
+            int32_t n = kend - k;
+            while (n > 0) {
+                const int32_t h = n >> 1;
+                if (colind[k + h] < lo) k += h + 1, n -= h + 1;
+                else n = h;
+            }
+        }
+        for (; k < kend; k++) {
             int32_t c = colind[k];
+            if (SORTED && c > hi) break;
             for (int e = 0; e < 3; e++)
                 if (f[e] >= 0 && c == f[e]) a[d][e] = val[k];
         }
@@ -128,10 +146,12 @@ void ensure_block_inverse64(axg_mesh *m) {
     m->binv32.ensure(9 * std::max<int64_t>(m->nb, 1));
     m->dinv32.ensure(3 * std::max<int64_t>(m->nb, 1));
     CsrView A = active_csr(m);
-    if (m->nb > 0)
-        hipLaunchKernelGGL(block_inverse_kernel, dim3((unsigned)((m->nb + 127) / 128)), dim3(128),
-                           0, m->ctx->stream, m->nb, A.rowptr, A.colind, A.val, m->free_of_blk.p,
-                           m->binv64.p, m->binv32.p, m->dinv32.p);
+    if (m->nb > 0) {
+        auto kern = m->cols_sorted ? block_inverse_kernel<true> : block_inverse_kernel<false>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)((m->nb + 127) / 128)), dim3(128), 0, m->ctx->stream,
+                           m->nb, A.rowptr, A.colind, A.val, m->free_of_blk.p, m->binv64.p,
+                           m->binv32.p, m->dinv32.p);
+    }
     AXG_HIP(hipGetLastError());
     AXG_HIP(hipStreamSynchronize(m->ctx->stream));
     m->binv_valid = true;

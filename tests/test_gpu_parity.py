@@ -187,12 +187,15 @@ def test_solve_kff_dropin_options(ctx, precision, precond):
     assert r.residual <= 1e-11 * np.linalg.norm(rhs)
 
 
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
 @pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("fp64", "block3"),
                                                ("mixed", "block3")])
-def test_solve_csr_generic_spd(ctx, precision, precond):
+def test_solve_csr_generic_spd(ctx, precision, precond, order):
     """A caller's CSR without node structure (n = 301, not a multiple of 3, rows of
-    different patterns) through axg_solve_csr in each mode, against LAPACK."""
-    say("case: solve_csr", precision, precond)
+    different patterns) through axg_solve_csr in each mode, against LAPACK; with the
+    columns of every row in ascending order, and shuffled (the 3x3 block inverses then
+    scan whole rows instead of searching them)."""
+    say("case: solve_csr", precision, precond, order)
     rng = np.random.default_rng(12)
     n = 301
     a = np.zeros((n, n))
@@ -203,7 +206,14 @@ def test_solve_csr_generic_spd(ctx, precision, precond):
     import scipy.sparse as sps
     csr = sps.csr_matrix(a)
     b = rng.standard_normal(n)
-    x, info = G.solve_csr(csr.indptr, csr.indices, csr.data, b, ctx=ctx, tol=1e-12,
+    indices, data = csr.indices.copy(), csr.data.copy()
+    if order == "shuffled":
+        for i in range(n):
+            lo, hi = csr.indptr[i], csr.indptr[i + 1]
+            perm = lo + rng.permutation(hi - lo)
+            indices[lo:hi], data[lo:hi] = csr.indices[perm], csr.data[perm]
+        assert any(np.any(np.diff(indices[csr.indptr[i]:csr.indptr[i + 1]]) < 0) for i in range(n))
+    x, info = G.solve_csr(csr.indptr, indices, data, b, ctx=ctx, tol=1e-12,
                           precision=precision, precond=precond)
     ref = np.linalg.solve(a, b)
     say(f"  {info['name']}: iters {info['iters']}, err {rel(x, ref):.2e}")
