@@ -935,12 +935,13 @@ __global__ void __launch_bounds__(MBS, 2) bsell_spmm_lds_kernel(
 // are double-buffered by iteration parity: inner_dir reads `cur` and its
 // workgroup 0 writes `next`, which the next iteration's kernels read.
 
-// x += alpha p, r -= alpha Ap, z = M r (3x3 block or diagonal), partials of
-// ||r||^2 and r.z; workgroup 0 keeps p.Ap for inner_dir's breakdown test.
+// r -= alpha Ap, z = M r (3x3 block or diagonal), partials of ||r||^2 and
+// r.z; workgroup 0 keeps p.Ap for inner_dir's breakdown test (x += alpha p is
+// inner_dir's, which reads p anyway).
 template <int K, bool BLOCK>
 __global__ void __launch_bounds__(MBS) inner_step_kernel(
-    int64_t nb, float *__restrict__ X, float *__restrict__ R, const float *__restrict__ P,
-    const float *__restrict__ AP, float *__restrict__ Z, const float *__restrict__ binv,
+    int64_t nb, float *__restrict__ R, const float *__restrict__ AP, float *__restrict__ Z,
+    const float *__restrict__ binv,
     const float *__restrict__ dinv, const double *__restrict__ part_s, int nblk_s,
     double *__restrict__ part_u, double *__restrict__ pap_out, const InnerScalars *sc) {
     if (sc->all_done) return;
@@ -967,17 +968,11 @@ __global__ void __launch_bounds__(MBS) inner_step_kernel(
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const int64_t o = d * ps + b * K;
-            float x[K], p[K], ap[K];
+            float ap[K];
             load_k<K>(R + o, r[d]);
-            load_k<K>(X + o, x);
-            load_k<K>(P + o, p);
             load_k<K>(AP + o, ap);
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                x[j] += alpha[j] * p[j];
-                r[d][j] -= alpha[j] * ap[j];
-            }
-            store_k<K>(X + o, x);
+            for (int j = 0; j < K; j++) r[d][j] -= alpha[j] * ap[j];
             store_k<K>(R + o, r[d]);
         }
 #pragma unroll
@@ -1004,18 +999,29 @@ __global__ void __launch_bounds__(MBS) inner_step_kernel(
         for (int j = 0; j < 2 * K; j++) part_u[(size_t)blockIdx.x * 2 * K + j] = acc[j];
 }
 
-// Per column: a breakdown (p.Ap <= 0) or ||r|| <= tol_abs or the iteration cap
-// ends it; otherwise beta = r.z_new / r.z and p = z + beta p.
+// *dst = *src, one word per thread of the workgroup (a copy through one
+// thread's registers would sit in scratch memory, and a word-by-word loop in
+// one thread waits for every load: src and dst may alias)
+__device__ inline void copy_scalars(InnerScalars *dst, const InnerScalars *src) {
+    static_assert(sizeof(InnerScalars) % 4 == 0 && sizeof(InnerScalars) / 4 <= MBS, "word copy");
+    if (threadIdx.x < sizeof(InnerScalars) / 4)
+        reinterpret_cast<int32_t *>(dst)[threadIdx.x] = reinterpret_cast<const int32_t *>(src)[threadIdx.x];
+}
+
+// x += alpha p with this iteration's alpha (formed as inner_step formed it);
+// then per column: a breakdown (p.Ap <= 0) or ||r|| <= tol_abs or the
+// iteration cap ends it; otherwise beta = r.z_new / r.z and p = z + beta p.
 template <int K>
 __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__restrict__ P,
                                                        const float *__restrict__ Z,
+                                                       float *__restrict__ X,
                                                        const double *__restrict__ part_u,
                                                        int nblk_u,
                                                        const double *__restrict__ pap_in,
                                                        const InnerScalars *cur,
                                                        InnerScalars *next) {
     if (cur->all_done) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *next = *cur;
+        if (blockIdx.x == 0) copy_scalars(next, cur);
         return;
     }
     const int max_iter = cur->max_iter;
@@ -1028,8 +1034,11 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__res
     double tot[2 * K];
     fin_sums<2 * K, MBS>(part_u, nblk_u, tot);
     int ndone[K];
-    float beta[K];
+    float beta[K], alpha[K];
     double res[K];
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        alpha[j] = (done[j] || !(pap[j] > 1e-300)) ? 0.f : (float)(rz[j] / pap[j]);
 #pragma unroll
     for (int j = 0; j < K; j++) {
         ndone[j] = done[j];
@@ -1049,28 +1058,34 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__res
     }
     for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * MBS) {
-        float p[K], z[K];
+        float p[K], z[K], x[K];
         load_k<K>(P + i * K, p);
         load_k<K>(Z + i * K, z);
+        load_k<K>(X + i * K, x);
 #pragma unroll
-        for (int j = 0; j < K; j++) p[j] = ndone[j] ? p[j] : z[j] + beta[j] * p[j];
+        for (int j = 0; j < K; j++) {
+            x[j] += alpha[j] * p[j];
+            p[j] = ndone[j] ? p[j] : z[j] + beta[j] * p[j];
+        }
+        store_k<K>(X + i * K, x);
         store_k<K>(P + i * K, p);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        InnerScalars nx = *cur;
+    if (blockIdx.x != 0) return;
+    copy_scalars(next, cur);
+    __syncthreads();  // the copy before the new values
+    if (threadIdx.x == 0) {
         int all = 1;
 #pragma unroll
         for (int j = 0; j < K; j++) {
             if (!done[j] && pap[j] > 1e-300) {
-                nx.res[j] = res[j];
-                nx.iters[j] = iters[j] + 1;
-                if (!ndone[j]) nx.beta[j] = beta[j], nx.rz[j] = tot[K + j];
+                next->res[j] = res[j];
+                next->iters[j] = iters[j] + 1;
+                if (!ndone[j]) next->beta[j] = beta[j], next->rz[j] = tot[K + j];
             }
-            nx.done[j] = ndone[j];
+            next->done[j] = ndone[j];
             all &= ndone[j];
         }
-        nx.all_done = all;
-        *next = nx;
+        next->all_done = all;
     }
 }
 
@@ -1525,14 +1540,14 @@ struct MixedSolver {
             InnerScalars *cur = dsc.p + par, *next = dsc.p + (par ^ 1);
             if (block)
                 hipLaunchKernelGGL((inner_step_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
-                                   w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p,
-                                   w.part.p, grid_mv, part_u.p, pap.p, cur);
+                                   w.f2.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p, w.part.p,
+                                   grid_mv, part_u.p, pap.p, cur);
             else
                 hipLaunchKernelGGL((inner_step_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s, nb,
-                                   w.f1.p, w.f2.p, w.f3.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p,
-                                   w.part.p, grid_mv, part_u.p, pap.p, cur);
+                                   w.f2.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p, w.part.p,
+                                   grid_mv, part_u.p, pap.p, cur);
             hipLaunchKernelGGL((inner_dir_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv, w.f3.p,
-                               Z.p, part_u.p, grid_v, pap.p, cur, next);
+                               Z.p, w.f1.p, part_u.p, grid_v, pap.p, cur, next);
         };
         const int grid_n = (int)((n + 255) / 256);
         std::vector<double> hpart((size_t)grid_v * K);
