@@ -16,7 +16,10 @@ brick (default)  C3D8 brick 80x80x80 (512,000 elements, 531,441 nodes,
     linalg::pcg: Jacobi, scalar CSR rows, one load case at a time), and
     fp64_best, the library's fastest FP64 Jacobi PCG (all load cases in one
     pass on the FP64 3x3 node-block operator). All three are checked against
-    an FP64 solution converged to 1e-13.
+    an FP64 solution converged to 1e-13. Every solve is timed on the solver's
+    production path (iterations replayed as hipGraphs, no per-launch events);
+    one more instrumented solve of each leg times every SpMM launch with HIP
+    events for the roofline and the operator bandwidths.
 cantilever  C3D8 cantilever 20x9x9 (2,100 nodes, 6,000 free DOFs), E 210000,
     nu 0.3, x = 0 clamped, -1000 N in z over the tip face; FP64 Jacobi PCG to
     1e-10 against the oracle's Cholesky solution.
@@ -322,13 +325,17 @@ def run_brick(a, G, MG, torch, np):
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     steps = []
+    # the timed steps run the solver's production path (hipGraph replay, no per-launch
+    # events); one instrumented step afterwards times every SpMM launch for the roofline
     for s_ in range(a.steps):
-        steps.append(step(True))
+        steps.append(step(False))
         log(f"step {s_}: solve {steps[-1][2]['solve_ms']:.1f} ms inner {steps[-1][2]['iters']} "
             f"sweeps {steps[-1][2]['outer_iters']}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ms_per_step = 1e3 * wall / a.steps
+    inst = step(True)
+    log(f"instrumented step: solve {inst[2]['solve_ms']:.1f} ms")
 
     def rel_err(Xa):
         d = (Xa - Xref).double()
@@ -340,30 +347,31 @@ def run_brick(a, G, MG, torch, np):
     # each inner iteration advances all k right-hand sides by one CG step over n DOFs
     # (inner_iters: per sweep the largest count over right-hand sides, summed)
     mixed_dof_iter_s = n * inner_iters * k / (solve_ms / 1e3)
-    spmv_ms = float(np.mean([avg_launch(s[2]) for s in steps]))
+    spmv_ms = avg_launch(inst[2])
     asm = {kk: float(np.mean([s[1][kk] for s in steps])) for kk in steps[0][1]}
     nelem = prob.nelem
     forms = asm32_forms(a, mesh, nelem)
-    # the same solve replayed as hipGraphs without per-launch events (untimed)
-    g = mesh.solve_device(B.data_ptr(), k, X.data_ptr(), precision="mixed", precond=a.precond,
-                          tol=a.tol, inner_tol=a.inner_tol, refinement=a.refinement)
 
     fp64 = plain = None
     indptr = indices = data = None
     if not a.no_fp64:
         X64 = torch.zeros_like(B)
+        mesh.assemble("fp64")
+        # warm (tuners, graphs), then timed on the production path, then instrumented
+        mesh.solve_device(B.data_ptr(), k, X64.data_ptr(), precision="fp64", tol=a.tol)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        mesh.assemble("fp64")
-        r64 = mesh.solve_device(B.data_ptr(), k, X64.data_ptr(), precision="fp64", tol=a.tol,
-                                timing=True)
+        r64 = mesh.solve_device(B.data_ptr(), k, X64.data_ptr(), precision="fp64", tol=a.tol)
         torch.cuda.synchronize()
         wall64 = time.perf_counter() - t1
+        r64i = mesh.solve_device(B.data_ptr(), k, X64.data_ptr(), precision="fp64", tol=a.tol,
+                                 timing=True)
         fp64 = {
             "solver": r64["name"],
             "iters_total": r64["iters"], "solve_ms": r64["solve_ms"], "wall_ms": 1e3 * wall64,
+            "solve_ms_instrumented": r64i["solve_ms"],
             "dof_iter_per_s": n * r64["iters"] / (r64["solve_ms"] / 1e3),
-            "spmv_ms_avg": avg_launch(r64),
+            "spmv_ms_avg": avg_launch(r64i),
             "rel_err_vs_ref": rel_err(X64), "rel_residual": r64["rel_residual"],
         }
         # the node-block operator takes all k right-hand sides in one pass
@@ -379,15 +387,17 @@ def run_brick(a, G, MG, torch, np):
         Xp = np.zeros_like(Bh)
         pl = []
         for j in range(k):
-            Xp[j], inf = G.solve_csr(indptr, indices, data, Bh[j], ctx=ctx, tol=a.tol, timing=True)
+            Xp[j], inf = G.solve_csr(indptr, indices, data, Bh[j], ctx=ctx, tol=a.tol)
             pl.append(inf)
+        # load case 0 again with every SpMV launch timed
+        _, pli = G.solve_csr(indptr, indices, data, Bh[0], ctx=ctx, tol=a.tol, timing=True)
         Xpt = torch.tensor(Xp, device=dev)
         plain = {
             "solver": pl[0]["name"], "calls": k,
             "iters_total": int(sum(r["iters"] for r in pl)),
             "solve_ms": float(sum(r["solve_ms"] for r in pl)),
-            "spmv_ms_avg": float(sum(r["spmv_ms"] for r in pl) / max(1, sum(r["spmv_launches"]
-                                                                        for r in pl))),
+            "solve_ms_instrumented_case0": pli["solve_ms"],
+            "spmv_ms_avg": avg_launch(pli),
             "rel_err_vs_ref": rel_err(Xpt),
             "rel_residual": float(max(r["rel_residual"] for r in pl)),
         }
@@ -420,7 +430,7 @@ def run_brick(a, G, MG, torch, np):
             "rel_err_vs_fp64_ref": err_mixed,
             "rel_residual": float(max(s[2]["rel_residual"] for s in steps)),
             "time_to_solution_per_rhs_ms": solve_ms / k,
-            "graph_replay_solve_ms": g["solve_ms"],
+            "solve_ms_instrumented": inst[2]["solve_ms"],
         },
         "fp64_best": fp64,
         "fp64_plain": plain,
@@ -482,7 +492,7 @@ def run_cantilever(a, G, MG, torch, np):
         step(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = [step(True) for _ in range(a.steps)]
+    res = [step(False) for _ in range(a.steps)]
     torch.cuda.synchronize()
     ms_per_step = 1e3 * (time.perf_counter() - t0) / a.steps
     iters = res[-1][1]["iters"]
@@ -492,7 +502,7 @@ def run_cantilever(a, G, MG, torch, np):
         "workload": "C3D8 cantilever 20x9x9 (2,100 nodes, 6,000 free DOFs), E 210000, nu 0.3, "
                     "x=0 clamped, -1000 N z on the tip face; FP64 Jacobi PCG to 1e-10",
         "elements": prob.nelem, "n_free": n, "nnz": nnz, "nrhs": 1, "precond": "jacobi"})
-    spmv = float(np.mean([avg_launch(r[1]) for r in res]))
+    spmv = avg_launch(step(True)[1])  # one instrumented solve: every SpMV launch timed
     out.update({
         "fp64": {"iters": iters, "solve_ms": solve_ms, "assembly_ms": asm_ms,
                  "elements_per_s": prob.nelem / (asm_ms / 1e3),
@@ -564,25 +574,29 @@ def run_c3d20(a, G, MG, torch, np):
         step(False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    res = [step(True) for _ in range(a.steps)]
+    # timed on the production path (graph replay); one instrumented step for the SpMM
+    res = [step(False) for _ in range(a.steps)]
     torch.cuda.synchronize()
     ms_per_step = 1e3 * (time.perf_counter() - t1) / a.steps
     e_mixed = err(x)
     solve_ms = float(np.mean([r[2]["solve_ms"] for r in res]))
     inner = float(np.mean([r[2]["iters"] for r in res]))
     asm = {kk: float(np.mean([r[1][kk] for r in res])) for kk in res[0][1]}
-    spmm = float(np.mean([avg_launch(r[2]) for r in res]))
+    spmm = avg_launch(step(True)[2])
     variants = {"mixed_block3": {"solve_ms": solve_ms, "iters": inner,
                                  "sweeps": res[-1][2]["outer_iters"], "rel_err": e_mixed}}
-    _, _, rj = step(True, "jacobi")
+    _, _, rj = step(False, "jacobi")
     variants["mixed_jacobi"] = {"solve_ms": rj["solve_ms"], "iters": rj["iters"],
                                 "sweeps": rj["outer_iters"], "rel_err": err(x)}
     mesh.assemble("fp64")
     for pc in ("block3", "jacobi"):
-        r = mesh.solve_device(b.data_ptr(), 1, x.data_ptr(), precision="fp64", precond=pc, tol=tol,
-                              timing=True)
-        variants[f"fp64_{pc}"] = {"solve_ms": r["solve_ms"], "iters": r["iters"], "rel_err": err(x),
-                                  "spmv_ms_avg": avg_launch(r)}
+        mesh.solve_device(b.data_ptr(), 1, x.data_ptr(), precision="fp64", precond=pc, tol=tol)
+        r = mesh.solve_device(b.data_ptr(), 1, x.data_ptr(), precision="fp64", precond=pc, tol=tol)
+        variants[f"fp64_{pc}"] = {"solve_ms": r["solve_ms"], "iters": r["iters"], "rel_err": err(x)}
+        if pc == "block3":
+            ri = mesh.solve_device(b.data_ptr(), 1, x.data_ptr(), precision="fp64", precond=pc,
+                                   tol=tol, timing=True)
+            variants[f"fp64_{pc}"]["spmv_ms_avg"] = avg_launch(ri)
         log(f"fp64 {pc}: {r['iters']} iterations {r['solve_ms']:.0f} ms")
     v64 = variants["fp64_block3"]
     # C3D20's nodes all keep 3 DOFs here (the clamped face is fully fixed): n / 3 block rows
