@@ -43,7 +43,8 @@ nlgeom  *STEP, NLGEOM: a C3D8 cantilever 200x20x20 (80,000 elements, 264,600
     free DOFs; 10 x 1 x 1, E 1000, nu 0.3, x = 0 clamped) under a tip load of
     0.5 in -z (about 20 % of the length in linear theory) in 4 increments: the
     reference's continuum Newton driver with the St. Venant-Kirchhoff tangent
-    re-assembled on the device at every iteration and block-Jacobi PCG solves.
+    re-assembled on the device at every iteration and Jacobi PCG solves on the
+    FP64 node-block operator.
     One step = one whole analysis.
 
 The default (brick) line also carries `other_configs`: the cantilever, C3D20,
