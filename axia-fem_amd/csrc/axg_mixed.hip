@@ -766,6 +766,19 @@ __global__ void __launch_bounds__(MBS) bsell_spmm_kernel(
 // x from LDS: the only global loads of the multiply phase are the value
 // stream, so its waits never queue behind dependent gathers. Columns beyond
 // the staged list (top bit set in the third plane) are read from global memory.
+// K floats per lane from global memory straight into LDS at lds + K lane
+// (lds the same for the whole wavefront): 16-byte form for K = 4, 4-byte for 1.
+template <int K>
+__device__ inline void load_to_lds(const float *g, float *lds);
+template <>
+__device__ inline void load_to_lds<4>(const float *g, float *lds) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
+template <>
+__device__ inline void load_to_lds<1>(const float *g, float *lds) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds, 4, 0, 0);
+}
+
 template <int K>
 constexpr int lds_floats() {
     return 3 * kGrpMaxCols * (K <= 4 ? K : 1);
@@ -823,28 +836,44 @@ __global__ void __launch_bounds__(MBS, 2) bsell_spmm_lds_kernel(
             }
         };
         if (nfull > 0) load_batch(0, va, vb, vc, lcs);
-        // stage x of the group's columns: every load of a thread issued before
-        // its LDS stores
+        // stage x of the group's columns
         {
             constexpr int kIt = (kGrpMaxCols + MBS - 1) / MBS;
             const int64_t grp = row0 / kGrpRows;
             const int c0 = grp_ptr[grp], mcols = grp_ptr[grp + 1] - c0;
-            float sv[kIt][3][K];
+            if constexpr (K == 4 || K == 1) {
+                // straight into LDS (global_load_lds: lane l of a wavefront writes its
+                // 4 K bytes at the wavefront's base + 4 K l, which is column t's place)
 #pragma unroll
-            for (int it = 0; it < kIt; it++) {
-                const int t = threadIdx.x + it * MBS;
-                if (t < mcols) {
-                    const int64_t J = grp_col[c0 + t];
+                for (int it = 0; it < kIt; it++) {
+                    const int t = threadIdx.x + it * MBS;
+                    if (t < mcols) {
+                        const int64_t J = grp_col[c0 + t];
+                        const int tw = it * MBS + (threadIdx.x & ~63);  // the wavefront's first column
 #pragma unroll
-                    for (int d = 0; d < 3; d++) load_k<K>(P + d * ps + J * K, sv[it][d]);
+                        for (int d = 0; d < 3; d++) load_to_lds<K>(P + d * ps + J * K, xs + d * plane + tw * K);
+                    }
                 }
-            }
+                __builtin_amdgcn_s_waitcnt(0);  // the LDS writes of those loads
+            } else {
+                // every load of a thread issued before its LDS stores
+                float sv[kIt][3][K];
 #pragma unroll
-            for (int it = 0; it < kIt; it++) {
-                const int t = threadIdx.x + it * MBS;
-                if (t < mcols)
+                for (int it = 0; it < kIt; it++) {
+                    const int t = threadIdx.x + it * MBS;
+                    if (t < mcols) {
+                        const int64_t J = grp_col[c0 + t];
 #pragma unroll
-                    for (int d = 0; d < 3; d++) store_k<K>(xs + d * plane + t * K, sv[it][d]);
+                        for (int d = 0; d < 3; d++) load_k<K>(P + d * ps + J * K, sv[it][d]);
+                    }
+                }
+#pragma unroll
+                for (int it = 0; it < kIt; it++) {
+                    const int t = threadIdx.x + it * MBS;
+                    if (t < mcols)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) store_k<K>(xs + d * plane + t * K, sv[it][d]);
+                }
             }
         }
         __syncthreads();
