@@ -463,7 +463,7 @@ def test_mixed_lds_shapes_wide_groups(ctx, shape):
     """C3D20 rows couple up to 81 nodes: on a 24 x 24 x 2 brick a group of 256 block rows
     has up to 2,080 distinct column blocks, more than the LDS-staged SpMM stages (1,360), so
     the rest come from global memory (the column rides in the third plane); mixed solves
-    with LDS shapes pinned match the FP64 solve."""
+    with LDS shapes pinned (4 and 1 right-hand sides) match the FP64 solve."""
     say("case: LDS shapes, wide groups", shape)
     xyz, conn = MG.brick_hex20(24, 24, 2)
     rng = np.random.default_rng(shape)
@@ -473,11 +473,12 @@ def test_mixed_lds_shapes_wide_groups(ctx, shape):
     n = mesh.n_free
     B = rng.standard_normal((4, n))
     ref, _ = mesh.solve(B, precision="fp64", tol=1e-13)
-    assert mesh.spmm_shape(4, shape) == shape
+    assert mesh.spmm_shape(4, shape) == shape and mesh.spmm_shape(1, shape) == shape
     X, info = mesh.solve(B, precision="mixed", precond="block3", tol=1e-11)
     err = max(rel(X[k], ref[k]) for k in range(4))
-    say(f"  n={n}, inner {info['iters']}, err {err:.2e}")
-    assert err < 1e-8
+    x1, _ = mesh.solve(B[1], precision="mixed", precond="block3", tol=1e-11)
+    say(f"  n={n}, inner {info['iters']}, err {err:.2e}, single {rel(x1, ref[1]):.2e}")
+    assert err < 1e-8 and rel(x1, ref[1]) < 1e-8
     mesh.close()
 
 
