@@ -132,7 +132,8 @@ __global__ void block_inverse_kernel(int64_t nb, const int32_t *__restrict__ row
         for (int e = 0; e < 3; e++) {
             double v = (f[d] >= 0 && f[e] >= 0) ? inv[d][e] : 0.0;
             binv[9 * b + 3 * d + e] = v;
-            if (binv32) binv32[9 * b + 3 * d + e] = (float)v;
+            // the FP32 copy: the upper triangle (the inverse of a symmetric block)
+            if (binv32 && d <= e) binv32[6 * b + (d == 0 ? e : d == 1 ? e + 2 : 5)] = (float)v;
         }
         if (dinv32)
             dinv32[3 * b + d] = (f[d] >= 0 && fabs(a[d][d]) > 1e-30) ? (float)(1.0 / a[d][d]) : 0.f;
@@ -143,7 +144,7 @@ void ensure_block_inverse64(axg_mesh *m) {
     ensure_block_maps(m);
     if (m->binv_valid) return;
     m->binv64.ensure(9 * std::max<int64_t>(m->nb, 1));
-    m->binv32.ensure(9 * std::max<int64_t>(m->nb, 1));
+    m->binv32.ensure(6 * std::max<int64_t>(m->nb, 1));
     m->dinv32.ensure(3 * std::max<int64_t>(m->nb, 1));
     CsrView A = active_csr(m);
     if (m->nb > 0) {
@@ -964,13 +965,37 @@ __global__ void __launch_bounds__(MBS, 2) bsell_spmm_lds_kernel(
 // are double-buffered by iteration parity: inner_dir reads `cur` and its
 // workgroup 0 writes `next`, which the next iteration's kernels read.
 
-// r -= alpha Ap, z = M r (3x3 block or diagonal), partials of ||r||^2 and
-// r.z; workgroup 0 keeps p.Ap for inner_dir's breakdown test (x += alpha p is
-// inner_dir's, which reads p anyway).
+// z = M r for block row b: the node's symmetric 3x3 inverse (FP32 upper
+// triangle m00 m01 m02 m11 m12 m22) or the diagonal inverse.
+template <int K, bool BLOCK>
+__device__ inline void precond3(const float *__restrict__ binv, const float *__restrict__ dinv,
+                                int64_t b, const float (&r)[3][K], float (&z)[3][K]) {
+    if constexpr (BLOCK) {
+        const float m00 = binv[6 * b], m01 = binv[6 * b + 1], m02 = binv[6 * b + 2];
+        const float m11 = binv[6 * b + 3], m12 = binv[6 * b + 4], m22 = binv[6 * b + 5];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            z[0][j] = m00 * r[0][j] + m01 * r[1][j] + m02 * r[2][j];
+            z[1][j] = m01 * r[0][j] + m11 * r[1][j] + m12 * r[2][j];
+            z[2][j] = m02 * r[0][j] + m12 * r[1][j] + m22 * r[2][j];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float dd = dinv[3 * b + d];
+#pragma unroll
+            for (int j = 0; j < K; j++) z[d][j] = dd * r[d][j];
+        }
+    }
+}
+
+// r -= alpha Ap, z = M r (3x3 block or diagonal) for the partials of ||r||^2
+// and r.z (z is not stored: inner_dir forms it again from r); workgroup 0 keeps
+// p.Ap for inner_dir's breakdown test (x += alpha p is inner_dir's, which reads
+// p anyway).
 template <int K, bool BLOCK>
 __global__ void __launch_bounds__(MBS) inner_step_kernel(
-    int64_t nb, float *__restrict__ R, const float *__restrict__ AP, float *__restrict__ Z,
-    const float *__restrict__ binv,
+    int64_t nb, float *__restrict__ R, const float *__restrict__ AP, const float *__restrict__ binv,
     const float *__restrict__ dinv, const double *__restrict__ part_s, int nblk_s,
     double *__restrict__ part_u, double *__restrict__ pap_out, const InnerScalars *sc) {
     if (sc->all_done) return;
@@ -1004,23 +1029,15 @@ __global__ void __launch_bounds__(MBS) inner_step_kernel(
             for (int j = 0; j < K; j++) r[d][j] -= alpha[j] * ap[j];
             store_k<K>(R + o, r[d]);
         }
+        float z[3][K];
+        precond3<K, BLOCK>(binv, dinv, b, r, z);
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            float m0 = 0.f, m1 = 0.f, m2 = 0.f, dd = 0.f;
-            if (BLOCK) {
-                m0 = binv[9 * b + 3 * d], m1 = binv[9 * b + 3 * d + 1], m2 = binv[9 * b + 3 * d + 2];
-            } else {
-                dd = dinv[3 * b + d];
-            }
-            float z[K];
+        for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                z[j] = BLOCK ? (m0 * r[0][j] + m1 * r[1][j] + m2 * r[2][j]) : dd * r[d][j];
                 acc[j] += (double)r[d][j] * r[d][j];
-                acc[K + j] += (double)r[d][j] * z[j];
+                acc[K + j] += (double)r[d][j] * z[d][j];
             }
-            store_k<K>(Z + d * ps + b * K, z);
-        }
     }
     group_sums<2 * K, MBS>(acc);
     if (threadIdx.x == 0)
@@ -1039,10 +1056,13 @@ __device__ inline void copy_scalars(InnerScalars *dst, const InnerScalars *src) 
 
 // x += alpha p with this iteration's alpha (formed as inner_step formed it);
 // then per column: a breakdown (p.Ap <= 0) or ||r|| <= tol_abs or the
-// iteration cap ends it; otherwise beta = r.z_new / r.z and p = z + beta p.
-template <int K>
-__global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__restrict__ P,
-                                                       const float *__restrict__ Z,
+// iteration cap ends it; otherwise beta = r.z_new / r.z and p = z + beta p,
+// z = M r formed again as inner_step formed it.
+template <int K, bool BLOCK>
+__global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nb, float *__restrict__ P,
+                                                       const float *__restrict__ R,
+                                                       const float *__restrict__ binv,
+                                                       const float *__restrict__ dinv,
                                                        float *__restrict__ X,
                                                        const double *__restrict__ part_u,
                                                        int nblk_u,
@@ -1085,19 +1105,27 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nv, float *__res
             beta[j] = (float)(tot[K + j] / rz[j]);
         }
     }
-    for (int64_t i = (int64_t)blockIdx.x * MBS + threadIdx.x; i < nv;
-         i += (int64_t)gridDim.x * MBS) {
-        float p[K], z[K], x[K];
-        load_k<K>(P + i * K, p);
-        load_k<K>(Z + i * K, z);
-        load_k<K>(X + i * K, x);
+    const int64_t ps = nb * K;
+    for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
+         b += (int64_t)gridDim.x * MBS) {
+        float r[3][K], z[3][K];
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            x[j] += alpha[j] * p[j];
-            p[j] = ndone[j] ? p[j] : z[j] + beta[j] * p[j];
+        for (int d = 0; d < 3; d++) load_k<K>(R + d * ps + b * K, r[d]);
+        precond3<K, BLOCK>(binv, dinv, b, r, z);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int64_t o = d * ps + b * K;
+            float p[K], x[K];
+            load_k<K>(P + o, p);
+            load_k<K>(X + o, x);
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                x[j] += alpha[j] * p[j];
+                p[j] = ndone[j] ? p[j] : z[d][j] + beta[j] * p[j];
+            }
+            store_k<K>(X + o, x);
+            store_k<K>(P + o, p);
         }
-        store_k<K>(X + i * K, x);
-        store_k<K>(P + i * K, p);
     }
     if (blockIdx.x != 0) return;
     copy_scalars(next, cur);
@@ -1207,18 +1235,16 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
                 store_k<K>(P + o, p);
             }
         }
+        float z[3][K];
+        precond3<K, BLOCK>(binv, dinv, b, r, z);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            float z[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                z[j] = BLOCK ? (binv[9 * b + 3 * d] * r[0][j] + binv[9 * b + 3 * d + 1] * r[1][j] +
-                                binv[9 * b + 3 * d + 2] * r[2][j])
-                             : dinv[3 * b + d] * r[d][j];
                 acc[j] += (double)r[d][j] * r[d][j];
-                acc[K + j] += (double)r[d][j] * z[j];
+                acc[K + j] += (double)r[d][j] * z[d][j];
             }
-            store_k<K>(Z + d * ps + b * K, z);
+            store_k<K>(Z + d * ps + b * K, z[d]);
         }
     }
     group_sums<2 * K, MBS>(acc);
@@ -1567,16 +1593,21 @@ struct MixedSolver {
         };
         auto launch_update = [&](int par) {
             InnerScalars *cur = dsc.p + par, *next = dsc.p + (par ^ 1);
-            if (block)
+            if (block) {
                 hipLaunchKernelGGL((inner_step_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
-                                   w.f2.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p, w.part.p,
-                                   grid_mv, part_u.p, pap.p, cur);
-            else
+                                   w.f2.p, w.f4.p, m->binv32.p, m->dinv32.p, w.part.p, grid_mv,
+                                   part_u.p, pap.p, cur);
+                hipLaunchKernelGGL((inner_dir_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
+                                   w.f3.p, w.f2.p, m->binv32.p, m->dinv32.p, w.f1.p, part_u.p, grid_v,
+                                   pap.p, cur, next);
+            } else {
                 hipLaunchKernelGGL((inner_step_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s, nb,
-                                   w.f2.p, w.f4.p, Z.p, m->binv32.p, m->dinv32.p, w.part.p,
-                                   grid_mv, part_u.p, pap.p, cur);
-            hipLaunchKernelGGL((inner_dir_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv, w.f3.p,
-                               Z.p, w.f1.p, part_u.p, grid_v, pap.p, cur, next);
+                                   w.f2.p, w.f4.p, m->binv32.p, m->dinv32.p, w.part.p, grid_mv,
+                                   part_u.p, pap.p, cur);
+                hipLaunchKernelGGL((inner_dir_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s, nb,
+                                   w.f3.p, w.f2.p, m->binv32.p, m->dinv32.p, w.f1.p, part_u.p, grid_v,
+                                   pap.p, cur, next);
+            }
         };
         const int grid_n = (int)((n + 255) / 256);
         std::vector<double> hpart((size_t)grid_v * K);
