@@ -1577,9 +1577,22 @@ struct MixedSolver {
         w.part.ensure((size_t)std::max({grid_mv, grid_v, grid_r}) * 2 * K + 8);
         DevBuf<InnerScalars> dsc(2);  // by iteration parity
         DevBuf<double> part_u((size_t)grid_v * 2 * K + 8), pap(8);
-        ctx->pinned.ensure(sizeof(InnerScalars) + 256);
+        // pinned staging: the status record, 64 doubles of device scalars (t1's 56),
+        // and a second status record (consecutive graph replays' status copies: the
+        // host reads one while the next replay runs)
+        constexpr size_t kHn = 64 * sizeof(double);
+        ctx->pinned.ensure(2 * sizeof(InnerScalars) + kHn);
         InnerScalars *h = (InnerScalars *)ctx->pinned.p;
         double *hn = (double *)((char *)ctx->pinned.p + sizeof(InnerScalars));
+        InnerScalars *hs[2] = {h, (InnerScalars *)((char *)ctx->pinned.p + sizeof(InnerScalars) + kHn)};
+        struct StatusEvents {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~StatusEvents() {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } sev;
+        for (hipEvent_t &e : sev.e) AXG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         std::memset(h, 0, sizeof(InnerScalars));
         AXG_HIP(hipMemsetAsync(w.f0.p, 0, sizeof(float) * nv * K, s));
         AXG_HIP(hipMemsetAsync(d_u, 0, sizeof(double) * n * K, s));
@@ -1714,30 +1727,44 @@ struct MixedSolver {
             AXG_HIP(hipGetLastError());
             int32_t prev_it[K];
             for (int j = 0; j < K; j++) prev_it[j] = 0;
-            for (;;) {
+            if (!o.timing) {
+                // replay k + 1 is queued before the host reads replay k's status, so the
+                // device never waits for the host; a replay after the end runs kernels
+                // that return at once and leaves the state as it is
+                AXG_HIP(hipGraphLaunch(gexec, s));
+                AXG_HIP(hipMemcpyAsync(hs[0], dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost, s));
+                AXG_HIP(hipEventRecord(sev.e[0], s));
+                for (int k = 0;; k++) {
+                    AXG_HIP(hipGraphLaunch(gexec, s));
+                    AXG_HIP(hipMemcpyAsync(hs[(k + 1) & 1], dsc.p, sizeof(InnerScalars),
+                                           hipMemcpyDeviceToHost, s));
+                    AXG_HIP(hipEventRecord(sev.e[(k + 1) & 1], s));
+                    AXG_HIP(hipEventSynchronize(sev.e[k & 1]));
+                    if (hs[k & 1]->all_done) break;
+                }
+                AXG_HIP(hipStreamSynchronize(s));
+                AXG_HIP(hipMemcpy(h, dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost));
+            }
+            // with timing: launches one by one, every SpMM stamped by events
+            for (; o.timing;) {
                 AXG_HIP(hipMemcpyAsync(h, dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost, s));
                 AXG_HIP(hipStreamSynchronize(s));
                 if (h->all_done) break;
-                if (o.timing) {
-                    for (int i = 0; i < inner_batch; i++) {
-                        launch_spmm(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1], i & 1);
-                        launch_update(i & 1);
-                    }
-                    AXG_HIP(hipMemcpyAsync(h, dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost,
-                                           s));
-                    AXG_HIP(hipStreamSynchronize(s));
-                    int32_t mx = 0;
-                    for (int j = 0; j < K; j++) mx = std::max(mx, h->iters[j] - prev_it[j]);
-                    for (int j = 0; j < K; j++) prev_it[j] = h->iters[j];
-                    for (int i = 0; i < mx && i < inner_batch; i++) {
-                        float ms = 0.f;
-                        AXG_HIP(hipEventElapsedTime(&ms, ctx->timing_events[2 * i],
-                                                    ctx->timing_events[2 * i + 1]));
-                        res.spmv_ms += ms;
-                        res.spmv_launches++;
-                    }
-                } else {
-                    AXG_HIP(hipGraphLaunch(gexec, s));
+                for (int i = 0; i < inner_batch; i++) {
+                    launch_spmm(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1], i & 1);
+                    launch_update(i & 1);
+                }
+                AXG_HIP(hipMemcpyAsync(h, dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost, s));
+                AXG_HIP(hipStreamSynchronize(s));
+                int32_t mx = 0;
+                for (int j = 0; j < K; j++) mx = std::max(mx, h->iters[j] - prev_it[j]);
+                for (int j = 0; j < K; j++) prev_it[j] = h->iters[j];
+                for (int i = 0; i < mx && i < inner_batch; i++) {
+                    float ms = 0.f;
+                    AXG_HIP(hipEventElapsedTime(&ms, ctx->timing_events[2 * i],
+                                                ctx->timing_events[2 * i + 1]));
+                    res.spmv_ms += ms;
+                    res.spmv_launches++;
                 }
             }
             int32_t mx = 0;
