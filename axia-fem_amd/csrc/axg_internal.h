@@ -141,7 +141,7 @@ static_assert(sizeof(PcgScalars) % 16 == 0, "PcgScalars alignment");
 struct Workspace {
     DevBuf<double> r, p, ap, z, part, t0, t1;
     DevBuf<PcgScalars> sc;
-    DevBuf<float> f0, f1, f2, f3, f4;
+    DevBuf<float> f0, f1, f2, f3, f4, f5;
     DevBuf<int32_t> err;
     DevBuf<uint32_t> cnt;
 };

@@ -544,11 +544,25 @@ __device__ inline void store_k(float *__restrict__ p, const float (&v)[K]) {
 
 // Per right-hand-side scalars of the inner solve (device).
 struct InnerScalars {
-    double rz[8], alpha[8], beta[8], res[8], tol_abs[8], bnorm[8];
+    double rz[8], beta[8], res[8], tol_abs[8], bnorm[8];
+    float ahist[4][8];  // alpha of the ring slots not yet added to x
     int32_t done[8];
     int32_t iters[8];
     int32_t all_done;
     int32_t max_iter;
+    int32_t pslot;  // the ring slot that holds the latest direction
+    int32_t pad;
+};
+
+// The inner CG's directions rotate through kRing buffers: iteration i of an
+// interval multiplies slot i % kRing and writes the next direction to slot
+// (i + 1) % kRing. x takes the pending alpha_k p_k of slots 0 .. i % kRing in
+// one pass, in iteration order (the same roundings as one update per
+// iteration), when the ring wraps or the interval ends: one read and write of
+// x per kRing iterations instead of per iteration.
+constexpr int kRing = 4;
+struct PRing {
+    float *p[kRing];
 };
 
 // ---------------------------------------- block-sliced-ELL SpMM (FP32)
@@ -1054,12 +1068,14 @@ __device__ inline void copy_scalars(InnerScalars *dst, const InnerScalars *src) 
         reinterpret_cast<int32_t *>(dst)[threadIdx.x] = reinterpret_cast<const int32_t *>(src)[threadIdx.x];
 }
 
-// x += alpha p with this iteration's alpha (formed as inner_step formed it);
-// then per column: a breakdown (p.Ap <= 0) or ||r|| <= tol_abs or the
-// iteration cap ends it; otherwise beta = r.z_new / r.z and p = z + beta p,
-// z = M r formed again as inner_step formed it.
+// Per column: a breakdown (p.Ap <= 0) or ||r|| <= tol_abs or the iteration
+// cap ends it; otherwise beta = r.z_new / r.z and p_next = z + beta p, z = M r
+// formed again as inner_step formed it (an ended column keeps p). p is ring
+// slot `slot`, p_next goes to the next slot. This iteration's alpha (formed as
+// inner_step formed it) is kept with the scalars; when the ring wraps or every
+// column has ended, x += alpha_s p_s for the ring's slots 0 .. slot in order.
 template <int K, bool BLOCK>
-__global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nb, float *__restrict__ P,
+__global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nb, PRing ring, int slot,
                                                        const float *__restrict__ R,
                                                        const float *__restrict__ binv,
                                                        const float *__restrict__ dinv,
@@ -1105,6 +1121,17 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nb, float *__res
             beta[j] = (float)(tot[K + j] / rz[j]);
         }
     }
+    int all = 1;
+#pragma unroll
+    for (int j = 0; j < K; j++) all &= ndone[j];
+    const bool flush = all || slot == kRing - 1;
+    float ah[kRing - 1][K];  // alphas of the earlier pending slots
+#pragma unroll
+    for (int s = 0; s < kRing - 1; s++)
+#pragma unroll
+        for (int j = 0; j < K; j++) ah[s][j] = s < slot ? cur->ahist[s][j] : 0.f;
+    const float *P = ring.p[slot];
+    float *Pn = ring.p[slot + 1 == kRing ? 0 : slot + 1];  // after the wrap: slot 0, read first
     const int64_t ps = nb * K;
     for (int64_t b = (int64_t)blockIdx.x * MBS + threadIdx.x; b < nb;
          b += (int64_t)gridDim.x * MBS) {
@@ -1115,23 +1142,33 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nb, float *__res
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const int64_t o = d * ps + b * K;
-            float p[K], x[K];
+            float p[K];
             load_k<K>(P + o, p);
-            load_k<K>(X + o, x);
+            if (flush) {
+                float x[K];
+                load_k<K>(X + o, x);
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                x[j] += alpha[j] * p[j];
-                p[j] = ndone[j] ? p[j] : z[d][j] + beta[j] * p[j];
+                for (int s = 0; s < kRing - 1; s++) {
+                    if (s < slot) {
+                        float q[K];
+                        load_k<K>(ring.p[s] + o, q);
+#pragma unroll
+                        for (int j = 0; j < K; j++) x[j] += ah[s][j] * q[j];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < K; j++) x[j] += alpha[j] * p[j];
+                store_k<K>(X + o, x);
             }
-            store_k<K>(X + o, x);
-            store_k<K>(P + o, p);
+#pragma unroll
+            for (int j = 0; j < K; j++) p[j] = ndone[j] ? p[j] : z[d][j] + beta[j] * p[j];
+            store_k<K>(Pn + o, p);
         }
     }
     if (blockIdx.x != 0) return;
     copy_scalars(next, cur);
     __syncthreads();  // the copy before the new values
     if (threadIdx.x == 0) {
-        int all = 1;
 #pragma unroll
         for (int j = 0; j < K; j++) {
             if (!done[j] && pap[j] > 1e-300) {
@@ -1140,9 +1177,10 @@ __global__ void __launch_bounds__(MBS) inner_dir_kernel(int64_t nb, float *__res
                 if (!ndone[j]) next->beta[j] = beta[j], next->rz[j] = tot[K + j];
             }
             next->done[j] = ndone[j];
-            all &= ndone[j];
+            next->ahist[slot][j] = alpha[j];
         }
         next->all_done = all;
+        next->pslot = slot + 1 == kRing ? 0 : slot + 1;
     }
 }
 
@@ -1201,13 +1239,14 @@ __global__ void __launch_bounds__(MBS) inner_update_p_kernel(int64_t nv, float *
 
 // Inner start: X = 0, R = B, Z = M R, norms; the finalize sets
 // tol_abs = inner_tol |B_j|, rz and the zero-rhs exits. With ratio (a restart
-// that keeps the search direction) P is rescaled to the new residual scale
-// (P *= ratio_j, and set to 0 where ratio_j is 0).
+// that keeps the search direction) the last direction Pin is rescaled to the
+// new residual scale into P (P = ratio_j Pin, and 0 where ratio_j is 0; Pin
+// may be P).
 template <int K, bool BLOCK>
 __global__ void __launch_bounds__(MBS) inner_init_kernel(
     int64_t nb, const float *__restrict__ B, float *__restrict__ X, float *__restrict__ R,
     float *__restrict__ Z, const float *__restrict__ binv, const float *__restrict__ dinv,
-    double *__restrict__ part, InnerScalars *sc, double inner_tol, float *__restrict__ P,
+    double *__restrict__ part, InnerScalars *sc, double inner_tol, const float *Pin, float *P,
     const double *__restrict__ ratio) {
     double acc[2 * K];
 #pragma unroll
@@ -1229,7 +1268,7 @@ __global__ void __launch_bounds__(MBS) inner_init_kernel(
             store_k<K>(X + o, zero);
             if (ratio) {
                 float p[K];
-                load_k<K>(P + o, p);
+                load_k<K>(Pin + o, p);
 #pragma unroll
                 for (int j = 0; j < K; j++) p[j] = rat[j] > 0.f ? p[j] * rat[j] : 0.f;
                 store_k<K>(P + o, p);
@@ -1563,8 +1602,11 @@ struct MixedSolver {
         w.f0.ensure(nv * K);  // B (inner rhs)
         w.f1.ensure(nv * K);  // X
         w.f2.ensure(nv * K);  // R
-        w.f3.ensure(nv * K);  // P
+        w.f3.ensure(nv * K);  // P (ring slot 0)
         w.f4.ensure(nv * K);  // AP
+        w.f5.ensure((size_t)(kRing - 1) * nv * K);  // ring slots 1 ..
+        PRing ring;
+        for (int q = 0; q < kRing; q++) ring.p[q] = q == 0 ? w.f3.p : w.f5.p + (size_t)(q - 1) * nv * K;
         DevBuf<float> Z(nv * K);
         w.t0.ensure(n * K);   // r64
         w.t1.ensure(56);      // scales, alpha, beta, restart ratio and r.z, inner tolerances
@@ -1600,26 +1642,27 @@ struct MixedSolver {
 
         // timing: the events are stamped by the dispatch itself (kernel start
         // and end), not by separate stream markers
-        auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1, int par) {
-            launch_spmm_shape<K, true>(m, shp, grid_mv, w.f3.p, w.f4.p, w.part.p, dsc.p + par, e0,
-                                       e1);
+        auto launch_spmm = [&](hipEvent_t e0, hipEvent_t e1, int i) {
+            launch_spmm_shape<K, true>(m, shp, grid_mv, ring.p[i % kRing], w.f4.p, w.part.p,
+                                       dsc.p + (i & 1), e0, e1);
         };
-        auto launch_update = [&](int par) {
+        auto launch_update = [&](int i) {
+            const int par = i & 1, slot = i % kRing;
             InnerScalars *cur = dsc.p + par, *next = dsc.p + (par ^ 1);
             if (block) {
                 hipLaunchKernelGGL((inner_step_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
                                    w.f2.p, w.f4.p, m->binv32.p, m->dinv32.p, w.part.p, grid_mv,
                                    part_u.p, pap.p, cur);
                 hipLaunchKernelGGL((inner_dir_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
-                                   w.f3.p, w.f2.p, m->binv32.p, m->dinv32.p, w.f1.p, part_u.p, grid_v,
-                                   pap.p, cur, next);
+                                   ring, slot, w.f2.p, m->binv32.p, m->dinv32.p, w.f1.p, part_u.p,
+                                   grid_v, pap.p, cur, next);
             } else {
                 hipLaunchKernelGGL((inner_step_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s, nb,
                                    w.f2.p, w.f4.p, m->binv32.p, m->dinv32.p, w.part.p, grid_mv,
                                    part_u.p, pap.p, cur);
                 hipLaunchKernelGGL((inner_dir_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s, nb,
-                                   w.f3.p, w.f2.p, m->binv32.p, m->dinv32.p, w.f1.p, part_u.p, grid_v,
-                                   pap.p, cur, next);
+                                   ring, slot, w.f2.p, m->binv32.p, m->dinv32.p, w.f1.p, part_u.p,
+                                   grid_v, pap.p, cur, next);
             }
         };
         const int grid_n = (int)((n + 255) / 256);
@@ -1657,6 +1700,8 @@ struct MixedSolver {
         double fnorm[K];
         for (int j = 0; j < K; j++) fnorm[j] = rnorm[j];
         const int inner_batch = nb > 100000 ? 16 : 64;
+        static_assert(16 % kRing == 0 && 64 % kRing == 0,
+                      "every interval starts at ring slot 0 of a replay");
         const int64_t inner_cap = o.max_iter > 0 ? o.max_iter : std::max<int64_t>(4 * n, 200);
         int64_t total_inner = 0;
         int sweeps = 0;
@@ -1666,9 +1711,10 @@ struct MixedSolver {
         if (!o.timing) {
             hipGraph_t graph;
             AXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < inner_batch; i++) {  // inner_batch is even: state back in dsc[0]
-                launch_spmm(nullptr, nullptr, i & 1);
-                launch_update(i & 1);
+            // inner_batch is even (state back in dsc[0]) and a multiple of kRing
+            for (int i = 0; i < inner_batch; i++) {
+                launch_spmm(nullptr, nullptr, i);
+                launch_update(i);
             }
             AXG_HIP(hipStreamEndCapture(s, &graph));
             AXG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -1684,6 +1730,7 @@ struct MixedSolver {
         // warm: the search direction of the previous interval runs on (its scale
         // was prev_scale), with beta from the previous interval's last r.z.
         double prev_scale[K], prev_rz[K];
+        int prev_slot = 0;
         // the inner tolerance of a sweep: inner_tol, loosened for a column whose
         // residual is within 1 / inner_tol of the target to a third of the
         // reduction it still needs (the last sweep stops near what it needs)
@@ -1712,14 +1759,16 @@ struct MixedSolver {
             h->max_iter = (int32_t)std::min<int64_t>(inner_cap - total_inner, INT32_MAX);
             AXG_HIP(hipMemcpyAsync(dsc.p, h, sizeof(InnerScalars), hipMemcpyHostToDevice, s));
             const double *ratio_p = warm ? w.t1.p + 32 : nullptr;
+            // the interval starts at ring slot 0 from the last one's latest direction
+            const float *p_last = ring.p[prev_slot];
             if (block)
                 hipLaunchKernelGGL((inner_init_kernel<K, true>), dim3(grid_v), dim3(MBS), 0, s, nb,
                                    w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p, w.part.p,
-                                   dsc.p, o.inner_tol, w.f3.p, ratio_p);
+                                   dsc.p, o.inner_tol, p_last, w.f3.p, ratio_p);
             else
                 hipLaunchKernelGGL((inner_init_kernel<K, false>), dim3(grid_v), dim3(MBS), 0, s,
                                    nb, w.f0.p, w.f1.p, w.f2.p, Z.p, m->binv32.p, m->dinv32.p,
-                                   w.part.p, dsc.p, o.inner_tol, w.f3.p, ratio_p);
+                                   w.part.p, dsc.p, o.inner_tol, p_last, w.f3.p, ratio_p);
             hipLaunchKernelGGL((fin_inner_init_kernel<K>), dim3(1), dim3(FIN_TB), 0, s, w.part.p,
                                grid_v, dsc.p, w.t1.p + 48, warm ? w.t1.p + 40 : nullptr);
             hipLaunchKernelGGL((inner_update_p_kernel<K>), dim3(grid_v), dim3(MBS), 0, s, nv,
@@ -1751,8 +1800,8 @@ struct MixedSolver {
                 AXG_HIP(hipStreamSynchronize(s));
                 if (h->all_done) break;
                 for (int i = 0; i < inner_batch; i++) {
-                    launch_spmm(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1], i & 1);
-                    launch_update(i & 1);
+                    launch_spmm(ctx->timing_events[2 * i], ctx->timing_events[2 * i + 1], i);
+                    launch_update(i);
                 }
                 AXG_HIP(hipMemcpyAsync(h, dsc.p, sizeof(InnerScalars), hipMemcpyDeviceToHost, s));
                 AXG_HIP(hipStreamSynchronize(s));
@@ -1771,6 +1820,7 @@ struct MixedSolver {
             for (int j = 0; j < K; j++) mx = std::max(mx, h->iters[j]);
             total_inner += mx;
             for (int j = 0; j < K; j++) prev_scale[j] = scale[j], prev_rz[j] = h->rz[j];
+            prev_slot = h->pslot >= 0 && h->pslot < kRing ? h->pslot : 0;
         };
         // the FP64 residual of a stiff problem has a round-off floor: stop once
         // two true residuals in a row improved no column by half
