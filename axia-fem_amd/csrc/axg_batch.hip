@@ -1263,6 +1263,13 @@ extern "C" int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *mes
         axg_solve_opts o;
         axg_default_opts(&o);
         if (opts) o = *opts;
+        // the batched solver is the FP64 Jacobi PCG: other forms are refused, not
+        // replaced (checked before the meshes or the device are touched)
+        AXG_REQUIRE(o.precision == AXG_PREC_FP64, AXG_E_INVALID,
+                    "axg_solve_batch: precision must be AXG_PREC_FP64 (the batched solver is FP64)");
+        AXG_REQUIRE(o.precond == AXG_PRECOND_JACOBI, AXG_E_INVALID,
+                    "axg_solve_batch: precond must be AXG_PRECOND_JACOBI (the batched solver is "
+                    "Jacobi PCG)");
         AXG_HIP(hipSetDevice(ctx->device));
         int kmax = 0;
         for (int m = 0; m < nmesh; m++) {

@@ -351,7 +351,9 @@ int axg_deck_write_frd(axg_deck *deck, const double *u_full, const double *stres
  * solved in one graph-captured launch sequence with per-(mesh, rhs)
  * convergence. x[m] receives the solutions, iters (sum nrhs entries) the
  * iteration count of each solve, solve_ms[] (sum nrhs entries, may be NULL)
- * the device time at which each solve converged. */
+ * the device time at which each solve converged. The batched solver is the
+ * FP64 Jacobi PCG: opts->precision other than AXG_PREC_FP64 or opts->precond
+ * other than AXG_PRECOND_JACOBI is refused with AXG_E_INVALID. */
 int axg_solve_batch(axg_ctx *ctx, int32_t nmesh, axg_mesh *const *meshes,
                     const double *const *rhs, const int32_t *nrhs, const axg_solve_opts *opts,
                     double *const *x, int64_t *iters, double *solve_ms, axg_solve_info *info);

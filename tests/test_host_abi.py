@@ -251,6 +251,27 @@ def test_solve_csr_rejects_malformed_csr_before_any_device_work():
     assert "unknown preconditioner" in L.axg_last_error().decode()
 
 
+def test_solve_batch_refuses_other_precisions_and_preconditioners():
+    """axg_solve_batch runs the FP64 Jacobi PCG only: a caller asking for the mixed form or
+    block-Jacobi gets AXG_E_INVALID before any mesh or device access (the handles below
+    are never dereferenced on these paths)."""
+    import ctypes as C
+    L = _lib.lib()
+    fake = C.c_void_p(1)
+    meshes = (C.c_void_p * 1)(fake)
+    buf = np.zeros(3)
+    ptrs = (C.POINTER(C.c_double) * 1)(_lib.dptr(buf))
+    nrhs = (C.c_int32 * 1)(1)
+    for field, value, msg in (("precision", 1, "precision must be AXG_PREC_FP64"),
+                              ("precond", 1, "precond must be AXG_PRECOND_JACOBI")):
+        o = _lib.SolveOpts()
+        L.axg_default_opts(C.byref(o))
+        setattr(o, field, value)
+        st = L.axg_solve_batch(fake, 1, meshes, ptrs, nrhs, C.byref(o), ptrs, None, None, None)
+        assert st == _lib.AXG_E_INVALID
+        assert msg in L.axg_last_error().decode()
+
+
 def test_python_binding_checks_sizes_before_the_call():
     """Mesh methods raise ValueError for arrays of the wrong length instead of letting the
     library read or write past them (the checks run before any library call)."""
