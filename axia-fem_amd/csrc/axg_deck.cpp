@@ -1270,6 +1270,10 @@ int axg_deck_nlgeom(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
     AXG_REQUIRE(ctx && deck && u_full, AXG_E_INVALID, "axg_deck_nlgeom: invalid argument");
     const axg_deck &d = *deck;
     AXG_REQUIRE(d.nlgeom, AXG_E_INVALID, "axg_deck_nlgeom: the deck has no *STEP, NLGEOM");
+    // the tangent is re-assembled in FP64 for every Newton iteration: no FP32 operator
+    AXG_REQUIRE(!opts || opts->precision == AXG_PREC_FP64, AXG_E_INVALID,
+                "axg_deck_nlgeom: precision must be AXG_PREC_FP64 (the Newton solves run on "
+                "the FP64 tangent)");
     const double newton_tol = 1e-8;  // model.rs:1173
     const int max_newton = 25;       // model.rs:1172
     const int64_t ndof = d.ndof;

@@ -319,7 +319,8 @@ int axg_deck_buckle(axg_ctx *ctx, axg_deck *deck, const axg_solve_opts *opts, do
  * plasticity or amplitudes): load steps of 1 / ninc with the reference's
  * growth and cutback, full Newton with the tangent and internal force
  * assembled on the device (axg_nl_assemble), follower pressure, the GPU PCG
- * of opts for each correction. u_full (3*nnode); rf = f_int - f_ext, nodal
+ * of opts for each correction (FP64: opts->precision other than
+ * AXG_PREC_FP64 is AXG_E_INVALID). u_full (3*nnode); rf = f_int - f_ext, nodal
  * Cauchy stress (nnode x 6), von Mises (nnode), nodal Green-Lagrange strain
  * (nnode x 6), element mean Cauchy stress_gp (nelem x 6) may be NULL;
  * ninc_done, newton_iters (the last increment's iterations) may be NULL. */

@@ -272,6 +272,26 @@ def test_solve_batch_refuses_other_precisions_and_preconditioners():
         assert msg in L.axg_last_error().decode()
 
 
+def test_nlgeom_refuses_mixed_precision():
+    """axg_deck_nlgeom re-assembles the FP64 tangent every Newton iteration, so it has no FP32
+    operator: precision = mixed is refused before the device is touched."""
+    import ctypes as C
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from eigen_decks import nl_cantilever_deck
+    from axia_gpu import deck as D
+    L = _lib.lib()
+    dk = D.Deck(nl_cantilever_deck(ninc=2))
+    o = _lib.SolveOpts()
+    L.axg_default_opts(C.byref(o))
+    o.precision = 1
+    buf = np.zeros(3 * dk.nnode)
+    st = L.axg_deck_nlgeom(C.c_void_p(1), dk._h, C.byref(o), _lib.dptr(buf), None, None, None,
+                           None, None, None, None, None)
+    assert st == _lib.AXG_E_INVALID
+    assert "precision must be AXG_PREC_FP64" in L.axg_last_error().decode()
+
+
 def test_python_binding_checks_sizes_before_the_call():
     """Mesh methods raise ValueError for arrays of the wrong length instead of letting the
     library read or write past them (the checks run before any library call)."""
