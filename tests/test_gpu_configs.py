@@ -159,7 +159,7 @@ def sliced_padding(lengths):
 
 
 def test_c3d20_length_sorted_block_rows_vs_oracle(ctx):
-    """C3D20 16 x 4 x 4 (1,450 block rows): its corner and midside rows alternate along the
+    """C3D20 16 x 4 x 4 (1,440 block rows): its corner and midside rows alternate along the
     lines, so the library orders the block rows by length within 256-row windows (the
     padding falls by more than 5 %, checked here on the host from the node adjacency). The
     FP32 operator still matches K_ff entry by entry, and the FP64 and mixed block-Jacobi
@@ -226,6 +226,28 @@ def test_batch_small_vs_oracle(ctx, groups):
     for p, m, X in zip(probs, meshes, xs):
         check_against_oracle(m, p, oracle_cases(p), X, p.name)
     assert (iters > 0).all() and (lat > 0).all()
+    for m in meshes:
+        m.close()
+
+
+def test_batch_of_length_sorted_c3d20_meshes_vs_oracle(ctx):
+    """Two jittered C3D20 bricks whose block rows are ordered by length within 256-row
+    windows (16 x 4 x 4 and 12 x 4 x 4, both above the 512-row minimum), plus an 8-node cube
+    that keeps node order, in one batched solve: the concatenated node-block operator and
+    the right-hand-side packing go through each mesh's block maps; every solve against the
+    oracle's direct solve."""
+    say("case: batch of length-sorted C3D20 meshes vs oracle")
+    probs = [MG.brick_c3d20_jitter(16, 4, 4, seed=3), MG.brick_c3d20_jitter(12, 4, 4, seed=4)]
+    probs += MG.batch_meshes(count=1, lo=6, hi=6, nloads=2, seed=22)
+    meshes = []
+    for p in probs:
+        m = G.Mesh(p.nn, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
+        m.assemble("fp64")
+        meshes.append(m)
+    xs, iters, lat, info = G.solve_batch(meshes, [p.rhs_all() for p in probs], tol=1e-12)
+    say(f"  {info['name']}: {len(iters)} solves, {info['solve_ms']:.1f} ms")
+    for p, m, X in zip(probs, meshes, xs):
+        check_against_oracle(m, p, oracle_cases(p), X, p.name)
     for m in meshes:
         m.close()
 
