@@ -1024,8 +1024,9 @@ __global__ void __launch_bounds__(MBS, 2) bsell_spmm_lds_kernel(
 
 // ------------------------------------------------------ inner iteration
 // One inner CG iteration is three kernels: the SpMM (Ap and p.Ap partials),
-// inner_step (alpha, x, r, z, and the ||r||^2, r.z partials) and inner_dir
-// (stop tests, beta, p). Each of the last two sums the previous kernel's
+// inner_step (alpha, r, z = M r for the ||r||^2 and r.z partials) and inner_dir
+// (stop tests, beta, the next direction into the ring, and every kRing
+// iterations x). Each of the last two sums the previous kernel's
 // partials in every workgroup, in the same fixed order, so all workgroups form
 // the same scalars and no separate finalize kernel runs. The per-column scalars
 // are double-buffered by iteration parity: inner_dir reads `cur` and its
