@@ -158,11 +158,13 @@ def test_meshgen_counts_of_the_benchmark_shapes():
 
 # ------------------------------------------------- struct layouts across the ABI
 RUST_SIZES = {"i32": (4, 4), "i64": (8, 8), "f64": (8, 8), "[c_char; 64]": (64, 1)}
+RUST_BINDING_DOC = "INTEGRATION.md"
 
 
 def rust_struct_fields(name):
-    """Fields of a #[repr(C)] struct of the Rust binding in INTEGRATION.md."""
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    """Fields of a #[repr(C)] struct of the published Rust binding (the integration guide
+    at the repository root, whose code blocks a Rust crate would copy)."""
+    text = open(os.path.join(ROOT, RUST_BINDING_DOC)).read()
     body = re.search(r"pub struct " + name + r" \{(.*?)\n\}", text, re.S).group(1)
     return re.findall(r"pub (\w+): ([^,]+),", body)
 
