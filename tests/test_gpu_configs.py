@@ -183,7 +183,9 @@ def test_c3d20_length_sorted_block_rows_vs_oracle(ctx):
     mesh = G.Mesh(20, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
     check_op32(mesh)
     for precision in ("fp64", "mixed"):
-        x, info = mesh.solve(p.rhs(), precision=precision, precond="block3", tol=1e-12)
+        # 1e-10: the FP64 residual of this slender stiff beam floors near 1e-11, and where
+        # exactly depends on the summation order, which the row order sets
+        x, info = mesh.solve(p.rhs(), precision=precision, precond="block3", tol=1e-10)
         say(f"  {info['name']}: iters {info['iters']}, rel.res {info['rel_residual']:.2e}")
         check_against_oracle(mesh, p, refs, [x], f"{precision}/block3")
     mesh.close()
@@ -244,7 +246,7 @@ def test_batch_of_length_sorted_c3d20_meshes_vs_oracle(ctx):
         m = G.Mesh(p.nn, p.xyz, p.conn, p.e, p.nu, p.ind_of, ctx=ctx)
         m.assemble("fp64")
         meshes.append(m)
-    xs, iters, lat, info = G.solve_batch(meshes, [p.rhs_all() for p in probs], tol=1e-12)
+    xs, iters, lat, info = G.solve_batch(meshes, [p.rhs_all() for p in probs], tol=1e-10)
     say(f"  {info['name']}: {len(iters)} solves, {info['solve_ms']:.1f} ms")
     for p, m, X in zip(probs, meshes, xs):
         check_against_oracle(m, p, oracle_cases(p), X, p.name)
